@@ -1,0 +1,158 @@
+/*
+ * kaos_hip.h -- C-ABI of libkaos_hip.so, the MI355X (gfx950) implementation
+ * of the KAOS per-timestep kinetic hot path.
+ *
+ * The reference has no FFI: its hot path is a set of argument-less Fortran
+ * module subroutines called from KineticSolverSub (_8_KineticSolver.f90:35-263)
+ * that communicate through module globals (_KineticMainParams.f90:33-554).
+ * These entry points replace those subroutine calls one for one, take plain
+ * pointers and sizes (host memory, valid only during the call) and return an
+ * int status (0 = ok).  A Fortran driver binds them with iso_c_binding.
+ *
+ * Index conventions follow the reference: config cells are 1..nq with
+ * 0 = escaped through the lower boundary and -1 = through the upper one;
+ * velocity cells are 1..n with 0 = none.  Per-cell arrays are passed as
+ * contiguous 0-based C arrays (cell Q of the reference at [Q-1]); the 2-D
+ * phase-space arrays are [Q-1][Vperpind-1][Vparind-1].
+ */
+#ifndef KAOS_HIP_H
+#define KAOS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define KAOS_OK            0
+#define KAOS_E_ARG         1   /* invalid argument or configuration */
+#define KAOS_E_CAPACITY    2   /* particle capacity of a tube exceeded */
+#define KAOS_E_DEVICE      3   /* HIP runtime error */
+#define KAOS_E_STATE       4   /* call out of order (e.g. step before upload) */
+#define KAOS_E_NAN         5   /* a NaN was produced in the particle state */
+
+/* Precision forms. */
+#define KAOS_FP64   0   /* every stage in binary64 */
+#define KAOS_MIXED  1   /* binary32 state/RK4 stages; binary64 root solve, moment sums, gradient */
+
+/* Run-wide configuration (replaces the flag copies of _1_SimParameterization.f90
+ * and the allocation of DataTypeAllocBSub, _8_0_5_DataTypeAllocB.f90). */
+typedef struct kaos_cfg {
+  int32_t ntubes;        /* flux tubes (SpecieT(s)%NfT) */
+  int32_t precision;     /* KAOS_FP64 or KAOS_MIXED */
+  int32_t ndatfac;       /* master-step cadence ndatfacGT (>= 1) */
+  int32_t device;        /* HIP device ordinal */
+  uint64_t seed;         /* seed of the injection stream (used when no uniforms are passed) */
+} kaos_cfg;
+
+/* One flux tube: grid, species and flags.  Replaces the FluxTubeT/QCellT/VCellT
+ * grid fields (_KineticMainParams.f90:397-520) and the per-tube flags. */
+typedef struct kaos_grid {
+  int32_t nq;            /* config cells NqUB-NqLB+1 (>= 11: moment filter window) */
+  int32_t nvpar;         /* NVparGT  (<= 255) */
+  int32_t nvperp;        /* NVperpGT (<= 255) */
+  int32_t grav;          /* GRAVflagT */
+  int32_t eamb;          /* EAMBflagT */
+  int32_t eambsign;      /* EAMBSIGNflagT */
+  int32_t filter_pt;     /* M0MAfilterPt (odd) */
+  int32_t lbcond;        /* LBCONDITIONflagT */
+  int32_t ubcond;        /* UBCONDITIONflagT */
+  int32_t pad0;
+  int64_t n_inj_lb;      /* ions injected at NqLB per master step: nint(nsnormCLBGT/ranksize) */
+  int64_t n_inj_ub;      /* same at NqUB */
+  double p;              /* L-shell pGCGT */
+  double cphi, sphi;     /* cos/sin of the tube longitude phiGCGT */
+  double h;              /* time step hT [s] */
+  double ms, qs;         /* species mass [kg] and charge [C] */
+  double nsnormfac;      /* macro-particle normalisation nsnormfacT */
+  double Ts_lb, Ts_ub;   /* ion temperature TsGT at NqLB / NqUB [K] */
+  /* config grid, nq each (qGLGT, qGHGT, qGCGT, hqCGT, dqCGT, d3xCGT,
+   * rGCGT, cos(thetaGCGT), ellGCGT, TeNT); edges must be contiguous:
+   * qGL[Q+1] == qGH[Q]. */
+  const double *qGL, *qGH, *qGC, *hqC, *dqC, *d3xC, *rGC, *cthGC, *ellGC, *Te;
+  /* velocity grid (VparGLT/GHT/GCGT/dVparGT, nvpar; VperpGLT/GHT/GCGT/dVperpGT,
+   * nvperp; d3vCT, nvperp*nvpar as [Vperpind-1][Vparind-1]); contiguous edges. */
+  const double *vparGL, *vparGH, *vparGC, *dvpar;
+  const double *vperpGL, *vperpGH, *vperpGC, *dvperp;
+  const double *d3v;
+} kaos_grid;
+
+/* Per-step summary. */
+typedef struct kaos_step_out {
+  int64_t n_alive;       /* ions pushed this step, all tubes (particle-steps of the step) */
+  int64_t n_injected;    /* ions injected this step */
+  int32_t master;        /* 1 when this step was a master step */
+  int32_t status;        /* KAOS_OK or an error code raised on the device */
+} kaos_step_out;
+
+typedef struct kaos_ctx kaos_ctx;
+
+/* Create a context on cfg->device.  Replaces DataTypeAllocBSub
+ * (_8_0_5_DataTypeAllocB.f90:23) and the flag copies of SimParameterizationSub. */
+int kaos_init(const kaos_cfg* cfg, kaos_ctx** out);
+
+/* Set tube `tube`'s grid, species and flags and reserve `capacity` particle
+ * slots for it.  Replaces the grid arrays built by SimParameterizationSub
+ * (_0_ConfigGridGenerator.f90, _0_0_VelGridGenerator.f90) as read by
+ * KineticSolverSub; GravfieldSub's cell values (_8_7_2_Gravfield.f90:57-60)
+ * are computed here once. */
+int kaos_set_grid(kaos_ctx* ctx, int32_t tube, const kaos_grid* g, int64_t capacity);
+
+/* Initial ions of a tube: dipole q0 (q0T), parallel velocity Vpar0T and
+ * perpendicular speed Vperp0T.  Positions follow from the root solve on the
+ * tube's L-shell and velocities are Vpar0 b-hat.  Replaces x0T..Vz0T of
+ * InitialConditionsSub (_7_InitialConditions.f90:51-101). */
+int kaos_upload_particles(kaos_ctx* ctx, int32_t tube, int64_t n, const double* q0,
+                          const double* vpar0, const double* vperp0);
+
+/* Advance every tube by time step n (n = 1 first).  One call replaces one
+ * pass of ComputationalTimeStepLoop (_8_KineticSolver.f90:54-239):
+ * BoundaryConditionsSub, AmbipolarEfieldSub, GravfieldSub, KineticSolverA/BSub
+ * (KineticRK4UpdateSub push loop and LB/UB injection), ParticleCountsSub,
+ * DistributionFunctionsSub and the six moment subroutines.
+ * uniforms: NULL for the seeded device stream, else 6 uniforms in (0,1] per
+ * injected ion of this step, tube-major, LB ions before UB ions
+ * (the GaussianRNGSub draws of KSB:326/623); n_uniforms their count.
+ * out may be NULL; when given the call waits for the step to finish. */
+int kaos_step(kaos_ctx* ctx, int64_t n, const double* uniforms, int64_t n_uniforms,
+              kaos_step_out* out);
+
+/* Steps n_first .. n_first+nsteps-1 with the seeded stream and no host
+ * synchronisation between steps; returns after the last one finishes.
+ * particle_steps (may be NULL) receives the ions pushed summed over steps. */
+int kaos_run(kaos_ctx* ctx, int64_t n_first, int64_t nsteps, int64_t* particle_steps);
+
+/* Outputs of the last master step of a tube.  Any pointer may be NULL.
+ * nq_counts (nq): NqReNormT; nph_counts (nq*nvperp*nvpar): NphReNormRT;
+ * NqRT (nq); M (6*nq): M0phRT, M1PerpphRT, M1ParphRT, M2phRT, M2PerpphRT,
+ * M2ParphRT; EA (nq): EAmagRT; EG (nq): EGmagRT; esc (4): LB and UB escaped
+ * ion counts and the LB/UB escape fluxes LBoutfluxIonRT/UBoutfluxIonRT
+ * (_8_0_7_BoundaryConditions.f90:201-208, 278-285). */
+int kaos_download_master(kaos_ctx* ctx, int32_t tube, int64_t* nq_counts, int64_t* nph_counts,
+                         double* NqRT, double* M, double* EA, double* EG, double* esc);
+
+/* Number of live ions of a tube. */
+int kaos_count(kaos_ctx* ctx, int32_t tube, int64_t* n);
+
+/* Live ions of a tube in the reference's order (survivors in order, then
+ * injected ions), any pointer may be NULL; n_cap is the capacity of the
+ * caller's arrays.  qk4: stage-4 dipole q of the last push at a step before a
+ * master step; aea/ag: AEAmagN/AGmagN; qind/vparind/vperpind: Qindk1,
+ * Vparindk1, Vperpindk1 of the last push. */
+int kaos_download_particles(kaos_ctx* ctx, int32_t tube, int64_t n_cap, int64_t* n_out,
+                            double* x, double* y, double* z, double* Vx, double* Vy, double* Vz,
+                            double* vperp, double* qk4, double* aea, double* ag,
+                            int32_t* qind, int32_t* vparind, int32_t* vperpind);
+
+/* Text of the last error of a context (or of kaos_init when ctx is NULL). */
+const char* kaos_last_error(kaos_ctx* ctx);
+
+/* Release every device buffer of the context. */
+int kaos_finalize(kaos_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KAOS_HIP_H */

@@ -1,0 +1,84 @@
+"""ctypes view of the libkaos_hip.so C-ABI (include/kaos_hip.h).
+
+This is the binding a Python caller uses; the Fortran driver uses the
+iso_c_binding module in kaos_amd/fortran/kaos_hip_iface.f90 with the same
+layouts.  The library is loaded from the package directory only: a missing or
+stale build raises instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkaos_hip.so")
+
+KAOS_OK, KAOS_E_ARG, KAOS_E_CAPACITY, KAOS_E_DEVICE, KAOS_E_STATE, KAOS_E_NAN = range(6)
+KAOS_FP64, KAOS_MIXED = 0, 1
+
+_dp = C.POINTER(C.c_double)
+
+
+class KaosCfg(C.Structure):
+    _fields_ = [("ntubes", C.c_int32), ("precision", C.c_int32), ("ndatfac", C.c_int32),
+                ("device", C.c_int32), ("seed", C.c_uint64)]
+
+
+GRID_ARRAYS = ("qGL", "qGH", "qGC", "hqC", "dqC", "d3xC", "rGC", "cthGC", "ellGC", "Te",
+               "vparGL", "vparGH", "vparGC", "dvpar", "vperpGL", "vperpGH", "vperpGC", "dvperp",
+               "d3v")
+
+
+class KaosGrid(C.Structure):
+    _fields_ = ([(n, C.c_int32) for n in ("nq", "nvpar", "nvperp", "grav", "eamb", "eambsign",
+                                          "filter_pt", "lbcond", "ubcond", "pad0")]
+                + [("n_inj_lb", C.c_int64), ("n_inj_ub", C.c_int64)]
+                + [(n, C.c_double) for n in ("p", "cphi", "sphi", "h", "ms", "qs", "nsnormfac",
+                                             "Ts_lb", "Ts_ub")]
+                + [(n, _dp) for n in GRID_ARRAYS])
+
+
+class KaosStepOut(C.Structure):
+    _fields_ = [("n_alive", C.c_int64), ("n_injected", C.c_int64), ("master", C.c_int32),
+                ("status", C.c_int32)]
+
+
+EXPORTS = ("kaos_init", "kaos_set_grid", "kaos_upload_particles", "kaos_step", "kaos_run",
+           "kaos_download_master", "kaos_count", "kaos_download_particles", "kaos_last_error",
+           "kaos_finalize")
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load libkaos_hip.so and declare the signatures of every export."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: run __graft_entry__.build() (hipcc, gfx950) first")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    i64p = C.POINTER(C.c_int64)
+    i32p = C.POINTER(C.c_int32)
+    lib.kaos_init.argtypes = [C.POINTER(KaosCfg), C.POINTER(P)]
+    lib.kaos_set_grid.argtypes = [P, C.c_int32, C.POINTER(KaosGrid), C.c_int64]
+    lib.kaos_upload_particles.argtypes = [P, C.c_int32, C.c_int64, _dp, _dp, _dp]
+    lib.kaos_step.argtypes = [P, C.c_int64, _dp, C.c_int64, C.POINTER(KaosStepOut)]
+    lib.kaos_run.argtypes = [P, C.c_int64, C.c_int64, i64p]
+    lib.kaos_download_master.argtypes = [P, C.c_int32, i64p, i64p, _dp, _dp, _dp, _dp, _dp]
+    lib.kaos_count.argtypes = [P, C.c_int32, i64p]
+    lib.kaos_download_particles.argtypes = [P, C.c_int32, C.c_int64, i64p] + [_dp] * 10 + [i32p] * 3
+    lib.kaos_last_error.argtypes = [P]
+    lib.kaos_last_error.restype = C.c_char_p
+    lib.kaos_finalize.argtypes = [P]
+    for n in EXPORTS:
+        if n != "kaos_last_error":
+            getattr(lib, n).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def ptr(a, ct=C.c_double):
+    """Pointer to a contiguous numpy array (None passes NULL)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ct))

@@ -1,0 +1,33 @@
+"""Build libkaos_hip.so in-tree for gfx950 with hipcc.
+
+Flags: -ffp-contract=off so the FP64 path evaluates the same IEEE operations
+as the host build of the shared arithmetic; packed FP32 ops disabled in the
+device code generator.
+"""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def build_lib(verbose=False):
+    src = os.path.join(HERE, "csrc", "kaos_hip.hip")
+    out = os.path.join(HERE, "libkaos_hip.so")
+    tmp = out + ".tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-O3", "-fPIC", "-shared", "-std=c++17",
+           "-ffp-contract=off", *DEVICE_FLAGS, "-Wall", "-Wno-unused-function",
+           "-I", os.path.join(HERE, "..", "include"), src, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + r.stderr)
+    if verbose and r.stderr:
+        print(r.stderr)
+    os.replace(tmp, out)
+    return out
+
+
+if __name__ == "__main__":
+    print(build_lib(verbose=True))

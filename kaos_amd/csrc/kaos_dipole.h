@@ -1,0 +1,631 @@
+/*
+ * kaos_dipole.h -- scalar dipole-geometry, RK4 and random-variate arithmetic of
+ * the KAOS kinetic hot path, compiled for both the gfx950 kernels (hipcc) and
+ * the host (gcc/clang).
+ *
+ * Every function here evaluates a fixed sequence of IEEE-754 basic operations
+ * (+ - * / sqrt, compares, frexp/ldexp) with floating-point contraction
+ * disabled by the build (-ffp-contract=off on both compilers).  Those
+ * operations are correctly rounded on gfx950 and on x86-64 SSE2, so a kernel
+ * and a host caller of the same function on the same inputs produce the same
+ * bits.
+ *
+ * Where the reference evaluates a transcendental of a transcendental, the
+ * exact algebraic value is used instead (all equal in real arithmetic):
+ *   cos(acos(z/r))            -> z/r                  (_KineticMainParams.f90:579-582)
+ *   sin(acos(z/r))            -> sqrt(x^2+y^2)/r      (theta in [0,pi])
+ *   sin(asin(sqrt(w)))        -> sqrt(w)              (_8_1_4_RK4DipolePolynomialSolver.f90:89-90)
+ *   cos(asin(sqrt(w)))        -> sqrt(1-w)
+ *   cos(pi - t)               -> -cos(t)              (_8_1_4_RK4DipolePolynomialSolver.f90:101-105)
+ *   Re{2 i sqrt(e) cos((acos(D/(i e^1.5)) + 4 pi)/3)}
+ *                             -> -2 sqrt(e) sinh(asinh(D/e^1.5)/3)
+ *                                                     (_8_1_4_RK4DipolePolynomialSolver.f90:58-67)
+ * and w = sinh(asinh(y)/3) is obtained as the real root of the triple-angle
+ * identity 4 w^3 + 3 w - y = 0 by a fixed number of Newton steps.
+ */
+#ifndef KAOS_DIPOLE_H
+#define KAOS_DIPOLE_H
+
+#include <stdint.h>
+#include <math.h>
+
+#if defined(__HIPCC__)
+#define KAOS_HD __host__ __device__ static inline
+#else
+#define KAOS_HD static inline
+#endif
+
+/* Physical constants, _ModelInput.f90:35-43 (same decimal literals). */
+#define KAOS_RE   6.38e6          /* RE= 638d4   [m]               */
+#define KAOS_RE2  4.07044e13      /* RE**2 (exact in binary64)     */
+#define KAOS_GG   6.67e-11        /* GG= 667d-13                   */
+#define KAOS_ME   5.98e24         /* ME= 598d22  [kg]              */
+#define KAOS_MDIP 8.06e15         /* m= 8.06d15  [T m^3]           */
+#define KAOS_KB   1.38e-23        /* kB= 138d-25                   */
+#define KAOS_PI   3.14159265358979311600   /* 4*atan(1d0) in binary64 */
+#define KAOS_TWOPI 6.28318530717958623200
+#define KAOS_ZERO_CUT 1e-6        /* _8_1_4_RK4DipolePolynomialSolver.f90:115-125 */
+
+/* ------------------------------------------------------------------------- */
+/* Geometry at a Cartesian point on the field line.                          */
+
+typedef struct {
+  double r;            /* |x|                                    */
+  double c;            /* cos(theta)                             */
+  double s;            /* sin(theta)                             */
+  double q;            /* dipole q = RE^2 cos(theta)/r^2         */
+  double bx, by, bz;   /* unit field direction b-hat             */
+} kaos_geom_t;
+
+/* rsub/thetasub/qsub/ellsub (_KineticMainParams.f90:568-615) and the
+ * field-aligned unit vector used by every "RE-ALIGN VELOCITY" and force
+ * reconstruction block (e.g. _8_1_1_KineticUpdateA.f90:978-980, 1327-1334). */
+KAOS_HD kaos_geom_t kaos_geom(double x, double y, double z, double cphi, double sphi) {
+  kaos_geom_t g;
+  double xy2 = x * x + y * y;
+  g.r = sqrt(xy2 + z * z);
+  g.c = z / g.r;
+  g.s = sqrt(xy2) / g.r;
+  g.q = (KAOS_RE2 * g.c) / (g.r * g.r);
+  double ell = 1.0 + 3.0 * (g.c * g.c);
+  double sl = sqrt(ell);
+  double t = (3.0 * g.c) * g.s;
+  g.bx = (cphi * t) / sl;
+  g.by = (sphi * t) / sl;
+  g.bz = (3.0 * (g.c * g.c) - 1.0) / sl;
+  return g;
+}
+
+/* q only (used on the RK4 predictor position, _8_1_KineticRK4Update.f90:161-163). */
+KAOS_HD double kaos_q_of(double x, double y, double z) {
+  double r = sqrt((x * x + y * y) + z * z);
+  double c = z / r;
+  return (KAOS_RE2 * c) / (r * r);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Resolvent-cubic root: w = sinh(asinh(y)/3), y >= 0, i.e. the real root of
+ * 4 w^3 + 3 w - y = 0.  Seeds: Taylor y/3 - 4y^3/81 below 1, a frexp-based
+ * cube-root estimate of (y/4)^(1/3) above; then 7 Newton steps
+ * w <- (8 w^3 + y)/(12 w^2 + 3).                                            */
+KAOS_HD double kaos_sinh_asinh_third(double y) {
+  double w;
+  if (y < 1.0) {
+    w = y * (1.0 / 3.0) - (4.0 / 81.0) * ((y * y) * y);
+  } else {
+    int e;
+    double m = frexp(0.25 * y, &e);             /* 0.25y = m 2^e, m in [0.5,1) */
+    int e3 = (e >= 0) ? (e / 3) : -((-e + 2) / 3);
+    int rem = e - 3 * e3;                       /* 0,1,2 */
+    double mm = ldexp(m, rem);                  /* in [0.5, 4) */
+    double cr = 0.5 + mm * (0.55 - mm * 0.05);  /* crude cbrt on [0.5,4) */
+    w = ldexp(cr, e3);
+    if (w < 0.3) w = 0.3;
+  }
+  for (int it = 0; it < 7; ++it) {
+    double w2 = w * w;
+    w = (8.0 * (w2 * w) + y) / (12.0 * w2 + 3.0);
+  }
+  return w;
+}
+
+/* RK4DipolePolynomialSolverSub (_8_1_4_RK4DipolePolynomialSolver.f90:26-88):
+ * the positive real root gamma of gamma^4 + C gamma + D = 0 with
+ * C = 1/(p q^2), D = -1/q^2 (Ferrari via the resolvent cubic), returned as
+ * r = gamma RE.  The final subtraction (pii - mu) is the reference's
+ * cancellation-prone step and is kept as is.                               */
+KAOS_HD double kaos_root_r(double q, double p) {
+  double q2 = q * q;
+  double C = 1.0 / (p * q2);                    /* CpRK4 */
+  double eps = (4.0 / q2) / 3.0;                /* EpsilonRK4 = Abb/3, Abb = -4 Dp */
+  double delta = (C * C) / 2.0;                 /* DeltaRK4 = -Bbb/2, Bbb = -Cp^2 */
+  double se = sqrt(eps);
+  double y = delta / (eps * se);
+  double w = kaos_sinh_asinh_third(y);
+  double sigma = (2.0 * se) * w;                /* sigmaRK4 */
+  double mu = sqrt(sigma);                      /* muRK4 */
+  double pii = sqrt((2.0 * C) / mu - mu * mu);  /* piiRK4 */
+  double gamma = 0.5 * (pii - mu);              /* gamma3RK4 */
+  return gamma * KAOS_RE;                       /* r3RK4 */
+}
+
+/* Cartesian point from the signed radius rs (sign of rs = sign used for the
+ * hemisphere phase shift: negative when q <= 0), _8_1_4:89-125.            */
+KAOS_HD void kaos_pos_from_rs(double rs, double p, double cphi, double sphi,
+                              double* x, double* y, double* z) {
+  double r = fabs(rs);
+  double w = r / (KAOS_RE * p);
+  double s = sqrt(w);
+  double c = sqrt(1.0 - w);
+  if (rs < 0.0) c = -c;
+  double rsn = r * s;
+  double xx = rsn * cphi, yy = rsn * sphi, zz = r * c;
+  if (fabs(xx) < KAOS_ZERO_CUT) xx = 0.0;
+  if (fabs(yy) < KAOS_ZERO_CUT) yy = 0.0;
+  if (fabs(zz) < KAOS_ZERO_CUT) zz = 0.0;
+  *x = xx; *y = yy; *z = zz;
+}
+
+/* Signed radius of the field-line point with dipole coordinate q. */
+KAOS_HD double kaos_rs_of_q(double q, double p) {
+  double r = kaos_root_r(q, p);
+  return (q <= 0.0) ? -r : r;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Cell searches.  Contiguous edges (qGL(Q+1) == qGH(Q), VGL(i+1) == VGH(i))
+ * make the reference's first-match linear scans equivalent to "first cell
+ * whose high edge is not passed".                                          */
+
+/* Config cell of q (_8_1_1_KineticUpdateA.f90:221-273): 1..nq, 0 = below the
+ * lower boundary, -1 = beyond the upper boundary.  desc != 0 when q decreases
+ * with the cell index (qGL(1) > 0).  qGH is the nq high edges, qGL0 the low
+ * edge of cell 1.  NaN q is reported as 0.                                 */
+KAOS_HD int kaos_find_qcell(double q, const double* qGH, double qGL0, double qGHlast,
+                            int nq, int desc) {
+  if (desc) {
+    if (!(q <= qGL0)) return 0;          /* qGL(NqLB) < q, or NaN */
+    if (qGHlast > q) return -1;
+    int lo = 0, hi = nq - 1;             /* first i with q >= qGH[i] */
+    while (lo < hi) { int mid = (lo + hi) >> 1; if (q >= qGH[mid]) hi = mid; else lo = mid + 1; }
+    return lo + 1;
+  } else {
+    if (!(q >= qGL0)) return 0;          /* qGL(NqLB) > q, or NaN */
+    if (qGHlast < q) return -1;
+    int lo = 0, hi = nq - 1;             /* first i with q <= qGH[i] */
+    while (lo < hi) { int mid = (lo + hi) >> 1; if (q <= qGH[mid]) hi = mid; else lo = mid + 1; }
+    return lo + 1;
+  }
+}
+
+/* Velocity cell of v on ascending contiguous edges (cell 1 is [GL,GH], the
+ * others (GL,GH]; _8_1_1_KineticUpdateA.f90:1087-1127).  1..n, 0 if off grid. */
+KAOS_HD int kaos_find_vcell(double v, const double* GH, double GL0, int n) {
+  if (!(v >= GL0) || !(v <= GH[n - 1])) return 0;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) { int mid = (lo + hi) >> 1; if (v <= GH[mid]) hi = mid; else lo = mid + 1; }
+  return lo + 1;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Field interpolation along the tube (_8_7_AmbipolarEfield.f90:1666-1860 for
+ * q increasing with the cell index, 2067-2230 for q decreasing;
+ * _8_7_2_Gravfield.f90:132-300).  Returns 1 and the interpolated value when q
+ * falls in [qGL(LB), qGH(UB)]; 0 (value untouched) otherwise.             */
+KAOS_HD int kaos_interp_cells(double q, const double* qGC, const double* E, int nq,
+                              double qGL0, double qGHlast, int desc, double* out) {
+  int ub = nq - 1;
+  if (desc) {
+    if (!(q <= qGL0) || !(q >= qGHlast)) return 0;
+    if (q >= qGC[0]) { *out = E[0]; return 1; }          /* LB half cell */
+    if (q <= qGC[ub]) { *out = E[ub]; return 1; }        /* UB half cell */
+    int lo = 1, hi = ub;                 /* first Q with q >= qGC[Q] */
+    while (lo < hi) { int mid = (lo + hi) >> 1; if (q >= qGC[mid]) hi = mid; else lo = mid + 1; }
+    double x1 = qGC[lo - 1], x2 = qGC[lo], y1 = E[lo - 1], y2 = E[lo];
+    *out = y1 + (((q - x1) / (x2 - x1)) * (y2 - y1));
+    return 1;
+  } else {
+    if (!(q >= qGL0) || !(q <= qGHlast)) return 0;
+    if (q <= qGC[0]) { *out = E[0]; return 1; }
+    if (q >= qGC[ub]) { *out = E[ub]; return 1; }
+    int lo = 1, hi = ub;                 /* first Q with q <= qGC[Q] */
+    while (lo < hi) { int mid = (lo + hi) >> 1; if (q <= qGC[mid]) hi = mid; else lo = mid + 1; }
+    double x1 = qGC[lo - 1], x2 = qGC[lo], y1 = E[lo - 1], y2 = E[lo];
+    *out = y1 + (((q - x1) / (x2 - x1)) * (y2 - y1));
+    return 1;
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* One RK4 push of one ion (_8_1_KineticRK4Update.f90:29-314 with
+ * KineticUpdateA/B/C).  Forces: gravity and ambipolar field only (mirror and
+ * parallel-potential flags are off on this path).  The parallel acceleration
+ * of each stage is (sG |AG| + sE AEA) and the Cartesian acceleration is that
+ * times b-hat; the reference projects each force separately and sums, which
+ * is the same in real arithmetic.                                          */
+
+typedef struct {
+  double h;            /* time step SpecieT%hT */
+  double p;            /* L-shell pGCGT */
+  double cphi, sphi;   /* cos/sin of the tube longitude phiGCGT */
+  int grav;            /* GRAVflagT */
+  int eamb;            /* EAMBflagT */
+  int eambsign;        /* EAMBSIGNflagT */
+} kaos_push_par_t;
+
+typedef struct {
+  double xN, yN, zN;     /* new position (xN,yN,zN)                       */
+  double VxN, VyN, VzN;  /* new velocity                                  */
+  double rsN;            /* signed radius of the new position              */
+  double vparN;          /* VparNp: parallel velocity of the new state     */
+  double vpar;           /* Vpar(j): parallel velocity of the input state  */
+  double qk1;            /* q of the input state (cell search input)       */
+  double qk4;            /* q at the full-step predictor (field coordinate) */
+} kaos_push_out_t;
+
+KAOS_HD double kaos_apar(double q, double AG, double AEA, const kaos_push_par_t* P, int with_grav) {
+  double a = 0.0;
+  if (P->grav && with_grav) a = ((q <= 0.0) ? 1.0 : -1.0) * fabs(AG);        /* AGmagS, KUA:480-484 */
+  if (P->eamb) {
+    if (P->eambsign) a = a + AEA;                                             /* AEAmagS=1, KUA:656-658 */
+    else a = a + ((q <= 0.0) ? -1.0 : 1.0) * fabs(AEA);                       /* KUA:649-655 */
+  }
+  return a;
+}
+
+/* Stage A..C and the combine.  valid_q = the input state's config cell is in
+ * the domain (stage-4 gravity gate, _8_1_3_KineticUpdateC.f90:345-346).    */
+KAOS_HD void kaos_push_ion(double x, double y, double z, double Vx, double Vy, double Vz,
+                           double AG, double AEA, const kaos_push_par_t* P,
+                           const double* qGH, double qGL0, double qGHlast, int nq, int desc,
+                           kaos_push_out_t* o, int* qcell_io) {
+  const double h = P->h, h2 = P->h / 2.0, hsq2 = (P->h * P->h) / 2.0, h6 = P->h / 6.0;
+  /* Stage A (KineticUpdateASub) */
+  kaos_geom_t g1 = kaos_geom(x, y, z, P->cphi, P->sphi);
+  int qc = kaos_find_qcell(g1.q, qGH, qGL0, qGHlast, nq, desc);
+  *qcell_io = qc;
+  double ap1 = kaos_apar(g1.q, AG, AEA, P, 1);
+  double a1x = ap1 * g1.bx, a1y = ap1 * g1.by, a1z = ap1 * g1.bz;
+  double vpar = (g1.bx * Vx + g1.by * Vy) + g1.bz * Vz;
+  double VRx = vpar * g1.bx, VRy = vpar * g1.by, VRz = vpar * g1.bz;
+  /* half-step position, stage B (KineticUpdateBSub) */
+  double xk2 = x + h2 * VRx, yk2 = y + h2 * VRy, zk2 = z + h2 * VRz;
+  kaos_geom_t g2 = kaos_geom(xk2, yk2, zk2, P->cphi, P->sphi);
+  double ap2 = kaos_apar(g2.q, AG, AEA, P, 1);
+  double a2x = ap2 * g2.bx, a2y = ap2 * g2.by, a2z = ap2 * g2.bz;
+  /* full-step predictor, stage C (KineticUpdateCSub) */
+  double xk4 = (x + h * VRx) + hsq2 * a1x;
+  double yk4 = (y + h * VRy) + hsq2 * a1y;
+  double zk4 = (z + h * VRz) + hsq2 * a1z;
+  kaos_geom_t g4 = kaos_geom(xk4, yk4, zk4, P->cphi, P->sphi);
+  double ap4 = kaos_apar(g4.q, AG, AEA, P, (qc != 0 && qc != -1));
+  double a4x = ap4 * g4.bx, a4y = ap4 * g4.by, a4z = ap4 * g4.bz;
+  /* velocity: k1 + 2 k2 + 2 k3 + k4 with k2 = k3 (_8_1_KineticRK4Update.f90:85-90) */
+  double VNx = VRx + h6 * (((a1x + 2.0 * a2x) + 2.0 * a2x) + a4x);
+  double VNy = VRy + h6 * (((a1y + 2.0 * a2y) + 2.0 * a2y) + a4y);
+  double VNz = VRz + h6 * (((a1z + 2.0 * a2z) + 2.0 * a2z) + a4z);
+  /* position (_8_1_KineticRK4Update.f90:124-156) */
+  double k2x = VRx + h2 * a1x, k2y = VRy + h2 * a1y, k2z = VRz + h2 * a1z;
+  double k4x = VRx + h * a2x, k4y = VRy + h * a2y, k4z = VRz + h * a2z;
+  double xNp = x + h6 * (((VRx + 2.0 * k2x) + 2.0 * k2x) + k4x);
+  double yNp = y + h6 * (((VRy + 2.0 * k2y) + 2.0 * k2y) + k4y);
+  double zNp = z + h6 * (((VRz + 2.0 * k2z) + 2.0 * k2z) + k4z);
+  /* re-project onto the L-shell (root solve) */
+  double qNp = kaos_q_of(xNp, yNp, zNp);
+  double rsN = kaos_rs_of_q(qNp, P->p);
+  double xN, yN, zN;
+  kaos_pos_from_rs(rsN, P->p, P->cphi, P->sphi, &xN, &yN, &zN);
+  /* re-align the velocity with b-hat at the new point (:273-314) */
+  kaos_geom_t gN = kaos_geom(xN, yN, zN, P->cphi, P->sphi);
+  double vparN = (gN.bx * VNx + gN.by * VNy) + gN.bz * VNz;
+  o->xN = xN; o->yN = yN; o->zN = zN;
+  o->VxN = vparN * gN.bx; o->VyN = vparN * gN.by; o->VzN = vparN * gN.bz;
+  o->rsN = rsN; o->vparN = vparN; o->vpar = vpar; o->qk1 = g1.q; o->qk4 = g4.q;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Random variates.                                                          */
+
+/* Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11), counter ctr, key key. */
+KAOS_HD void kaos_mulhilo64(uint64_t a, uint64_t b, uint64_t* hi, uint64_t* lo) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  *lo = a * b;
+  *hi = __umul64hi(a, b);
+#else
+  unsigned __int128 pr = (unsigned __int128)a * (unsigned __int128)b;
+  *lo = (uint64_t)pr;
+  *hi = (uint64_t)(pr >> 64);
+#endif
+}
+
+KAOS_HD void kaos_philox4x64(const uint64_t ctr_in[4], const uint64_t key_in[2], uint64_t out[4]) {
+  uint64_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+  uint64_t k0 = key_in[0], k1 = key_in[1];
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) { k0 += 0x9E3779B97F4A7C15ULL; k1 += 0xBB67AE8584CAA73BULL; }
+    uint64_t hi0, lo0, hi1, lo1;
+    kaos_mulhilo64(0xD2E7470EE14C6C93ULL, c0, &hi0, &lo0);
+    kaos_mulhilo64(0xCA5A826395121157ULL, c2, &hi1, &lo1);
+    uint64_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* 64 random bits -> uniform double in (0, 1]. */
+KAOS_HD double kaos_u01(uint64_t b) {
+  return (double)((b >> 11) + 1ULL) * (1.0 / 9007199254740992.0);
+}
+
+/* Natural log for finite x > 0: frexp reduction to [sqrt(1/2), sqrt(2)),
+ * log m = 2 atanh((m-1)/(m+1)) by its odd series, plus e ln2 in two parts. */
+KAOS_HD double kaos_log(double x) {
+  int e;
+  double m = frexp(x, &e);
+  if (m < 0.70710678118654752440) { m = m * 2.0; e = e - 1; }
+  double f = (m - 1.0) / (m + 1.0);
+  double s = f * f;
+  double pp = 1.0 / 25.0;
+  pp = pp * s + 1.0 / 23.0; pp = pp * s + 1.0 / 21.0; pp = pp * s + 1.0 / 19.0;
+  pp = pp * s + 1.0 / 17.0; pp = pp * s + 1.0 / 15.0; pp = pp * s + 1.0 / 13.0;
+  pp = pp * s + 1.0 / 11.0; pp = pp * s + 1.0 / 9.0;  pp = pp * s + 1.0 / 7.0;
+  pp = pp * s + 1.0 / 5.0;  pp = pp * s + 1.0 / 3.0;
+  double f2 = 2.0 * f;
+  double lm = f2 + (f2 * s) * pp;
+  double de = (double)e;
+  return de * 6.93147180369123816490e-01 + (de * 1.90821492927058770002e-10 + lm);
+}
+
+/* cos(2 pi u) for u in [0, 1]: octant reduction (exact subtractions) and the
+ * Taylor series of cos/sin on [-pi/4, pi/4].                               */
+KAOS_HD double kaos_cos_poly(double x) {
+  double s = x * x;
+  double p = -1.0 / 2432902008176640000.0;          /* -1/20! */
+  p = p * s + 1.0 / 6402373705728000.0;             /* 1/18! */
+  p = p * s - 1.0 / 20922789888000.0;               /* 1/16! */
+  p = p * s + 1.0 / 87178291200.0;
+  p = p * s - 1.0 / 479001600.0;
+  p = p * s + 1.0 / 3628800.0;
+  p = p * s - 1.0 / 40320.0;
+  p = p * s + 1.0 / 720.0;
+  p = p * s - 1.0 / 24.0;
+  p = p * s + 0.5;
+  return 1.0 - s * p;    /* 1 - x^2/2 + x^4/24 - ... */
+}
+KAOS_HD double kaos_sin_poly(double x) {
+  double s = x * x;
+  double p = 1.0 / 121645100408832000.0;            /* 1/19! */
+  p = p * s - 1.0 / 355687428096000.0;              /* 1/17! */
+  p = p * s + 1.0 / 1307674368000.0;
+  p = p * s - 1.0 / 6227020800.0;
+  p = p * s + 1.0 / 39916800.0;
+  p = p * s - 1.0 / 362880.0;
+  p = p * s + 1.0 / 5040.0;
+  p = p * s - 1.0 / 120.0;
+  p = p * s + 1.0 / 6.0;
+  return x - (x * s) * p;
+}
+KAOS_HD double kaos_cos2pi(double u) {
+  double t = u;
+  if (t > 0.5) t = 1.0 - t;                     /* cos symmetric about 1/2 */
+  if (t <= 0.125) return kaos_cos_poly(KAOS_TWOPI * t);
+  if (t <= 0.375) return kaos_sin_poly(KAOS_TWOPI * (0.25 - t));
+  return -kaos_cos_poly(KAOS_TWOPI * (0.5 - t));
+}
+
+/* The seeded uniform stream of one injected ion: six uniforms (Vx, Vy, Vz
+ * Box-Muller pairs, _6_1_MBVelocityDistribution.f90:34-47) from two Philox
+ * blocks keyed by (seed, "KAOS") with counter (ion k of the step, step n,
+ * tube, block).                                                            */
+KAOS_HD void kaos_inject_uniforms(uint64_t seed, int64_t step, int tube, int64_t k, double u[6]) {
+  uint64_t key[2] = { seed, 0x4B414F53ULL };
+  uint64_t ctr[4] = { (uint64_t)k, (uint64_t)step, (uint64_t)tube, 0x494E4A00ULL };
+  uint64_t o[4];
+  kaos_philox4x64(ctr, key, o);
+  u[0] = kaos_u01(o[0]); u[1] = kaos_u01(o[1]); u[2] = kaos_u01(o[2]); u[3] = kaos_u01(o[3]);
+  ctr[3] = 0x494E4A01ULL;
+  kaos_philox4x64(ctr, key, o);
+  u[4] = kaos_u01(o[0]); u[5] = kaos_u01(o[1]);
+}
+
+/* Box-Muller Gaussian (_3_1_GaussianRNG.f90:31-33). */
+KAOS_HD double kaos_gauss(double u1, double u2) {
+  return sqrt(-2.0 * kaos_log(u1)) * kaos_cos2pi(u2);
+}
+
+/* Maxwell-Boltzmann injection velocity (_6_1_MBVelocityDistribution.f90:29-108)
+ * at the field-line point with geometry g: projection of an isotropic
+ * Cartesian Maxwellian onto (Vp, Vq, Vphi); Vperp1 = Vp, Vperp2 = Vphi,
+ * Vperp = |(Vperp1, Vperp2)|, Vpar = Vq.                                    */
+KAOS_HD void kaos_mb_project(const double u[6], double sigma, double cphi, double sphi,
+                             kaos_geom_t g, double* vpar, double* vperp1,
+                             double* vperp2, double* vperp) {
+  double Vx = sigma * kaos_gauss(u[0], u[1]);
+  double Vy = sigma * kaos_gauss(u[2], u[3]);
+  double Vz = sigma * kaos_gauss(u[4], u[5]);
+  double ell = 1.0 + 3.0 * (g.c * g.c);
+  double sl = sqrt(ell);
+  double h = Vx * cphi + Vy * sphi;
+  double Vp = ((1.0 - 3.0 * (g.c * g.c)) / sl) * h + ((3.0 * Vz) * g.c * g.s) / sl;
+  double Vq = (((3.0 * g.c) * g.s) / sl) * h + (Vz * (3.0 * (g.c * g.c) - 1.0)) / sl;
+  double Vphi = -Vx * sphi + Vy * cphi;
+  *vperp1 = Vp;
+  *vperp2 = Vphi;
+  *vperp = fabs(sqrt(Vp * Vp + Vphi * Vphi));
+  *vpar = Vq;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Per-tube field arithmetic (run once per master step per tube).            */
+
+/* MomentFilterSub, _8_7_1_MomentFilter.f90:25-147: asymmetric sliding average
+ * with window W (odd) and shrinking one-sided windows near LB/UB.  in/out are
+ * 0-based arrays of the nq cells.                                          */
+KAOS_HD void kaos_moment_filter(const double* in, double* out, int nq, int W) {
+  int hw = (W - 1) / 2;
+  int LB = 1, UB = nq;
+  for (int Q = LB; Q <= UB; ++Q) {
+    double s1, s2, a1, a2;
+    if (Q <= LB - 1 + hw) {
+      s1 = in[Q - 1] + in[Q];
+      for (int k = 2; k <= hw; ++k) s1 = s1 + in[Q + k - 1];
+      a1 = s1 / (double)(hw + 1);
+      if (Q == LB) out[Q - 1] = a1;
+      else if (Q == LB + 1) { a2 = in[LB - 1]; out[Q - 1] = (a1 + a2) / 2.0; }
+      else {
+        s2 = in[Q - 2];
+        for (int k = 2; k <= Q - LB; ++k) s2 = s2 + in[Q - k - 1];
+        a2 = s2 / (double)(Q - LB);
+        out[Q - 1] = (a1 + a2) / 2.0;
+      }
+    }
+    if (Q > LB - 1 + hw && Q < UB - hw) {
+      s1 = in[Q - 1] + in[Q];
+      for (int k = 2; k <= hw; ++k) s1 = s1 + in[Q + k - 1];
+      s2 = in[Q - 2];
+      for (int k = 2; k <= hw; ++k) s2 = s2 + in[Q - k - 1];
+      a1 = s1 / (double)(hw + 1);
+      a2 = s2 / (double)hw;
+      out[Q - 1] = (a1 + a2) / 2.0;
+    }
+    if (Q >= UB - hw) {
+      s2 = in[Q - 1] + in[Q - 2];
+      for (int k = 2; k <= hw; ++k) s2 = s2 + in[Q - k - 1];
+      a2 = s2 / (double)(hw + 1);
+      if (Q == UB) out[Q - 1] = a2;
+      else if (Q == UB - 1) { a1 = in[UB - 1]; out[Q - 1] = (a1 + a2) / 2.0; }
+      else {
+        s1 = in[Q];
+        for (int k = 2; k <= UB - Q; ++k) s1 = s1 + in[Q + k - 1];
+        a1 = s1 / (double)(UB - Q);
+        out[Q - 1] = (a1 + a2) / 2.0;
+      }
+    }
+  }
+}
+
+/* AmbipolarEfieldSub electron-pressure field with the moment filter on,
+ * self-consistent density and no inertial term (_8_7_AmbipolarEfield.f90:
+ * 34-82 filter of M0(nn-1); 622-1269 stencils: 5-point one-sided at LB/UB,
+ * 3-point central at LB+1/UB-1, 5-point central inside; 1276-1287 LB..LB+2
+ * copy of LB+3; 1297-1299 EAmag = EAInertial(=0) + EAPressure).  south != 0
+ * when qGL(1) <= 0.  M0F receives the filtered density M0FiltAvrgRT.       */
+/* Stencil part: EA from the filtered density n (M0FiltAvrgRT) and the raw
+ * previous-step density M0prev (which only gates cells with no ions). */
+KAOS_HD void kaos_ambipolar_stencil(int nq, int south, int eamb, int eambsign, double qs,
+                                    const double* hqC, const double* dqC, const double* Te,
+                                    const double* M0prev, const double* n, double* EA) {
+  for (int Q = 0; Q < nq; ++Q) {
+    if (M0prev[Q] == 0.0 || !eamb) { EA[Q] = 0.0; continue; }
+    double dq = dqC[Q];
+    double R1 = (1.0 / fabs(hqC[Q])) * ((KAOS_KB * Te[Q]) / (qs * n[Q]));
+    double R2;
+    if (Q == 0) {
+      double S = (((-25.0 * n[Q] + 48.0 * n[Q + 1]) - 36.0 * n[Q + 2]) + 16.0 * n[Q + 3]) - 3.0 * n[Q + 4];
+      if (!eambsign) R2 = fabs(S) / (12.0 * dq);
+      else R2 = south ? (-S) / (-12.0 * dq) : (-S) / (12.0 * dq);
+    } else if (Q == nq - 1) {
+      double S = (((-25.0 * n[Q] + 48.0 * n[Q - 1]) - 36.0 * n[Q - 2]) + 16.0 * n[Q - 3]) - 3.0 * n[Q - 4];
+      if (!eambsign) R2 = fabs(S) / (-12.0 * dq);
+      else R2 = south ? (-S) / (12.0 * dq) : (-S) / (-12.0 * dq);
+    } else if (Q == 1 || Q == nq - 2) {
+      double S = n[Q + 1] - n[Q - 1];
+      if (!eambsign) R2 = fabs(S) / (2.0 * dq);
+      else R2 = south ? (-S) / (-2.0 * dq) : (-S) / (2.0 * dq);
+    } else {
+      double S = ((n[Q - 2] - 8.0 * n[Q - 1]) + 8.0 * n[Q + 1]) - n[Q + 2];
+      if (!eambsign) R2 = fabs(S) / (12.0 * dq);
+      else R2 = south ? (-S) / (-12.0 * dq) : (-S) / (12.0 * dq);
+    }
+    double P = R1 * R2;
+    EA[Q] = eambsign ? P : fabs(P);
+  }
+  double e3 = EA[3];
+  for (int Q = 0; Q < 3; ++Q)
+    if (M0prev[Q] != 0.0 && eamb) EA[Q] = e3;
+}
+
+KAOS_HD void kaos_ambipolar(int nq, int south, int eamb, int eambsign, int W, double qs,
+                            const double* hqC, const double* dqC, const double* Te,
+                            const double* M0prev, double* M0F, double* EA) {
+  kaos_moment_filter(M0prev, M0F, nq, W);
+  kaos_ambipolar_stencil(nq, south, eamb, eambsign, qs, hqC, dqC, Te, M0prev, M0F, EA);
+}
+
+/* GravfieldSub cell value, _8_7_2_Gravfield.f90:57-60. */
+KAOS_HD double kaos_grav_cell(int grav, double ms, double qs, double cth, double rc, double ellc) {
+  if (!grav) return 0.0;
+  return (-2.0 * KAOS_GG * KAOS_ME * ms * cth) / (qs * (rc * rc) * sqrt(ellc));
+}
+
+/* Particle field values from a cell-interpolated E (_8_7_AmbipolarEfield.f90:
+ * 1477-1493 with EAMBSIGN, _8_7_2_Gravfield.f90:156-162). */
+KAOS_HD double kaos_aea_of(double e_interp, double qsms, int eambsign) {
+  double e = eambsign ? e_interp : fabs(e_interp);
+  if (e == 0.0) return 0.0;
+  return eambsign ? qsms * e : fabs(qsms * e);
+}
+KAOS_HD double kaos_ag_of(double e_interp, double qsms) {
+  double e = fabs(e_interp);
+  if (e == 0.0) return 0.0;
+  return fabs(qsms * e);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Moments of one config cell from its 2-D phase-space counts (row-major
+ * [Vperpind-1][Vparind-1]): IonParticleCountsSub normalisation
+ * (_8_3_1:118-121, 713-725), IonDistributionFunctionsSub F = NphRT/d3v
+ * (_8_4_1:148-158), NonUniform2DIntegratorSub MM = 2 pi sum dV g
+ * (_8_5:29-43) and the integrands of _8_6_1.._8_6_6 (2-D branch, centring
+ * flags perp = 0, par = 1).  Summation order: for each Vpar column b the sum
+ * over Vperp rows a, then the column sums in b order.                      */
+
+typedef struct {
+  int nvpar, nvperp;
+  const double *d3v, *dvperp, *dvpar, *vperpGC, *vparGC;
+} kaos_vgrid_t;
+
+KAOS_HD double kaos_cell_F(uint32_t cnt, double k, double NqRT, double d3v) {
+  double c = (double)cnt;
+  double NphRT = (c == 0.0) ? 0.0 : c * k;
+  return (NqRT == 0.0) ? 0.0 : NphRT / d3v;
+}
+
+/* Pass 1 column b: sums of dV*g for M0, M1perp, M1par, M2perp. */
+KAOS_HD void kaos_mom_col1(const kaos_vgrid_t* v, const uint32_t* cnt, int b, double k,
+                           double NqRT, double o[4]) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s4 = 0.0;
+  for (int a = 0; a < v->nvperp; ++a) {
+    int i = a * v->nvpar + b;
+    double f = kaos_cell_F(cnt[i], k, NqRT, v->d3v[i]);
+    double w = v->dvperp[a] * v->dvpar[b];
+    double vc = v->vperpGC[a], uc = v->vparGC[b];
+    double g0 = (f == 0.0) ? 0.0 : vc * f;
+    double g1 = (f == 0.0) ? 0.0 : (vc * vc) * f;
+    double g2 = (f == 0.0) ? 0.0 : (vc * uc) * f;
+    double g4 = (f == 0.0) ? 0.0 : (vc * (vc * vc)) * f;
+    s0 = s0 + w * g0; s1 = s1 + w * g1; s2 = s2 + w * g2; s4 = s4 + w * g4;
+  }
+  o[0] = s0; o[1] = s1; o[2] = s2; o[3] = s4;
+}
+
+/* Pass 2 column b: centred M2 (_8_6_4:292-345) and M2par (_8_6_6:272-299),
+ * with the reference's fallback to the uncentred integrand where the centred
+ * one is NaN or zero. */
+KAOS_HD void kaos_mom_col2(const kaos_vgrid_t* v, const uint32_t* cnt, int b, double k,
+                           double NqRT, double M1par, double o[2]) {
+  double s3 = 0.0, s5 = 0.0;
+  for (int a = 0; a < v->nvperp; ++a) {
+    int i = a * v->nvpar + b;
+    double f = kaos_cell_F(cnt[i], k, NqRT, v->d3v[i]);
+    double w = v->dvperp[a] * v->dvpar[b];
+    double vc = v->vperpGC[a], uc = v->vparGC[b];
+    double g3 = 0.0, g5 = 0.0;
+    if (f != 0.0) {
+      double d = uc - M1par;
+      g3 = ((vc * (vc * vc)) * f) + ((vc * (d * d)) * f);
+      if (g3 != g3 || g3 == 0.0) g3 = ((vc * (vc * vc)) * f) + ((vc * (uc * uc)) * f);
+      g5 = (vc * (d * d)) * f;
+      if (g5 != g5 || g5 == 0.0) g5 = (vc * (uc * uc)) * f;
+    }
+    s3 = s3 + w * g3; s5 = s5 + w * g5;
+  }
+  o[0] = s3; o[1] = s5;
+}
+
+/* M from the pass sums.  M[0..5] = M0, M1perp, M1par, M2, M2perp, M2par. */
+KAOS_HD void kaos_mom_finish1(const double S[4], double NqRT, double ms, double M[6]) {
+  double MM0 = (2.0 * KAOS_PI) * S[0];
+  double M0 = (NqRT == 0.0) ? 0.0 : MM0;
+  M[0] = M0;
+  M[1] = (M0 == 0.0) ? 0.0 : ((2.0 * KAOS_PI) * S[1]) / M0;
+  M[2] = (M0 == 0.0) ? 0.0 : ((2.0 * KAOS_PI) * S[2]) / M0;
+  M[4] = (M0 == 0.0) ? 0.0 : (ms / 2.0) * (((2.0 * KAOS_PI) * S[3]) / M0);
+}
+KAOS_HD void kaos_mom_finish2(const double S[2], double ms, double M[6]) {
+  double M0 = M[0];
+  M[3] = (M0 == 0.0) ? 0.0 : (ms / 2.0) * (((2.0 * KAOS_PI) * S[0]) / M0);
+  M[5] = (M0 == 0.0) ? 0.0 : (ms / 2.0) * (((2.0 * KAOS_PI) * S[1]) / M0);
+}
+
+#endif /* KAOS_DIPOLE_H */

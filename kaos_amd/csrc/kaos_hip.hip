@@ -1,0 +1,865 @@
+// kaos_hip.hip -- gfx950 kernels and C-ABI of libkaos_hip.so: the KAOS
+// per-timestep kinetic hot path (push -> bin -> moments -> field -> boundary)
+// on one MI355X.
+//
+// Particle layout (structure of arrays in HBM, one segment per flux tube):
+//   rs    double  signed radius of the ion's field-line point (0 = dead slot)
+//   vp    double  parallel velocity V||; the Cartesian velocity is V|| b-hat
+//   vr    double  perpendicular speed Vperp (constant without wave heating)
+//   qk    double  stage-4 dipole q of the last push before a master step
+//   ae    double  AEAmag, ag double AGmag (per-ion field accelerations)
+//   cell  uint32  (Qindk1+1) << 16 | Vperpind << 8 | Vparind of the last push
+// (rs, vp) determine the reference's Cartesian state (x,y,z,Vx,Vy,Vz) exactly:
+// the push ends every step with xN = pos(rs), VN = VparNp b-hat(xN), so the
+// six Cartesian doubles are recomputed bit for bit from the two stored ones.
+//
+// Ions that leave the domain are marked dead (rs = 0) at the step that
+// precedes a master step, which is when BoundaryConditionsSub removes them;
+// dead slots are skipped, live ions keep their order, injected ions are
+// appended, so the live ions read in slot order are the reference's packed
+// arrays (BCReIndexSub, _8_0_7_1_BCReIndex.f90:33-133).
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+#include "kaos_dipole.h"
+#include "../../include/kaos_hip.h"
+
+#define KAOS_BLOCK 256
+#define KAOS_PPT 4                           // ions per thread per tile
+#define KAOS_TILE (KAOS_BLOCK * KAOS_PPT)
+#define KAOS_MAXQ 512
+#define KAOS_MAXV 255
+
+struct DevTube {
+  int nq, nvpar, nvperp, desc;
+  int grav, eamb, eambsign, filter_pt;
+  int lbcond, ubcond, south, pad;
+  long long off, cap, n_inj_lb, n_inj_ub;
+  long long goff, voff, woff, hoff, doff;    // grid / vpar / vperp / histogram / d3v offsets
+  double p, cphi, sphi, h, ms, qs, qsms, nsnormfac, sig_lb, sig_ub;
+  double qGL0, qGHlast, qGL_last0;
+};
+
+struct StepArgs {
+  const DevTube* tubes;
+  const int* tile_tube;
+  const long long* tile_start;
+  int ntiles, tiles_per_block;
+  // per-cell concatenated arrays
+  const double *qGH, *qGC, *EA, *EG;
+  const double *vparGH, *vperpGH;
+  // particles
+  double *rs, *vp, *vr, *qk, *ae, *ag;
+  uint32_t* cell;
+  const long long* count;
+  unsigned int *hist, *nqc;
+  unsigned long long* esc_acc;
+  unsigned int* status;
+  int do_interp, do_count, do_compact;
+};
+
+__device__ __forceinline__ void set_status(unsigned int* st, unsigned int bit) { atomicOr(st, bit); }
+
+// ---------------------------------------------------------------------------
+// Push + bin (FP64 form).  One RK4 step per live ion (KineticRK4UpdateSub,
+// _8_1_KineticRK4Update.f90:29-314) with, on master steps, the field
+// interpolation at the previous stage-4 q (_8_7_AmbipolarEfield.f90:1453-2720,
+// _8_7_2_Gravfield.f90:132-...) and the phase-space histogram of the
+// pre-push cells (_8_3_1_IonParticleCounts.f90:612-634); on the step before a
+// master step, ions whose pre-push cell is 0/-1 are marked dead
+// (_8_0_7_BoundaryConditions.f90:158-256).
+__global__ void __launch_bounds__(KAOS_BLOCK) k_push_fp64(StepArgs a) {
+  __shared__ double s_qGH[KAOS_MAXQ], s_qGC[KAOS_MAXQ], s_EA[KAOS_MAXQ], s_EG[KAOS_MAXQ];
+  __shared__ double s_vparGH[KAOS_MAXV + 1], s_vperpGH[KAOS_MAXV + 1];
+  const int tid = threadIdx.x;
+  int t0 = blockIdx.x * a.tiles_per_block;
+  int t1 = min(t0 + a.tiles_per_block, a.ntiles);
+  int cur = -1;
+  DevTube T;
+  long long end = 0;
+  for (int tile = t0; tile < t1; ++tile) {
+    int tb = a.tile_tube[tile];
+    if (tb != cur) {
+      __syncthreads();
+      T = a.tubes[tb];
+      for (int i = tid; i < T.nq; i += KAOS_BLOCK) {
+        s_qGH[i] = a.qGH[T.goff + i]; s_qGC[i] = a.qGC[T.goff + i];
+        s_EA[i] = a.EA[T.goff + i];   s_EG[i] = a.EG[T.goff + i];
+      }
+      for (int i = tid; i < T.nvpar; i += KAOS_BLOCK) s_vparGH[i] = a.vparGH[T.voff + i];
+      for (int i = tid; i < T.nvperp; i += KAOS_BLOCK) s_vperpGH[i] = a.vperpGH[T.woff + i];
+      __syncthreads();
+      cur = tb;
+      end = T.off + a.count[tb];
+    }
+    const long long start = a.tile_start[tile];
+    const double vparGL0 = a.vparGH[T.voff + T.nvpar] , vperpGL0 = a.vperpGH[T.woff + T.nvperp];
+    kaos_push_par_t P = { T.h, T.p, T.cphi, T.sphi, T.grav, T.eamb, T.eambsign };
+#pragma unroll 1
+    for (int it = 0; it < KAOS_PPT; ++it) {
+      long long j = start + (long long)it * KAOS_BLOCK + tid;
+      if (j >= end) break;
+      double rs = a.rs[j];
+      if (rs == 0.0) continue;
+      double vs = a.vp[j], vperp = a.vr[j];
+      double aea, ag;
+      if (a.do_interp) {
+        double q = a.qk[j], e;
+        if (kaos_interp_cells(q, s_qGC, s_EA, T.nq, T.qGL0, T.qGHlast, T.desc, &e)) {
+          aea = kaos_aea_of(e, T.qsms, T.eambsign);
+          kaos_interp_cells(q, s_qGC, s_EG, T.nq, T.qGL0, T.qGHlast, T.desc, &e);
+          ag = kaos_ag_of(e, T.qsms);
+          a.ae[j] = aea; a.ag[j] = ag;
+        } else {
+          aea = a.ae[j]; ag = a.ag[j];
+        }
+      } else {
+        aea = a.ae[j]; ag = a.ag[j];
+      }
+      double x, y, z;
+      kaos_pos_from_rs(rs, T.p, T.cphi, T.sphi, &x, &y, &z);
+      kaos_geom_t g = kaos_geom(x, y, z, T.cphi, T.sphi);
+      kaos_push_out_t o;
+      int qc;
+      kaos_push_ion(x, y, z, vs * g.bx, vs * g.by, vs * g.bz, ag, aea, &P,
+                    s_qGH, T.qGL0, T.qGHlast, T.nq, T.desc, &o, &qc);
+      // velocity cells (2-D): off-grid keeps the previous index (KUA:1087-1127)
+      int ip = kaos_find_vcell(o.vpar, s_vparGH, vparGL0, T.nvpar);
+      int iq = kaos_find_vcell(vperp, s_vperpGH, vperpGL0, T.nvperp);
+      if (ip == 0 || iq == 0) {
+        uint32_t old = a.cell[j];
+        if (ip == 0) ip = (int)(old & 0xFFu);
+        if (iq == 0) iq = (int)((old >> 8) & 0xFFu);
+      }
+      if (a.do_count && qc >= 1) {
+        atomicAdd(&a.nqc[T.goff + qc - 1], 1u);
+        if (ip >= 1 && iq >= 1)
+          atomicAdd(&a.hist[T.hoff + ((long long)(qc - 1) * T.nvperp + (iq - 1)) * T.nvpar + (ip - 1)], 1u);
+      }
+      if (a.do_compact && (qc == 0 || qc == -1)) {
+        a.rs[j] = 0.0;
+        atomicAdd(&a.esc_acc[2 * cur + (qc == 0 ? 0 : 1)], 1ull);
+        continue;
+      }
+      if (o.rsN != o.rsN || o.vparN != o.vparN) set_status(a.status, 1u << KAOS_E_NAN);
+      a.rs[j] = o.rsN;
+      a.vp[j] = o.vparN;
+      if (a.do_compact) a.qk[j] = o.qk4;
+      a.cell[j] = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Initial step bookkeeping (KineticSolverASub, KSA:49-326): cells of the
+// initial ions from the clamped q0 and the input Vpar0/Vperp0, counted into
+// the master-step histogram.
+__global__ void __launch_bounds__(KAOS_BLOCK) k_bin_initial(StepArgs a) {
+  int tile = blockIdx.x;
+  if (tile >= a.ntiles) return;
+  int tb = a.tile_tube[tile];
+  DevTube T = a.tubes[tb];
+  long long end = T.off + a.count[tb];
+  const double vparGL0 = a.vparGH[T.voff + T.nvpar], vperpGL0 = a.vperpGH[T.woff + T.nvperp];
+  for (int it = 0; it < KAOS_PPT; ++it) {
+    long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
+    if (j >= end) break;
+    int qc = kaos_find_qcell(a.qk[j], a.qGH + T.goff, T.qGL0, T.qGHlast, T.nq, T.desc);
+    int ip = kaos_find_vcell(a.vp[j], a.vparGH + T.voff, vparGL0, T.nvpar);
+    int iq = kaos_find_vcell(a.vr[j], a.vperpGH + T.woff, vperpGL0, T.nvperp);
+    if (qc >= 1) {
+      atomicAdd(&a.nqc[T.goff + qc - 1], 1u);
+      if (ip >= 1 && iq >= 1)
+        atomicAdd(&a.hist[T.hoff + ((long long)(qc - 1) * T.nvperp + (iq - 1)) * T.nvpar + (ip - 1)], 1u);
+    }
+    a.cell[j] = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip;
+  }
+}
+
+// Upload: positions from the root solve of q0 (InitialConditionsSub), q0
+// clamped into the grid as KSA:55-129 does before the k0 search.
+__global__ void k_upload(DevTube T, long long n, const double* q0, const double* vpar0,
+                         const double* vperp0, double* rs, double* vp, double* vr, double* qk,
+                         double* ae, double* ag, uint32_t* cell) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  long long j = T.off + i;
+  double q = q0[i];
+  rs[j] = kaos_rs_of_q(q, T.p);
+  vp[j] = vpar0[i];
+  vr[j] = vperp0[i];
+  if (T.south) {
+    if (q <= T.qGL0) q = T.qGL0;
+    else if (q > T.qGHlast) q = T.qGHlast;
+  } else {
+    if (q > T.qGL0) q = T.qGL0;
+    else if (q < T.qGHlast) q = T.qGHlast;
+  }
+  qk[j] = q;
+  ae[j] = 0.0; ag[j] = 0.0; cell[j] = 0u;
+}
+
+__global__ void k_fill_u32(unsigned int* p, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long st = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += st) p[i] = 0u;
+}
+
+// Field of a master step (AmbipolarEfieldSub + GravfieldSub cell values), one
+// block per tube; also moves the escape counters accumulated since the last
+// master step into the reported slot (BoundaryConditionsSub, BC:158-256).
+__global__ void k_field(const DevTube* tubes, const double* hq, const double* dq, const double* Te,
+                        const double* Mprev, double* M0F, double* EA,
+                        unsigned long long* esc_acc, unsigned long long* esc_rep, int rotate_esc) {
+  int t = blockIdx.x;
+  if (threadIdx.x != 0) return;
+  DevTube T = tubes[t];
+  const double* M0 = Mprev + 6 * T.goff;  // M0 block of tube t
+  kaos_ambipolar(T.nq, T.south, T.eamb, T.eambsign, T.filter_pt, T.qs, hq + T.goff, dq + T.goff,
+                 Te + T.goff, M0, M0F + T.goff, EA + T.goff);
+  if (rotate_esc) {
+    esc_rep[2 * t] = esc_acc[2 * t]; esc_rep[2 * t + 1] = esc_acc[2 * t + 1];
+    esc_acc[2 * t] = 0ull; esc_acc[2 * t + 1] = 0ull;
+  }
+}
+
+// LB/UB injection after the master push (KSB:195-793): ion k of tube t goes
+// to slot off + count + k.  Uniforms from the caller's stream (6 per ion) or
+// from the seeded Philox stream.
+__global__ void k_inject(const DevTube* tubes, const long long* inj_base, int ntubes,
+                         const double* qGH, const double* qGC, const double* EA, const double* EG,
+                         const double* vparGH, const double* vperpGH,
+                         double* rs, double* vp, double* vr, double* qk, double* ae, double* ag,
+                         uint32_t* cell, const long long* count, unsigned int* hist, unsigned int* nqc,
+                         const double* uni, const long long* uni_base, unsigned long long seed,
+                         long long step, long long total) {
+  long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= total) return;
+  int t = 0;
+  while (t + 1 < ntubes && inj_base[t + 1] <= gi) ++t;
+  DevTube T = tubes[t];
+  long long k = gi - inj_base[t];
+  long long nl = T.lbcond ? T.n_inj_lb : 0;
+  int lb = (k < nl);
+  int Qb = lb ? 0 : T.nq - 1;
+  long long j = T.off + count[t] + k;
+  double qc = qGC[T.goff + Qb];
+  double u[6];
+  if (uni) { for (int i = 0; i < 6; ++i) u[i] = uni[6 * (uni_base[t] + k) + i]; }
+  else kaos_inject_uniforms(seed, step, t, k, u);
+  int qcell = kaos_find_qcell(qc, qGH + T.goff, T.qGL0, T.qGHlast, T.nq, T.desc);
+  double r = kaos_rs_of_q(qc, T.p);
+  double x, y, z;
+  kaos_pos_from_rs(r, T.p, T.cphi, T.sphi, &x, &y, &z);
+  kaos_geom_t g = kaos_geom(x, y, z, T.cphi, T.sphi);
+  double vpar, v1, v2, vperp;
+  kaos_mb_project(u, lb ? T.sig_lb : T.sig_ub, T.cphi, T.sphi, g, &vpar, &v1, &v2, &vperp);
+  int ip = kaos_find_vcell(vpar, vparGH + T.voff, vparGH[T.voff + T.nvpar], T.nvpar);
+  int iq = kaos_find_vcell(vperp, vperpGH + T.woff, vperpGH[T.woff + T.nvperp], T.nvperp);
+  rs[j] = r; vp[j] = vpar; vr[j] = vperp; qk[j] = qc;
+  ae[j] = fabs(T.qsms * fabs(EA[T.goff + Qb]));
+  ag[j] = fabs(T.qsms * fabs(EG[T.goff + Qb]));
+  cell[j] = ((uint32_t)(qcell + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip;
+  if (qcell >= 1) {
+    atomicAdd(&nqc[T.goff + qcell - 1], 1u);
+    if (ip >= 1 && iq >= 1)
+      atomicAdd(&hist[T.hoff + ((long long)(qcell - 1) * T.nvperp + (iq - 1)) * T.nvpar + (ip - 1)], 1u);
+  }
+}
+
+// Counts -> distribution function -> moments of one (tube, config cell) per
+// block (_8_3_1, _8_4_1, _8_5, _8_6_1.._8_6_6).  Thread b sums Vpar column b
+// over Vperp rows; thread 0 adds the columns in order, so the sums are the
+// oracle's, bit for bit.  Block (t, 0) also advances the tube's slot count by
+// the ions injected this step.
+__global__ void __launch_bounds__(KAOS_BLOCK) k_moments(const DevTube* tubes, const int* blk_tube,
+    const int* blk_q, const double* d3xC, const double* d3v, const double* dvpar, const double* dvperp,
+    const double* vparGC, const double* vperpGC, const unsigned int* hist, const unsigned int* nqc,
+    double* NqRT, double* M, long long* count, const long long* inj_cnt) {
+  __shared__ double s_col[KAOS_MAXV + 1][4];
+  __shared__ double s_M[6];
+  int t = blk_tube[blockIdx.x], Q = blk_q[blockIdx.x];
+  DevTube T = tubes[t];
+  int b = threadIdx.x;
+  double k = T.nsnormfac / d3xC[T.goff + Q];
+  double nqr = (double)nqc[T.goff + Q] * k;
+  const uint32_t* N = hist + T.hoff + (long long)Q * T.nvperp * T.nvpar;
+  kaos_vgrid_t v = { T.nvpar, T.nvperp, d3v + T.doff, dvperp + T.woff, dvpar + T.voff,
+                     vperpGC + T.woff, vparGC + T.voff };
+  if (b < T.nvpar) kaos_mom_col1(&v, N, b, k, nqr, s_col[b]);
+  __syncthreads();
+  if (b == 0) {
+    double S[4] = {0, 0, 0, 0}, Mq[6] = {0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < T.nvpar; ++c) for (int i = 0; i < 4; ++i) S[i] = S[i] + s_col[c][i];
+    kaos_mom_finish1(S, nqr, T.ms, Mq);
+    for (int i = 0; i < 6; ++i) s_M[i] = Mq[i];
+  }
+  __syncthreads();
+  double M1par = s_M[2];
+  if (b < T.nvpar) kaos_mom_col2(&v, N, b, k, nqr, M1par, s_col[b]);
+  __syncthreads();
+  if (b == 0) {
+    double S[2] = {0, 0}, Mq[6];
+    for (int i = 0; i < 6; ++i) Mq[i] = s_M[i];
+    for (int c = 0; c < T.nvpar; ++c) { S[0] = S[0] + s_col[c][0]; S[1] = S[1] + s_col[c][1]; }
+    kaos_mom_finish2(S, T.ms, Mq);
+    for (int i = 0; i < 6; ++i) M[6 * T.goff + (long long)i * T.nq + Q] = Mq[i];
+    NqRT[T.goff + Q] = nqr;
+    if (Q == 0 && inj_cnt) count[t] += inj_cnt[t];
+  }
+}
+
+// Cartesian state of a tube segment for download.
+__global__ void k_reconstruct(DevTube T, long long n, const double* rs, const double* vp,
+                              double* out6) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  long long j = T.off + i;
+  double r = rs[j];
+  double x = 0, y = 0, z = 0, vx = 0, vy = 0, vz = 0;
+  if (r != 0.0) {
+    kaos_pos_from_rs(r, T.p, T.cphi, T.sphi, &x, &y, &z);
+    kaos_geom_t g = kaos_geom(x, y, z, T.cphi, T.sphi);
+    double v = vp[j];
+    vx = v * g.bx; vy = v * g.by; vz = v * g.bz;
+  }
+  out6[6 * i + 0] = x; out6[6 * i + 1] = y; out6[6 * i + 2] = z;
+  out6[6 * i + 3] = vx; out6[6 * i + 4] = vy; out6[6 * i + 5] = vz;
+}
+
+// ===========================================================================
+// Host side.
+
+struct HostTube {
+  kaos_grid g;
+  bool set = false, uploaded = false;
+  long long cap = 0, count = 0;
+  std::vector<double> qGL, qGH, qGC, hq, dq, d3x, rGC, cth, ell, Te;
+  std::vector<double> vparGL, vparGH, vparGC, dvpar, vperpGL, vperpGH, vperpGC, dvperp, d3v;
+};
+
+struct kaos_ctx {
+  kaos_cfg cfg;
+  std::string err;
+  std::vector<HostTube> ht;
+  std::vector<DevTube> dt;
+  bool built = false;
+  long long last_step = 0;
+  int last_master = 0;
+  // device buffers
+  DevTube* d_tubes = nullptr;
+  int* d_tile_tube = nullptr; long long* d_tile_start = nullptr; int ntiles = 0;
+  int* d_blk_tube = nullptr; int* d_blk_q = nullptr; int nblk_mom = 0;
+  double *d_qGH = nullptr, *d_qGC = nullptr, *d_hq = nullptr, *d_dq = nullptr, *d_Te = nullptr,
+         *d_d3x = nullptr, *d_EA = nullptr, *d_EG = nullptr, *d_M = nullptr, *d_M0F = nullptr,
+         *d_NqRT = nullptr;
+  double *d_vparGH = nullptr, *d_vperpGH = nullptr, *d_vparGC = nullptr, *d_vperpGC = nullptr,
+         *d_dvpar = nullptr, *d_dvperp = nullptr, *d_d3v = nullptr;
+  double *d_rs = nullptr, *d_vp = nullptr, *d_vr = nullptr, *d_qk = nullptr, *d_ae = nullptr,
+         *d_ag = nullptr;
+  uint32_t* d_cell = nullptr;
+  long long* d_count = nullptr;
+  unsigned int *d_hist = nullptr, *d_nqc = nullptr, *d_status = nullptr;
+  unsigned long long *d_esc_acc = nullptr, *d_esc_rep = nullptr;
+  long long *d_inj_base = nullptr, *d_inj_cnt = nullptr, *d_uni_base = nullptr;
+  double* d_uni = nullptr; long long uni_cap = 0;
+  long long ncell_tot = 0, nhist_tot = 0, cap_tot = 0;
+  long long inj_total = 0;
+  hipStream_t stream = nullptr;
+  std::vector<long long> inj_base_h;
+};
+
+static std::string g_init_err;
+
+#define HIPCHK(ctx, call)                                                          \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);              \
+      return KAOS_E_DEVICE;                                                        \
+    }                                                                              \
+  } while (0)
+
+static int fail(kaos_ctx* c, int code, const std::string& m) { c->err = m; return code; }
+
+extern "C" const char* kaos_last_error(kaos_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : g_init_err.c_str();
+}
+
+extern "C" int kaos_init(const kaos_cfg* cfg, kaos_ctx** out) {
+  if (!cfg || !out) { g_init_err = "null argument"; return KAOS_E_ARG; }
+  if (cfg->ntubes < 1 || cfg->ndatfac < 1 || (cfg->precision != KAOS_FP64 && cfg->precision != KAOS_MIXED)) {
+    g_init_err = "bad kaos_cfg (ntubes >= 1, ndatfac >= 1, precision FP64|MIXED)";
+    return KAOS_E_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0) {
+    g_init_err = "no HIP device " + std::to_string(cfg->device);
+    return KAOS_E_DEVICE;
+  }
+  if (hipSetDevice(cfg->device) != hipSuccess) { g_init_err = "hipSetDevice failed"; return KAOS_E_DEVICE; }
+  kaos_ctx* c = new kaos_ctx();
+  c->cfg = *cfg;
+  c->ht.resize(cfg->ntubes);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c; g_init_err = "hipStreamCreate failed"; return KAOS_E_DEVICE;
+  }
+  *out = c;
+  return KAOS_OK;
+}
+
+static bool contiguous(const double* lo, const double* hi, int n) {
+  for (int i = 0; i + 1 < n; ++i) if (!(lo[i + 1] == hi[i])) return false;
+  return true;
+}
+
+extern "C" int kaos_set_grid(kaos_ctx* c, int32_t tube, const kaos_grid* g, int64_t capacity) {
+  if (!c || !g) return KAOS_E_ARG;
+  if (c->built) return fail(c, KAOS_E_STATE, "kaos_set_grid after the first upload");
+  if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
+  if (g->nq < 11 || g->nq > KAOS_MAXQ) return fail(c, KAOS_E_ARG, "nq must be in [11, 512]");
+  if (g->nvpar < 1 || g->nvpar > KAOS_MAXV || g->nvperp < 1 || g->nvperp > KAOS_MAXV)
+    return fail(c, KAOS_E_ARG, "nvpar/nvperp must be in [1, 255]");
+  if (g->filter_pt < 3 || (g->filter_pt % 2) == 0 || (g->filter_pt - 1) / 2 + 1 > g->nq / 2)
+    return fail(c, KAOS_E_ARG, "filter_pt must be odd, >= 3 and fit the grid");
+  if (capacity < 0) return fail(c, KAOS_E_ARG, "negative capacity");
+  const double* need[] = {g->qGL, g->qGH, g->qGC, g->hqC, g->dqC, g->d3xC, g->rGC, g->cthGC, g->ellGC,
+                          g->Te, g->vparGL, g->vparGH, g->vparGC, g->dvpar, g->vperpGL, g->vperpGH,
+                          g->vperpGC, g->dvperp, g->d3v};
+  for (const double* p : need) if (!p) return fail(c, KAOS_E_ARG, "null grid array");
+  if (!contiguous(g->qGL, g->qGH, g->nq)) return fail(c, KAOS_E_ARG, "config edges not contiguous");
+  if (!contiguous(g->vparGL, g->vparGH, g->nvpar) || !contiguous(g->vperpGL, g->vperpGH, g->nvperp))
+    return fail(c, KAOS_E_ARG, "velocity edges not contiguous");
+  int desc = g->qGL[0] > 0;
+  for (int i = 0; i < g->nq; ++i) {
+    bool ok = desc ? (g->qGL[i] > g->qGH[i] && g->qGL[i] >= g->qGC[i] && g->qGC[i] >= g->qGH[i])
+                   : (g->qGL[i] < g->qGH[i] && g->qGL[i] <= g->qGC[i] && g->qGC[i] <= g->qGH[i]);
+    if (!ok) return fail(c, KAOS_E_ARG, "config cell " + std::to_string(i + 1) + " not monotone");
+  }
+  HostTube& h = c->ht[tube];
+  h.g = *g;
+  int nq = g->nq, np = g->nvpar, nr = g->nvperp;
+  h.qGL.assign(g->qGL, g->qGL + nq); h.qGH.assign(g->qGH, g->qGH + nq); h.qGC.assign(g->qGC, g->qGC + nq);
+  h.hq.assign(g->hqC, g->hqC + nq); h.dq.assign(g->dqC, g->dqC + nq); h.d3x.assign(g->d3xC, g->d3xC + nq);
+  h.rGC.assign(g->rGC, g->rGC + nq); h.cth.assign(g->cthGC, g->cthGC + nq); h.ell.assign(g->ellGC, g->ellGC + nq);
+  h.Te.assign(g->Te, g->Te + nq);
+  h.vparGL.assign(g->vparGL, g->vparGL + np); h.vparGH.assign(g->vparGH, g->vparGH + np);
+  h.vparGC.assign(g->vparGC, g->vparGC + np); h.dvpar.assign(g->dvpar, g->dvpar + np);
+  h.vperpGL.assign(g->vperpGL, g->vperpGL + nr); h.vperpGH.assign(g->vperpGH, g->vperpGH + nr);
+  h.vperpGC.assign(g->vperpGC, g->vperpGC + nr); h.dvperp.assign(g->dvperp, g->dvperp + nr);
+  h.d3v.assign(g->d3v, g->d3v + (size_t)np * nr);
+  h.cap = capacity;
+  h.set = true;
+  return KAOS_OK;
+}
+
+// Allocate the device arrays once every tube's grid is known.
+static int build_device(kaos_ctx* c) {
+  int nt = c->cfg.ntubes;
+  for (int t = 0; t < nt; ++t) if (!c->ht[t].set) return fail(c, KAOS_E_STATE, "tube " + std::to_string(t) + " has no grid");
+  c->dt.resize(nt);
+  long long goff = 0, voff = 0, woff = 0, hoff = 0, doff = 0, off = 0;
+  std::vector<double> qGH, qGC, hq, dq, Te, d3x, EG, vparGHx, vperpGHx, vparGC, vperpGC, dvpar, dvperp, d3v;
+  std::vector<int> tile_tube, blk_tube, blk_q;
+  std::vector<long long> tile_start;
+  for (int t = 0; t < nt; ++t) {
+    HostTube& h = c->ht[t];
+    const kaos_grid& g = h.g;
+    DevTube T;
+    memset(&T, 0, sizeof(T));
+    T.nq = g.nq; T.nvpar = g.nvpar; T.nvperp = g.nvperp; T.desc = g.qGL[0] > 0; T.south = !T.desc;
+    T.grav = g.grav; T.eamb = g.eamb; T.eambsign = g.eambsign; T.filter_pt = g.filter_pt;
+    T.lbcond = g.lbcond; T.ubcond = g.ubcond;
+    T.off = off; T.cap = h.cap; T.n_inj_lb = g.n_inj_lb; T.n_inj_ub = g.n_inj_ub;
+    T.goff = goff; T.voff = voff; T.woff = woff; T.hoff = hoff; T.doff = doff;
+    T.p = g.p; T.cphi = g.cphi; T.sphi = g.sphi; T.h = g.h; T.ms = g.ms; T.qs = g.qs;
+    T.qsms = g.qs / g.ms; T.nsnormfac = g.nsnormfac;
+    T.sig_lb = sqrt((KAOS_KB * g.Ts_lb) / g.ms);
+    T.sig_ub = sqrt((KAOS_KB * g.Ts_ub) / g.ms);
+    T.qGL0 = g.qGL[0]; T.qGHlast = g.qGH[g.nq - 1];
+    c->dt[t] = T;
+    for (int i = 0; i < g.nq; ++i) {
+      qGH.push_back(h.qGH[i]); qGC.push_back(h.qGC[i]); hq.push_back(h.hq[i]); dq.push_back(h.dq[i]);
+      Te.push_back(h.Te[i]); d3x.push_back(h.d3x[i]);
+      EG.push_back(kaos_grav_cell(g.grav, g.ms, g.qs, h.cth[i], h.rGC[i], h.ell[i]));
+      blk_tube.push_back(t); blk_q.push_back(i);
+    }
+    // velocity high edges followed by the low edge of cell 1 (read as GL0)
+    for (int i = 0; i < g.nvpar; ++i) { vparGHx.push_back(h.vparGH[i]); vparGC.push_back(h.vparGC[i]); dvpar.push_back(h.dvpar[i]); }
+    vparGHx.push_back(h.vparGL[0]); vparGC.push_back(0.0); dvpar.push_back(0.0);
+    for (int i = 0; i < g.nvperp; ++i) { vperpGHx.push_back(h.vperpGH[i]); vperpGC.push_back(h.vperpGC[i]); dvperp.push_back(h.dvperp[i]); }
+    vperpGHx.push_back(h.vperpGL[0]); vperpGC.push_back(0.0); dvperp.push_back(0.0);
+    for (size_t i = 0; i < h.d3v.size(); ++i) d3v.push_back(h.d3v[i]);
+    for (long long s = 0; s < h.cap; s += KAOS_TILE) { tile_tube.push_back(t); tile_start.push_back(off + s); }
+    goff += g.nq; voff += g.nvpar + 1; woff += g.nvperp + 1; doff += (long long)g.nvpar * g.nvperp;
+    hoff += (long long)g.nq * g.nvpar * g.nvperp; off += h.cap;
+  }
+  c->ncell_tot = goff; c->nhist_tot = hoff; c->cap_tot = off; c->ntiles = (int)tile_tube.size();
+  c->nblk_mom = (int)blk_tube.size();
+  auto up = [&](void** d, const void* h, size_t bytes) -> hipError_t {
+    hipError_t e = hipMalloc(d, bytes ? bytes : 8);
+    if (e != hipSuccess) return e;
+    if (h && bytes) e = hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice);
+    return e;
+  };
+  HIPCHK(c, up((void**)&c->d_tubes, c->dt.data(), sizeof(DevTube) * nt));
+  HIPCHK(c, up((void**)&c->d_tile_tube, tile_tube.data(), sizeof(int) * tile_tube.size()));
+  HIPCHK(c, up((void**)&c->d_tile_start, tile_start.data(), sizeof(long long) * tile_start.size()));
+  HIPCHK(c, up((void**)&c->d_blk_tube, blk_tube.data(), sizeof(int) * blk_tube.size()));
+  HIPCHK(c, up((void**)&c->d_blk_q, blk_q.data(), sizeof(int) * blk_q.size()));
+  HIPCHK(c, up((void**)&c->d_qGH, qGH.data(), sizeof(double) * goff));
+  HIPCHK(c, up((void**)&c->d_qGC, qGC.data(), sizeof(double) * goff));
+  HIPCHK(c, up((void**)&c->d_hq, hq.data(), sizeof(double) * goff));
+  HIPCHK(c, up((void**)&c->d_dq, dq.data(), sizeof(double) * goff));
+  HIPCHK(c, up((void**)&c->d_Te, Te.data(), sizeof(double) * goff));
+  HIPCHK(c, up((void**)&c->d_d3x, d3x.data(), sizeof(double) * goff));
+  HIPCHK(c, up((void**)&c->d_EG, EG.data(), sizeof(double) * goff));
+  std::vector<double> zeros(6 * goff, 0.0);
+  HIPCHK(c, up((void**)&c->d_EA, zeros.data(), sizeof(double) * goff));
+  HIPCHK(c, up((void**)&c->d_M0F, zeros.data(), sizeof(double) * goff));
+  HIPCHK(c, up((void**)&c->d_NqRT, zeros.data(), sizeof(double) * goff));
+  HIPCHK(c, up((void**)&c->d_M, zeros.data(), sizeof(double) * 6 * goff));
+  HIPCHK(c, up((void**)&c->d_vparGH, vparGHx.data(), sizeof(double) * voff));
+  HIPCHK(c, up((void**)&c->d_vparGC, vparGC.data(), sizeof(double) * voff));
+  HIPCHK(c, up((void**)&c->d_dvpar, dvpar.data(), sizeof(double) * voff));
+  HIPCHK(c, up((void**)&c->d_vperpGH, vperpGHx.data(), sizeof(double) * woff));
+  HIPCHK(c, up((void**)&c->d_vperpGC, vperpGC.data(), sizeof(double) * woff));
+  HIPCHK(c, up((void**)&c->d_dvperp, dvperp.data(), sizeof(double) * woff));
+  HIPCHK(c, up((void**)&c->d_d3v, d3v.data(), sizeof(double) * doff));
+  size_t capb = sizeof(double) * (size_t)(off ? off : 1);
+  HIPCHK(c, hipMalloc((void**)&c->d_rs, capb));
+  HIPCHK(c, hipMalloc((void**)&c->d_vp, capb));
+  HIPCHK(c, hipMalloc((void**)&c->d_vr, capb));
+  HIPCHK(c, hipMalloc((void**)&c->d_qk, capb));
+  HIPCHK(c, hipMalloc((void**)&c->d_ae, capb));
+  HIPCHK(c, hipMalloc((void**)&c->d_ag, capb));
+  HIPCHK(c, hipMalloc((void**)&c->d_cell, sizeof(uint32_t) * (size_t)(off ? off : 1)));
+  HIPCHK(c, hipMemset(c->d_rs, 0, capb));
+  std::vector<long long> zc(nt, 0);
+  HIPCHK(c, up((void**)&c->d_count, zc.data(), sizeof(long long) * nt));
+  HIPCHK(c, hipMalloc((void**)&c->d_hist, sizeof(unsigned int) * (size_t)(hoff ? hoff : 1)));
+  HIPCHK(c, hipMalloc((void**)&c->d_nqc, sizeof(unsigned int) * (size_t)goff));
+  HIPCHK(c, hipMemset(c->d_hist, 0, sizeof(unsigned int) * (size_t)(hoff ? hoff : 1)));
+  HIPCHK(c, hipMemset(c->d_nqc, 0, sizeof(unsigned int) * (size_t)goff));
+  HIPCHK(c, hipMalloc((void**)&c->d_status, sizeof(unsigned int)));
+  HIPCHK(c, hipMemset(c->d_status, 0, sizeof(unsigned int)));
+  std::vector<unsigned long long> ze(2 * nt, 0ull);
+  HIPCHK(c, up((void**)&c->d_esc_acc, ze.data(), sizeof(unsigned long long) * 2 * nt));
+  HIPCHK(c, up((void**)&c->d_esc_rep, ze.data(), sizeof(unsigned long long) * 2 * nt));
+  // injection bookkeeping: ions per tube per master step and their prefix
+  std::vector<long long> ib(nt + 1, 0), ic(nt, 0);
+  for (int t = 0; t < nt; ++t) {
+    const DevTube& T = c->dt[t];
+    ic[t] = (T.lbcond ? T.n_inj_lb : 0) + (T.ubcond ? T.n_inj_ub : 0);
+    ib[t + 1] = ib[t] + ic[t];
+  }
+  c->inj_total = ib[nt];
+  c->inj_base_h = ib;
+  HIPCHK(c, up((void**)&c->d_inj_base, ib.data(), sizeof(long long) * (nt + 1)));
+  HIPCHK(c, up((void**)&c->d_inj_cnt, ic.data(), sizeof(long long) * nt));
+  HIPCHK(c, up((void**)&c->d_uni_base, ib.data(), sizeof(long long) * (nt + 1)));
+  c->built = true;
+  return KAOS_OK;
+}
+
+extern "C" int kaos_upload_particles(kaos_ctx* c, int32_t tube, int64_t n, const double* q0,
+                                     const double* vpar0, const double* vperp0) {
+  if (!c) return KAOS_E_ARG;
+  if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
+  if (n < 0 || (n > 0 && (!q0 || !vpar0 || !vperp0))) return fail(c, KAOS_E_ARG, "bad particle arrays");
+  if (c->last_step != 0) return fail(c, KAOS_E_STATE, "upload after the first step");
+  if (!c->built) { int rc = build_device(c); if (rc) return rc; }
+  HostTube& h = c->ht[tube];
+  if (n > h.cap) return fail(c, KAOS_E_CAPACITY, "more ions than the tube capacity");
+  const DevTube& T = c->dt[tube];
+  double *dq0 = nullptr, *dv0 = nullptr, *dr0 = nullptr;
+  size_t nb = sizeof(double) * (size_t)(n ? n : 1);
+  HIPCHK(c, hipMalloc((void**)&dq0, nb));
+  HIPCHK(c, hipMalloc((void**)&dv0, nb));
+  HIPCHK(c, hipMalloc((void**)&dr0, nb));
+  if (n) {
+    HIPCHK(c, hipMemcpy(dq0, q0, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dv0, vpar0, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dr0, vperp0, sizeof(double) * n, hipMemcpyHostToDevice));
+    int nbk = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(k_upload, dim3(nbk), dim3(256), 0, c->stream, T, (long long)n, dq0, dv0, dr0,
+                       c->d_rs, c->d_vp, c->d_vr, c->d_qk, c->d_ae, c->d_ag, c->d_cell);
+    HIPCHK(c, hipGetLastError());
+  }
+  long long cnt = n;
+  HIPCHK(c, hipMemcpyAsync(c->d_count + tube, &cnt, sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(dq0); (void)hipFree(dv0); (void)hipFree(dr0);
+  h.count = n;
+  h.uploaded = true;
+  return KAOS_OK;
+}
+
+static StepArgs make_args(kaos_ctx* c, int interp, int count, int compact) {
+  StepArgs a;
+  a.tubes = c->d_tubes; a.tile_tube = c->d_tile_tube; a.tile_start = c->d_tile_start;
+  a.ntiles = c->ntiles;
+  a.tiles_per_block = 4;
+  a.qGH = c->d_qGH; a.qGC = c->d_qGC; a.EA = c->d_EA; a.EG = c->d_EG;
+  a.vparGH = c->d_vparGH; a.vperpGH = c->d_vperpGH;
+  a.rs = c->d_rs; a.vp = c->d_vp; a.vr = c->d_vr; a.qk = c->d_qk; a.ae = c->d_ae; a.ag = c->d_ag;
+  a.cell = c->d_cell; a.count = c->d_count; a.hist = c->d_hist; a.nqc = c->d_nqc;
+  a.esc_acc = c->d_esc_acc; a.status = c->d_status;
+  a.do_interp = interp; a.do_count = count; a.do_compact = compact;
+  return a;
+}
+
+static bool is_master(const kaos_ctx* c, long long n) { return n >= 2 && (n % c->cfg.ndatfac) == 0; }
+
+// Enqueue step n on the stream (no host synchronisation).
+static int enqueue_step(kaos_ctx* c, long long n, const double* d_uni) {
+  const int nt = c->cfg.ntubes;
+  const bool master = is_master(c, n);
+  const bool first = (n == 1);
+  const bool premaster = is_master(c, n + 1);
+  auto zero_hist = [&]() {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, c->stream, c->d_hist, c->nhist_tot);
+    hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, c->stream, c->d_nqc, c->ncell_tot);
+  };
+  if (first) {
+    StepArgs a = make_args(c, 1, 0, premaster);
+    zero_hist();
+    hipLaunchKernelGGL(k_bin_initial, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a);
+    // field of master step 1 from "master step 0" (no moments: M = 0), then moments of step 1
+    hipLaunchKernelGGL(k_field, dim3(nt), dim3(64), 0, c->stream, c->d_tubes, c->d_hq, c->d_dq, c->d_Te,
+                       c->d_M, c->d_M0F, c->d_EA, c->d_esc_acc, c->d_esc_rep, 0);
+    hipLaunchKernelGGL(k_moments, dim3(c->nblk_mom), dim3(KAOS_BLOCK), 0, c->stream, c->d_tubes, c->d_blk_tube,
+                       c->d_blk_q, c->d_d3x, c->d_d3v, c->d_dvpar, c->d_dvperp, c->d_vparGC, c->d_vperpGC,
+                       c->d_hist, c->d_nqc, c->d_NqRT, c->d_M, c->d_count, (const long long*)nullptr);
+    int nblocks = (c->ntiles + a.tiles_per_block - 1) / a.tiles_per_block;
+    hipLaunchKernelGGL(k_push_fp64, dim3(nblocks), dim3(KAOS_BLOCK), 0, c->stream, a);
+  } else {
+    if (master) {
+      hipLaunchKernelGGL(k_field, dim3(nt), dim3(64), 0, c->stream, c->d_tubes, c->d_hq, c->d_dq, c->d_Te,
+                         c->d_M, c->d_M0F, c->d_EA, c->d_esc_acc, c->d_esc_rep, 1);
+      zero_hist();
+    }
+    StepArgs a = make_args(c, master, master, premaster);
+    int nblocks = (c->ntiles + a.tiles_per_block - 1) / a.tiles_per_block;
+    hipLaunchKernelGGL(k_push_fp64, dim3(nblocks), dim3(KAOS_BLOCK), 0, c->stream, a);
+    if (master) {
+      if (c->inj_total > 0) {
+        int nb = (int)((c->inj_total + 255) / 256);
+        hipLaunchKernelGGL(k_inject, dim3(nb), dim3(256), 0, c->stream, c->d_tubes, c->d_inj_base, nt,
+                           c->d_qGH, c->d_qGC, c->d_EA, c->d_EG, c->d_vparGH, c->d_vperpGH,
+                           c->d_rs, c->d_vp, c->d_vr, c->d_qk, c->d_ae, c->d_ag, c->d_cell, c->d_count,
+                           c->d_hist, c->d_nqc, d_uni, c->d_uni_base, (unsigned long long)c->cfg.seed,
+                           (long long)n, c->inj_total);
+      }
+      hipLaunchKernelGGL(k_moments, dim3(c->nblk_mom), dim3(KAOS_BLOCK), 0, c->stream, c->d_tubes, c->d_blk_tube,
+                         c->d_blk_q, c->d_d3x, c->d_d3v, c->d_dvpar, c->d_dvperp, c->d_vparGC, c->d_vperpGC,
+                         c->d_hist, c->d_nqc, c->d_NqRT, c->d_M, c->d_count, c->d_inj_cnt);
+    }
+  }
+  HIPCHK(c, hipGetLastError());
+  // host mirror of the slot counts
+  if (master)
+    for (int t = 0; t < nt; ++t) c->ht[t].count += c->inj_base_h[t + 1] - c->inj_base_h[t];
+  c->last_step = n;
+  c->last_master = first ? 1 : (int)master;
+  return KAOS_OK;
+}
+
+static int check_next(kaos_ctx* c, long long n) {
+  if (!c->built) return fail(c, KAOS_E_STATE, "no particles uploaded");
+  for (int t = 0; t < c->cfg.ntubes; ++t)
+    if (!c->ht[t].uploaded) return fail(c, KAOS_E_STATE, "tube " + std::to_string(t) + " not uploaded");
+  if (n != c->last_step + 1) return fail(c, KAOS_E_STATE, "steps must be consecutive starting at 1");
+  if (is_master(c, n))
+    for (int t = 0; t < c->cfg.ntubes; ++t) {
+      long long add = c->inj_base_h[t + 1] - c->inj_base_h[t];
+      if (c->ht[t].count + add > c->ht[t].cap)
+        return fail(c, KAOS_E_CAPACITY, "tube " + std::to_string(t) + " capacity exhausted at step " + std::to_string(n));
+    }
+  return KAOS_OK;
+}
+
+static int read_status(kaos_ctx* c, int32_t* st) {
+  unsigned int s = 0;
+  HIPCHK(c, hipMemcpy(&s, c->d_status, sizeof(s), hipMemcpyDeviceToHost));
+  *st = (s & (1u << KAOS_E_NAN)) ? KAOS_E_NAN : KAOS_OK;
+  return KAOS_OK;
+}
+
+extern "C" int kaos_step(kaos_ctx* c, int64_t n, const double* uniforms, int64_t n_uniforms,
+                         kaos_step_out* out) {
+  if (!c) return KAOS_E_ARG;
+  int rc = check_next(c, n);
+  if (rc) return rc;
+  const double* d_uni = nullptr;
+  if (uniforms && is_master(c, n) && c->inj_total > 0) {
+    if (n_uniforms < 6 * c->inj_total) return fail(c, KAOS_E_ARG, "too few uniforms for this step's injection");
+    if (c->uni_cap < 6 * c->inj_total) {
+      if (c->d_uni) (void)hipFree(c->d_uni);
+      HIPCHK(c, hipMalloc((void**)&c->d_uni, sizeof(double) * 6 * c->inj_total));
+      c->uni_cap = 6 * c->inj_total;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_uni, uniforms, sizeof(double) * 6 * c->inj_total, hipMemcpyHostToDevice, c->stream));
+    d_uni = c->d_uni;
+  }
+  long long alive_before = 0;
+  for (int t = 0; t < c->cfg.ntubes; ++t) alive_before += c->ht[t].count;
+  rc = enqueue_step(c, n, d_uni);
+  if (rc) return rc;
+  if (out || d_uni) HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (out) {
+    memset(out, 0, sizeof(*out));
+    out->master = c->last_master;
+    out->n_injected = is_master(c, n) ? c->inj_total : 0;
+    int64_t alive = 0;
+    for (int t = 0; t < c->cfg.ntubes; ++t) { int64_t k = 0; kaos_count(c, t, &k); alive += k; }
+    out->n_alive = alive;
+    (void)alive_before;
+    rc = read_status(c, &out->status);
+    if (rc) return rc;
+    if (out->status) return fail(c, out->status, "NaN in the particle state");
+  }
+  return KAOS_OK;
+}
+
+extern "C" int kaos_run(kaos_ctx* c, int64_t n_first, int64_t nsteps, int64_t* particle_steps) {
+  if (!c) return KAOS_E_ARG;
+  for (int64_t i = 0; i < nsteps; ++i) {
+    int rc = check_next(c, n_first + i);
+    if (rc) return rc;
+    rc = enqueue_step(c, n_first + i, nullptr);
+    if (rc) return rc;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (particle_steps) *particle_steps = -1;
+  int32_t st = 0;
+  int rc = read_status(c, &st);
+  if (rc) return rc;
+  if (st) return fail(c, st, "NaN in the particle state");
+  return KAOS_OK;
+}
+
+extern "C" int kaos_count(kaos_ctx* c, int32_t tube, int64_t* n) {
+  if (!c || !n) return KAOS_E_ARG;
+  if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
+  if (!c->built) { *n = 0; return KAOS_OK; }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const HostTube& h = c->ht[tube];
+  const DevTube& T = c->dt[tube];
+  std::vector<double> rs((size_t)h.count);
+  if (h.count) HIPCHK(c, hipMemcpy(rs.data(), c->d_rs + T.off, sizeof(double) * h.count, hipMemcpyDeviceToHost));
+  int64_t k = 0;
+  for (double r : rs) k += (r != 0.0);
+  *n = k;
+  return KAOS_OK;
+}
+
+extern "C" int kaos_download_master(kaos_ctx* c, int32_t tube, int64_t* nq_counts, int64_t* nph_counts,
+                                    double* NqRT, double* M, double* EA, double* EG, double* esc) {
+  if (!c) return KAOS_E_ARG;
+  if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
+  if (!c->built) return fail(c, KAOS_E_STATE, "nothing computed yet");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const DevTube& T = c->dt[tube];
+  const HostTube& h = c->ht[tube];
+  int nq = T.nq;
+  long long ncell = (long long)T.nvpar * T.nvperp;
+  if (nq_counts) {
+    std::vector<unsigned int> u(nq);
+    HIPCHK(c, hipMemcpy(u.data(), c->d_nqc + T.goff, sizeof(unsigned int) * nq, hipMemcpyDeviceToHost));
+    for (int i = 0; i < nq; ++i) nq_counts[i] = u[i];
+  }
+  if (nph_counts) {
+    std::vector<unsigned int> u((size_t)(nq * ncell));
+    HIPCHK(c, hipMemcpy(u.data(), c->d_hist + T.hoff, sizeof(unsigned int) * nq * ncell, hipMemcpyDeviceToHost));
+    for (long long i = 0; i < nq * ncell; ++i) nph_counts[i] = u[i];
+  }
+  if (NqRT) HIPCHK(c, hipMemcpy(NqRT, c->d_NqRT + T.goff, sizeof(double) * nq, hipMemcpyDeviceToHost));
+  if (M) HIPCHK(c, hipMemcpy(M, c->d_M + 6 * T.goff, sizeof(double) * 6 * nq, hipMemcpyDeviceToHost));
+  if (EA) HIPCHK(c, hipMemcpy(EA, c->d_EA + T.goff, sizeof(double) * nq, hipMemcpyDeviceToHost));
+  if (EG) HIPCHK(c, hipMemcpy(EG, c->d_EG + T.goff, sizeof(double) * nq, hipMemcpyDeviceToHost));
+  if (esc) {
+    unsigned long long e[2];
+    HIPCHK(c, hipMemcpy(e, c->d_esc_rep + 2 * tube, sizeof(e), hipMemcpyDeviceToHost));
+    const kaos_grid& g = h.g;
+    int ub = nq - 1;
+    double dtm = (double)c->cfg.ndatfac * g.h;
+    double NLB = (double)e[0] * g.nsnormfac, NUB = (double)e[1] * g.nsnormfac;
+    esc[0] = (double)e[0]; esc[1] = (double)e[1];
+    esc[2] = NLB / ((h.d3x[0] / (h.hq[0] * h.dq[0])) * dtm);
+    esc[3] = NUB / ((h.d3x[ub] / (h.hq[ub] * h.dq[ub])) * dtm);
+  }
+  return KAOS_OK;
+}
+
+extern "C" int kaos_download_particles(kaos_ctx* c, int32_t tube, int64_t n_cap, int64_t* n_out,
+                                       double* x, double* y, double* z, double* Vx, double* Vy, double* Vz,
+                                       double* vperp, double* qk4, double* aea, double* ag,
+                                       int32_t* qind, int32_t* vparind, int32_t* vperpind) {
+  if (!c || !n_out) return KAOS_E_ARG;
+  if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
+  if (!c->built) { *n_out = 0; return KAOS_OK; }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const DevTube& T = c->dt[tube];
+  long long n = c->ht[tube].count;
+  std::vector<double> rs(n), vr(n), qk(n), ae(n), agv(n), xyz(6 * (size_t)n);
+  std::vector<uint32_t> cell(n);
+  if (n) {
+    double* d6 = nullptr;
+    HIPCHK(c, hipMalloc((void**)&d6, sizeof(double) * 6 * n));
+    hipLaunchKernelGGL(k_reconstruct, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, T, n,
+                       c->d_rs, c->d_vp, d6);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(xyz.data(), d6, sizeof(double) * 6 * n, hipMemcpyDeviceToHost));
+    (void)hipFree(d6);
+    HIPCHK(c, hipMemcpy(rs.data(), c->d_rs + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(vr.data(), c->d_vr + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(qk.data(), c->d_qk + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(ae.data(), c->d_ae + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(agv.data(), c->d_ag + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(cell.data(), c->d_cell + T.off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+  }
+  long long k = 0;
+  for (long long j = 0; j < n; ++j) {
+    if (rs[j] == 0.0) continue;
+    if (k >= n_cap) return fail(c, KAOS_E_CAPACITY, "output arrays too small");
+    if (x) x[k] = xyz[6 * j + 0];
+    if (y) y[k] = xyz[6 * j + 1];
+    if (z) z[k] = xyz[6 * j + 2];
+    if (Vx) Vx[k] = xyz[6 * j + 3];
+    if (Vy) Vy[k] = xyz[6 * j + 4];
+    if (Vz) Vz[k] = xyz[6 * j + 5];
+    if (vperp) vperp[k] = vr[j];
+    if (qk4) qk4[k] = qk[j];
+    if (aea) aea[k] = ae[j];
+    if (ag) ag[k] = agv[j];
+    if (qind) qind[k] = (int32_t)(cell[j] >> 16) - 1;
+    if (vperpind) vperpind[k] = (int32_t)((cell[j] >> 8) & 0xFFu);
+    if (vparind) vparind[k] = (int32_t)(cell[j] & 0xFFu);
+    ++k;
+  }
+  *n_out = k;
+  return KAOS_OK;
+}
+
+extern "C" int kaos_finalize(kaos_ctx* c) {
+  if (!c) return KAOS_OK;
+  (void)hipStreamSynchronize(c->stream);
+  void* bufs[] = {c->d_tubes, c->d_tile_tube, c->d_tile_start, c->d_blk_tube, c->d_blk_q, c->d_qGH, c->d_qGC,
+                  c->d_hq, c->d_dq, c->d_Te, c->d_d3x, c->d_EA, c->d_EG, c->d_M, c->d_M0F, c->d_NqRT,
+                  c->d_vparGH, c->d_vperpGH, c->d_vparGC, c->d_vperpGC, c->d_dvpar, c->d_dvperp, c->d_d3v,
+                  c->d_rs, c->d_vp, c->d_vr, c->d_qk, c->d_ae, c->d_ag, c->d_cell, c->d_count, c->d_hist,
+                  c->d_nqc, c->d_status, c->d_esc_acc, c->d_esc_rep, c->d_inj_base, c->d_inj_cnt,
+                  c->d_uni_base, c->d_uni};
+  for (void* p : bufs) if (p) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return KAOS_OK;
+}

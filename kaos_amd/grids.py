@@ -1,0 +1,191 @@
+"""Flux-tube grids and initial ions for the benchmark and test workloads.
+
+The reference builds its grids in SimParameterizationSub (uniform in q,
+_0_ConfigGridGenerator.f90:299-310; velocity grids _0_0_VelGridGenerator.f90)
+and places every initial ion at a cell centre (_4_DensityProfileB.f90:40-46).
+The workloads here are specified differently (arc-length geometric cells,
+exponential density, uniform velocity grids), so they are generated on the
+host with numpy and passed to the hot path as inputs, exactly as the Fortran
+driver would pass its own arrays.
+
+Dipole field line of L-shell p: r = p RE sin^2(theta), q = RE^2 cos(theta)/r^2.
+Arc length from the lower boundary: s = p RE [F(u_LB) - F(u)], u = cos(theta),
+F(u) = u sqrt(1+3u^2)/2 + asinh(sqrt(3) u)/(2 sqrt(3)).
+"""
+from dataclasses import dataclass, field
+import math
+import numpy as np
+
+RE = 6.38e6
+KB = 1.38e-23
+MO = 16.0 * 1.66054e-27      # O+ mass, _ModelInput.f90:186
+QO = 1.602e-19               # O+ charge, _ModelInput.f90:187
+MDIP = 8.06e15
+EV_K = 1.602e-19 / KB        # kelvin per eV with the reference's constants
+
+
+def _F(u):
+    return u * np.sqrt(1.0 + 3.0 * u * u) / 2.0 + np.arcsinh(math.sqrt(3.0) * u) / (2.0 * math.sqrt(3.0))
+
+
+def _u_of_s(s, p, u_lb):
+    """cos(theta) at arc length s above the lower boundary (Newton on F)."""
+    target = _F(u_lb) - s / (p * RE)
+    u = np.full_like(np.asarray(s, dtype=float), u_lb) - s / (p * RE * 2.0)
+    for _ in range(60):
+        f = _F(u) - target
+        du = f / np.sqrt(1.0 + 3.0 * u * u)
+        u = u - du
+        if np.max(np.abs(du)) < 1e-16:
+            break
+    return u
+
+
+@dataclass
+class Tube:
+    """One flux tube: the kaos_grid fields plus host-side helpers."""
+    nq: int
+    nvpar: int
+    nvperp: int
+    p: float
+    phi: float
+    h: float
+    ms: float
+    qs: float
+    nsnormfac: float
+    Ts_lb: float
+    Ts_ub: float
+    n_inj_lb: int
+    n_inj_ub: int
+    arrays: dict = field(default_factory=dict)
+    s_centre: np.ndarray = None
+    ds: np.ndarray = None
+    grav: int = 1
+    eamb: int = 1
+    eambsign: int = 1
+    filter_pt: int = 11
+    lbcond: int = 1
+    ubcond: int = 1
+
+    @property
+    def cphi(self):
+        return math.cos(self.phi)
+
+    @property
+    def sphi(self):
+        return math.sin(self.phi)
+
+
+def velocity_grid(nvpar, nvperp, vpar_max=1.0e4, vperp_max=1.0e4):
+    """Uniform V|| in [-vpar_max, vpar_max] and Vperp in [0, vperp_max] with
+    contiguous edges; d3v = 2 pi |Vperp_c| dVperp dVpar (_0_0_VelGridGenerator.f90:460-464)."""
+    e = -vpar_max + (2.0 * vpar_max) * np.arange(nvpar + 1) / nvpar
+    w = vperp_max * np.arange(nvperp + 1) / nvperp
+    vparGL, vparGH = e[:-1].copy(), e[1:].copy()
+    vperpGL, vperpGH = w[:-1].copy(), w[1:].copy()
+    vparGC = (vparGL + vparGH) / 2.0
+    vparGC[vparGC == 0.0] = 1e-15          # _0_0_VelGridGenerator.f90:1000
+    vperpGC = (vperpGL + vperpGH) / 2.0
+    dvpar, dvperp = vparGH - vparGL, vperpGH - vperpGL
+    d3v = ((np.abs(vperpGC)[:, None] * dvperp[:, None]) * dvpar[None, :]) * (2.0 * math.pi)
+    return dict(vparGL=vparGL, vparGH=vparGH, vparGC=vparGC, dvpar=dvpar, vperpGL=vperpGL,
+                vperpGH=vperpGH, vperpGC=vperpGC, dvperp=dvperp, d3v=np.ascontiguousarray(d3v))
+
+
+def dipole_tube(nq=64, p=4.0, r_lb=1.05, r_ub=3.0, ratio=1.05, south=False, nvpar=40, nvperp=40,
+                phi=math.pi / 2.0, h=0.05, T_eV=0.3, Te=1.0e4, n_total=100000, H=5.0e5,
+                ndatfac=1, n_lb_density=1e10, area0=1e6):
+    """Tube from r_lb to r_ub (RE) on L-shell p with nq cells whose arc lengths grow
+    geometrically by `ratio`; initial density profile ~ exp(-s/H)."""
+    u_lb = math.sqrt(1.0 - r_lb / p)
+    u_ub = math.sqrt(1.0 - r_ub / p)
+    S = p * RE * (_F(u_lb) - _F(u_ub))
+    k = np.arange(nq + 1, dtype=float)
+    w0 = S * (ratio - 1.0) / (ratio ** nq - 1.0) if ratio != 1.0 else S / nq
+    s_edge = w0 * (ratio ** k - 1.0) / (ratio - 1.0) if ratio != 1.0 else w0 * k
+    s_edge[-1] = S
+    s_cen = 0.5 * (s_edge[:-1] + s_edge[1:])
+    u_e = _u_of_s(s_edge, p, u_lb)
+    u_c = _u_of_s(s_cen, p, u_lb)
+    sign = -1.0 if south else 1.0
+    u_e, u_c = sign * u_e, sign * u_c
+    r_e = p * RE * (1.0 - u_e ** 2)
+    r_c = p * RE * (1.0 - u_c ** 2)
+    q_e = RE ** 2 * u_e / r_e ** 2
+    q_c = RE ** 2 * u_c / r_c ** 2
+    ell_c = 1.0 + 3.0 * u_c ** 2
+    hq = r_c ** 3 / (RE ** 2 * np.sqrt(ell_c))
+    ds = np.diff(s_edge)
+    B_c = MDIP * np.sqrt(ell_c) / r_c ** 3
+    d3x = area0 * (B_c[0] / B_c) * ds
+    qGL, qGH = q_e[:-1].copy(), q_e[1:].copy()
+    dq = np.abs(qGL - qGH)
+    vg = velocity_grid(nvpar, nvperp)
+    Ts = T_eV * EV_K
+    sigma = math.sqrt(KB * Ts / MO)
+    # macro-ions per cell ~ exp(-s/H) d3x, normalised to n_total
+    wgt = np.exp(-s_cen / H) * d3x
+    nc = np.floor(n_total * wgt / wgt.sum()).astype(np.int64)
+    nc[0] += n_total - nc.sum()
+    nsnormfac = n_lb_density * d3x[0] / max(nc[0], 1)
+    dt_m = ndatfac * h
+    flux_frac = sigma / math.sqrt(2.0 * math.pi) * dt_m
+    n_inj_lb = int(round(nc[0] * flux_frac / ds[0]))
+    n_inj_ub = int(round(nc[-1] * flux_frac / ds[-1]))
+    arrays = dict(qGL=qGL, qGH=qGH, qGC=q_c, hqC=hq, dqC=dq, d3xC=d3x, rGC=r_c, cthGC=u_c,
+                  ellGC=ell_c, Te=np.full(nq, Te))
+    arrays.update(vg)
+    t = Tube(nq=nq, nvpar=nvpar, nvperp=nvperp, p=p, phi=phi, h=h, ms=MO, qs=QO,
+             nsnormfac=nsnormfac, Ts_lb=Ts, Ts_ub=Ts, n_inj_lb=n_inj_lb, n_inj_ub=n_inj_ub,
+             arrays={k2: np.ascontiguousarray(v, dtype=np.float64) for k2, v in arrays.items()},
+             s_centre=s_cen, ds=ds)
+    t.cell_counts = nc
+    t.sigma = sigma
+    return t
+
+
+def initial_ions(tube, seed=0, counts=None):
+    """Initial (q0, Vpar0, Vperp0): q uniform inside each cell, a Maxwellian of
+    the tube's ion temperature (Vperp = |(Vperp1, Vperp2)|)."""
+    rng = np.random.default_rng(seed)
+    nc = tube.cell_counts if counts is None else counts
+    n = int(nc.sum())
+    a = tube.arrays
+    cell = np.repeat(np.arange(tube.nq), nc)
+    f = rng.random(n)
+    q0 = a["qGL"][cell] + f * (a["qGH"][cell] - a["qGL"][cell])
+    sig = math.sqrt(KB * tube.Ts_lb / tube.ms)
+    vpar0 = sig * rng.standard_normal(n)
+    v1 = sig * rng.standard_normal(n)
+    v2 = sig * rng.standard_normal(n)
+    vperp0 = np.sqrt(v1 * v1 + v2 * v2)
+    return q0, vpar0, vperp0
+
+
+# --- the workloads -----------------------------------------------------------
+
+def config1(n_total=100_000):
+    """CPU-reference case: L=4, 1.05-3 RE, 64 cells (ratio 1.05), 40x40, 1e5 O+, T=0.3 eV,
+    dt=0.05 s, gravity + ambipolar, LB and UB injection."""
+    return dipole_tube(nq=64, nvpar=40, nvperp=40, n_total=n_total, ratio=1.05)
+
+
+def config2(n_total=100_000_000):
+    """Full-size single tube: 256 cells (same stretching profile as config 1: ratio
+    1.05^(64/256) per cell), 64x64 velocity grid, 1e8 O+."""
+    return dipole_tube(nq=256, nvpar=64, nvperp=64, n_total=n_total, ratio=1.05 ** (64.0 / 256.0))
+
+
+def config4_tubes(ntubes=512, seed=4, scale=1.0):
+    """Batched case: L ~ U[3,8], nq ~ U{64..256}, ions log-U[1e4,1e6] (x scale), T ~ U[0.1,1] eV.
+    The tube spans 1.05 RE to min(3, 0.8 L) RE."""
+    rng = np.random.default_rng(seed)
+    tubes = []
+    for _ in range(ntubes):
+        L = rng.uniform(3.0, 8.0)
+        nq = int(rng.integers(64, 257))
+        n = int(round(10 ** rng.uniform(4.0, 6.0) * scale))
+        T = rng.uniform(0.1, 1.0)
+        tubes.append(dipole_tube(nq=nq, p=L, r_ub=min(3.0, 0.8 * L), nvpar=64, nvperp=64,
+                                 n_total=max(n, 2 * nq), T_eV=T, ratio=1.05 ** (64.0 / nq)))
+    return tubes

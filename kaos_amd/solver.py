@@ -1,0 +1,130 @@
+"""Host driver over the C-ABI: the Python mirror of KineticSolverSub's use of
+the hot path (_8_KineticSolver.f90:35-263).
+
+    ks = KineticSolver([tube], precision=KAOS_FP64, ndatfac=1, seed=1)
+    ks.upload(0, q0, vpar0, vperp0)          # InitialConditionsSub output
+    for n in range(1, Nt + 1):
+        ks.step(n)                           # one ComputationalTimeStepLoop pass
+    out = ks.master(0)                       # counts, moments, fields, escapes
+
+Every number comes from libkaos_hip.so on the GPU; this module only moves
+host arrays across the C-ABI.
+"""
+import ctypes as C
+import numpy as np
+
+from . import _abi
+from ._abi import KaosCfg, KaosGrid, KaosStepOut, ptr, KAOS_FP64, KAOS_MIXED  # noqa: F401
+
+MOMENT_NAMES = ("M0", "M1perp", "M1par", "M2", "M2perp", "M2par")
+
+
+class KaosError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"kaos error {code}: {msg}")
+        self.code = code
+
+
+def grid_struct(tube):
+    """kaos_grid for a grids.Tube; keeps the numpy arrays alive on the struct."""
+    g = KaosGrid()
+    for n in ("nq", "nvpar", "nvperp", "grav", "eamb", "eambsign", "filter_pt", "lbcond",
+              "ubcond", "n_inj_lb", "n_inj_ub", "p", "h", "ms", "qs", "nsnormfac", "Ts_lb",
+              "Ts_ub"):
+        setattr(g, n, getattr(tube, n))
+    g.cphi, g.sphi = tube.cphi, tube.sphi
+    keep = []
+    for n in _abi.GRID_ARRAYS:
+        a = np.ascontiguousarray(tube.arrays[n], dtype=np.float64)
+        keep.append(a)
+        setattr(g, n, ptr(a))
+    g._keep = keep
+    return g
+
+
+class KineticSolver:
+    def __init__(self, tubes, precision=KAOS_FP64, ndatfac=1, seed=1, capacity=None, device=0):
+        self.lib = _abi.load()
+        self.tubes = list(tubes)
+        self.ndatfac = ndatfac
+        cfg = KaosCfg(ntubes=len(self.tubes), precision=precision, ndatfac=ndatfac, device=device,
+                      seed=seed)
+        h = C.c_void_p()
+        rc = self.lib.kaos_init(C.byref(cfg), C.byref(h))
+        if rc:
+            raise KaosError(rc, self.lib.kaos_last_error(None).decode())
+        self.h = h
+        self._grids = []
+        for t, tube in enumerate(self.tubes):
+            g = grid_struct(tube)
+            self._grids.append(g)
+            cap = capacity[t] if capacity is not None else int(tube.cell_counts.sum()) + 1024
+            self._chk(self.lib.kaos_set_grid(self.h, t, C.byref(g), int(cap)))
+
+    def _chk(self, rc):
+        if rc:
+            raise KaosError(rc, self.lib.kaos_last_error(self.h).decode())
+
+    def upload(self, tube, q0, vpar0, vperp0):
+        q0, vpar0, vperp0 = (np.ascontiguousarray(a, dtype=np.float64) for a in (q0, vpar0, vperp0))
+        self._chk(self.lib.kaos_upload_particles(self.h, tube, len(q0), ptr(q0), ptr(vpar0),
+                                                 ptr(vperp0)))
+
+    def step(self, n, uniforms=None, wait=True):
+        out = KaosStepOut() if wait else None
+        u = None if uniforms is None else np.ascontiguousarray(uniforms, dtype=np.float64)
+        nu = 0 if u is None else len(u)
+        self._chk(self.lib.kaos_step(self.h, n, ptr(u), nu, C.byref(out) if out is not None else None))
+        return out
+
+    def run(self, n_first, nsteps):
+        ps = C.c_int64(0)
+        self._chk(self.lib.kaos_run(self.h, n_first, nsteps, C.byref(ps)))
+
+    def count(self, tube):
+        n = C.c_int64(0)
+        self._chk(self.lib.kaos_count(self.h, tube, C.byref(n)))
+        return n.value
+
+    def master(self, tube):
+        t = self.tubes[tube]
+        nq, nc = t.nq, t.nq * t.nvperp * t.nvpar
+        nqc = np.zeros(nq, np.int64)
+        nph = np.zeros(nc, np.int64)
+        NqRT = np.zeros(nq)
+        M = np.zeros(6 * nq)
+        EA = np.zeros(nq)
+        EG = np.zeros(nq)
+        esc = np.zeros(4)
+        self._chk(self.lib.kaos_download_master(self.h, tube, ptr(nqc, C.c_int64), ptr(nph, C.c_int64),
+                                                ptr(NqRT), ptr(M), ptr(EA), ptr(EG), ptr(esc)))
+        out = dict(nq_counts=nqc, nph_counts=nph.reshape(nq, t.nvperp, t.nvpar), NqRT=NqRT,
+                   EA=EA, EG=EG, esc=esc)
+        for i, n in enumerate(MOMENT_NAMES):
+            out[n] = M[i * nq:(i + 1) * nq].copy()
+        return out
+
+    def particles(self, tube, cap=None):
+        cap = int(cap if cap is not None else self.count(tube))
+        d = {n: np.zeros(cap) for n in ("x", "y", "z", "Vx", "Vy", "Vz", "vperp", "qk4", "aea", "ag")}
+        i = {n: np.zeros(cap, np.int32) for n in ("qind", "vparind", "vperpind")}
+        n = C.c_int64(0)
+        self._chk(self.lib.kaos_download_particles(
+            self.h, tube, cap, C.byref(n), *(ptr(d[k]) for k in ("x", "y", "z", "Vx", "Vy", "Vz",
+                                                                  "vperp", "qk4", "aea", "ag")),
+            *(ptr(i[k], C.c_int32) for k in ("qind", "vparind", "vperpind"))))
+        m = n.value
+        out = {k: v[:m] for k, v in d.items()}
+        out.update({k: v[:m] for k, v in i.items()})
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.kaos_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,162 @@
+"""TEST INFRASTRUCTURE ONLY -- the CPU oracle of the KAOS hot path.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this package, and only as the checker; the shipped library never calls it.
+
+liboracle.so is oracle/kaos_oracle.c built with gcc -O2 -ffp-contract=off
+(see oracle/build.py).  The Oracle class mirrors kaos_amd.solver.KineticSolver
+for one tube so the parity tests read the same on both sides.
+"""
+import ctypes as C
+import os
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_dp = C.POINTER(C.c_double)
+
+ARRAYS = ("qGL", "qGH", "qGC", "hqC", "dqC", "d3xC", "rGC", "cthGC", "ellGC", "Te",
+          "vparGL", "vparGH", "vparGC", "dvpar", "vperpGL", "vperpGH", "vperpGC", "dvperp", "d3v")
+
+
+class OrcCfg(C.Structure):
+    _fields_ = ([(n, C.c_int) for n in ("nq", "nvpar", "nvperp", "grav", "eamb", "eambsign",
+                                        "filter_pt", "ndatfac", "lbcond", "ubcond")]
+                + [("n_inj_lb", C.c_int64), ("n_inj_ub", C.c_int64)]
+                + [(n, C.c_double) for n in ("p", "cphi", "sphi", "h", "ms", "qs", "nsnormfac",
+                                             "Ts_lb", "Ts_ub")]
+                + [("tube", C.c_int), ("seed", C.c_uint64)]
+                + [(n, _dp) for n in ARRAYS])
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            from . import build
+            build.build()
+        lib = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        i32p = C.POINTER(C.c_int)
+        lib.orc_create.argtypes = [C.POINTER(OrcCfg), C.c_int64, _dp, _dp, _dp, C.c_int64]
+        lib.orc_create.restype = P
+        lib.orc_destroy.argtypes = [P]
+        lib.orc_step.argtypes = [P, C.c_int64, _dp]
+        lib.orc_step.restype = C.c_int
+        lib.orc_count.argtypes = [P]
+        lib.orc_count.restype = C.c_int64
+        lib.orc_get_particles.argtypes = [P] + [_dp] * 10 + [i32p] * 3
+        lib.orc_get_master.argtypes = [P] + [_dp] * 8
+        lib.orc_filter.argtypes = [_dp, _dp, C.c_int, C.c_int]
+        lib.orc_ambipolar.argtypes = [C.POINTER(OrcCfg), _dp, _dp, _dp]
+        lib.orc_gravity.argtypes = [C.POINTER(OrcCfg), _dp]
+        lib.orc_scan_qcell.argtypes = [C.POINTER(OrcCfg), C.c_double]
+        lib.orc_scan_qcell.restype = C.c_int
+        lib.orc_scan_vcell.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_int]
+        lib.orc_scan_vcell.restype = C.c_int
+        lib.orc_moments_from_counts.argtypes = [C.POINTER(OrcCfg), _dp, _dp, _dp, _dp]
+        lib.orc_push_ions.argtypes = [C.POINTER(OrcCfg), C.c_int64] + [_dp] * 8 + [i32p, _dp]
+        for n in ("orc_root_r",):
+            getattr(lib, n).argtypes = [C.c_double, C.c_double]
+            getattr(lib, n).restype = C.c_double
+        for n in ("orc_sinh_asinh_third", "orc_log", "orc_cos2pi"):
+            getattr(lib, n).argtypes = [C.c_double]
+            getattr(lib, n).restype = C.c_double
+        lib.orc_ea_stencil.argtypes = [C.POINTER(OrcCfg), _dp, _dp, _dp]
+        lib.orc_find_qcell.argtypes = [C.POINTER(OrcCfg), C.c_double]
+        lib.orc_find_qcell.restype = C.c_int
+        lib.orc_find_vcell.argtypes = [_dp, _dp, C.c_int, C.c_double]
+        lib.orc_find_vcell.restype = C.c_int
+        lib.orc_interp_cells.argtypes = [C.POINTER(OrcCfg), _dp, C.c_double, _dp]
+        lib.orc_interp_cells.restype = C.c_int
+        lib.orc_interp_scan.argtypes = [C.POINTER(OrcCfg), _dp, C.c_double, C.c_double, C.c_int]
+        lib.orc_interp_scan.restype = C.c_double
+        lib.orc_mb.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                               C.c_double, _dp]
+        lib.orc_gauss.argtypes = [C.c_double, C.c_double]
+        lib.orc_gauss.restype = C.c_double
+        u64p = C.POINTER(C.c_uint64)
+        lib.orc_philox.argtypes = [u64p, u64p, u64p]
+        lib.orc_inject_uniforms.argtypes = [C.c_uint64, C.c_int64, C.c_int, C.c_int64, _dp]
+        _lib = lib
+    return _lib
+
+
+def ptr(a, ct=C.c_double):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ct))
+
+
+def cfg_of(tube, ndatfac=1, seed=1, tube_index=0):
+    c = OrcCfg()
+    for n in ("nq", "nvpar", "nvperp", "grav", "eamb", "eambsign", "filter_pt", "lbcond", "ubcond",
+              "n_inj_lb", "n_inj_ub", "p", "h", "ms", "qs", "nsnormfac", "Ts_lb", "Ts_ub"):
+        setattr(c, n, getattr(tube, n))
+    c.cphi, c.sphi = tube.cphi, tube.sphi
+    c.ndatfac = ndatfac
+    c.tube = tube_index
+    c.seed = seed
+    keep = []
+    for n in ARRAYS:
+        a = np.ascontiguousarray(tube.arrays[n], dtype=np.float64)
+        keep.append(a)
+        setattr(c, n, ptr(a))
+    c._keep = keep
+    return c
+
+
+class Oracle:
+    """One flux tube stepped by the CPU restatement."""
+
+    def __init__(self, tube, q0, vpar0, vperp0, ndatfac=1, seed=1, capacity=None, tube_index=0):
+        self.lib = load()
+        self.tube = tube
+        self.cfg = cfg_of(tube, ndatfac, seed, tube_index)
+        q0, vpar0, vperp0 = (np.ascontiguousarray(a, dtype=np.float64) for a in (q0, vpar0, vperp0))
+        cap = capacity if capacity is not None else len(q0) + 1024
+        self.h = self.lib.orc_create(C.byref(self.cfg), len(q0), ptr(q0), ptr(vpar0), ptr(vperp0), cap)
+
+    def step(self, n, uniforms=None):
+        u = None if uniforms is None else np.ascontiguousarray(uniforms, dtype=np.float64)
+        rc = self.lib.orc_step(self.h, n, ptr(u))
+        if rc:
+            raise RuntimeError(f"oracle step {n} failed: {rc}")
+
+    def count(self):
+        return self.lib.orc_count(self.h)
+
+    def particles(self):
+        n = self.count()
+        d = {k: np.zeros(n) for k in ("x", "y", "z", "Vx", "Vy", "Vz", "vperp", "qk4", "aea", "ag")}
+        i = {k: np.zeros(n, np.int32) for k in ("qind", "vparind", "vperpind")}
+        self.lib.orc_get_particles(self.h, *(ptr(d[k]) for k in ("x", "y", "z", "Vx", "Vy", "Vz",
+                                                                   "vperp", "qk4", "aea", "ag")),
+                                   *(ptr(i[k], C.c_int) for k in ("qind", "vparind", "vperpind")))
+        d.update(i)
+        return d
+
+    def master(self):
+        t = self.tube
+        nq, nc = t.nq, t.nq * t.nvperp * t.nvpar
+        nqc, nph, NqRT = np.zeros(nq), np.zeros(nc), np.zeros(nq)
+        M, EA, EG, M0F, esc = np.zeros(6 * nq), np.zeros(nq), np.zeros(nq), np.zeros(nq), np.zeros(4)
+        self.lib.orc_get_master(self.h, ptr(nqc), ptr(nph), ptr(NqRT), ptr(M), ptr(EA), ptr(EG),
+                                ptr(M0F), ptr(esc))
+        out = dict(nq_counts=nqc.astype(np.int64), nph_counts=nph.astype(np.int64).reshape(nq, t.nvperp, t.nvpar),
+                   NqRT=NqRT, EA=EA, EG=EG, M0F=M0F, esc=esc)
+        for k, n in enumerate(("M0", "M1perp", "M1par", "M2", "M2perp", "M2par")):
+            out[n] = M[k * nq:(k + 1) * nq].copy()
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,506 @@
+/*
+ * kaos_oracle.c -- TEST INFRASTRUCTURE ONLY.  CPU restatement (FP64, one
+ * thread, plain loops) of the KAOS per-timestep kinetic hot path, used by the
+ * tests, the smoke check and the CPU-baseline leg of the benchmark as the
+ * checker of the HIP path.  Nothing in the shipped library links or calls it.
+ *
+ * It follows the reference's call sequence and data layout:
+ *   - Cartesian per-particle state x,y,z,Vx,Vy,Vz,Vperp plus the reference's
+ *     per-particle fields AEAmag/AGmag (current) and AEAmagN/AGmagN (next),
+ *     qk4 and the integer cell indices Qindk1/Vparindk1/Vperpindk1;
+ *   - KineticSolverSub step order (_8_KineticSolver.f90:54-239):
+ *     BoundaryConditionsSub -> [master] AmbipolarEfieldSub/GravfieldSub ->
+ *     KineticSolverA/BSub (rotation, push loop, injection) ->
+ *     ParticleCounts -> DistributionFunctions -> 6 moments;
+ *   - linear cell scans (first match), stable pack for removal, brute-force
+ *     equivalent histogram, column-major velocity sums.
+ * The per-particle RK4/root-solve arithmetic is the shared scalar header
+ * kaos_dipole.h; its agreement with the reference's own Fortran push
+ * (compiled from the reference sources under oracle/ref) is tested separately.
+ *
+ * Parity pins for each function are listed in the test files; nothing here
+ * is "parity unpinned" except where a test module says so.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include "../kaos_amd/csrc/kaos_dipole.h"
+
+typedef struct {
+  /* scalars */
+  int nq, nvpar, nvperp;
+  int grav, eamb, eambsign, filter_pt;
+  int ndatfac;
+  int lbcond, ubcond;
+  int64_t n_inj_lb, n_inj_ub;     /* nint(nsnormC{L,U}BGT/ranksize), BC:747-750, 806-809 */
+  double p, cphi, sphi, h, ms, qs, nsnormfac;
+  double Ts_lb, Ts_ub;            /* TsGT at NqLB / NqUB */
+  int tube;                       /* tube number used to key the seeded stream */
+  uint64_t seed;
+  /* config grid (nq each), QCellT/FluxTubeT arrays */
+  const double *qGL, *qGH, *qGC, *hqC, *dqC, *d3xC, *rGC, *cthGC, *ellGC, *Te;
+  /* velocity grid */
+  const double *vparGL, *vparGH, *vparGC, *dvpar;       /* nvpar  */
+  const double *vperpGL, *vperpGH, *vperpGC, *dvperp;   /* nvperp */
+  const double *d3v;                                    /* nvperp*nvpar, [iperp][ipar] */
+} orc_cfg;
+
+typedef struct {
+  orc_cfg c;
+  int64_t ns, cap;
+  double *x, *y, *z, *Vx, *Vy, *Vz, *vperp, *aea, *ag, *aeaN, *agN, *qk4, *q0;
+  int *qind, *vparind, *vperpind;
+  int master_last;                 /* last step was a master step */
+  int64_t nn;                      /* index of the last master step */
+  /* per master step outputs */
+  double *nq_counts, *nph_counts;  /* NqReNormT, NphReNormRT (integer valued) */
+  uint32_t *nq_u, *nph_u;
+  double *NqRT, *M;                /* M: 6*nq: M0, M1perp, M1par, M2, M2perp, M2par */
+  double *Mprev;                   /* moments of the previous master step (6*nq) */
+  double *EA, *EG, *M0F;           /* EAmagRT(nn), EGmagRT(nn), M0FiltAvrgRT(nn) */
+  double esc_lb, esc_ub, flux_lb, flux_ub;
+  int64_t injected;
+} orc_t;
+
+static void *xc(size_t n) { void *p = calloc(n ? n : 1, 1); return p; }
+
+/* ---- cell scans exactly as written in the reference ------------------------ */
+
+/* _8_1_1_KineticUpdateA.f90:221-273 */
+static int scan_qcell(const orc_cfg *c, double q, int prev) {
+  int nq = c->nq, out = prev;
+  if (c->qGL[0] <= 0) {
+    for (int Q = 0; Q < nq; ++Q) {
+      if (Q == 0 && c->qGL[Q] <= q && q <= c->qGH[Q]) { out = Q + 1; break; }
+      if (Q != 0 && c->qGL[Q] < q && q <= c->qGH[Q]) { out = Q + 1; break; }
+    }
+    if (c->qGL[0] > q) out = 0;
+    else if (c->qGH[nq - 1] < q) out = -1;
+  } else {
+    for (int Q = 0; Q < nq; ++Q) {
+      if (Q == 0 && c->qGL[Q] >= q && q >= c->qGH[Q]) { out = Q + 1; break; }
+      if (Q != 0 && c->qGL[Q] > q && q >= c->qGH[Q]) { out = Q + 1; break; }
+    }
+    if (c->qGL[0] < q) out = 0;
+    else if (c->qGH[nq - 1] > q) out = -1;
+  }
+  if (isnan(q)) out = 0;    /* documented: NaN positions leave through the LB */
+  return out;
+}
+
+/* 2-D velocity scan, _8_1_1_KineticUpdateA.f90:1087-1127: no match keeps prev. */
+static int scan_vcell(const double *GL, const double *GH, int n, double v, int prev) {
+  for (int i = 0; i < n; ++i) {
+    if (i == 0 && GL[i] <= v && v <= GH[i]) return 1;
+    if (i != 0 && GL[i] < v && v <= GH[i]) return i + 1;
+  }
+  return prev;
+}
+
+/* ---- fields ---------------------------------------------------------------- */
+
+/* Fields of a master step from the previous master step's density (shared
+ * per-tube arithmetic kaos_ambipolar/kaos_grav_cell). */
+static void ambipolar_field(const orc_cfg *c, const double *M0prev, double *M0F, double *EA) {
+  kaos_ambipolar(c->nq, c->qGL[0] <= 0, c->eamb, c->eambsign, c->filter_pt, c->qs,
+                 c->hqC, c->dqC, c->Te, M0prev, M0F, EA);
+}
+static void grav_field(const orc_cfg *c, double *EG) {
+  for (int Q = 0; Q < c->nq; ++Q)
+    EG[Q] = kaos_grav_cell(c->grav, c->ms, c->qs, c->cthGC[Q], c->rGC[Q], c->ellGC[Q]);
+}
+
+/* Per-particle interpolation at qq (_8_7_AmbipolarEfield.f90:1453-2720,
+ * _8_7_2_Gravfield.f90:132-...): Qind-outer, particle-inner scan over the
+ * half cells and centre intervals.  No match leaves *out unchanged.        */
+static void interp_scan(const orc_cfg *c, const double *E, double qq, double *out, int kind) {
+  int nq = c->nq, ub = nq - 1;
+  double v = 0.0; int hit = 0;
+  for (int Q = 0; Q < nq && !hit; ++Q) {
+    if (c->qGL[0] <= 0) {
+      if (Q == 0 && c->qGL[Q] <= qq && qq <= c->qGC[Q]) { v = E[Q]; hit = 1; }
+      if (Q != 0 && Q != ub && c->qGC[Q - 1] < qq && qq <= c->qGC[Q]) {
+        v = E[Q - 1] + (((qq - c->qGC[Q - 1]) / (c->qGC[Q] - c->qGC[Q - 1])) * (E[Q] - E[Q - 1])); hit = 1; }
+      if (Q == ub) {
+        if (c->qGC[Q - 1] < qq && qq < c->qGC[Q]) {
+          v = E[Q - 1] + (((qq - c->qGC[Q - 1]) / (c->qGC[Q] - c->qGC[Q - 1])) * (E[Q] - E[Q - 1])); hit = 1; }
+        if (c->qGC[Q] <= qq && qq <= c->qGH[Q]) { v = E[Q]; hit = 1; }
+      }
+    } else {
+      if (Q == 0 && c->qGL[Q] >= qq && qq >= c->qGC[Q]) { v = E[Q]; hit = 1; }
+      if (Q != 0 && Q != ub && c->qGC[Q - 1] > qq && qq >= c->qGC[Q]) {
+        v = E[Q - 1] + (((qq - c->qGC[Q - 1]) / (c->qGC[Q] - c->qGC[Q - 1])) * (E[Q] - E[Q - 1])); hit = 1; }
+      if (Q == ub) {
+        if (c->qGC[Q - 1] > qq && qq > c->qGC[Q]) {
+          v = E[Q - 1] + (((qq - c->qGC[Q - 1]) / (c->qGC[Q] - c->qGC[Q - 1])) * (E[Q] - E[Q - 1])); hit = 1; }
+        if (c->qGC[Q] >= qq && qq >= c->qGH[Q]) { v = E[Q]; hit = 1; }
+      }
+    }
+  }
+  if (!hit) return;
+  double qm = c->qs / c->ms;
+  if (kind == 0) *out = kaos_aea_of(v, qm, c->eambsign);   /* AEAmagN */
+  else *out = kaos_ag_of(v, qm);                           /* AGmagN */
+}
+
+/* ---- counts and moments ------------------------------------------------------ */
+
+/* IonParticleCountsSub 2-D branch (_8_3_1_IonParticleCounts.f90:33-137,
+ * 566-730): counts per config cell and per (config, Vperp, Vpar) cell of the
+ * ions with a valid Qindk1, the phase-space count requiring both velocity
+ * indices; then DistributionFunctions + moments per cell.                  */
+static void counts_and_moments(orc_t *o, const int *qi, const int *vpi, const int *vri, int64_t ns) {
+  orc_cfg *c = &o->c;
+  int nq = c->nq, np = c->nvpar, nr = c->nvperp;
+  int64_t ncell = (int64_t)nr * np;
+  memset(o->nq_u, 0, sizeof(uint32_t) * nq);
+  memset(o->nph_u, 0, sizeof(uint32_t) * nq * ncell);
+  for (int64_t j = 0; j < ns; ++j) {
+    int Q = qi[j];
+    if (Q < 1 || Q > nq) continue;
+    o->nq_u[Q - 1] += 1;
+    int a = vri[j], b = vpi[j];
+    if (a >= 1 && a <= nr && b >= 1 && b <= np)
+      o->nph_u[(int64_t)(Q - 1) * ncell + (int64_t)(a - 1) * np + (b - 1)] += 1;
+  }
+  for (int Q = 0; Q < nq; ++Q) o->nq_counts[Q] = (double)o->nq_u[Q];
+  for (int64_t i = 0; i < nq * ncell; ++i) o->nph_counts[i] = (double)o->nph_u[i];
+  kaos_vgrid_t v = { np, nr, c->d3v, c->dvperp, c->dvpar, c->vperpGC, c->vparGC };
+  for (int Q = 0; Q < nq; ++Q) {
+    double k = c->nsnormfac / c->d3xC[Q];
+    double NqRT = o->nq_counts[Q] * k;
+    o->NqRT[Q] = NqRT;
+    const uint32_t *N = o->nph_u + (int64_t)Q * ncell;
+    double S[4] = {0, 0, 0, 0}, col[4], Mq[6];
+    for (int b = 0; b < np; ++b) {
+      kaos_mom_col1(&v, N, b, k, NqRT, col);
+      for (int t = 0; t < 4; ++t) S[t] = S[t] + col[t];
+    }
+    kaos_mom_finish1(S, NqRT, c->ms, Mq);
+    double S2[2] = {0, 0}, col2[2];
+    for (int b = 0; b < np; ++b) {
+      kaos_mom_col2(&v, N, b, k, NqRT, Mq[2], col2);
+      S2[0] = S2[0] + col2[0]; S2[1] = S2[1] + col2[1];
+    }
+    kaos_mom_finish2(S2, c->ms, Mq);
+    for (int t = 0; t < 6; ++t) o->M[t * nq + Q] = Mq[t];
+  }
+}
+
+/* ---- state management ------------------------------------------------------ */
+
+#define ARR_D(X) X(x) X(y) X(z) X(Vx) X(Vy) X(Vz) X(vperp) X(aea) X(ag) X(aeaN) X(agN) X(qk4) X(q0)
+#define ARR_I(X) X(qind) X(vparind) X(vperpind)
+
+void *orc_create(const orc_cfg *cfg, int64_t ns, const double *q0, const double *vpar0,
+                 const double *vperp0, int64_t cap) {
+  orc_t *o = (orc_t *)xc(sizeof(orc_t));
+  o->c = *cfg;
+  o->ns = ns; o->cap = cap < ns ? ns : cap;
+#define AL(n) o->n = (double *)xc(sizeof(double) * o->cap);
+  ARR_D(AL)
+#undef AL
+#define AL(n) o->n = (int *)xc(sizeof(int) * o->cap);
+  ARR_I(AL)
+#undef AL
+  int nq = cfg->nq;
+  int64_t ncell = (int64_t)cfg->nvperp * cfg->nvpar;
+  o->nq_counts = (double *)xc(sizeof(double) * nq);
+  o->nph_counts = (double *)xc(sizeof(double) * nq * ncell);
+  o->nq_u = (uint32_t *)xc(sizeof(uint32_t) * nq);
+  o->nph_u = (uint32_t *)xc(sizeof(uint32_t) * nq * ncell);
+  o->NqRT = (double *)xc(sizeof(double) * nq);
+  o->M = (double *)xc(sizeof(double) * 6 * nq);
+  o->Mprev = (double *)xc(sizeof(double) * 6 * nq);
+  o->EA = (double *)xc(sizeof(double) * nq);
+  o->EG = (double *)xc(sizeof(double) * nq);
+  o->M0F = (double *)xc(sizeof(double) * nq);
+  /* initial conditions: x0 = root(q0), V0 = Vpar0 b-hat(x0) */
+  for (int64_t j = 0; j < ns; ++j) {
+    double rs = kaos_rs_of_q(q0[j], cfg->p);
+    kaos_pos_from_rs(rs, cfg->p, cfg->cphi, cfg->sphi, &o->x[j], &o->y[j], &o->z[j]);
+    kaos_geom_t g = kaos_geom(o->x[j], o->y[j], o->z[j], cfg->cphi, cfg->sphi);
+    o->Vx[j] = vpar0[j] * g.bx; o->Vy[j] = vpar0[j] * g.by; o->Vz[j] = vpar0[j] * g.bz;
+    o->vperp[j] = vperp0[j];
+    o->q0[j] = q0[j];
+    o->qk4[j] = vpar0[j];   /* scratch: Vpar0T for the k0 velocity index */
+  }
+  return o;
+}
+
+void orc_destroy(void *h) {
+  orc_t *o = (orc_t *)h;
+  if (!o) return;
+#define FR(n) free(o->n);
+  ARR_D(FR) ARR_I(FR)
+#undef FR
+  free(o->nq_counts); free(o->nph_counts); free(o->nq_u); free(o->nph_u); free(o->NqRT); free(o->M); free(o->Mprev);
+  free(o->EA); free(o->EG); free(o->M0F);
+  free(o);
+}
+
+/* BoundaryConditionsSub at a master step: escape counts and fluxes
+ * (_8_0_7_BoundaryConditions.f90:158-285) and stable pack (BCReIndexSub). */
+static void bc_remove(orc_t *o) {
+  orc_cfg *c = &o->c;
+  int64_t k = 0, nlb = 0, nub = 0;
+  for (int64_t j = 0; j < o->ns; ++j) {
+    if (o->qind[j] == 0) { ++nlb; continue; }
+    if (o->qind[j] == -1) { ++nub; continue; }
+    if (k != j) {
+#define MV(n) o->n[k] = o->n[j];
+      ARR_D(MV) ARR_I(MV)
+#undef MV
+    }
+    ++k;
+  }
+  o->ns = k;
+  o->esc_lb = (double)nlb; o->esc_ub = (double)nub;
+  int ub = c->nq - 1;
+  double dtm = (double)c->ndatfac * c->h;
+  double NLB = (double)nlb * c->nsnormfac, NUB = (double)nub * c->nsnormfac;
+  o->flux_lb = NLB / ((c->d3xC[0] / (c->hqC[0] * c->dqC[0])) * dtm);
+  o->flux_ub = NUB / ((c->d3xC[ub] / (c->hqC[ub] * c->dqC[ub])) * dtm);
+}
+
+/* LB/UB ion injection after the master push (KSB:195-793). */
+static int inject(orc_t *o, int64_t n, const double *uni) {
+  orc_cfg *c = &o->c;
+  int64_t nl = c->lbcond ? c->n_inj_lb : 0, nu = c->ubcond ? c->n_inj_ub : 0;
+  if (o->ns + nl + nu > o->cap) return 2;
+  double qm = c->qs / c->ms;
+  for (int64_t k = 0; k < nl + nu; ++k) {
+    int lb = (k < nl);
+    int Qb = lb ? 0 : c->nq - 1;
+    int64_t j = o->ns + k;
+    double qc = c->qGC[Qb];
+    double u[6];
+    if (uni) memcpy(u, uni + 6 * k, sizeof(u));
+    else kaos_inject_uniforms(c->seed, n, c->tube, k, u);
+    o->qind[j] = scan_qcell(c, qc, 0);
+    double rs = kaos_rs_of_q(qc, c->p);
+    kaos_pos_from_rs(rs, c->p, c->cphi, c->sphi, &o->x[j], &o->y[j], &o->z[j]);
+    kaos_geom_t g = kaos_geom(o->x[j], o->y[j], o->z[j], c->cphi, c->sphi);
+    double Ts = lb ? c->Ts_lb : c->Ts_ub;
+    double sigma = sqrt((KAOS_KB * Ts) / c->ms);
+    double vpar, v1, v2, vperp;
+    kaos_mb_project(u, sigma, c->cphi, c->sphi, g, &vpar, &v1, &v2, &vperp);
+    o->Vx[j] = vpar * g.bx; o->Vy[j] = vpar * g.by; o->Vz[j] = vpar * g.bz;
+    o->vperp[j] = vperp;
+    o->vparind[j] = scan_vcell(c->vparGL, c->vparGH, c->nvpar, vpar, 0);
+    o->vperpind[j] = scan_vcell(c->vperpGL, c->vperpGH, c->nvperp, vperp, 0);
+    o->aea[j] = o->aeaN[j] = fabs(qm * fabs(o->EA[Qb]));
+    o->ag[j] = o->agN[j] = fabs(qm * fabs(o->EG[Qb]));
+    o->qk4[j] = qc;
+  }
+  o->ns += nl + nu;
+  o->injected = nl + nu;
+  return 0;
+}
+
+static void push_all(orc_t *o) {
+  orc_cfg *c = &o->c;
+  kaos_push_par_t P = { c->h, c->p, c->cphi, c->sphi, c->grav, c->eamb, c->eambsign };
+  int desc = (c->qGL[0] > 0);
+  for (int64_t j = 0; j < o->ns; ++j) {
+    kaos_push_out_t r;
+    int qc;
+    kaos_push_ion(o->x[j], o->y[j], o->z[j], o->Vx[j], o->Vy[j], o->Vz[j], o->ag[j], o->aea[j], &P,
+                  c->qGH, c->qGL[0], c->qGH[c->nq - 1], c->nq, desc, &r, &qc);
+    /* the oracle re-derives the config cell with the reference's linear scan */
+    int qs = scan_qcell(c, r.qk1, o->qind[j]);
+    o->qind[j] = qs;
+    o->vparind[j] = scan_vcell(c->vparGL, c->vparGH, c->nvpar, r.vpar, o->vparind[j]);
+    o->vperpind[j] = scan_vcell(c->vperpGL, c->vperpGH, c->nvperp, o->vperp[j], o->vperpind[j]);
+    o->x[j] = r.xN; o->y[j] = r.yN; o->z[j] = r.zN;
+    o->Vx[j] = r.VxN; o->Vy[j] = r.VyN; o->Vz[j] = r.VzN;
+    o->qk4[j] = r.qk4;
+    (void)qc;
+  }
+}
+
+/* One time step n (n = 1 is KineticSolverASub).  uniforms: 6 per injected
+ * ion of this step (LB first), or NULL for the seeded Philox stream.        */
+int orc_step(void *h, int64_t n, const double *uniforms) {
+  orc_t *o = (orc_t *)h;
+  orc_cfg *c = &o->c;
+  int nq = c->nq;
+  o->injected = 0;
+  if (n == 1) {
+    /* k0 indices of the initial state, with q0 clamped into the grid (KSA:55-320) */
+    for (int64_t j = 0; j < o->ns; ++j) {
+      double q = o->q0[j];
+      if (c->qGL[0] <= 0) {
+        if (q <= c->qGL[0]) q = c->qGL[0];
+        else if (q > c->qGH[nq - 1]) q = c->qGH[nq - 1];
+      } else {
+        if (q > c->qGL[0]) q = c->qGL[0];
+        else if (q < c->qGH[nq - 1]) q = c->qGH[nq - 1];
+      }
+      o->q0[j] = q;
+      o->qind[j] = scan_qcell(c, q, 0);
+      o->vparind[j] = scan_vcell(c->vparGL, c->vparGH, c->nvpar, o->qk4[j], 0);
+      o->vperpind[j] = scan_vcell(c->vperpGL, c->vperpGH, c->nvperp, o->vperp[j], 0);
+    }
+    counts_and_moments(o, o->qind, o->vparind, o->vperpind, o->ns);
+    memset(o->Mprev, 0, sizeof(double) * 6 * nq);   /* moments of master step 0: none */
+    ambipolar_field(c, o->Mprev, o->M0F, o->EA);
+    grav_field(c, o->EG);
+    for (int64_t j = 0; j < o->ns; ++j) {
+      interp_scan(c, o->EA, o->q0[j], &o->aeaN[j], 0);
+      interp_scan(c, o->EG, o->q0[j], &o->agN[j], 1);
+    }
+    memcpy(o->Mprev, o->M, sizeof(double) * 6 * nq);
+    o->nn = 1;
+    o->master_last = 1;
+  } else {
+    int master = (n % c->ndatfac) == 0;
+    if (master) {
+      bc_remove(o);
+      ambipolar_field(c, o->Mprev, o->M0F, o->EA);
+      grav_field(c, o->EG);
+      for (int64_t j = 0; j < o->ns; ++j) {
+        interp_scan(c, o->EA, o->qk4[j], &o->aeaN[j], 0);
+        interp_scan(c, o->EG, o->qk4[j], &o->agN[j], 1);
+      }
+    }
+    o->master_last = master;
+  }
+  for (int64_t j = 0; j < o->ns; ++j) { o->aea[j] = o->aeaN[j]; o->ag[j] = o->agN[j]; }
+  push_all(o);
+  if (n > 1 && o->master_last) {
+    int rc = inject(o, n, uniforms);
+    if (rc) return rc;
+    counts_and_moments(o, o->qind, o->vparind, o->vperpind, o->ns);
+    memcpy(o->Mprev, o->M, sizeof(double) * 6 * nq);
+    o->nn = n / c->ndatfac + 1;
+  }
+  return 0;
+}
+
+int64_t orc_count(void *h) { return ((orc_t *)h)->ns; }
+int orc_master_last(void *h) { return ((orc_t *)h)->master_last; }
+
+void orc_get_particles(void *h, double *x, double *y, double *z, double *Vx, double *Vy,
+                       double *Vz, double *vperp, double *qk4, double *aea, double *ag,
+                       int *qind, int *vparind, int *vperpind) {
+  orc_t *o = (orc_t *)h;
+  size_t nd = sizeof(double) * o->ns, ni = sizeof(int) * o->ns;
+  if (x) memcpy(x, o->x, nd);
+  if (y) memcpy(y, o->y, nd);
+  if (z) memcpy(z, o->z, nd);
+  if (Vx) memcpy(Vx, o->Vx, nd);
+  if (Vy) memcpy(Vy, o->Vy, nd);
+  if (Vz) memcpy(Vz, o->Vz, nd);
+  if (vperp) memcpy(vperp, o->vperp, nd);
+  if (qk4) memcpy(qk4, o->qk4, nd);
+  if (aea) memcpy(aea, o->aeaN, nd);
+  if (ag) memcpy(ag, o->agN, nd);
+  if (qind) memcpy(qind, o->qind, ni);
+  if (vparind) memcpy(vparind, o->vparind, ni);
+  if (vperpind) memcpy(vperpind, o->vperpind, ni);
+}
+
+/* Outputs of the last master step. esc[0..3] = LB count, UB count, LB flux, UB flux. */
+void orc_get_master(void *h, double *nq_counts, double *nph_counts, double *NqRT, double *M,
+                    double *EA, double *EG, double *M0F, double *esc) {
+  orc_t *o = (orc_t *)h;
+  orc_cfg *c = &o->c;
+  int nq = c->nq;
+  int64_t ncell = (int64_t)c->nvperp * c->nvpar;
+  if (nq_counts) memcpy(nq_counts, o->nq_counts, sizeof(double) * nq);
+  if (nph_counts) memcpy(nph_counts, o->nph_counts, sizeof(double) * nq * ncell);
+  if (NqRT) memcpy(NqRT, o->NqRT, sizeof(double) * nq);
+  if (M) memcpy(M, o->M, sizeof(double) * 6 * nq);
+  if (EA) memcpy(EA, o->EA, sizeof(double) * nq);
+  if (EG) memcpy(EG, o->EG, sizeof(double) * nq);
+  if (M0F) memcpy(M0F, o->M0F, sizeof(double) * nq);
+  if (esc) { esc[0] = o->esc_lb; esc[1] = o->esc_ub; esc[2] = o->flux_lb; esc[3] = o->flux_ub; }
+}
+
+/* ---- standalone pieces exposed for the unit tests --------------------------- */
+
+void orc_filter(const double *in, double *out, int nq, int W) { kaos_moment_filter(in, out, nq, W); }
+void orc_ambipolar(const orc_cfg *c, const double *M0prev, double *M0F, double *EA) {
+  ambipolar_field(c, M0prev, M0F, EA);
+}
+void orc_gravity(const orc_cfg *c, double *EG) { grav_field(c, EG); }
+void orc_ea_stencil(const orc_cfg *c, const double *M0prev, const double *nbar, double *EA) {
+  kaos_ambipolar_stencil(c->nq, c->qGL[0] <= 0, c->eamb, c->eambsign, c->qs, c->hqC, c->dqC, c->Te,
+                         M0prev, nbar, EA);
+}
+int orc_find_qcell(const orc_cfg *c, double q) {
+  return kaos_find_qcell(q, c->qGH, c->qGL[0], c->qGH[c->nq - 1], c->nq, c->qGL[0] > 0);
+}
+int orc_find_vcell(const double *GL, const double *GH, int n, double v) { return kaos_find_vcell(v, GH, GL[0], n); }
+int orc_interp_cells(const orc_cfg *c, const double *E, double q, double *out) {
+  return kaos_interp_cells(q, c->qGC, E, c->nq, c->qGL[0], c->qGH[c->nq - 1], c->qGL[0] > 0, out);
+}
+double orc_interp_scan(const orc_cfg *c, const double *E, double q, double init, int kind) {
+  double v = init; interp_scan(c, E, q, &v, kind); return v;
+}
+void orc_mb(const double *u, double sigma, double cphi, double sphi, double x, double y, double z, double *o4) {
+  kaos_geom_t g = kaos_geom(x, y, z, cphi, sphi);
+  kaos_mb_project(u, sigma, cphi, sphi, g, &o4[0], &o4[1], &o4[2], &o4[3]);
+}
+double orc_gauss(double u1, double u2) { return kaos_gauss(u1, u2); }
+int orc_scan_qcell(const orc_cfg *c, double q) { return scan_qcell(c, q, 0); }
+int orc_scan_vcell(const double *GL, const double *GH, int n, double v, int prev) {
+  return scan_vcell(GL, GH, n, v, prev);
+}
+/* Moments from given integer counts (for closed-form tests). */
+void orc_moments_from_counts(const orc_cfg *cfg, const double *nq_counts, const double *nph_counts,
+                             double *NqRT, double *M) {
+  orc_t o;
+  memset(&o, 0, sizeof(o));
+  o.c = *cfg;
+  int nq = cfg->nq;
+  int64_t ncell = (int64_t)cfg->nvperp * cfg->nvpar;
+  int64_t tot = 0;
+  for (int Q = 0; Q < nq; ++Q) tot += (int64_t)nq_counts[Q];
+  int *qi = (int *)xc(sizeof(int) * (tot + 1)), *vp = (int *)xc(sizeof(int) * (tot + 1)),
+      *vr = (int *)xc(sizeof(int) * (tot + 1));
+  int64_t k = 0;
+  for (int Q = 0; Q < nq; ++Q) {
+    int64_t inph = 0;
+    for (int a = 0; a < cfg->nvperp; ++a)
+      for (int b = 0; b < cfg->nvpar; ++b) {
+        int64_t cnt = (int64_t)nph_counts[(int64_t)Q * ncell + a * cfg->nvpar + b];
+        for (int64_t t = 0; t < cnt; ++t) { qi[k] = Q + 1; vr[k] = a + 1; vp[k] = b + 1; ++k; }
+        inph += cnt;
+      }
+    for (int64_t t = inph; t < (int64_t)nq_counts[Q]; ++t) { qi[k] = Q + 1; vr[k] = 0; vp[k] = 0; ++k; }
+  }
+  o.nq_counts = (double *)xc(sizeof(double) * nq);
+  o.nph_counts = (double *)xc(sizeof(double) * nq * ncell);
+  o.nq_u = (uint32_t *)xc(sizeof(uint32_t) * nq);
+  o.nph_u = (uint32_t *)xc(sizeof(uint32_t) * nq * ncell);
+  o.NqRT = NqRT; o.M = M;
+  counts_and_moments(&o, qi, vp, vr, k);
+  free(qi); free(vp); free(vr); free(o.nq_counts); free(o.nph_counts); free(o.nq_u); free(o.nph_u);
+}
+/* One push of given Cartesian ions (no cell bookkeeping): for the tests. */
+void orc_push_ions(const orc_cfg *c, int64_t n, const double *x, const double *y, const double *z,
+                   const double *Vx, const double *Vy, const double *Vz, const double *ag,
+                   const double *aea, const int *qvalid, double *out /* n x 11 */) {
+  kaos_push_par_t P = { c->h, c->p, c->cphi, c->sphi, c->grav, c->eamb, c->eambsign };
+  int desc = (c->qGL[0] > 0);
+  for (int64_t j = 0; j < n; ++j) {
+    kaos_push_out_t r; int qc;
+    kaos_push_ion(x[j], y[j], z[j], Vx[j], Vy[j], Vz[j], ag[j], aea[j], &P,
+                  c->qGH, c->qGL[0], c->qGH[c->nq - 1], c->nq, desc, &r, &qc);
+    (void)qvalid;
+    double *w = out + 11 * j;
+    w[0] = r.xN; w[1] = r.yN; w[2] = r.zN; w[3] = r.VxN; w[4] = r.VyN; w[5] = r.VzN;
+    w[6] = r.rsN; w[7] = r.vparN; w[8] = r.vpar; w[9] = r.qk1; w[10] = r.qk4;
+  }
+}
+double orc_root_r(double q, double p) { return kaos_root_r(q, p); }
+double orc_sinh_asinh_third(double y) { return kaos_sinh_asinh_third(y); }
+double orc_log(double x) { return kaos_log(x); }
+double orc_cos2pi(double u) { return kaos_cos2pi(u); }
+void orc_philox(const uint64_t *ctr, const uint64_t *key, uint64_t *out) { kaos_philox4x64(ctr, key, out); }
+void orc_inject_uniforms(uint64_t seed, int64_t step, int tube, int64_t k, double *u) {
+  kaos_inject_uniforms(seed, step, tube, k, u);
+}
