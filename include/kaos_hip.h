@@ -145,6 +145,14 @@ int kaos_download_particles(kaos_ctx* ctx, int32_t tube, int64_t n_cap, int64_t*
                             double* vperp, double* qk4, double* aea, double* ag,
                             int32_t* qind, int32_t* vparind, int32_t* vperpind);
 
+/* Kernel timing: when enabled, hipEvents on the library's stream bracket every
+ * push launch and every kaos_run call.  kaos_profile_read returns the summed
+ * push-kernel time, the number of push launches and the summed kaos_run time
+ * since the last read (milliseconds).  The reference's only timing is
+ * cpu_time progress prints (_KineticMain.f90:49-52); this has no counterpart. */
+int kaos_profile(kaos_ctx* ctx, int32_t enable);
+int kaos_profile_read(kaos_ctx* ctx, double* push_ms, int64_t* push_launches, double* run_ms);
+
 /* Text of the last error of a context (or of kaos_init when ctx is NULL). */
 const char* kaos_last_error(kaos_ctx* ctx);
 

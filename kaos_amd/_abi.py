@@ -42,8 +42,8 @@ class KaosStepOut(C.Structure):
 
 
 EXPORTS = ("kaos_init", "kaos_set_grid", "kaos_upload_particles", "kaos_step", "kaos_run",
-           "kaos_download_master", "kaos_count", "kaos_download_particles", "kaos_last_error",
-           "kaos_finalize")
+           "kaos_download_master", "kaos_count", "kaos_download_particles", "kaos_profile",
+           "kaos_profile_read", "kaos_last_error", "kaos_finalize")
 
 _lib = None
 
@@ -67,6 +67,8 @@ def load(path=LIB_PATH):
     lib.kaos_download_master.argtypes = [P, C.c_int32, i64p, i64p, _dp, _dp, _dp, _dp, _dp]
     lib.kaos_count.argtypes = [P, C.c_int32, i64p]
     lib.kaos_download_particles.argtypes = [P, C.c_int32, C.c_int64, i64p] + [_dp] * 10 + [i32p] * 3
+    lib.kaos_profile.argtypes = [P, C.c_int32]
+    lib.kaos_profile_read.argtypes = [P, _dp, i64p, _dp]
     lib.kaos_last_error.argtypes = [P]
     lib.kaos_last_error.restype = C.c_char_p
     lib.kaos_finalize.argtypes = [P]

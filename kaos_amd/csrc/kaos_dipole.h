@@ -64,23 +64,24 @@ KAOS_HD kaos_geom_t kaos_geom(double x, double y, double z, double cphi, double 
   kaos_geom_t g;
   double xy2 = x * x + y * y;
   g.r = sqrt(xy2 + z * z);
-  g.c = z / g.r;
-  g.s = sqrt(xy2) / g.r;
-  g.q = (KAOS_RE2 * g.c) / (g.r * g.r);
-  double ell = 1.0 + 3.0 * (g.c * g.c);
-  double sl = sqrt(ell);
-  double t = (3.0 * g.c) * g.s;
-  g.bx = (cphi * t) / sl;
-  g.by = (sphi * t) / sl;
-  g.bz = (3.0 * (g.c * g.c) - 1.0) / sl;
+  double ir = 1.0 / g.r;
+  g.c = z * ir;
+  g.s = sqrt(xy2) * ir;
+  g.q = (KAOS_RE2 * g.c) * (ir * ir);
+  double c2 = g.c * g.c;
+  double isl = 1.0 / sqrt(1.0 + 3.0 * c2);      /* 1/sqrt(ell) */
+  double t = ((3.0 * g.c) * g.s) * isl;
+  g.bx = cphi * t;
+  g.by = sphi * t;
+  g.bz = (3.0 * c2 - 1.0) * isl;
   return g;
 }
 
 /* q only (used on the RK4 predictor position, _8_1_KineticRK4Update.f90:161-163). */
 KAOS_HD double kaos_q_of(double x, double y, double z) {
   double r = sqrt((x * x + y * y) + z * z);
-  double c = z / r;
-  return (KAOS_RE2 * c) / (r * r);
+  double ir = 1.0 / r;
+  return (KAOS_RE2 * (z * ir)) * (ir * ir);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -90,8 +91,11 @@ KAOS_HD double kaos_q_of(double x, double y, double z) {
  * w <- (8 w^3 + y)/(12 w^2 + 3).                                            */
 KAOS_HD double kaos_sinh_asinh_third(double y) {
   double w;
+  int nit;
   if (y < 1.0) {
-    w = y * (1.0 / 3.0) - (4.0 / 81.0) * ((y * y) * y);
+    double y2 = y * y;
+    w = y * ((1.0 / 3.0) - (4.0 / 81.0) * y2);
+    nit = 4;
   } else {
     int e;
     double m = frexp(0.25 * y, &e);             /* 0.25y = m 2^e, m in [0.5,1) */
@@ -101,8 +105,9 @@ KAOS_HD double kaos_sinh_asinh_third(double y) {
     double cr = 0.5 + mm * (0.55 - mm * 0.05);  /* crude cbrt on [0.5,4) */
     w = ldexp(cr, e3);
     if (w < 0.3) w = 0.3;
+    nit = 7;
   }
-  for (int it = 0; it < 7; ++it) {
+  for (int it = 0; it < nit; ++it) {
     double w2 = w * w;
     w = (8.0 * (w2 * w) + y) / (12.0 * w2 + 3.0);
   }
@@ -115,10 +120,10 @@ KAOS_HD double kaos_sinh_asinh_third(double y) {
  * r = gamma RE.  The final subtraction (pii - mu) is the reference's
  * cancellation-prone step and is kept as is.                               */
 KAOS_HD double kaos_root_r(double q, double p) {
-  double q2 = q * q;
-  double C = 1.0 / (p * q2);                    /* CpRK4 */
-  double eps = (4.0 / q2) / 3.0;                /* EpsilonRK4 = Abb/3, Abb = -4 Dp */
-  double delta = (C * C) / 2.0;                 /* DeltaRK4 = -Bbb/2, Bbb = -Cp^2 */
+  double iq2 = 1.0 / (q * q);
+  double C = iq2 * (1.0 / p);                   /* CpRK4 = 1/(p q^2) */
+  double eps = iq2 * (4.0 / 3.0);               /* EpsilonRK4 = Abb/3, Abb = -4 Dp */
+  double delta = (C * C) * 0.5;                 /* DeltaRK4 = -Bbb/2, Bbb = -Cp^2 */
   double se = sqrt(eps);
   double y = delta / (eps * se);
   double w = kaos_sinh_asinh_third(y);
@@ -134,7 +139,7 @@ KAOS_HD double kaos_root_r(double q, double p) {
 KAOS_HD void kaos_pos_from_rs(double rs, double p, double cphi, double sphi,
                               double* x, double* y, double* z) {
   double r = fabs(rs);
-  double w = r / (KAOS_RE * p);
+  double w = r * (1.0 / (KAOS_RE * p));
   double s = sqrt(w);
   double c = sqrt(1.0 - w);
   if (rs < 0.0) c = -c;

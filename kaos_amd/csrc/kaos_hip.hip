@@ -13,8 +13,8 @@
 // the push ends every step with xN = pos(rs), VN = VparNp b-hat(xN), so the
 // six Cartesian doubles are recomputed bit for bit from the two stored ones.
 //
-// Ions that leave the domain are marked dead (rs = 0) at the step that
-// precedes a master step, which is when BoundaryConditionsSub removes them;
+// Ions whose last push found them outside the domain are marked dead (rs = 0)
+// at the next master step, which is when BoundaryConditionsSub removes them;
 // dead slots are skipped, live ions keep their order, injected ions are
 // appended, so the live ions read in slot order are the reference's packed
 // arrays (BCReIndexSub, _8_0_7_1_BCReIndex.f90:33-133).
@@ -35,6 +35,8 @@
 #define KAOS_TILE (KAOS_BLOCK * KAOS_PPT)
 #define KAOS_MAXQ 512
 #define KAOS_MAXV 255
+#define KAOS_HWORDS 10240                    // LDS histogram window: 20480 16-bit counters
+#define KAOS_TPB 16                          // tiles per push block (<= 63: 16-bit counters)
 
 struct DevTube {
   int nq, nvpar, nvperp, desc;
@@ -58,10 +60,11 @@ struct StepArgs {
   double *rs, *vp, *vr, *qk, *ae, *ag;
   uint32_t* cell;
   const long long* count;
-  unsigned int *hist, *nqc;
+  unsigned int *hist, *nqoff;               // phase-space counts; config counts off the v-grid
   unsigned long long* esc_acc;
   unsigned int* status;
-  int do_interp, do_count, do_compact;
+  unsigned long long* pushed;                // live ions pushed (particle-steps)
+  int do_interp, do_count, do_remove, do_qk;
 };
 
 __device__ __forceinline__ void set_status(unsigned int* st, unsigned int bit) { atomicOr(st, bit); }
@@ -71,18 +74,47 @@ __device__ __forceinline__ void set_status(unsigned int* st, unsigned int bit) {
 // _8_1_KineticRK4Update.f90:29-314) with, on master steps, the field
 // interpolation at the previous stage-4 q (_8_7_AmbipolarEfield.f90:1453-2720,
 // _8_7_2_Gravfield.f90:132-...) and the phase-space histogram of the
-// pre-push cells (_8_3_1_IonParticleCounts.f90:612-634); on the step before a
-// master step, ions whose pre-push cell is 0/-1 are marked dead
+// pre-push cells (_8_3_1_IonParticleCounts.f90:612-634); at a master step,
+// ions whose previous-push cell is 0/-1 are first marked dead
 // (_8_0_7_BoundaryConditions.f90:158-256).
 __global__ void __launch_bounds__(KAOS_BLOCK) k_push_fp64(StepArgs a) {
   __shared__ double s_qGH[KAOS_MAXQ], s_qGC[KAOS_MAXQ], s_EA[KAOS_MAXQ], s_EG[KAOS_MAXQ];
   __shared__ double s_vparGH[KAOS_MAXV + 1], s_vperpGH[KAOS_MAXV + 1];
+  __shared__ unsigned int s_hist[KAOS_HWORDS];   // 16-bit counters, two per word
+  __shared__ unsigned int s_pushed;
+  __shared__ int s_wq0, s_wnq, s_wtube;
   const int tid = threadIdx.x;
   int t0 = blockIdx.x * a.tiles_per_block;
   int t1 = min(t0 + a.tiles_per_block, a.ntiles);
   int cur = -1;
   DevTube T;
   long long end = 0;
+  unsigned int npushed = 0;
+  if (tid == 0) {
+    s_pushed = 0u;
+    // histogram window: config cells [wq0, wq0 + wnq) of the block's first tube,
+    // starting one cell below the first ion's cell of the previous step (ions are
+    // stored near cell order and move less than a cell per step)
+    int tb0 = (t0 < t1) ? a.tile_tube[t0] : 0;
+    DevTube T0 = a.tubes[tb0];
+    int ncell = T0.nvpar * T0.nvperp;
+    int wnq = a.do_count ? (2 * KAOS_HWORDS) / ncell : 0;
+    if (wnq > T0.nq) wnq = T0.nq;
+    int q0 = 1;
+    if (t0 < t1 && wnq > 0) {
+      long long j0 = a.tile_start[t0];
+      if (j0 < T0.off + a.count[tb0]) {
+        int qprev = (int)(a.cell[j0] >> 16) - 1;
+        if (qprev >= 1) q0 = qprev - 1;
+      }
+      if (q0 < 1) q0 = 1;
+      if (q0 + wnq - 1 > T0.nq) q0 = T0.nq - wnq + 1;
+    }
+    s_wq0 = q0; s_wnq = wnq; s_wtube = tb0;
+  }
+  for (int i = tid; i < KAOS_HWORDS; i += KAOS_BLOCK) s_hist[i] = 0u;
+  __syncthreads();
+  const int wq0 = s_wq0, wnq = s_wnq, wtube = s_wtube;
   for (int tile = t0; tile < t1; ++tile) {
     int tb = a.tile_tube[tile];
     if (tb != cur) {
@@ -107,6 +139,21 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_push_fp64(StepArgs a) {
       if (j >= end) break;
       double rs = a.rs[j];
       if (rs == 0.0) continue;
+      uint32_t cold = 0u;
+      bool have_old = false;
+      if (a.do_remove) {
+        // BoundaryConditionsSub at a master step: ions whose cell of the previous
+        // push is 0 (LB) or -1 (UB) leave the arrays (BC:158-256, BCReIndex:33-133)
+        cold = a.cell[j];
+        have_old = true;
+        int qprev = (int)(cold >> 16) - 1;
+        if (qprev == 0 || qprev == -1) {
+          a.rs[j] = 0.0;
+          atomicAdd(&a.esc_acc[2 * cur + (qprev == 0 ? 0 : 1)], 1ull);
+          continue;
+        }
+      }
+      ++npushed;
       double vs = a.vp[j], vperp = a.vr[j];
       double aea, ag;
       if (a.do_interp) {
@@ -133,27 +180,44 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_push_fp64(StepArgs a) {
       int ip = kaos_find_vcell(o.vpar, s_vparGH, vparGL0, T.nvpar);
       int iq = kaos_find_vcell(vperp, s_vperpGH, vperpGL0, T.nvperp);
       if (ip == 0 || iq == 0) {
-        uint32_t old = a.cell[j];
+        uint32_t old = have_old ? cold : a.cell[j];
         if (ip == 0) ip = (int)(old & 0xFFu);
         if (iq == 0) iq = (int)((old >> 8) & 0xFFu);
       }
       if (a.do_count && qc >= 1) {
-        atomicAdd(&a.nqc[T.goff + qc - 1], 1u);
-        if (ip >= 1 && iq >= 1)
-          atomicAdd(&a.hist[T.hoff + ((long long)(qc - 1) * T.nvperp + (iq - 1)) * T.nvpar + (ip - 1)], 1u);
-      }
-      if (a.do_compact && (qc == 0 || qc == -1)) {
-        a.rs[j] = 0.0;
-        atomicAdd(&a.esc_acc[2 * cur + (qc == 0 ? 0 : 1)], 1ull);
-        continue;
+        if (ip >= 1 && iq >= 1) {
+          int wq = qc - wq0;
+          if (cur == wtube && wq >= 0 && wq < wnq) {
+            int li = (wq * T.nvperp + (iq - 1)) * T.nvpar + (ip - 1);
+            atomicAdd(&s_hist[li >> 1], 1u << (16 * (li & 1)));
+          } else {
+            atomicAdd(&a.hist[T.hoff + ((long long)(qc - 1) * T.nvperp + (iq - 1)) * T.nvpar + (ip - 1)], 1u);
+          }
+        } else {
+          atomicAdd(&a.nqoff[T.goff + qc - 1], 1u);
+        }
       }
       if (o.rsN != o.rsN || o.vparN != o.vparN) set_status(a.status, 1u << KAOS_E_NAN);
       a.rs[j] = o.rsN;
       a.vp[j] = o.vparN;
-      if (a.do_compact) a.qk[j] = o.qk4;
+      if (a.do_qk) a.qk[j] = o.qk4;
       a.cell[j] = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip;
     }
   }
+  __syncthreads();
+  atomicAdd(&s_pushed, npushed);
+  if (wnq > 0) {
+    const DevTube& TW = a.tubes[wtube];
+    long long base = TW.hoff + (long long)(wq0 - 1) * TW.nvperp * TW.nvpar;
+    int nwords = (wnq * TW.nvperp * TW.nvpar + 1) >> 1;
+    for (int w = tid; w < nwords; w += KAOS_BLOCK) {
+      unsigned int v = s_hist[w];
+      if (v & 0xFFFFu) atomicAdd(&a.hist[base + 2 * w], v & 0xFFFFu);
+      if (v >> 16) atomicAdd(&a.hist[base + 2 * w + 1], v >> 16);
+    }
+  }
+  __syncthreads();
+  if (tid == 0 && s_pushed) atomicAdd(a.pushed, (unsigned long long)s_pushed);
 }
 
 // ---------------------------------------------------------------------------
@@ -174,9 +238,10 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_bin_initial(StepArgs a) {
     int ip = kaos_find_vcell(a.vp[j], a.vparGH + T.voff, vparGL0, T.nvpar);
     int iq = kaos_find_vcell(a.vr[j], a.vperpGH + T.woff, vperpGL0, T.nvperp);
     if (qc >= 1) {
-      atomicAdd(&a.nqc[T.goff + qc - 1], 1u);
       if (ip >= 1 && iq >= 1)
         atomicAdd(&a.hist[T.hoff + ((long long)(qc - 1) * T.nvperp + (iq - 1)) * T.nvpar + (ip - 1)], 1u);
+      else
+        atomicAdd(&a.nqoff[T.goff + qc - 1], 1u);
     }
     a.cell[j] = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip;
   }
@@ -212,8 +277,8 @@ __global__ void k_fill_u32(unsigned int* p, long long n) {
 }
 
 // Field of a master step (AmbipolarEfieldSub + GravfieldSub cell values), one
-// block per tube; also moves the escape counters accumulated since the last
-// master step into the reported slot (BoundaryConditionsSub, BC:158-256).
+// block per tube; also clears the tube's escape counters, which the push of
+// this master step then fills (BoundaryConditionsSub, BC:158-256).
 __global__ void k_field(const DevTube* tubes, const double* hq, const double* dq, const double* Te,
                         const double* Mprev, double* M0F, double* EA,
                         unsigned long long* esc_acc, unsigned long long* esc_rep, int rotate_esc) {
@@ -223,10 +288,7 @@ __global__ void k_field(const DevTube* tubes, const double* hq, const double* dq
   const double* M0 = Mprev + 6 * T.goff;  // M0 block of tube t
   kaos_ambipolar(T.nq, T.south, T.eamb, T.eambsign, T.filter_pt, T.qs, hq + T.goff, dq + T.goff,
                  Te + T.goff, M0, M0F + T.goff, EA + T.goff);
-  if (rotate_esc) {
-    esc_rep[2 * t] = esc_acc[2 * t]; esc_rep[2 * t + 1] = esc_acc[2 * t + 1];
-    esc_acc[2 * t] = 0ull; esc_acc[2 * t + 1] = 0ull;
-  }
+  if (rotate_esc) { esc_acc[2 * t] = 0ull; esc_acc[2 * t + 1] = 0ull; }
 }
 
 // LB/UB injection after the master push (KSB:195-793): ion k of tube t goes
@@ -236,7 +298,7 @@ __global__ void k_inject(const DevTube* tubes, const long long* inj_base, int nt
                          const double* qGH, const double* qGC, const double* EA, const double* EG,
                          const double* vparGH, const double* vperpGH,
                          double* rs, double* vp, double* vr, double* qk, double* ae, double* ag,
-                         uint32_t* cell, const long long* count, unsigned int* hist, unsigned int* nqc,
+                         uint32_t* cell, const long long* count, unsigned int* hist, unsigned int* nqoff,
                          const double* uni, const long long* uni_base, unsigned long long seed,
                          long long step, long long total) {
   long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -267,9 +329,10 @@ __global__ void k_inject(const DevTube* tubes, const long long* inj_base, int nt
   ag[j] = fabs(T.qsms * fabs(EG[T.goff + Qb]));
   cell[j] = ((uint32_t)(qcell + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip;
   if (qcell >= 1) {
-    atomicAdd(&nqc[T.goff + qcell - 1], 1u);
     if (ip >= 1 && iq >= 1)
       atomicAdd(&hist[T.hoff + ((long long)(qcell - 1) * T.nvperp + (iq - 1)) * T.nvpar + (ip - 1)], 1u);
+    else
+      atomicAdd(&nqoff[T.goff + qcell - 1], 1u);
   }
 }
 
@@ -280,16 +343,32 @@ __global__ void k_inject(const DevTube* tubes, const long long* inj_base, int nt
 // the ions injected this step.
 __global__ void __launch_bounds__(KAOS_BLOCK) k_moments(const DevTube* tubes, const int* blk_tube,
     const int* blk_q, const double* d3xC, const double* d3v, const double* dvpar, const double* dvperp,
-    const double* vparGC, const double* vperpGC, const unsigned int* hist, const unsigned int* nqc,
-    double* NqRT, double* M, long long* count, const long long* inj_cnt) {
+    const double* vparGC, const double* vperpGC, const unsigned int* hist, const unsigned int* nqoff,
+    unsigned int* nqc, double* NqRT, double* M, long long* count, const long long* inj_cnt) {
   __shared__ double s_col[KAOS_MAXV + 1][4];
   __shared__ double s_M[6];
+  __shared__ unsigned int s_cnt[KAOS_MAXV + 1];
+  __shared__ double s_nqr;
   int t = blk_tube[blockIdx.x], Q = blk_q[blockIdx.x];
   DevTube T = tubes[t];
   int b = threadIdx.x;
   double k = T.nsnormfac / d3xC[T.goff + Q];
-  double nqr = (double)nqc[T.goff + Q] * k;
   const uint32_t* N = hist + T.hoff + (long long)Q * T.nvperp * T.nvpar;
+  // config count NqReNormT = ions on the velocity grid (column sums) + off it
+  if (b < T.nvpar) {
+    unsigned int cs = 0;
+    for (int r = 0; r < T.nvperp; ++r) cs += N[r * T.nvpar + b];
+    s_cnt[b] = cs;
+  }
+  __syncthreads();
+  if (b == 0) {
+    unsigned int tot = nqoff[T.goff + Q];
+    for (int c = 0; c < T.nvpar; ++c) tot += s_cnt[c];
+    nqc[T.goff + Q] = tot;
+    s_nqr = (double)tot * k;
+  }
+  __syncthreads();
+  double nqr = s_nqr;
   kaos_vgrid_t v = { T.nvpar, T.nvperp, d3v + T.doff, dvperp + T.woff, dvpar + T.voff,
                      vperpGC + T.woff, vparGC + T.voff };
   if (b < T.nvpar) kaos_mom_col1(&v, N, b, k, nqr, s_col[b]);
@@ -365,7 +444,7 @@ struct kaos_ctx {
          *d_ag = nullptr;
   uint32_t* d_cell = nullptr;
   long long* d_count = nullptr;
-  unsigned int *d_hist = nullptr, *d_nqc = nullptr, *d_status = nullptr;
+  unsigned int *d_hist = nullptr, *d_nqc = nullptr, *d_nqoff = nullptr, *d_status = nullptr;
   unsigned long long *d_esc_acc = nullptr, *d_esc_rep = nullptr;
   long long *d_inj_base = nullptr, *d_inj_cnt = nullptr, *d_uni_base = nullptr;
   double* d_uni = nullptr; long long uni_cap = 0;
@@ -373,6 +452,10 @@ struct kaos_ctx {
   long long inj_total = 0;
   hipStream_t stream = nullptr;
   std::vector<long long> inj_base_h;
+  unsigned long long* d_pushed = nullptr;
+  // kernel timing (hipEvents on the library stream around each push launch)
+  bool prof_on = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_push, ev_step;
 };
 
 static std::string g_init_err;
@@ -545,8 +628,12 @@ static int build_device(kaos_ctx* c) {
   HIPCHK(c, up((void**)&c->d_count, zc.data(), sizeof(long long) * nt));
   HIPCHK(c, hipMalloc((void**)&c->d_hist, sizeof(unsigned int) * (size_t)(hoff ? hoff : 1)));
   HIPCHK(c, hipMalloc((void**)&c->d_nqc, sizeof(unsigned int) * (size_t)goff));
+  HIPCHK(c, hipMalloc((void**)&c->d_nqoff, sizeof(unsigned int) * (size_t)goff));
   HIPCHK(c, hipMemset(c->d_hist, 0, sizeof(unsigned int) * (size_t)(hoff ? hoff : 1)));
   HIPCHK(c, hipMemset(c->d_nqc, 0, sizeof(unsigned int) * (size_t)goff));
+  HIPCHK(c, hipMemset(c->d_nqoff, 0, sizeof(unsigned int) * (size_t)goff));
+  HIPCHK(c, hipMalloc((void**)&c->d_pushed, sizeof(unsigned long long)));
+  HIPCHK(c, hipMemset(c->d_pushed, 0, sizeof(unsigned long long)));
   HIPCHK(c, hipMalloc((void**)&c->d_status, sizeof(unsigned int)));
   HIPCHK(c, hipMemset(c->d_status, 0, sizeof(unsigned int)));
   std::vector<unsigned long long> ze(2 * nt, 0ull);
@@ -601,21 +688,36 @@ extern "C" int kaos_upload_particles(kaos_ctx* c, int32_t tube, int64_t n, const
   return KAOS_OK;
 }
 
-static StepArgs make_args(kaos_ctx* c, int interp, int count, int compact) {
+static StepArgs make_args(kaos_ctx* c, int interp, int count, int remove, int qk) {
   StepArgs a;
   a.tubes = c->d_tubes; a.tile_tube = c->d_tile_tube; a.tile_start = c->d_tile_start;
   a.ntiles = c->ntiles;
-  a.tiles_per_block = 4;
+  a.tiles_per_block = KAOS_TPB;
   a.qGH = c->d_qGH; a.qGC = c->d_qGC; a.EA = c->d_EA; a.EG = c->d_EG;
   a.vparGH = c->d_vparGH; a.vperpGH = c->d_vperpGH;
   a.rs = c->d_rs; a.vp = c->d_vp; a.vr = c->d_vr; a.qk = c->d_qk; a.ae = c->d_ae; a.ag = c->d_ag;
-  a.cell = c->d_cell; a.count = c->d_count; a.hist = c->d_hist; a.nqc = c->d_nqc;
-  a.esc_acc = c->d_esc_acc; a.status = c->d_status;
-  a.do_interp = interp; a.do_count = count; a.do_compact = compact;
+  a.cell = c->d_cell; a.count = c->d_count; a.hist = c->d_hist; a.nqoff = c->d_nqoff;
+  a.esc_acc = c->d_esc_acc; a.status = c->d_status; a.pushed = c->d_pushed;
+  a.do_interp = interp; a.do_count = count; a.do_remove = remove; a.do_qk = qk;
   return a;
 }
 
 static bool is_master(const kaos_ctx* c, long long n) { return n >= 2 && (n % c->cfg.ndatfac) == 0; }
+
+static std::pair<hipEvent_t, hipEvent_t> ev_pair(kaos_ctx* c) {
+  hipEvent_t a = nullptr, b = nullptr;
+  (void)hipEventCreate(&a); (void)hipEventCreate(&b);
+  (void)hipEventRecord(a, c->stream);
+  return {a, b};
+}
+
+static void launch_push(kaos_ctx* c, const StepArgs& a) {
+  int nblocks = (c->ntiles + a.tiles_per_block - 1) / a.tiles_per_block;
+  std::pair<hipEvent_t, hipEvent_t> ev;
+  if (c->prof_on) ev = ev_pair(c);
+  hipLaunchKernelGGL(k_push_fp64, dim3(nblocks), dim3(KAOS_BLOCK), 0, c->stream, a);
+  if (c->prof_on) { (void)hipEventRecord(ev.second, c->stream); c->ev_push.push_back(ev); }
+}
 
 // Enqueue step n on the stream (no host synchronisation).
 static int enqueue_step(kaos_ctx* c, long long n, const double* d_uni) {
@@ -625,10 +727,10 @@ static int enqueue_step(kaos_ctx* c, long long n, const double* d_uni) {
   const bool premaster = is_master(c, n + 1);
   auto zero_hist = [&]() {
     hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, c->stream, c->d_hist, c->nhist_tot);
-    hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, c->stream, c->d_nqc, c->ncell_tot);
+    hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, c->stream, c->d_nqoff, c->ncell_tot);
   };
   if (first) {
-    StepArgs a = make_args(c, 1, 0, premaster);
+    StepArgs a = make_args(c, 1, 0, 0, premaster);
     zero_hist();
     hipLaunchKernelGGL(k_bin_initial, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a);
     // field of master step 1 from "master step 0" (no moments: M = 0), then moments of step 1
@@ -636,30 +738,28 @@ static int enqueue_step(kaos_ctx* c, long long n, const double* d_uni) {
                        c->d_M, c->d_M0F, c->d_EA, c->d_esc_acc, c->d_esc_rep, 0);
     hipLaunchKernelGGL(k_moments, dim3(c->nblk_mom), dim3(KAOS_BLOCK), 0, c->stream, c->d_tubes, c->d_blk_tube,
                        c->d_blk_q, c->d_d3x, c->d_d3v, c->d_dvpar, c->d_dvperp, c->d_vparGC, c->d_vperpGC,
-                       c->d_hist, c->d_nqc, c->d_NqRT, c->d_M, c->d_count, (const long long*)nullptr);
-    int nblocks = (c->ntiles + a.tiles_per_block - 1) / a.tiles_per_block;
-    hipLaunchKernelGGL(k_push_fp64, dim3(nblocks), dim3(KAOS_BLOCK), 0, c->stream, a);
+                       c->d_hist, c->d_nqoff, c->d_nqc, c->d_NqRT, c->d_M, c->d_count, (const long long*)nullptr);
+    launch_push(c, a);
   } else {
     if (master) {
       hipLaunchKernelGGL(k_field, dim3(nt), dim3(64), 0, c->stream, c->d_tubes, c->d_hq, c->d_dq, c->d_Te,
                          c->d_M, c->d_M0F, c->d_EA, c->d_esc_acc, c->d_esc_rep, 1);
       zero_hist();
     }
-    StepArgs a = make_args(c, master, master, premaster);
-    int nblocks = (c->ntiles + a.tiles_per_block - 1) / a.tiles_per_block;
-    hipLaunchKernelGGL(k_push_fp64, dim3(nblocks), dim3(KAOS_BLOCK), 0, c->stream, a);
+    StepArgs a = make_args(c, master, master, master, premaster);
+    launch_push(c, a);
     if (master) {
       if (c->inj_total > 0) {
         int nb = (int)((c->inj_total + 255) / 256);
         hipLaunchKernelGGL(k_inject, dim3(nb), dim3(256), 0, c->stream, c->d_tubes, c->d_inj_base, nt,
                            c->d_qGH, c->d_qGC, c->d_EA, c->d_EG, c->d_vparGH, c->d_vperpGH,
                            c->d_rs, c->d_vp, c->d_vr, c->d_qk, c->d_ae, c->d_ag, c->d_cell, c->d_count,
-                           c->d_hist, c->d_nqc, d_uni, c->d_uni_base, (unsigned long long)c->cfg.seed,
+                           c->d_hist, c->d_nqoff, d_uni, c->d_uni_base, (unsigned long long)c->cfg.seed,
                            (long long)n, c->inj_total);
       }
       hipLaunchKernelGGL(k_moments, dim3(c->nblk_mom), dim3(KAOS_BLOCK), 0, c->stream, c->d_tubes, c->d_blk_tube,
                          c->d_blk_q, c->d_d3x, c->d_d3v, c->d_dvpar, c->d_dvperp, c->d_vparGC, c->d_vperpGC,
-                         c->d_hist, c->d_nqc, c->d_NqRT, c->d_M, c->d_count, c->d_inj_cnt);
+                         c->d_hist, c->d_nqoff, c->d_nqc, c->d_NqRT, c->d_M, c->d_count, c->d_inj_cnt);
     }
   }
   HIPCHK(c, hipGetLastError());
@@ -730,14 +830,20 @@ extern "C" int kaos_step(kaos_ctx* c, int64_t n, const double* uniforms, int64_t
 
 extern "C" int kaos_run(kaos_ctx* c, int64_t n_first, int64_t nsteps, int64_t* particle_steps) {
   if (!c) return KAOS_E_ARG;
+  unsigned long long p0 = 0, p1 = 0;
+  HIPCHK(c, hipMemcpy(&p0, c->d_pushed, sizeof(p0), hipMemcpyDeviceToHost));
+  std::pair<hipEvent_t, hipEvent_t> ev;
+  if (c->prof_on) ev = ev_pair(c);
   for (int64_t i = 0; i < nsteps; ++i) {
     int rc = check_next(c, n_first + i);
     if (rc) return rc;
     rc = enqueue_step(c, n_first + i, nullptr);
     if (rc) return rc;
   }
+  if (c->prof_on) { (void)hipEventRecord(ev.second, c->stream); c->ev_step.push_back(ev); }
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (particle_steps) *particle_steps = -1;
+  HIPCHK(c, hipMemcpy(&p1, c->d_pushed, sizeof(p1), hipMemcpyDeviceToHost));
+  if (particle_steps) *particle_steps = (int64_t)(p1 - p0);
   int32_t st = 0;
   int rc = read_status(c, &st);
   if (rc) return rc;
@@ -786,7 +892,7 @@ extern "C" int kaos_download_master(kaos_ctx* c, int32_t tube, int64_t* nq_count
   if (EG) HIPCHK(c, hipMemcpy(EG, c->d_EG + T.goff, sizeof(double) * nq, hipMemcpyDeviceToHost));
   if (esc) {
     unsigned long long e[2];
-    HIPCHK(c, hipMemcpy(e, c->d_esc_rep + 2 * tube, sizeof(e), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(e, c->d_esc_acc + 2 * tube, sizeof(e), hipMemcpyDeviceToHost));
     const kaos_grid& g = h.g;
     int ub = nq - 1;
     double dtm = (double)c->cfg.ndatfac * g.h;
@@ -849,6 +955,35 @@ extern "C" int kaos_download_particles(kaos_ctx* c, int32_t tube, int64_t n_cap,
   return KAOS_OK;
 }
 
+extern "C" int kaos_profile(kaos_ctx* c, int32_t enable) {
+  if (!c) return KAOS_E_ARG;
+  c->prof_on = enable != 0;
+  return KAOS_OK;
+}
+
+extern "C" int kaos_profile_read(kaos_ctx* c, double* push_ms, int64_t* push_launches, double* run_ms) {
+  if (!c) return KAOS_E_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double tp = 0.0, tr = 0.0;
+  for (auto& e : c->ev_push) {
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e.first, e.second));
+    tp += ms;
+    (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second);
+  }
+  for (auto& e : c->ev_step) {
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e.first, e.second));
+    tr += ms;
+    (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second);
+  }
+  if (push_ms) *push_ms = tp;
+  if (push_launches) *push_launches = (int64_t)c->ev_push.size();
+  if (run_ms) *run_ms = tr;
+  c->ev_push.clear(); c->ev_step.clear();
+  return KAOS_OK;
+}
+
 extern "C" int kaos_finalize(kaos_ctx* c) {
   if (!c) return KAOS_OK;
   (void)hipStreamSynchronize(c->stream);
@@ -856,8 +991,8 @@ extern "C" int kaos_finalize(kaos_ctx* c) {
                   c->d_hq, c->d_dq, c->d_Te, c->d_d3x, c->d_EA, c->d_EG, c->d_M, c->d_M0F, c->d_NqRT,
                   c->d_vparGH, c->d_vperpGH, c->d_vparGC, c->d_vperpGC, c->d_dvpar, c->d_dvperp, c->d_d3v,
                   c->d_rs, c->d_vp, c->d_vr, c->d_qk, c->d_ae, c->d_ag, c->d_cell, c->d_count, c->d_hist,
-                  c->d_nqc, c->d_status, c->d_esc_acc, c->d_esc_rep, c->d_inj_base, c->d_inj_cnt,
-                  c->d_uni_base, c->d_uni};
+                  c->d_nqc, c->d_nqoff, c->d_status, c->d_esc_acc, c->d_esc_rep, c->d_inj_base, c->d_inj_cnt,
+                  c->d_uni_base, c->d_uni, c->d_pushed};
   for (void* p : bufs) if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -78,8 +78,20 @@ class KineticSolver:
         return out
 
     def run(self, n_first, nsteps):
+        """Steps n_first..n_first+nsteps-1 with no host sync between them; returns
+        the number of ion pushes (particle-steps)."""
         ps = C.c_int64(0)
         self._chk(self.lib.kaos_run(self.h, n_first, nsteps, C.byref(ps)))
+        return ps.value
+
+    def profile(self, enable=True):
+        self._chk(self.lib.kaos_profile(self.h, int(enable)))
+
+    def profile_read(self):
+        """(push-kernel ms summed, push launches, kaos_run ms summed) since the last read."""
+        pm, nl, rm = C.c_double(0), C.c_int64(0), C.c_double(0)
+        self._chk(self.lib.kaos_profile_read(self.h, C.byref(pm), C.byref(nl), C.byref(rm)))
+        return pm.value, nl.value, rm.value
 
     def count(self, tube):
         n = C.c_int64(0)
