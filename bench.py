@@ -22,8 +22,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 # bytes per ion per step of the FP64 push at ndatfac = 1:
-# read rs, vp, vr, qk (4 x 8) ; write rs, vp, qk, ae, ag (5 x 8) + cell (4)
-FP64_BYTES_PER_ION = 32 + 44
+# read rs, vp, vr, qk (4 x 8) + cell (4); write rs, vp, qk, ae, ag (5 x 8) + cell (4)
+FP64_BYTES_PER_ION = 36 + 44
 HBM_PEAK_GBS = 8000.0
 
 

@@ -13,7 +13,7 @@
  * Where the reference evaluates a transcendental of a transcendental, the
  * exact algebraic value is used instead (all equal in real arithmetic):
  *   cos(acos(z/r))            -> z/r                  (_KineticMainParams.f90:579-582)
- *   sin(acos(z/r))            -> sqrt(x^2+y^2)/r      (theta in [0,pi])
+ *   sin(acos(z/r))            -> rho/r, rho = sqrt(x^2+y^2)  (theta in [0,pi])
  *   sin(asin(sqrt(w)))        -> sqrt(w)              (_8_1_4_RK4DipolePolynomialSolver.f90:89-90)
  *   cos(asin(sqrt(w)))        -> sqrt(1-w)
  *   cos(pi - t)               -> -cos(t)              (_8_1_4_RK4DipolePolynomialSolver.f90:101-105)
@@ -45,44 +45,6 @@
 #define KAOS_PI   3.14159265358979311600   /* 4*atan(1d0) in binary64 */
 #define KAOS_TWOPI 6.28318530717958623200
 #define KAOS_ZERO_CUT 1e-6        /* _8_1_4_RK4DipolePolynomialSolver.f90:115-125 */
-
-/* ------------------------------------------------------------------------- */
-/* Geometry at a Cartesian point on the field line.                          */
-
-typedef struct {
-  double r;            /* |x|                                    */
-  double c;            /* cos(theta)                             */
-  double s;            /* sin(theta)                             */
-  double q;            /* dipole q = RE^2 cos(theta)/r^2         */
-  double bx, by, bz;   /* unit field direction b-hat             */
-} kaos_geom_t;
-
-/* rsub/thetasub/qsub/ellsub (_KineticMainParams.f90:568-615) and the
- * field-aligned unit vector used by every "RE-ALIGN VELOCITY" and force
- * reconstruction block (e.g. _8_1_1_KineticUpdateA.f90:978-980, 1327-1334). */
-KAOS_HD kaos_geom_t kaos_geom(double x, double y, double z, double cphi, double sphi) {
-  kaos_geom_t g;
-  double xy2 = x * x + y * y;
-  g.r = sqrt(xy2 + z * z);
-  double ir = 1.0 / g.r;
-  g.c = z * ir;
-  g.s = sqrt(xy2) * ir;
-  g.q = (KAOS_RE2 * g.c) * (ir * ir);
-  double c2 = g.c * g.c;
-  double isl = 1.0 / sqrt(1.0 + 3.0 * c2);      /* 1/sqrt(ell) */
-  double t = ((3.0 * g.c) * g.s) * isl;
-  g.bx = cphi * t;
-  g.by = sphi * t;
-  g.bz = (3.0 * c2 - 1.0) * isl;
-  return g;
-}
-
-/* q only (used on the RK4 predictor position, _8_1_KineticRK4Update.f90:161-163). */
-KAOS_HD double kaos_q_of(double x, double y, double z) {
-  double r = sqrt((x * x + y * y) + z * z);
-  double ir = 1.0 / r;
-  return (KAOS_RE2 * (z * ir)) * (ir * ir);
-}
 
 /* ------------------------------------------------------------------------- */
 /* Resolvent-cubic root: w = sinh(asinh(y)/3), y >= 0, i.e. the real root of
@@ -134,23 +96,6 @@ KAOS_HD double kaos_root_r(double q, double p) {
   return gamma * KAOS_RE;                       /* r3RK4 */
 }
 
-/* Cartesian point from the signed radius rs (sign of rs = sign used for the
- * hemisphere phase shift: negative when q <= 0), _8_1_4:89-125.            */
-KAOS_HD void kaos_pos_from_rs(double rs, double p, double cphi, double sphi,
-                              double* x, double* y, double* z) {
-  double r = fabs(rs);
-  double w = r * (1.0 / (KAOS_RE * p));
-  double s = sqrt(w);
-  double c = sqrt(1.0 - w);
-  if (rs < 0.0) c = -c;
-  double rsn = r * s;
-  double xx = rsn * cphi, yy = rsn * sphi, zz = r * c;
-  if (fabs(xx) < KAOS_ZERO_CUT) xx = 0.0;
-  if (fabs(yy) < KAOS_ZERO_CUT) yy = 0.0;
-  if (fabs(zz) < KAOS_ZERO_CUT) zz = 0.0;
-  *x = xx; *y = yy; *z = zz;
-}
-
 /* Signed radius of the field-line point with dipole coordinate q. */
 KAOS_HD double kaos_rs_of_q(double q, double p) {
   double r = kaos_root_r(q, p);
@@ -192,11 +137,73 @@ KAOS_HD int kaos_find_vcell(double v, const double* GH, double GL0, int n) {
   return lo + 1;
 }
 
+/* The same two searches started from a hint (the ion's cell of the previous
+ * step, 1-based; ions move less than a cell per step): a walk to the first
+ * cell whose high edge is not passed, identical in result to the bisection. */
+KAOS_HD int kaos_find_qcell_from(double q, const double* qGH, double qGL0, double qGHlast,
+                                 int nq, int desc, int hint) {
+  int i = hint - 1;
+  if (i < 0) i = 0;
+  if (i > nq - 1) i = nq - 1;
+  if (desc) {
+    if (!(q <= qGL0)) return 0;
+    if (qGHlast > q) return -1;
+    while (i > 0 && q >= qGH[i - 1]) --i;
+    while (!(q >= qGH[i])) ++i;
+  } else {
+    if (!(q >= qGL0)) return 0;
+    if (qGHlast < q) return -1;
+    while (i > 0 && q <= qGH[i - 1]) --i;
+    while (!(q <= qGH[i])) ++i;
+  }
+  return i + 1;
+}
+
+/* Velocity cell from a uniform-spacing guess (scale = n/(GH[n-1]-GL0)). */
+KAOS_HD int kaos_find_vcell_guess(double v, const double* GH, double GL0, int n, double scale) {
+  if (!(v >= GL0) || !(v <= GH[n - 1])) return 0;
+  int i = (int)((v - GL0) * scale);
+  if (i < 0) i = 0;
+  if (i > n - 1) i = n - 1;
+  while (i > 0 && v <= GH[i - 1]) --i;
+  while (!(v <= GH[i])) ++i;
+  return i + 1;
+}
+
 /* ------------------------------------------------------------------------- */
 /* Field interpolation along the tube (_8_7_AmbipolarEfield.f90:1666-1860 for
  * q increasing with the cell index, 2067-2230 for q decreasing;
  * _8_7_2_Gravfield.f90:132-300).  Returns 1 and the interpolated value when q
  * falls in [qGL(LB), qGH(UB)]; 0 (value untouched) otherwise.             */
+/* Interpolation with a walk from a hint cell (1-based) instead of bisection;
+ * E2 (may be NULL) is interpolated on the same interval into *out2.        */
+KAOS_HD int kaos_interp2_from(double q, const double* qGC, const double* E, const double* E2, int nq,
+                              double qGL0, double qGHlast, int desc, int hint, double* out,
+                              double* out2) {
+  int ub = nq - 1;
+  int lo = hint - 1;
+  if (lo < 1) lo = 1;
+  if (lo > ub) lo = ub;
+  if (desc) {
+    if (!(q <= qGL0) || !(q >= qGHlast)) return 0;
+    if (q >= qGC[0]) { *out = E[0]; if (E2) *out2 = E2[0]; return 1; }
+    if (q <= qGC[ub]) { *out = E[ub]; if (E2) *out2 = E2[ub]; return 1; }
+    while (lo > 1 && q >= qGC[lo - 1]) --lo;
+    while (!(q >= qGC[lo])) ++lo;
+  } else {
+    if (!(q >= qGL0) || !(q <= qGHlast)) return 0;
+    if (q <= qGC[0]) { *out = E[0]; if (E2) *out2 = E2[0]; return 1; }
+    if (q >= qGC[ub]) { *out = E[ub]; if (E2) *out2 = E2[ub]; return 1; }
+    while (lo > 1 && q <= qGC[lo - 1]) --lo;
+    while (!(q <= qGC[lo])) ++lo;
+  }
+  double x1 = qGC[lo - 1], x2 = qGC[lo];
+  double f = (q - x1) / (x2 - x1);
+  *out = E[lo - 1] + (f * (E[lo] - E[lo - 1]));
+  if (E2) *out2 = E2[lo - 1] + (f * (E2[lo] - E2[lo - 1]));
+  return 1;
+}
+
 KAOS_HD int kaos_interp_cells(double q, const double* qGC, const double* E, int nq,
                               double qGL0, double qGHlast, int desc, double* out) {
   int ub = nq - 1;
@@ -207,7 +214,8 @@ KAOS_HD int kaos_interp_cells(double q, const double* qGC, const double* E, int 
     int lo = 1, hi = ub;                 /* first Q with q >= qGC[Q] */
     while (lo < hi) { int mid = (lo + hi) >> 1; if (q >= qGC[mid]) hi = mid; else lo = mid + 1; }
     double x1 = qGC[lo - 1], x2 = qGC[lo], y1 = E[lo - 1], y2 = E[lo];
-    *out = y1 + (((q - x1) / (x2 - x1)) * (y2 - y1));
+    double f = (q - x1) / (x2 - x1);
+    *out = y1 + (f * (y2 - y1));
     return 1;
   } else {
     if (!(q >= qGL0) || !(q <= qGHlast)) return 0;
@@ -216,7 +224,8 @@ KAOS_HD int kaos_interp_cells(double q, const double* qGC, const double* E, int 
     int lo = 1, hi = ub;                 /* first Q with q <= qGC[Q] */
     while (lo < hi) { int mid = (lo + hi) >> 1; if (q <= qGC[mid]) hi = mid; else lo = mid + 1; }
     double x1 = qGC[lo - 1], x2 = qGC[lo], y1 = E[lo - 1], y2 = E[lo];
-    *out = y1 + (((q - x1) / (x2 - x1)) * (y2 - y1));
+    double f = (q - x1) / (x2 - x1);
+    *out = y1 + (f * (y2 - y1));
     return 1;
   }
 }
@@ -227,7 +236,14 @@ KAOS_HD int kaos_interp_cells(double q, const double* qGC, const double* E, int 
  * parallel-potential flags are off on this path).  The parallel acceleration
  * of each stage is (sG |AG| + sE AEA) and the Cartesian acceleration is that
  * times b-hat; the reference projects each force separately and sums, which
- * is the same in real arithmetic.                                          */
+ * is the same in real arithmetic.
+ *
+ * An ion stays in the meridional plane of its tube (the reference resets phi
+ * to the tube longitude every step, _8_1_KineticRK4Update.f90:165-166), so
+ * the push is carried out on (rho, z) = (sqrt(x^2+y^2), z), with
+ * (x, y) = rho (cos phi, sin phi); and the state at a step boundary is the
+ * signed radius rs of the root solve plus V|| (the reference's state is
+ * x = pos(rs), V = V|| b-hat(x), _8_1_KineticRK4Update.f90:170-314).        */
 
 typedef struct {
   double h;            /* time step SpecieT%hT */
@@ -239,14 +255,58 @@ typedef struct {
 } kaos_push_par_t;
 
 typedef struct {
-  double xN, yN, zN;     /* new position (xN,yN,zN)                       */
-  double VxN, VyN, VzN;  /* new velocity                                  */
-  double rsN;            /* signed radius of the new position              */
+  double rsN;            /* signed radius of the new point                 */
   double vparN;          /* VparNp: parallel velocity of the new state     */
   double vpar;           /* Vpar(j): parallel velocity of the input state  */
   double qk1;            /* q of the input state (cell search input)       */
   double qk4;            /* q at the full-step predictor (field coordinate) */
 } kaos_push_out_t;
+
+/* Point of the field line with signed radius rs: r, cos, sin and (rho, z)
+ * with the reference's zeroing of tiny coordinates (_8_1_4:111-125).       */
+typedef struct { double r, c, s, rho, z; } kaos_line_t;
+
+KAOS_HD kaos_line_t kaos_line(double rs, double p) {
+  kaos_line_t L;
+  L.r = fabs(rs);
+  double w = L.r * (1.0 / (KAOS_RE * p));
+  L.s = sqrt(w);
+  L.c = sqrt(1.0 - w);
+  if (rs < 0.0) L.c = -L.c;
+  L.rho = L.r * L.s;
+  L.z = L.r * L.c;
+  if (fabs(L.z) < KAOS_ZERO_CUT) L.z = 0.0;
+  return L;
+}
+
+/* Cartesian point and velocity V|| b-hat of a line state (for output). */
+KAOS_HD void kaos_cartesian(double rs, double vs, double p, double cphi, double sphi,
+                            double* x, double* y, double* z, double* Vx, double* Vy, double* Vz) {
+  kaos_line_t L = kaos_line(rs, p);
+  double xx = L.rho * cphi, yy = L.rho * sphi;
+  if (fabs(xx) < KAOS_ZERO_CUT) xx = 0.0;
+  if (fabs(yy) < KAOS_ZERO_CUT) yy = 0.0;
+  *x = xx; *y = yy; *z = L.z;
+  double isl = 1.0 / sqrt(1.0 + 3.0 * (L.c * L.c));
+  double bR = ((3.0 * L.c) * L.s) * isl, bZ = (3.0 * (L.c * L.c) - 1.0) * isl;
+  *Vx = (vs * bR) * cphi; *Vy = (vs * bR) * sphi; *Vz = vs * bZ;
+}
+
+/* q and b-hat (rho, z components) at a point of the meridional plane. */
+typedef struct { double q, bR, bZ; } kaos_qb_t;
+
+KAOS_HD kaos_qb_t kaos_qb(double rho, double z) {
+  kaos_qb_t g;
+  double r = sqrt(rho * rho + z * z);
+  double ir = 1.0 / r;
+  double c = z * ir, s = rho * ir;
+  g.q = (KAOS_RE2 * c) * (ir * ir);
+  double c2 = c * c;
+  double isl = 1.0 / sqrt(1.0 + 3.0 * c2);
+  g.bR = ((3.0 * c) * s) * isl;
+  g.bZ = (3.0 * c2 - 1.0) * isl;
+  return g;
+}
 
 KAOS_HD double kaos_apar(double q, double AG, double AEA, const kaos_push_par_t* P, int with_grav) {
   double a = 0.0;
@@ -258,54 +318,56 @@ KAOS_HD double kaos_apar(double q, double AG, double AEA, const kaos_push_par_t*
   return a;
 }
 
-/* Stage A..C and the combine.  valid_q = the input state's config cell is in
- * the domain (stage-4 gravity gate, _8_1_3_KineticUpdateC.f90:345-346).    */
-KAOS_HD void kaos_push_ion(double x, double y, double z, double Vx, double Vy, double Vz,
-                           double AG, double AEA, const kaos_push_par_t* P,
-                           const double* qGH, double qGL0, double qGHlast, int nq, int desc,
-                           kaos_push_out_t* o, int* qcell_io) {
+/* Stage A of the input state rs (KineticUpdateASub): q and b-hat. */
+KAOS_HD kaos_qb_t kaos_stageA(kaos_line_t L) {
+  kaos_qb_t g;
+  g.q = (KAOS_RE2 * L.c) / (L.r * L.r);
+  double c2 = L.c * L.c;
+  double isl = 1.0 / sqrt(1.0 + 3.0 * c2);
+  g.bR = ((3.0 * L.c) * L.s) * isl;
+  g.bZ = (3.0 * c2 - 1.0) * isl;
+  return g;
+}
+
+/* The push, given the stage-A geometry (L, gA) and the input cell qc of the
+ * ion (used only for the stage-4 gravity gate, _8_1_3_KineticUpdateC.f90:345-346). */
+KAOS_HD void kaos_push_ion(kaos_line_t L, kaos_qb_t gA, int qc, double vs, double AG, double AEA,
+                           const kaos_push_par_t* P, kaos_push_out_t* o) {
   const double h = P->h, h2 = P->h / 2.0, hsq2 = (P->h * P->h) / 2.0, h6 = P->h / 6.0;
-  /* Stage A (KineticUpdateASub) */
-  kaos_geom_t g1 = kaos_geom(x, y, z, P->cphi, P->sphi);
-  int qc = kaos_find_qcell(g1.q, qGH, qGL0, qGHlast, nq, desc);
-  *qcell_io = qc;
-  double ap1 = kaos_apar(g1.q, AG, AEA, P, 1);
-  double a1x = ap1 * g1.bx, a1y = ap1 * g1.by, a1z = ap1 * g1.bz;
-  double vpar = (g1.bx * Vx + g1.by * Vy) + g1.bz * Vz;
-  double VRx = vpar * g1.bx, VRy = vpar * g1.by, VRz = vpar * g1.bz;
-  /* half-step position, stage B (KineticUpdateBSub) */
-  double xk2 = x + h2 * VRx, yk2 = y + h2 * VRy, zk2 = z + h2 * VRz;
-  kaos_geom_t g2 = kaos_geom(xk2, yk2, zk2, P->cphi, P->sphi);
+  double ap1 = kaos_apar(gA.q, AG, AEA, P, 1);
+  double a1R = ap1 * gA.bR, a1Z = ap1 * gA.bZ;
+  /* V = V|| b-hat; Vpar(j) = b-hat . V (KUA:978-980), VR = Vpar b-hat (KUA:1327-1334) */
+  double vpar = gA.bR * (vs * gA.bR) + gA.bZ * (vs * gA.bZ);
+  double VRR = vpar * gA.bR, VRZ = vpar * gA.bZ;
+  /* half-step position and stage B (KineticUpdateBSub) */
+  kaos_qb_t g2 = kaos_qb(L.rho + h2 * VRR, L.z + h2 * VRZ);
   double ap2 = kaos_apar(g2.q, AG, AEA, P, 1);
-  double a2x = ap2 * g2.bx, a2y = ap2 * g2.by, a2z = ap2 * g2.bz;
-  /* full-step predictor, stage C (KineticUpdateCSub) */
-  double xk4 = (x + h * VRx) + hsq2 * a1x;
-  double yk4 = (y + h * VRy) + hsq2 * a1y;
-  double zk4 = (z + h * VRz) + hsq2 * a1z;
-  kaos_geom_t g4 = kaos_geom(xk4, yk4, zk4, P->cphi, P->sphi);
+  double a2R = ap2 * g2.bR, a2Z = ap2 * g2.bZ;
+  /* full-step predictor and stage C (KineticUpdateCSub) */
+  kaos_qb_t g4 = kaos_qb((L.rho + h * VRR) + hsq2 * a1R, (L.z + h * VRZ) + hsq2 * a1Z);
   double ap4 = kaos_apar(g4.q, AG, AEA, P, (qc != 0 && qc != -1));
-  double a4x = ap4 * g4.bx, a4y = ap4 * g4.by, a4z = ap4 * g4.bz;
+  double a4R = ap4 * g4.bR, a4Z = ap4 * g4.bZ;
   /* velocity: k1 + 2 k2 + 2 k3 + k4 with k2 = k3 (_8_1_KineticRK4Update.f90:85-90) */
-  double VNx = VRx + h6 * (((a1x + 2.0 * a2x) + 2.0 * a2x) + a4x);
-  double VNy = VRy + h6 * (((a1y + 2.0 * a2y) + 2.0 * a2y) + a4y);
-  double VNz = VRz + h6 * (((a1z + 2.0 * a2z) + 2.0 * a2z) + a4z);
+  double VNR = VRR + h6 * (((a1R + 2.0 * a2R) + 2.0 * a2R) + a4R);
+  double VNZ = VRZ + h6 * (((a1Z + 2.0 * a2Z) + 2.0 * a2Z) + a4Z);
   /* position (_8_1_KineticRK4Update.f90:124-156) */
-  double k2x = VRx + h2 * a1x, k2y = VRy + h2 * a1y, k2z = VRz + h2 * a1z;
-  double k4x = VRx + h * a2x, k4y = VRy + h * a2y, k4z = VRz + h * a2z;
-  double xNp = x + h6 * (((VRx + 2.0 * k2x) + 2.0 * k2x) + k4x);
-  double yNp = y + h6 * (((VRy + 2.0 * k2y) + 2.0 * k2y) + k4y);
-  double zNp = z + h6 * (((VRz + 2.0 * k2z) + 2.0 * k2z) + k4z);
-  /* re-project onto the L-shell (root solve) */
-  double qNp = kaos_q_of(xNp, yNp, zNp);
+  double k2R = VRR + h2 * a1R, k2Z = VRZ + h2 * a1Z;
+  double k4R = VRR + h * a2R, k4Z = VRZ + h * a2Z;
+  double RNp = L.rho + h6 * (((VRR + 2.0 * k2R) + 2.0 * k2R) + k4R);
+  double ZNp = L.z + h6 * (((VRZ + 2.0 * k2Z) + 2.0 * k2Z) + k4Z);
+  /* re-project onto the L-shell: q of the new point, root solve */
+  double rN = sqrt(RNp * RNp + ZNp * ZNp);
+  double irN = 1.0 / rN;
+  double qNp = (KAOS_RE2 * (ZNp * irN)) * (irN * irN);
   double rsN = kaos_rs_of_q(qNp, P->p);
-  double xN, yN, zN;
-  kaos_pos_from_rs(rsN, P->p, P->cphi, P->sphi, &xN, &yN, &zN);
   /* re-align the velocity with b-hat at the new point (:273-314) */
-  kaos_geom_t gN = kaos_geom(xN, yN, zN, P->cphi, P->sphi);
-  double vparN = (gN.bx * VNx + gN.by * VNy) + gN.bz * VNz;
-  o->xN = xN; o->yN = yN; o->zN = zN;
-  o->VxN = vparN * gN.bx; o->VyN = vparN * gN.by; o->VzN = vparN * gN.bz;
-  o->rsN = rsN; o->vparN = vparN; o->vpar = vpar; o->qk1 = g1.q; o->qk4 = g4.q;
+  kaos_line_t LN = kaos_line(rsN, P->p);
+  kaos_qb_t gN = kaos_stageA(LN);
+  o->rsN = rsN;
+  o->vparN = gN.bR * VNR + gN.bZ * VNZ;
+  o->vpar = vpar;
+  o->qk1 = gA.q;
+  o->qk4 = g4.q;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -423,7 +485,7 @@ KAOS_HD double kaos_gauss(double u1, double u2) {
  * Cartesian Maxwellian onto (Vp, Vq, Vphi); Vperp1 = Vp, Vperp2 = Vphi,
  * Vperp = |(Vperp1, Vperp2)|, Vpar = Vq.                                    */
 KAOS_HD void kaos_mb_project(const double u[6], double sigma, double cphi, double sphi,
-                             kaos_geom_t g, double* vpar, double* vperp1,
+                             kaos_line_t g, double* vpar, double* vperp1,
                              double* vperp2, double* vperp) {
   double Vx = sigma * kaos_gauss(u[0], u[1]);
   double Vy = sigma * kaos_gauss(u[2], u[3]);
@@ -431,7 +493,7 @@ KAOS_HD void kaos_mb_project(const double u[6], double sigma, double cphi, doubl
   double ell = 1.0 + 3.0 * (g.c * g.c);
   double sl = sqrt(ell);
   double h = Vx * cphi + Vy * sphi;
-  double Vp = ((1.0 - 3.0 * (g.c * g.c)) / sl) * h + ((3.0 * Vz) * g.c * g.s) / sl;
+  double Vp = ((1.0 - 3.0 * (g.c * g.c)) / sl) * h + (((3.0 * Vz) * g.c) * g.s) / sl;
   double Vq = (((3.0 * g.c) * g.s) / sl) * h + (Vz * (3.0 * (g.c * g.c) - 1.0)) / sl;
   double Vphi = -Vx * sphi + Vy * cphi;
   *vperp1 = Vp;
@@ -446,47 +508,50 @@ KAOS_HD void kaos_mb_project(const double u[6], double sigma, double cphi, doubl
 /* MomentFilterSub, _8_7_1_MomentFilter.f90:25-147: asymmetric sliding average
  * with window W (odd) and shrinking one-sided windows near LB/UB.  in/out are
  * 0-based arrays of the nq cells.                                          */
-KAOS_HD void kaos_moment_filter(const double* in, double* out, int nq, int W) {
+KAOS_HD double kaos_moment_filter_cell(const double* in, int nq, int W, int Q) {
   int hw = (W - 1) / 2;
   int LB = 1, UB = nq;
-  for (int Q = LB; Q <= UB; ++Q) {
-    double s1, s2, a1, a2;
-    if (Q <= LB - 1 + hw) {
-      s1 = in[Q - 1] + in[Q];
-      for (int k = 2; k <= hw; ++k) s1 = s1 + in[Q + k - 1];
-      a1 = s1 / (double)(hw + 1);
-      if (Q == LB) out[Q - 1] = a1;
-      else if (Q == LB + 1) { a2 = in[LB - 1]; out[Q - 1] = (a1 + a2) / 2.0; }
-      else {
-        s2 = in[Q - 2];
-        for (int k = 2; k <= Q - LB; ++k) s2 = s2 + in[Q - k - 1];
-        a2 = s2 / (double)(Q - LB);
-        out[Q - 1] = (a1 + a2) / 2.0;
-      }
-    }
-    if (Q > LB - 1 + hw && Q < UB - hw) {
-      s1 = in[Q - 1] + in[Q];
-      for (int k = 2; k <= hw; ++k) s1 = s1 + in[Q + k - 1];
+  double s1, s2, a1, a2, out = 0.0;
+  if (Q <= LB - 1 + hw) {
+    s1 = in[Q - 1] + in[Q];
+    for (int k = 2; k <= hw; ++k) s1 = s1 + in[Q + k - 1];
+    a1 = s1 / (double)(hw + 1);
+    if (Q == LB) out = a1;
+    else if (Q == LB + 1) { a2 = in[LB - 1]; out = (a1 + a2) / 2.0; }
+    else {
       s2 = in[Q - 2];
-      for (int k = 2; k <= hw; ++k) s2 = s2 + in[Q - k - 1];
-      a1 = s1 / (double)(hw + 1);
-      a2 = s2 / (double)hw;
-      out[Q - 1] = (a1 + a2) / 2.0;
-    }
-    if (Q >= UB - hw) {
-      s2 = in[Q - 1] + in[Q - 2];
-      for (int k = 2; k <= hw; ++k) s2 = s2 + in[Q - k - 1];
-      a2 = s2 / (double)(hw + 1);
-      if (Q == UB) out[Q - 1] = a2;
-      else if (Q == UB - 1) { a1 = in[UB - 1]; out[Q - 1] = (a1 + a2) / 2.0; }
-      else {
-        s1 = in[Q];
-        for (int k = 2; k <= UB - Q; ++k) s1 = s1 + in[Q + k - 1];
-        a1 = s1 / (double)(UB - Q);
-        out[Q - 1] = (a1 + a2) / 2.0;
-      }
+      for (int k = 2; k <= Q - LB; ++k) s2 = s2 + in[Q - k - 1];
+      a2 = s2 / (double)(Q - LB);
+      out = (a1 + a2) / 2.0;
     }
   }
+  if (Q > LB - 1 + hw && Q < UB - hw) {
+    s1 = in[Q - 1] + in[Q];
+    for (int k = 2; k <= hw; ++k) s1 = s1 + in[Q + k - 1];
+    s2 = in[Q - 2];
+    for (int k = 2; k <= hw; ++k) s2 = s2 + in[Q - k - 1];
+    a1 = s1 / (double)(hw + 1);
+    a2 = s2 / (double)hw;
+    out = (a1 + a2) / 2.0;
+  }
+  if (Q >= UB - hw) {
+    s2 = in[Q - 1] + in[Q - 2];
+    for (int k = 2; k <= hw; ++k) s2 = s2 + in[Q - k - 1];
+    a2 = s2 / (double)(hw + 1);
+    if (Q == UB) out = a2;
+    else if (Q == UB - 1) { a1 = in[UB - 1]; out = (a1 + a2) / 2.0; }
+    else {
+      s1 = in[Q];
+      for (int k = 2; k <= UB - Q; ++k) s1 = s1 + in[Q + k - 1];
+      a1 = s1 / (double)(UB - Q);
+      out = (a1 + a2) / 2.0;
+    }
+  }
+  return out;
+}
+
+KAOS_HD void kaos_moment_filter(const double* in, double* out, int nq, int W) {
+  for (int Q = 1; Q <= nq; ++Q) out[Q - 1] = kaos_moment_filter_cell(in, nq, W, Q);
 }
 
 /* AmbipolarEfieldSub electron-pressure field with the moment filter on,
@@ -497,37 +562,49 @@ KAOS_HD void kaos_moment_filter(const double* in, double* out, int nq, int W) {
  * when qGL(1) <= 0.  M0F receives the filtered density M0FiltAvrgRT.       */
 /* Stencil part: EA from the filtered density n (M0FiltAvrgRT) and the raw
  * previous-step density M0prev (which only gates cells with no ions). */
-KAOS_HD void kaos_ambipolar_stencil(int nq, int south, int eamb, int eambsign, double qs,
-                                    const double* hqC, const double* dqC, const double* Te,
-                                    const double* M0prev, const double* n, double* EA) {
-  for (int Q = 0; Q < nq; ++Q) {
-    if (M0prev[Q] == 0.0 || !eamb) { EA[Q] = 0.0; continue; }
-    double dq = dqC[Q];
-    double R1 = (1.0 / fabs(hqC[Q])) * ((KAOS_KB * Te[Q]) / (qs * n[Q]));
-    double R2;
-    if (Q == 0) {
-      double S = (((-25.0 * n[Q] + 48.0 * n[Q + 1]) - 36.0 * n[Q + 2]) + 16.0 * n[Q + 3]) - 3.0 * n[Q + 4];
-      if (!eambsign) R2 = fabs(S) / (12.0 * dq);
-      else R2 = south ? (-S) / (-12.0 * dq) : (-S) / (12.0 * dq);
-    } else if (Q == nq - 1) {
-      double S = (((-25.0 * n[Q] + 48.0 * n[Q - 1]) - 36.0 * n[Q - 2]) + 16.0 * n[Q - 3]) - 3.0 * n[Q - 4];
-      if (!eambsign) R2 = fabs(S) / (-12.0 * dq);
-      else R2 = south ? (-S) / (12.0 * dq) : (-S) / (-12.0 * dq);
-    } else if (Q == 1 || Q == nq - 2) {
-      double S = n[Q + 1] - n[Q - 1];
-      if (!eambsign) R2 = fabs(S) / (2.0 * dq);
-      else R2 = south ? (-S) / (-2.0 * dq) : (-S) / (2.0 * dq);
-    } else {
-      double S = ((n[Q - 2] - 8.0 * n[Q - 1]) + 8.0 * n[Q + 1]) - n[Q + 2];
-      if (!eambsign) R2 = fabs(S) / (12.0 * dq);
-      else R2 = south ? (-S) / (-12.0 * dq) : (-S) / (12.0 * dq);
-    }
-    double P = R1 * R2;
-    EA[Q] = eambsign ? P : fabs(P);
+KAOS_HD double kaos_ea_cell(int Q, int nq, int south, int eamb, int eambsign, double qs,
+                            const double* hqC, const double* dqC, const double* Te,
+                            const double* M0prev, const double* n) {
+  if (M0prev[Q] == 0.0 || !eamb) return 0.0;
+  double dq = dqC[Q];
+  double R1 = (1.0 / fabs(hqC[Q])) * ((KAOS_KB * Te[Q]) / (qs * n[Q]));
+  double R2;
+  if (Q == 0) {
+    double S = (((-25.0 * n[Q] + 48.0 * n[Q + 1]) - 36.0 * n[Q + 2]) + 16.0 * n[Q + 3]) - 3.0 * n[Q + 4];
+    if (!eambsign) R2 = fabs(S) / (12.0 * dq);
+    else R2 = south ? (-S) / (-12.0 * dq) : (-S) / (12.0 * dq);
+  } else if (Q == nq - 1) {
+    double S = (((-25.0 * n[Q] + 48.0 * n[Q - 1]) - 36.0 * n[Q - 2]) + 16.0 * n[Q - 3]) - 3.0 * n[Q - 4];
+    if (!eambsign) R2 = fabs(S) / (-12.0 * dq);
+    else R2 = south ? (-S) / (12.0 * dq) : (-S) / (-12.0 * dq);
+  } else if (Q == 1 || Q == nq - 2) {
+    double S = n[Q + 1] - n[Q - 1];
+    if (!eambsign) R2 = fabs(S) / (2.0 * dq);
+    else R2 = south ? (-S) / (-2.0 * dq) : (-S) / (2.0 * dq);
+  } else {
+    double S = ((n[Q - 2] - 8.0 * n[Q - 1]) + 8.0 * n[Q + 1]) - n[Q + 2];
+    if (!eambsign) R2 = fabs(S) / (12.0 * dq);
+    else R2 = south ? (-S) / (-12.0 * dq) : (-S) / (12.0 * dq);
   }
+  double P = R1 * R2;
+  return eambsign ? P : fabs(P);
+}
+
+/* LB, LB+1, LB+2 take the LB+3 value where they hold ions (:1276-1287). */
+KAOS_HD void kaos_ea_override(double* EA, const double* M0prev, int eamb) {
   double e3 = EA[3];
   for (int Q = 0; Q < 3; ++Q)
     if (M0prev[Q] != 0.0 && eamb) EA[Q] = e3;
+}
+
+/* Stencil part: EA from the filtered density n (M0FiltAvrgRT) and the raw
+ * previous-step density M0prev (which only gates cells with no ions). */
+KAOS_HD void kaos_ambipolar_stencil(int nq, int south, int eamb, int eambsign, double qs,
+                                    const double* hqC, const double* dqC, const double* Te,
+                                    const double* M0prev, const double* n, double* EA) {
+  for (int Q = 0; Q < nq; ++Q)
+    EA[Q] = kaos_ea_cell(Q, nq, south, eamb, eambsign, qs, hqC, dqC, Te, M0prev, n);
+  kaos_ea_override(EA, M0prev, eamb);
 }
 
 KAOS_HD void kaos_ambipolar(int nq, int south, int eamb, int eambsign, int W, double qs,

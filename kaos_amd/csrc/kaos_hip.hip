@@ -35,7 +35,7 @@
 #define KAOS_TILE (KAOS_BLOCK * KAOS_PPT)
 #define KAOS_MAXQ 512
 #define KAOS_MAXV 255
-#define KAOS_HWORDS 10240                    // LDS histogram window: 20480 16-bit counters
+#define KAOS_HWORDS 6144                     // LDS histogram window: 12288 16-bit counters
 #define KAOS_TPB 16                          // tiles per push block (<= 63: 16-bit counters)
 
 struct DevTube {
@@ -115,6 +115,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_push_fp64(StepArgs a) {
   for (int i = tid; i < KAOS_HWORDS; i += KAOS_BLOCK) s_hist[i] = 0u;
   __syncthreads();
   const int wq0 = s_wq0, wnq = s_wnq, wtube = s_wtube;
+  double vpar_scale = 0.0, vperp_scale = 0.0, vparGL0 = 0.0, vperpGL0 = 0.0;
   for (int tile = t0; tile < t1; ++tile) {
     int tb = a.tile_tube[tile];
     if (tb != cur) {
@@ -129,9 +130,12 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_push_fp64(StepArgs a) {
       __syncthreads();
       cur = tb;
       end = T.off + a.count[tb];
+      vparGL0 = a.vparGH[T.voff + T.nvpar];
+      vperpGL0 = a.vperpGH[T.woff + T.nvperp];
+      vpar_scale = (double)T.nvpar / (a.vparGH[T.voff + T.nvpar - 1] - vparGL0);
+      vperp_scale = (double)T.nvperp / (a.vperpGH[T.woff + T.nvperp - 1] - vperpGL0);
     }
     const long long start = a.tile_start[tile];
-    const double vparGL0 = a.vparGH[T.voff + T.nvpar] , vperpGL0 = a.vperpGH[T.woff + T.nvperp];
     kaos_push_par_t P = { T.h, T.p, T.cphi, T.sphi, T.grav, T.eamb, T.eambsign };
 #pragma unroll 1
     for (int it = 0; it < KAOS_PPT; ++it) {
@@ -139,29 +143,24 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_push_fp64(StepArgs a) {
       if (j >= end) break;
       double rs = a.rs[j];
       if (rs == 0.0) continue;
-      uint32_t cold = 0u;
-      bool have_old = false;
-      if (a.do_remove) {
+      uint32_t cold = a.cell[j];
+      int qprev = (int)(cold >> 16) - 1;
+      if (a.do_remove && (qprev == 0 || qprev == -1)) {
         // BoundaryConditionsSub at a master step: ions whose cell of the previous
         // push is 0 (LB) or -1 (UB) leave the arrays (BC:158-256, BCReIndex:33-133)
-        cold = a.cell[j];
-        have_old = true;
-        int qprev = (int)(cold >> 16) - 1;
-        if (qprev == 0 || qprev == -1) {
-          a.rs[j] = 0.0;
-          atomicAdd(&a.esc_acc[2 * cur + (qprev == 0 ? 0 : 1)], 1ull);
-          continue;
-        }
+        a.rs[j] = 0.0;
+        atomicAdd(&a.esc_acc[2 * cur + (qprev == 0 ? 0 : 1)], 1ull);
+        continue;
       }
       ++npushed;
+      int hint = (qprev >= 1) ? qprev : (qprev == 0 ? 1 : T.nq);
       double vs = a.vp[j], vperp = a.vr[j];
       double aea, ag;
       if (a.do_interp) {
-        double q = a.qk[j], e;
-        if (kaos_interp_cells(q, s_qGC, s_EA, T.nq, T.qGL0, T.qGHlast, T.desc, &e)) {
-          aea = kaos_aea_of(e, T.qsms, T.eambsign);
-          kaos_interp_cells(q, s_qGC, s_EG, T.nq, T.qGL0, T.qGHlast, T.desc, &e);
-          ag = kaos_ag_of(e, T.qsms);
+        double ea, eg;
+        if (kaos_interp2_from(a.qk[j], s_qGC, s_EA, s_EG, T.nq, T.qGL0, T.qGHlast, T.desc, hint, &ea, &eg)) {
+          aea = kaos_aea_of(ea, T.qsms, T.eambsign);
+          ag = kaos_ag_of(eg, T.qsms);
           a.ae[j] = aea; a.ag[j] = ag;
         } else {
           aea = a.ae[j]; ag = a.ag[j];
@@ -169,21 +168,16 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_push_fp64(StepArgs a) {
       } else {
         aea = a.ae[j]; ag = a.ag[j];
       }
-      double x, y, z;
-      kaos_pos_from_rs(rs, T.p, T.cphi, T.sphi, &x, &y, &z);
-      kaos_geom_t g = kaos_geom(x, y, z, T.cphi, T.sphi);
+      kaos_line_t L = kaos_line(rs, T.p);
+      kaos_qb_t gA = kaos_stageA(L);
+      int qc = kaos_find_qcell_from(gA.q, s_qGH, T.qGL0, T.qGHlast, T.nq, T.desc, hint);
       kaos_push_out_t o;
-      int qc;
-      kaos_push_ion(x, y, z, vs * g.bx, vs * g.by, vs * g.bz, ag, aea, &P,
-                    s_qGH, T.qGL0, T.qGHlast, T.nq, T.desc, &o, &qc);
+      kaos_push_ion(L, gA, qc, vs, ag, aea, &P, &o);
       // velocity cells (2-D): off-grid keeps the previous index (KUA:1087-1127)
-      int ip = kaos_find_vcell(o.vpar, s_vparGH, vparGL0, T.nvpar);
-      int iq = kaos_find_vcell(vperp, s_vperpGH, vperpGL0, T.nvperp);
-      if (ip == 0 || iq == 0) {
-        uint32_t old = have_old ? cold : a.cell[j];
-        if (ip == 0) ip = (int)(old & 0xFFu);
-        if (iq == 0) iq = (int)((old >> 8) & 0xFFu);
-      }
+      int ip = kaos_find_vcell_guess(o.vpar, s_vparGH, vparGL0, T.nvpar, vpar_scale);
+      int iq = kaos_find_vcell_guess(vperp, s_vperpGH, vperpGL0, T.nvperp, vperp_scale);
+      if (ip == 0) ip = (int)(cold & 0xFFu);
+      if (iq == 0) iq = (int)((cold >> 8) & 0xFFu);
       if (a.do_count && qc >= 1) {
         if (ip >= 1 && iq >= 1) {
           int wq = qc - wq0;
@@ -283,12 +277,21 @@ __global__ void k_field(const DevTube* tubes, const double* hq, const double* dq
                         const double* Mprev, double* M0F, double* EA,
                         unsigned long long* esc_acc, unsigned long long* esc_rep, int rotate_esc) {
   int t = blockIdx.x;
-  if (threadIdx.x != 0) return;
   DevTube T = tubes[t];
   const double* M0 = Mprev + 6 * T.goff;  // M0 block of tube t
-  kaos_ambipolar(T.nq, T.south, T.eamb, T.eambsign, T.filter_pt, T.qs, hq + T.goff, dq + T.goff,
-                 Te + T.goff, M0, M0F + T.goff, EA + T.goff);
-  if (rotate_esc) { esc_acc[2 * t] = 0ull; esc_acc[2 * t + 1] = 0ull; }
+  double* nF = M0F + T.goff;
+  double* E = EA + T.goff;
+  for (int Q = threadIdx.x; Q < T.nq; Q += blockDim.x)
+    nF[Q] = kaos_moment_filter_cell(M0, T.nq, T.filter_pt, Q + 1);
+  __syncthreads();
+  for (int Q = threadIdx.x; Q < T.nq; Q += blockDim.x)
+    E[Q] = kaos_ea_cell(Q, T.nq, T.south, T.eamb, T.eambsign, T.qs, hq + T.goff, dq + T.goff,
+                        Te + T.goff, M0, nF);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    kaos_ea_override(E, M0, T.eamb);
+    if (rotate_esc) { esc_acc[2 * t] = 0ull; esc_acc[2 * t + 1] = 0ull; }
+  }
 }
 
 // LB/UB injection after the master push (KSB:195-793): ion k of tube t goes
@@ -317,11 +320,9 @@ __global__ void k_inject(const DevTube* tubes, const long long* inj_base, int nt
   else kaos_inject_uniforms(seed, step, t, k, u);
   int qcell = kaos_find_qcell(qc, qGH + T.goff, T.qGL0, T.qGHlast, T.nq, T.desc);
   double r = kaos_rs_of_q(qc, T.p);
-  double x, y, z;
-  kaos_pos_from_rs(r, T.p, T.cphi, T.sphi, &x, &y, &z);
-  kaos_geom_t g = kaos_geom(x, y, z, T.cphi, T.sphi);
+  kaos_line_t L = kaos_line(r, T.p);
   double vpar, v1, v2, vperp;
-  kaos_mb_project(u, lb ? T.sig_lb : T.sig_ub, T.cphi, T.sphi, g, &vpar, &v1, &v2, &vperp);
+  kaos_mb_project(u, lb ? T.sig_lb : T.sig_ub, T.cphi, T.sphi, L, &vpar, &v1, &v2, &vperp);
   int ip = kaos_find_vcell(vpar, vparGH + T.voff, vparGH[T.voff + T.nvpar], T.nvpar);
   int iq = kaos_find_vcell(vperp, vperpGH + T.woff, vperpGH[T.woff + T.nvperp], T.nvperp);
   rs[j] = r; vp[j] = vpar; vr[j] = vperp; qk[j] = qc;
@@ -402,12 +403,7 @@ __global__ void k_reconstruct(DevTube T, long long n, const double* rs, const do
   long long j = T.off + i;
   double r = rs[j];
   double x = 0, y = 0, z = 0, vx = 0, vy = 0, vz = 0;
-  if (r != 0.0) {
-    kaos_pos_from_rs(r, T.p, T.cphi, T.sphi, &x, &y, &z);
-    kaos_geom_t g = kaos_geom(x, y, z, T.cphi, T.sphi);
-    double v = vp[j];
-    vx = v * g.bx; vy = v * g.by; vz = v * g.bz;
-  }
+  if (r != 0.0) kaos_cartesian(r, vp[j], T.p, T.cphi, T.sphi, &x, &y, &z, &vx, &vy, &vz);
   out6[6 * i + 0] = x; out6[6 * i + 1] = y; out6[6 * i + 2] = z;
   out6[6 * i + 3] = vx; out6[6 * i + 4] = vy; out6[6 * i + 5] = vz;
 }
@@ -734,7 +730,7 @@ static int enqueue_step(kaos_ctx* c, long long n, const double* d_uni) {
     zero_hist();
     hipLaunchKernelGGL(k_bin_initial, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a);
     // field of master step 1 from "master step 0" (no moments: M = 0), then moments of step 1
-    hipLaunchKernelGGL(k_field, dim3(nt), dim3(64), 0, c->stream, c->d_tubes, c->d_hq, c->d_dq, c->d_Te,
+    hipLaunchKernelGGL(k_field, dim3(nt), dim3(256), 0, c->stream, c->d_tubes, c->d_hq, c->d_dq, c->d_Te,
                        c->d_M, c->d_M0F, c->d_EA, c->d_esc_acc, c->d_esc_rep, 0);
     hipLaunchKernelGGL(k_moments, dim3(c->nblk_mom), dim3(KAOS_BLOCK), 0, c->stream, c->d_tubes, c->d_blk_tube,
                        c->d_blk_q, c->d_d3x, c->d_d3v, c->d_dvpar, c->d_dvperp, c->d_vparGC, c->d_vperpGC,
@@ -742,7 +738,7 @@ static int enqueue_step(kaos_ctx* c, long long n, const double* d_uni) {
     launch_push(c, a);
   } else {
     if (master) {
-      hipLaunchKernelGGL(k_field, dim3(nt), dim3(64), 0, c->stream, c->d_tubes, c->d_hq, c->d_dq, c->d_Te,
+      hipLaunchKernelGGL(k_field, dim3(nt), dim3(256), 0, c->stream, c->d_tubes, c->d_hq, c->d_dq, c->d_Te,
                          c->d_M, c->d_M0F, c->d_EA, c->d_esc_acc, c->d_esc_rep, 1);
       zero_hist();
     }

@@ -58,7 +58,15 @@ def load():
         lib.orc_scan_vcell.argtypes = [_dp, _dp, C.c_int, C.c_double, C.c_int]
         lib.orc_scan_vcell.restype = C.c_int
         lib.orc_moments_from_counts.argtypes = [C.POINTER(OrcCfg), _dp, _dp, _dp, _dp]
-        lib.orc_push_ions.argtypes = [C.POINTER(OrcCfg), C.c_int64] + [_dp] * 8 + [i32p, _dp]
+        lib.orc_push_lines.argtypes = [C.POINTER(OrcCfg), C.c_int64, _dp, _dp, _dp, _dp, _dp]
+        lib.orc_cartesian.argtypes = [C.POINTER(OrcCfg), C.c_int64, _dp, _dp, _dp]
+        lib.orc_get_lines.argtypes = [P, _dp, _dp]
+        lib.orc_find_qcell_from.argtypes = [C.POINTER(OrcCfg), C.c_double, C.c_int]
+        lib.orc_find_qcell_from.restype = C.c_int
+        lib.orc_find_vcell_guess.argtypes = [_dp, _dp, C.c_int, C.c_double]
+        lib.orc_find_vcell_guess.restype = C.c_int
+        lib.orc_interp2_from.argtypes = [C.POINTER(OrcCfg), _dp, C.c_double, C.c_int, _dp]
+        lib.orc_interp2_from.restype = C.c_int
         for n in ("orc_root_r",):
             getattr(lib, n).argtypes = [C.c_double, C.c_double]
             getattr(lib, n).restype = C.c_double
@@ -74,8 +82,7 @@ def load():
         lib.orc_interp_cells.restype = C.c_int
         lib.orc_interp_scan.argtypes = [C.POINTER(OrcCfg), _dp, C.c_double, C.c_double, C.c_int]
         lib.orc_interp_scan.restype = C.c_double
-        lib.orc_mb.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
-                               C.c_double, _dp]
+        lib.orc_mb.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, _dp]
         lib.orc_gauss.argtypes = [C.c_double, C.c_double]
         lib.orc_gauss.restype = C.c_double
         u64p = C.POINTER(C.c_uint64)
@@ -136,6 +143,12 @@ class Oracle:
                                    *(ptr(i[k], C.c_int) for k in ("qind", "vparind", "vperpind")))
         d.update(i)
         return d
+
+    def lines(self):
+        n = self.count()
+        rs, vs = np.zeros(n), np.zeros(n)
+        self.lib.orc_get_lines(self.h, ptr(rs), ptr(vs))
+        return rs, vs
 
     def master(self):
         t = self.tube

@@ -5,9 +5,12 @@
  * checker of the HIP path.  Nothing in the shipped library links or calls it.
  *
  * It follows the reference's call sequence and data layout:
- *   - Cartesian per-particle state x,y,z,Vx,Vy,Vz,Vperp plus the reference's
- *     per-particle fields AEAmag/AGmag (current) and AEAmagN/AGmagN (next),
- *     qk4 and the integer cell indices Qindk1/Vparindk1/Vperpindk1;
+ *   - per-ion state: the reference's Cartesian ion sits on its field line
+ *     with V = V|| b-hat after every push (_8_1_KineticRK4Update.f90:170-314),
+ *     so it is held as (signed radius rs, V||) and expanded to x,y,z,Vx,Vy,Vz
+ *     on output; plus Vperp, the reference's per-ion fields AEAmag/AGmag
+ *     (current) and AEAmagN/AGmagN (next), qk4 and the integer cell indices
+ *     Qindk1/Vparindk1/Vperpindk1;
  *   - KineticSolverSub step order (_8_KineticSolver.f90:54-239):
  *     BoundaryConditionsSub -> [master] AmbipolarEfieldSub/GravfieldSub ->
  *     KineticSolverA/BSub (rotation, push loop, injection) ->
@@ -49,7 +52,7 @@ typedef struct {
 typedef struct {
   orc_cfg c;
   int64_t ns, cap;
-  double *x, *y, *z, *Vx, *Vy, *Vz, *vperp, *aea, *ag, *aeaN, *agN, *qk4, *q0;
+  double *rs, *vs, *vperp, *aea, *ag, *aeaN, *agN, *qk4, *q0, *vpar0;
   int *qind, *vparind, *vperpind;
   int master_last;                 /* last step was a master step */
   int64_t nn;                      /* index of the last master step */
@@ -190,7 +193,7 @@ static void counts_and_moments(orc_t *o, const int *qi, const int *vpi, const in
 
 /* ---- state management ------------------------------------------------------ */
 
-#define ARR_D(X) X(x) X(y) X(z) X(Vx) X(Vy) X(Vz) X(vperp) X(aea) X(ag) X(aeaN) X(agN) X(qk4) X(q0)
+#define ARR_D(X) X(rs) X(vs) X(vperp) X(aea) X(ag) X(aeaN) X(agN) X(qk4) X(q0) X(vpar0)
 #define ARR_I(X) X(qind) X(vparind) X(vperpind)
 
 void *orc_create(const orc_cfg *cfg, int64_t ns, const double *q0, const double *vpar0,
@@ -218,13 +221,11 @@ void *orc_create(const orc_cfg *cfg, int64_t ns, const double *q0, const double 
   o->M0F = (double *)xc(sizeof(double) * nq);
   /* initial conditions: x0 = root(q0), V0 = Vpar0 b-hat(x0) */
   for (int64_t j = 0; j < ns; ++j) {
-    double rs = kaos_rs_of_q(q0[j], cfg->p);
-    kaos_pos_from_rs(rs, cfg->p, cfg->cphi, cfg->sphi, &o->x[j], &o->y[j], &o->z[j]);
-    kaos_geom_t g = kaos_geom(o->x[j], o->y[j], o->z[j], cfg->cphi, cfg->sphi);
-    o->Vx[j] = vpar0[j] * g.bx; o->Vy[j] = vpar0[j] * g.by; o->Vz[j] = vpar0[j] * g.bz;
+    o->rs[j] = kaos_rs_of_q(q0[j], cfg->p);
+    o->vs[j] = vpar0[j];
     o->vperp[j] = vperp0[j];
     o->q0[j] = q0[j];
-    o->qk4[j] = vpar0[j];   /* scratch: Vpar0T for the k0 velocity index */
+    o->vpar0[j] = vpar0[j];
   }
   return o;
 }
@@ -279,14 +280,13 @@ static int inject(orc_t *o, int64_t n, const double *uni) {
     if (uni) memcpy(u, uni + 6 * k, sizeof(u));
     else kaos_inject_uniforms(c->seed, n, c->tube, k, u);
     o->qind[j] = scan_qcell(c, qc, 0);
-    double rs = kaos_rs_of_q(qc, c->p);
-    kaos_pos_from_rs(rs, c->p, c->cphi, c->sphi, &o->x[j], &o->y[j], &o->z[j]);
-    kaos_geom_t g = kaos_geom(o->x[j], o->y[j], o->z[j], c->cphi, c->sphi);
+    o->rs[j] = kaos_rs_of_q(qc, c->p);
+    kaos_line_t L = kaos_line(o->rs[j], c->p);
     double Ts = lb ? c->Ts_lb : c->Ts_ub;
     double sigma = sqrt((KAOS_KB * Ts) / c->ms);
     double vpar, v1, v2, vperp;
-    kaos_mb_project(u, sigma, c->cphi, c->sphi, g, &vpar, &v1, &v2, &vperp);
-    o->Vx[j] = vpar * g.bx; o->Vy[j] = vpar * g.by; o->Vz[j] = vpar * g.bz;
+    kaos_mb_project(u, sigma, c->cphi, c->sphi, L, &vpar, &v1, &v2, &vperp);
+    o->vs[j] = vpar;
     o->vperp[j] = vperp;
     o->vparind[j] = scan_vcell(c->vparGL, c->vparGH, c->nvpar, vpar, 0);
     o->vperpind[j] = scan_vcell(c->vperpGL, c->vperpGH, c->nvperp, vperp, 0);
@@ -302,21 +302,18 @@ static int inject(orc_t *o, int64_t n, const double *uni) {
 static void push_all(orc_t *o) {
   orc_cfg *c = &o->c;
   kaos_push_par_t P = { c->h, c->p, c->cphi, c->sphi, c->grav, c->eamb, c->eambsign };
-  int desc = (c->qGL[0] > 0);
   for (int64_t j = 0; j < o->ns; ++j) {
+    kaos_line_t L = kaos_line(o->rs[j], c->p);
+    kaos_qb_t gA = kaos_stageA(L);
+    int qs = scan_qcell(c, gA.q, o->qind[j]);    /* the reference's linear scan */
     kaos_push_out_t r;
-    int qc;
-    kaos_push_ion(o->x[j], o->y[j], o->z[j], o->Vx[j], o->Vy[j], o->Vz[j], o->ag[j], o->aea[j], &P,
-                  c->qGH, c->qGL[0], c->qGH[c->nq - 1], c->nq, desc, &r, &qc);
-    /* the oracle re-derives the config cell with the reference's linear scan */
-    int qs = scan_qcell(c, r.qk1, o->qind[j]);
+    kaos_push_ion(L, gA, qs, o->vs[j], o->ag[j], o->aea[j], &P, &r);
     o->qind[j] = qs;
     o->vparind[j] = scan_vcell(c->vparGL, c->vparGH, c->nvpar, r.vpar, o->vparind[j]);
     o->vperpind[j] = scan_vcell(c->vperpGL, c->vperpGH, c->nvperp, o->vperp[j], o->vperpind[j]);
-    o->x[j] = r.xN; o->y[j] = r.yN; o->z[j] = r.zN;
-    o->Vx[j] = r.VxN; o->Vy[j] = r.VyN; o->Vz[j] = r.VzN;
+    o->rs[j] = r.rsN;
+    o->vs[j] = r.vparN;
     o->qk4[j] = r.qk4;
-    (void)qc;
   }
 }
 
@@ -340,7 +337,7 @@ int orc_step(void *h, int64_t n, const double *uniforms) {
       }
       o->q0[j] = q;
       o->qind[j] = scan_qcell(c, q, 0);
-      o->vparind[j] = scan_vcell(c->vparGL, c->vparGH, c->nvpar, o->qk4[j], 0);
+      o->vparind[j] = scan_vcell(c->vparGL, c->vparGH, c->nvpar, o->vpar0[j], 0);
       o->vperpind[j] = scan_vcell(c->vperpGL, c->vperpGH, c->nvperp, o->vperp[j], 0);
     }
     counts_and_moments(o, o->qind, o->vparind, o->vperpind, o->ns);
@@ -386,13 +383,18 @@ void orc_get_particles(void *h, double *x, double *y, double *z, double *Vx, dou
                        double *Vz, double *vperp, double *qk4, double *aea, double *ag,
                        int *qind, int *vparind, int *vperpind) {
   orc_t *o = (orc_t *)h;
+  orc_cfg *c = &o->c;
   size_t nd = sizeof(double) * o->ns, ni = sizeof(int) * o->ns;
-  if (x) memcpy(x, o->x, nd);
-  if (y) memcpy(y, o->y, nd);
-  if (z) memcpy(z, o->z, nd);
-  if (Vx) memcpy(Vx, o->Vx, nd);
-  if (Vy) memcpy(Vy, o->Vy, nd);
-  if (Vz) memcpy(Vz, o->Vz, nd);
+  for (int64_t j = 0; j < o->ns; ++j) {
+    double a[6];
+    kaos_cartesian(o->rs[j], o->vs[j], c->p, c->cphi, c->sphi, &a[0], &a[1], &a[2], &a[3], &a[4], &a[5]);
+    if (x) x[j] = a[0];
+    if (y) y[j] = a[1];
+    if (z) z[j] = a[2];
+    if (Vx) Vx[j] = a[3];
+    if (Vy) Vy[j] = a[4];
+    if (Vz) Vz[j] = a[5];
+  }
   if (vperp) memcpy(vperp, o->vperp, nd);
   if (qk4) memcpy(qk4, o->qk4, nd);
   if (aea) memcpy(aea, o->aeaN, nd);
@@ -400,6 +402,13 @@ void orc_get_particles(void *h, double *x, double *y, double *z, double *Vx, dou
   if (qind) memcpy(qind, o->qind, ni);
   if (vparind) memcpy(vparind, o->vparind, ni);
   if (vperpind) memcpy(vperpind, o->vperpind, ni);
+}
+
+/* Line state (signed radius, V||) of the live ions. */
+void orc_get_lines(void *h, double *rs, double *vs) {
+  orc_t *o = (orc_t *)h;
+  memcpy(rs, o->rs, sizeof(double) * o->ns);
+  memcpy(vs, o->vs, sizeof(double) * o->ns);
 }
 
 /* Outputs of the last master step. esc[0..3] = LB count, UB count, LB flux, UB flux. */
@@ -440,9 +449,9 @@ int orc_interp_cells(const orc_cfg *c, const double *E, double q, double *out) {
 double orc_interp_scan(const orc_cfg *c, const double *E, double q, double init, int kind) {
   double v = init; interp_scan(c, E, q, &v, kind); return v;
 }
-void orc_mb(const double *u, double sigma, double cphi, double sphi, double x, double y, double z, double *o4) {
-  kaos_geom_t g = kaos_geom(x, y, z, cphi, sphi);
-  kaos_mb_project(u, sigma, cphi, sphi, g, &o4[0], &o4[1], &o4[2], &o4[3]);
+void orc_mb(const double *u, double sigma, double cphi, double sphi, double rs, double p, double *o4) {
+  kaos_line_t L = kaos_line(rs, p);
+  kaos_mb_project(u, sigma, cphi, sphi, L, &o4[0], &o4[1], &o4[2], &o4[3]);
 }
 double orc_gauss(double u1, double u2) { return kaos_gauss(u1, u2); }
 int orc_scan_qcell(const orc_cfg *c, double q) { return scan_qcell(c, q, 0); }
@@ -480,21 +489,36 @@ void orc_moments_from_counts(const orc_cfg *cfg, const double *nq_counts, const 
   counts_and_moments(&o, qi, vp, vr, k);
   free(qi); free(vp); free(vr); free(o.nq_counts); free(o.nph_counts); free(o.nq_u); free(o.nph_u);
 }
-/* One push of given Cartesian ions (no cell bookkeeping): for the tests. */
-void orc_push_ions(const orc_cfg *c, int64_t n, const double *x, const double *y, const double *z,
-                   const double *Vx, const double *Vy, const double *Vz, const double *ag,
-                   const double *aea, const int *qvalid, double *out /* n x 11 */) {
+/* One push of given line states (no cell bookkeeping), for the tests:
+ * out[j] = (xN, yN, zN, VxN, VyN, VzN, rsN, VparN, Vpar, qk1, qk4). */
+void orc_push_lines(const orc_cfg *c, int64_t n, const double *rs, const double *vs,
+                    const double *ag, const double *aea, double *out) {
   kaos_push_par_t P = { c->h, c->p, c->cphi, c->sphi, c->grav, c->eamb, c->eambsign };
-  int desc = (c->qGL[0] > 0);
   for (int64_t j = 0; j < n; ++j) {
-    kaos_push_out_t r; int qc;
-    kaos_push_ion(x[j], y[j], z[j], Vx[j], Vy[j], Vz[j], ag[j], aea[j], &P,
-                  c->qGH, c->qGL[0], c->qGH[c->nq - 1], c->nq, desc, &r, &qc);
-    (void)qvalid;
+    kaos_line_t L = kaos_line(rs[j], c->p);
+    kaos_qb_t gA = kaos_stageA(L);
+    int qs = scan_qcell(c, gA.q, 0);
+    kaos_push_out_t r;
+    kaos_push_ion(L, gA, qs, vs[j], ag[j], aea[j], &P, &r);
     double *w = out + 11 * j;
-    w[0] = r.xN; w[1] = r.yN; w[2] = r.zN; w[3] = r.VxN; w[4] = r.VyN; w[5] = r.VzN;
+    kaos_cartesian(r.rsN, r.vparN, c->p, c->cphi, c->sphi, &w[0], &w[1], &w[2], &w[3], &w[4], &w[5]);
     w[6] = r.rsN; w[7] = r.vparN; w[8] = r.vpar; w[9] = r.qk1; w[10] = r.qk4;
   }
+}
+void orc_cartesian(const orc_cfg *c, int64_t n, const double *rs, const double *vs, double *out6) {
+  for (int64_t j = 0; j < n; ++j)
+    kaos_cartesian(rs[j], vs[j], c->p, c->cphi, c->sphi, &out6[6 * j], &out6[6 * j + 1], &out6[6 * j + 2],
+                   &out6[6 * j + 3], &out6[6 * j + 4], &out6[6 * j + 5]);
+}
+int orc_find_qcell_from(const orc_cfg *c, double q, int hint) {
+  return kaos_find_qcell_from(q, c->qGH, c->qGL[0], c->qGH[c->nq - 1], c->nq, c->qGL[0] > 0, hint);
+}
+int orc_find_vcell_guess(const double *GL, const double *GH, int n, double v) {
+  return kaos_find_vcell_guess(v, GH, GL[0], n, (double)n / (GH[n - 1] - GL[0]));
+}
+int orc_interp2_from(const orc_cfg *c, const double *E, double q, int hint, double *out) {
+  return kaos_interp2_from(q, c->qGC, E, (const double *)0, c->nq, c->qGL[0], c->qGH[c->nq - 1],
+                           c->qGL[0] > 0, hint, out, (double *)0);
 }
 double orc_root_r(double q, double p) { return kaos_root_r(q, p); }
 double orc_sinh_asinh_third(double y) { return kaos_sinh_asinh_third(y); }

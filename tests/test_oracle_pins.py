@@ -140,9 +140,7 @@ def test_mb_projection_is_a_rotation(lib, tube):
         u = 1.0 - rng.random(6)
         q = rng.uniform(tube.arrays["qGH"][-1], tube.arrays["qGL"][0])
         r = lib.orc_root_r(q, tube.p)
-        th = math.asin(math.sqrt(r / (RE * tube.p)))
-        x, y, z = r * math.sin(th) * tube.cphi, r * math.sin(th) * tube.sphi, r * math.cos(th)
-        lib.orc_mb(ptr(u), 1000.0, tube.cphi, tube.sphi, x, y, z, ptr(o4))
+        lib.orc_mb(ptr(u), 1000.0, tube.cphi, tube.sphi, r, tube.p, ptr(o4))
         gx, gy, gz = (1000.0 * lib.orc_gauss(u[2 * i], u[2 * i + 1]) for i in range(3))
         vpar, v1, v2, vperp = o4
         assert abs(vpar ** 2 + vperp ** 2 - (gx * gx + gy * gy + gz * gz)) <= 1e-9 * (gx * gx + gy * gy + gz * gz)
@@ -162,12 +160,12 @@ def test_push_matches_reference_fortran(lib):
     for n in range(1, 5):
         o.step(n)
     P = o.particles()
+    rs, vs = o.lines()
     r = rp.run(t, P["x"], P["y"], P["z"], P["Vx"], P["Vy"], P["Vz"], P["vperp"], P["aea"], P["ag"],
                P["qind"], P["vparind"], P["vperpind"])
     n = len(P["x"])
     out = np.zeros(11 * n)
-    lib.orc_push_ions(C.byref(o.cfg), n, *(ptr(P[k]) for k in ("x", "y", "z", "Vx", "Vy", "Vz", "ag", "aea")),
-                      ptr(np.ones(n, np.int32), C.c_int), ptr(out))
+    lib.orc_push_lines(C.byref(o.cfg), n, ptr(rs), ptr(vs), ptr(P["ag"]), ptr(P["aea"]), ptr(out))
     out = out.reshape(n, 11)
     # Ions whose new cell is an escape code take stage-4 gravity from the previous
     # ion in the reference (force registers not reset, _8_1_3_KineticUpdateC.f90:345-421);
@@ -189,6 +187,31 @@ def test_push_matches_reference_fortran(lib):
 
 
 # --- cell searches -------------------------------------------------------------
+
+@pytest.mark.parametrize("south", [False, True])
+def test_hinted_searches_equal_bisection(lib, south):
+    """The kernels start their cell searches at the ion's previous cell; the
+    result must not depend on the hint."""
+    t = grids.dipole_tube(nq=60, nvpar=33, nvperp=17, n_total=1000, south=south, ratio=1.07)
+    cfg = oracle.cfg_of(t)
+    a = t.arrays
+    rng = np.random.default_rng(11)
+    lo, hi = min(a["qGH"].min(), a["qGL"].min()), max(a["qGH"].max(), a["qGL"].max())
+    qs = np.concatenate([a["qGL"], a["qGH"], a["qGC"], rng.uniform(lo - 0.03, hi + 0.03, 1500)])
+    E = rng.normal(size=t.nq)
+    o1, o2 = np.zeros(1), np.zeros(1)
+    for q in qs:
+        ref = lib.orc_find_qcell(C.byref(cfg), q)
+        h1 = lib.orc_interp_cells(C.byref(cfg), ptr(E), q, ptr(o1))
+        for hint in (-1, 0, 1, 7, 30, 59, 60, 61):
+            assert lib.orc_find_qcell_from(C.byref(cfg), q, hint) == ref
+            h2 = lib.orc_interp2_from(C.byref(cfg), ptr(E), q, hint, ptr(o2))
+            assert h1 == h2 and (not h1 or o1[0] == o2[0])
+    GL, GH = a["vparGL"], a["vparGH"]
+    for v in np.concatenate([GL, GH, rng.uniform(-1.3e4, 1.3e4, 3000), [np.nan]]):
+        assert lib.orc_find_vcell_guess(ptr(GL), ptr(GH), t.nvpar, v) == \
+            lib.orc_find_vcell(ptr(GL), ptr(GH), t.nvpar, v)
+
 
 @pytest.mark.parametrize("south", [False, True])
 def test_binary_cell_search_equals_reference_scan(lib, south):
