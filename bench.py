@@ -58,6 +58,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--ions", type=int, default=100_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--form", choices=("fp64", "mixed"), default="fp64")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -65,14 +66,15 @@ def main():
         sys.exit(2)
 
     from kaos_amd import grids
-    from kaos_amd.solver import KineticSolver, KAOS_FP64
+    from kaos_amd.solver import KineticSolver, KAOS_FP64, KAOS_MIXED
     t0 = time.perf_counter()
     tube = grids.config2(n_total=args.ions)
     q0, v0, w0 = grids.initial_ions(tube, seed=1)
     n_steps_total = args.warmup + args.steps + 1
     cap = len(q0) + (n_steps_total + 2) * (tube.n_inj_lb + tube.n_inj_ub) + 1024
     log(f"[bench] ions {len(q0)}, inj/step {tube.n_inj_lb}+{tube.n_inj_ub}, setup {time.perf_counter()-t0:.1f}s")
-    ks = KineticSolver([tube], precision=KAOS_FP64, ndatfac=1, seed=1, capacity=[cap])
+    prec = KAOS_MIXED if args.form == "mixed" else KAOS_FP64
+    ks = KineticSolver([tube], precision=prec, ndatfac=1, seed=1, capacity=[cap])
     ks.upload(0, q0, v0, w0)
     del q0, v0, w0
     log(f"[bench] uploaded, {time.perf_counter()-t0:.1f}s")

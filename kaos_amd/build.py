@@ -13,12 +13,14 @@ ARCH = "gfx950"
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-def build_lib(verbose=False):
+def build_lib(verbose=False, out=None, defines=()):
+    """libkaos_hip.so (or `out`), with optional -D compile-time variants."""
     src = os.path.join(HERE, "csrc", "kaos_hip.hip")
-    out = os.path.join(HERE, "libkaos_hip.so")
+    out = out or os.path.join(HERE, "libkaos_hip.so")
     tmp = out + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-O3", "-fPIC", "-shared", "-std=c++17",
            "-ffp-contract=off", *DEVICE_FLAGS, "-Wall", "-Wno-unused-function",
+           *(f"-D{d}" for d in defines),
            "-I", os.path.join(HERE, "..", "include"), src, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -248,11 +248,22 @@ KAOS_HD int kaos_interp_cells(double q, const double* qGC, const double* E, int 
 typedef struct {
   double h;            /* time step SpecieT%hT */
   double p;            /* L-shell pGCGT */
+  double ip;           /* 1/p */
+  double irep;         /* 1/(RE p) */
   double cphi, sphi;   /* cos/sin of the tube longitude phiGCGT */
   int grav;            /* GRAVflagT */
   int eamb;            /* EAMBflagT */
   int eambsign;        /* EAMBSIGNflagT */
 } kaos_push_par_t;
+
+KAOS_HD kaos_push_par_t kaos_push_par(double h, double p, double cphi, double sphi, int grav,
+                                      int eamb, int eambsign) {
+  kaos_push_par_t P;
+  P.h = h; P.p = p; P.ip = 1.0 / p; P.irep = 1.0 / (KAOS_RE * p);
+  P.cphi = cphi; P.sphi = sphi;
+  P.grav = grav; P.eamb = eamb; P.eambsign = eambsign;
+  return P;
+}
 
 typedef struct {
   double rsN;            /* signed radius of the new point                 */
@@ -262,21 +273,50 @@ typedef struct {
   double qk4;            /* q at the full-step predictor (field coordinate) */
 } kaos_push_out_t;
 
-/* Point of the field line with signed radius rs: r, cos, sin and (rho, z)
- * with the reference's zeroing of tiny coordinates (_8_1_4:111-125).       */
-typedef struct { double r, c, s, rho, z; } kaos_line_t;
+/* Point of the field line with signed radius rs: r, w = r/(RE p) = sin^2,
+ * cos, sin and (rho, z), with the reference's zeroing of tiny coordinates
+ * (_8_1_4_RK4DipolePolynomialSolver.f90:111-125).                         */
+typedef struct { double r, w, c, s, rho, z; } kaos_line_t;
 
-KAOS_HD kaos_line_t kaos_line(double rs, double p) {
+KAOS_HD kaos_line_t kaos_line_i(double rs, double irep) {
   kaos_line_t L;
   L.r = fabs(rs);
-  double w = L.r * (1.0 / (KAOS_RE * p));
-  L.s = sqrt(w);
-  L.c = sqrt(1.0 - w);
+  L.w = L.r * irep;
+  L.s = sqrt(L.w);
+  L.c = sqrt(1.0 - L.w);
   if (rs < 0.0) L.c = -L.c;
   L.rho = L.r * L.s;
   L.z = L.r * L.c;
   if (fabs(L.z) < KAOS_ZERO_CUT) L.z = 0.0;
   return L;
+}
+
+KAOS_HD kaos_line_t kaos_line(double rs, double p) { return kaos_line_i(rs, 1.0 / (KAOS_RE * p)); }
+
+/* q and b-hat (rho, z components) at a point of the meridional plane. */
+typedef struct { double q, bR, bZ; } kaos_qb_t;
+
+/* On the line: b-hat = (3 c s, 3 c^2 - 1)/sqrt(1 + 3 c^2) with c^2 = 1 - w
+ * (_KineticMainParams.f90:614, 674-779), q = RE^2 c / r^2 (:593).           */
+KAOS_HD kaos_qb_t kaos_stageA(kaos_line_t L) {
+  kaos_qb_t g;
+  g.q = (KAOS_RE2 * L.c) / (L.r * L.r);
+  double isl = 1.0 / sqrt(4.0 - 3.0 * L.w);
+  g.bR = (3.0 * (L.c * L.s)) * isl;
+  g.bZ = (2.0 - 3.0 * L.w) * isl;
+  return g;
+}
+
+/* Off the line, at (rho, z) with u2 = rho^2 + z^2 = r^2: c = z/r, s = rho/r,
+ * so b-hat = (3 z rho, 2 z^2 - rho^2) / sqrt(u2 (u2 + 3 z^2)): one square
+ * root and one division.                                                    */
+KAOS_HD void kaos_bhat_pt(double rho, double z, double* bR, double* bZ, double* u2out) {
+  double z2 = z * z;
+  double u2 = rho * rho + z2;
+  double d = 1.0 / sqrt(u2 * (u2 + 3.0 * z2));
+  *bR = (3.0 * (z * rho)) * d;
+  *bZ = (2.0 * z2 - rho * rho) * d;
+  *u2out = u2;
 }
 
 /* Cartesian point and velocity V|| b-hat of a line state (for output). */
@@ -287,27 +327,12 @@ KAOS_HD void kaos_cartesian(double rs, double vs, double p, double cphi, double 
   if (fabs(xx) < KAOS_ZERO_CUT) xx = 0.0;
   if (fabs(yy) < KAOS_ZERO_CUT) yy = 0.0;
   *x = xx; *y = yy; *z = L.z;
-  double isl = 1.0 / sqrt(1.0 + 3.0 * (L.c * L.c));
-  double bR = ((3.0 * L.c) * L.s) * isl, bZ = (3.0 * (L.c * L.c) - 1.0) * isl;
-  *Vx = (vs * bR) * cphi; *Vy = (vs * bR) * sphi; *Vz = vs * bZ;
+  kaos_qb_t g = kaos_stageA(L);
+  *Vx = (vs * g.bR) * cphi; *Vy = (vs * g.bR) * sphi; *Vz = vs * g.bZ;
 }
 
-/* q and b-hat (rho, z components) at a point of the meridional plane. */
-typedef struct { double q, bR, bZ; } kaos_qb_t;
-
-KAOS_HD kaos_qb_t kaos_qb(double rho, double z) {
-  kaos_qb_t g;
-  double r = sqrt(rho * rho + z * z);
-  double ir = 1.0 / r;
-  double c = z * ir, s = rho * ir;
-  g.q = (KAOS_RE2 * c) * (ir * ir);
-  double c2 = c * c;
-  double isl = 1.0 / sqrt(1.0 + 3.0 * c2);
-  g.bR = ((3.0 * c) * s) * isl;
-  g.bZ = (3.0 * c2 - 1.0) * isl;
-  return g;
-}
-
+/* Parallel acceleration of one stage; only the sign of q enters
+ * (sign conventions of _8_1_1_KineticUpdateA.f90:480-484, 649-658).       */
 KAOS_HD double kaos_apar(double q, double AG, double AEA, const kaos_push_par_t* P, int with_grav) {
   double a = 0.0;
   if (P->grav && with_grav) a = ((q <= 0.0) ? 1.0 : -1.0) * fabs(AG);        /* AGmagS, KUA:480-484 */
@@ -318,14 +343,25 @@ KAOS_HD double kaos_apar(double q, double AG, double AEA, const kaos_push_par_t*
   return a;
 }
 
-/* Stage A of the input state rs (KineticUpdateASub): q and b-hat. */
-KAOS_HD kaos_qb_t kaos_stageA(kaos_line_t L) {
-  kaos_qb_t g;
-  g.q = (KAOS_RE2 * L.c) / (L.r * L.r);
-  double c2 = L.c * L.c;
-  double isl = 1.0 / sqrt(1.0 + 3.0 * c2);
-  g.bR = ((3.0 * L.c) * L.s) * isl;
-  g.bZ = (3.0 * c2 - 1.0) * isl;
+/* Positive root of the dipole quartic (_8_1_4_RK4DipolePolynomialSolver.f90:30-33,
+ * gamma^4 + gamma/(p q^2) - 1/q^2 = 0) for the point (R, Z) of the meridional
+ * plane, q^2 = RE^4 Z^2 / (R^2 + Z^2)^3 = A / B.  Multiplied by B q^2 the
+ * quartic is f(g) = A g^4 + B (g/p - 1); f is increasing and convex for
+ * g > 0, so Newton's method converges monotonically from any g0 > 0,
+ * quadratically with relative error e -> <= 1.5 e^2, one division per step.
+ * Steps continue while the last correction exceeds 1e-9 g; from a start
+ * within 1e-9 of the root one step reaches it to rounding (1.5e-18).       */
+KAOS_HD double kaos_quartic_newton(double A, double B, double ip, double g0) {
+  double g = g0, dg;
+  int it = 0;
+  do {
+    double g2 = g * g;
+    double f = (A * g2) * g2 + B * (g * ip - 1.0);
+    double fp = (4.0 * A) * (g2 * g) + B * ip;
+    dg = f / fp;
+    g = g - dg;
+    ++it;
+  } while (fabs(dg) > 1e-9 * g && it < 64);
   return g;
 }
 
@@ -339,14 +375,19 @@ KAOS_HD void kaos_push_ion(kaos_line_t L, kaos_qb_t gA, int qc, double vs, doubl
   /* V = V|| b-hat; Vpar(j) = b-hat . V (KUA:978-980), VR = Vpar b-hat (KUA:1327-1334) */
   double vpar = gA.bR * (vs * gA.bR) + gA.bZ * (vs * gA.bZ);
   double VRR = vpar * gA.bR, VRZ = vpar * gA.bZ;
-  /* half-step position and stage B (KineticUpdateBSub) */
-  kaos_qb_t g2 = kaos_qb(L.rho + h2 * VRR, L.z + h2 * VRZ);
-  double ap2 = kaos_apar(g2.q, AG, AEA, P, 1);
-  double a2R = ap2 * g2.bR, a2Z = ap2 * g2.bZ;
-  /* full-step predictor and stage C (KineticUpdateCSub) */
-  kaos_qb_t g4 = kaos_qb((L.rho + h * VRR) + hsq2 * a1R, (L.z + h * VRZ) + hsq2 * a1Z);
-  double ap4 = kaos_apar(g4.q, AG, AEA, P, (qc != 0 && qc != -1));
-  double a4R = ap4 * g4.bR, a4Z = ap4 * g4.bZ;
+  /* half-step position and stage B (KineticUpdateBSub): sign(q) = sign(z) */
+  double b2R, b2Z, u2B;
+  double zB = L.z + h2 * VRZ;
+  kaos_bhat_pt(L.rho + h2 * VRR, zB, &b2R, &b2Z, &u2B);
+  double ap2 = kaos_apar(zB, AG, AEA, P, 1);
+  double a2R = ap2 * b2R, a2Z = ap2 * b2Z;
+  /* full-step predictor and stage C (KineticUpdateCSub): q4 = RE^2 z / r^3 */
+  double b4R, b4Z, u24;
+  double zC = (L.z + h * VRZ) + hsq2 * a1Z;
+  kaos_bhat_pt((L.rho + h * VRR) + hsq2 * a1R, zC, &b4R, &b4Z, &u24);
+  double q4 = (KAOS_RE2 * zC) / (u24 * sqrt(u24));
+  double ap4 = kaos_apar(q4, AG, AEA, P, (qc != 0 && qc != -1));
+  double a4R = ap4 * b4R, a4Z = ap4 * b4Z;
   /* velocity: k1 + 2 k2 + 2 k3 + k4 with k2 = k3 (_8_1_KineticRK4Update.f90:85-90) */
   double VNR = VRR + h6 * (((a1R + 2.0 * a2R) + 2.0 * a2R) + a4R);
   double VNZ = VRZ + h6 * (((a1Z + 2.0 * a2Z) + 2.0 * a2Z) + a4Z);
@@ -355,19 +396,24 @@ KAOS_HD void kaos_push_ion(kaos_line_t L, kaos_qb_t gA, int qc, double vs, doubl
   double k4R = VRR + h * a2R, k4Z = VRZ + h * a2Z;
   double RNp = L.rho + h6 * (((VRR + 2.0 * k2R) + 2.0 * k2R) + k4R);
   double ZNp = L.z + h6 * (((VRZ + 2.0 * k2Z) + 2.0 * k2Z) + k4Z);
-  /* re-project onto the L-shell: q of the new point, root solve */
-  double rN = sqrt(RNp * RNp + ZNp * ZNp);
-  double irN = 1.0 / rN;
-  double qNp = (KAOS_RE2 * (ZNp * irN)) * (irN * irN);
-  double rsN = kaos_rs_of_q(qNp, P->p);
-  /* re-align the velocity with b-hat at the new point (:273-314) */
-  kaos_line_t LN = kaos_line(rsN, P->p);
-  kaos_qb_t gN = kaos_stageA(LN);
+  /* re-project onto the L-shell (:161-176): the point of the line with the
+   * same q, hemisphere from sign(Z); Newton starts from the first-order
+   * radius r + r-hat . (dR, dZ) (relative error ~ |dx|^2 / r^2 ~ 1e-10) */
+  double Z2 = ZNp * ZNp, u2N = RNp * RNp + Z2;
+  double g0 = (L.r + (L.s * (RNp - L.rho) + L.c * (ZNp - L.z))) * (1.0 / KAOS_RE);
+  double rN = kaos_quartic_newton((KAOS_RE2 * KAOS_RE2) * Z2, (u2N * u2N) * u2N, P->ip, g0) * KAOS_RE;
+  double rsN = (ZNp <= 0.0) ? -rN : rN;
+  /* re-align the velocity with b-hat at the new point (:273-314):
+   * c s = sign * sqrt(w (1 - w)), 3 c^2 - 1 = 2 - 3 w */
+  double wN = rN * P->irep;
+  double csN = sqrt(wN * (1.0 - wN));
+  if (rsN < 0.0) csN = -csN;
+  double islN = 1.0 / sqrt(4.0 - 3.0 * wN);
   o->rsN = rsN;
-  o->vparN = gN.bR * VNR + gN.bZ * VNZ;
+  o->vparN = ((3.0 * csN) * islN) * VNR + ((2.0 - 3.0 * wN) * islN) * VNZ;
   o->vpar = vpar;
   o->qk1 = gA.q;
-  o->qk4 = g4.q;
+  o->qk4 = q4;
 }
 
 /* ------------------------------------------------------------------------- */

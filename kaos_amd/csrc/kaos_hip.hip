@@ -26,6 +26,7 @@
 #include <string.h>
 #include <string>
 #include <vector>
+#include <algorithm>
 
 #include "kaos_dipole.h"
 #include "../../include/kaos_hip.h"
@@ -35,7 +36,9 @@
 #define KAOS_TILE (KAOS_BLOCK * KAOS_PPT)
 #define KAOS_MAXQ 512
 #define KAOS_MAXV 255
+#ifndef KAOS_HWORDS
 #define KAOS_HWORDS 6144                     // LDS histogram window: 12288 16-bit counters
+#endif
 #define KAOS_TPB 16                          // tiles per push block (<= 63: 16-bit counters)
 
 struct DevTube {
@@ -46,6 +49,7 @@ struct DevTube {
   long long goff, voff, woff, hoff, doff;    // grid / vpar / vperp / histogram / d3v offsets
   double p, cphi, sphi, h, ms, qs, qsms, nsnormfac, sig_lb, sig_ub;
   double qGL0, qGHlast, qGL_last0;
+  double rGL0, rGHlast, rsgn;               // |r| of the outer q edges; sign of rs in the tube
 };
 
 struct StepArgs {
@@ -57,7 +61,12 @@ struct StepArgs {
   const double *qGH, *qGC, *EA, *EG;
   const double *vparGH, *vperpGH;
   // particles
-  double *rs, *vp, *vr, *qk, *ae, *ag;
+  double *rs, *vp, *vr, *qk, *ae, *ag;       // binary64 form
+  int *rdi, *vpi;                            // mixed form: fixed-point radius / V|| offsets
+  float *vrf, *qkf, *aef, *agf;              // mixed form: Vperp, stage-4 q offset, fields
+  const double *rGH, *rref;                  // |r| of the qGH edges; signed r of the cell centres
+  const int *rk, *vk;                        // fixed-point exponents per config / Vpar cell
+  int lds_nq, lds_nv;                        // largest nq / nvpar over the tubes (LDS sizing)
   uint32_t* cell;
   const long long* count;
   unsigned int *hist, *nqoff;               // phase-space counts; config counts off the v-grid
@@ -69,6 +78,8 @@ struct StepArgs {
 
 __device__ __forceinline__ void set_status(unsigned int* st, unsigned int bit) { atomicOr(st, bit); }
 
+#include "kaos_mixed.h"
+
 // ---------------------------------------------------------------------------
 // Push + bin (FP64 form).  One RK4 step per live ion (KineticRK4UpdateSub,
 // _8_1_KineticRK4Update.f90:29-314) with, on master steps, the field
@@ -77,10 +88,19 @@ __device__ __forceinline__ void set_status(unsigned int* st, unsigned int bit) {
 // pre-push cells (_8_3_1_IonParticleCounts.f90:612-634); at a master step,
 // ions whose previous-push cell is 0/-1 are first marked dead
 // (_8_0_7_BoundaryConditions.f90:158-256).
-__global__ void __launch_bounds__(KAOS_BLOCK) k_push_fp64(StepArgs a) {
-  __shared__ double s_qGH[KAOS_MAXQ], s_qGC[KAOS_MAXQ], s_EA[KAOS_MAXQ], s_EG[KAOS_MAXQ];
-  __shared__ double s_vparGH[KAOS_MAXV + 1], s_vperpGH[KAOS_MAXV + 1];
-  __shared__ unsigned int s_hist[KAOS_HWORDS];   // 16-bit counters, two per word
+// Dynamic LDS: qGH, qGC, EA, EG (lds_nq each), vparGH (lds_nv); the window of
+// 16-bit phase-space counters (two per word).
+#ifndef KAOS_FP64_MIN_BLOCKS
+#define KAOS_FP64_MIN_BLOCKS 4
+#endif
+__global__ void __launch_bounds__(KAOS_BLOCK, KAOS_FP64_MIN_BLOCKS) k_push_fp64(StepArgs a) {
+  extern __shared__ double s_dyn[];
+  double* s_qGH = s_dyn;
+  double* s_qGC = s_qGH + a.lds_nq;
+  double* s_EA = s_qGC + a.lds_nq;
+  double* s_EG = s_EA + a.lds_nq;
+  double* s_vparGH = s_EG + a.lds_nq;
+  unsigned int* s_hist = (unsigned int*)(s_vparGH + a.lds_nv);
   __shared__ unsigned int s_pushed;
   __shared__ int s_wq0, s_wnq, s_wtube;
   const int tid = threadIdx.x;
@@ -115,7 +135,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_push_fp64(StepArgs a) {
   for (int i = tid; i < KAOS_HWORDS; i += KAOS_BLOCK) s_hist[i] = 0u;
   __syncthreads();
   const int wq0 = s_wq0, wnq = s_wnq, wtube = s_wtube;
-  double vpar_scale = 0.0, vperp_scale = 0.0, vparGL0 = 0.0, vperpGL0 = 0.0;
+  double vpar_scale = 0.0, vparGL0 = 0.0;
   for (int tile = t0; tile < t1; ++tile) {
     int tb = a.tile_tube[tile];
     if (tb != cur) {
@@ -126,17 +146,14 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_push_fp64(StepArgs a) {
         s_EA[i] = a.EA[T.goff + i];   s_EG[i] = a.EG[T.goff + i];
       }
       for (int i = tid; i < T.nvpar; i += KAOS_BLOCK) s_vparGH[i] = a.vparGH[T.voff + i];
-      for (int i = tid; i < T.nvperp; i += KAOS_BLOCK) s_vperpGH[i] = a.vperpGH[T.woff + i];
       __syncthreads();
       cur = tb;
       end = T.off + a.count[tb];
       vparGL0 = a.vparGH[T.voff + T.nvpar];
-      vperpGL0 = a.vperpGH[T.woff + T.nvperp];
       vpar_scale = (double)T.nvpar / (a.vparGH[T.voff + T.nvpar - 1] - vparGL0);
-      vperp_scale = (double)T.nvperp / (a.vperpGH[T.woff + T.nvperp - 1] - vperpGL0);
     }
     const long long start = a.tile_start[tile];
-    kaos_push_par_t P = { T.h, T.p, T.cphi, T.sphi, T.grav, T.eamb, T.eambsign };
+    kaos_push_par_t P = kaos_push_par(T.h, T.p, T.cphi, T.sphi, T.grav, T.eamb, T.eambsign);
 #pragma unroll 1
     for (int it = 0; it < KAOS_PPT; ++it) {
       long long j = start + (long long)it * KAOS_BLOCK + tid;
@@ -154,7 +171,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_push_fp64(StepArgs a) {
       }
       ++npushed;
       int hint = (qprev >= 1) ? qprev : (qprev == 0 ? 1 : T.nq);
-      double vs = a.vp[j], vperp = a.vr[j];
+      double vs = a.vp[j];
       double aea, ag;
       if (a.do_interp) {
         double ea, eg;
@@ -168,16 +185,17 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_push_fp64(StepArgs a) {
       } else {
         aea = a.ae[j]; ag = a.ag[j];
       }
-      kaos_line_t L = kaos_line(rs, T.p);
+      kaos_line_t L = kaos_line_i(rs, P.irep);
       kaos_qb_t gA = kaos_stageA(L);
       int qc = kaos_find_qcell_from(gA.q, s_qGH, T.qGL0, T.qGHlast, T.nq, T.desc, hint);
       kaos_push_out_t o;
       kaos_push_ion(L, gA, qc, vs, ag, aea, &P, &o);
       // velocity cells (2-D): off-grid keeps the previous index (KUA:1087-1127)
+      // Vperp is constant on this path (no wave heating), so its cell, found when
+      // the ion was binned first (k_bin_initial, k_inject), is carried in the cell word
       int ip = kaos_find_vcell_guess(o.vpar, s_vparGH, vparGL0, T.nvpar, vpar_scale);
-      int iq = kaos_find_vcell_guess(vperp, s_vperpGH, vperpGL0, T.nvperp, vperp_scale);
+      int iq = (int)((cold >> 8) & 0xFFu);
       if (ip == 0) ip = (int)(cold & 0xFFu);
-      if (iq == 0) iq = (int)((cold >> 8) & 0xFFu);
       if (a.do_count && qc >= 1) {
         if (ip >= 1 && iq >= 1) {
           int wq = qc - wq0;
@@ -224,35 +242,31 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_bin_initial(StepArgs a) {
   int tb = a.tile_tube[tile];
   DevTube T = a.tubes[tb];
   long long end = T.off + a.count[tb];
-  const double vparGL0 = a.vparGH[T.voff + T.nvpar], vperpGL0 = a.vperpGH[T.woff + T.nvperp];
   for (int it = 0; it < KAOS_PPT; ++it) {
     long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
     if (j >= end) break;
-    int qc = kaos_find_qcell(a.qk[j], a.qGH + T.goff, T.qGL0, T.qGHlast, T.nq, T.desc);
-    int ip = kaos_find_vcell(a.vp[j], a.vparGH + T.voff, vparGL0, T.nvpar);
-    int iq = kaos_find_vcell(a.vr[j], a.vperpGH + T.woff, vperpGL0, T.nvperp);
+    uint32_t cw = a.cell[j];
+    int qc = (int)((cw >> 16) & KAOS_QMASK) - 1, iq = (int)((cw >> 8) & 0xFFu), ip = (int)(cw & 0xFFu);
     if (qc >= 1) {
       if (ip >= 1 && iq >= 1)
         atomicAdd(&a.hist[T.hoff + ((long long)(qc - 1) * T.nvperp + (iq - 1)) * T.nvpar + (ip - 1)], 1u);
       else
         atomicAdd(&a.nqoff[T.goff + qc - 1], 1u);
     }
-    a.cell[j] = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip;
   }
 }
 
-// Upload: positions from the root solve of q0 (InitialConditionsSub), q0
-// clamped into the grid as KSA:55-129 does before the k0 search.
+// Upload: positions from the root solve of q0 (InitialConditionsSub); the k0
+// cells (KSA:49-326) from q0 clamped into the grid as KSA:55-129 does, and
+// from the input Vpar0/Vperp0.
 __global__ void k_upload(DevTube T, long long n, const double* q0, const double* vpar0,
-                         const double* vperp0, double* rs, double* vp, double* vr, double* qk,
-                         double* ae, double* ag, uint32_t* cell) {
+                         const double* vperp0, const double* qGH, const double* vparGH,
+                         const double* vperpGH, const double* rref, StepArgs a, int mixed) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   long long j = T.off + i;
   double q = q0[i];
-  rs[j] = kaos_rs_of_q(q, T.p);
-  vp[j] = vpar0[i];
-  vr[j] = vperp0[i];
+  double rs = kaos_rs_of_q(q, T.p);
   if (T.south) {
     if (q <= T.qGL0) q = T.qGL0;
     else if (q > T.qGHlast) q = T.qGHlast;
@@ -260,8 +274,25 @@ __global__ void k_upload(DevTube T, long long n, const double* q0, const double*
     if (q > T.qGL0) q = T.qGL0;
     else if (q < T.qGHlast) q = T.qGHlast;
   }
-  qk[j] = q;
-  ae[j] = 0.0; ag[j] = 0.0; cell[j] = 0u;
+  int qc = kaos_find_qcell(q, qGH, T.qGL0, T.qGHlast, T.nq, T.desc);
+  int ip = kaos_find_vcell(vpar0[i], vparGH, vparGH[T.nvpar], T.nvpar);
+  int iq = kaos_find_vcell(vperp0[i], vperpGH, vperpGH[T.nvperp], T.nvperp);
+  if (mixed) {
+    int Q = kx_refq(qc, T.nq);
+    bool rflag, vflag;
+    a.rdi[j] = kx_enc(rs - rref[Q - 1], a.rk[T.goff + Q - 1], rflag);
+    a.vpi[j] = kx_enc_v(vpar0[i], ip, vparGH, a.vk + T.voff, T.nvpar, vflag);
+    a.vrf[j] = (float)vperp0[i];
+    a.qkf[j] = (float)q;
+    a.aef[j] = 0.0f; a.agf[j] = 0.0f;
+    a.cell[j] = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip |
+                (rflag ? KAOS_RFLAG : 0u) | (vflag ? KAOS_VFLAG : 0u);
+  } else {
+    a.rs[j] = rs;
+    a.vp[j] = vpar0[i]; a.vr[j] = vperp0[i]; a.qk[j] = q;
+    a.ae[j] = 0.0; a.ag[j] = 0.0;
+    a.cell[j] = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip;
+  }
 }
 
 __global__ void k_fill_u32(unsigned int* p, long long n) {
@@ -297,43 +328,53 @@ __global__ void k_field(const DevTube* tubes, const double* hq, const double* dq
 // LB/UB injection after the master push (KSB:195-793): ion k of tube t goes
 // to slot off + count + k.  Uniforms from the caller's stream (6 per ion) or
 // from the seeded Philox stream.
-__global__ void k_inject(const DevTube* tubes, const long long* inj_base, int ntubes,
-                         const double* qGH, const double* qGC, const double* EA, const double* EG,
-                         const double* vparGH, const double* vperpGH,
-                         double* rs, double* vp, double* vr, double* qk, double* ae, double* ag,
-                         uint32_t* cell, const long long* count, unsigned int* hist, unsigned int* nqoff,
-                         const double* uni, const long long* uni_base, unsigned long long seed,
-                         long long step, long long total) {
+__global__ void k_inject(StepArgs a, const long long* inj_base, int ntubes, const double* EA,
+                         const double* EG, const double* vperpGH, const double* uni,
+                         const long long* uni_base, unsigned long long seed, long long step,
+                         long long total, int mixed) {
   long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gi >= total) return;
   int t = 0;
   while (t + 1 < ntubes && inj_base[t + 1] <= gi) ++t;
-  DevTube T = tubes[t];
+  DevTube T = a.tubes[t];
   long long k = gi - inj_base[t];
   long long nl = T.lbcond ? T.n_inj_lb : 0;
   int lb = (k < nl);
   int Qb = lb ? 0 : T.nq - 1;
-  long long j = T.off + count[t] + k;
-  double qc = qGC[T.goff + Qb];
+  long long j = T.off + a.count[t] + k;
+  double qc = a.qGC[T.goff + Qb];
   double u[6];
   if (uni) { for (int i = 0; i < 6; ++i) u[i] = uni[6 * (uni_base[t] + k) + i]; }
   else kaos_inject_uniforms(seed, step, t, k, u);
-  int qcell = kaos_find_qcell(qc, qGH + T.goff, T.qGL0, T.qGHlast, T.nq, T.desc);
+  int qcell = kaos_find_qcell(qc, a.qGH + T.goff, T.qGL0, T.qGHlast, T.nq, T.desc);
   double r = kaos_rs_of_q(qc, T.p);
   kaos_line_t L = kaos_line(r, T.p);
   double vpar, v1, v2, vperp;
   kaos_mb_project(u, lb ? T.sig_lb : T.sig_ub, T.cphi, T.sphi, L, &vpar, &v1, &v2, &vperp);
-  int ip = kaos_find_vcell(vpar, vparGH + T.voff, vparGH[T.voff + T.nvpar], T.nvpar);
+  int ip = kaos_find_vcell(vpar, a.vparGH + T.voff, a.vparGH[T.voff + T.nvpar], T.nvpar);
   int iq = kaos_find_vcell(vperp, vperpGH + T.woff, vperpGH[T.woff + T.nvperp], T.nvperp);
-  rs[j] = r; vp[j] = vpar; vr[j] = vperp; qk[j] = qc;
-  ae[j] = fabs(T.qsms * fabs(EA[T.goff + Qb]));
-  ag[j] = fabs(T.qsms * fabs(EG[T.goff + Qb]));
-  cell[j] = ((uint32_t)(qcell + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip;
+  double aea = fabs(T.qsms * fabs(EA[T.goff + Qb]));
+  double agv = fabs(T.qsms * fabs(EG[T.goff + Qb]));
+  uint32_t flags = 0u;
+  if (mixed) {
+    int Q = kx_refq(qcell, T.nq);
+    bool rflag, vflag;
+    a.rdi[j] = kx_enc(r - a.rref[T.goff + Q - 1], a.rk[T.goff + Q - 1], rflag);
+    a.vpi[j] = kx_enc_v(vpar, ip, a.vparGH + T.voff, a.vk + T.voff, T.nvpar, vflag);
+    a.vrf[j] = (float)vperp;
+    a.qkf[j] = (float)qc;
+    a.aef[j] = (float)aea; a.agf[j] = (float)agv;
+    flags = (rflag ? KAOS_RFLAG : 0u) | (vflag ? KAOS_VFLAG : 0u);
+  } else {
+    a.rs[j] = r; a.vp[j] = vpar; a.vr[j] = vperp; a.qk[j] = qc;
+    a.ae[j] = aea; a.ag[j] = agv;
+  }
+  a.cell[j] = ((uint32_t)(qcell + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip | flags;
   if (qcell >= 1) {
     if (ip >= 1 && iq >= 1)
-      atomicAdd(&hist[T.hoff + ((long long)(qcell - 1) * T.nvperp + (iq - 1)) * T.nvpar + (ip - 1)], 1u);
+      atomicAdd(&a.hist[T.hoff + ((long long)(qcell - 1) * T.nvperp + (iq - 1)) * T.nvpar + (ip - 1)], 1u);
     else
-      atomicAdd(&nqoff[T.goff + qcell - 1], 1u);
+      atomicAdd(&a.nqoff[T.goff + qcell - 1], 1u);
   }
 }
 
@@ -395,15 +436,26 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments(const DevTube* tubes, co
   }
 }
 
-// Cartesian state of a tube segment for download.
-__global__ void k_reconstruct(DevTube T, long long n, const double* rs, const double* vp,
-                              double* out6) {
+// Cartesian state of a tube segment for download (zeros for a dead slot).
+__global__ void k_reconstruct(DevTube T, long long n, StepArgs a, int mixed, double* out6) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   long long j = T.off + i;
-  double r = rs[j];
+  double r = 0.0, v = 0.0;
+  if (mixed) {
+    uint32_t cw = a.cell[j];
+    if (cw != KAOS_DEAD) {
+      int Q = kx_refq((int)((cw >> 16) & KAOS_QMASK) - 1, T.nq);
+      r = a.rref[T.goff + Q - 1] + kx_dec(a.rdi[j], a.rk[T.goff + Q - 1], (cw & KAOS_RFLAG) != 0);
+      v = kx_dec_v(a.vpi[j], (int)(cw & 0xFFu), (cw & KAOS_VFLAG) != 0, a.vparGH + T.voff, a.vk + T.voff,
+                   T.nvpar);
+    }
+  } else {
+    r = a.rs[j];
+    v = a.vp[j];
+  }
   double x = 0, y = 0, z = 0, vx = 0, vy = 0, vz = 0;
-  if (r != 0.0) kaos_cartesian(r, vp[j], T.p, T.cphi, T.sphi, &x, &y, &z, &vx, &vy, &vz);
+  if (r != 0.0) kaos_cartesian(r, v, T.p, T.cphi, T.sphi, &x, &y, &z, &vx, &vy, &vz);
   out6[6 * i + 0] = x; out6[6 * i + 1] = y; out6[6 * i + 2] = z;
   out6[6 * i + 3] = vx; out6[6 * i + 4] = vy; out6[6 * i + 5] = vz;
 }
@@ -438,6 +490,10 @@ struct kaos_ctx {
          *d_dvpar = nullptr, *d_dvperp = nullptr, *d_d3v = nullptr;
   double *d_rs = nullptr, *d_vp = nullptr, *d_vr = nullptr, *d_qk = nullptr, *d_ae = nullptr,
          *d_ag = nullptr;
+  int *d_rdi = nullptr, *d_vpi = nullptr, *d_rk = nullptr, *d_vk = nullptr;
+  float *d_vrf = nullptr, *d_qkf = nullptr, *d_aef = nullptr, *d_agf = nullptr;
+  double *d_rGH = nullptr, *d_rref = nullptr;
+  int lds_nq = 0, lds_nv = 0;
   uint32_t* d_cell = nullptr;
   long long* d_count = nullptr;
   unsigned int *d_hist = nullptr, *d_nqc = nullptr, *d_nqoff = nullptr, *d_status = nullptr;
@@ -545,6 +601,9 @@ static int build_device(kaos_ctx* c) {
   c->dt.resize(nt);
   long long goff = 0, voff = 0, woff = 0, hoff = 0, doff = 0, off = 0;
   std::vector<double> qGH, qGC, hq, dq, Te, d3x, EG, vparGHx, vperpGHx, vparGC, vperpGC, dvpar, dvperp, d3v;
+  std::vector<double> rGH, rref;
+  std::vector<int> rk, vk;
+  const bool mixed = c->cfg.precision == KAOS_MIXED;
   std::vector<int> tile_tube, blk_tube, blk_q;
   std::vector<long long> tile_start;
   for (int t = 0; t < nt; ++t) {
@@ -562,6 +621,28 @@ static int build_device(kaos_ctx* c) {
     T.sig_lb = sqrt((KAOS_KB * g.Ts_lb) / g.ms);
     T.sig_ub = sqrt((KAOS_KB * g.Ts_ub) / g.ms);
     T.qGL0 = g.qGL[0]; T.qGHlast = g.qGH[g.nq - 1];
+    // radii of the q edges on the line (the mixed form bins |r| against them)
+    T.rGL0 = fabs(kaos_rs_of_q(g.qGL[0], g.p));
+    T.rGHlast = fabs(kaos_rs_of_q(g.qGH[g.nq - 1], g.p));
+    T.rsgn = T.desc ? 1.0 : -1.0;
+    // fixed-point exponent of a cell of width wd: 2^k wd < 2^30, so offsets up to
+    // two widths fit in 31 bits
+    auto fx = [](double wd) { int e = 0; frexp(wd, &e); return 30 - e; };
+    double rprev = T.rGL0;
+    for (int i = 0; i < g.nq; ++i) {
+      double re = fabs(kaos_rs_of_q(h.qGH[i], g.p));
+      if (mixed && !(re > rprev))
+        return fail(c, KAOS_E_ARG, "mixed form: tube " + std::to_string(t) +
+                                     " needs radii increasing with the cell index");
+      rGH.push_back(re);
+      rref.push_back(kaos_rs_of_q(h.qGC[i], g.p));
+      rk.push_back(fx(re - rprev));
+      rprev = re;
+    }
+    for (int i = 0; i < g.nvpar; ++i) vk.push_back(fx(h.vparGH[i] - h.vparGL[i]));
+    vk.push_back(0);
+    c->lds_nq = std::max(c->lds_nq, g.nq);
+    c->lds_nv = std::max(c->lds_nv, g.nvpar);
     c->dt[t] = T;
     for (int i = 0; i < g.nq; ++i) {
       qGH.push_back(h.qGH[i]); qGC.push_back(h.qGC[i]); hq.push_back(h.hq[i]); dq.push_back(h.dq[i]);
@@ -611,15 +692,29 @@ static int build_device(kaos_ctx* c) {
   HIPCHK(c, up((void**)&c->d_vperpGC, vperpGC.data(), sizeof(double) * woff));
   HIPCHK(c, up((void**)&c->d_dvperp, dvperp.data(), sizeof(double) * woff));
   HIPCHK(c, up((void**)&c->d_d3v, d3v.data(), sizeof(double) * doff));
-  size_t capb = sizeof(double) * (size_t)(off ? off : 1);
-  HIPCHK(c, hipMalloc((void**)&c->d_rs, capb));
-  HIPCHK(c, hipMalloc((void**)&c->d_vp, capb));
-  HIPCHK(c, hipMalloc((void**)&c->d_vr, capb));
-  HIPCHK(c, hipMalloc((void**)&c->d_qk, capb));
-  HIPCHK(c, hipMalloc((void**)&c->d_ae, capb));
-  HIPCHK(c, hipMalloc((void**)&c->d_ag, capb));
-  HIPCHK(c, hipMalloc((void**)&c->d_cell, sizeof(uint32_t) * (size_t)(off ? off : 1)));
-  HIPCHK(c, hipMemset(c->d_rs, 0, capb));
+  HIPCHK(c, up((void**)&c->d_rGH, rGH.data(), sizeof(double) * goff));
+  HIPCHK(c, up((void**)&c->d_rref, rref.data(), sizeof(double) * goff));
+  size_t nslot = (size_t)(off ? off : 1);
+  HIPCHK(c, up((void**)&c->d_rk, rk.data(), sizeof(int) * goff));
+  HIPCHK(c, up((void**)&c->d_vk, vk.data(), sizeof(int) * voff));
+  if (mixed) {
+    HIPCHK(c, hipMalloc((void**)&c->d_rdi, sizeof(int) * nslot));
+    HIPCHK(c, hipMalloc((void**)&c->d_vpi, sizeof(int) * nslot));
+    HIPCHK(c, hipMalloc((void**)&c->d_vrf, sizeof(float) * nslot));
+    HIPCHK(c, hipMalloc((void**)&c->d_qkf, sizeof(float) * nslot));
+    HIPCHK(c, hipMalloc((void**)&c->d_aef, sizeof(float) * nslot));
+    HIPCHK(c, hipMalloc((void**)&c->d_agf, sizeof(float) * nslot));
+  } else {
+    HIPCHK(c, hipMalloc((void**)&c->d_rs, sizeof(double) * nslot));
+    HIPCHK(c, hipMalloc((void**)&c->d_vp, sizeof(double) * nslot));
+    HIPCHK(c, hipMalloc((void**)&c->d_vr, sizeof(double) * nslot));
+    HIPCHK(c, hipMalloc((void**)&c->d_qk, sizeof(double) * nslot));
+    HIPCHK(c, hipMalloc((void**)&c->d_ae, sizeof(double) * nslot));
+    HIPCHK(c, hipMalloc((void**)&c->d_ag, sizeof(double) * nslot));
+    HIPCHK(c, hipMemset(c->d_rs, 0, sizeof(double) * nslot));
+  }
+  HIPCHK(c, hipMalloc((void**)&c->d_cell, sizeof(uint32_t) * nslot));
+  HIPCHK(c, hipMemset(c->d_cell, 0xFF, sizeof(uint32_t) * nslot));
   std::vector<long long> zc(nt, 0);
   HIPCHK(c, up((void**)&c->d_count, zc.data(), sizeof(long long) * nt));
   HIPCHK(c, hipMalloc((void**)&c->d_hist, sizeof(unsigned int) * (size_t)(hoff ? hoff : 1)));
@@ -651,6 +746,23 @@ static int build_device(kaos_ctx* c) {
   return KAOS_OK;
 }
 
+static StepArgs make_args(kaos_ctx* c, int interp, int count, int remove, int qk) {
+  StepArgs a;
+  a.tubes = c->d_tubes; a.tile_tube = c->d_tile_tube; a.tile_start = c->d_tile_start;
+  a.ntiles = c->ntiles;
+  a.tiles_per_block = KAOS_TPB;
+  a.qGH = c->d_qGH; a.qGC = c->d_qGC; a.EA = c->d_EA; a.EG = c->d_EG;
+  a.vparGH = c->d_vparGH; a.vperpGH = c->d_vperpGH;
+  a.rs = c->d_rs; a.vp = c->d_vp; a.vr = c->d_vr; a.qk = c->d_qk; a.ae = c->d_ae; a.ag = c->d_ag;
+  a.rdi = c->d_rdi; a.vpi = c->d_vpi; a.vrf = c->d_vrf; a.qkf = c->d_qkf; a.aef = c->d_aef; a.agf = c->d_agf;
+  a.rGH = c->d_rGH; a.rref = c->d_rref; a.rk = c->d_rk; a.vk = c->d_vk;
+  a.lds_nq = c->lds_nq; a.lds_nv = c->lds_nv;
+  a.cell = c->d_cell; a.count = c->d_count; a.hist = c->d_hist; a.nqoff = c->d_nqoff;
+  a.esc_acc = c->d_esc_acc; a.status = c->d_status; a.pushed = c->d_pushed;
+  a.do_interp = interp; a.do_count = count; a.do_remove = remove; a.do_qk = qk;
+  return a;
+}
+
 extern "C" int kaos_upload_particles(kaos_ctx* c, int32_t tube, int64_t n, const double* q0,
                                      const double* vpar0, const double* vperp0) {
   if (!c) return KAOS_E_ARG;
@@ -671,8 +783,10 @@ extern "C" int kaos_upload_particles(kaos_ctx* c, int32_t tube, int64_t n, const
     HIPCHK(c, hipMemcpy(dv0, vpar0, sizeof(double) * n, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(dr0, vperp0, sizeof(double) * n, hipMemcpyHostToDevice));
     int nbk = (int)((n + 255) / 256);
+    StepArgs a = make_args(c, 0, 0, 0, 0);
     hipLaunchKernelGGL(k_upload, dim3(nbk), dim3(256), 0, c->stream, T, (long long)n, dq0, dv0, dr0,
-                       c->d_rs, c->d_vp, c->d_vr, c->d_qk, c->d_ae, c->d_ag, c->d_cell);
+                       c->d_qGH + T.goff, c->d_vparGH + T.voff, c->d_vperpGH + T.woff, c->d_rref + T.goff,
+                       a, (int)(c->cfg.precision == KAOS_MIXED));
     HIPCHK(c, hipGetLastError());
   }
   long long cnt = n;
@@ -684,20 +798,6 @@ extern "C" int kaos_upload_particles(kaos_ctx* c, int32_t tube, int64_t n, const
   return KAOS_OK;
 }
 
-static StepArgs make_args(kaos_ctx* c, int interp, int count, int remove, int qk) {
-  StepArgs a;
-  a.tubes = c->d_tubes; a.tile_tube = c->d_tile_tube; a.tile_start = c->d_tile_start;
-  a.ntiles = c->ntiles;
-  a.tiles_per_block = KAOS_TPB;
-  a.qGH = c->d_qGH; a.qGC = c->d_qGC; a.EA = c->d_EA; a.EG = c->d_EG;
-  a.vparGH = c->d_vparGH; a.vperpGH = c->d_vperpGH;
-  a.rs = c->d_rs; a.vp = c->d_vp; a.vr = c->d_vr; a.qk = c->d_qk; a.ae = c->d_ae; a.ag = c->d_ag;
-  a.cell = c->d_cell; a.count = c->d_count; a.hist = c->d_hist; a.nqoff = c->d_nqoff;
-  a.esc_acc = c->d_esc_acc; a.status = c->d_status; a.pushed = c->d_pushed;
-  a.do_interp = interp; a.do_count = count; a.do_remove = remove; a.do_qk = qk;
-  return a;
-}
-
 static bool is_master(const kaos_ctx* c, long long n) { return n >= 2 && (n % c->cfg.ndatfac) == 0; }
 
 static std::pair<hipEvent_t, hipEvent_t> ev_pair(kaos_ctx* c) {
@@ -707,11 +807,25 @@ static std::pair<hipEvent_t, hipEvent_t> ev_pair(kaos_ctx* c) {
   return {a, b};
 }
 
+static size_t fp64_lds_bytes(const kaos_ctx* c) {
+  return sizeof(double) * (4 * (size_t)c->lds_nq + c->lds_nv) + sizeof(unsigned int) * KAOS_HWORDS;
+}
+
+// Dynamic LDS of k_push_mixed: binary64 rGH, rref, vparGH + GL0; binary32
+// qGC, EA, EG; the int exponents; the histogram window.
+static size_t mixed_lds_bytes(const kaos_ctx* c) {
+  return sizeof(double) * (2 * (size_t)c->lds_nq + c->lds_nv + 1) + sizeof(float) * 3 * (size_t)c->lds_nq +
+         sizeof(int) * ((size_t)c->lds_nq + c->lds_nv) + sizeof(unsigned int) * KAOS_HWORDS;
+}
+
 static void launch_push(kaos_ctx* c, const StepArgs& a) {
   int nblocks = (c->ntiles + a.tiles_per_block - 1) / a.tiles_per_block;
   std::pair<hipEvent_t, hipEvent_t> ev;
   if (c->prof_on) ev = ev_pair(c);
-  hipLaunchKernelGGL(k_push_fp64, dim3(nblocks), dim3(KAOS_BLOCK), 0, c->stream, a);
+  if (c->cfg.precision == KAOS_MIXED)
+    hipLaunchKernelGGL(k_push_mixed, dim3(nblocks), dim3(KAOS_BLOCK), mixed_lds_bytes(c), c->stream, a);
+  else
+    hipLaunchKernelGGL(k_push_fp64, dim3(nblocks), dim3(KAOS_BLOCK), fp64_lds_bytes(c), c->stream, a);
   if (c->prof_on) { (void)hipEventRecord(ev.second, c->stream); c->ev_push.push_back(ev); }
 }
 
@@ -747,11 +861,9 @@ static int enqueue_step(kaos_ctx* c, long long n, const double* d_uni) {
     if (master) {
       if (c->inj_total > 0) {
         int nb = (int)((c->inj_total + 255) / 256);
-        hipLaunchKernelGGL(k_inject, dim3(nb), dim3(256), 0, c->stream, c->d_tubes, c->d_inj_base, nt,
-                           c->d_qGH, c->d_qGC, c->d_EA, c->d_EG, c->d_vparGH, c->d_vperpGH,
-                           c->d_rs, c->d_vp, c->d_vr, c->d_qk, c->d_ae, c->d_ag, c->d_cell, c->d_count,
-                           c->d_hist, c->d_nqoff, d_uni, c->d_uni_base, (unsigned long long)c->cfg.seed,
-                           (long long)n, c->inj_total);
+        hipLaunchKernelGGL(k_inject, dim3(nb), dim3(256), 0, c->stream, a, c->d_inj_base, nt, c->d_EA,
+                           c->d_EG, c->d_vperpGH, d_uni, c->d_uni_base, (unsigned long long)c->cfg.seed,
+                           (long long)n, c->inj_total, (int)(c->cfg.precision == KAOS_MIXED));
       }
       hipLaunchKernelGGL(k_moments, dim3(c->nblk_mom), dim3(KAOS_BLOCK), 0, c->stream, c->d_tubes, c->d_blk_tube,
                          c->d_blk_q, c->d_d3x, c->d_d3v, c->d_dvpar, c->d_dvperp, c->d_vparGC, c->d_vperpGC,
@@ -854,10 +966,16 @@ extern "C" int kaos_count(kaos_ctx* c, int32_t tube, int64_t* n) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const HostTube& h = c->ht[tube];
   const DevTube& T = c->dt[tube];
-  std::vector<double> rs((size_t)h.count);
-  if (h.count) HIPCHK(c, hipMemcpy(rs.data(), c->d_rs + T.off, sizeof(double) * h.count, hipMemcpyDeviceToHost));
   int64_t k = 0;
-  for (double r : rs) k += (r != 0.0);
+  if (c->cfg.precision == KAOS_MIXED) {
+    std::vector<uint32_t> cw((size_t)h.count);
+    if (h.count) HIPCHK(c, hipMemcpy(cw.data(), c->d_cell + T.off, sizeof(uint32_t) * h.count, hipMemcpyDeviceToHost));
+    for (uint32_t w : cw) k += (w != KAOS_DEAD);
+  } else {
+    std::vector<double> rs((size_t)h.count);
+    if (h.count) HIPCHK(c, hipMemcpy(rs.data(), c->d_rs + T.off, sizeof(double) * h.count, hipMemcpyDeviceToHost));
+    for (double r : rs) k += (r != 0.0);
+  }
   *n = k;
   return KAOS_OK;
 }
@@ -910,27 +1028,43 @@ extern "C" int kaos_download_particles(kaos_ctx* c, int32_t tube, int64_t n_cap,
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const DevTube& T = c->dt[tube];
   long long n = c->ht[tube].count;
-  std::vector<double> rs(n), vr(n), qk(n), ae(n), agv(n), xyz(6 * (size_t)n);
+  const bool mixed = c->cfg.precision == KAOS_MIXED;
+  std::vector<double> vr(n), qk(n), ae(n), agv(n), xyz(6 * (size_t)n);
   std::vector<uint32_t> cell(n);
+  std::vector<char> live(n, 0);
   if (n) {
     double* d6 = nullptr;
     HIPCHK(c, hipMalloc((void**)&d6, sizeof(double) * 6 * n));
-    hipLaunchKernelGGL(k_reconstruct, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, T, n,
-                       c->d_rs, c->d_vp, d6);
+    StepArgs a = make_args(c, 0, 0, 0, 0);
+    hipLaunchKernelGGL(k_reconstruct, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, T, n, a,
+                       (int)mixed, d6);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(xyz.data(), d6, sizeof(double) * 6 * n, hipMemcpyDeviceToHost));
     (void)hipFree(d6);
-    HIPCHK(c, hipMemcpy(rs.data(), c->d_rs + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(vr.data(), c->d_vr + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(qk.data(), c->d_qk + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(ae.data(), c->d_ae + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(agv.data(), c->d_ag + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(cell.data(), c->d_cell + T.off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    if (mixed) {
+      std::vector<float> f(n);
+      float* src[4] = {c->d_vrf, c->d_qkf, c->d_aef, c->d_agf};
+      std::vector<double>* dst[4] = {&vr, &qk, &ae, &agv};
+      for (int i = 0; i < 4; ++i) {
+        HIPCHK(c, hipMemcpy(f.data(), src[i] + T.off, sizeof(float) * n, hipMemcpyDeviceToHost));
+        for (long long j = 0; j < n; ++j) (*dst[i])[j] = (double)f[j];
+      }
+      for (long long j = 0; j < n; ++j) live[j] = cell[j] != KAOS_DEAD;
+    } else {
+      std::vector<double> rs(n);
+      HIPCHK(c, hipMemcpy(rs.data(), c->d_rs + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(vr.data(), c->d_vr + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(qk.data(), c->d_qk + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(ae.data(), c->d_ae + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(agv.data(), c->d_ag + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+      for (long long j = 0; j < n; ++j) live[j] = rs[j] != 0.0;
+    }
   }
   long long k = 0;
   for (long long j = 0; j < n; ++j) {
-    if (rs[j] == 0.0) continue;
+    if (!live[j]) continue;
     if (k >= n_cap) return fail(c, KAOS_E_CAPACITY, "output arrays too small");
     if (x) x[k] = xyz[6 * j + 0];
     if (y) y[k] = xyz[6 * j + 1];
@@ -942,7 +1076,7 @@ extern "C" int kaos_download_particles(kaos_ctx* c, int32_t tube, int64_t n_cap,
     if (qk4) qk4[k] = qk[j];
     if (aea) aea[k] = ae[j];
     if (ag) ag[k] = agv[j];
-    if (qind) qind[k] = (int32_t)(cell[j] >> 16) - 1;
+    if (qind) qind[k] = (int32_t)((cell[j] >> 16) & KAOS_QMASK) - 1;
     if (vperpind) vperpind[k] = (int32_t)((cell[j] >> 8) & 0xFFu);
     if (vparind) vparind[k] = (int32_t)(cell[j] & 0xFFu);
     ++k;
@@ -986,7 +1120,8 @@ extern "C" int kaos_finalize(kaos_ctx* c) {
   void* bufs[] = {c->d_tubes, c->d_tile_tube, c->d_tile_start, c->d_blk_tube, c->d_blk_q, c->d_qGH, c->d_qGC,
                   c->d_hq, c->d_dq, c->d_Te, c->d_d3x, c->d_EA, c->d_EG, c->d_M, c->d_M0F, c->d_NqRT,
                   c->d_vparGH, c->d_vperpGH, c->d_vparGC, c->d_vperpGC, c->d_dvpar, c->d_dvperp, c->d_d3v,
-                  c->d_rs, c->d_vp, c->d_vr, c->d_qk, c->d_ae, c->d_ag, c->d_cell, c->d_count, c->d_hist,
+                  c->d_rs, c->d_vp, c->d_vr, c->d_qk, c->d_ae, c->d_ag, c->d_rdi, c->d_vpi, c->d_vrf,
+                  c->d_qkf, c->d_aef, c->d_agf, c->d_rGH, c->d_rref, c->d_rk, c->d_vk, c->d_cell, c->d_count, c->d_hist,
                   c->d_nqc, c->d_nqoff, c->d_status, c->d_esc_acc, c->d_esc_rep, c->d_inj_base, c->d_inj_cnt,
                   c->d_uni_base, c->d_uni, c->d_pushed};
   for (void* p : bufs) if (p) (void)hipFree(p);

@@ -1,0 +1,377 @@
+// kaos_mixed.h -- mixed-precision form of the push + bin kernel (included by
+// kaos_hip.hip after DevTube and StepArgs).
+//
+// State per ion, one 32-bit word each:
+//   rdi  radius: fixed-point offset from rref (binary64 radius of the centre of
+//        the cell-word cell), in units of 2^-k metres, k per cell so that two
+//        cell widths fit in 31 bits (resolution <= width / 2^30); cell-word
+//        flag KAOS_RFLAG: the word holds the binary32 offset instead (ions far
+//        outside the domain).
+//   vpi  V||: fixed-point offset from the centre of the velocity cell in the
+//        cell word (resolution <= dVpar / 2^30); flag KAOS_VFLAG (ions off the
+//        velocity grid): the word holds the binary32 offset of V|| from the
+//        nearer grid edge (positive above the top edge, negative below GL0).
+//   qkf  stage-4 q of the last push before a master step (binary32; it only
+//        sets the field interpolation weight, so its 1e-7 relative rounding
+//        moves the interpolated field by ~1e-7 of the cell-to-cell change).
+//   aef/agf AEAmag/AGmag (binary32); vrf Vperp (binary32, read for output only).
+//   cell (qc+1) << 16 | Vperpind << 8 | Vparind, plus the two flags; Vparind is
+//        the cell of the stored V|| (the binning index of the next push, kept
+//        from the previous push when V|| is off the grid); KAOS_DEAD = dead slot.
+//
+// Stage arithmetic (b-hat, forces, RK4 combinations) runs in binary32.
+// binary64 is kept where binary32 would lose the answer: the radius and V||
+// themselves (decoded offsets, the new values), the cell search (|r| against
+// the binary64 radii of the q edges, so no binary32 q meets an edge), the
+// root solve for the new point
+// (_8_1_4_RK4DipolePolynomialSolver.f90), whose input q^2 = A/B is assembled
+// as A = RE^4 [r^2 (1 - w) + Tz], B = [r^2 + Tu]^3 from the binary64
+// displacement (dR, dZ) = h v b-hat_A + h^2 (2 a1 + a2) / 6 (the first term
+// binary64 with sin, cos and 1/sqrt(1 + 3 cos^2) from binary32 seeds plus one
+// binary64 Newton step, ~1e-14; the h^2 force term binary32); and, in the
+// other kernels, the moments and the ambipolar gradient.  The new V|| is
+// b-hat_N . (v b-hat_A + dV) = v (1 - |b-hat_N - b-hat_A|^2 / 2) + b-hat_N . dV
+// for unit vectors, so binary32 rounding of b-hat touches only the small
+// increment dV and the tiny rotation term.
+
+#define KAOS_DEAD 0xFFFFFFFFu
+#define KAOS_RFLAG (1u << 30)
+#define KAOS_VFLAG (1u << 29)
+#define KAOS_QMASK 0x3FFu
+
+__device__ __forceinline__ float kf_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float kf_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float kf_sqrt(float x) { return __builtin_sqrtf(x); }
+
+// 1/sqrt(x) in binary64 from the binary32 estimate and one Newton step
+// (relative error ~3 e^2 with e ~ 1e-7 of the estimate).
+__device__ __forceinline__ double kx_rsq(double x) {
+  double y = (double)kf_rsq((float)x);
+  return y * (1.5 - 0.5 * (x * (y * y)));
+}
+
+// 32-bit fixed-point offset with scale 2^k; binary32 fallback when it does not fit.
+__device__ __forceinline__ int kx_enc(double off, int k, bool& fallback) {
+  double x = ldexp(off, k);
+  if (!(fabs(x) < 2147483000.0)) { fallback = true; return __float_as_int((float)off); }
+  fallback = false;
+  return (int)rint(x);
+}
+__device__ __forceinline__ double kx_dec(int w, int k, bool fallback) {
+  return fallback ? (double)__int_as_float(w) : ldexp((double)w, -k);
+}
+
+// Reference cell (1-based) of a cell-word config index qc: 0 -> 1, -1 -> nq.
+__device__ __forceinline__ int kx_refq(int qc, int nq) { return (qc >= 1) ? qc : (qc == 0 ? 1 : nq); }
+
+// Parallel acceleration (signs as kaos_apar; qle0 = stage q <= 0).
+__device__ __forceinline__ float kf_apar(bool qle0, float AG, float AEA, int grav, int eamb,
+                                         int eambsign, bool with_grav) {
+  float a = 0.0f;
+  if (grav && with_grav) a = qle0 ? fabsf(AG) : -fabsf(AG);
+  if (eamb) {
+    if (eambsign) a = a + AEA;
+    else a = a + (qle0 ? -fabsf(AEA) : fabsf(AEA));
+  }
+  return a;
+}
+
+// b-hat at (rho, z) off the line, as kaos_bhat_pt.
+__device__ __forceinline__ void kf_bhat_pt(float rho, float z, float& bR, float& bZ) {
+  float z2 = z * z;
+  float u2 = rho * rho + z2;
+  float d = kf_rsq(u2 * (u2 + 3.0f * z2));
+  bR = (3.0f * (z * rho)) * d;
+  bZ = (2.0f * z2 - rho * rho) * d;
+}
+
+// Cell of |r| on ascending binary64 radius edges (rGH[i] = |r| of qGH of cell
+// i+1, rGL0 of qGL of cell 1), walked from the previous cell: 1..nq, 0 below
+// the lower boundary (or NaN), -1 above the upper one.
+__device__ __forceinline__ int kf_find_rcell_from(double r, const double* rGH, double rGL0,
+                                                  double rGHlast, int nq, int hint) {
+  if (!(r >= rGL0)) return 0;
+  if (r > rGHlast) return -1;
+  int i = hint - 1;
+  if (i < 0) i = 0;
+  if (i > nq - 1) i = nq - 1;
+  while (i > 0 && r <= rGH[i - 1]) --i;
+  while (!(r <= rGH[i])) ++i;
+  return i + 1;
+}
+
+// Field interpolation at q in binary32 (as kaos_interp2_from).
+__device__ __forceinline__ int kf_interp2_from(float q, const float* qGC, const float* E, const float* E2,
+                                               int nq, float qGL0, float qGHlast, int desc, int hint,
+                                               float& out, float& out2) {
+  int ub = nq - 1;
+  int lo = hint - 1;
+  if (lo < 1) lo = 1;
+  if (lo > ub) lo = ub;
+  if (desc) {
+    if (!(q <= qGL0) || !(q >= qGHlast)) return 0;
+    if (q >= qGC[0]) { out = E[0]; out2 = E2[0]; return 1; }
+    if (q <= qGC[ub]) { out = E[ub]; out2 = E2[ub]; return 1; }
+    while (lo > 1 && q >= qGC[lo - 1]) --lo;
+    while (!(q >= qGC[lo])) ++lo;
+  } else {
+    if (!(q >= qGL0) || !(q <= qGHlast)) return 0;
+    if (q <= qGC[0]) { out = E[0]; out2 = E2[0]; return 1; }
+    if (q >= qGC[ub]) { out = E[ub]; out2 = E2[ub]; return 1; }
+    while (lo > 1 && q <= qGC[lo - 1]) --lo;
+    while (!(q <= qGC[lo])) ++lo;
+  }
+  float x1 = qGC[lo - 1], x2 = qGC[lo];
+  float f = (q - x1) * kf_rcp(x2 - x1);
+  out = E[lo - 1] + f * (E[lo] - E[lo - 1]);
+  out2 = E2[lo - 1] + f * (E2[lo] - E2[lo - 1]);
+  return 1;
+}
+
+__device__ __forceinline__ float kf_aea_of(float e_interp, float qsms, int eambsign) {
+  float e = eambsign ? e_interp : fabsf(e_interp);
+  if (e == 0.0f) return 0.0f;
+  return eambsign ? qsms * e : fabsf(qsms * e);
+}
+__device__ __forceinline__ float kf_ag_of(float e_interp, float qsms) {
+  float e = fabsf(e_interp);
+  if (e == 0.0f) return 0.0f;
+  return fabsf(qsms * e);
+}
+
+// Encode the velocity state v (binary64) given the new Vparind ipn (0 = off grid).
+__device__ __forceinline__ int kx_enc_v(double v, int ipn, const double* vGH, const int* vk, int nv,
+                                        bool& vflag) {
+  if (ipn >= 1) {
+    double lo = (ipn == 1) ? vGH[nv] : vGH[ipn - 2];
+    double ref = 0.5 * (lo + vGH[ipn - 1]);
+    return kx_enc(v - ref, vk[ipn - 1], vflag);
+  }
+  vflag = true;
+  return __float_as_int((float)(v - ((v > vGH[nv - 1]) ? vGH[nv - 1] : vGH[nv])));
+}
+__device__ __forceinline__ double kx_dec_v(int w, int ip, bool vflag, const double* vGH, const int* vk, int nv) {
+  if (vflag) {
+    double off = (double)__int_as_float(w);
+    return ((off > 0.0) ? vGH[nv - 1] : vGH[nv]) + off;
+  }
+  double lo = (ip == 1) ? vGH[nv] : vGH[ip - 2];
+  double ref = 0.5 * (lo + vGH[ip - 1]);
+  return ref + ldexp((double)w, -vk[ip - 1]);
+}
+
+// Push + bin, mixed form.  Same step semantics as k_push_fp64 (removal at a
+// master step, pre-push binning, stale 2-D velocity index, stage-4 gravity
+// gate, field interpolation at the previous stage-4 q on master steps).
+// Dynamic LDS: binary64 rGH, rref (lds_nq each), vparGH + GL0 (lds_nv + 1);
+// binary32 qGC, EA, EG (lds_nq each); int rk (lds_nq), vk (lds_nv); the
+// histogram window.
+#ifndef KAOS_MIXED_MIN_BLOCKS
+#define KAOS_MIXED_MIN_BLOCKS 4
+#endif
+__global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixed(StepArgs a) {
+  extern __shared__ double s_dyn[];
+  const int NQ = a.lds_nq, NV = a.lds_nv;
+  double* s_rGH = s_dyn;
+  double* s_rref = s_rGH + NQ;
+  double* s_vGH = s_rref + NQ;
+  float* s_qGC = (float*)(s_vGH + NV + 1);
+  float* s_EA = s_qGC + NQ;
+  float* s_EG = s_EA + NQ;
+  int* s_rk = (int*)(s_EG + NQ);
+  int* s_vk = s_rk + NQ;
+  unsigned int* s_hist = (unsigned int*)(s_vk + NV);
+  __shared__ unsigned int s_pushed;
+  __shared__ int s_wq0, s_wnq, s_wtube;
+  const int tid = threadIdx.x;
+  int t0 = blockIdx.x * a.tiles_per_block;
+  int t1 = min(t0 + a.tiles_per_block, a.ntiles);
+  if (tid == 0) {
+    s_pushed = 0u;
+    int tb0 = (t0 < t1) ? a.tile_tube[t0] : 0;
+    const DevTube& T0 = a.tubes[tb0];
+    int ncell = T0.nvpar * T0.nvperp;
+    int wnq = a.do_count ? (2 * KAOS_HWORDS) / ncell : 0;
+    if (wnq > T0.nq) wnq = T0.nq;
+    int q0 = 1;
+    if (t0 < t1 && wnq > 0) {
+      long long j0 = a.tile_start[t0];
+      if (j0 < T0.off + a.count[tb0]) {
+        uint32_t c0 = a.cell[j0];
+        int qprev = (c0 == KAOS_DEAD) ? 1 : (int)((c0 >> 16) & KAOS_QMASK) - 1;
+        if (qprev >= 1) q0 = qprev - 1;
+      }
+      if (q0 < 1) q0 = 1;
+      if (q0 + wnq - 1 > T0.nq) q0 = T0.nq - wnq + 1;
+    }
+    s_wq0 = q0; s_wnq = wnq; s_wtube = tb0;
+  }
+  for (int i = tid; i < KAOS_HWORDS; i += KAOS_BLOCK) s_hist[i] = 0u;
+  __syncthreads();
+  const int wq0 = s_wq0, wnq = s_wnq, wtube = s_wtube;
+  int cur = -1;
+  unsigned int npushed = 0;
+  // per-tube scalars
+  long long end = 0, hoff = 0, goff = 0;
+  int nq = 0, nvpar = 0, nvperp = 0, desc = 0, grav = 0, eamb = 0, eambsign = 0;
+  double rGL0 = 0.0, rGHlast = 0.0, rsgn = 1.0;
+  float qGL0 = 0.0f, qGHlast = 0.0f, qsms = 0.0f;
+  double ip = 0.0, irep = 0.0;
+  float h = 0.0f, h2 = 0.0f, hsq2 = 0.0f, h6 = 0.0f, hh6 = 0.0f;
+  double vscale = 0.0, hd = 0.0;
+  for (int tile = t0; tile < t1; ++tile) {
+    int tb = a.tile_tube[tile];
+    if (tb != cur) {
+      __syncthreads();
+      const DevTube& T = a.tubes[tb];
+      nq = T.nq; nvpar = T.nvpar; nvperp = T.nvperp; desc = T.desc;
+      grav = T.grav; eamb = T.eamb; eambsign = T.eambsign;
+      goff = T.goff; hoff = T.hoff;
+      qGL0 = (float)T.qGL0; qGHlast = (float)T.qGHlast; rGL0 = T.rGL0; rGHlast = T.rGHlast; rsgn = T.rsgn;
+      qsms = (float)T.qsms;
+      h = (float)T.h; h2 = (float)(T.h / 2.0); hsq2 = (float)((T.h * T.h) / 2.0); h6 = (float)(T.h / 6.0);
+      hh6 = (float)((T.h * T.h) / 6.0); hd = T.h;
+      ip = 1.0 / T.p; irep = 1.0 / (KAOS_RE * T.p);
+      for (int i = tid; i < nq; i += KAOS_BLOCK) {
+        s_rGH[i] = a.rGH[goff + i]; s_rref[i] = a.rref[goff + i];
+        s_qGC[i] = (float)a.qGC[goff + i]; s_EA[i] = (float)a.EA[goff + i]; s_EG[i] = (float)a.EG[goff + i];
+        s_rk[i] = a.rk[goff + i];
+      }
+      for (int i = tid; i <= nvpar; i += KAOS_BLOCK) s_vGH[i] = a.vparGH[T.voff + i];
+      for (int i = tid; i < nvpar; i += KAOS_BLOCK) s_vk[i] = a.vk[T.voff + i];
+      end = T.off + a.count[tb];
+      vscale = (double)nvpar / (a.vparGH[T.voff + nvpar - 1] - a.vparGH[T.voff + nvpar]);
+      cur = tb;
+      __syncthreads();
+    }
+    const long long start = a.tile_start[tile];
+#pragma unroll 1
+    for (int it = 0; it < KAOS_PPT; ++it) {
+      long long j = start + (long long)it * KAOS_BLOCK + tid;
+      if (j >= end) break;
+      uint32_t cold = a.cell[j];
+      if (cold == KAOS_DEAD) continue;
+      int qprev = (int)((cold >> 16) & KAOS_QMASK) - 1;
+      if (a.do_remove && (qprev == 0 || qprev == -1)) {
+        a.cell[j] = KAOS_DEAD;
+        atomicAdd(&a.esc_acc[2 * cur + (qprev == 0 ? 0 : 1)], 1ull);
+        continue;
+      }
+      ++npushed;
+      const int hint = kx_refq(qprev, nq);
+      const int ipc = (int)(cold & 0xFFu), iq = (int)((cold >> 8) & 0xFFu);
+      const double rs = s_rref[hint - 1] + kx_dec(a.rdi[j], s_rk[hint - 1], (cold & KAOS_RFLAG) != 0);
+      const double vs64 = kx_dec_v(a.vpi[j], ipc, (cold & KAOS_VFLAG) != 0, s_vGH, s_vk, nvpar);
+      float aea, ag;
+      if (a.do_interp) {
+        float ea, eg;
+        if (kf_interp2_from(a.qkf[j], s_qGC, s_EA, s_EG, nq, qGL0, qGHlast, desc, hint, ea, eg)) {
+          aea = kf_aea_of(ea, qsms, eambsign);
+          ag = kf_ag_of(eg, qsms);
+          a.aef[j] = aea; a.agf[j] = ag;
+        } else {
+          aea = a.aef[j]; ag = a.agf[j];
+        }
+      } else {
+        aea = a.aef[j]; ag = a.agf[j];
+      }
+      // stage A on the line: cell from |r| (binary64); sin, cos, b-hat to ~1e-14
+      const double r = fabs(rs);
+      const double w = r * irep;
+      const double omw = 1.0 - w;
+      const int qc = (rs * rsgn < 0.0) ? -1 : kf_find_rcell_from(r, s_rGH, rGL0, rGHlast, nq, hint);
+      const double s64 = w * kx_rsq(w);
+      double c64 = omw * kx_rsq(omw);
+      if (rs < 0.0) c64 = -c64;
+      const double isl64 = kx_rsq(4.0 - 3.0 * w);
+      const double bR64 = (3.0 * (c64 * s64)) * isl64, bZ64 = (2.0 - 3.0 * w) * isl64;
+      const double rho64 = r * s64, z64 = r * c64;
+      const float vs = (float)vs64;
+      const float sf = (float)s64, cf = (float)c64;
+      const float rho = (float)rho64, z = (float)z64;
+      const float bR = (float)bR64, bZ = (float)bZ64;
+      const float ap1 = kf_apar(cf <= 0.0f, ag, aea, grav, eamb, eambsign, true);
+      const float a1R = ap1 * bR, a1Z = ap1 * bZ;
+      const float VRR = vs * bR, VRZ = vs * bZ;
+      // stage B at the half step
+      float b2R, b2Z;
+      const float zB = z + h2 * VRZ;
+      kf_bhat_pt(rho + h2 * VRR, zB, b2R, b2Z);
+      const float ap2 = kf_apar(zB <= 0.0f, ag, aea, grav, eamb, eambsign, true);
+      const float a2R = ap2 * b2R, a2Z = ap2 * b2Z;
+      // stage C at the full-step predictor
+      float b4R, b4Z;
+      const float dRC = h * VRR + hsq2 * a1R, dZC = h * VRZ + hsq2 * a1Z;
+      const float zC = z + dZC;
+      kf_bhat_pt(rho + dRC, zC, b4R, b4Z);
+      const float ap4 = kf_apar(zC <= 0.0f, ag, aea, grav, eamb, eambsign, qc != 0 && qc != -1);
+      const float a4R = ap4 * b4R, a4Z = ap4 * b4Z;
+      // RK4 (k2 = k3): dV = h (a1 + 4 a2 + a4)/6, dx = h V + h^2 (2 a1 + a2)/6
+      const float dVR = h6 * (((a1R + 2.0f * a2R) + 2.0f * a2R) + a4R);
+      const float dVZ = h6 * (((a1Z + 2.0f * a2Z) + 2.0f * a2Z) + a4Z);
+      const double hv = hd * vs64;
+      const double dR = hv * bR64 + (double)(hh6 * (2.0f * a1R + a2R));
+      const double dZ = hv * bZ64 + (double)(hh6 * (2.0f * a1Z + a2Z));
+      // new point and root solve: u2 = r^2 + Tu, Z^2 = r^2 (1 - w) + Tz
+      const double Tu = 2.0 * (rho64 * dR + z64 * dZ) + (dR * dR + dZ * dZ);
+      const double Tz = 2.0 * (z64 * dZ) + dZ * dZ;
+      const double r2 = r * r;
+      const double u2N = r2 + Tu;
+      const double Z2 = r2 * omw + Tz;
+      const double g0 = (r + (s64 * dR + c64 * dZ)) * (1.0 / KAOS_RE);
+      const double rN = kaos_quartic_newton((KAOS_RE2 * KAOS_RE2) * Z2, (u2N * u2N) * u2N, ip, g0) * KAOS_RE;
+      const double rsN = (z64 + dZ <= 0.0) ? -rN : rN;
+      // V|| at the new point
+      const float wN = (float)(rN * irep);
+      float csN = kf_sqrt(wN * (1.0f - wN));
+      if (rsN < 0.0) csN = -csN;
+      const float islN = kf_rsq(4.0f - 3.0f * wN);
+      const float bNR = (3.0f * csN) * islN, bNZ = (2.0f - 3.0f * wN) * islN;
+      const float eR = bNR - bR, eZ = bNZ - bZ;
+      const double vN = (vs64 - vs64 * (double)(0.5f * (eR * eR + eZ * eZ))) + (double)(bNR * dVR + bNZ * dVZ);
+      // binning of the pre-push state: Vparind from the cell word
+      if (a.do_count && qc >= 1) {
+        if (ipc >= 1 && iq >= 1) {
+          int wq = qc - wq0;
+          if (cur == wtube && wq >= 0 && wq < wnq) {
+            int li = (wq * nvperp + (iq - 1)) * nvpar + (ipc - 1);
+            atomicAdd(&s_hist[li >> 1], 1u << (16 * (li & 1)));
+          } else {
+            atomicAdd(&a.hist[hoff + ((long long)(qc - 1) * nvperp + (iq - 1)) * nvpar + (ipc - 1)], 1u);
+          }
+        } else {
+          atomicAdd(&a.nqoff[goff + qc - 1], 1u);
+        }
+      }
+      if (rsN != rsN || vN != vN) set_status(a.status, 1u << KAOS_E_NAN);
+      // stage-4 q = RE^2 z / r^3 at the full-step predictor
+      if (a.do_qk) {
+        const float rhoC = rho + dRC;
+        const float u2C = rhoC * rhoC + zC * zC;
+        a.qkf[j] = (((float)KAOS_RE2) * zC) * (kf_rsq(u2C) * kf_rcp(u2C));
+      }
+      // next push's Vparind (stale when off the grid) and the new state words
+      const int ipn = kaos_find_vcell_guess(vN, s_vGH, s_vGH[nvpar], nvpar, vscale);
+      bool rflag, vflag;
+      const int Qn = kx_refq(qc, nq);
+      a.rdi[j] = kx_enc(rsN - s_rref[Qn - 1], s_rk[Qn - 1], rflag);
+      a.vpi[j] = kx_enc_v(vN, ipn, s_vGH, s_vk, nvpar, vflag);
+      a.cell[j] = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)(ipn >= 1 ? ipn : ipc) |
+                  (rflag ? KAOS_RFLAG : 0u) | (vflag ? KAOS_VFLAG : 0u);
+    }
+  }
+  __syncthreads();
+  atomicAdd(&s_pushed, npushed);
+  if (wnq > 0) {
+    const DevTube& TW = a.tubes[wtube];
+    long long base = TW.hoff + (long long)(wq0 - 1) * TW.nvperp * TW.nvpar;
+    int nwords = (wnq * TW.nvperp * TW.nvpar + 1) >> 1;
+    for (int w = tid; w < nwords; w += KAOS_BLOCK) {
+      unsigned int v = s_hist[w];
+      if (v & 0xFFFFu) atomicAdd(&a.hist[base + 2 * w], v & 0xFFFFu);
+      if (v >> 16) atomicAdd(&a.hist[base + 2 * w + 1], v >> 16);
+    }
+  }
+  __syncthreads();
+  if (tid == 0 && s_pushed) atomicAdd(a.pushed, (unsigned long long)s_pushed);
+}

@@ -70,6 +70,8 @@ def load():
         for n in ("orc_root_r",):
             getattr(lib, n).argtypes = [C.c_double, C.c_double]
             getattr(lib, n).restype = C.c_double
+        lib.orc_quartic_newton.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double]
+        lib.orc_quartic_newton.restype = C.c_double
         for n in ("orc_sinh_asinh_third", "orc_log", "orc_cos2pi"):
             getattr(lib, n).argtypes = [C.c_double]
             getattr(lib, n).restype = C.c_double

@@ -301,7 +301,7 @@ static int inject(orc_t *o, int64_t n, const double *uni) {
 
 static void push_all(orc_t *o) {
   orc_cfg *c = &o->c;
-  kaos_push_par_t P = { c->h, c->p, c->cphi, c->sphi, c->grav, c->eamb, c->eambsign };
+  kaos_push_par_t P = kaos_push_par(c->h, c->p, c->cphi, c->sphi, c->grav, c->eamb, c->eambsign);
   for (int64_t j = 0; j < o->ns; ++j) {
     kaos_line_t L = kaos_line(o->rs[j], c->p);
     kaos_qb_t gA = kaos_stageA(L);
@@ -493,7 +493,7 @@ void orc_moments_from_counts(const orc_cfg *cfg, const double *nq_counts, const 
  * out[j] = (xN, yN, zN, VxN, VyN, VzN, rsN, VparN, Vpar, qk1, qk4). */
 void orc_push_lines(const orc_cfg *c, int64_t n, const double *rs, const double *vs,
                     const double *ag, const double *aea, double *out) {
-  kaos_push_par_t P = { c->h, c->p, c->cphi, c->sphi, c->grav, c->eamb, c->eambsign };
+  kaos_push_par_t P = kaos_push_par(c->h, c->p, c->cphi, c->sphi, c->grav, c->eamb, c->eambsign);
   for (int64_t j = 0; j < n; ++j) {
     kaos_line_t L = kaos_line(rs[j], c->p);
     kaos_qb_t gA = kaos_stageA(L);
@@ -521,6 +521,7 @@ int orc_interp2_from(const orc_cfg *c, const double *E, double q, int hint, doub
                            c->qGL[0] > 0, hint, out, (double *)0);
 }
 double orc_root_r(double q, double p) { return kaos_root_r(q, p); }
+double orc_quartic_newton(double A, double B, double ip, double g0) { return kaos_quartic_newton(A, B, ip, g0); }
 double orc_sinh_asinh_third(double y) { return kaos_sinh_asinh_third(y); }
 double orc_log(double x) { return kaos_log(x); }
 double orc_cos2pi(double u) { return kaos_cos2pi(u); }

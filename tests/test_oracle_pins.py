@@ -149,6 +149,31 @@ def test_mb_projection_is_a_rotation(lib, tube):
 
 # --- push vs the reference's compiled Fortran push ---------------------------------
 
+@pytest.mark.parametrize("p", [3.0, 4.0, 6.0, 8.0])
+def test_quartic_newton_reprojection(lib, p):
+    """The push's re-projection solves A g^4 + B (g/p - 1) = 0 (q^2 = A/B of
+    the point (R, Z)) by Newton; pinned against numpy.roots, the closed form
+    r(theta) of the line, and the residual."""
+    rr = np.linspace(1.05, 0.95 * p, 200) * RE
+    th = np.arcsin(np.sqrt(rr / (p * RE)))
+    for r_true, t in zip(rr, th):
+        # an off-line point with the same q: scale the line point by k and its z by k^3
+        for k in (1.0, 1.0001, 0.999):
+            R, Z = k * r_true * np.sin(t), k ** 3 * r_true * np.cos(t)
+            u2 = R * R + Z * Z
+            A, B = RE ** 4 * Z * Z, u2 ** 3
+            qq = math.sqrt(A / B)
+            roots = np.roots([qq * qq, 0, 0, 1.0 / p, -1.0])
+            gr = max(x.real for x in roots if abs(x.imag) < 1e-9 and x.real > 0)
+            for rel in (0.0, 1e-9, -1e-9, 1e-4, -1e-4, 0.3, -0.5, 4.0):
+                g = lib.orc_quartic_newton(A, B, 1.0 / p, (r_true / RE) * (1.0 + rel))
+                assert abs(g - gr) / gr < 2e-15, (k, rel)
+                assert abs(qq * qq * g ** 4 + g / p - 1.0) < 1e-14
+            if k == 1.0:
+                g = lib.orc_quartic_newton(A, B, 1.0 / p, r_true / RE * (1 + 1e-4))
+                assert abs(g * RE - r_true) / r_true < 2e-13
+
+
 def test_push_matches_reference_fortran(lib):
     import oracle.ref_push as rp
     if not rp.available():
@@ -174,9 +199,9 @@ def test_push_matches_reference_fortran(lib):
     assert keep.mean() > 0.99
     for i, k in enumerate(("xN", "yN", "zN", "VxN", "VyN", "VzN")):
         scale = max(np.abs(r[k]).max(), 1e-300)
-        assert np.max(np.abs(out[keep, i] - r[k][keep])) / scale < 4e-15, k
-    assert np.max(np.abs(out[:, 8] - r["Vpar"]) / np.abs(r["Vpar"]).max()) < 4e-15
-    assert np.max(np.abs(out[:, 10] - r["qk4"]) / np.abs(r["qk4"]).max()) < 4e-15
+        assert np.max(np.abs(out[keep, i] - r[k][keep])) / scale < 2e-15, k
+    assert np.max(np.abs(out[:, 8] - r["Vpar"]) / np.abs(r["Vpar"]).max()) < 2e-15
+    assert np.max(np.abs(out[:, 10] - r["qk4"]) / np.abs(r["qk4"]).max()) < 2e-15
     # cell indices of the reference's own linear scans, same input state
     cfg = o.cfg
     qs = np.array([lib.orc_scan_qcell(C.byref(cfg), v) for v in out[:, 9]])
