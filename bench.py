@@ -4,8 +4,13 @@ Workload (the full-size single-GPU configuration): one dipole flux tube at
 L=4 from 1.05 to 3 RE, 256 config cells, 64x64 (Vpar, Vperp) velocity grid,
 1e8 O+ ions (Maxwellian 0.3 eV, density ~ exp(-s/500 km)), dt = 0.05 s, every
 step a master step: push, binning, zeroth-second moments, ambipolar and
-gravity fields, open-boundary removal and injection.  Synthetic inputs, FP64.
+gravity fields, open-boundary removal and injection.  Synthetic inputs.
 
+Both precision forms run the same inputs and the same steps:
+  * FP64 (binary64 throughout, bit-exact against the oracle): `value`;
+  * mixed (binary32 stages and state words): the `mixed` object, with the
+    relative errors of its moments and field against the FP64 run after the
+    last timed step.
 Prints one JSON line.  value = ion pushes (live ions summed over the timed
 steps, counted on the device) / wall time of the timed steps, inputs resident
 in HBM.  roofline: the push kernel's algorithmic HBM bytes per launch over its
@@ -18,26 +23,40 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# bytes per ion per step of the FP64 push at ndatfac = 1:
-# read rs, vp, vr, qk (4 x 8) + cell (4); write rs, vp, qk, ae, ag (5 x 8) + cell (4)
-FP64_BYTES_PER_ION = 36 + 44
+# Algorithmic HBM bytes per ion and step of the push kernels when every step
+# is a master step (interpolation, counting and the stage-4 q every step):
+#   FP64:  read cell 4 + rs, vp, qk 3x8            = 28
+#          write rs, vp, qk, ae, ag 5x8 + cell 4    = 44
+#   mixed: read cell, rdi, vpi, qkf 4x4             = 16
+#          write rdi, vpi, qkf, aef, agf, cell 6x4  = 24
+BYTES_PER_ION = {"fp64": 28 + 44, "mixed": 16 + 24}
+# Measured HBM bytes per ion and step of the same launches: rocprofv3
+# FETCH_SIZE and WRITE_SIZE of the push kernel at 1e8 ions, each scaled by the
+# ratio of a plain 1 GiB device copy's true bytes to its counter reading
+# (fetch x2.000, write x1.000), from
+# profiles/k_push_1e8_ions_256x64x64_hbm_traffic.txt.
+TRAFFIC_BYTES_PER_ION = {"fp64": 77.4, "mixed": 41.0}
 HBM_PEAK_GBS = 8000.0
+MOMENTS = ("M0", "M1perp", "M1par", "M2", "M2perp", "M2par", "EA")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n_ions=1_000_000, steps=15):
+def cpu_baseline(n_ions=2_000_000, steps=20):
     """Oracle (C, one thread) on the same geometry with a bounded ion sample."""
     import oracle
     from kaos_amd import grids
     tube = grids.config2(n_total=n_ions)
     q0, v0, w0 = grids.initial_ions(tube, seed=1)
-    orc = oracle.Oracle(tube, q0, v0, w0, ndatfac=1, seed=1, capacity=n_ions + 10_000)
+    orc = oracle.Oracle(tube, q0, v0, w0, ndatfac=1, seed=1,
+                        capacity=len(q0) + (steps + 4) * (tube.n_inj_lb + tube.n_inj_ub) + 1024)
     orc.step(1)
     t0 = time.perf_counter()
     ps = 0
@@ -47,8 +66,64 @@ def cpu_baseline(n_ions=1_000_000, steps=15):
     dt = time.perf_counter() - t0
     orc.close()
     return dict(value=ps / dt, unit="particle-steps/s", cores=1, kind="port",
-                sample=f"oracle/kaos_oracle.c, same tube and grids, {n_ions} ions x {steps} master steps, "
-                       f"{dt:.1f} s")
+                sample=f"oracle/kaos_oracle.c (FP64), same tube and grids, {len(q0)} ions x {steps} "
+                       f"master steps, {dt:.1f} s")
+
+
+def timed_run(form, tube, q0, v0, w0, cap, args):
+    from kaos_amd.solver import KineticSolver, KAOS_FP64, KAOS_MIXED
+    prec = KAOS_MIXED if form == "mixed" else KAOS_FP64
+    ks = KineticSolver([tube], precision=prec, ndatfac=1, seed=1, capacity=[cap])
+    ks.upload(0, q0, v0, w0)
+    ks.step(1)
+    if args.warmup:
+        ks.run(2, args.warmup)
+    ks.profile(True)
+    ks.profile_read()
+    first = 2 + args.warmup
+    t1 = time.perf_counter()
+    pushes = ks.run(first, args.steps)
+    wall = time.perf_counter() - t1
+    push_ms, launches, run_ms = ks.profile_read()
+    ks.profile(False)
+    mean_push_s = push_ms / 1e3 / max(launches, 1)
+    ions_per_launch = pushes / max(launches, 1)
+    achieved = BYTES_PER_ION[form] * ions_per_launch / mean_push_s / 1e9
+    log(f"[bench] {form}: {args.steps} steps, wall {wall:.3f}s, run events {run_ms/1e3:.3f}s, "
+        f"push kernel {push_ms/max(launches,1):.3f} ms x {launches}")
+    res = dict(value=pushes / wall, ms_per_step=wall / args.steps * 1e3,
+               roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": TRAFFIC_BYTES_PER_ION[form] * ions_per_launch,
+                         "traffic_unit": "bytes per launch (counter-measured bytes per ion x ions per launch)",
+                         "kernel": "k_push_mixed" if form == "mixed" else "k_push_fp64",
+                         "bytes_per_ion": BYTES_PER_ION[form], "mean_launch_ms": mean_push_s * 1e3})
+    return ks, res
+
+
+def moment_errors(ma, mb):
+    out = {}
+    for k in MOMENTS:
+        a, b = ma[k], mb[k]
+        na, sa = np.linalg.norm(a), np.abs(a).max()
+        out[k] = {"l2_rel": float(np.linalg.norm(a - b) / na) if na > 0 else 0.0,
+                  "max_rel": float(np.abs(a - b).max() / sa) if sa > 0 else 0.0}
+    out["counts_abs_diff"] = int(np.abs(ma["nq_counts"] - mb["nq_counts"]).sum())
+    out["phase_counts_abs_diff"] = int(np.abs(ma["nph_counts"] - mb["nph_counts"]).sum())
+    return out
+
+
+def copy_probe(mib):
+    """A plain device copy of `mib` MiB (torch), three times: the reference kernel
+    whose FETCH_SIZE / WRITE_SIZE counters calibrate the push kernels' traffic."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"[bench] copy probe: 3 copies of {mib} MiB ({n * 4} bytes read and written each)")
 
 
 def main():
@@ -58,66 +133,60 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--ions", type=int, default=100_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--form", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--form", choices=("both", "fp64", "mixed"), default="both")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run the traffic-calibration copy of MIB MiB and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU sharding is a design note, not an implemented path.", file=sys.stderr)
         sys.exit(2)
 
     from kaos_amd import grids
-    from kaos_amd.solver import KineticSolver, KAOS_FP64, KAOS_MIXED
     t0 = time.perf_counter()
     tube = grids.config2(n_total=args.ions)
     q0, v0, w0 = grids.initial_ions(tube, seed=1)
     n_steps_total = args.warmup + args.steps + 1
     cap = len(q0) + (n_steps_total + 2) * (tube.n_inj_lb + tube.n_inj_ub) + 1024
     log(f"[bench] ions {len(q0)}, inj/step {tube.n_inj_lb}+{tube.n_inj_ub}, setup {time.perf_counter()-t0:.1f}s")
-    prec = KAOS_MIXED if args.form == "mixed" else KAOS_FP64
-    ks = KineticSolver([tube], precision=prec, ndatfac=1, seed=1, capacity=[cap])
-    ks.upload(0, q0, v0, w0)
+    forms = ("fp64", "mixed") if args.form == "both" else (args.form,)
+    runs = {}
+    for f in forms:
+        runs[f] = timed_run(f, tube, q0, v0, w0, cap, args)
     del q0, v0, w0
-    log(f"[bench] uploaded, {time.perf_counter()-t0:.1f}s")
-    ks.step(1)
-    if args.warmup:
-        ks.run(2, args.warmup)
-    log(f"[bench] warmup done ({args.warmup} steps)")
-    ks.profile(True)
-    ks.profile_read()
-    first = 2 + args.warmup
-    t1 = time.perf_counter()
-    pushes = ks.run(first, args.steps)
-    wall = time.perf_counter() - t1
-    push_ms, launches, run_ms = ks.profile_read()
-    log(f"[bench] timed {args.steps} steps: wall {wall:.3f}s, run events {run_ms/1e3:.3f}s, "
-        f"push kernel {push_ms/max(launches,1):.3f} ms x {launches}")
-    m = ks.master(0)
-    ks.close()
-    value = pushes / wall
-    mean_push_s = push_ms / 1e3 / max(launches, 1)
-    ions_per_launch = pushes / max(launches, 1)
-    achieved = FP64_BYTES_PER_ION * ions_per_launch / mean_push_s / 1e9
+    head = runs[forms[0]][1]
+    ions_end = int(runs[forms[0]][0].master(0)["nq_counts"].sum())
     out = {
         "metric": "particle-steps/s incl. moments; mixed vs FP64 at <=1e-6 moment rel. error",
-        "value": value,
+        "value": head["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if forms[0] == "fp64" else "f32+f64",
         "data": "synthetic (Maxwellian ions, exp(-s/500 km) density, seeded)",
         "config": {"workload": "one L=4 dipole tube, 1.05-3 RE, 256 cells, 64x64 (Vpar,Vperp), "
                                f"{args.ions:.0e} O+ ions, dt=0.05 s, every step a master step",
-                   "form": "FP64", "ions_end": int(m["nq_counts"].sum())},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_push_fp64", "bytes_per_ion": FP64_BYTES_PER_ION,
-                     "mean_launch_ms": mean_push_s * 1e3},
+                   "form": forms[0], "ions_end": ions_end},
+        "roofline": head["roofline"],
     }
+    if "mixed" in runs and "fp64" in runs:
+        m = runs["mixed"][1]
+        err = moment_errors(runs["fp64"][0].master(0), runs["mixed"][0].master(0))
+        out["mixed"] = {"value": m["value"], "ms_per_step": m["ms_per_step"],
+                        "speedup_vs_fp64": m["value"] / head["value"], "dtype": "f32+f64",
+                        "roofline": m["roofline"],
+                        "error_vs_fp64_after_last_step": err,
+                        "max_l2_rel_moment_error": max(err[k]["l2_rel"] for k in MOMENTS)}
+    for ks, _ in runs.values():
+        ks.close()
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline()

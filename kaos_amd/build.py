@@ -31,5 +31,28 @@ def build_lib(verbose=False, out=None, defines=()):
     return out
 
 
+FLANG = "/opt/rocm/lib/llvm/bin/flang"
+
+
+def build_fortran(verbose=False):
+    """The iso_c_binding module and the Fortran driver kaos_fdriver, linked
+    against libkaos_hip.so in the package directory (rpath $ORIGIN/..)."""
+    fdir = os.path.join(HERE, "fortran")
+    obj = os.path.join(fdir, "_build")
+    os.makedirs(obj, exist_ok=True)
+    out = os.path.join(fdir, "kaos_fdriver")
+    tmp = out + ".tmp"
+    cmd = [FLANG, "-O2", "-J", obj, os.path.join(fdir, "kaos_hip_iface.f90"),
+           os.path.join(fdir, "kaos_fdriver.f90"), "-L", HERE, "-lkaos_hip",
+           "-Wl,-rpath,$ORIGIN/..", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=obj)
+    if r.returncode != 0:
+        raise RuntimeError("flang failed:\n" + r.stderr)
+    if verbose and r.stderr:
+        print(r.stderr)
+    os.replace(tmp, out)
+    return out
+
+
 if __name__ == "__main__":
     print(build_lib(verbose=True))

@@ -1,0 +1,150 @@
+! kaos_hip_iface.f90 -- iso_c_binding interface of libkaos_hip.so for a Fortran
+! driver (the layouts and entry points of include/kaos_hip.h, one for one).
+!
+! The reference's hot-path subroutines take no arguments and share module
+! globals (_KineticMainParams.f90:33-554); a driver that keeps the reference's
+! call sequence (_8_KineticSolver.f90:54-239) calls these functions in place of
+! KineticSolverA/BSub's push loop, ParticleCountsSub, DistributionFunctionsSub,
+! the moment subroutines, AmbipolarEfieldSub, GravfieldSub and
+! BoundaryConditionsSub, passing c_loc of its own arrays (valid only during the
+! call).  Index conventions: config cells 1..nq (0 = LB escape, -1 = UB), the
+! 2-D counts as [Q][Vperp][Vpar] with Vpar fastest.
+module kaos_hip
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: KAOS_OK = 0, KAOS_E_ARG = 1, KAOS_E_CAPACITY = 2, &
+                                       KAOS_E_DEVICE = 3, KAOS_E_STATE = 4, KAOS_E_NAN = 5
+  integer(c_int32_t), parameter, public :: KAOS_FP64 = 0, KAOS_MIXED = 1
+
+  type, bind(c), public :: kaos_cfg
+    integer(c_int32_t) :: ntubes, precision, ndatfac, device
+    integer(c_int64_t) :: seed                   ! uint64_t on the C side
+  end type kaos_cfg
+
+  type, bind(c), public :: kaos_grid
+    integer(c_int32_t) :: nq, nvpar, nvperp, grav, eamb, eambsign, filter_pt, lbcond, ubcond, pad0
+    integer(c_int64_t) :: n_inj_lb, n_inj_ub
+    real(c_double) :: p, cphi, sphi, h, ms, qs, nsnormfac, Ts_lb, Ts_ub
+    type(c_ptr) :: qGL, qGH, qGC, hqC, dqC, d3xC, rGC, cthGC, ellGC, Te
+    type(c_ptr) :: vparGL, vparGH, vparGC, dvpar, vperpGL, vperpGH, vperpGC, dvperp, d3v
+  end type kaos_grid
+
+  type, bind(c), public :: kaos_step_out
+    integer(c_int64_t) :: n_alive, n_injected
+    integer(c_int32_t) :: master, status
+  end type kaos_step_out
+
+  public :: kaos_init, kaos_set_grid, kaos_upload_particles, kaos_step, kaos_run, &
+            kaos_download_master, kaos_count, kaos_download_particles, kaos_profile, &
+            kaos_profile_read, kaos_last_error, kaos_finalize, kaos_error_message
+
+  interface
+    integer(c_int) function kaos_init(cfg, ctx) bind(c, name='kaos_init')
+      import :: c_int, c_ptr, kaos_cfg
+      type(kaos_cfg), intent(in) :: cfg
+      type(c_ptr), intent(out) :: ctx
+    end function kaos_init
+
+    integer(c_int) function kaos_set_grid(ctx, tube, g, capacity) bind(c, name='kaos_set_grid')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, kaos_grid
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tube
+      type(kaos_grid), intent(in) :: g
+      integer(c_int64_t), value :: capacity
+    end function kaos_set_grid
+
+    integer(c_int) function kaos_upload_particles(ctx, tube, n, q0, vpar0, vperp0) &
+        bind(c, name='kaos_upload_particles')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tube
+      integer(c_int64_t), value :: n
+      real(c_double), intent(in) :: q0(*), vpar0(*), vperp0(*)
+    end function kaos_upload_particles
+
+    integer(c_int) function kaos_step(ctx, n, uniforms, n_uniforms, out) bind(c, name='kaos_step')
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n
+      type(c_ptr), value :: uniforms             ! c_null_ptr: seeded device stream
+      integer(c_int64_t), value :: n_uniforms
+      type(c_ptr), value :: out                  ! c_loc(kaos_step_out) or c_null_ptr
+    end function kaos_step
+
+    integer(c_int) function kaos_run(ctx, n_first, nsteps, particle_steps) bind(c, name='kaos_run')
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n_first, nsteps
+      integer(c_int64_t), intent(out) :: particle_steps
+    end function kaos_run
+
+    integer(c_int) function kaos_download_master(ctx, tube, nq_counts, nph_counts, NqRT, M, EA, EG, esc) &
+        bind(c, name='kaos_download_master')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tube
+      type(c_ptr), value :: nq_counts, nph_counts, NqRT, M, EA, EG, esc
+    end function kaos_download_master
+
+    integer(c_int) function kaos_count(ctx, tube, n) bind(c, name='kaos_count')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tube
+      integer(c_int64_t), intent(out) :: n
+    end function kaos_count
+
+    integer(c_int) function kaos_download_particles(ctx, tube, n_cap, n_out, x, y, z, Vx, Vy, Vz, &
+        vperp, qk4, aea, ag, qind, vparind, vperpind) bind(c, name='kaos_download_particles')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tube
+      integer(c_int64_t), value :: n_cap
+      integer(c_int64_t), intent(out) :: n_out
+      type(c_ptr), value :: x, y, z, Vx, Vy, Vz, vperp, qk4, aea, ag, qind, vparind, vperpind
+    end function kaos_download_particles
+
+    integer(c_int) function kaos_profile(ctx, enable) bind(c, name='kaos_profile')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: enable
+    end function kaos_profile
+
+    integer(c_int) function kaos_profile_read(ctx, push_ms, push_launches, run_ms) &
+        bind(c, name='kaos_profile_read')
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: push_ms, run_ms
+      integer(c_int64_t), intent(out) :: push_launches
+    end function kaos_profile_read
+
+    type(c_ptr) function kaos_last_error(ctx) bind(c, name='kaos_last_error')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function kaos_last_error
+
+    integer(c_int) function kaos_finalize(ctx) bind(c, name='kaos_finalize')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function kaos_finalize
+  end interface
+
+contains
+
+  ! Text of kaos_last_error(ctx) as a Fortran string (the reference reports
+  ! errors by printing 'ERROR: ...' and continuing; a driver can do the same).
+  function kaos_error_message(ctx) result(msg)
+    type(c_ptr), intent(in) :: ctx
+    character(len=256) :: msg
+    character(kind=c_char), pointer :: s(:)
+    integer :: i
+    msg = ' '
+    call c_f_pointer(kaos_last_error(ctx), s, [256])
+    do i = 1, 256
+      if (s(i) == c_null_char) exit
+      msg(i:i) = s(i)
+    end do
+  end function kaos_error_message
+
+end module kaos_hip

@@ -1,0 +1,68 @@
+"""The C-ABI library loads and exports every entry point include/kaos_hip.h
+declares (CPU only: nothing is called that touches a device), and the ctypes
+mirror's struct layouts match the header's field order and sizes."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from kaos_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "kaos_hip.h")
+
+
+def _declared():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(kaos_\w+)\s*\(", txt, re.M)))
+
+
+def test_header_declares_the_python_exports():
+    assert _declared() == sorted(_abi.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_abi.LIB_PATH):
+        pytest.skip("libkaos_hip.so not built")
+    nm = subprocess.run(["nm", "-D", "--defined-only", _abi.LIB_PATH], capture_output=True, text=True)
+    assert nm.returncode == 0, nm.stderr
+    syms = {l.split()[-1] for l in nm.stdout.splitlines() if l.strip()}
+    missing = [s for s in _declared() if s not in syms]
+    assert not missing, missing
+    lib = C.CDLL(_abi.LIB_PATH)
+    for s in _declared():
+        assert getattr(lib, s) is not None
+
+
+def test_struct_layouts_match_the_header():
+    # offsets/sizes as a C compiler lays out include/kaos_hip.h
+    src = os.path.join(ROOT, "tests", "_layout.c")
+    exe = os.path.join(ROOT, "tests", "_layout.bin")
+    prog = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "kaos_hip.h"
+int main(void) {
+  printf("%zu %zu %zu %zu\n", sizeof(kaos_cfg), offsetof(kaos_cfg, seed), sizeof(kaos_grid), offsetof(kaos_grid, d3v));
+  printf("%zu %zu %zu\n", offsetof(kaos_grid, n_inj_lb), offsetof(kaos_grid, p), offsetof(kaos_grid, qGL));
+  printf("%zu %zu\n", sizeof(kaos_step_out), offsetof(kaos_step_out, status));
+  return 0;
+}
+'''
+    with open(src, "w") as f:
+        f.write(prog)
+    try:
+        r = subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), src, "-o", exe], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        out = subprocess.run([exe], capture_output=True, text=True).stdout.split()
+    finally:
+        for p in (src, exe):
+            if os.path.exists(p):
+                os.remove(p)
+    got = [int(x) for x in out]
+    want = [C.sizeof(_abi.KaosCfg), _abi.KaosCfg.seed.offset, C.sizeof(_abi.KaosGrid), _abi.KaosGrid.d3v.offset,
+            _abi.KaosGrid.n_inj_lb.offset, _abi.KaosGrid.p.offset, _abi.KaosGrid.qGL.offset,
+            C.sizeof(_abi.KaosStepOut), _abi.KaosStepOut.status.offset]
+    assert got == want
