@@ -44,6 +44,9 @@ typedef struct kaos_cfg {
   int32_t ndatfac;       /* master-step cadence ndatfacGT (>= 1) */
   int32_t device;        /* HIP device ordinal */
   uint64_t seed;         /* seed of the injection stream (used when no uniforms are passed) */
+  int32_t reorder_every; /* master steps between on-device compaction + sort of the ions by
+                            cell (0 = library default, < 0 = never); results do not depend on it */
+  int32_t reserved;      /* 0 */
 } kaos_cfg;
 
 /* One flux tube: grid, species and flags.  Replaces the FluxTubeT/QCellT/VCellT

@@ -19,7 +19,8 @@ _dp = C.POINTER(C.c_double)
 
 class KaosCfg(C.Structure):
     _fields_ = [("ntubes", C.c_int32), ("precision", C.c_int32), ("ndatfac", C.c_int32),
-                ("device", C.c_int32), ("seed", C.c_uint64)]
+                ("device", C.c_int32), ("seed", C.c_uint64), ("reorder_every", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 GRID_ARRAYS = ("qGL", "qGH", "qGC", "hqC", "dqC", "d3xC", "rGC", "cthGC", "ellGC", "Te",

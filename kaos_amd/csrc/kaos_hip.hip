@@ -40,6 +40,7 @@
 #define KAOS_HWORDS 6144                     // LDS histogram window: 12288 16-bit counters
 #endif
 #define KAOS_TPB 16                          // tiles per push block (<= 63: 16-bit counters)
+#define KAOS_REORDER_DEFAULT 16              // master steps between reorders by default
 
 struct DevTube {
   int nq, nvpar, nvperp, desc;
@@ -50,6 +51,7 @@ struct DevTube {
   double p, cphi, sphi, h, ms, qs, qsms, nsnormfac, sig_lb, sig_ub;
   double qGL0, qGHlast, qGL_last0;
   double rGL0, rGHlast, rsgn;               // |r| of the outer q edges; sign of rs in the tube
+  long long koff;                           // offset of the tube's nq + 2 reorder keys
 };
 
 struct StepArgs {
@@ -68,6 +70,7 @@ struct StepArgs {
   const int *rk, *vk;                        // fixed-point exponents per config / Vpar cell
   int lds_nq, lds_nv;                        // largest nq / nvpar over the tubes (LDS sizing)
   uint32_t* cell;
+  int* id;                                   // order key: upload index, then injection order
   const long long* count;
   unsigned int *hist, *nqoff;               // phase-space counts; config counts off the v-grid
   unsigned long long* esc_acc;
@@ -275,6 +278,7 @@ __global__ void k_upload(DevTube T, long long n, const double* q0, const double*
     else if (q < T.qGHlast) q = T.qGHlast;
   }
   int qc = kaos_find_qcell(q, qGH, T.qGL0, T.qGHlast, T.nq, T.desc);
+  a.id[j] = (int)i;
   int ip = kaos_find_vcell(vpar0[i], vparGH, vparGH[T.nvpar], T.nvpar);
   int iq = kaos_find_vcell(vperp0[i], vperpGH, vperpGH[T.nvperp], T.nvperp);
   if (mixed) {
@@ -328,8 +332,8 @@ __global__ void k_field(const DevTube* tubes, const double* hq, const double* dq
 // LB/UB injection after the master push (KSB:195-793): ion k of tube t goes
 // to slot off + count + k.  Uniforms from the caller's stream (6 per ion) or
 // from the seeded Philox stream.
-__global__ void k_inject(StepArgs a, const long long* inj_base, int ntubes, const double* EA,
-                         const double* EG, const double* vperpGH, const double* uni,
+__global__ void k_inject(StepArgs a, const long long* inj_base, const long long* next_id, int ntubes,
+                         const double* EA, const double* EG, const double* vperpGH, const double* uni,
                          const long long* uni_base, unsigned long long seed, long long step,
                          long long total, int mixed) {
   long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -370,6 +374,7 @@ __global__ void k_inject(StepArgs a, const long long* inj_base, int ntubes, cons
     a.ae[j] = aea; a.ag[j] = agv;
   }
   a.cell[j] = ((uint32_t)(qcell + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip | flags;
+  a.id[j] = (int)(next_id[t] + k);
   if (qcell >= 1) {
     if (ip >= 1 && iq >= 1)
       atomicAdd(&a.hist[T.hoff + ((long long)(qcell - 1) * T.nvperp + (iq - 1)) * T.nvpar + (ip - 1)], 1u);
@@ -386,7 +391,8 @@ __global__ void k_inject(StepArgs a, const long long* inj_base, int ntubes, cons
 __global__ void __launch_bounds__(KAOS_BLOCK) k_moments(const DevTube* tubes, const int* blk_tube,
     const int* blk_q, const double* d3xC, const double* d3v, const double* dvpar, const double* dvperp,
     const double* vparGC, const double* vperpGC, const unsigned int* hist, const unsigned int* nqoff,
-    unsigned int* nqc, double* NqRT, double* M, long long* count, const long long* inj_cnt) {
+    unsigned int* nqc, double* NqRT, double* M, long long* count, long long* next_id,
+    const long long* inj_cnt) {
   __shared__ double s_col[KAOS_MAXV + 1][4];
   __shared__ double s_M[6];
   __shared__ unsigned int s_cnt[KAOS_MAXV + 1];
@@ -432,8 +438,103 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments(const DevTube* tubes, co
     kaos_mom_finish2(S, T.ms, Mq);
     for (int i = 0; i < 6; ++i) M[6 * T.goff + (long long)i * T.nq + Q] = Mq[i];
     NqRT[T.goff + Q] = nqr;
-    if (Q == 0 && inj_cnt) count[t] += inj_cnt[t];
+    if (Q == 0 && inj_cnt) { count[t] += inj_cnt[t]; next_id[t] += inj_cnt[t]; }
   }
+}
+
+// Reorder (compaction + counting sort by config cell), run every few master
+// steps: live ions of each tube are packed to the front of its segment,
+// grouped by their cell-word cell, so the push's block-local histogram window
+// keeps covering a block's ions as they diffuse.  Dead slots are dropped (the
+// physical form of BCReIndexSub's pack, _8_0_7_1_BCReIndex.f90:33-133); each
+// ion keeps its id, which orders downloads as the reference orders its
+// arrays.  Within a cell the new order depends on block scheduling; no result
+// depends on the slot order (counts are integers, ions are independent).
+__device__ __forceinline__ int reorder_key(const StepArgs& a, long long j, int mixed) {
+  uint32_t cw = a.cell[j];
+  if (mixed ? (cw == KAOS_DEAD) : (a.rs[j] == 0.0)) return -1;
+  return (int)((cw >> 16) & KAOS_QMASK);     // qc + 1 in [0, nq + 1]
+}
+
+__global__ void __launch_bounds__(KAOS_BLOCK) k_reorder_count(StepArgs a, int mixed, unsigned int* keycnt) {
+  __shared__ unsigned int s_cnt[KAOS_MAXQ + 2];
+  int tile = blockIdx.x;
+  int tb = a.tile_tube[tile];
+  const DevTube& T = a.tubes[tb];
+  int nk = T.nq + 2;
+  for (int k = threadIdx.x; k < nk; k += KAOS_BLOCK) s_cnt[k] = 0u;
+  __syncthreads();
+  long long end = T.off + a.count[tb];
+  for (int it = 0; it < KAOS_PPT; ++it) {
+    long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
+    if (j >= end) break;
+    int key = reorder_key(a, j, mixed);
+    if (key >= 0) atomicAdd(&s_cnt[key], 1u);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nk; k += KAOS_BLOCK)
+    if (s_cnt[k]) atomicAdd(&keycnt[T.koff + k], s_cnt[k]);
+}
+
+// One thread per tube: cursor = segment start + exclusive prefix of the key
+// counts; the new ion count; the counts cleared for the next reorder.
+__global__ void k_reorder_scan(const DevTube* tubes, int ntubes, unsigned int* keycnt,
+                               unsigned long long* cursor, long long* cnt_new) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntubes) return;
+  const DevTube& T = tubes[t];
+  unsigned long long run = (unsigned long long)T.off;
+  for (int k = 0; k < T.nq + 2; ++k) {
+    cursor[T.koff + k] = run;
+    run += keycnt[T.koff + k];
+    keycnt[T.koff + k] = 0u;
+  }
+  cnt_new[t] = (long long)run - T.off;
+}
+
+__global__ void __launch_bounds__(KAOS_BLOCK) k_reorder_scatter(StepArgs a, StepArgs b, int mixed,
+                                                                unsigned long long* cursor) {
+  __shared__ unsigned int s_cnt[KAOS_MAXQ + 2];
+  __shared__ unsigned long long s_base[KAOS_MAXQ + 2];
+  int tile = blockIdx.x;
+  int tb = a.tile_tube[tile];
+  const DevTube& T = a.tubes[tb];
+  int nk = T.nq + 2;
+  for (int k = threadIdx.x; k < nk; k += KAOS_BLOCK) s_cnt[k] = 0u;
+  __syncthreads();
+  long long end = T.off + a.count[tb];
+  int key[KAOS_PPT];
+  for (int it = 0; it < KAOS_PPT; ++it) {
+    long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
+    key[it] = (j < end) ? reorder_key(a, j, mixed) : -1;
+    if (key[it] >= 0) atomicAdd(&s_cnt[key[it]], 1u);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nk; k += KAOS_BLOCK) {
+    unsigned int n = s_cnt[k];
+    s_base[k] = n ? atomicAdd(&cursor[T.koff + k], (unsigned long long)n) : 0ull;
+    s_cnt[k] = 0u;
+  }
+  __syncthreads();
+  for (int it = 0; it < KAOS_PPT; ++it) {
+    if (key[it] < 0) continue;
+    long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
+    long long d = (long long)(s_base[key[it]] + atomicAdd(&s_cnt[key[it]], 1u));
+    if (mixed) {
+      b.rdi[d] = a.rdi[j]; b.vpi[d] = a.vpi[j]; b.vrf[d] = a.vrf[j];
+      b.qkf[d] = a.qkf[j]; b.aef[d] = a.aef[j]; b.agf[d] = a.agf[j];
+    } else {
+      b.rs[d] = a.rs[j]; b.vp[d] = a.vp[j]; b.vr[d] = a.vr[j];
+      b.qk[d] = a.qk[j]; b.ae[d] = a.ae[j]; b.ag[d] = a.ag[j];
+    }
+    b.cell[d] = a.cell[j];
+    b.id[d] = a.id[j];
+  }
+}
+
+__global__ void k_reorder_finish(int ntubes, const long long* cnt_new, long long* count) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ntubes) count[t] = cnt_new[t];
 }
 
 // Cartesian state of a tube segment for download (zeros for a dead slot).
@@ -471,6 +572,15 @@ struct HostTube {
   std::vector<double> vparGL, vparGH, vparGC, dvpar, vperpGL, vperpGH, vperpGC, dvperp, d3v;
 };
 
+// One set of particle-state arrays (the binary64 or the mixed form's words).
+struct StateBufs {
+  double *rs = nullptr, *vp = nullptr, *vr = nullptr, *qk = nullptr, *ae = nullptr, *ag = nullptr;
+  int *rdi = nullptr, *vpi = nullptr;
+  float *vrf = nullptr, *qkf = nullptr, *aef = nullptr, *agf = nullptr;
+  uint32_t* cell = nullptr;
+  int* id = nullptr;
+};
+
 struct kaos_ctx {
   kaos_cfg cfg;
   std::string err;
@@ -488,13 +598,16 @@ struct kaos_ctx {
          *d_NqRT = nullptr;
   double *d_vparGH = nullptr, *d_vperpGH = nullptr, *d_vparGC = nullptr, *d_vperpGC = nullptr,
          *d_dvpar = nullptr, *d_dvperp = nullptr, *d_d3v = nullptr;
-  double *d_rs = nullptr, *d_vp = nullptr, *d_vr = nullptr, *d_qk = nullptr, *d_ae = nullptr,
-         *d_ag = nullptr;
-  int *d_rdi = nullptr, *d_vpi = nullptr, *d_rk = nullptr, *d_vk = nullptr;
-  float *d_vrf = nullptr, *d_qkf = nullptr, *d_aef = nullptr, *d_agf = nullptr;
+  StateBufs st[2];                           // particle state; st[cur] is live, the other the reorder target
+  int cur = 0;
+  int *d_rk = nullptr, *d_vk = nullptr;
+  unsigned int* d_keycnt = nullptr;          // reorder: ions per (tube, key)
+  unsigned long long* d_cursor = nullptr;    // reorder: next free slot per (tube, key)
+  long long *d_cnt_new = nullptr, *d_next_id = nullptr;
+  long long nkeys_tot = 0;
+  int masters_since_reorder = 0;
   double *d_rGH = nullptr, *d_rref = nullptr;
   int lds_nq = 0, lds_nv = 0;
-  uint32_t* d_cell = nullptr;
   long long* d_count = nullptr;
   unsigned int *d_hist = nullptr, *d_nqc = nullptr, *d_nqoff = nullptr, *d_status = nullptr;
   unsigned long long *d_esc_acc = nullptr, *d_esc_rep = nullptr;
@@ -616,6 +729,7 @@ static int build_device(kaos_ctx* c) {
     T.lbcond = g.lbcond; T.ubcond = g.ubcond;
     T.off = off; T.cap = h.cap; T.n_inj_lb = g.n_inj_lb; T.n_inj_ub = g.n_inj_ub;
     T.goff = goff; T.voff = voff; T.woff = woff; T.hoff = hoff; T.doff = doff;
+    T.koff = goff + 2LL * t;
     T.p = g.p; T.cphi = g.cphi; T.sphi = g.sphi; T.h = g.h; T.ms = g.ms; T.qs = g.qs;
     T.qsms = g.qs / g.ms; T.nsnormfac = g.nsnormfac;
     T.sig_lb = sqrt((KAOS_KB * g.Ts_lb) / g.ms);
@@ -697,26 +811,36 @@ static int build_device(kaos_ctx* c) {
   size_t nslot = (size_t)(off ? off : 1);
   HIPCHK(c, up((void**)&c->d_rk, rk.data(), sizeof(int) * goff));
   HIPCHK(c, up((void**)&c->d_vk, vk.data(), sizeof(int) * voff));
-  if (mixed) {
-    HIPCHK(c, hipMalloc((void**)&c->d_rdi, sizeof(int) * nslot));
-    HIPCHK(c, hipMalloc((void**)&c->d_vpi, sizeof(int) * nslot));
-    HIPCHK(c, hipMalloc((void**)&c->d_vrf, sizeof(float) * nslot));
-    HIPCHK(c, hipMalloc((void**)&c->d_qkf, sizeof(float) * nslot));
-    HIPCHK(c, hipMalloc((void**)&c->d_aef, sizeof(float) * nslot));
-    HIPCHK(c, hipMalloc((void**)&c->d_agf, sizeof(float) * nslot));
-  } else {
-    HIPCHK(c, hipMalloc((void**)&c->d_rs, sizeof(double) * nslot));
-    HIPCHK(c, hipMalloc((void**)&c->d_vp, sizeof(double) * nslot));
-    HIPCHK(c, hipMalloc((void**)&c->d_vr, sizeof(double) * nslot));
-    HIPCHK(c, hipMalloc((void**)&c->d_qk, sizeof(double) * nslot));
-    HIPCHK(c, hipMalloc((void**)&c->d_ae, sizeof(double) * nslot));
-    HIPCHK(c, hipMalloc((void**)&c->d_ag, sizeof(double) * nslot));
-    HIPCHK(c, hipMemset(c->d_rs, 0, sizeof(double) * nslot));
+  for (int b = 0; b < 2; ++b) {
+    StateBufs& B = c->st[b];
+    if (mixed) {
+      HIPCHK(c, hipMalloc((void**)&B.rdi, sizeof(int) * nslot));
+      HIPCHK(c, hipMalloc((void**)&B.vpi, sizeof(int) * nslot));
+      HIPCHK(c, hipMalloc((void**)&B.vrf, sizeof(float) * nslot));
+      HIPCHK(c, hipMalloc((void**)&B.qkf, sizeof(float) * nslot));
+      HIPCHK(c, hipMalloc((void**)&B.aef, sizeof(float) * nslot));
+      HIPCHK(c, hipMalloc((void**)&B.agf, sizeof(float) * nslot));
+    } else {
+      HIPCHK(c, hipMalloc((void**)&B.rs, sizeof(double) * nslot));
+      HIPCHK(c, hipMalloc((void**)&B.vp, sizeof(double) * nslot));
+      HIPCHK(c, hipMalloc((void**)&B.vr, sizeof(double) * nslot));
+      HIPCHK(c, hipMalloc((void**)&B.qk, sizeof(double) * nslot));
+      HIPCHK(c, hipMalloc((void**)&B.ae, sizeof(double) * nslot));
+      HIPCHK(c, hipMalloc((void**)&B.ag, sizeof(double) * nslot));
+      HIPCHK(c, hipMemset(B.rs, 0, sizeof(double) * nslot));
+    }
+    HIPCHK(c, hipMalloc((void**)&B.cell, sizeof(uint32_t) * nslot));
+    HIPCHK(c, hipMemset(B.cell, 0xFF, sizeof(uint32_t) * nslot));
+    HIPCHK(c, hipMalloc((void**)&B.id, sizeof(int) * nslot));
   }
-  HIPCHK(c, hipMalloc((void**)&c->d_cell, sizeof(uint32_t) * nslot));
-  HIPCHK(c, hipMemset(c->d_cell, 0xFF, sizeof(uint32_t) * nslot));
+  c->nkeys_tot = goff + 2LL * nt;
+  HIPCHK(c, hipMalloc((void**)&c->d_keycnt, sizeof(unsigned int) * c->nkeys_tot));
+  HIPCHK(c, hipMemset(c->d_keycnt, 0, sizeof(unsigned int) * c->nkeys_tot));
+  HIPCHK(c, hipMalloc((void**)&c->d_cursor, sizeof(unsigned long long) * c->nkeys_tot));
+  HIPCHK(c, hipMalloc((void**)&c->d_cnt_new, sizeof(long long) * nt));
   std::vector<long long> zc(nt, 0);
   HIPCHK(c, up((void**)&c->d_count, zc.data(), sizeof(long long) * nt));
+  HIPCHK(c, up((void**)&c->d_next_id, zc.data(), sizeof(long long) * nt));
   HIPCHK(c, hipMalloc((void**)&c->d_hist, sizeof(unsigned int) * (size_t)(hoff ? hoff : 1)));
   HIPCHK(c, hipMalloc((void**)&c->d_nqc, sizeof(unsigned int) * (size_t)goff));
   HIPCHK(c, hipMalloc((void**)&c->d_nqoff, sizeof(unsigned int) * (size_t)goff));
@@ -746,6 +870,12 @@ static int build_device(kaos_ctx* c) {
   return KAOS_OK;
 }
 
+static void set_state(StepArgs& a, const StateBufs& B) {
+  a.rs = B.rs; a.vp = B.vp; a.vr = B.vr; a.qk = B.qk; a.ae = B.ae; a.ag = B.ag;
+  a.rdi = B.rdi; a.vpi = B.vpi; a.vrf = B.vrf; a.qkf = B.qkf; a.aef = B.aef; a.agf = B.agf;
+  a.cell = B.cell; a.id = B.id;
+}
+
 static StepArgs make_args(kaos_ctx* c, int interp, int count, int remove, int qk) {
   StepArgs a;
   a.tubes = c->d_tubes; a.tile_tube = c->d_tile_tube; a.tile_start = c->d_tile_start;
@@ -753,11 +883,10 @@ static StepArgs make_args(kaos_ctx* c, int interp, int count, int remove, int qk
   a.tiles_per_block = KAOS_TPB;
   a.qGH = c->d_qGH; a.qGC = c->d_qGC; a.EA = c->d_EA; a.EG = c->d_EG;
   a.vparGH = c->d_vparGH; a.vperpGH = c->d_vperpGH;
-  a.rs = c->d_rs; a.vp = c->d_vp; a.vr = c->d_vr; a.qk = c->d_qk; a.ae = c->d_ae; a.ag = c->d_ag;
-  a.rdi = c->d_rdi; a.vpi = c->d_vpi; a.vrf = c->d_vrf; a.qkf = c->d_qkf; a.aef = c->d_aef; a.agf = c->d_agf;
+  set_state(a, c->st[c->cur]);
   a.rGH = c->d_rGH; a.rref = c->d_rref; a.rk = c->d_rk; a.vk = c->d_vk;
   a.lds_nq = c->lds_nq; a.lds_nv = c->lds_nv;
-  a.cell = c->d_cell; a.count = c->d_count; a.hist = c->d_hist; a.nqoff = c->d_nqoff;
+  a.count = c->d_count; a.hist = c->d_hist; a.nqoff = c->d_nqoff;
   a.esc_acc = c->d_esc_acc; a.status = c->d_status; a.pushed = c->d_pushed;
   a.do_interp = interp; a.do_count = count; a.do_remove = remove; a.do_qk = qk;
   return a;
@@ -791,6 +920,7 @@ extern "C" int kaos_upload_particles(kaos_ctx* c, int32_t tube, int64_t n, const
   }
   long long cnt = n;
   HIPCHK(c, hipMemcpyAsync(c->d_count + tube, &cnt, sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_next_id + tube, &cnt, sizeof(long long), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   (void)hipFree(dq0); (void)hipFree(dv0); (void)hipFree(dr0);
   h.count = n;
@@ -829,7 +959,35 @@ static void launch_push(kaos_ctx* c, const StepArgs& a) {
   if (c->prof_on) { (void)hipEventRecord(ev.second, c->stream); c->ev_push.push_back(ev); }
 }
 
-// Enqueue step n on the stream (no host synchronisation).
+// Master steps between reorders: cfg.reorder_every, 0 = the default, < 0 = never.
+static int reorder_period(const kaos_ctx* c) {
+  return c->cfg.reorder_every == 0 ? KAOS_REORDER_DEFAULT : c->cfg.reorder_every;
+}
+
+// Compact and sort every tube's ions by cell into the other state buffer, swap,
+// and bring the host mirror of the slot counts up to date (one synchronisation).
+static int enqueue_reorder(kaos_ctx* c) {
+  const int nt = c->cfg.ntubes;
+  const int mixed = c->cfg.precision == KAOS_MIXED;
+  StepArgs a = make_args(c, 0, 0, 0, 0);
+  StepArgs b = a;
+  set_state(b, c->st[c->cur ^ 1]);
+  hipLaunchKernelGGL(k_reorder_count, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a, mixed, c->d_keycnt);
+  hipLaunchKernelGGL(k_reorder_scan, dim3((nt + 63) / 64), dim3(64), 0, c->stream, c->d_tubes, nt, c->d_keycnt,
+                     c->d_cursor, c->d_cnt_new);
+  hipLaunchKernelGGL(k_reorder_scatter, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a, b, mixed, c->d_cursor);
+  hipLaunchKernelGGL(k_reorder_finish, dim3((nt + 63) / 64), dim3(64), 0, c->stream, nt, c->d_cnt_new, c->d_count);
+  HIPCHK(c, hipGetLastError());
+  c->cur ^= 1;
+  c->masters_since_reorder = 0;
+  std::vector<long long> cnt(nt);
+  HIPCHK(c, hipMemcpyAsync(cnt.data(), c->d_count, sizeof(long long) * nt, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int t = 0; t < nt; ++t) c->ht[t].count = cnt[t];
+  return KAOS_OK;
+}
+
+// Enqueue step n on the stream (no host synchronisation but at a reorder).
 static int enqueue_step(kaos_ctx* c, long long n, const double* d_uni) {
   const int nt = c->cfg.ntubes;
   const bool master = is_master(c, n);
@@ -848,7 +1006,8 @@ static int enqueue_step(kaos_ctx* c, long long n, const double* d_uni) {
                        c->d_M, c->d_M0F, c->d_EA, c->d_esc_acc, c->d_esc_rep, 0);
     hipLaunchKernelGGL(k_moments, dim3(c->nblk_mom), dim3(KAOS_BLOCK), 0, c->stream, c->d_tubes, c->d_blk_tube,
                        c->d_blk_q, c->d_d3x, c->d_d3v, c->d_dvpar, c->d_dvperp, c->d_vparGC, c->d_vperpGC,
-                       c->d_hist, c->d_nqoff, c->d_nqc, c->d_NqRT, c->d_M, c->d_count, (const long long*)nullptr);
+                       c->d_hist, c->d_nqoff, c->d_nqc, c->d_NqRT, c->d_M, c->d_count, c->d_next_id,
+                       (const long long*)nullptr);
     launch_push(c, a);
   } else {
     if (master) {
@@ -861,19 +1020,24 @@ static int enqueue_step(kaos_ctx* c, long long n, const double* d_uni) {
     if (master) {
       if (c->inj_total > 0) {
         int nb = (int)((c->inj_total + 255) / 256);
-        hipLaunchKernelGGL(k_inject, dim3(nb), dim3(256), 0, c->stream, a, c->d_inj_base, nt, c->d_EA,
+        hipLaunchKernelGGL(k_inject, dim3(nb), dim3(256), 0, c->stream, a, c->d_inj_base, c->d_next_id, nt, c->d_EA,
                            c->d_EG, c->d_vperpGH, d_uni, c->d_uni_base, (unsigned long long)c->cfg.seed,
                            (long long)n, c->inj_total, (int)(c->cfg.precision == KAOS_MIXED));
       }
       hipLaunchKernelGGL(k_moments, dim3(c->nblk_mom), dim3(KAOS_BLOCK), 0, c->stream, c->d_tubes, c->d_blk_tube,
                          c->d_blk_q, c->d_d3x, c->d_d3v, c->d_dvpar, c->d_dvperp, c->d_vparGC, c->d_vperpGC,
-                         c->d_hist, c->d_nqoff, c->d_nqc, c->d_NqRT, c->d_M, c->d_count, c->d_inj_cnt);
+                         c->d_hist, c->d_nqoff, c->d_nqc, c->d_NqRT, c->d_M, c->d_count, c->d_next_id,
+                         c->d_inj_cnt);
     }
   }
   HIPCHK(c, hipGetLastError());
   // host mirror of the slot counts
   if (master)
     for (int t = 0; t < nt; ++t) c->ht[t].count += c->inj_base_h[t + 1] - c->inj_base_h[t];
+  if (master && !first && reorder_period(c) > 0 && ++c->masters_since_reorder >= reorder_period(c)) {
+    int rc = enqueue_reorder(c);
+    if (rc) return rc;
+  }
   c->last_step = n;
   c->last_master = first ? 1 : (int)master;
   return KAOS_OK;
@@ -969,11 +1133,11 @@ extern "C" int kaos_count(kaos_ctx* c, int32_t tube, int64_t* n) {
   int64_t k = 0;
   if (c->cfg.precision == KAOS_MIXED) {
     std::vector<uint32_t> cw((size_t)h.count);
-    if (h.count) HIPCHK(c, hipMemcpy(cw.data(), c->d_cell + T.off, sizeof(uint32_t) * h.count, hipMemcpyDeviceToHost));
+    if (h.count) HIPCHK(c, hipMemcpy(cw.data(), c->st[c->cur].cell + T.off, sizeof(uint32_t) * h.count, hipMemcpyDeviceToHost));
     for (uint32_t w : cw) k += (w != KAOS_DEAD);
   } else {
     std::vector<double> rs((size_t)h.count);
-    if (h.count) HIPCHK(c, hipMemcpy(rs.data(), c->d_rs + T.off, sizeof(double) * h.count, hipMemcpyDeviceToHost));
+    if (h.count) HIPCHK(c, hipMemcpy(rs.data(), c->st[c->cur].rs + T.off, sizeof(double) * h.count, hipMemcpyDeviceToHost));
     for (double r : rs) k += (r != 0.0);
   }
   *n = k;
@@ -1031,6 +1195,7 @@ extern "C" int kaos_download_particles(kaos_ctx* c, int32_t tube, int64_t n_cap,
   const bool mixed = c->cfg.precision == KAOS_MIXED;
   std::vector<double> vr(n), qk(n), ae(n), agv(n), xyz(6 * (size_t)n);
   std::vector<uint32_t> cell(n);
+  std::vector<int> ids(n);
   std::vector<char> live(n, 0);
   if (n) {
     double* d6 = nullptr;
@@ -1042,10 +1207,12 @@ extern "C" int kaos_download_particles(kaos_ctx* c, int32_t tube, int64_t n_cap,
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(xyz.data(), d6, sizeof(double) * 6 * n, hipMemcpyDeviceToHost));
     (void)hipFree(d6);
-    HIPCHK(c, hipMemcpy(cell.data(), c->d_cell + T.off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    const StateBufs& B = c->st[c->cur];
+    HIPCHK(c, hipMemcpy(cell.data(), B.cell + T.off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(ids.data(), B.id + T.off, sizeof(int) * n, hipMemcpyDeviceToHost));
     if (mixed) {
       std::vector<float> f(n);
-      float* src[4] = {c->d_vrf, c->d_qkf, c->d_aef, c->d_agf};
+      float* src[4] = {B.vrf, B.qkf, B.aef, B.agf};
       std::vector<double>* dst[4] = {&vr, &qk, &ae, &agv};
       for (int i = 0; i < 4; ++i) {
         HIPCHK(c, hipMemcpy(f.data(), src[i] + T.off, sizeof(float) * n, hipMemcpyDeviceToHost));
@@ -1054,17 +1221,21 @@ extern "C" int kaos_download_particles(kaos_ctx* c, int32_t tube, int64_t n_cap,
       for (long long j = 0; j < n; ++j) live[j] = cell[j] != KAOS_DEAD;
     } else {
       std::vector<double> rs(n);
-      HIPCHK(c, hipMemcpy(rs.data(), c->d_rs + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
-      HIPCHK(c, hipMemcpy(vr.data(), c->d_vr + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
-      HIPCHK(c, hipMemcpy(qk.data(), c->d_qk + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
-      HIPCHK(c, hipMemcpy(ae.data(), c->d_ae + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
-      HIPCHK(c, hipMemcpy(agv.data(), c->d_ag + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(rs.data(), B.rs + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(vr.data(), B.vr + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(qk.data(), B.qk + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(ae.data(), B.ae + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(agv.data(), B.ag + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
       for (long long j = 0; j < n; ++j) live[j] = rs[j] != 0.0;
     }
   }
+  // live ions in the reference's order: by id (upload index, then injection order)
+  std::vector<long long> order;
+  order.reserve(n);
+  for (long long j = 0; j < n; ++j) if (live[j]) order.push_back(j);
+  std::sort(order.begin(), order.end(), [&](long long p, long long q) { return ids[p] < ids[q]; });
   long long k = 0;
-  for (long long j = 0; j < n; ++j) {
-    if (!live[j]) continue;
+  for (long long j : order) {
     if (k >= n_cap) return fail(c, KAOS_E_CAPACITY, "output arrays too small");
     if (x) x[k] = xyz[6 * j + 0];
     if (y) y[k] = xyz[6 * j + 1];
@@ -1120,11 +1291,15 @@ extern "C" int kaos_finalize(kaos_ctx* c) {
   void* bufs[] = {c->d_tubes, c->d_tile_tube, c->d_tile_start, c->d_blk_tube, c->d_blk_q, c->d_qGH, c->d_qGC,
                   c->d_hq, c->d_dq, c->d_Te, c->d_d3x, c->d_EA, c->d_EG, c->d_M, c->d_M0F, c->d_NqRT,
                   c->d_vparGH, c->d_vperpGH, c->d_vparGC, c->d_vperpGC, c->d_dvpar, c->d_dvperp, c->d_d3v,
-                  c->d_rs, c->d_vp, c->d_vr, c->d_qk, c->d_ae, c->d_ag, c->d_rdi, c->d_vpi, c->d_vrf,
-                  c->d_qkf, c->d_aef, c->d_agf, c->d_rGH, c->d_rref, c->d_rk, c->d_vk, c->d_cell, c->d_count, c->d_hist,
+                  c->d_rGH, c->d_rref, c->d_rk, c->d_vk, c->d_keycnt, c->d_cursor, c->d_cnt_new,
+                  c->d_next_id, c->d_count, c->d_hist,
                   c->d_nqc, c->d_nqoff, c->d_status, c->d_esc_acc, c->d_esc_rep, c->d_inj_base, c->d_inj_cnt,
                   c->d_uni_base, c->d_uni, c->d_pushed};
   for (void* p : bufs) if (p) (void)hipFree(p);
+  for (StateBufs& B : c->st) {
+    void* sb[] = {B.rs, B.vp, B.vr, B.qk, B.ae, B.ag, B.rdi, B.vpi, B.vrf, B.qkf, B.aef, B.agf, B.cell, B.id};
+    for (void* p : sb) if (p) (void)hipFree(p);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return KAOS_OK;

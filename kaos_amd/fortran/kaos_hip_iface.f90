@@ -21,6 +21,8 @@ module kaos_hip
   type, bind(c), public :: kaos_cfg
     integer(c_int32_t) :: ntubes, precision, ndatfac, device
     integer(c_int64_t) :: seed                   ! uint64_t on the C side
+    integer(c_int32_t) :: reorder_every = 0      ! 0: library default; < 0: never
+    integer(c_int32_t) :: reserved = 0
   end type kaos_cfg
 
   type, bind(c), public :: kaos_grid

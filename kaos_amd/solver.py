@@ -43,12 +43,13 @@ def grid_struct(tube):
 
 
 class KineticSolver:
-    def __init__(self, tubes, precision=KAOS_FP64, ndatfac=1, seed=1, capacity=None, device=0):
+    def __init__(self, tubes, precision=KAOS_FP64, ndatfac=1, seed=1, capacity=None, device=0,
+                 reorder_every=0):
         self.lib = _abi.load()
         self.tubes = list(tubes)
         self.ndatfac = ndatfac
         cfg = KaosCfg(ntubes=len(self.tubes), precision=precision, ndatfac=ndatfac, device=device,
-                      seed=seed)
+                      seed=seed, reorder_every=reorder_every)
         h = C.c_void_p()
         rc = self.lib.kaos_init(C.byref(cfg), C.byref(h))
         if rc:

@@ -30,11 +30,13 @@ def _l2rel(a, b):
     return np.linalg.norm(a - b) / n if n > 0 else np.linalg.norm(a - b)
 
 
-def _run(tube, nsteps, ndatfac=1, uniforms=False, ions_seed=3, seed=11, max_dcount=2, tol_l2=1e-4):
+def _run(tube, nsteps, ndatfac=1, uniforms=False, ions_seed=3, seed=11, max_dcount=2, tol_l2=1e-4,
+         reorder_every=0):
     q0, v0, w0 = grids.initial_ions(tube, seed=ions_seed)
     cap = len(q0) + 20000
     orc = oracle.Oracle(tube, q0, v0, w0, ndatfac=ndatfac, seed=seed, capacity=cap)
-    ks = KineticSolver([tube], precision=KAOS_MIXED, ndatfac=ndatfac, seed=seed, capacity=[cap])
+    ks = KineticSolver([tube], precision=KAOS_MIXED, ndatfac=ndatfac, seed=seed, capacity=[cap],
+                       reorder_every=reorder_every)
     ks.upload(0, q0, v0, w0)
     rng = np.random.default_rng(99)
     ninj = (tube.n_inj_lb if tube.lbcond else 0) + (tube.n_inj_ub if tube.ubcond else 0)
@@ -95,3 +97,7 @@ def test_mixed_vs_oracle_full_grid_1e6_ions():
     # the 256-cell, 64x64 grid of the full-size workload with 1e6 ions
     worst = _run(grids.config2(n_total=1_000_000), 10, max_dcount=6, tol_l2=1e-5)
     print("worst relative L2 errors:", {k: f"{v:.1e}" for k, v in worst.items()})
+
+
+def test_mixed_vs_oracle_with_reorder_every_master_step():
+    _run(grids.config1(n_total=20000), 12, reorder_every=1)

@@ -22,11 +22,13 @@ MASTER_KEYS = ("nq_counts", "nph_counts", "NqRT", "M0", "M1perp", "M1par", "M2",
                "M2par", "EA", "EG", "esc")
 
 
-def _run_pair(tube, nsteps, ndatfac=1, seed=11, ions_seed=3, uniforms=False, cap_extra=20000):
+def _run_pair(tube, nsteps, ndatfac=1, seed=11, ions_seed=3, uniforms=False, cap_extra=20000,
+              reorder_every=0):
     q0, v0, w0 = grids.initial_ions(tube, seed=ions_seed)
     cap = len(q0) + cap_extra
     orc = oracle.Oracle(tube, q0, v0, w0, ndatfac=ndatfac, seed=seed, capacity=cap)
-    ks = KineticSolver([tube], precision=KAOS_FP64, ndatfac=ndatfac, seed=seed, capacity=[cap])
+    ks = KineticSolver([tube], precision=KAOS_FP64, ndatfac=ndatfac, seed=seed, capacity=[cap],
+                       reorder_every=reorder_every)
     ks.upload(0, q0, v0, w0)
     rng = np.random.default_rng(99)
     ninj = (tube.n_inj_lb if tube.lbcond else 0) + (tube.n_inj_ub if tube.ubcond else 0)
@@ -74,4 +76,20 @@ def test_fp64_bitexact_south_hemisphere_eambsign0():
     tube.eambsign = 0
     tube.n_inj_ub = 5
     for n, master, orc, ks in _run_pair(tube, 9, ndatfac=2):
+        _compare(orc, ks, n, master)
+
+
+def test_fp64_bitexact_with_reorder_every_master_step():
+    # compaction + sort by cell after every master step: slot order changes,
+    # results and the downloaded (id-ordered) ions must not
+    tube = grids.config1(n_total=20000)
+    for n, master, orc, ks in _run_pair(tube, 12, reorder_every=1):
+        _compare(orc, ks, n, master)
+
+
+def test_fp64_bitexact_with_reorder_ndatfac3_uniform_stream():
+    tube = grids.dipole_tube(nq=40, nvpar=24, nvperp=20, n_total=15000, ratio=1.08)
+    tube.n_inj_ub = 17
+    tube.n_inj_lb = 23
+    for n, master, orc, ks in _run_pair(tube, 13, ndatfac=3, uniforms=True, reorder_every=1):
         _compare(orc, ks, n, master)
