@@ -430,3 +430,72 @@ def test_moments_single_cell_closed_forms(lib):
     assert abs(M[2, 6]) <= 1e-12 * abs(a["vparGC"][2])
     # empty cells give zero moments
     assert np.all(M[:, 0] == 0)
+
+
+# --- boundary conditions and injection (_8_0_7_BoundaryConditions.f90, KSB:195-793) ----
+
+def _dipole_B(r, p):
+    """|B| of the dipole on L-shell p at radius r: m sqrt(1 + 3 cos^2) / r^3."""
+    c2 = 1.0 - r / (p * RE)
+    return 8.06e15 * math.sqrt(1.0 + 3.0 * c2) / r ** 3
+
+
+@pytest.mark.parametrize("ndatfac", [1, 3])
+def test_escape_flux_is_count_weight_over_area_time(ndatfac):
+    """Escape flux = escaped ions x weight / (tube cross-section x time between
+    master steps); the cross-section of a flux tube goes as 1/B, so it is fixed
+    independently of d3x by the dipole field at the boundary cells.  Tolerance
+    1e-4: the reference's area d3x/(hq dq) uses the midpoint metric hq dq for
+    the cell's arc length (1e-5 close on these grids); a dropped ndatfac, h or
+    metric factor is off by 3x or more."""
+    t = grids.config1(n_total=20000)
+    q0, v0, w0 = grids.initial_ions(t, seed=8)
+    o = oracle.Oracle(t, q0, v0, w0, ndatfac=ndatfac, seed=2, capacity=60000)
+    seen = 0
+    rc = t.arrays["rGC"]
+    B0 = _dipole_B(rc[0], t.p)
+    for n in range(1, 4 * ndatfac + 2):
+        o.step(n)
+        if n < 2 or n % ndatfac:
+            continue
+        esc = o.master()["esc"]
+        for side, Q in ((0, 0), (1, t.nq - 1)):
+            area = 1e6 * B0 / _dipole_B(rc[Q], t.p)          # area0 * B(LB) / B(r)
+            want = esc[side] * t.nsnormfac / (area * ndatfac * t.h)
+            assert abs(esc[2 + side] - want) <= 1e-4 * max(abs(want), 1e-300)
+        seen += esc[0] + esc[1]
+    assert seen > 0
+    o.close()
+
+
+def test_injected_ions_at_boundary_cell_centre():
+    """Injected ions sit at qGC of the LB cell (root solve), are appended after
+    the survivors, and carry the LB cell index and the caller's uniforms' MB
+    velocities (KSB:203-326)."""
+    t = grids.dipole_tube(nq=40, nvpar=24, nvperp=20, n_total=3000, ratio=1.08)
+    t.n_inj_lb, t.n_inj_ub = 7, 5
+    q0, v0, w0 = grids.initial_ions(t, seed=9)
+    o = oracle.Oracle(t, q0, v0, w0, ndatfac=1, seed=2, capacity=10000)
+    lib = oracle.load()
+    o.step(1)
+    o.step(2)
+    before = o.count()
+    u = 1.0 - np.random.default_rng(4).random(6 * 12)
+    o.step(3, u)
+    P = o.particles()
+    assert o.count() <= before + 12
+    inj = {k: v[-12:] for k, v in P.items()}
+    for k, Q in ((slice(0, 7), 0), (slice(7, 12), t.nq - 1)):
+        r = lib.orc_root_r(t.arrays["qGC"][Q], t.p)
+        rr = np.sqrt(inj["x"][k] ** 2 + inj["y"][k] ** 2 + inj["z"][k] ** 2)
+        assert np.allclose(rr, r, rtol=1e-14, atol=0)
+        assert np.all(inj["qind"][k] == Q + 1)
+    # velocities: the MB projection of the caller's uniforms (first LB ion)
+    o4 = np.zeros(4)
+    sigma = math.sqrt(1.38e-23 * t.Ts_lb / t.ms)
+    rs = lib.orc_root_r(t.arrays["qGC"][0], t.p) * (1.0 if t.arrays["qGC"][0] > 0 else -1.0)
+    lib.orc_mb(ptr(u[:6]), sigma, t.cphi, t.sphi, rs, t.p, ptr(o4))
+    V = math.sqrt(inj["Vx"][0] ** 2 + inj["Vy"][0] ** 2 + inj["Vz"][0] ** 2)
+    assert abs(V - abs(o4[0])) <= 1e-12 * max(abs(o4[0]), 1.0)
+    assert abs(inj["vperp"][0] - o4[3]) <= 1e-12 * o4[3]
+    o.close()
