@@ -676,7 +676,8 @@ extern "C" int kaos_set_grid(kaos_ctx* c, int32_t tube, const kaos_grid* g, int6
     return fail(c, KAOS_E_ARG, "nvpar/nvperp must be in [1, 255]");
   if (g->filter_pt < 3 || (g->filter_pt % 2) == 0 || (g->filter_pt - 1) / 2 + 1 > g->nq / 2)
     return fail(c, KAOS_E_ARG, "filter_pt must be odd, >= 3 and fit the grid");
-  if (capacity < 0) return fail(c, KAOS_E_ARG, "negative capacity");
+  if (capacity < 0 || capacity >= (1LL << 28))
+    return fail(c, KAOS_E_ARG, "capacity must be in [0, 2^28) slots per tube");
   const double* need[] = {g->qGL, g->qGH, g->qGC, g->hqC, g->dqC, g->d3xC, g->rGC, g->cthGC, g->ellGC,
                           g->Te, g->vparGL, g->vparGH, g->vparGC, g->dvpar, g->vperpGL, g->vperpGH,
                           g->vperpGC, g->dvperp, g->d3v};

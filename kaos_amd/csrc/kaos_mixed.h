@@ -35,6 +35,15 @@
 // increment dV and the tiny rotation term.
 
 #define KAOS_DEAD 0xFFFFFFFFu
+
+// The mixed form is not bit-exact by design, so its kernel body lets the
+// compiler fuse multiply-adds (the FP64 form and the shared header stay
+// uncontracted for bit-exactness against the oracle).
+#ifndef KAOS_MIXED_NO_CONTRACT
+#define KAOS_MIXED_CONTRACT _Pragma("clang fp contract(fast)")
+#else
+#define KAOS_MIXED_CONTRACT
+#endif
 #define KAOS_RFLAG (1u << 30)
 #define KAOS_VFLAG (1u << 29)
 #define KAOS_QMASK 0x3FFu
@@ -61,12 +70,19 @@ __device__ __forceinline__ double kx_dec(int w, int k, bool fallback) {
   return fallback ? (double)__int_as_float(w) : ldexp((double)w, -k);
 }
 
+// Element at byte offset `boff` of a per-tube base pointer: the base is uniform
+// and the offset a 32-bit lane value, so loads and stores take the scalar-base
+// + 32-bit offset addressing form (a tube segment holds < 2^28 slots).
+template <typename T>
+__device__ __forceinline__ T& kx_at(T* base, uint32_t boff) { return *(T*)((char*)base + boff); }
+
 // Reference cell (1-based) of a cell-word config index qc: 0 -> 1, -1 -> nq.
 __device__ __forceinline__ int kx_refq(int qc, int nq) { return (qc >= 1) ? qc : (qc == 0 ? 1 : nq); }
 
 // Parallel acceleration (signs as kaos_apar; qle0 = stage q <= 0).
 __device__ __forceinline__ float kf_apar(bool qle0, float AG, float AEA, int grav, int eamb,
                                          int eambsign, bool with_grav) {
+  KAOS_MIXED_CONTRACT
   float a = 0.0f;
   if (grav && with_grav) a = qle0 ? fabsf(AG) : -fabsf(AG);
   if (eamb) {
@@ -78,6 +94,7 @@ __device__ __forceinline__ float kf_apar(bool qle0, float AG, float AEA, int gra
 
 // b-hat at (rho, z) off the line, as kaos_bhat_pt.
 __device__ __forceinline__ void kf_bhat_pt(float rho, float z, float& bR, float& bZ) {
+  KAOS_MIXED_CONTRACT
   float z2 = z * z;
   float u2 = rho * rho + z2;
   float d = kf_rsq(u2 * (u2 + 3.0f * z2));
@@ -104,6 +121,7 @@ __device__ __forceinline__ int kf_find_rcell_from(double r, const double* rGH, d
 __device__ __forceinline__ int kf_interp2_from(float q, const float* qGC, const float* E, const float* E2,
                                                int nq, float qGL0, float qGHlast, int desc, int hint,
                                                float& out, float& out2) {
+  KAOS_MIXED_CONTRACT
   int ub = nq - 1;
   int lo = hint - 1;
   if (lo < 1) lo = 1;
@@ -170,6 +188,7 @@ __device__ __forceinline__ double kx_dec_v(int w, int ip, bool vflag, const doub
 #define KAOS_MIXED_MIN_BLOCKS 4
 #endif
 __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixed(StepArgs a) {
+  KAOS_MIXED_CONTRACT
   extern __shared__ double s_dyn[];
   const int NQ = a.lds_nq, NV = a.lds_nv;
   double* s_rGH = s_dyn;
@@ -211,8 +230,12 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
   const int wq0 = s_wq0, wnq = s_wnq, wtube = s_wtube;
   int cur = -1;
   unsigned int npushed = 0;
-  // per-tube scalars
-  long long end = 0, hoff = 0, goff = 0;
+  // per-tube scalars and base pointers
+  long long hoff = 0, goff = 0, toff = 0;
+  uint32_t tcount = 0;
+  uint32_t* cellT = nullptr;
+  int *rdiT = nullptr, *vpiT = nullptr;
+  float *qkfT = nullptr, *aefT = nullptr, *agfT = nullptr;
   int nq = 0, nvpar = 0, nvperp = 0, desc = 0, grav = 0, eamb = 0, eambsign = 0;
   double rGL0 = 0.0, rGHlast = 0.0, rsgn = 1.0;
   float qGL0 = 0.0f, qGHlast = 0.0f, qsms = 0.0f;
@@ -239,41 +262,45 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
       }
       for (int i = tid; i <= nvpar; i += KAOS_BLOCK) s_vGH[i] = a.vparGH[T.voff + i];
       for (int i = tid; i < nvpar; i += KAOS_BLOCK) s_vk[i] = a.vk[T.voff + i];
-      end = T.off + a.count[tb];
+      toff = T.off;
+      tcount = (uint32_t)a.count[tb];
+      cellT = a.cell + toff; rdiT = a.rdi + toff; vpiT = a.vpi + toff;
+      qkfT = a.qkf + toff; aefT = a.aef + toff; agfT = a.agf + toff;
       vscale = (double)nvpar / (a.vparGH[T.voff + nvpar - 1] - a.vparGH[T.voff + nvpar]);
       cur = tb;
       __syncthreads();
     }
-    const long long start = a.tile_start[tile];
+    const uint32_t start = (uint32_t)(a.tile_start[tile] - toff);
 #pragma unroll 1
     for (int it = 0; it < KAOS_PPT; ++it) {
-      long long j = start + (long long)it * KAOS_BLOCK + tid;
-      if (j >= end) break;
-      uint32_t cold = a.cell[j];
+      const uint32_t jl = start + (uint32_t)(it * KAOS_BLOCK + tid);
+      if (jl >= tcount) break;
+      const uint32_t b4 = jl * 4u;
+      uint32_t cold = kx_at(cellT, b4);
       if (cold == KAOS_DEAD) continue;
       int qprev = (int)((cold >> 16) & KAOS_QMASK) - 1;
       if (a.do_remove && (qprev == 0 || qprev == -1)) {
-        a.cell[j] = KAOS_DEAD;
+        kx_at(cellT, b4) = KAOS_DEAD;
         atomicAdd(&a.esc_acc[2 * cur + (qprev == 0 ? 0 : 1)], 1ull);
         continue;
       }
       ++npushed;
       const int hint = kx_refq(qprev, nq);
       const int ipc = (int)(cold & 0xFFu), iq = (int)((cold >> 8) & 0xFFu);
-      const double rs = s_rref[hint - 1] + kx_dec(a.rdi[j], s_rk[hint - 1], (cold & KAOS_RFLAG) != 0);
-      const double vs64 = kx_dec_v(a.vpi[j], ipc, (cold & KAOS_VFLAG) != 0, s_vGH, s_vk, nvpar);
+      const double rs = s_rref[hint - 1] + kx_dec(kx_at(rdiT, b4), s_rk[hint - 1], (cold & KAOS_RFLAG) != 0);
+      const double vs64 = kx_dec_v(kx_at(vpiT, b4), ipc, (cold & KAOS_VFLAG) != 0, s_vGH, s_vk, nvpar);
       float aea, ag;
       if (a.do_interp) {
         float ea, eg;
-        if (kf_interp2_from(a.qkf[j], s_qGC, s_EA, s_EG, nq, qGL0, qGHlast, desc, hint, ea, eg)) {
+        if (kf_interp2_from(kx_at(qkfT, b4), s_qGC, s_EA, s_EG, nq, qGL0, qGHlast, desc, hint, ea, eg)) {
           aea = kf_aea_of(ea, qsms, eambsign);
           ag = kf_ag_of(eg, qsms);
-          a.aef[j] = aea; a.agf[j] = ag;
+          kx_at(aefT, b4) = aea; kx_at(agfT, b4) = ag;
         } else {
-          aea = a.aef[j]; ag = a.agf[j];
+          aea = kx_at(aefT, b4); ag = kx_at(agfT, b4);
         }
       } else {
-        aea = a.aef[j]; ag = a.agf[j];
+        aea = kx_at(aefT, b4); ag = kx_at(agfT, b4);
       }
       // stage A on the line: cell from |r| (binary64); sin, cos, b-hat to ~1e-14
       const double r = fabs(rs);
@@ -348,15 +375,15 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
       if (a.do_qk) {
         const float rhoC = rho + dRC;
         const float u2C = rhoC * rhoC + zC * zC;
-        a.qkf[j] = (((float)KAOS_RE2) * zC) * (kf_rsq(u2C) * kf_rcp(u2C));
+        kx_at(qkfT, b4) = (((float)KAOS_RE2) * zC) * (kf_rsq(u2C) * kf_rcp(u2C));
       }
       // next push's Vparind (stale when off the grid) and the new state words
       const int ipn = kaos_find_vcell_guess(vN, s_vGH, s_vGH[nvpar], nvpar, vscale);
       bool rflag, vflag;
       const int Qn = kx_refq(qc, nq);
-      a.rdi[j] = kx_enc(rsN - s_rref[Qn - 1], s_rk[Qn - 1], rflag);
-      a.vpi[j] = kx_enc_v(vN, ipn, s_vGH, s_vk, nvpar, vflag);
-      a.cell[j] = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)(ipn >= 1 ? ipn : ipc) |
+      kx_at(rdiT, b4) = kx_enc(rsN - s_rref[Qn - 1], s_rk[Qn - 1], rflag);
+      kx_at(vpiT, b4) = kx_enc_v(vN, ipn, s_vGH, s_vk, nvpar, vflag);
+      kx_at(cellT, b4) = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)(ipn >= 1 ? ipn : ipc) |
                   (rflag ? KAOS_RFLAG : 0u) | (vflag ? KAOS_VFLAG : 0u);
     }
   }
