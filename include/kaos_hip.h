@@ -46,8 +46,13 @@ typedef struct kaos_cfg {
   uint64_t seed;         /* seed of the injection stream (used when no uniforms are passed) */
   int32_t reorder_every; /* master steps between on-device compaction + sort of the ions by
                             cell (0 = library default, < 0 = never); results do not depend on it */
-  int32_t reserved;      /* 0 */
+  int32_t flags;         /* KAOS_FLAG_* (0 = defaults: autotuned launches, graph replay in kaos_run) */
 } kaos_cfg;
+
+#define KAOS_FLAG_NO_GRAPH     1   /* kaos_run enqueues every step eagerly instead of replaying a
+                                      captured hipGraph of the step's kernels */
+#define KAOS_FLAG_NO_AUTOTUNE  2   /* keep the default launch shape instead of timing candidates
+                                      on the first master steps */
 
 /* One flux tube: grid, species and flags.  Replaces the FluxTubeT/QCellT/VCellT
  * grid fields (_KineticMainParams.f90:397-520) and the per-tube flags. */
@@ -147,6 +152,18 @@ int kaos_download_particles(kaos_ctx* ctx, int32_t tube, int64_t n_cap, int64_t*
                             double* x, double* y, double* z, double* Vx, double* Vy, double* Vz,
                             double* vperp, double* qk4, double* aea, double* ag,
                             int32_t* qind, int32_t* vparind, int32_t* vperpind);
+
+/* Launch shape of the push kernels: ions are processed in tiles of 1024, a
+ * group of `tiles_per_group` tiles (1..63) shares one block-local LDS window of
+ * `hist_words` 32-bit words (two 16-bit phase-space counters each; 0 = all
+ * counts go to global atomics); `persistent` != 0 launches a few blocks per CU
+ * that take groups from a counter, else one block per group.  Setting it ends
+ * the autotuner.  Results do not depend on the launch shape.  No counterpart in
+ * the reference (its loop is serial, KSB:139-1041). */
+int kaos_set_launch(kaos_ctx* ctx, int32_t tiles_per_group, int32_t hist_words, int32_t persistent);
+/* The launch shape in use and whether the autotuner is still timing candidates. */
+int kaos_get_launch(kaos_ctx* ctx, int32_t* tiles_per_group, int32_t* hist_words, int32_t* persistent,
+                    int32_t* tuning);
 
 /* Kernel timing: when enabled, hipEvents on the library's stream bracket every
  * push launch and every kaos_run call.  kaos_profile_read returns the summed

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libkaos_hip.so")
 
 KAOS_OK, KAOS_E_ARG, KAOS_E_CAPACITY, KAOS_E_DEVICE, KAOS_E_STATE, KAOS_E_NAN = range(6)
 KAOS_FP64, KAOS_MIXED = 0, 1
+KAOS_FLAG_NO_GRAPH, KAOS_FLAG_NO_AUTOTUNE = 1, 2
 
 _dp = C.POINTER(C.c_double)
 
@@ -20,7 +21,7 @@ _dp = C.POINTER(C.c_double)
 class KaosCfg(C.Structure):
     _fields_ = [("ntubes", C.c_int32), ("precision", C.c_int32), ("ndatfac", C.c_int32),
                 ("device", C.c_int32), ("seed", C.c_uint64), ("reorder_every", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("flags", C.c_int32)]
 
 
 GRID_ARRAYS = ("qGL", "qGH", "qGC", "hqC", "dqC", "d3xC", "rGC", "cthGC", "ellGC", "Te",
@@ -43,8 +44,8 @@ class KaosStepOut(C.Structure):
 
 
 EXPORTS = ("kaos_init", "kaos_set_grid", "kaos_upload_particles", "kaos_step", "kaos_run",
-           "kaos_download_master", "kaos_count", "kaos_download_particles", "kaos_profile",
-           "kaos_profile_read", "kaos_last_error", "kaos_finalize")
+           "kaos_download_master", "kaos_count", "kaos_download_particles", "kaos_set_launch",
+           "kaos_get_launch", "kaos_profile", "kaos_profile_read", "kaos_last_error", "kaos_finalize")
 
 _lib = None
 
@@ -68,6 +69,8 @@ def load(path=LIB_PATH):
     lib.kaos_download_master.argtypes = [P, C.c_int32, i64p, i64p, _dp, _dp, _dp, _dp, _dp]
     lib.kaos_count.argtypes = [P, C.c_int32, i64p]
     lib.kaos_download_particles.argtypes = [P, C.c_int32, C.c_int64, i64p] + [_dp] * 10 + [i32p] * 3
+    lib.kaos_set_launch.argtypes = [P, C.c_int32, C.c_int32, C.c_int32]
+    lib.kaos_get_launch.argtypes = [P, i32p, i32p, i32p, i32p]
     lib.kaos_profile.argtypes = [P, C.c_int32]
     lib.kaos_profile_read.argtypes = [P, _dp, i64p, _dp]
     lib.kaos_last_error.argtypes = [P]

@@ -27,6 +27,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <map>
 
 #include "kaos_dipole.h"
 #include "../../include/kaos_hip.h"
@@ -77,9 +78,26 @@ struct StepArgs {
   unsigned int* status;
   unsigned long long* pushed;                // live ions pushed (particle-steps)
   int do_interp, do_count, do_remove, do_qk;
+  // launch shape: tiles per group, LDS histogram window (32-bit words), persistent
+  // blocks taking groups from the counter `work` (else one group per block)
+  int hwords, persistent, ngroups;
+  unsigned int* work;
 };
 
 __device__ __forceinline__ void set_status(unsigned int* st, unsigned int bit) { atomicOr(st, bit); }
+
+// Work distribution of the push kernels: group g is tiles [g tpb, g tpb + tpb).
+// Grid launches give block b group b; persistent launches (a few blocks per
+// CU) take groups from an atomic counter until none is left, so every block
+// exits once the groups are drained.
+__device__ __forceinline__ int kaos_next_group(const StepArgs& a, int pass, int* s_group) {
+  if (!a.persistent) return (pass == 0 && (int)blockIdx.x < a.ngroups) ? (int)blockIdx.x : -1;
+  __syncthreads();
+  if (threadIdx.x == 0) *s_group = (int)atomicAdd(a.work, 1u);
+  __syncthreads();
+  int g = *s_group;
+  return (g < a.ngroups) ? g : -1;
+}
 
 #include "kaos_mixed.h"
 
@@ -105,23 +123,27 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_FP64_MIN_BLOCKS) k_push_fp64(
   double* s_vparGH = s_EG + a.lds_nq;
   unsigned int* s_hist = (unsigned int*)(s_vparGH + a.lds_nv);
   __shared__ unsigned int s_pushed;
-  __shared__ int s_wq0, s_wnq, s_wtube;
+  __shared__ int s_wq0, s_wnq, s_wtube, s_group;
   const int tid = threadIdx.x;
-  int t0 = blockIdx.x * a.tiles_per_block;
-  int t1 = min(t0 + a.tiles_per_block, a.ntiles);
   int cur = -1;
   DevTube T;
   long long end = 0;
+  double vpar_scale = 0.0, vparGL0 = 0.0;
+  for (int pass = 0;; ++pass) {
+  const int g = kaos_next_group(a, pass, &s_group);
+  if (g < 0) break;
+  int t0 = g * a.tiles_per_block;
+  int t1 = min(t0 + a.tiles_per_block, a.ntiles);
   unsigned int npushed = 0;
   if (tid == 0) {
     s_pushed = 0u;
-    // histogram window: config cells [wq0, wq0 + wnq) of the block's first tube,
+    // histogram window: config cells [wq0, wq0 + wnq) of the group's first tube,
     // starting one cell below the first ion's cell of the previous step (ions are
     // stored near cell order and move less than a cell per step)
     int tb0 = (t0 < t1) ? a.tile_tube[t0] : 0;
     DevTube T0 = a.tubes[tb0];
     int ncell = T0.nvpar * T0.nvperp;
-    int wnq = a.do_count ? (2 * KAOS_HWORDS) / ncell : 0;
+    int wnq = a.do_count ? (2 * a.hwords) / ncell : 0;
     if (wnq > T0.nq) wnq = T0.nq;
     int q0 = 1;
     if (t0 < t1 && wnq > 0) {
@@ -135,10 +157,9 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_FP64_MIN_BLOCKS) k_push_fp64(
     }
     s_wq0 = q0; s_wnq = wnq; s_wtube = tb0;
   }
-  for (int i = tid; i < KAOS_HWORDS; i += KAOS_BLOCK) s_hist[i] = 0u;
+  for (int i = tid; i < a.hwords; i += KAOS_BLOCK) s_hist[i] = 0u;
   __syncthreads();
   const int wq0 = s_wq0, wnq = s_wnq, wtube = s_wtube;
-  double vpar_scale = 0.0, vparGL0 = 0.0;
   for (int tile = t0; tile < t1; ++tile) {
     int tb = a.tile_tube[tile];
     if (tb != cur) {
@@ -233,6 +254,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_FP64_MIN_BLOCKS) k_push_fp64(
   }
   __syncthreads();
   if (tid == 0 && s_pushed) atomicAdd(a.pushed, (unsigned long long)s_pushed);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -299,6 +321,10 @@ __global__ void k_upload(DevTube T, long long n, const double* q0, const double*
   }
 }
 
+// Step counter on the device (the injection stream is keyed by the step), so
+// that a captured step replays unchanged.
+__global__ void k_step_tick(long long* stepno) { *stepno += 1; }
+
 __global__ void k_fill_u32(unsigned int* p, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long st = (long long)gridDim.x * blockDim.x;
@@ -334,7 +360,7 @@ __global__ void k_field(const DevTube* tubes, const double* hq, const double* dq
 // from the seeded Philox stream.
 __global__ void k_inject(StepArgs a, const long long* inj_base, const long long* next_id, int ntubes,
                          const double* EA, const double* EG, const double* vperpGH, const double* uni,
-                         const long long* uni_base, unsigned long long seed, long long step,
+                         const long long* uni_base, unsigned long long seed, const long long* stepno,
                          long long total, int mixed) {
   long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gi >= total) return;
@@ -349,7 +375,7 @@ __global__ void k_inject(StepArgs a, const long long* inj_base, const long long*
   double qc = a.qGC[T.goff + Qb];
   double u[6];
   if (uni) { for (int i = 0; i < 6; ++i) u[i] = uni[6 * (uni_base[t] + k) + i]; }
-  else kaos_inject_uniforms(seed, step, t, k, u);
+  else kaos_inject_uniforms(seed, *stepno, t, k, u);
   int qcell = kaos_find_qcell(qc, a.qGH + T.goff, T.qGL0, T.qGHlast, T.nq, T.desc);
   double r = kaos_rs_of_q(qc, T.p);
   kaos_line_t L = kaos_line(r, T.p);
@@ -620,6 +646,17 @@ struct kaos_ctx {
   unsigned long long* d_pushed = nullptr;
   // kernel timing (hipEvents on the library stream around each push launch)
   bool prof_on = false;
+  // launch shape of the push kernels, the online autotuner and captured steps
+  struct Launch { int tpb, hwords, persistent; };
+  Launch launch = {KAOS_TPB, KAOS_HWORDS, 0};
+  unsigned int* d_work = nullptr;
+  long long* d_stepno = nullptr;
+  int n_cu = 256;
+  bool tuning = false;
+  size_t tune_i = 0;
+  std::vector<Launch> tune_cand;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> tune_ev;
+  std::map<unsigned long long, hipGraphExec_t> graphs;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_push, ev_step;
 };
 
@@ -654,6 +691,15 @@ extern "C" int kaos_init(const kaos_cfg* cfg, kaos_ctx** out) {
   if (hipSetDevice(cfg->device) != hipSuccess) { g_init_err = "hipSetDevice failed"; return KAOS_E_DEVICE; }
   kaos_ctx* c = new kaos_ctx();
   c->cfg = *cfg;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, cfg->device) == hipSuccess && ncu > 0)
+    c->n_cu = ncu;
+  if (!(cfg->flags & KAOS_FLAG_NO_AUTOTUNE)) {
+    c->tuning = true;
+    for (int pers = 0; pers < 2; ++pers)
+      for (int hw : {4096, 6144})
+        for (int tpb : {8, 16, 32}) c->tune_cand.push_back({tpb, hw, pers});
+  }
   c->ht.resize(cfg->ntubes);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c; g_init_err = "hipStreamCreate failed"; return KAOS_E_DEVICE;
@@ -848,6 +894,9 @@ static int build_device(kaos_ctx* c) {
   HIPCHK(c, hipMemset(c->d_hist, 0, sizeof(unsigned int) * (size_t)(hoff ? hoff : 1)));
   HIPCHK(c, hipMemset(c->d_nqc, 0, sizeof(unsigned int) * (size_t)goff));
   HIPCHK(c, hipMemset(c->d_nqoff, 0, sizeof(unsigned int) * (size_t)goff));
+  HIPCHK(c, hipMalloc((void**)&c->d_work, sizeof(unsigned int)));
+  HIPCHK(c, hipMalloc((void**)&c->d_stepno, sizeof(long long)));
+  HIPCHK(c, hipMemset(c->d_stepno, 0, sizeof(long long)));
   HIPCHK(c, hipMalloc((void**)&c->d_pushed, sizeof(unsigned long long)));
   HIPCHK(c, hipMemset(c->d_pushed, 0, sizeof(unsigned long long)));
   HIPCHK(c, hipMalloc((void**)&c->d_status, sizeof(unsigned int)));
@@ -881,7 +930,6 @@ static StepArgs make_args(kaos_ctx* c, int interp, int count, int remove, int qk
   StepArgs a;
   a.tubes = c->d_tubes; a.tile_tube = c->d_tile_tube; a.tile_start = c->d_tile_start;
   a.ntiles = c->ntiles;
-  a.tiles_per_block = KAOS_TPB;
   a.qGH = c->d_qGH; a.qGC = c->d_qGC; a.EA = c->d_EA; a.EG = c->d_EG;
   a.vparGH = c->d_vparGH; a.vperpGH = c->d_vperpGH;
   set_state(a, c->st[c->cur]);
@@ -889,6 +937,11 @@ static StepArgs make_args(kaos_ctx* c, int interp, int count, int remove, int qk
   a.lds_nq = c->lds_nq; a.lds_nv = c->lds_nv;
   a.count = c->d_count; a.hist = c->d_hist; a.nqoff = c->d_nqoff;
   a.esc_acc = c->d_esc_acc; a.status = c->d_status; a.pushed = c->d_pushed;
+  a.tiles_per_block = c->launch.tpb;
+  a.hwords = c->launch.hwords;
+  a.persistent = c->launch.persistent;
+  a.ngroups = (c->ntiles + c->launch.tpb - 1) / c->launch.tpb;
+  a.work = c->d_work;
   a.do_interp = interp; a.do_count = count; a.do_remove = remove; a.do_qk = qk;
   return a;
 }
@@ -939,25 +992,65 @@ static std::pair<hipEvent_t, hipEvent_t> ev_pair(kaos_ctx* c) {
 }
 
 static size_t fp64_lds_bytes(const kaos_ctx* c) {
-  return sizeof(double) * (4 * (size_t)c->lds_nq + c->lds_nv) + sizeof(unsigned int) * KAOS_HWORDS;
+  return sizeof(double) * (4 * (size_t)c->lds_nq + c->lds_nv) + sizeof(unsigned int) * c->launch.hwords;
 }
 
 // Dynamic LDS of k_push_mixed: binary64 rGH, rref, vparGH + GL0; binary32
 // qGC, EA, EG; the int exponents; the histogram window.
 static size_t mixed_lds_bytes(const kaos_ctx* c) {
   return sizeof(double) * (2 * (size_t)c->lds_nq + c->lds_nv + 1) + sizeof(float) * 3 * (size_t)c->lds_nq +
-         sizeof(int) * ((size_t)c->lds_nq + c->lds_nv) + sizeof(unsigned int) * KAOS_HWORDS;
+         sizeof(int) * ((size_t)c->lds_nq + c->lds_nv) + sizeof(unsigned int) * c->launch.hwords;
 }
 
-static void launch_push(kaos_ctx* c, const StepArgs& a) {
-  int nblocks = (c->ntiles + a.tiles_per_block - 1) / a.tiles_per_block;
+// Persistent launches: this many blocks per CU (the push kernels' launch bounds).
+#define KAOS_PERSIST_BLOCKS_PER_CU 4
+
+static void launch_push(kaos_ctx* c, const StepArgs& a, hipEvent_t tune0 = nullptr, hipEvent_t tune1 = nullptr) {
+  int nblocks = a.ngroups;
+  if (a.persistent) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, c->d_work, 1LL);
+    nblocks = std::min(nblocks, KAOS_PERSIST_BLOCKS_PER_CU * c->n_cu);
+  }
   std::pair<hipEvent_t, hipEvent_t> ev;
   if (c->prof_on) ev = ev_pair(c);
+  if (tune0) (void)hipEventRecord(tune0, c->stream);
   if (c->cfg.precision == KAOS_MIXED)
     hipLaunchKernelGGL(k_push_mixed, dim3(nblocks), dim3(KAOS_BLOCK), mixed_lds_bytes(c), c->stream, a);
   else
     hipLaunchKernelGGL(k_push_fp64, dim3(nblocks), dim3(KAOS_BLOCK), fp64_lds_bytes(c), c->stream, a);
+  if (tune1) (void)hipEventRecord(tune1, c->stream);
   if (c->prof_on) { (void)hipEventRecord(ev.second, c->stream); c->ev_push.push_back(ev); }
+}
+
+// Online autotuner: the first master steps each run the push with one
+// candidate launch shape (tiles per group, histogram window, grid or
+// persistent); the next master step reads the timings and keeps the fastest.
+// Results do not depend on the launch shape (integer histograms, independent
+// ions), so tuning steps are ordinary steps.
+static int tune_before_push(kaos_ctx* c, hipEvent_t* e0, hipEvent_t* e1) {
+  *e0 = *e1 = nullptr;
+  if (c->tune_i < c->tune_cand.size()) {
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    HIPCHK(c, hipEventCreate(&ev.first));
+    HIPCHK(c, hipEventCreate(&ev.second));
+    c->tune_ev.push_back(ev);
+    c->launch = c->tune_cand[c->tune_i++];
+    *e0 = ev.first; *e1 = ev.second;
+    return KAOS_OK;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  float best = 0.f;
+  size_t ib = 0;
+  for (size_t i = 0; i < c->tune_ev.size(); ++i) {
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->tune_ev[i].first, c->tune_ev[i].second));
+    if (i == 0 || ms < best) { best = ms; ib = i; }
+    (void)hipEventDestroy(c->tune_ev[i].first); (void)hipEventDestroy(c->tune_ev[i].second);
+  }
+  c->launch = c->tune_cand[ib];
+  c->tune_ev.clear();
+  c->tuning = false;
+  return KAOS_OK;
 }
 
 // Master steps between reorders: cfg.reorder_every, 0 = the default, < 0 = never.
@@ -988,12 +1081,13 @@ static int enqueue_reorder(kaos_ctx* c) {
   return KAOS_OK;
 }
 
-// Enqueue step n on the stream (no host synchronisation but at a reorder).
-static int enqueue_step(kaos_ctx* c, long long n, const double* d_uni) {
+// The kernels of step n, in stream order (captured as one graph by kaos_run).
+static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
   const int nt = c->cfg.ntubes;
   const bool master = is_master(c, n);
   const bool first = (n == 1);
   const bool premaster = is_master(c, n + 1);
+  hipLaunchKernelGGL(k_step_tick, dim3(1), dim3(1), 0, c->stream, c->d_stepno);
   auto zero_hist = [&]() {
     hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, c->stream, c->d_hist, c->nhist_tot);
     hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, c->stream, c->d_nqoff, c->ncell_tot);
@@ -1016,14 +1110,19 @@ static int enqueue_step(kaos_ctx* c, long long n, const double* d_uni) {
                          c->d_M, c->d_M0F, c->d_EA, c->d_esc_acc, c->d_esc_rep, 1);
       zero_hist();
     }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->tuning && master) {
+      int rc = tune_before_push(c, &e0, &e1);
+      if (rc) return rc;
+    }
     StepArgs a = make_args(c, master, master, master, premaster);
-    launch_push(c, a);
+    launch_push(c, a, e0, e1);
     if (master) {
       if (c->inj_total > 0) {
         int nb = (int)((c->inj_total + 255) / 256);
         hipLaunchKernelGGL(k_inject, dim3(nb), dim3(256), 0, c->stream, a, c->d_inj_base, c->d_next_id, nt, c->d_EA,
                            c->d_EG, c->d_vperpGH, d_uni, c->d_uni_base, (unsigned long long)c->cfg.seed,
-                           (long long)n, c->inj_total, (int)(c->cfg.precision == KAOS_MIXED));
+                           c->d_stepno, c->inj_total, (int)(c->cfg.precision == KAOS_MIXED));
       }
       hipLaunchKernelGGL(k_moments, dim3(c->nblk_mom), dim3(KAOS_BLOCK), 0, c->stream, c->d_tubes, c->d_blk_tube,
                          c->d_blk_q, c->d_d3x, c->d_d3v, c->d_dvpar, c->d_dvperp, c->d_vparGC, c->d_vperpGC,
@@ -1032,6 +1131,14 @@ static int enqueue_step(kaos_ctx* c, long long n, const double* d_uni) {
     }
   }
   HIPCHK(c, hipGetLastError());
+  return KAOS_OK;
+}
+
+// Host bookkeeping after the kernels of step n: slot counts, the reorder.
+static int finish_step(kaos_ctx* c, long long n) {
+  const int nt = c->cfg.ntubes;
+  const bool master = is_master(c, n);
+  const bool first = (n == 1);
   // host mirror of the slot counts
   if (master)
     for (int t = 0; t < nt; ++t) c->ht[t].count += c->inj_base_h[t + 1] - c->inj_base_h[t];
@@ -1042,6 +1149,37 @@ static int enqueue_step(kaos_ctx* c, long long n, const double* d_uni) {
   c->last_step = n;
   c->last_master = first ? 1 : (int)master;
   return KAOS_OK;
+}
+
+// Enqueue step n eagerly (no host synchronisation but at a reorder).
+static int enqueue_step(kaos_ctx* c, long long n, const double* d_uni) {
+  int rc = enqueue_step_kernels(c, n, d_uni);
+  if (rc) return rc;
+  return finish_step(c, n);
+}
+
+// Enqueue step n as a replay of a captured graph of its kernels.  The graph
+// depends on the step kind (master, next-step master), the live state buffer
+// and the launch shape; it is captured the first time that combination occurs.
+static int enqueue_step_graph(kaos_ctx* c, long long n) {
+  unsigned long long key = (unsigned long long)is_master(c, n) | ((unsigned long long)is_master(c, n + 1) << 1) |
+                           ((unsigned long long)c->cur << 2) | ((unsigned long long)c->launch.persistent << 3) |
+                           ((unsigned long long)c->launch.tpb << 4) | ((unsigned long long)c->launch.hwords << 20);
+  auto it = c->graphs.find(key);
+  if (it == c->graphs.end()) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step_kernels(c, n, nullptr);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    HIPCHK(c, e);
+    HIPCHK(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    it = c->graphs.emplace(key, ge).first;
+  }
+  HIPCHK(c, hipGraphLaunch(it->second, c->stream));
+  return finish_step(c, n);
 }
 
 static int check_next(kaos_ctx* c, long long n) {
@@ -1108,9 +1246,11 @@ extern "C" int kaos_run(kaos_ctx* c, int64_t n_first, int64_t nsteps, int64_t* p
   std::pair<hipEvent_t, hipEvent_t> ev;
   if (c->prof_on) ev = ev_pair(c);
   for (int64_t i = 0; i < nsteps; ++i) {
-    int rc = check_next(c, n_first + i);
+    const long long n = n_first + i;
+    int rc = check_next(c, n);
     if (rc) return rc;
-    rc = enqueue_step(c, n_first + i, nullptr);
+    const bool graph = !(c->cfg.flags & KAOS_FLAG_NO_GRAPH) && !c->prof_on && !c->tuning && n > 1;
+    rc = graph ? enqueue_step_graph(c, n) : enqueue_step(c, n, nullptr);
     if (rc) return rc;
   }
   if (c->prof_on) { (void)hipEventRecord(ev.second, c->stream); c->ev_step.push_back(ev); }
@@ -1257,6 +1397,28 @@ extern "C" int kaos_download_particles(kaos_ctx* c, int32_t tube, int64_t n_cap,
   return KAOS_OK;
 }
 
+extern "C" int kaos_set_launch(kaos_ctx* c, int32_t tiles_per_group, int32_t hist_words, int32_t persistent) {
+  if (!c) return KAOS_E_ARG;
+  if (tiles_per_group < 1 || tiles_per_group > 63)
+    return fail(c, KAOS_E_ARG, "tiles_per_group must be in [1, 63] (16-bit LDS counters)");
+  if (hist_words < 0 || hist_words > 12288) return fail(c, KAOS_E_ARG, "hist_words must be in [0, 12288]");
+  c->launch = {tiles_per_group, hist_words, persistent ? 1 : 0};
+  c->tuning = false;
+  for (auto& ev : c->tune_ev) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+  c->tune_ev.clear();
+  return KAOS_OK;
+}
+
+extern "C" int kaos_get_launch(kaos_ctx* c, int32_t* tiles_per_group, int32_t* hist_words, int32_t* persistent,
+                               int32_t* tuning) {
+  if (!c) return KAOS_E_ARG;
+  if (tiles_per_group) *tiles_per_group = c->launch.tpb;
+  if (hist_words) *hist_words = c->launch.hwords;
+  if (persistent) *persistent = c->launch.persistent;
+  if (tuning) *tuning = c->tuning ? 1 : 0;
+  return KAOS_OK;
+}
+
 extern "C" int kaos_profile(kaos_ctx* c, int32_t enable) {
   if (!c) return KAOS_E_ARG;
   c->prof_on = enable != 0;
@@ -1297,6 +1459,10 @@ extern "C" int kaos_finalize(kaos_ctx* c) {
                   c->d_nqc, c->d_nqoff, c->d_status, c->d_esc_acc, c->d_esc_rep, c->d_inj_base, c->d_inj_cnt,
                   c->d_uni_base, c->d_uni, c->d_pushed};
   for (void* p : bufs) if (p) (void)hipFree(p);
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& ev : c->tune_ev) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+  (void)hipFree(c->d_work);
+  (void)hipFree(c->d_stepno);
   for (StateBufs& B : c->st) {
     void* sb[] = {B.rs, B.vp, B.vr, B.qk, B.ae, B.ag, B.rdi, B.vpi, B.vrf, B.qkf, B.aef, B.agf, B.cell, B.id};
     for (void* p : sb) if (p) (void)hipFree(p);

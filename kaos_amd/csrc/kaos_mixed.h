@@ -199,18 +199,35 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
   float* s_EG = s_EA + NQ;
   int* s_rk = (int*)(s_EG + NQ);
   int* s_vk = s_rk + NQ;
-  unsigned int* s_hist = (unsigned int*)(s_vk + NV);
+  unsigned int* s_hist = (unsigned int*)(s_vk + NV);   // a.hwords words
   __shared__ unsigned int s_pushed;
-  __shared__ int s_wq0, s_wnq, s_wtube;
+  __shared__ int s_wq0, s_wnq, s_wtube, s_group;
   const int tid = threadIdx.x;
-  int t0 = blockIdx.x * a.tiles_per_block;
+  int cur = -1;
+  // per-tube scalars and base pointers
+  long long hoff = 0, goff = 0, toff = 0;
+  uint32_t tcount = 0;
+  uint32_t* cellT = nullptr;
+  int *rdiT = nullptr, *vpiT = nullptr;
+  float *qkfT = nullptr, *aefT = nullptr, *agfT = nullptr;
+  int nq = 0, nvpar = 0, nvperp = 0, desc = 0, grav = 0, eamb = 0, eambsign = 0;
+  double rGL0 = 0.0, rGHlast = 0.0, rsgn = 1.0;
+  float qGL0 = 0.0f, qGHlast = 0.0f, qsms = 0.0f;
+  double ip = 0.0, irep = 0.0;
+  float h = 0.0f, h2 = 0.0f, hsq2 = 0.0f, h6 = 0.0f, hh6 = 0.0f;
+  double vscale = 0.0, hd = 0.0;
+  for (int pass = 0;; ++pass) {
+  const int g = kaos_next_group(a, pass, &s_group);
+  if (g < 0) break;
+  int t0 = g * a.tiles_per_block;
   int t1 = min(t0 + a.tiles_per_block, a.ntiles);
+  unsigned int npushed = 0;
   if (tid == 0) {
     s_pushed = 0u;
     int tb0 = (t0 < t1) ? a.tile_tube[t0] : 0;
     const DevTube& T0 = a.tubes[tb0];
     int ncell = T0.nvpar * T0.nvperp;
-    int wnq = a.do_count ? (2 * KAOS_HWORDS) / ncell : 0;
+    int wnq = a.do_count ? (2 * a.hwords) / ncell : 0;
     if (wnq > T0.nq) wnq = T0.nq;
     int q0 = 1;
     if (t0 < t1 && wnq > 0) {
@@ -225,23 +242,9 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
     }
     s_wq0 = q0; s_wnq = wnq; s_wtube = tb0;
   }
-  for (int i = tid; i < KAOS_HWORDS; i += KAOS_BLOCK) s_hist[i] = 0u;
+  for (int i = tid; i < a.hwords; i += KAOS_BLOCK) s_hist[i] = 0u;
   __syncthreads();
   const int wq0 = s_wq0, wnq = s_wnq, wtube = s_wtube;
-  int cur = -1;
-  unsigned int npushed = 0;
-  // per-tube scalars and base pointers
-  long long hoff = 0, goff = 0, toff = 0;
-  uint32_t tcount = 0;
-  uint32_t* cellT = nullptr;
-  int *rdiT = nullptr, *vpiT = nullptr;
-  float *qkfT = nullptr, *aefT = nullptr, *agfT = nullptr;
-  int nq = 0, nvpar = 0, nvperp = 0, desc = 0, grav = 0, eamb = 0, eambsign = 0;
-  double rGL0 = 0.0, rGHlast = 0.0, rsgn = 1.0;
-  float qGL0 = 0.0f, qGHlast = 0.0f, qsms = 0.0f;
-  double ip = 0.0, irep = 0.0;
-  float h = 0.0f, h2 = 0.0f, hsq2 = 0.0f, h6 = 0.0f, hh6 = 0.0f;
-  double vscale = 0.0, hd = 0.0;
   for (int tile = t0; tile < t1; ++tile) {
     int tb = a.tile_tube[tile];
     if (tb != cur) {
@@ -401,4 +404,5 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
   }
   __syncthreads();
   if (tid == 0 && s_pushed) atomicAdd(a.pushed, (unsigned long long)s_pushed);
+  }
 }

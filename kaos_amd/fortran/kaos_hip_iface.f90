@@ -17,12 +17,13 @@ module kaos_hip
   integer(c_int), parameter, public :: KAOS_OK = 0, KAOS_E_ARG = 1, KAOS_E_CAPACITY = 2, &
                                        KAOS_E_DEVICE = 3, KAOS_E_STATE = 4, KAOS_E_NAN = 5
   integer(c_int32_t), parameter, public :: KAOS_FP64 = 0, KAOS_MIXED = 1
+  integer(c_int32_t), parameter, public :: KAOS_FLAG_NO_GRAPH = 1, KAOS_FLAG_NO_AUTOTUNE = 2
 
   type, bind(c), public :: kaos_cfg
     integer(c_int32_t) :: ntubes, precision, ndatfac, device
     integer(c_int64_t) :: seed                   ! uint64_t on the C side
     integer(c_int32_t) :: reorder_every = 0      ! 0: library default; < 0: never
-    integer(c_int32_t) :: reserved = 0
+    integer(c_int32_t) :: flags = 0              ! KAOS_FLAG_*
   end type kaos_cfg
 
   type, bind(c), public :: kaos_grid
@@ -39,7 +40,8 @@ module kaos_hip
   end type kaos_step_out
 
   public :: kaos_init, kaos_set_grid, kaos_upload_particles, kaos_step, kaos_run, &
-            kaos_download_master, kaos_count, kaos_download_particles, kaos_profile, &
+            kaos_download_master, kaos_count, kaos_download_particles, kaos_set_launch, &
+            kaos_get_launch, kaos_profile, &
             kaos_profile_read, kaos_last_error, kaos_finalize, kaos_error_message
 
   interface
@@ -106,6 +108,20 @@ module kaos_hip
       integer(c_int64_t), intent(out) :: n_out
       type(c_ptr), value :: x, y, z, Vx, Vy, Vz, vperp, qk4, aea, ag, qind, vparind, vperpind
     end function kaos_download_particles
+
+    integer(c_int) function kaos_set_launch(ctx, tiles_per_group, hist_words, persistent) &
+        bind(c, name='kaos_set_launch')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tiles_per_group, hist_words, persistent
+    end function kaos_set_launch
+
+    integer(c_int) function kaos_get_launch(ctx, tiles_per_group, hist_words, persistent, tuning) &
+        bind(c, name='kaos_get_launch')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: tiles_per_group, hist_words, persistent, tuning
+    end function kaos_get_launch
 
     integer(c_int) function kaos_profile(ctx, enable) bind(c, name='kaos_profile')
       import :: c_int, c_ptr, c_int32_t

@@ -14,7 +14,8 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
-from ._abi import KaosCfg, KaosGrid, KaosStepOut, ptr, KAOS_FP64, KAOS_MIXED  # noqa: F401
+from ._abi import (KaosCfg, KaosGrid, KaosStepOut, ptr, KAOS_FP64, KAOS_MIXED,  # noqa: F401
+                   KAOS_FLAG_NO_GRAPH, KAOS_FLAG_NO_AUTOTUNE)
 
 MOMENT_NAMES = ("M0", "M1perp", "M1par", "M2", "M2perp", "M2par")
 
@@ -44,12 +45,12 @@ def grid_struct(tube):
 
 class KineticSolver:
     def __init__(self, tubes, precision=KAOS_FP64, ndatfac=1, seed=1, capacity=None, device=0,
-                 reorder_every=0):
+                 reorder_every=0, flags=0):
         self.lib = _abi.load()
         self.tubes = list(tubes)
         self.ndatfac = ndatfac
         cfg = KaosCfg(ntubes=len(self.tubes), precision=precision, ndatfac=ndatfac, device=device,
-                      seed=seed, reorder_every=reorder_every)
+                      seed=seed, reorder_every=reorder_every, flags=flags)
         h = C.c_void_p()
         rc = self.lib.kaos_init(C.byref(cfg), C.byref(h))
         if rc:
@@ -84,6 +85,15 @@ class KineticSolver:
         ps = C.c_int64(0)
         self._chk(self.lib.kaos_run(self.h, n_first, nsteps, C.byref(ps)))
         return ps.value
+
+    def set_launch(self, tiles_per_group, hist_words, persistent):
+        self._chk(self.lib.kaos_set_launch(self.h, tiles_per_group, hist_words, int(persistent)))
+
+    def get_launch(self):
+        """(tiles_per_group, hist_words, persistent, tuning) of the push kernels."""
+        v = [C.c_int32(0) for _ in range(4)]
+        self._chk(self.lib.kaos_get_launch(self.h, *(C.byref(x) for x in v)))
+        return tuple(x.value for x in v)
 
     def profile(self, enable=True):
         self._chk(self.lib.kaos_profile(self.h, int(enable)))
