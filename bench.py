@@ -11,6 +11,11 @@ Both precision forms run the same inputs and the same steps:
   * mixed (binary32 stages and state words): the `mixed` object, with the
     relative errors of its moments and field against the FP64 run after the
     last timed step.
+The `batched` object is the many-tube workload: 512 independent tubes (L ~
+U[3,8], 64..256 cells, 64x64, log-uniform 1e4..1e6 ions each, T ~ U[0.1,1] eV),
+mixed form, every step replayed as one captured hipGraph, launch shape from
+the autotuner, 200 steps.
+The warmup steps include the autotuner's candidate steps (12 master steps).
 Prints one JSON line.  value = ion pushes (live ions summed over the timed
 steps, counted on the device) / wall time of the timed steps, inputs resident
 in HBM.  roofline: the push kernel's algorithmic HBM bytes per launch over its
@@ -91,7 +96,7 @@ def timed_run(form, tube, q0, v0, w0, cap, args):
     achieved = BYTES_PER_ION[form] * ions_per_launch / mean_push_s / 1e9
     log(f"[bench] {form}: {args.steps} steps, wall {wall:.3f}s, run events {run_ms/1e3:.3f}s, "
         f"push kernel {push_ms/max(launches,1):.3f} ms x {launches}")
-    res = dict(value=pushes / wall, ms_per_step=wall / args.steps * 1e3,
+    res = dict(value=pushes / wall, ms_per_step=wall / args.steps * 1e3, launch=ks.get_launch()[:3],
                roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS,
                          "traffic": TRAFFIC_BYTES_PER_ION[form] * ions_per_launch,
@@ -99,6 +104,51 @@ def timed_run(form, tube, q0, v0, w0, cap, args):
                          "kernel": "k_push_mixed" if form == "mixed" else "k_push_fp64",
                          "bytes_per_ion": BYTES_PER_ION[form], "mean_launch_ms": mean_push_s * 1e3})
     return ks, res
+
+
+def batched_run(args):
+    """512 tubes, mixed form: timed with graph replay (no per-kernel events), then
+    a few eager steps with hipEvents for the push kernel's mean duration."""
+    from kaos_amd import grids
+    from kaos_amd.solver import KineticSolver, KAOS_MIXED
+    t0 = time.perf_counter()
+    tubes = grids.config4_tubes(ntubes=args.tubes)
+    n_steps_total = args.warmup + args.steps + 12
+    caps, ions = [], []
+    for i, t in enumerate(tubes):
+        q, v, w = grids.initial_ions(t, seed=100 + i)
+        ions.append((q, v, w))
+        caps.append(len(q) + (n_steps_total + 2) * (t.n_inj_lb + t.n_inj_ub) + 1024)
+    n_ions = sum(len(x[0]) for x in ions)
+    ks = KineticSolver(tubes, precision=KAOS_MIXED, ndatfac=1, seed=1, capacity=caps)
+    for i, (q, v, w) in enumerate(ions):
+        ks.upload(i, q, v, w)
+    del ions
+    log(f"[bench] batched: {len(tubes)} tubes, {n_ions} ions, setup {time.perf_counter() - t0:.1f}s")
+    ks.step(1)
+    ks.run(2, args.warmup)
+    first = 2 + args.warmup
+    t1 = time.perf_counter()
+    pushes = ks.run(first, args.steps)
+    wall = time.perf_counter() - t1
+    ks.profile(True)
+    ks.profile_read()
+    k = 10
+    p2 = ks.run(first + args.steps, k)
+    push_ms, launches, _ = ks.profile_read()
+    mean_push_s = push_ms / 1e3 / max(launches, 1)
+    achieved = BYTES_PER_ION["mixed"] * (p2 / max(launches, 1)) / mean_push_s / 1e9
+    launch = ks.get_launch()
+    ks.close()
+    log(f"[bench] batched: {args.steps} steps (graphs), wall {wall:.3f}s; push {mean_push_s*1e3:.3f} ms "
+        f"(eager, {launches} launches); launch shape {launch[:3]}")
+    return {"value": pushes / wall, "ms_per_step": wall / args.steps * 1e3, "steps": args.steps,
+            "tubes": len(tubes), "ions_start": n_ions, "dtype": "f32+f64", "graphs": True,
+            "launch": {"tiles_per_group": launch[0], "hist_words": launch[1], "persistent": launch[2]},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_push_mixed",
+                         "bytes_per_ion": BYTES_PER_ION["mixed"], "mean_launch_ms": mean_push_s * 1e3,
+                         "timing": f"hipEvents over {launches} eager launches after the timed steps"}}
 
 
 def moment_errors(ma, mb):
@@ -130,10 +180,11 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=14)
     ap.add_argument("--ions", type=int, default=100_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--form", choices=("both", "fp64", "mixed"), default="both")
+    ap.add_argument("--tubes", type=int, default=512, help="tubes of the batched run (0 = skip it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the traffic-calibration copy of MIB MiB and exit")
     args = ap.parse_args()
@@ -187,6 +238,8 @@ def main():
                         "max_l2_rel_moment_error": max(err[k]["l2_rel"] for k in MOMENTS)}
     for ks, _ in runs.values():
         ks.close()
+    if args.tubes > 0:
+        out["batched"] = batched_run(args)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline()

@@ -685,71 +685,92 @@ KAOS_HD double kaos_ag_of(double e_interp, double qsms) {
  * (_8_3_1:118-121, 713-725), IonDistributionFunctionsSub F = NphRT/d3v
  * (_8_4_1:148-158), NonUniform2DIntegratorSub MM = 2 pi sum dV g
  * (_8_5:29-43) and the integrands of _8_6_1.._8_6_6 (2-D branch, centring
- * flags perp = 0, par = 1).  Summation order: for each Vpar column b the sum
- * over Vperp rows a, then the column sums in b order.                      */
+ * flags perp = 0, par = 1).
+ *
+ * One pass over the counts: within a Vpar column b, V||c and dVpar are
+ * constant, so every integrand is a column aggregate times a column factor:
+ *   A0 = sum_a dV vc F,  A1 = sum_a dV vc^2 F,  A3 = sum_a dV vc^3 F
+ * (dV = dVperp_a dVpar_b, vc = Vperp centre), and
+ *   M0 ~ sum_b A0, M1perp ~ sum_b A1, M1par ~ sum_b uc A0, M2perp ~ sum_b A3,
+ *   M2 ~ sum_b (A3 + d^2 A0), M2par ~ sum_b d^2 A0,  d = uc - M1par,
+ * with the reference's fallback to the uncentred integrand where the centred
+ * one is zero or NaN (_8_6_4:292-345, _8_6_6:272-299) applied per column.
+ * Summation order (fixed): rows a = g, g + G, g + 2G, ... for row group
+ * g = 0..G-1 (G = kaos_mom_groups), the G partials added in g order, then the
+ * columns by the pairwise tree kaos_tree_sum.                               */
+
+#define KAOS_MOM_LANES 256
 
 typedef struct {
   int nvpar, nvperp;
   const double *d3v, *dvperp, *dvpar, *vperpGC, *vparGC;
 } kaos_vgrid_t;
 
-KAOS_HD double kaos_cell_F(uint32_t cnt, double k, double NqRT, double d3v) {
+KAOS_HD int kaos_mom_groups(int nvpar) {
+  int g = KAOS_MOM_LANES / nvpar;
+  return g < 1 ? 1 : g;
+}
+
+KAOS_HD double kaos_cell_F(uint32_t cnt, double k, double d3v) {
   double c = (double)cnt;
-  double NphRT = (c == 0.0) ? 0.0 : c * k;
-  return (NqRT == 0.0) ? 0.0 : NphRT / d3v;
+  return (c == 0.0) ? 0.0 : (c * k) / d3v;
 }
 
-/* Pass 1 column b: sums of dV*g for M0, M1perp, M1par, M2perp. */
-KAOS_HD void kaos_mom_col1(const kaos_vgrid_t* v, const uint32_t* cnt, int b, double k,
-                           double NqRT, double o[4]) {
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s4 = 0.0;
-  for (int a = 0; a < v->nvperp; ++a) {
+/* Partial column aggregates of column b over rows g0, g0 + G, ..., and the
+ * partial count of those cells. */
+KAOS_HD void kaos_mom_part(const kaos_vgrid_t* v, const uint32_t* cnt, int b, int g0, int G, double k,
+                           double P[3], uint32_t* csum) {
+  double s0 = 0.0, s1 = 0.0, s3 = 0.0;
+  uint32_t cs = 0;
+  const double dvb = v->dvpar[b];
+  for (int a = g0; a < v->nvperp; a += G) {
     int i = a * v->nvpar + b;
-    double f = kaos_cell_F(cnt[i], k, NqRT, v->d3v[i]);
-    double w = v->dvperp[a] * v->dvpar[b];
-    double vc = v->vperpGC[a], uc = v->vparGC[b];
-    double g0 = (f == 0.0) ? 0.0 : vc * f;
-    double g1 = (f == 0.0) ? 0.0 : (vc * vc) * f;
-    double g2 = (f == 0.0) ? 0.0 : (vc * uc) * f;
-    double g4 = (f == 0.0) ? 0.0 : (vc * (vc * vc)) * f;
-    s0 = s0 + w * g0; s1 = s1 + w * g1; s2 = s2 + w * g2; s4 = s4 + w * g4;
-  }
-  o[0] = s0; o[1] = s1; o[2] = s2; o[3] = s4;
-}
-
-/* Pass 2 column b: centred M2 (_8_6_4:292-345) and M2par (_8_6_6:272-299),
- * with the reference's fallback to the uncentred integrand where the centred
- * one is NaN or zero. */
-KAOS_HD void kaos_mom_col2(const kaos_vgrid_t* v, const uint32_t* cnt, int b, double k,
-                           double NqRT, double M1par, double o[2]) {
-  double s3 = 0.0, s5 = 0.0;
-  for (int a = 0; a < v->nvperp; ++a) {
-    int i = a * v->nvpar + b;
-    double f = kaos_cell_F(cnt[i], k, NqRT, v->d3v[i]);
-    double w = v->dvperp[a] * v->dvpar[b];
-    double vc = v->vperpGC[a], uc = v->vparGC[b];
-    double g3 = 0.0, g5 = 0.0;
+    cs += cnt[i];
+    double f = kaos_cell_F(cnt[i], k, v->d3v[i]);
     if (f != 0.0) {
-      double d = uc - M1par;
-      g3 = ((vc * (vc * vc)) * f) + ((vc * (d * d)) * f);
-      if (g3 != g3 || g3 == 0.0) g3 = ((vc * (vc * vc)) * f) + ((vc * (uc * uc)) * f);
-      g5 = (vc * (d * d)) * f;
-      if (g5 != g5 || g5 == 0.0) g5 = (vc * (uc * uc)) * f;
+      double w = v->dvperp[a] * dvb;
+      double vc = v->vperpGC[a];
+      s0 = s0 + w * (vc * f);
+      s1 = s1 + w * ((vc * vc) * f);
+      s3 = s3 + w * ((vc * (vc * vc)) * f);
     }
-    s3 = s3 + w * g3; s5 = s5 + w * g5;
   }
-  o[0] = s3; o[1] = s5;
+  P[0] = s0; P[1] = s1; P[2] = s3;
+  *csum = cs;
 }
 
-/* M from the pass sums.  M[0..5] = M0, M1perp, M1par, M2, M2perp, M2par. */
-KAOS_HD void kaos_mom_finish1(const double S[4], double NqRT, double ms, double M[6]) {
-  double MM0 = (2.0 * KAOS_PI) * S[0];
-  double M0 = (NqRT == 0.0) ? 0.0 : MM0;
+/* Pairwise tree sum of x[0..n) in place (x[0] holds the result): at stride
+ * s = 1, 2, 4, ... x[i] += x[i + s] for i = 0, 2s, 4s, ...                  */
+KAOS_HD void kaos_tree_sum(double* x, int n) {
+  for (int st = 1; st < n; st *= 2)
+    for (int i = 0; i + st < n; i += 2 * st) x[i] = x[i] + x[i + st];
+}
+
+/* Column values of the first stage: [A0, A1, uc A0, A3]. */
+KAOS_HD void kaos_mom_col_first(double uc, const double A[3], double o[4]) {
+  o[0] = A[0]; o[1] = A[1]; o[2] = uc * A[0]; o[3] = A[2];
+}
+
+/* M0, M1perp, M1par, M2perp from the first-stage sums S[0..3]. */
+KAOS_HD void kaos_mom_finish1(const double S[4], double ms, double M[6]) {
+  double M0 = (2.0 * KAOS_PI) * S[0];
   M[0] = M0;
   M[1] = (M0 == 0.0) ? 0.0 : ((2.0 * KAOS_PI) * S[1]) / M0;
   M[2] = (M0 == 0.0) ? 0.0 : ((2.0 * KAOS_PI) * S[2]) / M0;
   M[4] = (M0 == 0.0) ? 0.0 : (ms / 2.0) * (((2.0 * KAOS_PI) * S[3]) / M0);
 }
+
+/* Column values of the second stage (centred by M1par): [M2, M2par] integrands. */
+KAOS_HD void kaos_mom_col_second(double uc, const double A[3], double M1par, double o[2]) {
+  double d = uc - M1par;
+  double c3 = A[2] + (d * d) * A[0];
+  if (c3 != c3 || c3 == 0.0) c3 = A[2] + (uc * uc) * A[0];
+  double c5 = (d * d) * A[0];
+  if (c5 != c5 || c5 == 0.0) c5 = (uc * uc) * A[0];
+  o[0] = c3; o[1] = c5;
+}
+
+/* M2, M2par from the second-stage sums. */
 KAOS_HD void kaos_mom_finish2(const double S[2], double ms, double M[6]) {
   double M0 = M[0];
   M[3] = (M0 == 0.0) ? 0.0 : (ms / 2.0) * (((2.0 * KAOS_PI) * S[0]) / M0);

@@ -410,60 +410,86 @@ __global__ void k_inject(StepArgs a, const long long* inj_base, const long long*
 }
 
 // Counts -> distribution function -> moments of one (tube, config cell) per
-// block (_8_3_1, _8_4_1, _8_5, _8_6_1.._8_6_6).  Thread b sums Vpar column b
-// over Vperp rows; thread 0 adds the columns in order, so the sums are the
-// oracle's, bit for bit.  Block (t, 0) also advances the tube's slot count by
-// the ions injected this step.
+// block (_8_3_1, _8_4_1, _8_5, _8_6_1.._8_6_6), one pass over the counts: lane
+// (g, b) sums column b over rows g, g + G, ...; lane b adds the G partials in
+// g order and forms the column values; pairwise trees over the columns give
+// the sums, in the oracle's order (kaos_mom_part / kaos_tree_sum).  Block
+// (t, 0) also advances the tube's slot count and next id by this step's
+// injected ions.
 __global__ void __launch_bounds__(KAOS_BLOCK) k_moments(const DevTube* tubes, const int* blk_tube,
     const int* blk_q, const double* d3xC, const double* d3v, const double* dvpar, const double* dvperp,
     const double* vparGC, const double* vperpGC, const unsigned int* hist, const unsigned int* nqoff,
     unsigned int* nqc, double* NqRT, double* M, long long* count, long long* next_id,
     const long long* inj_cnt) {
-  __shared__ double s_col[KAOS_MAXV + 1][4];
+  __shared__ double s_P[KAOS_MOM_LANES][3];
+  __shared__ unsigned int s_pc[KAOS_MOM_LANES];
+  __shared__ double s_A[KAOS_MAXV + 1][3];
+  __shared__ double s_x[6][KAOS_MAXV + 1];
+  __shared__ unsigned int s_cc[KAOS_MAXV + 1];
   __shared__ double s_M[6];
-  __shared__ unsigned int s_cnt[KAOS_MAXV + 1];
-  __shared__ double s_nqr;
-  int t = blk_tube[blockIdx.x], Q = blk_q[blockIdx.x];
-  DevTube T = tubes[t];
-  int b = threadIdx.x;
-  double k = T.nsnormfac / d3xC[T.goff + Q];
-  const uint32_t* N = hist + T.hoff + (long long)Q * T.nvperp * T.nvpar;
-  // config count NqReNormT = ions on the velocity grid (column sums) + off it
-  if (b < T.nvpar) {
-    unsigned int cs = 0;
-    for (int r = 0; r < T.nvperp; ++r) cs += N[r * T.nvpar + b];
-    s_cnt[b] = cs;
-  }
-  __syncthreads();
-  if (b == 0) {
-    unsigned int tot = nqoff[T.goff + Q];
-    for (int c = 0; c < T.nvpar; ++c) tot += s_cnt[c];
-    nqc[T.goff + Q] = tot;
-    s_nqr = (double)tot * k;
-  }
-  __syncthreads();
-  double nqr = s_nqr;
-  kaos_vgrid_t v = { T.nvpar, T.nvperp, d3v + T.doff, dvperp + T.woff, dvpar + T.voff,
+  const int t = blk_tube[blockIdx.x], Q = blk_q[blockIdx.x];
+  const DevTube& T = tubes[t];
+  const int np = T.nvpar, G = kaos_mom_groups(np);
+  const int tid = threadIdx.x;
+  const double k = T.nsnormfac / d3xC[T.goff + Q];
+  const uint32_t* N = hist + T.hoff + (long long)Q * T.nvperp * np;
+  kaos_vgrid_t v = { np, T.nvperp, d3v + T.doff, dvperp + T.woff, dvpar + T.voff,
                      vperpGC + T.woff, vparGC + T.voff };
-  if (b < T.nvpar) kaos_mom_col1(&v, N, b, k, nqr, s_col[b]);
+  if (tid < G * np) {
+    double P[3];
+    uint32_t cs;
+    kaos_mom_part(&v, N, tid % np, tid / np, G, k, P, &cs);
+    s_P[tid][0] = P[0]; s_P[tid][1] = P[1]; s_P[tid][2] = P[2];
+    s_pc[tid] = cs;
+  }
   __syncthreads();
-  if (b == 0) {
-    double S[4] = {0, 0, 0, 0}, Mq[6] = {0, 0, 0, 0, 0, 0};
-    for (int c = 0; c < T.nvpar; ++c) for (int i = 0; i < 4; ++i) S[i] = S[i] + s_col[c][i];
-    kaos_mom_finish1(S, nqr, T.ms, Mq);
+  if (tid < np) {
+    double A[3] = {s_P[tid][0], s_P[tid][1], s_P[tid][2]};
+    unsigned int cs = s_pc[tid];
+    for (int g = 1; g < G; ++g) {
+      for (int i = 0; i < 3; ++i) A[i] = A[i] + s_P[g * np + tid][i];
+      cs += s_pc[g * np + tid];
+    }
+    for (int i = 0; i < 3; ++i) s_A[tid][i] = A[i];
+    s_cc[tid] = cs;
+    double o4[4];
+    kaos_mom_col_first(v.vparGC[tid], A, o4);
+    for (int i = 0; i < 4; ++i) s_x[i][tid] = o4[i];
+  }
+  __syncthreads();
+  for (int st = 1; st < np; st *= 2) {
+    if (tid < np && (tid % (2 * st)) == 0 && tid + st < np)
+      for (int i = 0; i < 4; ++i) s_x[i][tid] = s_x[i][tid] + s_x[i][tid + st];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    unsigned int tot = nqoff[T.goff + Q];
+    for (int c = 0; c < np; ++c) tot += s_cc[c];
+    nqc[T.goff + Q] = tot;
+    NqRT[T.goff + Q] = (double)tot * k;
+    double S[4] = {s_x[0][0], s_x[1][0], s_x[2][0], s_x[3][0]}, Mq[6] = {0, 0, 0, 0, 0, 0};
+    kaos_mom_finish1(S, T.ms, Mq);
     for (int i = 0; i < 6; ++i) s_M[i] = Mq[i];
   }
   __syncthreads();
-  double M1par = s_M[2];
-  if (b < T.nvpar) kaos_mom_col2(&v, N, b, k, nqr, M1par, s_col[b]);
+  if (tid < np) {
+    double o2[2];
+    kaos_mom_col_second(v.vparGC[tid], s_A[tid], s_M[2], o2);
+    s_x[4][tid] = o2[0]; s_x[5][tid] = o2[1];
+  }
   __syncthreads();
-  if (b == 0) {
-    double S[2] = {0, 0}, Mq[6];
+  for (int st = 1; st < np; st *= 2) {
+    if (tid < np && (tid % (2 * st)) == 0 && tid + st < np) {
+      s_x[4][tid] = s_x[4][tid] + s_x[4][tid + st];
+      s_x[5][tid] = s_x[5][tid] + s_x[5][tid + st];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double S2[2] = {s_x[4][0], s_x[5][0]}, Mq[6];
     for (int i = 0; i < 6; ++i) Mq[i] = s_M[i];
-    for (int c = 0; c < T.nvpar; ++c) { S[0] = S[0] + s_col[c][0]; S[1] = S[1] + s_col[c][1]; }
-    kaos_mom_finish2(S, T.ms, Mq);
+    kaos_mom_finish2(S2, T.ms, Mq);
     for (int i = 0; i < 6; ++i) M[6 * T.goff + (long long)i * T.nq + Q] = Mq[i];
-    NqRT[T.goff + Q] = nqr;
     if (Q == 0 && inj_cnt) { count[t] += inj_cnt[t]; next_id[t] += inj_cnt[t]; }
   }
 }

@@ -170,25 +170,40 @@ static void counts_and_moments(orc_t *o, const int *qi, const int *vpi, const in
   for (int Q = 0; Q < nq; ++Q) o->nq_counts[Q] = (double)o->nq_u[Q];
   for (int64_t i = 0; i < nq * ncell; ++i) o->nph_counts[i] = (double)o->nph_u[i];
   kaos_vgrid_t v = { np, nr, c->d3v, c->dvperp, c->dvpar, c->vperpGC, c->vparGC };
+  const int G = kaos_mom_groups(np);
+  double *A = (double *)xc(sizeof(double) * 3 * np), *x = (double *)xc(sizeof(double) * 6 * np);
   for (int Q = 0; Q < nq; ++Q) {
     double k = c->nsnormfac / c->d3xC[Q];
-    double NqRT = o->nq_counts[Q] * k;
-    o->NqRT[Q] = NqRT;
+    o->NqRT[Q] = o->nq_counts[Q] * k;
     const uint32_t *N = o->nph_u + (int64_t)Q * ncell;
-    double S[4] = {0, 0, 0, 0}, col[4], Mq[6];
+    /* column aggregates: row groups g = 0..G-1 added in order */
     for (int b = 0; b < np; ++b) {
-      kaos_mom_col1(&v, N, b, k, NqRT, col);
-      for (int t = 0; t < 4; ++t) S[t] = S[t] + col[t];
+      double P[3];
+      uint32_t cs;
+      kaos_mom_part(&v, N, b, 0, G, k, &A[3 * b], &cs);
+      for (int g = 1; g < G; ++g) {
+        kaos_mom_part(&v, N, b, g, G, k, P, &cs);
+        for (int t = 0; t < 3; ++t) A[3 * b + t] = A[3 * b + t] + P[t];
+      }
+      double o4[4];
+      kaos_mom_col_first(c->vparGC[b], &A[3 * b], o4);
+      for (int t = 0; t < 4; ++t) x[t * np + b] = o4[t];
     }
-    kaos_mom_finish1(S, NqRT, c->ms, Mq);
-    double S2[2] = {0, 0}, col2[2];
+    double S[4], Mq[6];
+    for (int t = 0; t < 4; ++t) { kaos_tree_sum(&x[t * np], np); S[t] = x[t * np]; }
+    kaos_mom_finish1(S, c->ms, Mq);
     for (int b = 0; b < np; ++b) {
-      kaos_mom_col2(&v, N, b, k, NqRT, Mq[2], col2);
-      S2[0] = S2[0] + col2[0]; S2[1] = S2[1] + col2[1];
+      double o2[2];
+      kaos_mom_col_second(c->vparGC[b], &A[3 * b], Mq[2], o2);
+      x[4 * np + b] = o2[0]; x[5 * np + b] = o2[1];
     }
+    double S2[2];
+    for (int t = 0; t < 2; ++t) { kaos_tree_sum(&x[(4 + t) * np], np); S2[t] = x[(4 + t) * np]; }
     kaos_mom_finish2(S2, c->ms, Mq);
     for (int t = 0; t < 6; ++t) o->M[t * nq + Q] = Mq[t];
   }
+  free(A);
+  free(x);
 }
 
 /* ---- state management ------------------------------------------------------ */
