@@ -706,28 +706,35 @@ typedef struct {
   const double *d3v, *dvperp, *dvpar, *vperpGC, *vparGC;
 } kaos_vgrid_t;
 
+/* Row groups G: one (each column summed over all rows in order) when a
+ * wavefront holds every column (nvpar <= 64), else KAOS_MOM_LANES / nvpar. */
 KAOS_HD int kaos_mom_groups(int nvpar) {
+  if (nvpar <= 64) return 1;
   int g = KAOS_MOM_LANES / nvpar;
   return g < 1 ? 1 : g;
 }
 
-KAOS_HD double kaos_cell_F(uint32_t cnt, double k, double d3v) {
-  double c = (double)cnt;
-  return (c == 0.0) ? 0.0 : (c * k) / d3v;
-}
-
 /* Partial column aggregates of column b over rows g0, g0 + G, ..., and the
- * partial count of those cells. */
+ * partial count of those cells (empty cells contribute nothing).  The device
+ * compile unrolls the row loop so the count loads issue together; the
+ * arithmetic order is the same on both compilers. */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define KAOS_UNROLL8 _Pragma("unroll 8")
+#else
+#define KAOS_UNROLL8
+#endif
 KAOS_HD void kaos_mom_part(const kaos_vgrid_t* v, const uint32_t* cnt, int b, int g0, int G, double k,
                            double P[3], uint32_t* csum) {
   double s0 = 0.0, s1 = 0.0, s3 = 0.0;
   uint32_t cs = 0;
   const double dvb = v->dvpar[b];
+  KAOS_UNROLL8
   for (int a = g0; a < v->nvperp; a += G) {
     int i = a * v->nvpar + b;
-    cs += cnt[i];
-    double f = kaos_cell_F(cnt[i], k, v->d3v[i]);
-    if (f != 0.0) {
+    uint32_t c = cnt[i];
+    cs += c;
+    if (c != 0u) {
+      double f = ((double)c * k) / v->d3v[i];
       double w = v->dvperp[a] * dvb;
       double vc = v->vperpGC[a];
       s0 = s0 + w * (vc * f);

@@ -420,14 +420,15 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments(const DevTube* tubes, co
     const int* blk_q, const double* d3xC, const double* d3v, const double* dvpar, const double* dvperp,
     const double* vparGC, const double* vperpGC, const unsigned int* hist, const unsigned int* nqoff,
     unsigned int* nqc, double* NqRT, double* M, long long* count, long long* next_id,
-    const long long* inj_cnt) {
+    const long long* inj_cnt, const int* cells) {
   __shared__ double s_P[KAOS_MOM_LANES][3];
   __shared__ unsigned int s_pc[KAOS_MOM_LANES];
   __shared__ double s_A[KAOS_MAXV + 1][3];
   __shared__ double s_x[6][KAOS_MAXV + 1];
   __shared__ unsigned int s_cc[KAOS_MAXV + 1];
   __shared__ double s_M[6];
-  const int t = blk_tube[blockIdx.x], Q = blk_q[blockIdx.x];
+  const int cell = cells[blockIdx.x];
+  const int t = blk_tube[cell], Q = blk_q[cell];
   const DevTube& T = tubes[t];
   const int np = T.nvpar, G = kaos_mom_groups(np);
   const int tid = threadIdx.x;
@@ -489,6 +490,61 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments(const DevTube* tubes, co
     double S2[2] = {s_x[4][0], s_x[5][0]}, Mq[6];
     for (int i = 0; i < 6; ++i) Mq[i] = s_M[i];
     kaos_mom_finish2(S2, T.ms, Mq);
+    for (int i = 0; i < 6; ++i) M[6 * T.goff + (long long)i * T.nq + Q] = Mq[i];
+    if (Q == 0 && inj_cnt) { count[t] += inj_cnt[t]; next_id[t] += inj_cnt[t]; }
+  }
+}
+
+// The same moments with one wavefront per config cell (grids of <= 64 Vpar
+// cells, G = 1): lane b owns column b; the pairwise trees run across lanes
+// with shuffles (lane i adds lane i + s for i = 0, 2s, 4s, ...), so the sums
+// are kaos_tree_sum's.  `cells` lists the (tube, cell) blocks of such tubes.
+__device__ __forceinline__ double kaos_wave_tree(double x, int lane, int n) {
+  for (int st = 1; st < n; st *= 2) {
+    double y = __shfl_down(x, st, 64);
+    if ((lane % (2 * st)) == 0 && lane + st < n) x = x + y;
+  }
+  return x;
+}
+
+__global__ void __launch_bounds__(KAOS_BLOCK) k_moments_wave(const DevTube* tubes, const int* blk_tube,
+    const int* blk_q, const int* cells, int ncells, const double* d3xC, const double* d3v,
+    const double* dvpar, const double* dvperp, const double* vparGC, const double* vperpGC,
+    const unsigned int* hist, const unsigned int* nqoff, unsigned int* nqc, double* NqRT, double* M,
+    long long* count, long long* next_id, const long long* inj_cnt) {
+  const int lane = threadIdx.x & 63;
+  const int ci = blockIdx.x * (KAOS_BLOCK / 64) + (threadIdx.x >> 6);
+  if (ci >= ncells) return;
+  const int cell = cells[ci];
+  const int t = blk_tube[cell], Q = blk_q[cell];
+  const DevTube& T = tubes[t];
+  const int np = T.nvpar;
+  const double k = T.nsnormfac / d3xC[T.goff + Q];
+  const uint32_t* N = hist + T.hoff + (long long)Q * T.nvperp * np;
+  kaos_vgrid_t v = { np, T.nvperp, d3v + T.doff, dvperp + T.woff, dvpar + T.voff,
+                     vperpGC + T.woff, vparGC + T.voff };
+  double A[3] = {0.0, 0.0, 0.0}, o4[4] = {0.0, 0.0, 0.0, 0.0};
+  uint32_t cs = 0;
+  double uc = 0.0;
+  if (lane < np) {
+    kaos_mom_part(&v, N, lane, 0, 1, k, A, &cs);
+    uc = v.vparGC[lane];
+    kaos_mom_col_first(uc, A, o4);
+  }
+  for (int st = 1; st < 64; st *= 2) cs += __shfl_xor(cs, st, 64);   // integer: any order
+  double S[4];
+  for (int i = 0; i < 4; ++i) S[i] = __shfl(kaos_wave_tree(o4[i], lane, np), 0, 64);
+  double Mq[6] = {0, 0, 0, 0, 0, 0};
+  kaos_mom_finish1(S, T.ms, Mq);
+  double o2[2] = {0.0, 0.0};
+  if (lane < np) kaos_mom_col_second(uc, A, Mq[2], o2);
+  double S2[2];
+  for (int i = 0; i < 2; ++i) S2[i] = __shfl(kaos_wave_tree(o2[i], lane, np), 0, 64);
+  kaos_mom_finish2(S2, T.ms, Mq);
+  if (lane == 0) {
+    unsigned int tot = nqoff[T.goff + Q] + cs;
+    nqc[T.goff + Q] = tot;
+    NqRT[T.goff + Q] = (double)tot * k;
     for (int i = 0; i < 6; ++i) M[6 * T.goff + (long long)i * T.nq + Q] = Mq[i];
     if (Q == 0 && inj_cnt) { count[t] += inj_cnt[t]; next_id[t] += inj_cnt[t]; }
   }
@@ -645,6 +701,7 @@ struct kaos_ctx {
   DevTube* d_tubes = nullptr;
   int* d_tile_tube = nullptr; long long* d_tile_start = nullptr; int ntiles = 0;
   int* d_blk_tube = nullptr; int* d_blk_q = nullptr; int nblk_mom = 0;
+  int *d_cells_wave = nullptr, *d_cells_block = nullptr; int n_cells_wave = 0, n_cells_block = 0;
   double *d_qGH = nullptr, *d_qGC = nullptr, *d_hq = nullptr, *d_dq = nullptr, *d_Te = nullptr,
          *d_d3x = nullptr, *d_EA = nullptr, *d_EG = nullptr, *d_M = nullptr, *d_M0F = nullptr,
          *d_NqRT = nullptr;
@@ -849,6 +906,10 @@ static int build_device(kaos_ctx* c) {
   }
   c->ncell_tot = goff; c->nhist_tot = hoff; c->cap_tot = off; c->ntiles = (int)tile_tube.size();
   c->nblk_mom = (int)blk_tube.size();
+  // moment blocks: a wavefront per cell where the Vpar grid fits one, else a block
+  std::vector<int> cw, cb;
+  for (int i = 0; i < c->nblk_mom; ++i) (c->dt[blk_tube[i]].nvpar <= 64 ? cw : cb).push_back(i);
+  c->n_cells_wave = (int)cw.size(); c->n_cells_block = (int)cb.size();
   auto up = [&](void** d, const void* h, size_t bytes) -> hipError_t {
     hipError_t e = hipMalloc(d, bytes ? bytes : 8);
     if (e != hipSuccess) return e;
@@ -860,6 +921,8 @@ static int build_device(kaos_ctx* c) {
   HIPCHK(c, up((void**)&c->d_tile_start, tile_start.data(), sizeof(long long) * tile_start.size()));
   HIPCHK(c, up((void**)&c->d_blk_tube, blk_tube.data(), sizeof(int) * blk_tube.size()));
   HIPCHK(c, up((void**)&c->d_blk_q, blk_q.data(), sizeof(int) * blk_q.size()));
+  HIPCHK(c, up((void**)&c->d_cells_wave, cw.data(), sizeof(int) * cw.size()));
+  HIPCHK(c, up((void**)&c->d_cells_block, cb.data(), sizeof(int) * cb.size()));
   HIPCHK(c, up((void**)&c->d_qGH, qGH.data(), sizeof(double) * goff));
   HIPCHK(c, up((void**)&c->d_qGC, qGC.data(), sizeof(double) * goff));
   HIPCHK(c, up((void**)&c->d_hq, hq.data(), sizeof(double) * goff));
@@ -1028,6 +1091,22 @@ static size_t mixed_lds_bytes(const kaos_ctx* c) {
          sizeof(int) * ((size_t)c->lds_nq + c->lds_nv) + sizeof(unsigned int) * c->launch.hwords;
 }
 
+// Moments of every (tube, cell): wavefront-per-cell kernel for tubes with <= 64
+// Vpar cells, block-per-cell kernel for the others.
+static void launch_moments(kaos_ctx* c, const long long* inj) {
+  const int wpb = KAOS_BLOCK / 64;
+  if (c->n_cells_wave)
+    hipLaunchKernelGGL(k_moments_wave, dim3((c->n_cells_wave + wpb - 1) / wpb), dim3(KAOS_BLOCK), 0, c->stream,
+                       c->d_tubes, c->d_blk_tube, c->d_blk_q, c->d_cells_wave, c->n_cells_wave, c->d_d3x, c->d_d3v,
+                       c->d_dvpar, c->d_dvperp, c->d_vparGC, c->d_vperpGC, c->d_hist, c->d_nqoff, c->d_nqc,
+                       c->d_NqRT, c->d_M, c->d_count, c->d_next_id, inj);
+  if (c->n_cells_block)
+    hipLaunchKernelGGL(k_moments, dim3(c->n_cells_block), dim3(KAOS_BLOCK), 0, c->stream, c->d_tubes,
+                       c->d_blk_tube, c->d_blk_q, c->d_d3x, c->d_d3v, c->d_dvpar, c->d_dvperp, c->d_vparGC,
+                       c->d_vperpGC, c->d_hist, c->d_nqoff, c->d_nqc, c->d_NqRT, c->d_M, c->d_count, c->d_next_id,
+                       inj, c->d_cells_block);
+}
+
 // Persistent launches: this many blocks per CU (the push kernels' launch bounds).
 #define KAOS_PERSIST_BLOCKS_PER_CU 4
 
@@ -1125,10 +1204,7 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
     // field of master step 1 from "master step 0" (no moments: M = 0), then moments of step 1
     hipLaunchKernelGGL(k_field, dim3(nt), dim3(256), 0, c->stream, c->d_tubes, c->d_hq, c->d_dq, c->d_Te,
                        c->d_M, c->d_M0F, c->d_EA, c->d_esc_acc, c->d_esc_rep, 0);
-    hipLaunchKernelGGL(k_moments, dim3(c->nblk_mom), dim3(KAOS_BLOCK), 0, c->stream, c->d_tubes, c->d_blk_tube,
-                       c->d_blk_q, c->d_d3x, c->d_d3v, c->d_dvpar, c->d_dvperp, c->d_vparGC, c->d_vperpGC,
-                       c->d_hist, c->d_nqoff, c->d_nqc, c->d_NqRT, c->d_M, c->d_count, c->d_next_id,
-                       (const long long*)nullptr);
+    launch_moments(c, nullptr);
     launch_push(c, a);
   } else {
     if (master) {
@@ -1150,10 +1226,7 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
                            c->d_EG, c->d_vperpGH, d_uni, c->d_uni_base, (unsigned long long)c->cfg.seed,
                            c->d_stepno, c->inj_total, (int)(c->cfg.precision == KAOS_MIXED));
       }
-      hipLaunchKernelGGL(k_moments, dim3(c->nblk_mom), dim3(KAOS_BLOCK), 0, c->stream, c->d_tubes, c->d_blk_tube,
-                         c->d_blk_q, c->d_d3x, c->d_d3v, c->d_dvpar, c->d_dvperp, c->d_vparGC, c->d_vperpGC,
-                         c->d_hist, c->d_nqoff, c->d_nqc, c->d_NqRT, c->d_M, c->d_count, c->d_next_id,
-                         c->d_inj_cnt);
+      launch_moments(c, c->d_inj_cnt);
     }
   }
   HIPCHK(c, hipGetLastError());
@@ -1477,7 +1550,8 @@ extern "C" int kaos_profile_read(kaos_ctx* c, double* push_ms, int64_t* push_lau
 extern "C" int kaos_finalize(kaos_ctx* c) {
   if (!c) return KAOS_OK;
   (void)hipStreamSynchronize(c->stream);
-  void* bufs[] = {c->d_tubes, c->d_tile_tube, c->d_tile_start, c->d_blk_tube, c->d_blk_q, c->d_qGH, c->d_qGC,
+  void* bufs[] = {c->d_tubes, c->d_tile_tube, c->d_tile_start, c->d_blk_tube, c->d_blk_q, c->d_cells_wave,
+                  c->d_cells_block, c->d_qGH, c->d_qGC,
                   c->d_hq, c->d_dq, c->d_Te, c->d_d3x, c->d_EA, c->d_EG, c->d_M, c->d_M0F, c->d_NqRT,
                   c->d_vparGH, c->d_vperpGH, c->d_vparGC, c->d_vperpGC, c->d_dvpar, c->d_dvperp, c->d_d3v,
                   c->d_rGH, c->d_rref, c->d_rk, c->d_vk, c->d_keycnt, c->d_cursor, c->d_cnt_new,

@@ -18,7 +18,8 @@ MASTER_KEYS = ("nq_counts", "nph_counts", "NqRT", "M0", "M1perp", "M1par", "M2",
 
 def _tubes():
     t = [grids.config1(n_total=9000),
-         grids.dipole_tube(nq=100, nvpar=32, nvperp=24, n_total=15000, p=5.0, r_ub=3.5, ratio=1.02, T_eV=0.6),
+         # > 64 Vpar cells: moments by the block-per-cell kernel (row groups G = 2)
+         grids.dipole_tube(nq=100, nvpar=100, nvperp=24, n_total=15000, p=5.0, r_ub=3.5, ratio=1.02, T_eV=0.6),
          grids.dipole_tube(nq=24, nvpar=16, nvperp=12, n_total=700, ratio=1.1),          # smaller than one tile
          grids.dipole_tube(nq=48, nvpar=40, nvperp=40, n_total=12000, south=True, p=6.0, T_eV=0.2),
          grids.dipole_tube(nq=64, nvpar=64, nvperp=64, n_total=5000, p=3.5, r_ub=2.6, ratio=1.03)]
