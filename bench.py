@@ -75,6 +75,34 @@ def cpu_baseline(n_ions=2_000_000, steps=20):
                        f"master steps, {dt:.1f} s")
 
 
+def cpu_reference_push(n_ions=2_000_000):
+    """The reference's own compiled push (KineticRK4UpdateSub and callees, built
+    from its Fortran sources into oracle/_ref) timed on one core over one step
+    of a sample of the same tube's ions.  Push only: the reference's binning
+    and moments (O(cells x ions)) are not in this figure."""
+    import oracle
+    import oracle.ref_push as rp
+    from kaos_amd import grids
+    if not rp.available():
+        return None
+    tube = grids.config2(n_total=n_ions)
+    q0, v0, w0 = grids.initial_ions(tube, seed=1)
+    orc = oracle.Oracle(tube, q0, v0, w0, ndatfac=1, seed=1, capacity=len(q0) + 20000)
+    orc.step(1)
+    orc.step(2)
+    P = orc.particles()
+    orc.close()
+    r = rp.run(tube, P["x"], P["y"], P["z"], P["Vx"], P["Vy"], P["Vz"], P["vperp"], P["aea"], P["ag"],
+               P["qind"], P["vparind"], P["vperpind"])
+    t = r.get("push_seconds")
+    if not t:
+        return None
+    n = len(P["x"])
+    return dict(value=n / t, unit="ion pushes/s (push only)", cores=1, kind="reference",
+                sample=f"oracle/_ref/kaos_ref_push (reference Fortran, flang -O2), {n} ions x 1 step, "
+                       f"{t:.2f} s; excludes binning and moments")
+
+
 def timed_run(form, tube, q0, v0, w0, cap, args):
     from kaos_amd.solver import KineticSolver, KAOS_FP64, KAOS_MIXED
     prec = KAOS_MIXED if form == "mixed" else KAOS_FP64
@@ -243,6 +271,9 @@ def main():
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline()
+        ref = cpu_reference_push()
+        if ref:
+            out["cpu_reference_push"] = ref
     print(json.dumps(out), flush=True)
 
 

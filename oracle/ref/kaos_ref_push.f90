@@ -21,6 +21,7 @@ program kaos_ref_push
   real(kind= dp), allocatable :: qL(:), qH(:), vpL(:), vpH(:), vrL(:), vrH(:)
   character(len= 1024) :: fin, fout
   integer :: u
+  integer(kind= 8) :: clk0, clk1, clkrate
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -89,9 +90,13 @@ program kaos_ref_push
   rank= 0
   n= fl(6)
   Time(1)= 0d0
+  call system_clock(clk0, clkrate)
   do j= 1, hns
     call KineticRK4UpdateSub
   end do
+  call system_clock(clk1)
+  ! wall time of the push loop alone (for the CPU baseline)
+  write(*, '(a, es14.6)') 'push_seconds ', real(clk1 - clk0, 8)/real(clkrate, 8)
   open(newunit= u, file= trim(fout), access= 'stream', form= 'unformatted', status= 'replace')
   write(u) xN, yN, zN, VxN, VyN, VzN, Vpar, qk4
   write(u) Qindk1, Vparindk1, Vperpindk1

@@ -53,4 +53,7 @@ def run(tube, x, y, z, Vx, Vy, Vz, vperp, aea, ag, qind, vparind, vperpind, n=5,
     out = {k: raw[i * ns:(i + 1) * ns].copy() for i, k in enumerate(names)}
     out.update(qind=ints[:ns].copy(), vparind=ints[ns:2 * ns].copy(), vperpind=ints[2 * ns:].copy())
     out["stdout"] = r.stdout
+    for line in r.stdout.splitlines():
+        if line.strip().startswith("push_seconds"):
+            out["push_seconds"] = float(line.split()[1])
     return out
