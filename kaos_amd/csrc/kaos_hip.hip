@@ -160,9 +160,16 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_FP64_MIN_BLOCKS) k_push_fp64(
   for (int i = tid; i < a.hwords; i += KAOS_BLOCK) s_hist[i] = 0u;
   __syncthreads();
   const int wq0 = s_wq0, wnq = s_wnq, wtube = s_wtube;
+  // software pipeline: the state of this thread's next ion (same tube) is
+  // loaded while the current ion is pushed
+  bool pf = false;
+  long long pf_j = 0;
+  double pf_r = 0.0, pf_v = 0.0, pf_q = 0.0;
+  uint32_t pf_c = 0u;
   for (int tile = t0; tile < t1; ++tile) {
     int tb = a.tile_tube[tile];
     if (tb != cur) {
+      pf = false;
       __syncthreads();
       T = a.tubes[tb];
       for (int i = tid; i < T.nq; i += KAOS_BLOCK) {
@@ -182,9 +189,19 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_FP64_MIN_BLOCKS) k_push_fp64(
     for (int it = 0; it < KAOS_PPT; ++it) {
       long long j = start + (long long)it * KAOS_BLOCK + tid;
       if (j >= end) break;
-      double rs = a.rs[j];
+      double rs, vs, wq;
+      uint32_t cold;
+      if (pf && pf_j == j) { rs = pf_r; vs = pf_v; wq = pf_q; cold = pf_c; }
+      else { rs = a.rs[j]; vs = a.vp[j]; wq = a.qk[j]; cold = a.cell[j]; }
+      {
+        long long jn = 0;
+        bool nxt = false;
+        if (it + 1 < KAOS_PPT) { jn = j + KAOS_BLOCK; nxt = jn < end; }
+        else if (tile + 1 < t1 && a.tile_tube[tile + 1] == cur) { jn = a.tile_start[tile + 1] + tid; nxt = jn < end; }
+        pf = nxt;
+        if (nxt) { pf_j = jn; pf_r = a.rs[jn]; pf_v = a.vp[jn]; pf_q = a.qk[jn]; pf_c = a.cell[jn]; }
+      }
       if (rs == 0.0) continue;
-      uint32_t cold = a.cell[j];
       int qprev = (int)(cold >> 16) - 1;
       if (a.do_remove && (qprev == 0 || qprev == -1)) {
         // BoundaryConditionsSub at a master step: ions whose cell of the previous
@@ -195,11 +212,10 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_FP64_MIN_BLOCKS) k_push_fp64(
       }
       ++npushed;
       int hint = (qprev >= 1) ? qprev : (qprev == 0 ? 1 : T.nq);
-      double vs = a.vp[j];
       double aea, ag;
       if (a.do_interp) {
         double ea, eg;
-        if (kaos_interp2_from(a.qk[j], s_qGC, s_EA, s_EG, T.nq, T.qGL0, T.qGHlast, T.desc, hint, &ea, &eg)) {
+        if (kaos_interp2_from(wq, s_qGC, s_EA, s_EG, T.nq, T.qGL0, T.qGHlast, T.desc, hint, &ea, &eg)) {
           aea = kaos_aea_of(ea, T.qsms, T.eambsign);
           ag = kaos_ag_of(eg, T.qsms);
           a.ae[j] = aea; a.ag[j] = ag;

@@ -245,9 +245,16 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
   for (int i = tid; i < a.hwords; i += KAOS_BLOCK) s_hist[i] = 0u;
   __syncthreads();
   const int wq0 = s_wq0, wnq = s_wnq, wtube = s_wtube;
+  // software pipeline: the state words of this thread's next ion (same tube)
+  // are loaded while the current ion is pushed
+  bool pf = false;
+  uint32_t pf_j = 0u, pf_c = 0u;
+  int pf_r = 0, pf_v = 0;
+  float pf_q = 0.0f;
   for (int tile = t0; tile < t1; ++tile) {
     int tb = a.tile_tube[tile];
     if (tb != cur) {
+      pf = false;
       __syncthreads();
       const DevTube& T = a.tubes[tb];
       nq = T.nq; nvpar = T.nvpar; nvperp = T.nvperp; desc = T.desc;
@@ -279,7 +286,25 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
       const uint32_t jl = start + (uint32_t)(it * KAOS_BLOCK + tid);
       if (jl >= tcount) break;
       const uint32_t b4 = jl * 4u;
-      uint32_t cold = kx_at(cellT, b4);
+      uint32_t cold;
+      int wr, wv;
+      float wq;
+      if (pf && pf_j == jl) { cold = pf_c; wr = pf_r; wv = pf_v; wq = pf_q; }
+      else { cold = kx_at(cellT, b4); wr = kx_at(rdiT, b4); wv = kx_at(vpiT, b4); wq = kx_at(qkfT, b4); }
+      {
+        uint32_t jn = 0u;
+        bool nxt = false;
+        if (it + 1 < KAOS_PPT) { jn = jl + KAOS_BLOCK; nxt = jn < tcount; }
+        else if (tile + 1 < t1 && a.tile_tube[tile + 1] == cur) {
+          jn = (uint32_t)(a.tile_start[tile + 1] - toff) + (uint32_t)tid;
+          nxt = jn < tcount;
+        }
+        pf = nxt;
+        if (nxt) {
+          const uint32_t bn = jn * 4u;
+          pf_j = jn; pf_c = kx_at(cellT, bn); pf_r = kx_at(rdiT, bn); pf_v = kx_at(vpiT, bn); pf_q = kx_at(qkfT, bn);
+        }
+      }
       if (cold == KAOS_DEAD) continue;
       int qprev = (int)((cold >> 16) & KAOS_QMASK) - 1;
       if (a.do_remove && (qprev == 0 || qprev == -1)) {
@@ -290,12 +315,12 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
       ++npushed;
       const int hint = kx_refq(qprev, nq);
       const int ipc = (int)(cold & 0xFFu), iq = (int)((cold >> 8) & 0xFFu);
-      const double rs = s_rref[hint - 1] + kx_dec(kx_at(rdiT, b4), s_rk[hint - 1], (cold & KAOS_RFLAG) != 0);
-      const double vs64 = kx_dec_v(kx_at(vpiT, b4), ipc, (cold & KAOS_VFLAG) != 0, s_vGH, s_vk, nvpar);
+      const double rs = s_rref[hint - 1] + kx_dec(wr, s_rk[hint - 1], (cold & KAOS_RFLAG) != 0);
+      const double vs64 = kx_dec_v(wv, ipc, (cold & KAOS_VFLAG) != 0, s_vGH, s_vk, nvpar);
       float aea, ag;
       if (a.do_interp) {
         float ea, eg;
-        if (kf_interp2_from(kx_at(qkfT, b4), s_qGC, s_EA, s_EG, nq, qGL0, qGHlast, desc, hint, ea, eg)) {
+        if (kf_interp2_from(wq, s_qGC, s_EA, s_EG, nq, qGL0, qGHlast, desc, hint, ea, eg)) {
           aea = kf_aea_of(ea, qsms, eambsign);
           ag = kf_ag_of(eg, qsms);
           kx_at(aefT, b4) = aea; kx_at(agfT, b4) = ag;
