@@ -1100,12 +1100,8 @@ static size_t fp64_lds_bytes(const kaos_ctx* c) {
   return sizeof(double) * (4 * (size_t)c->lds_nq + c->lds_nv) + sizeof(unsigned int) * c->launch.hwords;
 }
 
-// Dynamic LDS of k_push_mixed: binary64 rGH, rref, vparGH + GL0; binary32
-// qGC, EA, EG; the int exponents; the histogram window.
-static size_t mixed_lds_bytes(const kaos_ctx* c) {
-  return sizeof(double) * (2 * (size_t)c->lds_nq + c->lds_nv + 1) + sizeof(float) * 3 * (size_t)c->lds_nq +
-         sizeof(int) * ((size_t)c->lds_nq + c->lds_nv) + sizeof(unsigned int) * c->launch.hwords;
-}
+// Dynamic LDS of k_push_mixed: the histogram window (its grid arrays are static).
+static size_t mixed_lds_bytes(const kaos_ctx* c) { return sizeof(unsigned int) * c->launch.hwords; }
 
 // Moments of every (tube, cell): wavefront-per-cell kernel for tubes with <= 64
 // Vpar cells, block-per-cell kernel for the others.
@@ -1123,20 +1119,22 @@ static void launch_moments(kaos_ctx* c, const long long* inj) {
                        inj, c->d_cells_block);
 }
 
-// Persistent launches: this many blocks per CU (the push kernels' launch bounds).
-#define KAOS_PERSIST_BLOCKS_PER_CU 4
-
 static void launch_push(kaos_ctx* c, const StepArgs& a, hipEvent_t tune0 = nullptr, hipEvent_t tune1 = nullptr) {
   int nblocks = a.ngroups;
   if (a.persistent) {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, c->d_work, 1LL);
-    nblocks = std::min(nblocks, KAOS_PERSIST_BLOCKS_PER_CU * c->n_cu);
+    nblocks = std::min(nblocks, (c->cfg.precision == KAOS_MIXED ? KAOS_MIXED_MIN_BLOCKS : KAOS_FP64_MIN_BLOCKS) *
+                                    c->n_cu);
   }
   std::pair<hipEvent_t, hipEvent_t> ev;
   if (c->prof_on) ev = ev_pair(c);
   if (tune0) (void)hipEventRecord(tune0, c->stream);
-  if (c->cfg.precision == KAOS_MIXED)
-    hipLaunchKernelGGL(k_push_mixed, dim3(nblocks), dim3(KAOS_BLOCK), mixed_lds_bytes(c), c->stream, a);
+  if (c->cfg.precision == KAOS_MIXED) {
+    if (c->lds_nq <= 256)
+      hipLaunchKernelGGL(k_push_mixed<256>, dim3(nblocks), dim3(KAOS_BLOCK), mixed_lds_bytes(c), c->stream, a);
+    else
+      hipLaunchKernelGGL(k_push_mixed<KAOS_MAXQ>, dim3(nblocks), dim3(KAOS_BLOCK), mixed_lds_bytes(c), c->stream, a);
+  }
   else
     hipLaunchKernelGGL(k_push_fp64, dim3(nblocks), dim3(KAOS_BLOCK), fp64_lds_bytes(c), c->stream, a);
   if (tune1) (void)hipEventRecord(tune1, c->stream);

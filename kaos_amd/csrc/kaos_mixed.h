@@ -181,41 +181,38 @@ __device__ __forceinline__ double kx_dec_v(int w, int ip, bool vflag, const doub
 // Push + bin, mixed form.  Same step semantics as k_push_fp64 (removal at a
 // master step, pre-push binning, stale 2-D velocity index, stage-4 gravity
 // gate, field interpolation at the previous stage-4 q on master steps).
-// Dynamic LDS: binary64 rGH, rref (lds_nq each), vparGH + GL0 (lds_nv + 1);
-// binary32 qGC, EA, EG (lds_nq each); int rk (lds_nq), vk (lds_nv); the
-// histogram window.
+// Static LDS (grids of up to MAXQ cells, so the arrays sit at fixed offsets):
+// binary64 rGH, rref, vparGH + GL0; binary32 qGC, EA, EG; int rk, vk.
+// Dynamic LDS: the histogram window (a.hwords words).  Uniform values are
+// kept few (flag words, 32-bit offsets, h-multiples derived per ion) so the
+// loop does not spill scalar registers.
 #ifndef KAOS_MIXED_MIN_BLOCKS
-#define KAOS_MIXED_MIN_BLOCKS 4
+#define KAOS_MIXED_MIN_BLOCKS 5
 #endif
+template <int MAXQ>
 __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixed(StepArgs a) {
   KAOS_MIXED_CONTRACT
-  extern __shared__ double s_dyn[];
-  const int NQ = a.lds_nq, NV = a.lds_nv;
-  double* s_rGH = s_dyn;
-  double* s_rref = s_rGH + NQ;
-  double* s_vGH = s_rref + NQ;
-  float* s_qGC = (float*)(s_vGH + NV + 1);
-  float* s_EA = s_qGC + NQ;
-  float* s_EG = s_EA + NQ;
-  int* s_rk = (int*)(s_EG + NQ);
-  int* s_vk = s_rk + NQ;
-  unsigned int* s_hist = (unsigned int*)(s_vk + NV);   // a.hwords words
+  __shared__ double s_rGH[MAXQ], s_rref[MAXQ], s_vGH[KAOS_MAXV + 1];
+  __shared__ float s_qGC[MAXQ], s_EA[MAXQ], s_EG[MAXQ];
+  __shared__ int s_rk[MAXQ], s_vk[KAOS_MAXV + 1];
+  extern __shared__ unsigned int s_hist[];
   __shared__ unsigned int s_pushed;
   __shared__ int s_wq0, s_wnq, s_wtube, s_group;
   const int tid = threadIdx.x;
   int cur = -1;
+  const int sflags = (a.do_interp ? 1 : 0) | (a.do_count ? 2 : 0) | (a.do_remove ? 4 : 0) | (a.do_qk ? 8 : 0);
   // per-tube scalars and base pointers
-  long long hoff = 0, goff = 0, toff = 0;
+  int hoff = 0, goff = 0;
+  long long toff = 0;
   uint32_t tcount = 0;
   uint32_t* cellT = nullptr;
   int *rdiT = nullptr, *vpiT = nullptr;
   float *qkfT = nullptr, *aefT = nullptr, *agfT = nullptr;
-  int nq = 0, nvpar = 0, nvperp = 0, desc = 0, grav = 0, eamb = 0, eambsign = 0;
-  double rGL0 = 0.0, rGHlast = 0.0, rsgn = 1.0;
-  float qGL0 = 0.0f, qGHlast = 0.0f, qsms = 0.0f;
-  double ip = 0.0, irep = 0.0;
-  float h = 0.0f, h2 = 0.0f, hsq2 = 0.0f, h6 = 0.0f, hh6 = 0.0f;
-  double vscale = 0.0, hd = 0.0;
+  int nq = 0, nvpar = 0, nvperp = 0;
+  int tflags = 0;                      // desc | grav << 1 | eamb << 2 | eambsign << 3 | (rs < 0) << 4
+  double rGL0 = 0.0, rGHlast = 0.0;
+  float qGL0 = 0.0f, qGHlast = 0.0f, qsms = 0.0f, vscale = 0.0f;
+  double ip = 0.0, hd = 0.0;
   for (int pass = 0;; ++pass) {
   const int g = kaos_next_group(a, pass, &s_group);
   if (g < 0) break;
@@ -227,7 +224,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
     int tb0 = (t0 < t1) ? a.tile_tube[t0] : 0;
     const DevTube& T0 = a.tubes[tb0];
     int ncell = T0.nvpar * T0.nvperp;
-    int wnq = a.do_count ? (2 * a.hwords) / ncell : 0;
+    int wnq = (sflags & 2) ? (2 * a.hwords) / ncell : 0;
     if (wnq > T0.nq) wnq = T0.nq;
     int q0 = 1;
     if (t0 < t1 && wnq > 0) {
@@ -257,14 +254,14 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
       pf = false;
       __syncthreads();
       const DevTube& T = a.tubes[tb];
-      nq = T.nq; nvpar = T.nvpar; nvperp = T.nvperp; desc = T.desc;
-      grav = T.grav; eamb = T.eamb; eambsign = T.eambsign;
-      goff = T.goff; hoff = T.hoff;
-      qGL0 = (float)T.qGL0; qGHlast = (float)T.qGHlast; rGL0 = T.rGL0; rGHlast = T.rGHlast; rsgn = T.rsgn;
+      nq = T.nq; nvpar = T.nvpar; nvperp = T.nvperp;
+      tflags = (T.desc ? 1 : 0) | (T.grav ? 2 : 0) | (T.eamb ? 4 : 0) | (T.eambsign ? 8 : 0) |
+               (T.rsgn < 0.0 ? 16 : 0);
+      goff = (int)T.goff; hoff = (int)T.hoff;
+      qGL0 = (float)T.qGL0; qGHlast = (float)T.qGHlast; rGL0 = T.rGL0; rGHlast = T.rGHlast;
       qsms = (float)T.qsms;
-      h = (float)T.h; h2 = (float)(T.h / 2.0); hsq2 = (float)((T.h * T.h) / 2.0); h6 = (float)(T.h / 6.0);
-      hh6 = (float)((T.h * T.h) / 6.0); hd = T.h;
-      ip = 1.0 / T.p; irep = 1.0 / (KAOS_RE * T.p);
+      hd = T.h;
+      ip = 1.0 / T.p;
       for (int i = tid; i < nq; i += KAOS_BLOCK) {
         s_rGH[i] = a.rGH[goff + i]; s_rref[i] = a.rref[goff + i];
         s_qGC[i] = (float)a.qGC[goff + i]; s_EA[i] = (float)a.EA[goff + i]; s_EG[i] = (float)a.EG[goff + i];
@@ -276,7 +273,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
       tcount = (uint32_t)a.count[tb];
       cellT = a.cell + toff; rdiT = a.rdi + toff; vpiT = a.vpi + toff;
       qkfT = a.qkf + toff; aefT = a.aef + toff; agfT = a.agf + toff;
-      vscale = (double)nvpar / (a.vparGH[T.voff + nvpar - 1] - a.vparGH[T.voff + nvpar]);
+      vscale = (float)((double)nvpar / (a.vparGH[T.voff + nvpar - 1] - a.vparGH[T.voff + nvpar]));
       cur = tb;
       __syncthreads();
     }
@@ -307,7 +304,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
       }
       if (cold == KAOS_DEAD) continue;
       int qprev = (int)((cold >> 16) & KAOS_QMASK) - 1;
-      if (a.do_remove && (qprev == 0 || qprev == -1)) {
+      if ((sflags & 4) && (qprev == 0 || qprev == -1)) {
         kx_at(cellT, b4) = KAOS_DEAD;
         atomicAdd(&a.esc_acc[2 * cur + (qprev == 0 ? 0 : 1)], 1ull);
         continue;
@@ -318,9 +315,10 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
       const double rs = s_rref[hint - 1] + kx_dec(wr, s_rk[hint - 1], (cold & KAOS_RFLAG) != 0);
       const double vs64 = kx_dec_v(wv, ipc, (cold & KAOS_VFLAG) != 0, s_vGH, s_vk, nvpar);
       float aea, ag;
-      if (a.do_interp) {
+      const int eambsign = (tflags >> 3) & 1, grav = (tflags >> 1) & 1, eamb = (tflags >> 2) & 1;
+      if (sflags & 1) {
         float ea, eg;
-        if (kf_interp2_from(wq, s_qGC, s_EA, s_EG, nq, qGL0, qGHlast, desc, hint, ea, eg)) {
+        if (kf_interp2_from(wq, s_qGC, s_EA, s_EG, nq, qGL0, qGHlast, tflags & 1, hint, ea, eg)) {
           aea = kf_aea_of(ea, qsms, eambsign);
           ag = kf_ag_of(eg, qsms);
           kx_at(aefT, b4) = aea; kx_at(agfT, b4) = ag;
@@ -331,10 +329,14 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
         aea = kx_at(aefT, b4); ag = kx_at(agfT, b4);
       }
       // stage A on the line: cell from |r| (binary64); sin, cos, b-hat to ~1e-14
+      const float h = (float)hd, h2 = 0.5f * h, hsq2 = 0.5f * (h * h), h6 = h * (1.0f / 6.0f),
+                  hh6 = (h * h) * (1.0f / 6.0f);
       const double r = fabs(rs);
+      const double irep = ip * (1.0 / KAOS_RE);
       const double w = r * irep;
       const double omw = 1.0 - w;
-      const int qc = (rs * rsgn < 0.0) ? -1 : kf_find_rcell_from(r, s_rGH, rGL0, rGHlast, nq, hint);
+      const int qc = ((rs < 0.0) != ((tflags & 16) != 0)) ? -1
+                                                             : kf_find_rcell_from(r, s_rGH, rGL0, rGHlast, nq, hint);
       const double s64 = w * kx_rsq(w);
       double c64 = omw * kx_rsq(omw);
       if (rs < 0.0) c64 = -c64;
@@ -385,7 +387,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
       const float eR = bNR - bR, eZ = bNZ - bZ;
       const double vN = (vs64 - vs64 * (double)(0.5f * (eR * eR + eZ * eZ))) + (double)(bNR * dVR + bNZ * dVZ);
       // binning of the pre-push state: Vparind from the cell word
-      if (a.do_count && qc >= 1) {
+      if ((sflags & 2) && qc >= 1) {
         if (ipc >= 1 && iq >= 1) {
           int wq = qc - wq0;
           if (cur == wtube && wq >= 0 && wq < wnq) {
@@ -400,13 +402,13 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
       }
       if (rsN != rsN || vN != vN) set_status(a.status, 1u << KAOS_E_NAN);
       // stage-4 q = RE^2 z / r^3 at the full-step predictor
-      if (a.do_qk) {
+      if (sflags & 8) {
         const float rhoC = rho + dRC;
         const float u2C = rhoC * rhoC + zC * zC;
         kx_at(qkfT, b4) = (((float)KAOS_RE2) * zC) * (kf_rsq(u2C) * kf_rcp(u2C));
       }
       // next push's Vparind (stale when off the grid) and the new state words
-      const int ipn = kaos_find_vcell_guess(vN, s_vGH, s_vGH[nvpar], nvpar, vscale);
+      const int ipn = kaos_find_vcell_guess(vN, s_vGH, s_vGH[nvpar], nvpar, (double)vscale);
       bool rflag, vflag;
       const int Qn = kx_refq(qc, nq);
       kx_at(rdiT, b4) = kx_enc(rsN - s_rref[Qn - 1], s_rk[Qn - 1], rflag);
