@@ -117,10 +117,10 @@ __device__ __forceinline__ int kf_find_rcell_from(double r, const double* rGH, d
   return i + 1;
 }
 
-// Field interpolation at q in binary32 (as kaos_interp2_from).
-__device__ __forceinline__ int kf_interp2_from(float q, const float* qGC, const float* E, const float* E2,
-                                               int nq, float qGL0, float qGHlast, int desc, int hint,
-                                               float& out, float& out2) {
+// Field interpolation at q in binary32 (as kaos_interp2_from), on the cell
+// centres packed as (qGC, EA, EG, 0).
+__device__ __forceinline__ int kf_interp2_from(float q, const float4* cE, int nq, float qGL0, float qGHlast,
+                                               int desc, int hint, float& out, float& out2) {
   KAOS_MIXED_CONTRACT
   int ub = nq - 1;
   int lo = hint - 1;
@@ -128,22 +128,45 @@ __device__ __forceinline__ int kf_interp2_from(float q, const float* qGC, const 
   if (lo > ub) lo = ub;
   if (desc) {
     if (!(q <= qGL0) || !(q >= qGHlast)) return 0;
-    if (q >= qGC[0]) { out = E[0]; out2 = E2[0]; return 1; }
-    if (q <= qGC[ub]) { out = E[ub]; out2 = E2[ub]; return 1; }
-    while (lo > 1 && q >= qGC[lo - 1]) --lo;
-    while (!(q >= qGC[lo])) ++lo;
+    if (q >= cE[0].x) { out = cE[0].y; out2 = cE[0].z; return 1; }
+    if (q <= cE[ub].x) { out = cE[ub].y; out2 = cE[ub].z; return 1; }
+    while (lo > 1 && q >= cE[lo - 1].x) --lo;
+    while (!(q >= cE[lo].x)) ++lo;
   } else {
     if (!(q >= qGL0) || !(q <= qGHlast)) return 0;
-    if (q <= qGC[0]) { out = E[0]; out2 = E2[0]; return 1; }
-    if (q >= qGC[ub]) { out = E[ub]; out2 = E2[ub]; return 1; }
-    while (lo > 1 && q <= qGC[lo - 1]) --lo;
-    while (!(q <= qGC[lo])) ++lo;
+    if (q <= cE[0].x) { out = cE[0].y; out2 = cE[0].z; return 1; }
+    if (q >= cE[ub].x) { out = cE[ub].y; out2 = cE[ub].z; return 1; }
+    while (lo > 1 && q <= cE[lo - 1].x) --lo;
+    while (!(q <= cE[lo].x)) ++lo;
   }
-  float x1 = qGC[lo - 1], x2 = qGC[lo];
-  float f = (q - x1) * kf_rcp(x2 - x1);
-  out = E[lo - 1] + f * (E[lo] - E[lo - 1]);
-  out2 = E2[lo - 1] + f * (E2[lo] - E2[lo - 1]);
+  const float4 A = cE[lo - 1], B = cE[lo];
+  float f = (q - A.x) * kf_rcp(B.x - A.x);
+  out = A.y + f * (B.y - A.y);
+  out2 = A.z + f * (B.z - A.z);
   return 1;
+}
+
+// The same interpolation with the common case first: q between the centre of
+// the hint cell's walk start and one of its neighbours.  The three centres
+// are read together, without a loop; a q elsewhere (a longer walk, beyond the
+// first or last centre, off the grid, NaN) takes the walk above.  Both give
+// the same interval, so the same value.  Needs nq >= 2.
+__device__ __forceinline__ int kf_interp2_fast(float q, const float4* cE, int nq, float qGL0, float qGHlast,
+                                               int desc, int hint, float& out, float& out2) {
+  KAOS_MIXED_CONTRACT
+  const int ub = nq - 1;
+  const int lo = max(1, min(hint - 1, ub));
+  const float4 L = cE[lo - 1], M = cE[lo], H = cE[min(lo + 1, ub)];
+  const bool up = desc ? !(q >= M.x) : !(q <= M.x);
+  const bool ok = up ? ((desc ? (q >= H.x) : (q <= H.x)) && lo < ub) : (desc ? (q < L.x) : (q > L.x));
+  if (ok) {
+    const float4 A = up ? M : L, B = up ? H : M;
+    const float f = (q - A.x) * kf_rcp(B.x - A.x);
+    out = A.y + f * (B.y - A.y);
+    out2 = A.z + f * (B.z - A.z);
+    return 1;
+  }
+  return kf_interp2_from(q, cE, nq, qGL0, qGHlast, desc, hint, out, out2);
 }
 
 __device__ __forceinline__ float kf_aea_of(float e_interp, float qsms, int eambsign) {
@@ -193,7 +216,7 @@ template <int MAXQ>
 __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixed(StepArgs a) {
   KAOS_MIXED_CONTRACT
   __shared__ double s_rGH[MAXQ], s_rref[MAXQ], s_vGH[KAOS_MAXV + 1];
-  __shared__ float s_qGC[MAXQ], s_EA[MAXQ], s_EG[MAXQ];
+  __shared__ float4 s_cE[MAXQ];
   __shared__ int s_rk[MAXQ], s_vk[KAOS_MAXV + 1];
   extern __shared__ unsigned int s_hist[];
   __shared__ unsigned int s_pushed;
@@ -264,7 +287,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
       ip = 1.0 / T.p;
       for (int i = tid; i < nq; i += KAOS_BLOCK) {
         s_rGH[i] = a.rGH[goff + i]; s_rref[i] = a.rref[goff + i];
-        s_qGC[i] = (float)a.qGC[goff + i]; s_EA[i] = (float)a.EA[goff + i]; s_EG[i] = (float)a.EG[goff + i];
+        s_cE[i] = make_float4((float)a.qGC[goff + i], (float)a.EA[goff + i], (float)a.EG[goff + i], 0.0f);
         s_rk[i] = a.rk[goff + i];
       }
       for (int i = tid; i <= nvpar; i += KAOS_BLOCK) s_vGH[i] = a.vparGH[T.voff + i];
@@ -312,13 +335,21 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
       ++npushed;
       const int hint = kx_refq(qprev, nq);
       const int ipc = (int)(cold & 0xFFu), iq = (int)((cold >> 8) & 0xFFu);
-      const double rs = s_rref[hint - 1] + kx_dec(wr, s_rk[hint - 1], (cold & KAOS_RFLAG) != 0);
-      const double vs64 = kx_dec_v(wv, ipc, (cold & KAOS_VFLAG) != 0, s_vGH, s_vk, nvpar);
+      // decode, with the hint cell's |r| edges read alongside (no loop: the
+      // search below walks only when the ion has left the hint cell)
+      const double rlo_h = (hint > 1) ? s_rGH[hint - 2] : rGL0, rhi_h = s_rGH[hint - 1];
+      const int ipv = max(ipc, 1);
+      const bool rfl = (cold & KAOS_RFLAG) != 0, vfl = (cold & KAOS_VFLAG) != 0;
+      const double rs = s_rref[hint - 1] + (rfl ? (double)__int_as_float(wr) : ldexp((double)wr, -s_rk[hint - 1]));
+      const double voff = (double)__int_as_float(wv);
+      const double vs64 =
+          vfl ? (((voff > 0.0) ? s_vGH[nvpar - 1] : s_vGH[nvpar]) + voff)
+              : 0.5 * (((ipv == 1) ? s_vGH[nvpar] : s_vGH[ipv - 2]) + s_vGH[ipv - 1]) + ldexp((double)wv, -s_vk[ipv - 1]);
       float aea, ag;
       const int eambsign = (tflags >> 3) & 1, grav = (tflags >> 1) & 1, eamb = (tflags >> 2) & 1;
       if (sflags & 1) {
         float ea, eg;
-        if (kf_interp2_from(wq, s_qGC, s_EA, s_EG, nq, qGL0, qGHlast, tflags & 1, hint, ea, eg)) {
+        if (kf_interp2_fast(wq, s_cE, nq, qGL0, qGHlast, tflags & 1, hint, ea, eg)) {
           aea = kf_aea_of(ea, qsms, eambsign);
           ag = kf_ag_of(eg, qsms);
           kx_at(aefT, b4) = aea; kx_at(agfT, b4) = ag;
@@ -335,8 +366,12 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
       const double irep = ip * (1.0 / KAOS_RE);
       const double w = r * irep;
       const double omw = 1.0 - w;
-      const int qc = ((rs < 0.0) != ((tflags & 16) != 0)) ? -1
-                                                             : kf_find_rcell_from(r, s_rGH, rGL0, rGHlast, nq, hint);
+      int qc = hint;
+      {
+        const bool other_hemi = (rs < 0.0) != ((tflags & 16) != 0);
+        const bool in_hint = ((hint > 1) ? (r > rlo_h) : (r >= rlo_h)) && r <= rhi_h;
+        if (other_hemi || !in_hint) qc = other_hemi ? -1 : kf_find_rcell_from(r, s_rGH, rGL0, rGHlast, nq, hint);
+      }
       const double s64 = w * kx_rsq(w);
       double c64 = omw * kx_rsq(omw);
       if (rs < 0.0) c64 = -c64;
@@ -344,7 +379,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
       const double bR64 = (3.0 * (c64 * s64)) * isl64, bZ64 = (2.0 - 3.0 * w) * isl64;
       const double rho64 = r * s64, z64 = r * c64;
       const float vs = (float)vs64;
-      const float sf = (float)s64, cf = (float)c64;
+      const float cf = (float)c64;
       const float rho = (float)rho64, z = (float)z64;
       const float bR = (float)bR64, bZ = (float)bZ64;
       const float ap1 = kf_apar(cf <= 0.0f, ag, aea, grav, eamb, eambsign, true);
@@ -408,11 +443,23 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
         kx_at(qkfT, b4) = (((float)KAOS_RE2) * zC) * (kf_rsq(u2C) * kf_rcp(u2C));
       }
       // next push's Vparind (stale when off the grid) and the new state words
-      const int ipn = kaos_find_vcell_guess(vN, s_vGH, s_vGH[nvpar], nvpar, (double)vscale);
+      // (the cell of the stored V|| first, then the general search)
+      int ipn = ipc;
+      {
+        const double vlo = (ipv == 1) ? s_vGH[nvpar] : s_vGH[ipv - 2], vhi = s_vGH[ipv - 1];
+        if (!(ipc >= 1 && ((ipc > 1) ? (vN > vlo) : (vN >= vlo)) && vN <= vhi))
+          ipn = kaos_find_vcell_guess(vN, s_vGH, s_vGH[nvpar], nvpar, (double)vscale);
+      }
       bool rflag, vflag;
       const int Qn = kx_refq(qc, nq);
       kx_at(rdiT, b4) = kx_enc(rsN - s_rref[Qn - 1], s_rk[Qn - 1], rflag);
-      kx_at(vpiT, b4) = kx_enc_v(vN, ipn, s_vGH, s_vk, nvpar, vflag);
+      if (ipn >= 1) {
+        const double vref = 0.5 * (((ipn == 1) ? s_vGH[nvpar] : s_vGH[ipn - 2]) + s_vGH[ipn - 1]);
+        kx_at(vpiT, b4) = kx_enc(vN - vref, s_vk[ipn - 1], vflag);
+      } else {
+        vflag = true;
+        kx_at(vpiT, b4) = __float_as_int((float)(vN - ((vN > s_vGH[nvpar - 1]) ? s_vGH[nvpar - 1] : s_vGH[nvpar])));
+      }
       kx_at(cellT, b4) = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)(ipn >= 1 ? ipn : ipc) |
                   (rflag ? KAOS_RFLAG : 0u) | (vflag ? KAOS_VFLAG : 0u);
     }
