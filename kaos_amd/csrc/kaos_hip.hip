@@ -109,26 +109,96 @@ __device__ __forceinline__ int kaos_next_group(const StepArgs& a, int pass, int*
 // pre-push cells (_8_3_1_IonParticleCounts.f90:612-634); at a master step,
 // ions whose previous-push cell is 0/-1 are first marked dead
 // (_8_0_7_BoundaryConditions.f90:158-256).
-// Dynamic LDS: qGH, qGC, EA, EG (lds_nq each), vparGH (lds_nv); the window of
-// 16-bit phase-space counters (two per word).
+
+// Cell centres packed for the field interpolation: q, EA, EG.
+struct __align__(16) KaosCE64 { double q, ea, eg, pad; };
+
+// kaos_interp2_from on packed centres: the same walk, interval and formula.
+__device__ __forceinline__ int kd_interp2_from(double q, const KaosCE64* cE, int nq, double qGL0, double qGHlast,
+                                               int desc, int hint, double* out, double* out2) {
+  int ub = nq - 1;
+  int lo = hint - 1;
+  if (lo < 1) lo = 1;
+  if (lo > ub) lo = ub;
+  if (desc) {
+    if (!(q <= qGL0) || !(q >= qGHlast)) return 0;
+    if (q >= cE[0].q) { *out = cE[0].ea; *out2 = cE[0].eg; return 1; }
+    if (q <= cE[ub].q) { *out = cE[ub].ea; *out2 = cE[ub].eg; return 1; }
+    while (lo > 1 && q >= cE[lo - 1].q) --lo;
+    while (!(q >= cE[lo].q)) ++lo;
+  } else {
+    if (!(q >= qGL0) || !(q <= qGHlast)) return 0;
+    if (q <= cE[0].q) { *out = cE[0].ea; *out2 = cE[0].eg; return 1; }
+    if (q >= cE[ub].q) { *out = cE[ub].ea; *out2 = cE[ub].eg; return 1; }
+    while (lo > 1 && q <= cE[lo - 1].q) --lo;
+    while (!(q <= cE[lo].q)) ++lo;
+  }
+  const KaosCE64 A = cE[lo - 1], B = cE[lo];
+  double f = (q - A.q) / (B.q - A.q);
+  *out = A.ea + (f * (B.ea - A.ea));
+  *out2 = A.eg + (f * (B.eg - A.eg));
+  return 1;
+}
+
+// Common case first: q between the walk start's centre and one of its
+// neighbours (three centres read together, no loop); else the walk.  Both
+// give the same interval, so the same value.  Needs nq >= 2.
+__device__ __forceinline__ int kd_interp2_fast(double q, const KaosCE64* cE, int nq, double qGL0, double qGHlast,
+                                               int desc, int hint, double* out, double* out2) {
+  const int ub = nq - 1;
+  const int lo = max(1, min(hint - 1, ub));
+  const KaosCE64 L = cE[lo - 1], M = cE[lo], H = cE[min(lo + 1, ub)];
+  const bool up = desc ? !(q >= M.q) : !(q <= M.q);
+  const bool ok = up ? ((desc ? (q >= H.q) : (q <= H.q)) && lo < ub) : (desc ? (q < L.q) : (q > L.q));
+  if (ok) {
+    const double x1 = up ? M.q : L.q, x2 = up ? H.q : M.q;
+    const double e1 = up ? M.ea : L.ea, e2 = up ? H.ea : M.ea;
+    const double g1 = up ? M.eg : L.eg, g2 = up ? H.eg : M.eg;
+    double f = (q - x1) / (x2 - x1);
+    *out = e1 + (f * (e2 - e1));
+    *out2 = g1 + (f * (g2 - g1));
+    return 1;
+  }
+  return kd_interp2_from(q, cE, nq, qGL0, qGHlast, desc, hint, out, out2);
+}
+
+// Config cell of q: the hint cell tested first (both of its edges read
+// together), else kaos_find_qcell_from (same result).
+__device__ __forceinline__ int kd_find_qcell_fast(double q, const double* qGH, double qGL0, double qGHlast, int nq,
+                                                  int desc, int hint) {
+  const int i = max(0, min(hint - 1, nq - 1));
+  const double hi = qGH[i], lo = (i > 0) ? qGH[i - 1] : qGL0;
+  const bool ok = desc ? (((i > 0) ? (q < lo) : (q <= lo)) && q >= hi)
+                       : (((i > 0) ? (q > lo) : (q >= lo)) && q <= hi);
+  return ok ? i + 1 : kaos_find_qcell_from(q, qGH, qGL0, qGHlast, nq, desc, hint);
+}
+
+// Static LDS (grids of up to MAXQ cells): qGH, the packed centres, vparGH
+// (high edges, then GL0).  Dynamic LDS: the window of 16-bit phase-space
+// counters (two per word, a.hwords words).  Loads and stores address the
+// tube's segment through a uniform base and a 32-bit byte offset.
 #ifndef KAOS_FP64_MIN_BLOCKS
 #define KAOS_FP64_MIN_BLOCKS 4
 #endif
+template <int MAXQ>
 __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_FP64_MIN_BLOCKS) k_push_fp64(StepArgs a) {
-  extern __shared__ double s_dyn[];
-  double* s_qGH = s_dyn;
-  double* s_qGC = s_qGH + a.lds_nq;
-  double* s_EA = s_qGC + a.lds_nq;
-  double* s_EG = s_EA + a.lds_nq;
-  double* s_vparGH = s_EG + a.lds_nq;
-  unsigned int* s_hist = (unsigned int*)(s_vparGH + a.lds_nv);
+  __shared__ double s_qGH[MAXQ], s_vGH[KAOS_MAXV + 1];
+  __shared__ KaosCE64 s_cE[MAXQ];
+  extern __shared__ unsigned int s_hist[];
   __shared__ unsigned int s_pushed;
   __shared__ int s_wq0, s_wnq, s_wtube, s_group;
   const int tid = threadIdx.x;
   int cur = -1;
-  DevTube T;
-  long long end = 0;
-  double vpar_scale = 0.0, vparGL0 = 0.0;
+  const int sflags = (a.do_interp ? 1 : 0) | (a.do_count ? 2 : 0) | (a.do_remove ? 4 : 0) | (a.do_qk ? 8 : 0);
+  // per-tube scalars and base pointers
+  int hoff = 0, goff = 0, nq = 0, nvpar = 0, nvperp = 0, desc = 0;
+  uint32_t tcount = 0;
+  long long toff = 0;
+  double *rsT = nullptr, *vpT = nullptr, *qkT = nullptr, *aeT = nullptr, *agT = nullptr;
+  uint32_t* cellT = nullptr;
+  double qGL0 = 0.0, qGHlast = 0.0, qsms = 0.0, vscale = 0.0;
+  int eambsign = 0;
+  kaos_push_par_t P = kaos_push_par(1.0, 1.0, 1.0, 0.0, 0, 0, 0);
   for (int pass = 0;; ++pass) {
   const int g = kaos_next_group(a, pass, &s_group);
   if (g < 0) break;
@@ -141,9 +211,9 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_FP64_MIN_BLOCKS) k_push_fp64(
     // starting one cell below the first ion's cell of the previous step (ions are
     // stored near cell order and move less than a cell per step)
     int tb0 = (t0 < t1) ? a.tile_tube[t0] : 0;
-    DevTube T0 = a.tubes[tb0];
+    const DevTube& T0 = a.tubes[tb0];
     int ncell = T0.nvpar * T0.nvperp;
-    int wnq = a.do_count ? (2 * a.hwords) / ncell : 0;
+    int wnq = (sflags & 2) ? (2 * a.hwords) / ncell : 0;
     if (wnq > T0.nq) wnq = T0.nq;
     int q0 = 1;
     if (t0 < t1 && wnq > 0) {
@@ -163,97 +233,119 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_FP64_MIN_BLOCKS) k_push_fp64(
   // software pipeline: the state of this thread's next ion (same tube) is
   // loaded while the current ion is pushed
   bool pf = false;
-  long long pf_j = 0;
+  uint32_t pf_j = 0u, pf_c = 0u;
   double pf_r = 0.0, pf_v = 0.0, pf_q = 0.0;
-  uint32_t pf_c = 0u;
   for (int tile = t0; tile < t1; ++tile) {
     int tb = a.tile_tube[tile];
     if (tb != cur) {
       pf = false;
       __syncthreads();
-      T = a.tubes[tb];
-      for (int i = tid; i < T.nq; i += KAOS_BLOCK) {
-        s_qGH[i] = a.qGH[T.goff + i]; s_qGC[i] = a.qGC[T.goff + i];
-        s_EA[i] = a.EA[T.goff + i];   s_EG[i] = a.EG[T.goff + i];
+      const DevTube& T = a.tubes[tb];
+      nq = T.nq; nvpar = T.nvpar; nvperp = T.nvperp; desc = T.desc;
+      goff = (int)T.goff; hoff = (int)T.hoff;
+      qGL0 = T.qGL0; qGHlast = T.qGHlast; qsms = T.qsms; eambsign = T.eambsign;
+      P = kaos_push_par(T.h, T.p, T.cphi, T.sphi, T.grav, T.eamb, T.eambsign);
+      for (int i = tid; i < nq; i += KAOS_BLOCK) {
+        s_qGH[i] = a.qGH[goff + i];
+        KaosCE64 e;
+        e.q = a.qGC[goff + i]; e.ea = a.EA[goff + i]; e.eg = a.EG[goff + i]; e.pad = 0.0;
+        s_cE[i] = e;
       }
-      for (int i = tid; i < T.nvpar; i += KAOS_BLOCK) s_vparGH[i] = a.vparGH[T.voff + i];
-      __syncthreads();
+      for (int i = tid; i <= nvpar; i += KAOS_BLOCK) s_vGH[i] = a.vparGH[T.voff + i];
+      toff = T.off;
+      tcount = (uint32_t)a.count[tb];
+      rsT = a.rs + toff; vpT = a.vp + toff; qkT = a.qk + toff; aeT = a.ae + toff; agT = a.ag + toff;
+      cellT = a.cell + toff;
+      vscale = (double)nvpar / (a.vparGH[T.voff + nvpar - 1] - a.vparGH[T.voff + nvpar]);
       cur = tb;
-      end = T.off + a.count[tb];
-      vparGL0 = a.vparGH[T.voff + T.nvpar];
-      vpar_scale = (double)T.nvpar / (a.vparGH[T.voff + T.nvpar - 1] - vparGL0);
+      __syncthreads();
     }
-    const long long start = a.tile_start[tile];
-    kaos_push_par_t P = kaos_push_par(T.h, T.p, T.cphi, T.sphi, T.grav, T.eamb, T.eambsign);
+    const uint32_t start = (uint32_t)(a.tile_start[tile] - toff);
 #pragma unroll 1
     for (int it = 0; it < KAOS_PPT; ++it) {
-      long long j = start + (long long)it * KAOS_BLOCK + tid;
-      if (j >= end) break;
+      const uint32_t jl = start + (uint32_t)(it * KAOS_BLOCK + tid);
+      if (jl >= tcount) break;
+      const uint32_t b8 = jl * 8u, b4 = jl * 4u;
       double rs, vs, wq;
       uint32_t cold;
-      if (pf && pf_j == j) { rs = pf_r; vs = pf_v; wq = pf_q; cold = pf_c; }
-      else { rs = a.rs[j]; vs = a.vp[j]; wq = a.qk[j]; cold = a.cell[j]; }
+      if (pf && pf_j == jl) { rs = pf_r; vs = pf_v; wq = pf_q; cold = pf_c; }
+      else { rs = kx_at(rsT, b8); vs = kx_at(vpT, b8); wq = kx_at(qkT, b8); cold = kx_at(cellT, b4); }
       {
-        long long jn = 0;
+        uint32_t jn = 0u;
         bool nxt = false;
-        if (it + 1 < KAOS_PPT) { jn = j + KAOS_BLOCK; nxt = jn < end; }
-        else if (tile + 1 < t1 && a.tile_tube[tile + 1] == cur) { jn = a.tile_start[tile + 1] + tid; nxt = jn < end; }
+        if (it + 1 < KAOS_PPT) { jn = jl + KAOS_BLOCK; nxt = jn < tcount; }
+        else if (tile + 1 < t1 && a.tile_tube[tile + 1] == cur) {
+          jn = (uint32_t)(a.tile_start[tile + 1] - toff) + (uint32_t)tid;
+          nxt = jn < tcount;
+        }
         pf = nxt;
-        if (nxt) { pf_j = jn; pf_r = a.rs[jn]; pf_v = a.vp[jn]; pf_q = a.qk[jn]; pf_c = a.cell[jn]; }
+        if (nxt) {
+          pf_j = jn;
+          pf_r = kx_at(rsT, jn * 8u); pf_v = kx_at(vpT, jn * 8u); pf_q = kx_at(qkT, jn * 8u);
+          pf_c = kx_at(cellT, jn * 4u);
+        }
       }
       if (rs == 0.0) continue;
-      int qprev = (int)(cold >> 16) - 1;
-      if (a.do_remove && (qprev == 0 || qprev == -1)) {
+      const int qprev = (int)(cold >> 16) - 1;
+      if ((sflags & 4) && (qprev == 0 || qprev == -1)) {
         // BoundaryConditionsSub at a master step: ions whose cell of the previous
         // push is 0 (LB) or -1 (UB) leave the arrays (BC:158-256, BCReIndex:33-133)
-        a.rs[j] = 0.0;
+        kx_at(rsT, b8) = 0.0;
         atomicAdd(&a.esc_acc[2 * cur + (qprev == 0 ? 0 : 1)], 1ull);
         continue;
       }
       ++npushed;
-      int hint = (qprev >= 1) ? qprev : (qprev == 0 ? 1 : T.nq);
+      const int hint = (qprev >= 1) ? qprev : (qprev == 0 ? 1 : nq);
       double aea, ag;
-      if (a.do_interp) {
+      if (sflags & 1) {
         double ea, eg;
-        if (kaos_interp2_from(wq, s_qGC, s_EA, s_EG, T.nq, T.qGL0, T.qGHlast, T.desc, hint, &ea, &eg)) {
-          aea = kaos_aea_of(ea, T.qsms, T.eambsign);
-          ag = kaos_ag_of(eg, T.qsms);
-          a.ae[j] = aea; a.ag[j] = ag;
+        if (kd_interp2_fast(wq, s_cE, nq, qGL0, qGHlast, desc, hint, &ea, &eg)) {
+          aea = kaos_aea_of(ea, qsms, eambsign);
+          ag = kaos_ag_of(eg, qsms);
+          kx_at(aeT, b8) = aea; kx_at(agT, b8) = ag;
         } else {
-          aea = a.ae[j]; ag = a.ag[j];
+          aea = kx_at(aeT, b8); ag = kx_at(agT, b8);
         }
       } else {
-        aea = a.ae[j]; ag = a.ag[j];
+        aea = kx_at(aeT, b8); ag = kx_at(agT, b8);
       }
       kaos_line_t L = kaos_line_i(rs, P.irep);
       kaos_qb_t gA = kaos_stageA(L);
-      int qc = kaos_find_qcell_from(gA.q, s_qGH, T.qGL0, T.qGHlast, T.nq, T.desc, hint);
+      const int qc = kd_find_qcell_fast(gA.q, s_qGH, qGL0, qGHlast, nq, desc, hint);
       kaos_push_out_t o;
       kaos_push_ion(L, gA, qc, vs, ag, aea, &P, &o);
-      // velocity cells (2-D): off-grid keeps the previous index (KUA:1087-1127)
-      // Vperp is constant on this path (no wave heating), so its cell, found when
-      // the ion was binned first (k_bin_initial, k_inject), is carried in the cell word
-      int ip = kaos_find_vcell_guess(o.vpar, s_vparGH, vparGL0, T.nvpar, vpar_scale);
-      int iq = (int)((cold >> 8) & 0xFFu);
-      if (ip == 0) ip = (int)(cold & 0xFFu);
-      if (a.do_count && qc >= 1) {
+      // velocity cells (2-D): the stored Vpar cell first, then the search;
+      // off-grid keeps the previous index (KUA:1087-1127).  Vperp is constant on
+      // this path (no wave heating), so its cell, found when the ion was binned
+      // first (k_bin_initial, k_inject), is carried in the cell word.
+      const int ipc = (int)(cold & 0xFFu), iq = (int)((cold >> 8) & 0xFFu);
+      int ip = ipc;
+      {
+        const int ipx = max(ipc, 1);
+        const double vlo = (ipx == 1) ? s_vGH[nvpar] : s_vGH[ipx - 2], vhi = s_vGH[ipx - 1];
+        if (!(ipc >= 1 && ((ipc > 1) ? (o.vpar > vlo) : (o.vpar >= vlo)) && o.vpar <= vhi)) {
+          ip = kaos_find_vcell_guess(o.vpar, s_vGH, s_vGH[nvpar], nvpar, vscale);
+          if (ip == 0) ip = ipc;
+        }
+      }
+      if ((sflags & 2) && qc >= 1) {
         if (ip >= 1 && iq >= 1) {
-          int wq = qc - wq0;
-          if (cur == wtube && wq >= 0 && wq < wnq) {
-            int li = (wq * T.nvperp + (iq - 1)) * T.nvpar + (ip - 1);
+          int wql = qc - wq0;
+          if (cur == wtube && wql >= 0 && wql < wnq) {
+            int li = (wql * nvperp + (iq - 1)) * nvpar + (ip - 1);
             atomicAdd(&s_hist[li >> 1], 1u << (16 * (li & 1)));
           } else {
-            atomicAdd(&a.hist[T.hoff + ((long long)(qc - 1) * T.nvperp + (iq - 1)) * T.nvpar + (ip - 1)], 1u);
+            atomicAdd(&a.hist[hoff + ((long long)(qc - 1) * nvperp + (iq - 1)) * nvpar + (ip - 1)], 1u);
           }
         } else {
-          atomicAdd(&a.nqoff[T.goff + qc - 1], 1u);
+          atomicAdd(&a.nqoff[goff + qc - 1], 1u);
         }
       }
       if (o.rsN != o.rsN || o.vparN != o.vparN) set_status(a.status, 1u << KAOS_E_NAN);
-      a.rs[j] = o.rsN;
-      a.vp[j] = o.vparN;
-      if (a.do_qk) a.qk[j] = o.qk4;
-      a.cell[j] = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip;
+      kx_at(rsT, b8) = o.rsN;
+      kx_at(vpT, b8) = o.vparN;
+      if (sflags & 8) kx_at(qkT, b8) = o.qk4;
+      kx_at(cellT, b4) = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip;
     }
   }
   __syncthreads();
@@ -1096,12 +1188,8 @@ static std::pair<hipEvent_t, hipEvent_t> ev_pair(kaos_ctx* c) {
   return {a, b};
 }
 
-static size_t fp64_lds_bytes(const kaos_ctx* c) {
-  return sizeof(double) * (4 * (size_t)c->lds_nq + c->lds_nv) + sizeof(unsigned int) * c->launch.hwords;
-}
-
-// Dynamic LDS of k_push_mixed: the histogram window (its grid arrays are static).
-static size_t mixed_lds_bytes(const kaos_ctx* c) { return sizeof(unsigned int) * c->launch.hwords; }
+// Dynamic LDS of both push kernels: the histogram window (their grid arrays are static).
+static size_t push_lds_bytes(const kaos_ctx* c) { return sizeof(unsigned int) * c->launch.hwords; }
 
 // Moments of every (tube, cell): wavefront-per-cell kernel for tubes with <= 64
 // Vpar cells, block-per-cell kernel for the others.
@@ -1131,12 +1219,14 @@ static void launch_push(kaos_ctx* c, const StepArgs& a, hipEvent_t tune0 = nullp
   if (tune0) (void)hipEventRecord(tune0, c->stream);
   if (c->cfg.precision == KAOS_MIXED) {
     if (c->lds_nq <= 256)
-      hipLaunchKernelGGL(k_push_mixed<256>, dim3(nblocks), dim3(KAOS_BLOCK), mixed_lds_bytes(c), c->stream, a);
+      hipLaunchKernelGGL(k_push_mixed<256>, dim3(nblocks), dim3(KAOS_BLOCK), push_lds_bytes(c), c->stream, a);
     else
-      hipLaunchKernelGGL(k_push_mixed<KAOS_MAXQ>, dim3(nblocks), dim3(KAOS_BLOCK), mixed_lds_bytes(c), c->stream, a);
+      hipLaunchKernelGGL(k_push_mixed<KAOS_MAXQ>, dim3(nblocks), dim3(KAOS_BLOCK), push_lds_bytes(c), c->stream, a);
   }
+  else if (c->lds_nq <= 256)
+    hipLaunchKernelGGL(k_push_fp64<256>, dim3(nblocks), dim3(KAOS_BLOCK), push_lds_bytes(c), c->stream, a);
   else
-    hipLaunchKernelGGL(k_push_fp64, dim3(nblocks), dim3(KAOS_BLOCK), fp64_lds_bytes(c), c->stream, a);
+    hipLaunchKernelGGL(k_push_fp64<KAOS_MAXQ>, dim3(nblocks), dim3(KAOS_BLOCK), push_lds_bytes(c), c->stream, a);
   if (tune1) (void)hipEventRecord(tune1, c->stream);
   if (c->prof_on) { (void)hipEventRecord(ev.second, c->stream); c->ev_push.push_back(ev); }
 }
