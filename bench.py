@@ -15,7 +15,8 @@ The `batched` object is the many-tube workload: 512 independent tubes (L ~
 U[3,8], 64..256 cells, 64x64, log-uniform 1e4..1e6 ions each, T ~ U[0.1,1] eV),
 mixed form, every step replayed as one captured hipGraph, launch shape from
 the autotuner, 200 steps.
-The warmup steps include the autotuner's candidate steps (12 master steps).
+The warmup steps include the autotuner's steps (2 warm-up and 16 candidate
+master steps, then the choice).
 Prints one JSON line.  value = ion pushes (live ions summed over the timed
 steps, counted on the device) / wall time of the timed steps, inputs resident
 in HBM.  roofline: the push kernel's algorithmic HBM bytes per launch over its
@@ -208,7 +209,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=14)
+    ap.add_argument("--warmup", type=int, default=24)
     ap.add_argument("--ions", type=int, default=100_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--form", choices=("both", "fp64", "mixed"), default="both")

@@ -45,7 +45,8 @@ typedef struct kaos_cfg {
   int32_t device;        /* HIP device ordinal */
   uint64_t seed;         /* seed of the injection stream (used when no uniforms are passed) */
   int32_t reorder_every; /* master steps between on-device compaction + sort of the ions by
-                            cell (0 = library default, < 0 = never); results do not depend on it */
+                            cell (0 = library default, 128; < 0 = never); results do not
+                            depend on it */
   int32_t flags;         /* KAOS_FLAG_* (0 = defaults: autotuned launches, graph replay in kaos_run) */
 } kaos_cfg;
 

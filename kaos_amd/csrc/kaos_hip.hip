@@ -32,16 +32,24 @@
 #include "kaos_dipole.h"
 #include "../../include/kaos_hip.h"
 
-#define KAOS_BLOCK 256
-#define KAOS_PPT 4                           // ions per thread per tile
-#define KAOS_TILE (KAOS_BLOCK * KAOS_PPT)
+#define KAOS_BLOCK 256                       // threads of the generic per-ion kernels
+#define KAOS_TILE 1024                       // ions per tile
+#define KAOS_TILE_PPT (KAOS_TILE / KAOS_BLOCK)
+// Threads of the push blocks (256-thread blocks measured faster than 320 at
+// the same wavefronts per SIMD)
+#define KAOS_MIXED_BLOCK 256
+#define KAOS_FP64_BLOCK 256
 #define KAOS_MAXQ 512
 #define KAOS_MAXV 255
 #ifndef KAOS_HWORDS
 #define KAOS_HWORDS 6144                     // LDS histogram window: 12288 16-bit counters
 #endif
-#define KAOS_TPB 16                          // tiles per push block (<= 63: 16-bit counters)
-#define KAOS_REORDER_DEFAULT 16              // master steps between reorders by default
+#define KAOS_TPB 16                          // tiles per push group (<= 63: 16-bit counters)
+// Master steps between reorders by default (200-step averages on the 1e8-ion
+// tube: the push keeps its speed for ~100 steps after a reorder once its LDS
+// window spans the tube's velocity box, so reorders are rare).
+#define KAOS_REORDER_FP64 128
+#define KAOS_REORDER_MIXED 128
 
 struct DevTube {
   int nq, nvpar, nvperp, desc;
@@ -78,6 +86,7 @@ struct StepArgs {
   unsigned int* status;
   unsigned long long* pushed;                // live ions pushed (particle-steps)
   int do_interp, do_count, do_remove, do_qk;
+  const int4* vbox;                          // per tube: Vpar cells [x, x + y), Vperp cells [z, z + w)
   // launch shape: tiles per group, LDS histogram window (32-bit words), persistent
   // blocks taking groups from the counter `work` (else one group per block)
   int hwords, persistent, ngroups;
@@ -97,6 +106,47 @@ __device__ __forceinline__ int kaos_next_group(const StepArgs& a, int pass, int*
   __syncthreads();
   int g = *s_group;
   return (g < a.ngroups) ? g : -1;
+}
+
+// First config cell of a push group's histogram window of wnq cells: centred
+// on the cells (cell words of the previous push, 10-bit field qc + 1) of the
+// group's first and last ions, which bracket the group's cells when the ions
+// were last sorted.
+__device__ __forceinline__ int kaos_window_q0(const StepArgs& a, int t0, int t1, int tb0, int nq, long long jend,
+                                              int wnq) {
+  const long long j0 = a.tile_start[t0];
+  long long j1 = (a.tile_tube[t1 - 1] == tb0) ? a.tile_start[t1 - 1] + KAOS_TILE - 1 : jend - 1;
+  if (j1 >= jend) j1 = jend - 1;
+  int qa = 0, qb = 0;
+  if (j0 < jend) qa = (int)((a.cell[j0] >> 16) & 0x3FFu) - 1;
+  if (j1 >= j0 && j1 < jend) qb = (int)((a.cell[j1] >> 16) & 0x3FFu) - 1;
+  if (qa < 1 || qa > nq) qa = 0;
+  if (qb < 1 || qb > nq) qb = 0;
+  const int qm = (qa && qb) ? (qa + qb) / 2 : (qa ? qa : (qb ? qb : 1));
+  int q0 = qm - (wnq - 1) / 2;
+  if (q0 + wnq - 1 > nq) q0 = nq - wnq + 1;
+  if (q0 < 1) q0 = 1;
+  return q0;
+}
+
+// Flush of a push group's LDS window (16-bit counters, two per word) to the
+// global histogram.  The window holds config cells [wq0, wq0 + wnq) x the
+// tube's velocity box b (Vpar cells [b.x, b.x + b.y), Vperp cells [b.z, b.z +
+// b.w)), Vpar fastest; wavefront w of nw takes rows (cell, Vperp cell) w, w +
+// nw, ..., its lanes the row's Vpar cells.
+__device__ __forceinline__ void kaos_window_flush(const unsigned int* s_hist, unsigned int* hist, long long hoff,
+                                                  int nvperp, int nvpar, int wq0, int wnq, int4 b, int w, int nw,
+                                                  int lane) {
+  const int nrows = wnq * b.w;
+  for (int rr = w; rr < nrows; rr += nw) {
+    const int wql = rr / b.w, ir = rr - wql * b.w;
+    const long long g0 = hoff + ((long long)(wq0 - 1 + wql) * nvperp + (b.z - 1 + ir)) * nvpar + (b.x - 1);
+    for (int jr = lane; jr < b.y; jr += 64) {
+      const int li = rr * b.y + jr;
+      const unsigned int v = (s_hist[li >> 1] >> (16 * (li & 1))) & 0xFFFFu;
+      if (v) atomicAdd(&hist[g0 + jr], v);
+    }
+  }
 }
 
 #include "kaos_mixed.h"
@@ -177,16 +227,19 @@ __device__ __forceinline__ int kd_find_qcell_fast(double q, const double* qGH, d
 // (high edges, then GL0).  Dynamic LDS: the window of 16-bit phase-space
 // counters (two per word, a.hwords words).  Loads and stores address the
 // tube's segment through a uniform base and a 32-bit byte offset.
-#ifndef KAOS_FP64_MIN_BLOCKS
-#define KAOS_FP64_MIN_BLOCKS 4
+// (launch bounds: minimum resident wavefronts per SIMD, which caps the VGPRs)
+#ifndef KAOS_FP64_WAVES_EU
+#define KAOS_FP64_WAVES_EU 4
 #endif
 template <int MAXQ>
-__global__ void __launch_bounds__(KAOS_BLOCK, KAOS_FP64_MIN_BLOCKS) k_push_fp64(StepArgs a) {
+__global__ void __launch_bounds__(KAOS_FP64_BLOCK, KAOS_FP64_WAVES_EU) k_push_fp64(StepArgs a) {
+  constexpr int BLK = KAOS_FP64_BLOCK, PPT = KAOS_TILE / KAOS_FP64_BLOCK;
   __shared__ double s_qGH[MAXQ], s_vGH[KAOS_MAXV + 1];
   __shared__ KaosCE64 s_cE[MAXQ];
   extern __shared__ unsigned int s_hist[];
   __shared__ unsigned int s_pushed;
   __shared__ int s_wq0, s_wnq, s_wtube, s_group;
+  __shared__ int4 s_wbox;
   const int tid = threadIdx.x;
   int cur = -1;
   const int sflags = (a.do_interp ? 1 : 0) | (a.do_count ? 2 : 0) | (a.do_remove ? 4 : 0) | (a.do_qk ? 8 : 0);
@@ -207,29 +260,22 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_FP64_MIN_BLOCKS) k_push_fp64(
   unsigned int npushed = 0;
   if (tid == 0) {
     s_pushed = 0u;
-    // histogram window: config cells [wq0, wq0 + wnq) of the group's first tube,
-    // starting one cell below the first ion's cell of the previous step (ions are
-    // stored near cell order and move less than a cell per step)
+    // histogram window: config cells [wq0, wq0 + wnq) x the velocity box of the
+    // group's first tube (ions are stored near cell order and move less than a
+    // cell per step)
     int tb0 = (t0 < t1) ? a.tile_tube[t0] : 0;
     const DevTube& T0 = a.tubes[tb0];
-    int ncell = T0.nvpar * T0.nvperp;
-    int wnq = (sflags & 2) ? (2 * a.hwords) / ncell : 0;
+    const int4 bx = a.vbox[tb0];
+    int wnq = (sflags & 2) ? (2 * a.hwords) / (bx.y * bx.w) : 0;
     if (wnq > T0.nq) wnq = T0.nq;
     int q0 = 1;
-    if (t0 < t1 && wnq > 0) {
-      long long j0 = a.tile_start[t0];
-      if (j0 < T0.off + a.count[tb0]) {
-        int qprev = (int)(a.cell[j0] >> 16) - 1;
-        if (qprev >= 1) q0 = qprev - 1;
-      }
-      if (q0 < 1) q0 = 1;
-      if (q0 + wnq - 1 > T0.nq) q0 = T0.nq - wnq + 1;
-    }
-    s_wq0 = q0; s_wnq = wnq; s_wtube = tb0;
+    if (t0 < t1 && wnq > 0) q0 = kaos_window_q0(a, t0, t1, tb0, T0.nq, T0.off + a.count[tb0], wnq);
+    s_wq0 = q0; s_wnq = wnq; s_wtube = tb0; s_wbox = bx;
   }
-  for (int i = tid; i < a.hwords; i += KAOS_BLOCK) s_hist[i] = 0u;
+  for (int i = tid; i < a.hwords; i += BLK) s_hist[i] = 0u;
   __syncthreads();
   const int wq0 = s_wq0, wnq = s_wnq, wtube = s_wtube;
+  const int4 wb = s_wbox;
   // software pipeline: the state of this thread's next ion (same tube) is
   // loaded while the current ion is pushed
   bool pf = false;
@@ -245,13 +291,13 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_FP64_MIN_BLOCKS) k_push_fp64(
       goff = (int)T.goff; hoff = (int)T.hoff;
       qGL0 = T.qGL0; qGHlast = T.qGHlast; qsms = T.qsms; eambsign = T.eambsign;
       P = kaos_push_par(T.h, T.p, T.cphi, T.sphi, T.grav, T.eamb, T.eambsign);
-      for (int i = tid; i < nq; i += KAOS_BLOCK) {
+      for (int i = tid; i < nq; i += BLK) {
         s_qGH[i] = a.qGH[goff + i];
         KaosCE64 e;
         e.q = a.qGC[goff + i]; e.ea = a.EA[goff + i]; e.eg = a.EG[goff + i]; e.pad = 0.0;
         s_cE[i] = e;
       }
-      for (int i = tid; i <= nvpar; i += KAOS_BLOCK) s_vGH[i] = a.vparGH[T.voff + i];
+      for (int i = tid; i <= nvpar; i += BLK) s_vGH[i] = a.vparGH[T.voff + i];
       toff = T.off;
       tcount = (uint32_t)a.count[tb];
       rsT = a.rs + toff; vpT = a.vp + toff; qkT = a.qk + toff; aeT = a.ae + toff; agT = a.ag + toff;
@@ -262,8 +308,8 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_FP64_MIN_BLOCKS) k_push_fp64(
     }
     const uint32_t start = (uint32_t)(a.tile_start[tile] - toff);
 #pragma unroll 1
-    for (int it = 0; it < KAOS_PPT; ++it) {
-      const uint32_t jl = start + (uint32_t)(it * KAOS_BLOCK + tid);
+    for (int it = 0; it < PPT; ++it) {
+      const uint32_t jl = start + (uint32_t)(it * BLK + tid);
       if (jl >= tcount) break;
       const uint32_t b8 = jl * 8u, b4 = jl * 4u;
       double rs, vs, wq;
@@ -273,7 +319,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_FP64_MIN_BLOCKS) k_push_fp64(
       {
         uint32_t jn = 0u;
         bool nxt = false;
-        if (it + 1 < KAOS_PPT) { jn = jl + KAOS_BLOCK; nxt = jn < tcount; }
+        if (it + 1 < PPT) { jn = jl + BLK; nxt = jn < tcount; }
         else if (tile + 1 < t1 && a.tile_tube[tile + 1] == cur) {
           jn = (uint32_t)(a.tile_start[tile + 1] - toff) + (uint32_t)tid;
           nxt = jn < tcount;
@@ -330,9 +376,10 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_FP64_MIN_BLOCKS) k_push_fp64(
       }
       if ((sflags & 2) && qc >= 1) {
         if (ip >= 1 && iq >= 1) {
-          int wql = qc - wq0;
-          if (cur == wtube && wql >= 0 && wql < wnq) {
-            int li = (wql * nvperp + (iq - 1)) * nvpar + (ip - 1);
+          const int wql = qc - wq0, jr = ip - wb.x, ir = iq - wb.z;
+          if (cur == wtube && (unsigned)wql < (unsigned)wnq && (unsigned)jr < (unsigned)wb.y &&
+              (unsigned)ir < (unsigned)wb.w) {
+            const int li = (wql * wb.w + ir) * wb.y + jr;
             atomicAdd(&s_hist[li >> 1], 1u << (16 * (li & 1)));
           } else {
             atomicAdd(&a.hist[hoff + ((long long)(qc - 1) * nvperp + (iq - 1)) * nvpar + (ip - 1)], 1u);
@@ -352,13 +399,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_FP64_MIN_BLOCKS) k_push_fp64(
   atomicAdd(&s_pushed, npushed);
   if (wnq > 0) {
     const DevTube& TW = a.tubes[wtube];
-    long long base = TW.hoff + (long long)(wq0 - 1) * TW.nvperp * TW.nvpar;
-    int nwords = (wnq * TW.nvperp * TW.nvpar + 1) >> 1;
-    for (int w = tid; w < nwords; w += KAOS_BLOCK) {
-      unsigned int v = s_hist[w];
-      if (v & 0xFFFFu) atomicAdd(&a.hist[base + 2 * w], v & 0xFFFFu);
-      if (v >> 16) atomicAdd(&a.hist[base + 2 * w + 1], v >> 16);
-    }
+    kaos_window_flush(s_hist, a.hist, TW.hoff, TW.nvperp, TW.nvpar, wq0, wnq, wb, tid >> 6, BLK >> 6, tid & 63);
   }
   __syncthreads();
   if (tid == 0 && s_pushed) atomicAdd(a.pushed, (unsigned long long)s_pushed);
@@ -375,7 +416,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_bin_initial(StepArgs a) {
   int tb = a.tile_tube[tile];
   DevTube T = a.tubes[tb];
   long long end = T.off + a.count[tb];
-  for (int it = 0; it < KAOS_PPT; ++it) {
+  for (int it = 0; it < KAOS_TILE_PPT; ++it) {
     long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
     if (j >= end) break;
     uint32_t cw = a.cell[j];
@@ -517,6 +558,21 @@ __global__ void k_inject(StepArgs a, const long long* inj_base, const long long*
   }
 }
 
+// Vperp-row sums of one cell's counts added to the tube's row marginals:
+// thread i of n sums rows i, i + n, ... serially (the rows were just read by
+// the moment sums, so these loads hit the cache; the column marginals come
+// from the moment kernels' column counts).
+__device__ __forceinline__ void kaos_row_marginals(const uint32_t* N, int np, int nr, int i, int n,
+                                                   unsigned int* vmr) {
+  for (int r = i; r < nr; r += n) {
+    const uint32_t* row = N + r * np;
+    unsigned int x = 0u;
+#pragma unroll 8
+    for (int b = 0; b < np; ++b) x += row[b];
+    if (x) atomicAdd(&vmr[r], x);
+  }
+}
+
 // Counts -> distribution function -> moments of one (tube, config cell) per
 // block (_8_3_1, _8_4_1, _8_5, _8_6_1.._8_6_6), one pass over the counts: lane
 // (g, b) sums column b over rows g, g + G, ...; lane b adds the G partials in
@@ -528,7 +584,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments(const DevTube* tubes, co
     const int* blk_q, const double* d3xC, const double* d3v, const double* dvpar, const double* dvperp,
     const double* vparGC, const double* vperpGC, const unsigned int* hist, const unsigned int* nqoff,
     unsigned int* nqc, double* NqRT, double* M, long long* count, long long* next_id,
-    const long long* inj_cnt, const int* cells) {
+    const long long* inj_cnt, const int* cells, unsigned int* vmp, unsigned int* vmr) {
   __shared__ double s_P[KAOS_MOM_LANES][3];
   __shared__ unsigned int s_pc[KAOS_MOM_LANES];
   __shared__ double s_A[KAOS_MAXV + 1][3];
@@ -561,6 +617,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments(const DevTube* tubes, co
     }
     for (int i = 0; i < 3; ++i) s_A[tid][i] = A[i];
     s_cc[tid] = cs;
+    if (cs) atomicAdd(&vmp[T.voff + tid], cs);
     double o4[4];
     kaos_mom_col_first(v.vparGC[tid], A, o4);
     for (int i = 0; i < 4; ++i) s_x[i][tid] = o4[i];
@@ -571,6 +628,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments(const DevTube* tubes, co
       for (int i = 0; i < 4; ++i) s_x[i][tid] = s_x[i][tid] + s_x[i][tid + st];
     __syncthreads();
   }
+  kaos_row_marginals(N, np, T.nvperp, tid, KAOS_BLOCK, vmr + T.woff);
   if (tid == 0) {
     unsigned int tot = nqoff[T.goff + Q];
     for (int c = 0; c < np; ++c) tot += s_cc[c];
@@ -619,7 +677,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments_wave(const DevTube* tube
     const int* blk_q, const int* cells, int ncells, const double* d3xC, const double* d3v,
     const double* dvpar, const double* dvperp, const double* vparGC, const double* vperpGC,
     const unsigned int* hist, const unsigned int* nqoff, unsigned int* nqc, double* NqRT, double* M,
-    long long* count, long long* next_id, const long long* inj_cnt) {
+    long long* count, long long* next_id, const long long* inj_cnt, unsigned int* vmp, unsigned int* vmr) {
   const int lane = threadIdx.x & 63;
   const int ci = blockIdx.x * (KAOS_BLOCK / 64) + (threadIdx.x >> 6);
   if (ci >= ncells) return;
@@ -639,6 +697,8 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments_wave(const DevTube* tube
     uc = v.vparGC[lane];
     kaos_mom_col_first(uc, A, o4);
   }
+  if (lane < np && cs) atomicAdd(&vmp[T.voff + lane], cs);
+  kaos_row_marginals(N, np, T.nvperp, lane, 64, vmr + T.woff);
   for (int st = 1; st < 64; st *= 2) cs += __shfl_xor(cs, st, 64);   // integer: any order
   double S[4];
   for (int i = 0; i < 4; ++i) S[i] = __shfl(kaos_wave_tree(o4[i], lane, np), 0, 64);
@@ -656,6 +716,46 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments_wave(const DevTube* tube
     for (int i = 0; i < 6; ++i) M[6 * T.goff + (long long)i * T.nq + Q] = Mq[i];
     if (Q == 0 && inj_cnt) { count[t] += inj_cnt[t]; next_id[t] += inj_cnt[t]; }
   }
+}
+
+// Velocity box of each tube's LDS histogram window, from the marginal counts
+// of the master step just binned: the Vpar and Vperp index ranges that leave
+// out at most 2.5e-5 of the binned ions at each end, widened by one cell for
+// the next step's moves (all-zero marginals: the whole grid).  The marginals
+// are cleared for the next master step.  Counts outside the box go to global
+// atomics, so no result depends on it.
+__device__ void kaos_box_range(const unsigned int* m, int n, int* lo, int* cnt) {
+  unsigned long long tot = 0;
+  for (int i = 0; i < n; ++i) tot += m[i];
+  if (tot == 0) { *lo = 0; *cnt = n; return; }
+  const unsigned long long thr = tot / 40000ull;
+  unsigned long long c = 0;
+  int a = 0;
+  while (a < n - 1 && c + m[a] <= thr) { c += m[a]; ++a; }
+  int b = n - 1;
+  c = 0;
+  while (b > a && c + m[b] <= thr) { c += m[b]; --b; }
+  a = max(a - 1, 0);
+  b = min(b + 1, n - 1);
+  *lo = a; *cnt = b - a + 1;
+}
+
+__global__ void k_vbox(const DevTube* tubes, unsigned int* vmp, unsigned int* vmr, int4* box) {
+  const DevTube& T = tubes[blockIdx.x];
+  __shared__ int s_b[4];
+  if (threadIdx.x == 0) {
+    int lo, n;
+    kaos_box_range(vmp + T.voff, T.nvpar, &lo, &n);
+    s_b[0] = lo + 1; s_b[1] = n;
+  } else if (threadIdx.x == 1) {
+    int lo, n;
+    kaos_box_range(vmr + T.woff, T.nvperp, &lo, &n);
+    s_b[2] = lo + 1; s_b[3] = n;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) box[blockIdx.x] = make_int4(s_b[0], s_b[1], s_b[2], s_b[3]);
+  for (int i = threadIdx.x; i < T.nvpar; i += blockDim.x) vmp[T.voff + i] = 0u;
+  for (int i = threadIdx.x; i < T.nvperp; i += blockDim.x) vmr[T.woff + i] = 0u;
 }
 
 // Reorder (compaction + counting sort by config cell), run every few master
@@ -681,7 +781,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_reorder_count(StepArgs a, int mi
   for (int k = threadIdx.x; k < nk; k += KAOS_BLOCK) s_cnt[k] = 0u;
   __syncthreads();
   long long end = T.off + a.count[tb];
-  for (int it = 0; it < KAOS_PPT; ++it) {
+  for (int it = 0; it < KAOS_TILE_PPT; ++it) {
     long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
     if (j >= end) break;
     int key = reorder_key(a, j, mixed);
@@ -719,8 +819,8 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_reorder_scatter(StepArgs a, Step
   for (int k = threadIdx.x; k < nk; k += KAOS_BLOCK) s_cnt[k] = 0u;
   __syncthreads();
   long long end = T.off + a.count[tb];
-  int key[KAOS_PPT];
-  for (int it = 0; it < KAOS_PPT; ++it) {
+  int key[KAOS_TILE_PPT];
+  for (int it = 0; it < KAOS_TILE_PPT; ++it) {
     long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
     key[it] = (j < end) ? reorder_key(a, j, mixed) : -1;
     if (key[it] >= 0) atomicAdd(&s_cnt[key[it]], 1u);
@@ -732,7 +832,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_reorder_scatter(StepArgs a, Step
     s_cnt[k] = 0u;
   }
   __syncthreads();
-  for (int it = 0; it < KAOS_PPT; ++it) {
+  for (int it = 0; it < KAOS_TILE_PPT; ++it) {
     if (key[it] < 0) continue;
     long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
     long long d = (long long)(s_base[key[it]] + atomicAdd(&s_cnt[key[it]], 1u));
@@ -841,10 +941,13 @@ struct kaos_ctx {
   struct Launch { int tpb, hwords, persistent; };
   Launch launch = {KAOS_TPB, KAOS_HWORDS, 0};
   unsigned int* d_work = nullptr;
+  int4* d_vbox = nullptr;                       // per tube velocity box of the push's LDS window
+  unsigned int *d_vmarg_par = nullptr, *d_vmarg_perp = nullptr;
   long long* d_stepno = nullptr;
   int n_cu = 256;
   bool tuning = false;
   size_t tune_i = 0;
+  int tune_skip = 2;                            // master steps run before the first candidate
   std::vector<Launch> tune_cand;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> tune_ev;
   std::map<unsigned long long, hipGraphExec_t> graphs;
@@ -889,7 +992,7 @@ extern "C" int kaos_init(const kaos_cfg* cfg, kaos_ctx** out) {
     c->tuning = true;
     for (int pers = 0; pers < 2; ++pers)
       for (int hw : {4096, 6144})
-        for (int tpb : {8, 16, 32}) c->tune_cand.push_back({tpb, hw, pers});
+        for (int tpb : {8, 16}) c->tune_cand.push_back({tpb, hw, pers});
   }
   c->ht.resize(cfg->ntubes);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1092,6 +1195,15 @@ static int build_device(kaos_ctx* c) {
   HIPCHK(c, hipMemset(c->d_nqc, 0, sizeof(unsigned int) * (size_t)goff));
   HIPCHK(c, hipMemset(c->d_nqoff, 0, sizeof(unsigned int) * (size_t)goff));
   HIPCHK(c, hipMalloc((void**)&c->d_work, sizeof(unsigned int)));
+  {
+    std::vector<int4> bx(nt);
+    for (int t = 0; t < nt; ++t) bx[t] = make_int4(1, c->dt[t].nvpar, 1, c->dt[t].nvperp);
+    HIPCHK(c, up((void**)&c->d_vbox, bx.data(), sizeof(int4) * nt));
+    HIPCHK(c, hipMalloc((void**)&c->d_vmarg_par, sizeof(unsigned int) * (size_t)voff));
+    HIPCHK(c, hipMalloc((void**)&c->d_vmarg_perp, sizeof(unsigned int) * (size_t)woff));
+    HIPCHK(c, hipMemset(c->d_vmarg_par, 0, sizeof(unsigned int) * (size_t)voff));
+    HIPCHK(c, hipMemset(c->d_vmarg_perp, 0, sizeof(unsigned int) * (size_t)woff));
+  }
   HIPCHK(c, hipMalloc((void**)&c->d_stepno, sizeof(long long)));
   HIPCHK(c, hipMemset(c->d_stepno, 0, sizeof(long long)));
   HIPCHK(c, hipMalloc((void**)&c->d_pushed, sizeof(unsigned long long)));
@@ -1139,6 +1251,7 @@ static StepArgs make_args(kaos_ctx* c, int interp, int count, int remove, int qk
   a.persistent = c->launch.persistent;
   a.ngroups = (c->ntiles + c->launch.tpb - 1) / c->launch.tpb;
   a.work = c->d_work;
+  a.vbox = c->d_vbox;
   a.do_interp = interp; a.do_count = count; a.do_remove = remove; a.do_qk = qk;
   return a;
 }
@@ -1199,63 +1312,78 @@ static void launch_moments(kaos_ctx* c, const long long* inj) {
     hipLaunchKernelGGL(k_moments_wave, dim3((c->n_cells_wave + wpb - 1) / wpb), dim3(KAOS_BLOCK), 0, c->stream,
                        c->d_tubes, c->d_blk_tube, c->d_blk_q, c->d_cells_wave, c->n_cells_wave, c->d_d3x, c->d_d3v,
                        c->d_dvpar, c->d_dvperp, c->d_vparGC, c->d_vperpGC, c->d_hist, c->d_nqoff, c->d_nqc,
-                       c->d_NqRT, c->d_M, c->d_count, c->d_next_id, inj);
+                       c->d_NqRT, c->d_M, c->d_count, c->d_next_id, inj, c->d_vmarg_par, c->d_vmarg_perp);
   if (c->n_cells_block)
     hipLaunchKernelGGL(k_moments, dim3(c->n_cells_block), dim3(KAOS_BLOCK), 0, c->stream, c->d_tubes,
                        c->d_blk_tube, c->d_blk_q, c->d_d3x, c->d_d3v, c->d_dvpar, c->d_dvperp, c->d_vparGC,
                        c->d_vperpGC, c->d_hist, c->d_nqoff, c->d_nqc, c->d_NqRT, c->d_M, c->d_count, c->d_next_id,
-                       inj, c->d_cells_block);
+                       inj, c->d_cells_block, c->d_vmarg_par, c->d_vmarg_perp);
+  hipLaunchKernelGGL(k_vbox, dim3(c->cfg.ntubes), dim3(64), 0, c->stream, c->d_tubes, c->d_vmarg_par,
+                     c->d_vmarg_perp, c->d_vbox);
 }
 
 static void launch_push(kaos_ctx* c, const StepArgs& a, hipEvent_t tune0 = nullptr, hipEvent_t tune1 = nullptr) {
   int nblocks = a.ngroups;
   if (a.persistent) {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, c->d_work, 1LL);
-    nblocks = std::min(nblocks, (c->cfg.precision == KAOS_MIXED ? KAOS_MIXED_MIN_BLOCKS : KAOS_FP64_MIN_BLOCKS) *
-                                    c->n_cu);
+    // as many blocks as are resident at the kernel's wavefronts per SIMD (4 SIMDs per CU)
+    const int per_cu = c->cfg.precision == KAOS_MIXED ? KAOS_MIXED_WAVES_EU * 256 / KAOS_MIXED_BLOCK
+                                                      : KAOS_FP64_WAVES_EU * 256 / KAOS_FP64_BLOCK;
+    nblocks = std::min(nblocks, per_cu * c->n_cu);
   }
   std::pair<hipEvent_t, hipEvent_t> ev;
   if (c->prof_on) ev = ev_pair(c);
   if (tune0) (void)hipEventRecord(tune0, c->stream);
   if (c->cfg.precision == KAOS_MIXED) {
     if (c->lds_nq <= 256)
-      hipLaunchKernelGGL(k_push_mixed<256>, dim3(nblocks), dim3(KAOS_BLOCK), push_lds_bytes(c), c->stream, a);
+      hipLaunchKernelGGL(k_push_mixed<256>, dim3(nblocks), dim3(KAOS_MIXED_BLOCK), push_lds_bytes(c), c->stream, a);
     else
-      hipLaunchKernelGGL(k_push_mixed<KAOS_MAXQ>, dim3(nblocks), dim3(KAOS_BLOCK), push_lds_bytes(c), c->stream, a);
+      hipLaunchKernelGGL(k_push_mixed<KAOS_MAXQ>, dim3(nblocks), dim3(KAOS_MIXED_BLOCK), push_lds_bytes(c), c->stream,
+                         a);
   }
   else if (c->lds_nq <= 256)
-    hipLaunchKernelGGL(k_push_fp64<256>, dim3(nblocks), dim3(KAOS_BLOCK), push_lds_bytes(c), c->stream, a);
+    hipLaunchKernelGGL(k_push_fp64<256>, dim3(nblocks), dim3(KAOS_FP64_BLOCK), push_lds_bytes(c), c->stream, a);
   else
-    hipLaunchKernelGGL(k_push_fp64<KAOS_MAXQ>, dim3(nblocks), dim3(KAOS_BLOCK), push_lds_bytes(c), c->stream, a);
+    hipLaunchKernelGGL(k_push_fp64<KAOS_MAXQ>, dim3(nblocks), dim3(KAOS_FP64_BLOCK), push_lds_bytes(c), c->stream, a);
   if (tune1) (void)hipEventRecord(tune1, c->stream);
   if (c->prof_on) { (void)hipEventRecord(ev.second, c->stream); c->ev_push.push_back(ev); }
 }
 
-// Online autotuner: the first master steps each run the push with one
-// candidate launch shape (tiles per group, histogram window, grid or
-// persistent); the next master step reads the timings and keeps the fastest.
-// Results do not depend on the launch shape (integer histograms, independent
-// ions), so tuning steps are ordinary steps.
+// Online autotuner: after two warm-up master steps, the next master steps
+// each run the push with one candidate launch shape (tiles per group,
+// histogram window, grid or persistent), every candidate twice in two passes;
+// the next master step reads the timings and keeps the candidate with the
+// smallest of its two times (two passes, so that a trend over the first steps
+// does not favour the candidates timed last).  Results do not depend on the
+// launch shape (integer histograms, independent ions), so tuning steps are
+// ordinary steps.
 static int tune_before_push(kaos_ctx* c, hipEvent_t* e0, hipEvent_t* e1) {
   *e0 = *e1 = nullptr;
-  if (c->tune_i < c->tune_cand.size()) {
+  if (c->tune_skip > 0) {
+    --c->tune_skip;
+    return KAOS_OK;
+  }
+  const size_t nc = c->tune_cand.size();
+  if (c->tune_i < 2 * nc) {
     std::pair<hipEvent_t, hipEvent_t> ev;
     HIPCHK(c, hipEventCreate(&ev.first));
     HIPCHK(c, hipEventCreate(&ev.second));
     c->tune_ev.push_back(ev);
-    c->launch = c->tune_cand[c->tune_i++];
+    c->launch = c->tune_cand[c->tune_i++ % nc];
     *e0 = ev.first; *e1 = ev.second;
     return KAOS_OK;
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  float best = 0.f;
-  size_t ib = 0;
+  std::vector<float> best(nc, 0.f);
   for (size_t i = 0; i < c->tune_ev.size(); ++i) {
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->tune_ev[i].first, c->tune_ev[i].second));
-    if (i == 0 || ms < best) { best = ms; ib = i; }
+    if (i < nc || ms < best[i % nc]) best[i % nc] = ms;
     (void)hipEventDestroy(c->tune_ev[i].first); (void)hipEventDestroy(c->tune_ev[i].second);
   }
+  size_t ib = 0;
+  for (size_t i = 1; i < nc; ++i)
+    if (best[i] < best[ib]) ib = i;
   c->launch = c->tune_cand[ib];
   c->tune_ev.clear();
   c->tuning = false;
@@ -1264,7 +1392,8 @@ static int tune_before_push(kaos_ctx* c, hipEvent_t* e0, hipEvent_t* e1) {
 
 // Master steps between reorders: cfg.reorder_every, 0 = the default, < 0 = never.
 static int reorder_period(const kaos_ctx* c) {
-  return c->cfg.reorder_every == 0 ? KAOS_REORDER_DEFAULT : c->cfg.reorder_every;
+  if (c->cfg.reorder_every != 0) return c->cfg.reorder_every;
+  return c->cfg.precision == KAOS_MIXED ? KAOS_REORDER_MIXED : KAOS_REORDER_FP64;
 }
 
 // Compact and sort every tube's ions by cell into the other state buffer, swap,
@@ -1602,7 +1731,7 @@ extern "C" int kaos_download_particles(kaos_ctx* c, int32_t tube, int64_t n_cap,
 
 extern "C" int kaos_set_launch(kaos_ctx* c, int32_t tiles_per_group, int32_t hist_words, int32_t persistent) {
   if (!c) return KAOS_E_ARG;
-  if (tiles_per_group < 1 || tiles_per_group > 63)
+  if (tiles_per_group < 1 || tiles_per_group * KAOS_TILE > 65535)
     return fail(c, KAOS_E_ARG, "tiles_per_group must be in [1, 63] (16-bit LDS counters)");
   if (hist_words < 0 || hist_words > 12288) return fail(c, KAOS_E_ARG, "hist_words must be in [0, 12288]");
   c->launch = {tiles_per_group, hist_words, persistent ? 1 : 0};
@@ -1666,6 +1795,7 @@ extern "C" int kaos_finalize(kaos_ctx* c) {
   for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
   for (auto& ev : c->tune_ev) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
   (void)hipFree(c->d_work);
+  (void)hipFree(c->d_vbox); (void)hipFree(c->d_vmarg_par); (void)hipFree(c->d_vmarg_perp);
   (void)hipFree(c->d_stepno);
   for (StateBufs& B : c->st) {
     void* sb[] = {B.rs, B.vp, B.vr, B.qk, B.ae, B.ag, B.rdi, B.vpi, B.vrf, B.qkf, B.aef, B.agf, B.cell, B.id};

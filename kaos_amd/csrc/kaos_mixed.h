@@ -209,18 +209,21 @@ __device__ __forceinline__ double kx_dec_v(int w, int ip, bool vflag, const doub
 // Dynamic LDS: the histogram window (a.hwords words).  Uniform values are
 // kept few (flag words, 32-bit offsets, h-multiples derived per ion) so the
 // loop does not spill scalar registers.
-#ifndef KAOS_MIXED_MIN_BLOCKS
-#define KAOS_MIXED_MIN_BLOCKS 5
+// (launch bounds: minimum resident wavefronts per SIMD, which caps the VGPRs)
+#ifndef KAOS_MIXED_WAVES_EU
+#define KAOS_MIXED_WAVES_EU 5
 #endif
 template <int MAXQ>
-__global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixed(StepArgs a) {
+__global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_mixed(StepArgs a) {
   KAOS_MIXED_CONTRACT
+  constexpr int BLK = KAOS_MIXED_BLOCK, PPT = KAOS_TILE / KAOS_MIXED_BLOCK;
   __shared__ double s_rGH[MAXQ], s_rref[MAXQ], s_vGH[KAOS_MAXV + 1];
   __shared__ float4 s_cE[MAXQ];
   __shared__ int s_rk[MAXQ], s_vk[KAOS_MAXV + 1];
   extern __shared__ unsigned int s_hist[];
   __shared__ unsigned int s_pushed;
   __shared__ int s_wq0, s_wnq, s_wtube, s_group;
+  __shared__ int4 s_wbox;
   const int tid = threadIdx.x;
   int cur = -1;
   const int sflags = (a.do_interp ? 1 : 0) | (a.do_count ? 2 : 0) | (a.do_remove ? 4 : 0) | (a.do_qk ? 8 : 0);
@@ -246,25 +249,17 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
     s_pushed = 0u;
     int tb0 = (t0 < t1) ? a.tile_tube[t0] : 0;
     const DevTube& T0 = a.tubes[tb0];
-    int ncell = T0.nvpar * T0.nvperp;
-    int wnq = (sflags & 2) ? (2 * a.hwords) / ncell : 0;
+    const int4 bx = a.vbox[tb0];
+    int wnq = (sflags & 2) ? (2 * a.hwords) / (bx.y * bx.w) : 0;
     if (wnq > T0.nq) wnq = T0.nq;
     int q0 = 1;
-    if (t0 < t1 && wnq > 0) {
-      long long j0 = a.tile_start[t0];
-      if (j0 < T0.off + a.count[tb0]) {
-        uint32_t c0 = a.cell[j0];
-        int qprev = (c0 == KAOS_DEAD) ? 1 : (int)((c0 >> 16) & KAOS_QMASK) - 1;
-        if (qprev >= 1) q0 = qprev - 1;
-      }
-      if (q0 < 1) q0 = 1;
-      if (q0 + wnq - 1 > T0.nq) q0 = T0.nq - wnq + 1;
-    }
-    s_wq0 = q0; s_wnq = wnq; s_wtube = tb0;
+    if (t0 < t1 && wnq > 0) q0 = kaos_window_q0(a, t0, t1, tb0, T0.nq, T0.off + a.count[tb0], wnq);
+    s_wq0 = q0; s_wnq = wnq; s_wtube = tb0; s_wbox = bx;
   }
-  for (int i = tid; i < a.hwords; i += KAOS_BLOCK) s_hist[i] = 0u;
+  for (int i = tid; i < a.hwords; i += BLK) s_hist[i] = 0u;
   __syncthreads();
   const int wq0 = s_wq0, wnq = s_wnq, wtube = s_wtube;
+  const int4 wb = s_wbox;
   // software pipeline: the state words of this thread's next ion (same tube)
   // are loaded while the current ion is pushed
   bool pf = false;
@@ -285,13 +280,13 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
       qsms = (float)T.qsms;
       hd = T.h;
       ip = 1.0 / T.p;
-      for (int i = tid; i < nq; i += KAOS_BLOCK) {
+      for (int i = tid; i < nq; i += BLK) {
         s_rGH[i] = a.rGH[goff + i]; s_rref[i] = a.rref[goff + i];
         s_cE[i] = make_float4((float)a.qGC[goff + i], (float)a.EA[goff + i], (float)a.EG[goff + i], 0.0f);
         s_rk[i] = a.rk[goff + i];
       }
-      for (int i = tid; i <= nvpar; i += KAOS_BLOCK) s_vGH[i] = a.vparGH[T.voff + i];
-      for (int i = tid; i < nvpar; i += KAOS_BLOCK) s_vk[i] = a.vk[T.voff + i];
+      for (int i = tid; i <= nvpar; i += BLK) s_vGH[i] = a.vparGH[T.voff + i];
+      for (int i = tid; i < nvpar; i += BLK) s_vk[i] = a.vk[T.voff + i];
       toff = T.off;
       tcount = (uint32_t)a.count[tb];
       cellT = a.cell + toff; rdiT = a.rdi + toff; vpiT = a.vpi + toff;
@@ -302,8 +297,8 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
     }
     const uint32_t start = (uint32_t)(a.tile_start[tile] - toff);
 #pragma unroll 1
-    for (int it = 0; it < KAOS_PPT; ++it) {
-      const uint32_t jl = start + (uint32_t)(it * KAOS_BLOCK + tid);
+    for (int it = 0; it < PPT; ++it) {
+      const uint32_t jl = start + (uint32_t)(it * BLK + tid);
       if (jl >= tcount) break;
       const uint32_t b4 = jl * 4u;
       uint32_t cold;
@@ -314,7 +309,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
       {
         uint32_t jn = 0u;
         bool nxt = false;
-        if (it + 1 < KAOS_PPT) { jn = jl + KAOS_BLOCK; nxt = jn < tcount; }
+        if (it + 1 < PPT) { jn = jl + BLK; nxt = jn < tcount; }
         else if (tile + 1 < t1 && a.tile_tube[tile + 1] == cur) {
           jn = (uint32_t)(a.tile_start[tile + 1] - toff) + (uint32_t)tid;
           nxt = jn < tcount;
@@ -424,9 +419,10 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
       // binning of the pre-push state: Vparind from the cell word
       if ((sflags & 2) && qc >= 1) {
         if (ipc >= 1 && iq >= 1) {
-          int wq = qc - wq0;
-          if (cur == wtube && wq >= 0 && wq < wnq) {
-            int li = (wq * nvperp + (iq - 1)) * nvpar + (ipc - 1);
+          const int wql = qc - wq0, jr = ipc - wb.x, ir = iq - wb.z;
+          if (cur == wtube && (unsigned)wql < (unsigned)wnq && (unsigned)jr < (unsigned)wb.y &&
+              (unsigned)ir < (unsigned)wb.w) {
+            const int li = (wql * wb.w + ir) * wb.y + jr;
             atomicAdd(&s_hist[li >> 1], 1u << (16 * (li & 1)));
           } else {
             atomicAdd(&a.hist[hoff + ((long long)(qc - 1) * nvperp + (iq - 1)) * nvpar + (ipc - 1)], 1u);
@@ -468,13 +464,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MIXED_MIN_BLOCKS) k_push_mixe
   atomicAdd(&s_pushed, npushed);
   if (wnq > 0) {
     const DevTube& TW = a.tubes[wtube];
-    long long base = TW.hoff + (long long)(wq0 - 1) * TW.nvperp * TW.nvpar;
-    int nwords = (wnq * TW.nvperp * TW.nvpar + 1) >> 1;
-    for (int w = tid; w < nwords; w += KAOS_BLOCK) {
-      unsigned int v = s_hist[w];
-      if (v & 0xFFFFu) atomicAdd(&a.hist[base + 2 * w], v & 0xFFFFu);
-      if (v >> 16) atomicAdd(&a.hist[base + 2 * w + 1], v >> 16);
-    }
+    kaos_window_flush(s_hist, a.hist, TW.hoff, TW.nvperp, TW.nvpar, wq0, wnq, wb, tid >> 6, BLK >> 6, tid & 63);
   }
   __syncthreads();
   if (tid == 0 && s_pushed) atomicAdd(a.pushed, (unsigned long long)s_pushed);

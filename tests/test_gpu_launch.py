@@ -78,14 +78,14 @@ def test_autotuned_run_bitexact_and_settles():
     ks.step(1)
     assert ks.get_launch()[3] == 1          # still tuning
     n = 2
-    while n < 18:                           # 12 candidate steps + the choice + graph steps
+    while n < 24:                           # 2 warm-up + 16 candidate steps + the choice + graph steps
         for o in orcs:
             o.step(n)
         ks.run(n, 1)
         _check(orcs, ks, n)
         n += 1
     tpb, hw, pers, tuning = ks.get_launch()
-    assert tuning == 0 and tpb in (8, 16, 32) and hw in (4096, 6144) and pers in (0, 1)
+    assert tuning == 0 and tpb in (8, 16) and hw in (4096, 6144) and pers in (0, 1)
     ks.close()
 
 
@@ -100,7 +100,7 @@ def test_mixed_graph_and_tuning_match_eager_untuned():
     for s in (a, b):
         s.upload(0, q0, v0, w0)
         s.step(1)
-    for n in range(2, 20):
+    for n in range(2, 28):
         a.run(n, 1)
         b.step(n)
         ma, mb = a.master(0), b.master(0)
