@@ -714,8 +714,9 @@ KAOS_HD int kaos_mom_groups(int nvpar) {
   return g < 1 ? 1 : g;
 }
 
-/* Partial column aggregates of column b over rows g0, g0 + G, ..., and the
- * partial count of those cells (empty cells contribute nothing).  The device
+/* Partial column aggregates of column b over rows g0, g0 + G, ... within
+ * [a0, a1], and the partial count of those cells (empty cells contribute
+ * nothing, so rows left out that hold no counts change nothing).  The device
  * compile unrolls the row loop so the count loads issue together; the
  * arithmetic order is the same on both compilers. */
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -723,13 +724,14 @@ KAOS_HD int kaos_mom_groups(int nvpar) {
 #else
 #define KAOS_UNROLL8
 #endif
-KAOS_HD void kaos_mom_part(const kaos_vgrid_t* v, const uint32_t* cnt, int b, int g0, int G, double k,
-                           double P[3], uint32_t* csum) {
+KAOS_HD void kaos_mom_part(const kaos_vgrid_t* v, const uint32_t* cnt, int b, int g0, int G, int a0, int a1,
+                           double k, double P[3], uint32_t* csum) {
   double s0 = 0.0, s1 = 0.0, s3 = 0.0;
   uint32_t cs = 0;
   const double dvb = v->dvpar[b];
+  int a = a0 + (((g0 - a0) % G) + G) % G;   /* first row >= a0 of this group */
   KAOS_UNROLL8
-  for (int a = g0; a < v->nvperp; a += G) {
+  for (; a <= a1; a += G) {
     int i = a * v->nvpar + b;
     uint32_t c = cnt[i];
     cs += c;

@@ -87,6 +87,7 @@ struct StepArgs {
   unsigned long long* pushed;                // live ions pushed (particle-steps)
   int do_interp, do_count, do_remove, do_qk;
   const int4* vbox;                          // per tube: Vpar cells [x, x + y), Vperp cells [z, z + w)
+  int4* hbox;                                // per (tube, cell): Vpar cells [x, y], Vperp cells [z, w] holding counts
   // launch shape: tiles per group, LDS histogram window (32-bit words), persistent
   // blocks taking groups from the counter `work` (else one group per block)
   int hwords, persistent, ngroups;
@@ -106,6 +107,25 @@ __device__ __forceinline__ int kaos_next_group(const StepArgs& a, int pass, int*
   __syncthreads();
   int g = *s_group;
   return (g < a.ngroups) ? g : -1;
+}
+
+// A tube's velocity box packed in 32 bits: first Vpar cell | count << 8 |
+// first Vperp cell << 16 | count << 24 (cell indices and counts <= 255).
+__device__ __forceinline__ uint32_t kaos_pack_box(int4 b) {
+  return (uint32_t)b.x | ((uint32_t)b.y << 8) | ((uint32_t)b.z << 16) | ((uint32_t)b.w << 24);
+}
+
+// Bounding box of a (tube, config cell)'s counts this master step (int4 as
+// x0, x1, z0, z1, inclusive), set to the tube's velocity box before the push;
+// a count added outside that box (packed in pb) widens it.  The histogram
+// clear and the moment sums then touch only the box.
+__device__ __forceinline__ void kaos_hbox_note(int4* hb, uint32_t pb, int ip, int iq) {
+  if ((unsigned)(ip - (int)(pb & 0xFFu)) >= ((pb >> 8) & 0xFFu) ||
+      (unsigned)(iq - (int)((pb >> 16) & 0xFFu)) >= (pb >> 24)) {
+    int* h = (int*)hb;
+    atomicMin(&h[0], ip); atomicMax(&h[1], ip);
+    atomicMin(&h[2], iq); atomicMax(&h[3], iq);
+  }
 }
 
 // First config cell of a push group's histogram window of wnq cells: centred
@@ -251,6 +271,7 @@ __global__ void __launch_bounds__(KAOS_FP64_BLOCK, KAOS_FP64_WAVES_EU) k_push_fp
   uint32_t* cellT = nullptr;
   double qGL0 = 0.0, qGHlast = 0.0, qsms = 0.0, vscale = 0.0;
   int eambsign = 0;
+  uint32_t tbox = 0u;
   kaos_push_par_t P = kaos_push_par(1.0, 1.0, 1.0, 0.0, 0, 0, 0);
   for (int pass = 0;; ++pass) {
   const int g = kaos_next_group(a, pass, &s_group);
@@ -291,6 +312,7 @@ __global__ void __launch_bounds__(KAOS_FP64_BLOCK, KAOS_FP64_WAVES_EU) k_push_fp
       goff = (int)T.goff; hoff = (int)T.hoff;
       qGL0 = T.qGL0; qGHlast = T.qGHlast; qsms = T.qsms; eambsign = T.eambsign;
       P = kaos_push_par(T.h, T.p, T.cphi, T.sphi, T.grav, T.eamb, T.eambsign);
+      tbox = kaos_pack_box(a.vbox[tb]);
       for (int i = tid; i < nq; i += BLK) {
         s_qGH[i] = a.qGH[goff + i];
         KaosCE64 e;
@@ -383,6 +405,7 @@ __global__ void __launch_bounds__(KAOS_FP64_BLOCK, KAOS_FP64_WAVES_EU) k_push_fp
             atomicAdd(&s_hist[li >> 1], 1u << (16 * (li & 1)));
           } else {
             atomicAdd(&a.hist[hoff + ((long long)(qc - 1) * nvperp + (iq - 1)) * nvpar + (ip - 1)], 1u);
+            kaos_hbox_note(&a.hbox[goff + qc - 1], tbox, ip, iq);
           }
         } else {
           atomicAdd(&a.nqoff[goff + qc - 1], 1u);
@@ -483,6 +506,18 @@ __global__ void k_fill_u32(unsigned int* p, long long n) {
 // Field of a master step (AmbipolarEfieldSub + GravfieldSub cell values), one
 // block per tube; also clears the tube's escape counters, which the push of
 // this master step then fills (BoundaryConditionsSub, BC:158-256).
+// Start of a master step's bounding boxes of counts: every (tube, config
+// cell) box is the tube's velocity box (the histogram itself is cleared whole
+// by a streaming fill, which measured faster than clearing the boxes only).
+__global__ void k_hbox_reset(int ncells, const DevTube* tubes, const int* blk_tube, const int* blk_q,
+                             const int4* vbox, int4* hbox) {
+  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= ncells) return;
+  const int t = blk_tube[ci];
+  const int4 b = vbox[t];
+  hbox[tubes[t].goff + blk_q[ci]] = make_int4(b.x, b.x + b.y - 1, b.z, b.z + b.w - 1);
+}
+
 __global__ void k_field(const DevTube* tubes, const double* hq, const double* dq, const double* Te,
                         const double* Mprev, double* M0F, double* EA,
                         unsigned long long* esc_acc, unsigned long long* esc_rep, int rotate_esc) {
@@ -551,25 +586,39 @@ __global__ void k_inject(StepArgs a, const long long* inj_base, const long long*
   a.cell[j] = ((uint32_t)(qcell + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip | flags;
   a.id[j] = (int)(next_id[t] + k);
   if (qcell >= 1) {
-    if (ip >= 1 && iq >= 1)
+    if (ip >= 1 && iq >= 1) {
       atomicAdd(&a.hist[T.hoff + ((long long)(qcell - 1) * T.nvperp + (iq - 1)) * T.nvpar + (ip - 1)], 1u);
-    else
+      kaos_hbox_note(&a.hbox[T.goff + qcell - 1], kaos_pack_box(a.vbox[t]), ip, iq);
+    } else {
       atomicAdd(&a.nqoff[T.goff + qcell - 1], 1u);
+    }
   }
 }
 
-// Vperp-row sums of one cell's counts added to the tube's row marginals:
-// thread i of n sums rows i, i + n, ... serially (the rows were just read by
-// the moment sums, so these loads hit the cache; the column marginals come
-// from the moment kernels' column counts).
-__device__ __forceinline__ void kaos_row_marginals(const uint32_t* N, int np, int nr, int i, int n,
-                                                   unsigned int* vmr) {
-  for (int r = i; r < nr; r += n) {
-    const uint32_t* row = N + r * np;
+// Marginal counts of every tube over Vpar cells and over Vperp cells (inside
+// the tube's bounding box of counts), one wavefront per (tube, config cell):
+// lanes sum columns, then each row is summed across the lanes.
+__global__ void __launch_bounds__(KAOS_BLOCK) k_marginals(const DevTube* tubes, const int* blk_tube,
+    const int* blk_q, int ncells, const int4* hbox, const unsigned int* hist, unsigned int* vmp,
+    unsigned int* vmr) {
+  const int lane = threadIdx.x & 63;
+  const int ci = blockIdx.x * (KAOS_BLOCK / 64) + (threadIdx.x >> 6);
+  if (ci >= ncells) return;
+  const int t = blk_tube[ci], Q = blk_q[ci];
+  const DevTube& T = tubes[t];
+  const int4 hb = hbox[T.goff + Q];
+  if (hb.x > hb.y) return;
+  const unsigned int* N = hist + T.hoff + (long long)Q * T.nvperp * T.nvpar;
+  for (int b = hb.x - 1 + lane; b < hb.y; b += 64) {
     unsigned int x = 0u;
-#pragma unroll 8
-    for (int b = 0; b < np; ++b) x += row[b];
-    if (x) atomicAdd(&vmr[r], x);
+    for (int r = hb.z - 1; r < hb.w; ++r) x += N[r * T.nvpar + b];
+    if (x) atomicAdd(&vmp[T.voff + b], x);
+  }
+  for (int r = hb.z - 1; r < hb.w; ++r) {
+    unsigned int x = 0u;
+    for (int b = hb.x - 1 + lane; b < hb.y; b += 64) x += N[r * T.nvpar + b];
+    for (int st = 32; st >= 1; st >>= 1) x += __shfl_xor(x, st, 64);
+    if (lane == 0 && x) atomicAdd(&vmr[T.woff + r], x);
   }
 }
 
@@ -584,7 +633,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments(const DevTube* tubes, co
     const int* blk_q, const double* d3xC, const double* d3v, const double* dvpar, const double* dvperp,
     const double* vparGC, const double* vperpGC, const unsigned int* hist, const unsigned int* nqoff,
     unsigned int* nqc, double* NqRT, double* M, long long* count, long long* next_id,
-    const long long* inj_cnt, const int* cells, unsigned int* vmp, unsigned int* vmr) {
+    const long long* inj_cnt, const int* cells, const int4* hbox) {
   __shared__ double s_P[KAOS_MOM_LANES][3];
   __shared__ unsigned int s_pc[KAOS_MOM_LANES];
   __shared__ double s_A[KAOS_MAXV + 1][3];
@@ -600,10 +649,12 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments(const DevTube* tubes, co
   const uint32_t* N = hist + T.hoff + (long long)Q * T.nvperp * np;
   kaos_vgrid_t v = { np, T.nvperp, d3v + T.doff, dvperp + T.woff, dvpar + T.voff,
                      vperpGC + T.woff, vparGC + T.voff };
+  const int4 hb = hbox[T.goff + Q];
   if (tid < G * np) {
-    double P[3];
-    uint32_t cs;
-    kaos_mom_part(&v, N, tid % np, tid / np, G, k, P, &cs);
+    double P[3] = {0.0, 0.0, 0.0};
+    uint32_t cs = 0u;
+    const int b = tid % np;
+    if (b + 1 >= hb.x && b + 1 <= hb.y) kaos_mom_part(&v, N, b, tid / np, G, hb.z - 1, hb.w - 1, k, P, &cs);
     s_P[tid][0] = P[0]; s_P[tid][1] = P[1]; s_P[tid][2] = P[2];
     s_pc[tid] = cs;
   }
@@ -617,7 +668,6 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments(const DevTube* tubes, co
     }
     for (int i = 0; i < 3; ++i) s_A[tid][i] = A[i];
     s_cc[tid] = cs;
-    if (cs) atomicAdd(&vmp[T.voff + tid], cs);
     double o4[4];
     kaos_mom_col_first(v.vparGC[tid], A, o4);
     for (int i = 0; i < 4; ++i) s_x[i][tid] = o4[i];
@@ -628,7 +678,6 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments(const DevTube* tubes, co
       for (int i = 0; i < 4; ++i) s_x[i][tid] = s_x[i][tid] + s_x[i][tid + st];
     __syncthreads();
   }
-  kaos_row_marginals(N, np, T.nvperp, tid, KAOS_BLOCK, vmr + T.woff);
   if (tid == 0) {
     unsigned int tot = nqoff[T.goff + Q];
     for (int c = 0; c < np; ++c) tot += s_cc[c];
@@ -677,7 +726,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments_wave(const DevTube* tube
     const int* blk_q, const int* cells, int ncells, const double* d3xC, const double* d3v,
     const double* dvpar, const double* dvperp, const double* vparGC, const double* vperpGC,
     const unsigned int* hist, const unsigned int* nqoff, unsigned int* nqc, double* NqRT, double* M,
-    long long* count, long long* next_id, const long long* inj_cnt, unsigned int* vmp, unsigned int* vmr) {
+    long long* count, long long* next_id, const long long* inj_cnt, const int4* hbox) {
   const int lane = threadIdx.x & 63;
   const int ci = blockIdx.x * (KAOS_BLOCK / 64) + (threadIdx.x >> 6);
   if (ci >= ncells) return;
@@ -692,13 +741,12 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments_wave(const DevTube* tube
   double A[3] = {0.0, 0.0, 0.0}, o4[4] = {0.0, 0.0, 0.0, 0.0};
   uint32_t cs = 0;
   double uc = 0.0;
+  const int4 hb = hbox[T.goff + Q];
   if (lane < np) {
-    kaos_mom_part(&v, N, lane, 0, 1, k, A, &cs);
+    if (lane + 1 >= hb.x && lane + 1 <= hb.y) kaos_mom_part(&v, N, lane, 0, 1, hb.z - 1, hb.w - 1, k, A, &cs);
     uc = v.vparGC[lane];
     kaos_mom_col_first(uc, A, o4);
   }
-  if (lane < np && cs) atomicAdd(&vmp[T.voff + lane], cs);
-  kaos_row_marginals(N, np, T.nvperp, lane, 64, vmr + T.woff);
   for (int st = 1; st < 64; st *= 2) cs += __shfl_xor(cs, st, 64);   // integer: any order
   double S[4];
   for (int i = 0; i < 4; ++i) S[i] = __shfl(kaos_wave_tree(o4[i], lane, np), 0, 64);
@@ -719,7 +767,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments_wave(const DevTube* tube
 }
 
 // Velocity box of each tube's LDS histogram window, from the marginal counts
-// of the master step just binned: the Vpar and Vperp index ranges that leave
+// (k_marginals) of the master step just binned: the Vpar and Vperp index ranges that leave
 // out at most 2.5e-5 of the binned ions at each end, widened by one cell for
 // the next step's moves (all-zero marginals: the whole grid).  The marginals
 // are cleared for the next master step.  Counts outside the box go to global
@@ -942,6 +990,7 @@ struct kaos_ctx {
   Launch launch = {KAOS_TPB, KAOS_HWORDS, 0};
   unsigned int* d_work = nullptr;
   int4* d_vbox = nullptr;                       // per tube velocity box of the push's LDS window
+  int4* d_hbox = nullptr;                       // per (tube, cell) bounding box of the master step's counts
   unsigned int *d_vmarg_par = nullptr, *d_vmarg_perp = nullptr;
   long long* d_stepno = nullptr;
   int n_cu = 256;
@@ -1199,6 +1248,10 @@ static int build_device(kaos_ctx* c) {
     std::vector<int4> bx(nt);
     for (int t = 0; t < nt; ++t) bx[t] = make_int4(1, c->dt[t].nvpar, 1, c->dt[t].nvperp);
     HIPCHK(c, up((void**)&c->d_vbox, bx.data(), sizeof(int4) * nt));
+    std::vector<int4> hb;
+    for (int t = 0; t < nt; ++t)
+      for (int q = 0; q < c->dt[t].nq; ++q) hb.push_back(make_int4(1, c->dt[t].nvpar, 1, c->dt[t].nvperp));
+    HIPCHK(c, up((void**)&c->d_hbox, hb.data(), sizeof(int4) * hb.size()));
     HIPCHK(c, hipMalloc((void**)&c->d_vmarg_par, sizeof(unsigned int) * (size_t)voff));
     HIPCHK(c, hipMalloc((void**)&c->d_vmarg_perp, sizeof(unsigned int) * (size_t)woff));
     HIPCHK(c, hipMemset(c->d_vmarg_par, 0, sizeof(unsigned int) * (size_t)voff));
@@ -1252,6 +1305,7 @@ static StepArgs make_args(kaos_ctx* c, int interp, int count, int remove, int qk
   a.ngroups = (c->ntiles + c->launch.tpb - 1) / c->launch.tpb;
   a.work = c->d_work;
   a.vbox = c->d_vbox;
+  a.hbox = c->d_hbox;
   a.do_interp = interp; a.do_count = count; a.do_remove = remove; a.do_qk = qk;
   return a;
 }
@@ -1312,12 +1366,22 @@ static void launch_moments(kaos_ctx* c, const long long* inj) {
     hipLaunchKernelGGL(k_moments_wave, dim3((c->n_cells_wave + wpb - 1) / wpb), dim3(KAOS_BLOCK), 0, c->stream,
                        c->d_tubes, c->d_blk_tube, c->d_blk_q, c->d_cells_wave, c->n_cells_wave, c->d_d3x, c->d_d3v,
                        c->d_dvpar, c->d_dvperp, c->d_vparGC, c->d_vperpGC, c->d_hist, c->d_nqoff, c->d_nqc,
-                       c->d_NqRT, c->d_M, c->d_count, c->d_next_id, inj, c->d_vmarg_par, c->d_vmarg_perp);
+                       c->d_NqRT, c->d_M, c->d_count, c->d_next_id, inj, c->d_hbox);
   if (c->n_cells_block)
     hipLaunchKernelGGL(k_moments, dim3(c->n_cells_block), dim3(KAOS_BLOCK), 0, c->stream, c->d_tubes,
                        c->d_blk_tube, c->d_blk_q, c->d_d3x, c->d_d3v, c->d_dvpar, c->d_dvperp, c->d_vparGC,
                        c->d_vperpGC, c->d_hist, c->d_nqoff, c->d_nqc, c->d_NqRT, c->d_M, c->d_count, c->d_next_id,
-                       inj, c->d_cells_block, c->d_vmarg_par, c->d_vmarg_perp);
+                       inj, c->d_cells_block, c->d_hbox);
+}
+
+// Velocity boxes of the push's LDS windows from the counts of the master step
+// just binned (at the first step and at every reorder; between updates a box
+// only steers where counts accumulate, so a stale box costs time, never a
+// result).
+static void launch_box_update(kaos_ctx* c) {
+  const int wpb = KAOS_BLOCK / 64;
+  hipLaunchKernelGGL(k_marginals, dim3((c->nblk_mom + wpb - 1) / wpb), dim3(KAOS_BLOCK), 0, c->stream, c->d_tubes,
+                     c->d_blk_tube, c->d_blk_q, c->nblk_mom, c->d_hbox, c->d_hist, c->d_vmarg_par, c->d_vmarg_perp);
   hipLaunchKernelGGL(k_vbox, dim3(c->cfg.ntubes), dim3(64), 0, c->stream, c->d_tubes, c->d_vmarg_par,
                      c->d_vmarg_perp, c->d_vbox);
 }
@@ -1404,6 +1468,7 @@ static int enqueue_reorder(kaos_ctx* c) {
   StepArgs a = make_args(c, 0, 0, 0, 0);
   StepArgs b = a;
   set_state(b, c->st[c->cur ^ 1]);
+  launch_box_update(c);
   hipLaunchKernelGGL(k_reorder_count, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a, mixed, c->d_keycnt);
   hipLaunchKernelGGL(k_reorder_scan, dim3((nt + 63) / 64), dim3(64), 0, c->stream, c->d_tubes, nt, c->d_keycnt,
                      c->d_cursor, c->d_cnt_new);
@@ -1429,6 +1494,8 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
   auto zero_hist = [&]() {
     hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, c->stream, c->d_hist, c->nhist_tot);
     hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, c->stream, c->d_nqoff, c->ncell_tot);
+    hipLaunchKernelGGL(k_hbox_reset, dim3((c->nblk_mom + 255) / 256), dim3(256), 0, c->stream, c->nblk_mom,
+                       c->d_tubes, c->d_blk_tube, c->d_blk_q, c->d_vbox, c->d_hbox);
   };
   if (first) {
     StepArgs a = make_args(c, 1, 0, 0, premaster);
@@ -1438,6 +1505,7 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
     hipLaunchKernelGGL(k_field, dim3(nt), dim3(256), 0, c->stream, c->d_tubes, c->d_hq, c->d_dq, c->d_Te,
                        c->d_M, c->d_M0F, c->d_EA, c->d_esc_acc, c->d_esc_rep, 0);
     launch_moments(c, nullptr);
+    launch_box_update(c);
     launch_push(c, a);
   } else {
     if (master) {
@@ -1795,7 +1863,8 @@ extern "C" int kaos_finalize(kaos_ctx* c) {
   for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
   for (auto& ev : c->tune_ev) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
   (void)hipFree(c->d_work);
-  (void)hipFree(c->d_vbox); (void)hipFree(c->d_vmarg_par); (void)hipFree(c->d_vmarg_perp);
+  (void)hipFree(c->d_vbox); (void)hipFree(c->d_hbox);
+  (void)hipFree(c->d_vmarg_par); (void)hipFree(c->d_vmarg_perp);
   (void)hipFree(c->d_stepno);
   for (StateBufs& B : c->st) {
     void* sb[] = {B.rs, B.vp, B.vr, B.qk, B.ae, B.ag, B.rdi, B.vpi, B.vrf, B.qkf, B.aef, B.agf, B.cell, B.id};

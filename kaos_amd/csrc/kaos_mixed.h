@@ -239,6 +239,7 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
   double rGL0 = 0.0, rGHlast = 0.0;
   float qGL0 = 0.0f, qGHlast = 0.0f, qsms = 0.0f, vscale = 0.0f;
   double ip = 0.0, hd = 0.0;
+  uint32_t tbox = 0u;
   for (int pass = 0;; ++pass) {
   const int g = kaos_next_group(a, pass, &s_group);
   if (g < 0) break;
@@ -292,6 +293,7 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
       cellT = a.cell + toff; rdiT = a.rdi + toff; vpiT = a.vpi + toff;
       qkfT = a.qkf + toff; aefT = a.aef + toff; agfT = a.agf + toff;
       vscale = (float)((double)nvpar / (a.vparGH[T.voff + nvpar - 1] - a.vparGH[T.voff + nvpar]));
+      tbox = kaos_pack_box(a.vbox[tb]);
       cur = tb;
       __syncthreads();
     }
@@ -426,6 +428,7 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
             atomicAdd(&s_hist[li >> 1], 1u << (16 * (li & 1)));
           } else {
             atomicAdd(&a.hist[hoff + ((long long)(qc - 1) * nvperp + (iq - 1)) * nvpar + (ipc - 1)], 1u);
+            kaos_hbox_note(&a.hbox[goff + qc - 1], tbox, ipc, iq);
           }
         } else {
           atomicAdd(&a.nqoff[goff + qc - 1], 1u);

@@ -180,9 +180,9 @@ static void counts_and_moments(orc_t *o, const int *qi, const int *vpi, const in
     for (int b = 0; b < np; ++b) {
       double P[3];
       uint32_t cs;
-      kaos_mom_part(&v, N, b, 0, G, k, &A[3 * b], &cs);
+      kaos_mom_part(&v, N, b, 0, G, 0, v.nvperp - 1, k, &A[3 * b], &cs);
       for (int g = 1; g < G; ++g) {
-        kaos_mom_part(&v, N, b, g, G, k, P, &cs);
+        kaos_mom_part(&v, N, b, g, G, 0, v.nvperp - 1, k, P, &cs);
         for (int t = 0; t < 3; ++t) A[3 * b + t] = A[3 * b + t] + P[t];
       }
       double o4[4];
