@@ -85,6 +85,20 @@ typedef struct kaos_grid {
   const double *vparGL, *vparGH, *vparGC, *dvpar;
   const double *vperpGL, *vperpGH, *vperpGC, *dvperp;
   const double *d3v;
+  /* ICR wave heating (ICRflagT = 1; stochastic, ICRCOHERENCEflag = 0), with
+   * the parameters of _ModelInput.f90:214-229, uniform over the cells as
+   * SimParameterizationSub sets them (_1_SimParameterization.f90:1430-1450).
+   * A zero-initialised struct has heating off. */
+  int32_t icr;
+  int32_t pad1;
+  double icr_lambda_perp;  /* lambdaPerpp [m], 0 = infinite-wavelength limit */
+  double icr_eta_lh;       /* EtaLHp */
+  double icr_xi_perp1;     /* XiPerp1p */
+  double icr_xi_perp2;     /* XiPerp2p */
+  double icr_s0;           /* S0p [(V^2/m^2)/Hz] */
+  double icr_f0;           /* f0p [Hz] (OmegaG0p = 2 pi f0p) */
+  double icr_chi_perp1;    /* ChiPerp1p */
+  double icr_chi_perp2;    /* ChiPerp2p */
 } kaos_grid;
 
 /* Per-step summary. */
@@ -114,6 +128,12 @@ int kaos_set_grid(kaos_ctx* ctx, int32_t tube, const kaos_grid* g, int64_t capac
  * InitialConditionsSub (_7_InitialConditions.f90:51-101). */
 int kaos_upload_particles(kaos_ctx* ctx, int32_t tube, int64_t n, const double* q0,
                           const double* vpar0, const double* vperp0);
+
+/* Both perpendicular components (Vperp10T, Vperp20T; KSA:389-391) of the n
+ * ions just uploaded to `tube`, before its first step; the Vperp cells follow
+ * from |(vperp1, vperp2)|.  Without this call an uploaded ion has Vperp1 =
+ * vperp0 and Vperp2 = 0 (only the magnitude enters unless ICR heating is on). */
+int kaos_upload_vperp(kaos_ctx* ctx, int32_t tube, int64_t n, const double* vperp1, const double* vperp2);
 
 /* Advance every tube by time step n (n = 1 first).  One call replaces one
  * pass of ComputationalTimeStepLoop (_8_KineticSolver.f90:54-239):

@@ -29,13 +29,19 @@ GRID_ARRAYS = ("qGL", "qGH", "qGC", "hqC", "dqC", "d3xC", "rGC", "cthGC", "ellGC
                "d3v")
 
 
+ICR_FIELDS = ("icr_lambda_perp", "icr_eta_lh", "icr_xi_perp1", "icr_xi_perp2", "icr_s0", "icr_f0",
+              "icr_chi_perp1", "icr_chi_perp2")
+
+
 class KaosGrid(C.Structure):
     _fields_ = ([(n, C.c_int32) for n in ("nq", "nvpar", "nvperp", "grav", "eamb", "eambsign",
                                           "filter_pt", "lbcond", "ubcond", "pad0")]
                 + [("n_inj_lb", C.c_int64), ("n_inj_ub", C.c_int64)]
                 + [(n, C.c_double) for n in ("p", "cphi", "sphi", "h", "ms", "qs", "nsnormfac",
                                              "Ts_lb", "Ts_ub")]
-                + [(n, _dp) for n in GRID_ARRAYS])
+                + [(n, _dp) for n in GRID_ARRAYS]
+                + [("icr", C.c_int32), ("pad1", C.c_int32)]
+                + [(n, C.c_double) for n in ICR_FIELDS])
 
 
 class KaosStepOut(C.Structure):
@@ -43,7 +49,7 @@ class KaosStepOut(C.Structure):
                 ("status", C.c_int32)]
 
 
-EXPORTS = ("kaos_init", "kaos_set_grid", "kaos_upload_particles", "kaos_step", "kaos_run",
+EXPORTS = ("kaos_init", "kaos_set_grid", "kaos_upload_particles", "kaos_upload_vperp", "kaos_step", "kaos_run",
            "kaos_download_master", "kaos_count", "kaos_download_particles", "kaos_set_launch",
            "kaos_get_launch", "kaos_profile", "kaos_profile_read", "kaos_last_error", "kaos_finalize")
 
@@ -64,6 +70,7 @@ def load(path=LIB_PATH):
     lib.kaos_init.argtypes = [C.POINTER(KaosCfg), C.POINTER(P)]
     lib.kaos_set_grid.argtypes = [P, C.c_int32, C.POINTER(KaosGrid), C.c_int64]
     lib.kaos_upload_particles.argtypes = [P, C.c_int32, C.c_int64, _dp, _dp, _dp]
+    lib.kaos_upload_vperp.argtypes = [P, C.c_int32, C.c_int64, _dp, _dp]
     lib.kaos_step.argtypes = [P, C.c_int64, _dp, C.c_int64, C.POINTER(KaosStepOut)]
     lib.kaos_run.argtypes = [P, C.c_int64, C.c_int64, i64p]
     lib.kaos_download_master.argtypes = [P, C.c_int32, i64p, i64p, _dp, _dp, _dp, _dp, _dp]

@@ -549,6 +549,97 @@ KAOS_HD void kaos_mb_project(const double u[6], double sigma, double cphi, doubl
 }
 
 /* ------------------------------------------------------------------------- */
+/* ICR wave heating (WaveHeatingSub, _8_1_1_0_WaveHeating.f90:32-314).       */
+
+/* exp(x) for |x| < 700: x = k ln2 + r, |r| <= ln2/2 (ln2 in two parts, the
+ * same split as kaos_log), exp(r) by its Taylor series to r^14/14!, then
+ * the exact scaling by 2^k.                                                 */
+KAOS_HD double kaos_exp(double x) {
+  double k = rint(x * 1.44269504088896338700);
+  double r = (x - k * 6.93147180369123816490e-01) - k * 1.90821492927058770002e-10;
+  double p = 1.0 / 87178291200.0;                 /* 1/14! */
+  p = p * r + 1.0 / 6227020800.0; p = p * r + 1.0 / 479001600.0; p = p * r + 1.0 / 39916800.0;
+  p = p * r + 1.0 / 3628800.0;    p = p * r + 1.0 / 362880.0;    p = p * r + 1.0 / 40320.0;
+  p = p * r + 1.0 / 5040.0;       p = p * r + 1.0 / 720.0;       p = p * r + 1.0 / 120.0;
+  p = p * r + 1.0 / 24.0;         p = p * r + 1.0 / 6.0;         p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  return ldexp(p, (int)k);
+}
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11). */
+KAOS_HD void kaos_philox4x32(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+  uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+  uint32_t k0 = key_in[0], k1 = key_in[1];
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n1 = (uint32_t)p1;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1, n3 = (uint32_t)p0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* The four uniforms in (0, 1] of one ion's two Gaussian kicks at step n
+ * (GaussianRNGSub twice, _8_1_1_0_WaveHeating.f90:56-64): Philox4x32-10 keyed
+ * by the seed, counter (ion id, step, tube, "ICR").                          */
+KAOS_HD void kaos_icr_uniforms(uint64_t seed, int64_t step, int tube, int64_t id, double u[4]) {
+  uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) ^ 0x49435248u };
+  uint32_t ctr[4] = { (uint32_t)id, (uint32_t)step, (uint32_t)tube, 0x49435230u };
+  uint32_t o[4];
+  kaos_philox4x32(ctr, key, o);
+  for (int i = 0; i < 4; ++i) u[i] = ((double)o[i] + 1.0) * (1.0 / 4294967296.0);
+}
+
+/* Heating parameters of a tube (the per-cell arrays of SimParameterizationSub,
+ * _1_SimParameterization.f90:1430-1450, are uniform over the cells). */
+typedef struct {
+  double qs, ms, h;
+  double lambda_perp;    /* lambdaPerpp: 0 = infinite-wavelength limit   */
+  double eta_lh;         /* EtaLHp                                       */
+  double xi1, xi2;       /* XiPerp1p, XiPerp2p                           */
+  double s0;             /* S0p [(V^2/m^2)/Hz]                           */
+  double omega_g0;       /* 2 pi f0p [rad/s]                             */
+  double chi1, chi2;     /* ChiPerp1p, ChiPerp2p                         */
+  double sum_ds;         /* sum of dsICRGT = hqC dqC over the cells      */
+} kaos_icr_t;
+
+/* Stochastic ICR kick of one ion (ICRCOHERENCEflag = 0) at the stage-A point
+ * of line geometry L (r, w) with input parallel velocity vpar and
+ * perpendicular components (v1, v2); u: kaos_icr_uniforms.  WaveHeating:
+ * SigmaPerp (:75-84), DPerp1/2 (:87-92), epsPerp and tauPerp (:100-105),
+ * the kicks (:140-151), Vperp1N/2N and VperpN (:296-303).  Gyrofrequency
+ * OmegaG = qs B / ms with B = m sqrt(ell) / r^3 (_KineticMainParams.f90:621-633),
+ * ell = 1 + 3 cos^2 theta = 4 - 3 w. */
+KAOS_HD void kaos_icr_kick(const kaos_icr_t* P, double r, double w, double vpar, double v1, double v2,
+                           const double u[4], double* v1N, double* v2N, double* vpN) {
+  const double g1 = kaos_gauss(u[0], u[1]), g2 = kaos_gauss(u[2], u[3]);
+  const double B = (KAOS_MDIP * sqrt(4.0 - 3.0 * w)) / ((r * r) * r);
+  const double og = (P->qs * B) / P->ms;
+  const double vperp = sqrt(v1 * v1 + v2 * v2);
+  double sig = 1.0;
+  if (P->lambda_perp != 0.0 && ((KAOS_TWOPI * vperp) / og) >= P->lambda_perp)
+    sig = (KAOS_TWOPI * vperp) / (P->lambda_perp * og);
+  const double isig3 = 1.0 / ((sig * sig) * sig);
+  const double lr = kaos_log(og / P->omega_g0);
+  const double c = (P->qs * P->qs) / (2.0 * (P->ms * P->ms));
+  const double D1 = (((c * (P->xi1 * P->eta_lh)) * P->s0) * kaos_exp(-P->chi1 * lr)) * isig3;
+  const double D2 = (((c * (P->xi2 * P->eta_lh)) * P->s0) * kaos_exp(-P->chi2 * lr)) * isig3;
+  const double eps = fabs(fabs(vpar) / (og * P->sum_ds));
+  const double tau = 1.0 / ((og / KAOS_TWOPI) * fabs(sqrt(eps)));
+  double dv1 = 0.0, dv2 = 0.0;
+  if (P->h <= tau) {
+    dv1 = ((P->h / tau) * sqrt((2.0 * D1) * tau)) * g1;
+    dv2 = ((P->h / tau) * sqrt((2.0 * D2) * tau)) * g2;
+  }
+  *v1N = v1 + dv1;
+  *v2N = v2 + dv2;
+  *vpN = fabs(sqrt((*v1N) * (*v1N) + (*v2N) * (*v2N)));
+}
+
+/* ------------------------------------------------------------------------- */
 /* Per-tube field arithmetic (run once per master step per tube).            */
 
 /* MomentFilterSub, _8_7_1_MomentFilter.f90:25-147: asymmetric sliding average

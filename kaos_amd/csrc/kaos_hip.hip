@@ -61,6 +61,8 @@ struct DevTube {
   double qGL0, qGHlast, qGL_last0;
   double rGL0, rGHlast, rsgn;               // |r| of the outer q edges; sign of rs in the tube
   long long koff;                           // offset of the tube's nq + 2 reorder keys
+  int icr, pad2;                            // ICR wave heating on; its parameters
+  kaos_icr_t icrp;
 };
 
 struct StepArgs {
@@ -72,9 +74,11 @@ struct StepArgs {
   const double *qGH, *qGC, *EA, *EG;
   const double *vparGH, *vperpGH;
   // particles
-  double *rs, *vp, *vr, *qk, *ae, *ag;       // binary64 form
+  double *rs, *vp, *vr, *qk, *ae, *ag;       // binary64 form (vr: Vperp1)
+  double *vr2;                               // binary64 Vperp2
   int *rdi, *vpi;                            // mixed form: fixed-point radius / V|| offsets
-  float *vrf, *qkf, *aef, *agf;              // mixed form: Vperp, stage-4 q offset, fields
+  float *vrf, *qkf, *aef, *agf;              // mixed form: Vperp1, stage-4 q offset, fields
+  float *vrf2;                               // mixed form: Vperp2
   const double *rGH, *rref;                  // |r| of the qGH edges; signed r of the cell centres
   const int *rk, *vk;                        // fixed-point exponents per config / Vpar cell
   int lds_nq, lds_nv;                        // largest nq / nvpar over the tubes (LDS sizing)
@@ -88,6 +92,8 @@ struct StepArgs {
   int do_interp, do_count, do_remove, do_qk;
   const int4* vbox;                          // per tube: Vpar cells [x, x + y), Vperp cells [z, z + w)
   int4* hbox;                                // per (tube, cell): Vpar cells [x, y], Vperp cells [z, w] holding counts
+  const long long* stepno;                   // device step counter (keys the heating stream)
+  unsigned long long seed;
   // launch shape: tiles per group, LDS histogram window (32-bit words), persistent
   // blocks taking groups from the counter `work` (else one group per block)
   int hwords, persistent, ngroups;
@@ -243,6 +249,27 @@ __device__ __forceinline__ int kd_find_qcell_fast(double q, const double* qGH, d
   return ok ? i + 1 : kaos_find_qcell_from(q, qGH, qGL0, qGHlast, nq, desc, hint);
 }
 
+// ICR heating of one ion in the FP64 push (WaveHeatingSub at stage A,
+// _8_1_1_KineticUpdateA.f90:1392): the binning Vperp cell iq becomes the cell
+// of the pre-kick |(Vperp1, Vperp2)| (Vperpindk1, KUA:1108-1127: off the grid
+// the previous index stays), then ions inside the grid get the kick.
+__device__ __noinline__ void kd_icr_push(const StepArgs& a, int t, uint32_t jl, int& iq, int qc, kaos_line_t L,
+                                         double vpar) {
+  const DevTube& T = a.tubes[t];
+  const long long j = T.off + jl;
+  const double v1 = a.vr[j], v2 = a.vr2[j];
+  const double* GH = a.vperpGH + T.woff;
+  const int iqn = kaos_find_vcell(fabs(sqrt(v1 * v1 + v2 * v2)), GH, GH[T.nvperp], T.nvperp);
+  if (iqn >= 1) iq = iqn;
+  if (qc != 0 && qc != -1) {
+    double u[4], v1N, v2N, vpN;
+    kaos_icr_uniforms(a.seed, *a.stepno, t, a.id[j], u);
+    kaos_icr_kick(&T.icrp, L.r, L.w, vpar, v1, v2, u, &v1N, &v2N, &vpN);
+    a.vr[j] = v1N;
+    a.vr2[j] = v2N;
+  }
+}
+
 // Static LDS (grids of up to MAXQ cells): qGH, the packed centres, vparGH
 // (high edges, then GL0).  Dynamic LDS: the window of 16-bit phase-space
 // counters (two per word, a.hwords words).  Loads and stores address the
@@ -269,6 +296,7 @@ __global__ void __launch_bounds__(KAOS_FP64_BLOCK, KAOS_FP64_WAVES_EU) k_push_fp
   long long toff = 0;
   double *rsT = nullptr, *vpT = nullptr, *qkT = nullptr, *aeT = nullptr, *agT = nullptr;
   uint32_t* cellT = nullptr;
+  int icr = 0;                        // ICR heating in this tube
   double qGL0 = 0.0, qGHlast = 0.0, qsms = 0.0, vscale = 0.0;
   int eambsign = 0;
   uint32_t tbox = 0u;
@@ -324,6 +352,7 @@ __global__ void __launch_bounds__(KAOS_FP64_BLOCK, KAOS_FP64_WAVES_EU) k_push_fp
       tcount = (uint32_t)a.count[tb];
       rsT = a.rs + toff; vpT = a.vp + toff; qkT = a.qk + toff; aeT = a.ae + toff; agT = a.ag + toff;
       cellT = a.cell + toff;
+      icr = T.icr;
       vscale = (double)nvpar / (a.vparGH[T.voff + nvpar - 1] - a.vparGH[T.voff + nvpar]);
       cur = tb;
       __syncthreads();
@@ -386,7 +415,9 @@ __global__ void __launch_bounds__(KAOS_FP64_BLOCK, KAOS_FP64_WAVES_EU) k_push_fp
       // off-grid keeps the previous index (KUA:1087-1127).  Vperp is constant on
       // this path (no wave heating), so its cell, found when the ion was binned
       // first (k_bin_initial, k_inject), is carried in the cell word.
-      const int ipc = (int)(cold & 0xFFu), iq = (int)((cold >> 8) & 0xFFu);
+      const int ipc = (int)(cold & 0xFFu);
+      int iq = (int)((cold >> 8) & 0xFFu);
+      if (icr) kd_icr_push(a, cur, jl, iq, qc, L, o.vpar);
       int ip = ipc;
       {
         const int ipx = max(ipc, 1);
@@ -481,16 +512,31 @@ __global__ void k_upload(DevTube T, long long n, const double* q0, const double*
     a.rdi[j] = kx_enc(rs - rref[Q - 1], a.rk[T.goff + Q - 1], rflag);
     a.vpi[j] = kx_enc_v(vpar0[i], ip, vparGH, a.vk + T.voff, T.nvpar, vflag);
     a.vrf[j] = (float)vperp0[i];
+    a.vrf2[j] = 0.0f;
     a.qkf[j] = (float)q;
     a.aef[j] = 0.0f; a.agf[j] = 0.0f;
     a.cell[j] = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip |
                 (rflag ? KAOS_RFLAG : 0u) | (vflag ? KAOS_VFLAG : 0u);
   } else {
     a.rs[j] = rs;
-    a.vp[j] = vpar0[i]; a.vr[j] = vperp0[i]; a.qk[j] = q;
+    a.vp[j] = vpar0[i]; a.vr[j] = vperp0[i]; a.vr2[j] = 0.0; a.qk[j] = q;
     a.ae[j] = 0.0; a.ag[j] = 0.0;
     a.cell[j] = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip;
   }
+}
+
+// Both perpendicular components of the ions just uploaded; the Vperp cell of
+// the cell word follows from |(Vperp1, Vperp2)|.
+__global__ void k_set_vperp(DevTube T, long long n, const double* v1, const double* v2, const double* vperpGH,
+                            StepArgs a, int mixed) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  long long j = T.off + i;
+  const double vp = fabs(sqrt(v1[i] * v1[i] + v2[i] * v2[i]));
+  const int iq = kaos_find_vcell(vp, vperpGH, vperpGH[T.nvperp], T.nvperp);
+  a.cell[j] = (a.cell[j] & ~(0xFFu << 8)) | ((uint32_t)iq << 8);
+  if (mixed) { a.vrf[j] = (float)v1[i]; a.vrf2[j] = (float)v2[i]; }
+  else { a.vr[j] = v1[i]; a.vr2[j] = v2[i]; }
 }
 
 // Step counter on the device (the injection stream is keyed by the step), so
@@ -575,12 +621,13 @@ __global__ void k_inject(StepArgs a, const long long* inj_base, const long long*
     bool rflag, vflag;
     a.rdi[j] = kx_enc(r - a.rref[T.goff + Q - 1], a.rk[T.goff + Q - 1], rflag);
     a.vpi[j] = kx_enc_v(vpar, ip, a.vparGH + T.voff, a.vk + T.voff, T.nvpar, vflag);
-    a.vrf[j] = (float)vperp;
+    a.vrf[j] = (float)v1;
+    a.vrf2[j] = (float)v2;
     a.qkf[j] = (float)qc;
     a.aef[j] = (float)aea; a.agf[j] = (float)agv;
     flags = (rflag ? KAOS_RFLAG : 0u) | (vflag ? KAOS_VFLAG : 0u);
   } else {
-    a.rs[j] = r; a.vp[j] = vpar; a.vr[j] = vperp; a.qk[j] = qc;
+    a.rs[j] = r; a.vp[j] = vpar; a.vr[j] = v1; a.vr2[j] = v2; a.qk[j] = qc;
     a.ae[j] = aea; a.ag[j] = agv;
   }
   a.cell[j] = ((uint32_t)(qcell + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip | flags;
@@ -885,10 +932,10 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_reorder_scatter(StepArgs a, Step
     long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
     long long d = (long long)(s_base[key[it]] + atomicAdd(&s_cnt[key[it]], 1u));
     if (mixed) {
-      b.rdi[d] = a.rdi[j]; b.vpi[d] = a.vpi[j]; b.vrf[d] = a.vrf[j];
+      b.rdi[d] = a.rdi[j]; b.vpi[d] = a.vpi[j]; b.vrf[d] = a.vrf[j]; b.vrf2[d] = a.vrf2[j];
       b.qkf[d] = a.qkf[j]; b.aef[d] = a.aef[j]; b.agf[d] = a.agf[j];
     } else {
-      b.rs[d] = a.rs[j]; b.vp[d] = a.vp[j]; b.vr[d] = a.vr[j];
+      b.rs[d] = a.rs[j]; b.vp[d] = a.vp[j]; b.vr[d] = a.vr[j]; b.vr2[d] = a.vr2[j];
       b.qk[d] = a.qk[j]; b.ae[d] = a.ae[j]; b.ag[d] = a.ag[j];
     }
     b.cell[d] = a.cell[j];
@@ -939,8 +986,10 @@ struct HostTube {
 // One set of particle-state arrays (the binary64 or the mixed form's words).
 struct StateBufs {
   double *rs = nullptr, *vp = nullptr, *vr = nullptr, *qk = nullptr, *ae = nullptr, *ag = nullptr;
+  double* vr2 = nullptr;
   int *rdi = nullptr, *vpi = nullptr;
   float *vrf = nullptr, *qkf = nullptr, *aef = nullptr, *agf = nullptr;
+  float* vrf2 = nullptr;
   uint32_t* cell = nullptr;
   int* id = nullptr;
 };
@@ -1080,6 +1129,13 @@ extern "C" int kaos_set_grid(kaos_ctx* c, int32_t tube, const kaos_grid* g, int6
                    : (g->qGL[i] < g->qGH[i] && g->qGL[i] <= g->qGC[i] && g->qGC[i] <= g->qGH[i]);
     if (!ok) return fail(c, KAOS_E_ARG, "config cell " + std::to_string(i + 1) + " not monotone");
   }
+  if (g->icr) {
+    const double pv[] = {g->icr_eta_lh, g->icr_xi_perp1, g->icr_xi_perp2, g->icr_s0, g->icr_f0};
+    for (double v : pv)
+      if (!(v > 0.0) || !std::isfinite(v)) return fail(c, KAOS_E_ARG, "ICR parameters must be positive");
+    if (!(g->icr_lambda_perp >= 0.0) || !std::isfinite(g->icr_chi_perp1) || !std::isfinite(g->icr_chi_perp2))
+      return fail(c, KAOS_E_ARG, "bad ICR wavelength or spectral index");
+  }
   HostTube& h = c->ht[tube];
   h.g = *g;
   int nq = g->nq, np = g->nvpar, nr = g->nvperp;
@@ -1129,6 +1185,17 @@ static int build_device(kaos_ctx* c) {
     T.rGL0 = fabs(kaos_rs_of_q(g.qGL[0], g.p));
     T.rGHlast = fabs(kaos_rs_of_q(g.qGH[g.nq - 1], g.p));
     T.rsgn = T.desc ? 1.0 : -1.0;
+    T.icr = g.icr ? 1 : 0;
+    T.icrp.qs = g.qs; T.icrp.ms = g.ms; T.icrp.h = g.h;
+    T.icrp.lambda_perp = g.icr_lambda_perp; T.icrp.eta_lh = g.icr_eta_lh;
+    T.icrp.xi1 = g.icr_xi_perp1; T.icrp.xi2 = g.icr_xi_perp2; T.icrp.s0 = g.icr_s0;
+    T.icrp.omega_g0 = KAOS_TWOPI * g.icr_f0;
+    T.icrp.chi1 = g.icr_chi_perp1; T.icrp.chi2 = g.icr_chi_perp2;
+    {
+      double sds = 0.0;       // heating region: the whole tube, sum of hqC dqC in cell order
+      for (int i = 0; i < g.nq; ++i) sds = sds + h.hq[i] * h.dq[i];
+      T.icrp.sum_ds = sds;
+    }
     // fixed-point exponent of a cell of width wd: 2^k wd < 2^30, so offsets up to
     // two widths fit in 31 bits
     auto fx = [](double wd) { int e = 0; frexp(wd, &e); return 30 - e; };
@@ -1213,6 +1280,7 @@ static int build_device(kaos_ctx* c) {
       HIPCHK(c, hipMalloc((void**)&B.rdi, sizeof(int) * nslot));
       HIPCHK(c, hipMalloc((void**)&B.vpi, sizeof(int) * nslot));
       HIPCHK(c, hipMalloc((void**)&B.vrf, sizeof(float) * nslot));
+      HIPCHK(c, hipMalloc((void**)&B.vrf2, sizeof(float) * nslot));
       HIPCHK(c, hipMalloc((void**)&B.qkf, sizeof(float) * nslot));
       HIPCHK(c, hipMalloc((void**)&B.aef, sizeof(float) * nslot));
       HIPCHK(c, hipMalloc((void**)&B.agf, sizeof(float) * nslot));
@@ -1220,6 +1288,7 @@ static int build_device(kaos_ctx* c) {
       HIPCHK(c, hipMalloc((void**)&B.rs, sizeof(double) * nslot));
       HIPCHK(c, hipMalloc((void**)&B.vp, sizeof(double) * nslot));
       HIPCHK(c, hipMalloc((void**)&B.vr, sizeof(double) * nslot));
+      HIPCHK(c, hipMalloc((void**)&B.vr2, sizeof(double) * nslot));
       HIPCHK(c, hipMalloc((void**)&B.qk, sizeof(double) * nslot));
       HIPCHK(c, hipMalloc((void**)&B.ae, sizeof(double) * nslot));
       HIPCHK(c, hipMalloc((void**)&B.ag, sizeof(double) * nslot));
@@ -1283,8 +1352,8 @@ static int build_device(kaos_ctx* c) {
 }
 
 static void set_state(StepArgs& a, const StateBufs& B) {
-  a.rs = B.rs; a.vp = B.vp; a.vr = B.vr; a.qk = B.qk; a.ae = B.ae; a.ag = B.ag;
-  a.rdi = B.rdi; a.vpi = B.vpi; a.vrf = B.vrf; a.qkf = B.qkf; a.aef = B.aef; a.agf = B.agf;
+  a.rs = B.rs; a.vp = B.vp; a.vr = B.vr; a.qk = B.qk; a.ae = B.ae; a.ag = B.ag; a.vr2 = B.vr2;
+  a.rdi = B.rdi; a.vpi = B.vpi; a.vrf = B.vrf; a.qkf = B.qkf; a.aef = B.aef; a.agf = B.agf; a.vrf2 = B.vrf2;
   a.cell = B.cell; a.id = B.id;
 }
 
@@ -1305,9 +1374,34 @@ static StepArgs make_args(kaos_ctx* c, int interp, int count, int remove, int qk
   a.ngroups = (c->ntiles + c->launch.tpb - 1) / c->launch.tpb;
   a.work = c->d_work;
   a.vbox = c->d_vbox;
+  a.stepno = c->d_stepno;
+  a.seed = c->cfg.seed;
   a.hbox = c->d_hbox;
   a.do_interp = interp; a.do_count = count; a.do_remove = remove; a.do_qk = qk;
   return a;
+}
+
+extern "C" int kaos_upload_vperp(kaos_ctx* c, int32_t tube, int64_t n, const double* vperp1, const double* vperp2) {
+  if (!c) return KAOS_E_ARG;
+  if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
+  if (c->last_step != 0) return fail(c, KAOS_E_STATE, "kaos_upload_vperp after the first step");
+  if (!c->ht[tube].uploaded || n != c->ht[tube].count)
+    return fail(c, KAOS_E_STATE, "kaos_upload_vperp needs the tube's uploaded ions (same n)");
+  if (n > 0 && (!vperp1 || !vperp2)) return fail(c, KAOS_E_ARG, "bad vperp arrays");
+  if (n == 0) return KAOS_OK;
+  const DevTube& T = c->dt[tube];
+  double *d1 = nullptr, *d2 = nullptr;
+  HIPCHK(c, hipMalloc((void**)&d1, sizeof(double) * n));
+  HIPCHK(c, hipMalloc((void**)&d2, sizeof(double) * n));
+  HIPCHK(c, hipMemcpy(d1, vperp1, sizeof(double) * n, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(d2, vperp2, sizeof(double) * n, hipMemcpyHostToDevice));
+  StepArgs a = make_args(c, 0, 0, 0, 0);
+  hipLaunchKernelGGL(k_set_vperp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, T, (long long)n, d1,
+                     d2, c->d_vperpGH + T.woff, a, (int)(c->cfg.precision == KAOS_MIXED));
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(d1); (void)hipFree(d2);
+  return KAOS_OK;
 }
 
 extern "C" int kaos_upload_particles(kaos_ctx* c, int32_t tube, int64_t n, const double* q0,
@@ -1734,7 +1828,7 @@ extern "C" int kaos_download_particles(kaos_ctx* c, int32_t tube, int64_t n_cap,
   const DevTube& T = c->dt[tube];
   long long n = c->ht[tube].count;
   const bool mixed = c->cfg.precision == KAOS_MIXED;
-  std::vector<double> vr(n), qk(n), ae(n), agv(n), xyz(6 * (size_t)n);
+  std::vector<double> vr(n), vr2(n), qk(n), ae(n), agv(n), xyz(6 * (size_t)n);
   std::vector<uint32_t> cell(n);
   std::vector<int> ids(n);
   std::vector<char> live(n, 0);
@@ -1753,9 +1847,9 @@ extern "C" int kaos_download_particles(kaos_ctx* c, int32_t tube, int64_t n_cap,
     HIPCHK(c, hipMemcpy(ids.data(), B.id + T.off, sizeof(int) * n, hipMemcpyDeviceToHost));
     if (mixed) {
       std::vector<float> f(n);
-      float* src[4] = {B.vrf, B.qkf, B.aef, B.agf};
-      std::vector<double>* dst[4] = {&vr, &qk, &ae, &agv};
-      for (int i = 0; i < 4; ++i) {
+      float* src[5] = {B.vrf, B.vrf2, B.qkf, B.aef, B.agf};
+      std::vector<double>* dst[5] = {&vr, &vr2, &qk, &ae, &agv};
+      for (int i = 0; i < 5; ++i) {
         HIPCHK(c, hipMemcpy(f.data(), src[i] + T.off, sizeof(float) * n, hipMemcpyDeviceToHost));
         for (long long j = 0; j < n; ++j) (*dst[i])[j] = (double)f[j];
       }
@@ -1764,6 +1858,7 @@ extern "C" int kaos_download_particles(kaos_ctx* c, int32_t tube, int64_t n_cap,
       std::vector<double> rs(n);
       HIPCHK(c, hipMemcpy(rs.data(), B.rs + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
       HIPCHK(c, hipMemcpy(vr.data(), B.vr + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(vr2.data(), B.vr2 + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
       HIPCHK(c, hipMemcpy(qk.data(), B.qk + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
       HIPCHK(c, hipMemcpy(ae.data(), B.ae + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
       HIPCHK(c, hipMemcpy(agv.data(), B.ag + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
@@ -1784,7 +1879,7 @@ extern "C" int kaos_download_particles(kaos_ctx* c, int32_t tube, int64_t n_cap,
     if (Vx) Vx[k] = xyz[6 * j + 3];
     if (Vy) Vy[k] = xyz[6 * j + 4];
     if (Vz) Vz[k] = xyz[6 * j + 5];
-    if (vperp) vperp[k] = vr[j];
+    if (vperp) vperp[k] = fabs(sqrt(vr[j] * vr[j] + vr2[j] * vr2[j]));   // |(Vperp1, Vperp2)|
     if (qk4) qk4[k] = qk[j];
     if (aea) aea[k] = ae[j];
     if (ag) ag[k] = agv[j];
@@ -1867,7 +1962,8 @@ extern "C" int kaos_finalize(kaos_ctx* c) {
   (void)hipFree(c->d_vmarg_par); (void)hipFree(c->d_vmarg_perp);
   (void)hipFree(c->d_stepno);
   for (StateBufs& B : c->st) {
-    void* sb[] = {B.rs, B.vp, B.vr, B.qk, B.ae, B.ag, B.rdi, B.vpi, B.vrf, B.qkf, B.aef, B.agf, B.cell, B.id};
+    void* sb[] = {B.rs, B.vp, B.vr, B.qk, B.ae, B.ag, B.rdi, B.vpi, B.vrf, B.qkf, B.aef, B.agf, B.cell, B.id,
+                  B.vr2, B.vrf2};
     for (void* p : sb) if (p) (void)hipFree(p);
   }
   if (c->stream) (void)hipStreamDestroy(c->stream);

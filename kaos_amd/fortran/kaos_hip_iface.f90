@@ -32,6 +32,10 @@ module kaos_hip
     real(c_double) :: p, cphi, sphi, h, ms, qs, nsnormfac, Ts_lb, Ts_ub
     type(c_ptr) :: qGL, qGH, qGC, hqC, dqC, d3xC, rGC, cthGC, ellGC, Te
     type(c_ptr) :: vparGL, vparGH, vparGC, dvpar, vperpGL, vperpGH, vperpGC, dvperp, d3v
+    ! ICR wave heating (ICRflagT; stochastic): off unless icr = 1
+    integer(c_int32_t) :: icr = 0, pad1 = 0
+    real(c_double) :: icr_lambda_perp = 0, icr_eta_lh = 0, icr_xi_perp1 = 0, icr_xi_perp2 = 0
+    real(c_double) :: icr_s0 = 0, icr_f0 = 0, icr_chi_perp1 = 0, icr_chi_perp2 = 0
   end type kaos_grid
 
   type, bind(c), public :: kaos_step_out
@@ -39,7 +43,7 @@ module kaos_hip
     integer(c_int32_t) :: master, status
   end type kaos_step_out
 
-  public :: kaos_init, kaos_set_grid, kaos_upload_particles, kaos_step, kaos_run, &
+  public :: kaos_init, kaos_set_grid, kaos_upload_particles, kaos_upload_vperp, kaos_step, kaos_run, &
             kaos_download_master, kaos_count, kaos_download_particles, kaos_set_launch, &
             kaos_get_launch, kaos_profile, &
             kaos_profile_read, kaos_last_error, kaos_finalize, kaos_error_message
@@ -67,6 +71,14 @@ module kaos_hip
       integer(c_int64_t), value :: n
       real(c_double), intent(in) :: q0(*), vpar0(*), vperp0(*)
     end function kaos_upload_particles
+
+    integer(c_int) function kaos_upload_vperp(ctx, tube, n, vperp1, vperp2) bind(c, name='kaos_upload_vperp')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tube
+      integer(c_int64_t), value :: n
+      real(c_double), intent(in) :: vperp1(*), vperp2(*)
+    end function kaos_upload_vperp
 
     integer(c_int) function kaos_step(ctx, n, uniforms, n_uniforms, out) bind(c, name='kaos_step')
       import :: c_int, c_ptr, c_int64_t

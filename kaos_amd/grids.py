@@ -66,6 +66,17 @@ class Tube:
     filter_pt: int = 11
     lbcond: int = 1
     ubcond: int = 1
+    # ICR wave heating (stochastic); defaults are the reference's
+    # _ModelInput.f90:214-229 values
+    icr: int = 0
+    icr_lambda_perp: float = 0.0
+    icr_eta_lh: float = 0.125
+    icr_xi_perp1: float = 0.5
+    icr_xi_perp2: float = 0.5
+    icr_s0: float = 5e-5
+    icr_f0: float = 6.5
+    icr_chi_perp1: float = 2.1
+    icr_chi_perp2: float = 2.1
 
     @property
     def cphi(self):

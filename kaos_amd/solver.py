@@ -34,6 +34,9 @@ def grid_struct(tube):
               "Ts_ub"):
         setattr(g, n, getattr(tube, n))
     g.cphi, g.sphi = tube.cphi, tube.sphi
+    g.icr = getattr(tube, "icr", 0)
+    for n in _abi.ICR_FIELDS:
+        setattr(g, n, getattr(tube, n, 0.0))
     keep = []
     for n in _abi.GRID_ARRAYS:
         a = np.ascontiguousarray(tube.arrays[n], dtype=np.float64)
@@ -71,6 +74,10 @@ class KineticSolver:
         q0, vpar0, vperp0 = (np.ascontiguousarray(a, dtype=np.float64) for a in (q0, vpar0, vperp0))
         self._chk(self.lib.kaos_upload_particles(self.h, tube, len(q0), ptr(q0), ptr(vpar0),
                                                  ptr(vperp0)))
+
+    def upload_vperp(self, tube, vperp1, vperp2):
+        v1, v2 = (np.ascontiguousarray(a, dtype=np.float64) for a in (vperp1, vperp2))
+        self._chk(self.lib.kaos_upload_vperp(self.h, tube, len(v1), ptr(v1), ptr(v2)))
 
     def step(self, n, uniforms=None, wait=True):
         out = KaosStepOut() if wait else None

@@ -15,6 +15,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _dp = C.POINTER(C.c_double)
 
+ICR_FIELDS = ("icr_lambda_perp", "icr_eta_lh", "icr_xi_perp1", "icr_xi_perp2", "icr_s0", "icr_f0",
+              "icr_chi_perp1", "icr_chi_perp2")
 ARRAYS = ("qGL", "qGH", "qGC", "hqC", "dqC", "d3xC", "rGC", "cthGC", "ellGC", "Te",
           "vparGL", "vparGH", "vparGC", "dvpar", "vperpGL", "vperpGH", "vperpGC", "dvperp", "d3v")
 
@@ -26,7 +28,9 @@ class OrcCfg(C.Structure):
                 + [(n, C.c_double) for n in ("p", "cphi", "sphi", "h", "ms", "qs", "nsnormfac",
                                              "Ts_lb", "Ts_ub")]
                 + [("tube", C.c_int), ("seed", C.c_uint64)]
-                + [(n, _dp) for n in ARRAYS])
+                + [(n, _dp) for n in ARRAYS]
+                + [("icr", C.c_int)]
+                + [(n, C.c_double) for n in ICR_FIELDS])
 
 
 _lib = None
@@ -90,6 +94,15 @@ def load():
         u64p = C.POINTER(C.c_uint64)
         lib.orc_philox.argtypes = [u64p, u64p, u64p]
         lib.orc_inject_uniforms.argtypes = [C.c_uint64, C.c_int64, C.c_int, C.c_int64, _dp]
+        lib.orc_exp.argtypes = [C.c_double]
+        lib.orc_exp.restype = C.c_double
+        u32p = C.POINTER(C.c_uint32)
+        lib.orc_philox4x32.argtypes = [u32p, u32p, u32p]
+        lib.orc_icr_uniforms.argtypes = [C.c_uint64, C.c_int64, C.c_int, C.c_int64, _dp]
+        lib.orc_icr_kick.argtypes = [C.POINTER(OrcCfg), C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.c_double, _dp, _dp]
+        lib.orc_set_vperp.argtypes = [P, _dp, _dp]
+        lib.orc_get_vperp12.argtypes = [P, _dp, _dp, C.POINTER(C.c_int64)]
         _lib = lib
     return _lib
 
@@ -107,6 +120,9 @@ def cfg_of(tube, ndatfac=1, seed=1, tube_index=0):
     c.ndatfac = ndatfac
     c.tube = tube_index
     c.seed = seed
+    c.icr = getattr(tube, "icr", 0)
+    for n in ICR_FIELDS:
+        setattr(c, n, getattr(tube, n, 0.0))
     keep = []
     for n in ARRAYS:
         a = np.ascontiguousarray(tube.arrays[n], dtype=np.float64)
@@ -135,6 +151,16 @@ class Oracle:
 
     def count(self):
         return self.lib.orc_count(self.h)
+
+    def set_vperp(self, v1, v2):
+        v1, v2 = (np.ascontiguousarray(a, dtype=np.float64) for a in (v1, v2))
+        self.lib.orc_set_vperp(self.h, ptr(v1), ptr(v2))
+
+    def vperp12(self):
+        n = self.count()
+        v1, v2, ids = np.zeros(n), np.zeros(n), np.zeros(n, np.int64)
+        self.lib.orc_get_vperp12(self.h, ptr(v1), ptr(v2), ptr(ids, C.c_int64))
+        return v1, v2, ids
 
     def particles(self):
         n = self.count()

@@ -47,13 +47,20 @@ typedef struct {
   const double *vparGL, *vparGH, *vparGC, *dvpar;       /* nvpar  */
   const double *vperpGL, *vperpGH, *vperpGC, *dvperp;   /* nvperp */
   const double *d3v;                                    /* nvperp*nvpar, [iperp][ipar] */
+  /* ICR wave heating (ICRflagT, stochastic; _ModelInput.f90:214-229) */
+  int icr;
+  double icr_lambda_perp, icr_eta_lh, icr_xi_perp1, icr_xi_perp2, icr_s0, icr_f0, icr_chi_perp1,
+      icr_chi_perp2;
 } orc_cfg;
 
 typedef struct {
   orc_cfg c;
   int64_t ns, cap;
   double *rs, *vs, *vperp, *aea, *ag, *aeaN, *agN, *qk4, *q0, *vpar0;
+  double *v1, *v2;                 /* Vperp1, Vperp2 */
   int *qind, *vparind, *vperpind;
+  int64_t *id;                     /* upload index, then injection order (keys the heating stream) */
+  int64_t next_id;
   int master_last;                 /* last step was a master step */
   int64_t nn;                      /* index of the last master step */
   /* per master step outputs */
@@ -208,8 +215,9 @@ static void counts_and_moments(orc_t *o, const int *qi, const int *vpi, const in
 
 /* ---- state management ------------------------------------------------------ */
 
-#define ARR_D(X) X(rs) X(vs) X(vperp) X(aea) X(ag) X(aeaN) X(agN) X(qk4) X(q0) X(vpar0)
+#define ARR_D(X) X(rs) X(vs) X(vperp) X(aea) X(ag) X(aeaN) X(agN) X(qk4) X(q0) X(vpar0) X(v1) X(v2)
 #define ARR_I(X) X(qind) X(vparind) X(vperpind)
+#define ARR_L(X) X(id)
 
 void *orc_create(const orc_cfg *cfg, int64_t ns, const double *q0, const double *vpar0,
                  const double *vperp0, int64_t cap) {
@@ -221,6 +229,9 @@ void *orc_create(const orc_cfg *cfg, int64_t ns, const double *q0, const double 
 #undef AL
 #define AL(n) o->n = (int *)xc(sizeof(int) * o->cap);
   ARR_I(AL)
+#undef AL
+#define AL(n) o->n = (int64_t *)xc(sizeof(int64_t) * o->cap);
+  ARR_L(AL)
 #undef AL
   int nq = cfg->nq;
   int64_t ncell = (int64_t)cfg->nvperp * cfg->nvpar;
@@ -239,9 +250,13 @@ void *orc_create(const orc_cfg *cfg, int64_t ns, const double *q0, const double 
     o->rs[j] = kaos_rs_of_q(q0[j], cfg->p);
     o->vs[j] = vpar0[j];
     o->vperp[j] = vperp0[j];
+    o->v1[j] = vperp0[j];        /* gyrophase 0 unless orc_set_vperp sets both components */
+    o->v2[j] = 0.0;
     o->q0[j] = q0[j];
     o->vpar0[j] = vpar0[j];
+    o->id[j] = j;
   }
+  o->next_id = ns;
   return o;
 }
 
@@ -249,7 +264,7 @@ void orc_destroy(void *h) {
   orc_t *o = (orc_t *)h;
   if (!o) return;
 #define FR(n) free(o->n);
-  ARR_D(FR) ARR_I(FR)
+  ARR_D(FR) ARR_I(FR) ARR_L(FR)
 #undef FR
   free(o->nq_counts); free(o->nph_counts); free(o->nq_u); free(o->nph_u); free(o->NqRT); free(o->M); free(o->Mprev);
   free(o->EA); free(o->EG); free(o->M0F);
@@ -266,7 +281,7 @@ static void bc_remove(orc_t *o) {
     if (o->qind[j] == -1) { ++nub; continue; }
     if (k != j) {
 #define MV(n) o->n[k] = o->n[j];
-      ARR_D(MV) ARR_I(MV)
+      ARR_D(MV) ARR_I(MV) ARR_L(MV)
 #undef MV
     }
     ++k;
@@ -303,6 +318,9 @@ static int inject(orc_t *o, int64_t n, const double *uni) {
     kaos_mb_project(u, sigma, c->cphi, c->sphi, L, &vpar, &v1, &v2, &vperp);
     o->vs[j] = vpar;
     o->vperp[j] = vperp;
+    o->v1[j] = v1;
+    o->v2[j] = v2;
+    o->id[j] = o->next_id + k;
     o->vparind[j] = scan_vcell(c->vparGL, c->vparGH, c->nvpar, vpar, 0);
     o->vperpind[j] = scan_vcell(c->vperpGL, c->vperpGH, c->nvperp, vperp, 0);
     o->aea[j] = o->aeaN[j] = fabs(qm * fabs(o->EA[Qb]));
@@ -310,13 +328,34 @@ static int inject(orc_t *o, int64_t n, const double *uni) {
     o->qk4[j] = qc;
   }
   o->ns += nl + nu;
+  o->next_id += nl + nu;
   o->injected = nl + nu;
   return 0;
 }
 
-static void push_all(orc_t *o) {
+/* Heating parameters of the tube; the heating region's arc length is the sum
+ * of dsICRGT = hqC dqC over the cells (_1_SimParameterization.f90:1185-1186,
+ * summed in WaveHeatingSub, _8_1_1_0_WaveHeating.f90:100). */
+static kaos_icr_t icr_params(const orc_cfg *c) {
+  kaos_icr_t P;
+  P.qs = c->qs; P.ms = c->ms; P.h = c->h;
+  P.lambda_perp = c->icr_lambda_perp; P.eta_lh = c->icr_eta_lh;
+  P.xi1 = c->icr_xi_perp1; P.xi2 = c->icr_xi_perp2; P.s0 = c->icr_s0;
+  P.omega_g0 = KAOS_TWOPI * c->icr_f0;
+  P.chi1 = c->icr_chi_perp1; P.chi2 = c->icr_chi_perp2;
+  double s = 0.0;
+  for (int i = 0; i < c->nq; ++i) s = s + c->hqC[i] * c->dqC[i];
+  P.sum_ds = s;
+  return P;
+}
+
+/* KineticRK4UpdateSub over the live ions; with ICR on, WaveHeatingSub's kick
+ * at stage A for ions inside the grid (_8_1_1_KineticUpdateA.f90:1392,
+ * _8_1_1_0_WaveHeating.f90:32-33), after the pre-push Vperp cell is taken. */
+static void push_all(orc_t *o, int64_t n) {
   orc_cfg *c = &o->c;
   kaos_push_par_t P = kaos_push_par(c->h, c->p, c->cphi, c->sphi, c->grav, c->eamb, c->eambsign);
+  kaos_icr_t IP = icr_params(c);
   for (int64_t j = 0; j < o->ns; ++j) {
     kaos_line_t L = kaos_line(o->rs[j], c->p);
     kaos_qb_t gA = kaos_stageA(L);
@@ -326,6 +365,12 @@ static void push_all(orc_t *o) {
     o->qind[j] = qs;
     o->vparind[j] = scan_vcell(c->vparGL, c->vparGH, c->nvpar, r.vpar, o->vparind[j]);
     o->vperpind[j] = scan_vcell(c->vperpGL, c->vperpGH, c->nvperp, o->vperp[j], o->vperpind[j]);
+    if (c->icr && qs != 0 && qs != -1) {
+      double u[4], a1, a2, ap;
+      kaos_icr_uniforms(c->seed, n, c->tube, o->id[j], u);
+      kaos_icr_kick(&IP, L.r, L.w, r.vpar, o->v1[j], o->v2[j], u, &a1, &a2, &ap);
+      o->v1[j] = a1; o->v2[j] = a2; o->vperp[j] = ap;
+    }
     o->rs[j] = r.rsN;
     o->vs[j] = r.vparN;
     o->qk4[j] = r.qk4;
@@ -380,7 +425,7 @@ int orc_step(void *h, int64_t n, const double *uniforms) {
     o->master_last = master;
   }
   for (int64_t j = 0; j < o->ns; ++j) { o->aea[j] = o->aeaN[j]; o->ag[j] = o->agN[j]; }
-  push_all(o);
+  push_all(o, n);
   if (n > 1 && o->master_last) {
     int rc = inject(o, n, uniforms);
     if (rc) return rc;
@@ -417,6 +462,23 @@ void orc_get_particles(void *h, double *x, double *y, double *z, double *Vx, dou
   if (qind) memcpy(qind, o->qind, ni);
   if (vparind) memcpy(vparind, o->vparind, ni);
   if (vperpind) memcpy(vperpind, o->vperpind, ni);
+}
+
+/* Both perpendicular components of the uploaded ions (Vperp10T, Vperp20T;
+ * KSA:389-391); Vperp = |(Vperp1, Vperp2)|. */
+void orc_set_vperp(void *h, const double *v1, const double *v2) {
+  orc_t *o = (orc_t *)h;
+  for (int64_t j = 0; j < o->ns; ++j) {
+    o->v1[j] = v1[j]; o->v2[j] = v2[j];
+    o->vperp[j] = fabs(sqrt(v1[j] * v1[j] + v2[j] * v2[j]));
+  }
+}
+
+void orc_get_vperp12(void *h, double *v1, double *v2, int64_t *id) {
+  orc_t *o = (orc_t *)h;
+  if (v1) memcpy(v1, o->v1, sizeof(double) * o->ns);
+  if (v2) memcpy(v2, o->v2, sizeof(double) * o->ns);
+  if (id) memcpy(id, o->id, sizeof(int64_t) * o->ns);
 }
 
 /* Line state (signed radius, V||) of the live ions. */
@@ -539,6 +601,16 @@ double orc_root_r(double q, double p) { return kaos_root_r(q, p); }
 double orc_quartic_newton(double A, double B, double ip, double g0) { return kaos_quartic_newton(A, B, ip, g0); }
 double orc_sinh_asinh_third(double y) { return kaos_sinh_asinh_third(y); }
 double orc_log(double x) { return kaos_log(x); }
+double orc_exp(double x) { return kaos_exp(x); }
+void orc_philox4x32(const uint32_t *ctr, const uint32_t *key, uint32_t *out) { kaos_philox4x32(ctr, key, out); }
+void orc_icr_uniforms(uint64_t seed, int64_t step, int tube, int64_t id, double *u) {
+  kaos_icr_uniforms(seed, step, tube, id, u);
+}
+void orc_icr_kick(const orc_cfg *c, double r, double w, double vpar, double v1, double v2, const double *u,
+                  double *out3) {
+  kaos_icr_t P = icr_params(c);
+  kaos_icr_kick(&P, r, w, vpar, v1, v2, u, &out3[0], &out3[1], &out3[2]);
+}
 double orc_cos2pi(double u) { return kaos_cos2pi(u); }
 void orc_philox(const uint64_t *ctr, const uint64_t *key, uint64_t *out) { kaos_philox4x64(ctr, key, out); }
 void orc_inject_uniforms(uint64_t seed, int64_t step, int tube, int64_t k, double *u) {

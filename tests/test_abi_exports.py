@@ -48,6 +48,7 @@ int main(void) {
   printf("%zu %zu %zu %zu\n", sizeof(kaos_cfg), offsetof(kaos_cfg, seed), sizeof(kaos_grid), offsetof(kaos_grid, d3v));
   printf("%zu %zu %zu\n", offsetof(kaos_grid, n_inj_lb), offsetof(kaos_grid, p), offsetof(kaos_grid, qGL));
   printf("%zu %zu\n", sizeof(kaos_step_out), offsetof(kaos_step_out, status));
+  printf("%zu %zu %zu\n", offsetof(kaos_grid, icr), offsetof(kaos_grid, icr_lambda_perp), offsetof(kaos_grid, icr_chi_perp2));
   return 0;
 }
 '''
@@ -64,5 +65,6 @@ int main(void) {
     got = [int(x) for x in out]
     want = [C.sizeof(_abi.KaosCfg), _abi.KaosCfg.seed.offset, C.sizeof(_abi.KaosGrid), _abi.KaosGrid.d3v.offset,
             _abi.KaosGrid.n_inj_lb.offset, _abi.KaosGrid.p.offset, _abi.KaosGrid.qGL.offset,
-            C.sizeof(_abi.KaosStepOut), _abi.KaosStepOut.status.offset]
+            C.sizeof(_abi.KaosStepOut), _abi.KaosStepOut.status.offset,
+            _abi.KaosGrid.icr.offset, _abi.KaosGrid.icr_lambda_perp.offset, _abi.KaosGrid.icr_chi_perp2.offset]
     assert got == want

@@ -499,3 +499,116 @@ def test_injected_ions_at_boundary_cell_centre():
     assert abs(V - abs(o4[0])) <= 1e-12 * max(abs(o4[0]), 1.0)
     assert abs(inj["vperp"][0] - o4[3]) <= 1e-12 * o4[3]
     o.close()
+
+
+# --- ICR wave heating (_8_1_1_0_WaveHeating.f90:32-314) -----------------------
+
+def _philox4x32(lib, ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib.orc_philox4x32(c, k, o)
+    return list(o)
+
+
+def test_philox4x32_known_answers(lib):
+    # Random123's published known-answer vectors for Philox4x32-10
+    assert _philox4x32(lib, [0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert _philox4x32(lib, [0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6,
+                                                                     0x6D5451FD]
+    assert _philox4x32(lib, [0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0]) == \
+        [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_exp_vs_libm(lib):
+    for x in np.concatenate([np.linspace(-700.0, 700.0, 4001), np.linspace(-1.0, 1.0, 2001)]):
+        e = math.exp(x)
+        assert abs(lib.orc_exp(float(x)) - e) <= 4e-16 * e
+
+
+def _icr_tube(**kw):
+    t = grids.config1(n_total=2000)
+    t.icr = 1
+    for k, v in kw.items():
+        setattr(t, k, v)
+    return t
+
+
+def _kick(lib, cfg, r, w, vpar, v1, v2, u):
+    o = np.zeros(3)
+    ua = np.ascontiguousarray(u, dtype=np.float64)
+    lib.orc_icr_kick(C.byref(cfg), r, w, vpar, v1, v2, ptr(ua), ptr(o))
+    return o
+
+
+def _ref_kick_unit(t, r, w, vpar, vperp):
+    """DPerp1/2, tauPerp and the kick for unit Gaussians, from the reference's
+    lines (WaveHeating:75-92, 100-105, 140-146) and B, OmegaG
+    (_KineticMainParams.f90:621-633), in numpy."""
+    ell = 1.0 + 3.0 * (1.0 - w)
+    B = 8.06e15 * math.sqrt(ell) / r ** 3
+    og = t.qs * B / t.ms
+    og0 = 2.0 * math.pi * t.icr_f0
+    lam = t.icr_lambda_perp
+    sig = 1.0
+    if lam != 0.0 and 2.0 * math.pi * vperp / og >= lam:
+        sig = 2.0 * math.pi * vperp / (lam * og)
+    D1 = (t.qs ** 2 / (2 * t.ms ** 2)) * (t.icr_xi_perp1 * t.icr_eta_lh) * t.icr_s0 * (og / og0) ** (-t.icr_chi_perp1) * sig ** -3
+    D2 = (t.qs ** 2 / (2 * t.ms ** 2)) * (t.icr_xi_perp2 * t.icr_eta_lh) * t.icr_s0 * (og / og0) ** (-t.icr_chi_perp2) * sig ** -3
+    sumds = float(np.sum(t.arrays["hqC"] * t.arrays["dqC"]))
+    eps = abs(abs(vpar) / (og * sumds))
+    tau = 1.0 / ((og / (2 * math.pi)) * math.sqrt(eps))
+    if t.h > tau:
+        return 0.0, 0.0, tau
+    return (t.h / tau) * math.sqrt(2 * D1 * tau), (t.h / tau) * math.sqrt(2 * D2 * tau), tau
+
+
+@pytest.mark.parametrize("chi2,lam", [(2.1, 0.0), (2.0, 0.0), (2.1, 3.0e3)])
+def test_icr_kick_closed_form_unit_gaussians(lib, chi2, lam):
+    # u = (exp(-1/2), 1, exp(-1/2), 1): both Gaussians are exactly sqrt(1) cos(2 pi) = 1
+    t = _icr_tube(icr_chi_perp2=chi2, icr_lambda_perp=lam)
+    cfg = oracle.cfg_of(t, 1, 1, 0)
+    u = [math.exp(-0.5), 1.0, math.exp(-0.5), 1.0]
+    for r_re, vpar, v1, v2 in ((1.2, 900.0, 300.0, -200.0), (2.5, -1500.0, 50.0, 4000.0), (1.05, 20.0, 0.0, 0.0)):
+        r = r_re * RE
+        w = r / (t.p * RE)
+        vperp = math.hypot(v1, v2)
+        k1, k2, tau = _ref_kick_unit(t, r, w, vpar, vperp)
+        o = _kick(lib, cfg, r, w, vpar, v1, v2, u)
+        assert abs(o[0] - (v1 + k1)) <= 1e-12 * max(abs(v1 + k1), 1.0)
+        assert abs(o[1] - (v2 + k2)) <= 1e-12 * max(abs(v2 + k2), 1.0)
+        assert abs(o[2] - math.hypot(o[0], o[1])) <= 1e-15 * o[2]
+    # u2 = 1/4: cos(pi/2) = 0, so the first component is not kicked at all
+    o = _kick(lib, cfg, 1.5 * RE, 1.5 / t.p, 800.0, 123.0, 45.0, [0.3, 0.25, 0.3, 0.25])
+    assert o[0] == 123.0 and o[1] == 45.0
+
+
+def test_icr_no_kick_when_step_exceeds_interaction_time(lib):
+    # tauPerp < h (fast ions, short heating region) -> DVperp = 0 (WaveHeating:147-150)
+    t = _icr_tube()
+    t.arrays = dict(t.arrays)
+    t.arrays["hqC"] = t.arrays["hqC"] * 1e-12
+    cfg = oracle.cfg_of(t, 1, 1, 0)
+    k1, k2, tau = _ref_kick_unit(t, 1.5 * RE, 1.5 / t.p, 5000.0, 10.0)
+    assert tau < t.h and k1 == 0.0
+    o = _kick(lib, cfg, 1.5 * RE, 1.5 / t.p, 5000.0, 10.0, -3.0, [0.7, 1.0, 0.7, 1.0])
+    assert o[0] == 10.0 and o[1] == -3.0
+
+
+def test_icr_kick_statistics(lib):
+    # the kicks are (h/tau) sqrt(2 D tau) times unit Gaussians from the seeded stream
+    t = _icr_tube()
+    cfg = oracle.cfg_of(t, 1, 1, 0)
+    r, w, vpar = 1.3 * RE, 1.3 / t.p, 700.0
+    k1, k2, _ = _ref_kick_unit(t, r, w, vpar, 0.0)
+    n = 40000
+    d1, d2 = np.zeros(n), np.zeros(n)
+    u = np.zeros(4)
+    for i in range(n):
+        lib.orc_icr_uniforms(7, 3, 0, i, ptr(u))
+        o = _kick(lib, cfg, r, w, vpar, 0.0, 0.0, u)
+        d1[i], d2[i] = o[0], o[1]
+    for d, k in ((d1, k1), (d2, k2)):
+        assert abs(d.mean()) < 4.0 * k / math.sqrt(n)
+        assert abs(d.std() / k - 1.0) < 0.02
+    assert abs(np.corrcoef(d1, d2)[0, 1]) < 0.02
