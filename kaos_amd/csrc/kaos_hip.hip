@@ -249,27 +249,6 @@ __device__ __forceinline__ int kd_find_qcell_fast(double q, const double* qGH, d
   return ok ? i + 1 : kaos_find_qcell_from(q, qGH, qGL0, qGHlast, nq, desc, hint);
 }
 
-// ICR heating of one ion in the FP64 push (WaveHeatingSub at stage A,
-// _8_1_1_KineticUpdateA.f90:1392): the binning Vperp cell iq becomes the cell
-// of the pre-kick |(Vperp1, Vperp2)| (Vperpindk1, KUA:1108-1127: off the grid
-// the previous index stays), then ions inside the grid get the kick.
-__device__ __noinline__ void kd_icr_push(const StepArgs& a, int t, uint32_t jl, int& iq, int qc, kaos_line_t L,
-                                         double vpar) {
-  const DevTube& T = a.tubes[t];
-  const long long j = T.off + jl;
-  const double v1 = a.vr[j], v2 = a.vr2[j];
-  const double* GH = a.vperpGH + T.woff;
-  const int iqn = kaos_find_vcell(fabs(sqrt(v1 * v1 + v2 * v2)), GH, GH[T.nvperp], T.nvperp);
-  if (iqn >= 1) iq = iqn;
-  if (qc != 0 && qc != -1) {
-    double u[4], v1N, v2N, vpN;
-    kaos_icr_uniforms(a.seed, *a.stepno, t, a.id[j], u);
-    kaos_icr_kick(&T.icrp, L.r, L.w, vpar, v1, v2, u, &v1N, &v2N, &vpN);
-    a.vr[j] = v1N;
-    a.vr2[j] = v2N;
-  }
-}
-
 // Static LDS (grids of up to MAXQ cells): qGH, the packed centres, vparGH
 // (high edges, then GL0).  Dynamic LDS: the window of 16-bit phase-space
 // counters (two per word, a.hwords words).  Loads and stores address the
@@ -296,7 +275,6 @@ __global__ void __launch_bounds__(KAOS_FP64_BLOCK, KAOS_FP64_WAVES_EU) k_push_fp
   long long toff = 0;
   double *rsT = nullptr, *vpT = nullptr, *qkT = nullptr, *aeT = nullptr, *agT = nullptr;
   uint32_t* cellT = nullptr;
-  int icr = 0;                        // ICR heating in this tube
   double qGL0 = 0.0, qGHlast = 0.0, qsms = 0.0, vscale = 0.0;
   int eambsign = 0;
   uint32_t tbox = 0u;
@@ -352,7 +330,6 @@ __global__ void __launch_bounds__(KAOS_FP64_BLOCK, KAOS_FP64_WAVES_EU) k_push_fp
       tcount = (uint32_t)a.count[tb];
       rsT = a.rs + toff; vpT = a.vp + toff; qkT = a.qk + toff; aeT = a.ae + toff; agT = a.ag + toff;
       cellT = a.cell + toff;
-      icr = T.icr;
       vscale = (double)nvpar / (a.vparGH[T.voff + nvpar - 1] - a.vparGH[T.voff + nvpar]);
       cur = tb;
       __syncthreads();
@@ -415,9 +392,7 @@ __global__ void __launch_bounds__(KAOS_FP64_BLOCK, KAOS_FP64_WAVES_EU) k_push_fp
       // off-grid keeps the previous index (KUA:1087-1127).  Vperp is constant on
       // this path (no wave heating), so its cell, found when the ion was binned
       // first (k_bin_initial, k_inject), is carried in the cell word.
-      const int ipc = (int)(cold & 0xFFu);
-      int iq = (int)((cold >> 8) & 0xFFu);
-      if (icr) kd_icr_push(a, cur, jl, iq, qc, L, o.vpar);
+      const int ipc = (int)(cold & 0xFFu), iq = (int)((cold >> 8) & 0xFFu);
       int ip = ipc;
       {
         const int ipx = max(ipc, 1);
@@ -457,6 +432,64 @@ __global__ void __launch_bounds__(KAOS_FP64_BLOCK, KAOS_FP64_WAVES_EU) k_push_fp
   }
   __syncthreads();
   if (tid == 0 && s_pushed) atomicAdd(a.pushed, (unsigned long long)s_pushed);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ICR wave heating of a step (WaveHeatingSub, called at stage A of the push,
+// _8_1_1_KineticUpdateA.f90:1392), as a pass before the push over the
+// pre-push state, in tubes with heating on.  Per live ion: the Vperp cell of
+// the cell word becomes the cell of the pre-kick |(Vperp1, Vperp2)| (the
+// push bins with it, Vperpindk1, KUA:1108-1127; off the grid the previous
+// index stays); an ion whose stage-A cell is inside the grid (WaveHeating:
+// 32-33) gets the kick of kaos_icr_kick from the seeded Philox4x32 stream.
+// The stage-A quantities (r, w, cell, V||) are the push's own: the same
+// shared functions on the same state.  The mixed form kicks in binary64 from
+// its binary32 components.
+__global__ void __launch_bounds__(KAOS_BLOCK) k_icr(StepArgs a, int mixed) {
+  const int tile = blockIdx.x;
+  if (tile >= a.ntiles) return;
+  const int t = a.tile_tube[tile];
+  const DevTube& T = a.tubes[t];
+  if (!T.icr) return;
+  const long long end = T.off + a.count[t];
+  const long long step = *a.stepno;
+  const double* GH = a.vperpGH + T.woff;
+  const double irep = 1.0 / (KAOS_RE * T.p);
+  for (int it = 0; it < KAOS_TILE_PPT; ++it) {
+    const long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
+    if (j >= end) break;
+    const uint32_t cw = a.cell[j];
+    double rs, vs, v1, v2;
+    if (mixed) {
+      if (cw == KAOS_DEAD) continue;
+      const int Q = kx_refq((int)((cw >> 16) & KAOS_QMASK) - 1, T.nq);
+      rs = a.rref[T.goff + Q - 1] + kx_dec(a.rdi[j], a.rk[T.goff + Q - 1], (cw & KAOS_RFLAG) != 0);
+      vs = kx_dec_v(a.vpi[j], (int)(cw & 0xFFu), (cw & KAOS_VFLAG) != 0, a.vparGH + T.voff, a.vk + T.voff,
+                    T.nvpar);
+      v1 = (double)a.vrf[j]; v2 = (double)a.vrf2[j];
+    } else {
+      rs = a.rs[j];
+      if (rs == 0.0) continue;
+      vs = a.vp[j];
+      v1 = a.vr[j]; v2 = a.vr2[j];
+    }
+    int iq = (int)((cw >> 8) & 0xFFu);
+    const int iqn = kaos_find_vcell(fabs(sqrt(v1 * v1 + v2 * v2)), GH, GH[T.nvperp], T.nvperp);
+    if (iqn >= 1) iq = iqn;
+    a.cell[j] = (cw & ~(0xFFu << 8)) | ((uint32_t)iq << 8);
+    const kaos_line_t L = kaos_line_i(rs, irep);
+    const kaos_qb_t gA = kaos_stageA(L);
+    const int qprev = (int)((cw >> 16) & KAOS_QMASK) - 1;
+    const int hint = (qprev >= 1) ? qprev : (qprev == 0 ? 1 : T.nq);
+    const int qc = kaos_find_qcell_from(gA.q, a.qGH + T.goff, T.qGL0, T.qGHlast, T.nq, T.desc, hint);
+    if (qc == 0 || qc == -1) continue;
+    const double vpar = gA.bR * (vs * gA.bR) + gA.bZ * (vs * gA.bZ);
+    double u[4], v1N, v2N, vpN;
+    kaos_icr_uniforms(a.seed, step, t, a.id[j], u);
+    kaos_icr_kick(&T.icrp, L.r, L.w, vpar, v1, v2, u, &v1N, &v2N, &vpN);
+    if (mixed) { a.vrf[j] = (float)v1N; a.vrf2[j] = (float)v2N; }
+    else { a.vr[j] = v1N; a.vr2[j] = v2N; }
   }
 }
 
@@ -1038,6 +1071,7 @@ struct kaos_ctx {
   struct Launch { int tpb, hwords, persistent; };
   Launch launch = {KAOS_TPB, KAOS_HWORDS, 0};
   unsigned int* d_work = nullptr;
+  int any_icr = 0;                              // some tube has ICR heating on
   int4* d_vbox = nullptr;                       // per tube velocity box of the push's LDS window
   int4* d_hbox = nullptr;                       // per (tube, cell) bounding box of the master step's counts
   unsigned int *d_vmarg_par = nullptr, *d_vmarg_perp = nullptr;
@@ -1186,6 +1220,7 @@ static int build_device(kaos_ctx* c) {
     T.rGHlast = fabs(kaos_rs_of_q(g.qGH[g.nq - 1], g.p));
     T.rsgn = T.desc ? 1.0 : -1.0;
     T.icr = g.icr ? 1 : 0;
+    if (T.icr) c->any_icr = 1;
     T.icrp.qs = g.qs; T.icrp.ms = g.ms; T.icrp.h = g.h;
     T.icrp.lambda_perp = g.icr_lambda_perp; T.icrp.eta_lh = g.icr_eta_lh;
     T.icrp.xi1 = g.icr_xi_perp1; T.icrp.xi2 = g.icr_xi_perp2; T.icrp.s0 = g.icr_s0;
@@ -1600,6 +1635,9 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
                        c->d_M, c->d_M0F, c->d_EA, c->d_esc_acc, c->d_esc_rep, 0);
     launch_moments(c, nullptr);
     launch_box_update(c);
+    if (c->any_icr)
+      hipLaunchKernelGGL(k_icr, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a,
+                         (int)(c->cfg.precision == KAOS_MIXED));
     launch_push(c, a);
   } else {
     if (master) {
@@ -1613,6 +1651,9 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
       if (rc) return rc;
     }
     StepArgs a = make_args(c, master, master, master, premaster);
+    if (c->any_icr)
+      hipLaunchKernelGGL(k_icr, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a,
+                         (int)(c->cfg.precision == KAOS_MIXED));
     launch_push(c, a, e0, e1);
     if (master) {
       if (c->inj_total > 0) {

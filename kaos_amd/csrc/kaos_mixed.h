@@ -201,28 +201,6 @@ __device__ __forceinline__ double kx_dec_v(int w, int ip, bool vflag, const doub
   return ref + ldexp((double)w, -vk[ip - 1]);
 }
 
-// ICR heating of one ion in the mixed push: the kick in binary64 from the
-// binary32 components (WaveHeatingSub at stage A, _8_1_1_KineticUpdateA.f90:1392),
-// for ions inside the grid; returns the Vperp cell of the new |(Vperp1,
-// Vperp2)| for the next binning (off the grid the previous index iq stays,
-// KUA:1108-1127).
-__device__ __noinline__ int kx_icr_push(const StepArgs& a, int t, uint32_t jl, int iq, int qc, double r, double w,
-                                        double vpar) {
-  const DevTube& T = a.tubes[t];
-  const long long j = T.off + jl;
-  if (qc == 0 || qc == -1) return iq;
-  const double v1 = (double)a.vrf[j], v2 = (double)a.vrf2[j];
-  double u[4], v1N, v2N, vpN;
-  kaos_icr_uniforms(a.seed, *a.stepno, t, a.id[j], u);
-  kaos_icr_kick(&T.icrp, r, w, vpar, v1, v2, u, &v1N, &v2N, &vpN);
-  a.vrf[j] = (float)v1N;
-  a.vrf2[j] = (float)v2N;
-  const double* GH = a.vperpGH + T.woff;
-  const double vp = fabs(sqrt((double)(float)v1N * (double)(float)v1N + (double)(float)v2N * (double)(float)v2N));
-  const int iqn = kaos_find_vcell(vp, GH, GH[T.nvperp], T.nvperp);
-  return iqn >= 1 ? iqn : iq;
-}
-
 // Push + bin, mixed form.  Same step semantics as k_push_fp64 (removal at a
 // master step, pre-push binning, stale 2-D velocity index, stage-4 gravity
 // gate, field interpolation at the previous stage-4 q on master steps).
@@ -262,7 +240,6 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
   float qGL0 = 0.0f, qGHlast = 0.0f, qsms = 0.0f, vscale = 0.0f;
   double ip = 0.0, hd = 0.0;
   uint32_t tbox = 0u;
-  int icr = 0;                         // ICR heating in this tube
   for (int pass = 0;; ++pass) {
   const int g = kaos_next_group(a, pass, &s_group);
   if (g < 0) break;
@@ -317,7 +294,6 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
       qkfT = a.qkf + toff; aefT = a.aef + toff; agfT = a.agf + toff;
       vscale = (float)((double)nvpar / (a.vparGH[T.voff + nvpar - 1] - a.vparGH[T.voff + nvpar]));
       tbox = kaos_pack_box(a.vbox[tb]);
-      icr = T.icr;
       cur = tb;
       __syncthreads();
     }
@@ -483,9 +459,7 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
         vflag = true;
         kx_at(vpiT, b4) = __float_as_int((float)(vN - ((vN > s_vGH[nvpar - 1]) ? s_vGH[nvpar - 1] : s_vGH[nvpar])));
       }
-      int iqn = iq;
-      if (icr) iqn = kx_icr_push(a, cur, jl, iq, qc, r, w, vs64);
-      kx_at(cellT, b4) = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iqn << 8) | (uint32_t)(ipn >= 1 ? ipn : ipc) |
+      kx_at(cellT, b4) = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)(ipn >= 1 ? ipn : ipc) |
                   (rflag ? KAOS_RFLAG : 0u) | (vflag ? KAOS_VFLAG : 0u);
     }
   }
