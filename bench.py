@@ -192,6 +192,45 @@ def moment_errors(ma, mb):
     return out
 
 
+def full_physics_run(args):
+    """Heated two-species tube (grids.config5_tubes: H+ and O+ on one L=6 tube,
+    ICR heating, dt = 0.02 s), FP64 and mixed, each timed over args.steps
+    graph-replayed steps after the warm-up; the mixed run's moment errors
+    against the FP64 run per species."""
+    from kaos_amd import grids
+    from kaos_amd.solver import KineticSolver, KAOS_FP64, KAOS_MIXED
+    t0 = time.perf_counter()
+    tubes = grids.config5_tubes(n_h=args.fp_ions // 4, n_o=args.fp_ions - args.fp_ions // 4)
+    n_steps_total = args.warmup + args.steps + 2
+    ions = [grids.initial_ions(t, seed=500 + i) for i, t in enumerate(tubes)]
+    caps = [len(q) + (n_steps_total + 2) * (t.n_inj_lb + t.n_inj_ub) + 1024 for t, (q, _, _) in zip(tubes, ions)]
+    n_ions = sum(len(x[0]) for x in ions)
+    log(f"[bench] full physics: {n_ions} ions (H+ {len(ions[0][0])}, O+ {len(ions[1][0])}), "
+        f"setup {time.perf_counter() - t0:.1f}s")
+    res, masters = {}, {}
+    for form, prec in (("fp64", KAOS_FP64), ("mixed", KAOS_MIXED)):
+        ks = KineticSolver(tubes, precision=prec, ndatfac=1, seed=1, capacity=caps)
+        for i, (q, v, w) in enumerate(ions):
+            ks.upload(i, q, v, w)
+        ks.step(1)
+        ks.run(2, args.warmup)
+        first = 2 + args.warmup
+        t1 = time.perf_counter()
+        pushes = ks.run(first, args.steps)
+        wall = time.perf_counter() - t1
+        masters[form] = [ks.master(i) for i in range(len(tubes))]
+        ks.close()
+        res[form] = {"value": pushes / wall, "ms_per_step": wall / args.steps * 1e3}
+        log(f"[bench] full physics {form}: {args.steps} steps (graphs), wall {wall:.3f}s")
+    errs = [moment_errors(a, b) for a, b in zip(masters["fp64"], masters["mixed"])]
+    return {"workload": "H+ and O+ on one L=6 tube, 1.05-3 RE, 256 cells, 64x64, ICR heating, dt=0.02 s",
+            "ions_start": n_ions, "steps": args.steps, "fp64": res["fp64"], "mixed": res["mixed"],
+            "speedup_mixed_vs_fp64": res["mixed"]["value"] / res["fp64"]["value"],
+            "mixed_max_l2_rel_moment_error": max(e[k]["l2_rel"] for e in errs for k in MOMENTS if k != "EA"),
+            "mixed_l2_rel_field_error_EA": max(e["EA"]["l2_rel"] for e in errs),
+            "counts_abs_diff": sum(e["counts_abs_diff"] for e in errs)}
+
+
 def copy_probe(mib):
     """A plain device copy of `mib` MiB (torch), three times: the reference kernel
     whose FETCH_SIZE / WRITE_SIZE counters calibrate the push kernels' traffic."""
@@ -214,6 +253,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--form", choices=("both", "fp64", "mixed"), default="both")
     ap.add_argument("--tubes", type=int, default=512, help="tubes of the batched run (0 = skip it)")
+    ap.add_argument("--full-physics", action="store_true",
+                    help="add the heated two-species run (H+ and O+, ICR heating) as 'full_physics'")
+    ap.add_argument("--fp-ions", type=int, default=200_000_000, help="ions of the heated run (1/4 H+)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the traffic-calibration copy of MIB MiB and exit")
     args = ap.parse_args()
@@ -274,6 +316,8 @@ def main():
         ks.close()
     if args.tubes > 0:
         out["batched"] = batched_run(args)
+    if args.full_physics:
+        out["full_physics"] = full_physics_run(args)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline()

@@ -19,6 +19,7 @@ import numpy as np
 RE = 6.38e6
 KB = 1.38e-23
 MO = 16.0 * 1.66054e-27      # O+ mass, _ModelInput.f90:186
+MH = 1.66054e-27             # H+ mass (one atomic mass unit, as the O+ mass is written)
 QO = 1.602e-19               # O+ charge, _ModelInput.f90:187
 MDIP = 8.06e15
 EV_K = 1.602e-19 / KB        # kelvin per eV with the reference's constants
@@ -105,7 +106,7 @@ def velocity_grid(nvpar, nvperp, vpar_max=1.0e4, vperp_max=1.0e4):
 
 def dipole_tube(nq=64, p=4.0, r_lb=1.05, r_ub=3.0, ratio=1.05, south=False, nvpar=40, nvperp=40,
                 phi=math.pi / 2.0, h=0.05, T_eV=0.3, Te=1.0e4, n_total=100000, H=5.0e5,
-                ndatfac=1, n_lb_density=1e10, area0=1e6):
+                ndatfac=1, n_lb_density=1e10, area0=1e6, ms=MO, vmax=1.0e4):
     """Tube from r_lb to r_ub (RE) on L-shell p with nq cells whose arc lengths grow
     geometrically by `ratio`; initial density profile ~ exp(-s/H)."""
     u_lb = math.sqrt(1.0 - r_lb / p)
@@ -131,9 +132,9 @@ def dipole_tube(nq=64, p=4.0, r_lb=1.05, r_ub=3.0, ratio=1.05, south=False, nvpa
     d3x = area0 * (B_c[0] / B_c) * ds
     qGL, qGH = q_e[:-1].copy(), q_e[1:].copy()
     dq = np.abs(qGL - qGH)
-    vg = velocity_grid(nvpar, nvperp)
+    vg = velocity_grid(nvpar, nvperp, vmax, vmax)
     Ts = T_eV * EV_K
-    sigma = math.sqrt(KB * Ts / MO)
+    sigma = math.sqrt(KB * Ts / ms)
     # macro-ions per cell ~ exp(-s/H) d3x, normalised to n_total
     wgt = np.exp(-s_cen / H) * d3x
     nc = np.floor(n_total * wgt / wgt.sum()).astype(np.int64)
@@ -146,7 +147,7 @@ def dipole_tube(nq=64, p=4.0, r_lb=1.05, r_ub=3.0, ratio=1.05, south=False, nvpa
     arrays = dict(qGL=qGL, qGH=qGH, qGC=q_c, hqC=hq, dqC=dq, d3xC=d3x, rGC=r_c, cthGC=u_c,
                   ellGC=ell_c, Te=np.full(nq, Te))
     arrays.update(vg)
-    t = Tube(nq=nq, nvpar=nvpar, nvperp=nvperp, p=p, phi=phi, h=h, ms=MO, qs=QO,
+    t = Tube(nq=nq, nvpar=nvpar, nvperp=nvperp, p=p, phi=phi, h=h, ms=ms, qs=QO,
              nsnormfac=nsnormfac, Ts_lb=Ts, Ts_ub=Ts, n_inj_lb=n_inj_lb, n_inj_ub=n_inj_ub,
              arrays={k2: np.ascontiguousarray(v, dtype=np.float64) for k2, v in arrays.items()},
              s_centre=s_cen, ds=ds)
@@ -199,4 +200,19 @@ def config4_tubes(ntubes=512, seed=4, scale=1.0):
         T = rng.uniform(0.1, 1.0)
         tubes.append(dipole_tube(nq=nq, p=L, r_ub=min(3.0, 0.8 * L), nvpar=64, nvperp=64,
                                  n_total=max(n, 2 * nq), T_eV=T, ratio=1.05 ** (64.0 / nq)))
+    return tubes
+
+
+def config5_tubes(n_h=50_000_000, n_o=150_000_000):
+    """Heated two-species case: one L=6 tube (1.05-3 RE, 256 cells, 64x64), H+
+    (n_h ions, 0.5 eV, V grid +-40 km/s) and O+ (n_o ions, 0.3 eV, +-10 km/s) as two
+    tubes of one context, dt = 0.02 s, stochastic ICR heating with the reference's
+    parameters except spectral indices 2 (diffusion coefficient ~ B^-2)."""
+    tubes = []
+    for ms, n, T, vmax in ((MH, n_h, 0.5, 4.0e4), (MO, n_o, 0.3, 1.0e4)):
+        t = dipole_tube(nq=256, p=6.0, r_ub=3.0, nvpar=64, nvperp=64, n_total=n, T_eV=T, h=0.02,
+                        ratio=1.05 ** (64.0 / 256.0), ms=ms, vmax=vmax)
+        t.icr = 1
+        t.icr_chi_perp1 = t.icr_chi_perp2 = 2.0
+        tubes.append(t)
     return tubes
