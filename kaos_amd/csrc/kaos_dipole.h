@@ -498,6 +498,14 @@ KAOS_HD double kaos_sin_poly(double x) {
   p = p * s + 1.0 / 6.0;
   return x - (x * s) * p;
 }
+/* sin(2 pi u) for u in [0, 1], with the same exact octant reductions. */
+KAOS_HD double kaos_sin2pi(double u) {
+  double t = u, sg = 1.0;
+  if (t > 0.5) { t = t - 0.5; sg = -1.0; }     /* sin(x + pi) = -sin x   */
+  if (t > 0.25) t = 0.5 - t;                    /* sin(pi - x) = sin x    */
+  double r = (t <= 0.125) ? kaos_sin_poly(KAOS_TWOPI * t) : kaos_cos_poly(KAOS_TWOPI * (0.25 - t));
+  return sg * r;
+}
 KAOS_HD double kaos_cos2pi(double u) {
   double t = u;
   if (t > 0.5) t = 1.0 - t;                     /* cos symmetric about 1/2 */
@@ -582,9 +590,10 @@ KAOS_HD void kaos_philox4x32(const uint32_t ctr_in[4], const uint32_t key_in[2],
   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-/* The four uniforms in (0, 1] of one ion's two Gaussian kicks at step n
- * (GaussianRNGSub twice, _8_1_1_0_WaveHeating.f90:56-64): Philox4x32-10 keyed
- * by the seed, counter (ion id, step, tube, "ICR").                          */
+/* The uniforms in (0, 1] of one ion's two Gaussian kicks at step n
+ * (GaussianRNGSub twice in the reference, _8_1_1_0_WaveHeating.f90:56-64;
+ * here one Box-Muller pair from u[0], u[1]): Philox4x32-10 keyed by the seed,
+ * counter (ion id, step, tube, "ICR").                                      */
 KAOS_HD void kaos_icr_uniforms(uint64_t seed, int64_t step, int tube, int64_t id, double u[4]) {
   uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) ^ 0x49435248u };
   uint32_t ctr[4] = { (uint32_t)id, (uint32_t)step, (uint32_t)tube, 0x49435230u };
@@ -606,6 +615,18 @@ typedef struct {
   double sum_ds;         /* sum of dsICRGT = hqC dqC over the cells      */
 } kaos_icr_t;
 
+/* x^(-chi) for x > 0: exactly 1/(x ... x) for a whole chi up to 8, else
+ * exp(-chi ln x). */
+KAOS_HD double kaos_pow_neg(double x, double chi) {
+  if (chi == rint(chi) && fabs(chi) <= 8.0) {
+    int k = (int)fabs(chi);
+    double p = 1.0;
+    for (int i = 0; i < k; ++i) p = p * x;
+    return (chi >= 0.0) ? 1.0 / p : p;
+  }
+  return kaos_exp(-chi * kaos_log(x));
+}
+
 /* Stochastic ICR kick of one ion (ICRCOHERENCEflag = 0) at the stage-A point
  * of line geometry L (r, w) with input parallel velocity vpar and
  * perpendicular components (v1, v2); u: kaos_icr_uniforms.  WaveHeating:
@@ -615,7 +636,8 @@ typedef struct {
  * ell = 1 + 3 cos^2 theta = 4 - 3 w. */
 KAOS_HD void kaos_icr_kick(const kaos_icr_t* P, double r, double w, double vpar, double v1, double v2,
                            const double u[4], double* v1N, double* v2N, double* vpN) {
-  const double g1 = kaos_gauss(u[0], u[1]), g2 = kaos_gauss(u[2], u[3]);
+  const double R = sqrt(-2.0 * kaos_log(u[0]));
+  const double g1 = R * kaos_cos2pi(u[1]), g2 = R * kaos_sin2pi(u[1]);
   const double B = (KAOS_MDIP * sqrt(4.0 - 3.0 * w)) / ((r * r) * r);
   const double og = (P->qs * B) / P->ms;
   const double vperp = sqrt(v1 * v1 + v2 * v2);
@@ -623,10 +645,10 @@ KAOS_HD void kaos_icr_kick(const kaos_icr_t* P, double r, double w, double vpar,
   if (P->lambda_perp != 0.0 && ((KAOS_TWOPI * vperp) / og) >= P->lambda_perp)
     sig = (KAOS_TWOPI * vperp) / (P->lambda_perp * og);
   const double isig3 = 1.0 / ((sig * sig) * sig);
-  const double lr = kaos_log(og / P->omega_g0);
+  const double x = og / P->omega_g0;
   const double c = (P->qs * P->qs) / (2.0 * (P->ms * P->ms));
-  const double D1 = (((c * (P->xi1 * P->eta_lh)) * P->s0) * kaos_exp(-P->chi1 * lr)) * isig3;
-  const double D2 = (((c * (P->xi2 * P->eta_lh)) * P->s0) * kaos_exp(-P->chi2 * lr)) * isig3;
+  const double D1 = (((c * (P->xi1 * P->eta_lh)) * P->s0) * kaos_pow_neg(x, P->chi1)) * isig3;
+  const double D2 = (((c * (P->xi2 * P->eta_lh)) * P->s0) * kaos_pow_neg(x, P->chi2)) * isig3;
   const double eps = fabs(fabs(vpar) / (og * P->sum_ds));
   const double tau = 1.0 / ((og / KAOS_TWOPI) * fabs(sqrt(eps)));
   double dv1 = 0.0, dv2 = 0.0;

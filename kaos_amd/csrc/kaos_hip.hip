@@ -436,17 +436,17 @@ __global__ void __launch_bounds__(KAOS_FP64_BLOCK, KAOS_FP64_WAVES_EU) k_push_fp
 }
 
 // ---------------------------------------------------------------------------
-// ICR wave heating of a step (WaveHeatingSub, called at stage A of the push,
-// _8_1_1_KineticUpdateA.f90:1392), as a pass before the push over the
-// pre-push state, in tubes with heating on.  Per live ion: the Vperp cell of
+// ICR wave heating of a step in the FP64 form (WaveHeatingSub, called at
+// stage A of the push, _8_1_1_KineticUpdateA.f90:1392), as a pass before the
+// push over the pre-push state, in tubes with heating on (the mixed form
+// heats inside its push, k_push_mixed<..., true>).  Per live ion: the Vperp cell of
 // the cell word becomes the cell of the pre-kick |(Vperp1, Vperp2)| (the
 // push bins with it, Vperpindk1, KUA:1108-1127; off the grid the previous
 // index stays); an ion whose stage-A cell is inside the grid (WaveHeating:
 // 32-33) gets the kick of kaos_icr_kick from the seeded Philox4x32 stream.
 // The stage-A quantities (r, w, cell, V||) are the push's own: the same
-// shared functions on the same state.  The mixed form kicks in binary64 from
-// its binary32 components.
-__global__ void __launch_bounds__(KAOS_BLOCK) k_icr(StepArgs a, int mixed) {
+// shared functions on the same state.
+__global__ void __launch_bounds__(KAOS_BLOCK) k_icr(StepArgs a) {
   const int tile = blockIdx.x;
   if (tile >= a.ntiles) return;
   const int t = a.tile_tube[tile];
@@ -460,36 +460,26 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_icr(StepArgs a, int mixed) {
     const long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
     if (j >= end) break;
     const uint32_t cw = a.cell[j];
-    double rs, vs, v1, v2;
-    if (mixed) {
-      if (cw == KAOS_DEAD) continue;
-      const int Q = kx_refq((int)((cw >> 16) & KAOS_QMASK) - 1, T.nq);
-      rs = a.rref[T.goff + Q - 1] + kx_dec(a.rdi[j], a.rk[T.goff + Q - 1], (cw & KAOS_RFLAG) != 0);
-      vs = kx_dec_v(a.vpi[j], (int)(cw & 0xFFu), (cw & KAOS_VFLAG) != 0, a.vparGH + T.voff, a.vk + T.voff,
-                    T.nvpar);
-      v1 = (double)a.vrf[j]; v2 = (double)a.vrf2[j];
-    } else {
-      rs = a.rs[j];
-      if (rs == 0.0) continue;
-      vs = a.vp[j];
-      v1 = a.vr[j]; v2 = a.vr2[j];
-    }
+    const int qprev = (int)((cw >> 16) & KAOS_QMASK) - 1;
+    const int hint = (qprev >= 1) ? qprev : (qprev == 0 ? 1 : T.nq);
+    const double rs = a.rs[j];
+    if (rs == 0.0) continue;
+    const double vs = a.vp[j];
+    const double v1 = a.vr[j], v2 = a.vr2[j];
     int iq = (int)((cw >> 8) & 0xFFu);
     const int iqn = kaos_find_vcell(fabs(sqrt(v1 * v1 + v2 * v2)), GH, GH[T.nvperp], T.nvperp);
     if (iqn >= 1) iq = iqn;
     a.cell[j] = (cw & ~(0xFFu << 8)) | ((uint32_t)iq << 8);
     const kaos_line_t L = kaos_line_i(rs, irep);
     const kaos_qb_t gA = kaos_stageA(L);
-    const int qprev = (int)((cw >> 16) & KAOS_QMASK) - 1;
-    const int hint = (qprev >= 1) ? qprev : (qprev == 0 ? 1 : T.nq);
     const int qc = kaos_find_qcell_from(gA.q, a.qGH + T.goff, T.qGL0, T.qGHlast, T.nq, T.desc, hint);
     if (qc == 0 || qc == -1) continue;
     const double vpar = gA.bR * (vs * gA.bR) + gA.bZ * (vs * gA.bZ);
     double u[4], v1N, v2N, vpN;
     kaos_icr_uniforms(a.seed, step, t, a.id[j], u);
     kaos_icr_kick(&T.icrp, L.r, L.w, vpar, v1, v2, u, &v1N, &v2N, &vpN);
-    if (mixed) { a.vrf[j] = (float)v1N; a.vrf2[j] = (float)v2N; }
-    else { a.vr[j] = v1N; a.vr2[j] = v2N; }
+    a.vr[j] = v1N;
+    a.vr2[j] = v2N;
   }
 }
 
@@ -1528,11 +1518,20 @@ static void launch_push(kaos_ctx* c, const StepArgs& a, hipEvent_t tune0 = nullp
   if (c->prof_on) ev = ev_pair(c);
   if (tune0) (void)hipEventRecord(tune0, c->stream);
   if (c->cfg.precision == KAOS_MIXED) {
-    if (c->lds_nq <= 256)
-      hipLaunchKernelGGL(k_push_mixed<256>, dim3(nblocks), dim3(KAOS_MIXED_BLOCK), push_lds_bytes(c), c->stream, a);
+    // the heating instantiation only when a tube heats (its extra registers
+    // never cost the plain push)
+    if (c->lds_nq <= 256 && !c->any_icr)
+      hipLaunchKernelGGL((k_push_mixed<256, false>), dim3(nblocks), dim3(KAOS_MIXED_BLOCK), push_lds_bytes(c),
+                         c->stream, a);
+    else if (c->lds_nq <= 256)
+      hipLaunchKernelGGL((k_push_mixed<256, true>), dim3(nblocks), dim3(KAOS_MIXED_BLOCK), push_lds_bytes(c),
+                         c->stream, a);
+    else if (!c->any_icr)
+      hipLaunchKernelGGL((k_push_mixed<KAOS_MAXQ, false>), dim3(nblocks), dim3(KAOS_MIXED_BLOCK), push_lds_bytes(c),
+                         c->stream, a);
     else
-      hipLaunchKernelGGL(k_push_mixed<KAOS_MAXQ>, dim3(nblocks), dim3(KAOS_MIXED_BLOCK), push_lds_bytes(c), c->stream,
-                         a);
+      hipLaunchKernelGGL((k_push_mixed<KAOS_MAXQ, true>), dim3(nblocks), dim3(KAOS_MIXED_BLOCK), push_lds_bytes(c),
+                         c->stream, a);
   }
   else if (c->lds_nq <= 256)
     hipLaunchKernelGGL(k_push_fp64<256>, dim3(nblocks), dim3(KAOS_FP64_BLOCK), push_lds_bytes(c), c->stream, a);
@@ -1635,9 +1634,8 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
                        c->d_M, c->d_M0F, c->d_EA, c->d_esc_acc, c->d_esc_rep, 0);
     launch_moments(c, nullptr);
     launch_box_update(c);
-    if (c->any_icr)
-      hipLaunchKernelGGL(k_icr, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a,
-                         (int)(c->cfg.precision == KAOS_MIXED));
+    if (c->any_icr && c->cfg.precision != KAOS_MIXED)
+      hipLaunchKernelGGL(k_icr, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a);
     launch_push(c, a);
   } else {
     if (master) {
@@ -1651,9 +1649,8 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
       if (rc) return rc;
     }
     StepArgs a = make_args(c, master, master, master, premaster);
-    if (c->any_icr)
-      hipLaunchKernelGGL(k_icr, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a,
-                         (int)(c->cfg.precision == KAOS_MIXED));
+    if (c->any_icr && c->cfg.precision != KAOS_MIXED)
+      hipLaunchKernelGGL(k_icr, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a);
     launch_push(c, a, e0, e1);
     if (master) {
       if (c->inj_total > 0) {

@@ -201,6 +201,42 @@ __device__ __forceinline__ double kx_dec_v(int w, int ip, bool vflag, const doub
   return ref + ldexp((double)w, -vk[ip - 1]);
 }
 
+// Stochastic ICR kick of one ion in binary32 (the mixed form's heating; the
+// arithmetic of kaos_icr_kick with hardware log2/exp2/sqrt/rcp and the
+// sine/cosine of revolutions).  u0, u1 in (0, 1]: one Box-Muller pair.
+// Products are ordered so that no intermediate leaves the binary32 range
+// (qs/ms first: qs^2 and ms^2 alone underflow).
+__device__ __forceinline__ void kf_icr_kick(const kaos_icr_t& P, float r, float w, float vpar, float v1, float v2,
+                                            float u0, float u1, float& v1N, float& v2N) {
+  KAOS_MIXED_CONTRACT
+  const float R = kf_sqrt(-2.0f * 0.69314718f * __builtin_amdgcn_logf(u0));
+  const float g1 = R * __builtin_amdgcn_cosf(u1), g2 = R * __builtin_amdgcn_sinf(u1);
+  const float qm = (float)(P.qs / P.ms);
+  const float B = (float)KAOS_MDIP * (kf_sqrt(4.0f - 3.0f * w) * kf_rcp((r * r) * r));
+  const float og = qm * B;
+  const float vperp = kf_sqrt(v1 * v1 + v2 * v2);
+  float isig3 = 1.0f;
+  if (P.lambda_perp != 0.0 && ((float)KAOS_TWOPI * vperp) * kf_rcp(og) >= (float)P.lambda_perp) {
+    const float sig = ((float)KAOS_TWOPI * vperp) * kf_rcp((float)P.lambda_perp * og);
+    isig3 = kf_rcp((sig * sig) * sig);
+  }
+  const float lx = __builtin_amdgcn_logf(og * (float)(1.0 / P.omega_g0));
+  const float c = 0.5f * (qm * qm) * (float)(P.eta_lh * P.s0);
+  const float D1 = (c * (float)P.xi1) * __builtin_amdgcn_exp2f(-(float)P.chi1 * lx) * isig3;
+  const float D2 = (c * (float)P.xi2) * __builtin_amdgcn_exp2f(-(float)P.chi2 * lx) * isig3;
+  const float eps = fabsf(vpar) * kf_rcp(og * (float)P.sum_ds);
+  const float tau = (float)KAOS_TWOPI * kf_rcp(og * kf_sqrt(eps));
+  const float h = (float)P.h;
+  float dv1 = 0.0f, dv2 = 0.0f;
+  if (h <= tau) {
+    const float f = h * kf_rcp(tau);
+    dv1 = (f * kf_sqrt((2.0f * D1) * tau)) * g1;
+    dv2 = (f * kf_sqrt((2.0f * D2) * tau)) * g2;
+  }
+  v1N = v1 + dv1;
+  v2N = v2 + dv2;
+}
+
 // Push + bin, mixed form.  Same step semantics as k_push_fp64 (removal at a
 // master step, pre-push binning, stale 2-D velocity index, stage-4 gravity
 // gate, field interpolation at the previous stage-4 q on master steps).
@@ -213,7 +249,7 @@ __device__ __forceinline__ double kx_dec_v(int w, int ip, bool vflag, const doub
 #ifndef KAOS_MIXED_WAVES_EU
 #define KAOS_MIXED_WAVES_EU 5
 #endif
-template <int MAXQ>
+template <int MAXQ, bool ICR>
 __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_mixed(StepArgs a) {
   KAOS_MIXED_CONTRACT
   constexpr int BLK = KAOS_MIXED_BLOCK, PPT = KAOS_TILE / KAOS_MIXED_BLOCK;
@@ -240,6 +276,7 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
   float qGL0 = 0.0f, qGHlast = 0.0f, qsms = 0.0f, vscale = 0.0f;
   double ip = 0.0, hd = 0.0;
   uint32_t tbox = 0u;
+  int icr = 0;                         // ICR heating in this tube (ICR instantiation only)
   for (int pass = 0;; ++pass) {
   const int g = kaos_next_group(a, pass, &s_group);
   if (g < 0) break;
@@ -294,6 +331,7 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
       qkfT = a.qkf + toff; aefT = a.aef + toff; agfT = a.agf + toff;
       vscale = (float)((double)nvpar / (a.vparGH[T.voff + nvpar - 1] - a.vparGH[T.voff + nvpar]));
       tbox = kaos_pack_box(a.vbox[tb]);
+      if (ICR) icr = T.icr;
       cur = tb;
       __syncthreads();
     }
@@ -459,7 +497,29 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
         vflag = true;
         kx_at(vpiT, b4) = __float_as_int((float)(vN - ((vN > s_vGH[nvpar - 1]) ? s_vGH[nvpar - 1] : s_vGH[nvpar])));
       }
-      kx_at(cellT, b4) = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)(ipn >= 1 ? ipn : ipc) |
+      // ICR heating (WaveHeatingSub at stage A, _8_1_1_KineticUpdateA.f90:1392) of
+      // an ion inside the grid: binary32 kick (kf_icr_kick) from the Philox4x32
+      // stream; the cell word then carries the Vperp cell of the kicked speed,
+      // the next push's binning cell (off the grid the previous one stays)
+      int iqw = iq;
+      if (ICR && icr && qc != 0 && qc != -1) {
+        const DevTube& TI = a.tubes[cur];
+        const float v1 = kx_at(a.vrf + toff, b4), v2 = kx_at(a.vrf2 + toff, b4);
+        uint32_t key[2] = { (uint32_t)a.seed, (uint32_t)(a.seed >> 32) ^ 0x49435248u };
+        uint32_t ctr[4] = { (uint32_t)kx_at(a.id + toff, b4), (uint32_t)(*a.stepno), (uint32_t)cur, 0x49435230u };
+        uint32_t o4[4];
+        kaos_philox4x32(ctr, key, o4);
+        const float u0 = (float)(((double)o4[0] + 1.0) * (1.0 / 4294967296.0));
+        const float u1 = (float)(((double)o4[1] + 1.0) * (1.0 / 4294967296.0));
+        float v1N, v2N;
+        kf_icr_kick(TI.icrp, (float)r, (float)w, vs, v1, v2, u0, u1, v1N, v2N);
+        kx_at(a.vrf + toff, b4) = v1N;
+        kx_at(a.vrf2 + toff, b4) = v2N;
+        const double* GH = a.vperpGH + TI.woff;
+        const int iqn = kaos_find_vcell((double)kf_sqrt(v1N * v1N + v2N * v2N), GH, GH[nvperp], nvperp);
+        if (iqn >= 1) iqw = iqn;
+      }
+      kx_at(cellT, b4) = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iqw << 8) | (uint32_t)(ipn >= 1 ? ipn : ipc) |
                   (rflag ? KAOS_RFLAG : 0u) | (vflag ? KAOS_VFLAG : 0u);
     }
   }

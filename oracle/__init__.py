@@ -96,6 +96,10 @@ def load():
         lib.orc_inject_uniforms.argtypes = [C.c_uint64, C.c_int64, C.c_int, C.c_int64, _dp]
         lib.orc_exp.argtypes = [C.c_double]
         lib.orc_exp.restype = C.c_double
+        lib.orc_sin2pi.argtypes = [C.c_double]
+        lib.orc_sin2pi.restype = C.c_double
+        lib.orc_pow_neg.argtypes = [C.c_double, C.c_double]
+        lib.orc_pow_neg.restype = C.c_double
         u32p = C.POINTER(C.c_uint32)
         lib.orc_philox4x32.argtypes = [u32p, u32p, u32p]
         lib.orc_icr_uniforms.argtypes = [C.c_uint64, C.c_int64, C.c_int, C.c_int64, _dp]

@@ -602,6 +602,8 @@ double orc_quartic_newton(double A, double B, double ip, double g0) { return kao
 double orc_sinh_asinh_third(double y) { return kaos_sinh_asinh_third(y); }
 double orc_log(double x) { return kaos_log(x); }
 double orc_exp(double x) { return kaos_exp(x); }
+double orc_sin2pi(double u) { return kaos_sin2pi(u); }
+double orc_pow_neg(double x, double chi) { return kaos_pow_neg(x, chi); }
 void orc_philox4x32(const uint32_t *ctr, const uint32_t *key, uint32_t *out) { kaos_philox4x32(ctr, key, out); }
 void orc_icr_uniforms(uint64_t seed, int64_t step, int tube, int64_t id, double *u) {
   kaos_icr_uniforms(seed, step, tube, id, u);

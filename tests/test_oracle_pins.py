@@ -526,6 +526,15 @@ def test_exp_vs_libm(lib):
         assert abs(lib.orc_exp(float(x)) - e) <= 4e-16 * e
 
 
+def test_sin2pi_and_negative_powers_vs_libm(lib):
+    for u in np.linspace(0.0, 1.0, 4097):
+        assert abs(lib.orc_sin2pi(float(u)) - math.sin(2.0 * math.pi * u)) <= 2e-15
+    for x in (0.01, 0.37, 1.0, 2.5, 7.9, 125.0):
+        for chi in (2.0, 3.0, -1.0, 0.0, 2.1, 1.5, 11.0):
+            want = x ** (-chi)
+            assert abs(lib.orc_pow_neg(x, chi) - want) <= 8e-15 * want, (x, chi)
+
+
 def _icr_tube(**kw):
     t = grids.config1(n_total=2000)
     t.icr = 1
@@ -565,22 +574,20 @@ def _ref_kick_unit(t, r, w, vpar, vperp):
 
 @pytest.mark.parametrize("chi2,lam", [(2.1, 0.0), (2.0, 0.0), (2.1, 3.0e3)])
 def test_icr_kick_closed_form_unit_gaussians(lib, chi2, lam):
-    # u = (exp(-1/2), 1, exp(-1/2), 1): both Gaussians are exactly sqrt(1) cos(2 pi) = 1
+    # one Box-Muller pair: radius sqrt(-2 ln u0) = 1 for u0 = exp(-1/2); angle
+    # 2 pi u1 = 2 pi gives the unit Gaussians (1, 0), 2 pi u1 = pi/2 gives (0, 1)
     t = _icr_tube(icr_chi_perp2=chi2, icr_lambda_perp=lam)
     cfg = oracle.cfg_of(t, 1, 1, 0)
-    u = [math.exp(-0.5), 1.0, math.exp(-0.5), 1.0]
     for r_re, vpar, v1, v2 in ((1.2, 900.0, 300.0, -200.0), (2.5, -1500.0, 50.0, 4000.0), (1.05, 20.0, 0.0, 0.0)):
         r = r_re * RE
         w = r / (t.p * RE)
         vperp = math.hypot(v1, v2)
         k1, k2, tau = _ref_kick_unit(t, r, w, vpar, vperp)
-        o = _kick(lib, cfg, r, w, vpar, v1, v2, u)
-        assert abs(o[0] - (v1 + k1)) <= 1e-12 * max(abs(v1 + k1), 1.0)
-        assert abs(o[1] - (v2 + k2)) <= 1e-12 * max(abs(v2 + k2), 1.0)
-        assert abs(o[2] - math.hypot(o[0], o[1])) <= 1e-15 * o[2]
-    # u2 = 1/4: cos(pi/2) = 0, so the first component is not kicked at all
-    o = _kick(lib, cfg, 1.5 * RE, 1.5 / t.p, 800.0, 123.0, 45.0, [0.3, 0.25, 0.3, 0.25])
-    assert o[0] == 123.0 and o[1] == 45.0
+        o = _kick(lib, cfg, r, w, vpar, v1, v2, [math.exp(-0.5), 1.0, 0.5, 0.5])
+        assert abs(o[0] - (v1 + k1)) <= 1e-12 * max(abs(v1 + k1), 1.0) and o[1] == v2
+        assert abs(o[2] - math.hypot(o[0], o[1])) <= 1e-15 * max(o[2], 1e-300)
+        o = _kick(lib, cfg, r, w, vpar, v1, v2, [math.exp(-0.5), 0.25, 0.5, 0.5])
+        assert o[0] == v1 and abs(o[1] - (v2 + k2)) <= 1e-12 * max(abs(v2 + k2), 1.0)
 
 
 def test_icr_no_kick_when_step_exceeds_interaction_time(lib):
