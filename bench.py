@@ -46,7 +46,7 @@ BYTES_PER_ION = {"fp64": 28 + 44, "mixed": 16 + 24}
 # ratio of a plain 1 GiB device copy's true bytes to its counter reading
 # (fetch x2.000, write x1.000), from
 # profiles/k_push_1e8_ions_256x64x64_hbm_traffic.txt.
-TRAFFIC_BYTES_PER_ION = {"fp64": 77.4, "mixed": 41.0}
+TRAFFIC_BYTES_PER_ION = {"fp64": 73.2, "mixed": 41.2}
 HBM_PEAK_GBS = 8000.0
 MOMENTS = ("M0", "M1perp", "M1par", "M2", "M2perp", "M2par", "EA")
 
