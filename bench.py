@@ -108,7 +108,9 @@ def timed_run(form, tube, q0, v0, w0, cap, args):
     from kaos_amd.solver import KineticSolver, KAOS_FP64, KAOS_MIXED
     prec = KAOS_MIXED if form == "mixed" else KAOS_FP64
     ks = KineticSolver([tube], precision=prec, ndatfac=1, seed=1, capacity=[cap])
+    tu = time.perf_counter()
     ks.upload(0, q0, v0, w0)
+    upload_s = time.perf_counter() - tu
     ks.step(1)
     if args.warmup:
         ks.run(2, args.warmup)
@@ -126,6 +128,7 @@ def timed_run(form, tube, q0, v0, w0, cap, args):
     log(f"[bench] {form}: {args.steps} steps, wall {wall:.3f}s, run events {run_ms/1e3:.3f}s, "
         f"push kernel {push_ms/max(launches,1):.3f} ms x {launches}")
     res = dict(value=pushes / wall, ms_per_step=wall / args.steps * 1e3, launch=ks.get_launch()[:3],
+               upload_s=upload_s,
                roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS,
                          "traffic": TRAFFIC_BYTES_PER_ION[form] * ions_per_launch,
@@ -133,6 +136,40 @@ def timed_run(form, tube, q0, v0, w0, cap, args):
                          "kernel": "k_push_mixed" if form == "mixed" else "k_push_fp64",
                          "bytes_per_ion": BYTES_PER_ION[form], "mean_launch_ms": mean_push_s * 1e3})
     return ks, res
+
+
+def seed_run(args, host_setup_s):
+    """Initial ions generated on the device (kaos_seed_particles) for the
+    single-tube workload: the tube's per-cell counts at cell centres, the
+    tube's ion temperature in every cell; wall time of the call (kernel, two
+    small host-to-device copies, one synchronisation), after a small warm-up
+    call that loads the kernel and an empty one that allocates the context's
+    device buffers.  Checked: the tube holds exactly the
+    requested ions."""
+    from kaos_amd import grids
+    from kaos_amd.solver import KineticSolver, KAOS_FP64
+    tube = grids.config2(n_total=args.ions)
+    counts = tube.cell_counts
+    temp = np.full(tube.nq, tube.Ts_lb)
+    cap = int(counts.sum()) + 4 * (tube.n_inj_lb + tube.n_inj_ub) + 1024
+    warm = np.zeros(tube.nq, np.int64)
+    warm[:8] = 1000
+    ks = KineticSolver([tube], precision=KAOS_FP64, ndatfac=1, seed=1, capacity=[20000])
+    ks.seed(0, warm, temp, 11)
+    ks.close()
+    ks = KineticSolver([tube], precision=KAOS_FP64, ndatfac=1, seed=1, capacity=[cap])
+    ks.seed(0, np.zeros(tube.nq, np.int64), temp, 11)      # device buffers of the context
+    t1 = time.perf_counter()
+    ks.seed(0, counts, temp, 11)
+    dt = time.perf_counter() - t1
+    n = ks.count(0)
+    assert n == int(counts.sum()), (n, int(counts.sum()))
+    ks.close()
+    log(f"[bench] device initial ions: {n} in {dt*1e3:.1f} ms")
+    return {"ions": n, "seconds": dt, "ions_per_s": n / dt, "kernel": "k_seed",
+            "host_numpy_plus_upload_s": host_setup_s,
+            "note": "wall time of kaos_seed_particles (FP64 state); host_numpy_plus_upload_s is the "
+                    "host generation of the benchmark's ions plus kaos_upload_particles"}
 
 
 def batched_run(args):
@@ -274,6 +311,7 @@ def main():
     n_steps_total = args.warmup + args.steps + 1
     cap = len(q0) + (n_steps_total + 2) * (tube.n_inj_lb + tube.n_inj_ub) + 1024
     log(f"[bench] ions {len(q0)}, inj/step {tube.n_inj_lb}+{tube.n_inj_ub}, setup {time.perf_counter()-t0:.1f}s")
+    host_gen_s = time.perf_counter() - t0
     forms = ("fp64", "mixed") if args.form == "both" else (args.form,)
     runs = {}
     for f in forms:
@@ -314,6 +352,7 @@ def main():
                         "l2_rel_field_error_EA": err["EA"]["l2_rel"]}
     for ks, _ in runs.values():
         ks.close()
+    out["initial_conditions"] = seed_run(args, host_gen_s + runs[forms[0]][1]["upload_s"])
     if args.tubes > 0:
         out["batched"] = batched_run(args)
     if args.full_physics:

@@ -135,6 +135,28 @@ int kaos_upload_particles(kaos_ctx* ctx, int32_t tube, int64_t n, const double* 
  * vperp0 and Vperp2 = 0 (only the magnitude enters unless ICR heating is on). */
 int kaos_upload_vperp(kaos_ctx* ctx, int32_t tube, int64_t n, const double* vperp1, const double* vperp2);
 
+/* Macro-ions per config cell of a tube: counts[Q] = nint((density[Q] /
+ * nsnormfac) d3x[Q]), halves away from zero; density in m^-3, d3x the cell
+ * volumes.  Replaces nsnormC0 (_0_1_1_ConfigGridGeneratorB.f90:83-85) and
+ * NsFARRp (_2_1_DensityProfileA1.f90:45) for one rank.  Host arithmetic
+ * only: no context, no device.  KAOS_E_ARG for nq < 1, nsnormfac <= 0 or a
+ * count that is not finite or does not fit in int64. */
+int kaos_cell_counts(int32_t nq, const double* density, const double* d3x, double nsnormfac, int64_t* counts);
+
+/* Initial ions of a tube generated on the device, in place of
+ * kaos_upload_particles + kaos_upload_vperp: counts[Q] ions at the centre
+ * qGC of config cell Q (DensityProfileBSub, _4_DensityProfileB.f90:40-46),
+ * signed radius from the root solve (DipolePolynomialSolverSub), and a
+ * Maxwell-Boltzmann velocity of temperature temperature[Q] (K) projected on
+ * the field line at that point (VelocityDistributionSub,
+ * _6_VelocityDistribution.f90:60-85; MBVelocityDistributionSub).  Ions are
+ * numbered cell by cell (InitialConditionsSub, _7_InitialConditions.f90:72-87);
+ * ion i draws its six uniforms from Philox4x64-10 keyed by seed (counter i,
+ * tube).  Replaces the reference's GaussianRNGSub draws, which come from
+ * Fortran's random_number and cannot be reproduced. */
+int kaos_seed_particles(kaos_ctx* ctx, int32_t tube, const int64_t* counts, const double* temperature,
+                        uint64_t seed);
+
 /* Advance every tube by time step n (n = 1 first).  One call replaces one
  * pass of ComputationalTimeStepLoop (_8_KineticSolver.f90:54-239):
  * BoundaryConditionsSub, AmbipolarEfieldSub, GravfieldSub, KineticSolverA/BSub

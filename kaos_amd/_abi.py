@@ -49,7 +49,8 @@ class KaosStepOut(C.Structure):
                 ("status", C.c_int32)]
 
 
-EXPORTS = ("kaos_init", "kaos_set_grid", "kaos_upload_particles", "kaos_upload_vperp", "kaos_step", "kaos_run",
+EXPORTS = ("kaos_init", "kaos_set_grid", "kaos_upload_particles", "kaos_upload_vperp", "kaos_cell_counts",
+           "kaos_seed_particles", "kaos_step", "kaos_run",
            "kaos_download_master", "kaos_count", "kaos_download_particles", "kaos_set_launch",
            "kaos_get_launch", "kaos_profile", "kaos_profile_read", "kaos_last_error", "kaos_finalize")
 
@@ -71,6 +72,8 @@ def load(path=LIB_PATH):
     lib.kaos_set_grid.argtypes = [P, C.c_int32, C.POINTER(KaosGrid), C.c_int64]
     lib.kaos_upload_particles.argtypes = [P, C.c_int32, C.c_int64, _dp, _dp, _dp]
     lib.kaos_upload_vperp.argtypes = [P, C.c_int32, C.c_int64, _dp, _dp]
+    lib.kaos_cell_counts.argtypes = [C.c_int32, _dp, _dp, C.c_double, i64p]
+    lib.kaos_seed_particles.argtypes = [P, C.c_int32, i64p, _dp, C.c_uint64]
     lib.kaos_step.argtypes = [P, C.c_int64, _dp, C.c_int64, C.POINTER(KaosStepOut)]
     lib.kaos_run.argtypes = [P, C.c_int64, C.c_int64, i64p]
     lib.kaos_download_master.argtypes = [P, C.c_int32, i64p, i64p, _dp, _dp, _dp, _dp, _dp]

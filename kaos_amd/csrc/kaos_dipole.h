@@ -557,6 +557,59 @@ KAOS_HD void kaos_mb_project(const double u[6], double sigma, double cphi, doubl
 }
 
 /* ------------------------------------------------------------------------- */
+/* Initial ions (DensityProfileA1Sub, DensityProfileBSub, DipolePolynomial-  */
+/* SolverSub, VelocityDistributionSub, InitialConditionsSub).                */
+
+/* Macro-ions of one config cell: nint((n / nsnormfac) d3x), the reference's
+ * nsnormC0 (_0_1_1_ConfigGridGeneratorB.f90:83-85) rounded as NsFARRp
+ * (_2_1_DensityProfileA1.f90:45): to the nearest integer, halves away from
+ * zero.  -1 when the value is not finite or does not fit in int64.         */
+KAOS_HD int64_t kaos_cell_count(double density, double d3x, double nsnormfac) {
+  double x = (density / nsnormfac) * d3x;
+  if (!(x > -9.2e18 && x < 9.2e18)) return -1;
+  double t = trunc(x);
+  double f = x - t;                       /* exact: |f| < 1, same sign as x */
+  if (f >= 0.5) t = t + 1.0;
+  else if (f <= -0.5) t = t - 1.0;
+  return (int64_t)t;
+}
+
+/* Thermal speed sqrt(kB T / m) of MBVelocityDistributionSub
+ * (_6_1_MBVelocityDistribution.f90:42), in its order of operations.        */
+KAOS_HD double kaos_thermal_speed(double T, double ms) { return sqrt((KAOS_KB * T) / ms); }
+
+/* The seeded uniform stream of initial ion i of a tube (ions numbered cell
+ * by cell, as InitialConditionsSub concatenates them,
+ * _7_InitialConditions.f90:72-87): six uniforms for the Vx, Vy, Vz
+ * Box-Muller pairs of MBVelocityDistributionSub, from two Philox4x64-10
+ * blocks keyed by (seed, "KAOS") with counter (i, 0, tube, "INI" block).
+ * The "INI" words keep this stream apart from the injection stream's "INJ"
+ * blocks.                                                                  */
+KAOS_HD void kaos_seed_uniforms(uint64_t seed, int tube, int64_t i, double u[6]) {
+  uint64_t key[2] = { seed, 0x4B414F53ULL };
+  uint64_t ctr[4] = { (uint64_t)i, 0ULL, (uint64_t)tube, 0x494E4900ULL };
+  uint64_t o[4];
+  kaos_philox4x64(ctr, key, o);
+  u[0] = kaos_u01(o[0]); u[1] = kaos_u01(o[1]); u[2] = kaos_u01(o[2]); u[3] = kaos_u01(o[3]);
+  ctr[3] = 0x494E4901ULL;
+  kaos_philox4x64(ctr, key, o);
+  u[4] = kaos_u01(o[0]); u[5] = kaos_u01(o[1]);
+}
+
+/* One initial ion at dipole coordinate q of L-shell p (DensityProfileBSub
+ * puts it at its cell centre, _4_DensityProfileB.f90:40-46): signed radius
+ * from the root solve (DipolePolynomialSolverSub), Maxwell-Boltzmann
+ * velocity of thermal speed sigma at that point from the uniforms u
+ * (VelocityDistributionSub, _6_VelocityDistribution.f90:60-85).            */
+KAOS_HD void kaos_seed_ion(double q, double p, double sigma, double cphi, double sphi, const double u[6],
+                           double* rs, double* vpar, double* vperp1, double* vperp2, double* vperp) {
+  double r = kaos_rs_of_q(q, p);
+  kaos_line_t L = kaos_line(r, p);
+  *rs = r;
+  kaos_mb_project(u, sigma, cphi, sphi, L, vpar, vperp1, vperp2, vperp);
+}
+
+/* ------------------------------------------------------------------------- */
 /* ICR wave heating (WaveHeatingSub, _8_1_1_0_WaveHeating.f90:32-314).       */
 
 /* exp(x) for |x| < 700: x = k ln2 + r, |r| <= ln2/2 (ln2 in two parts, the

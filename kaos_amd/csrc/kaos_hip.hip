@@ -27,6 +27,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <cmath>
 #include <map>
 
 #include "kaos_dipole.h"
@@ -507,17 +508,14 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_bin_initial(StepArgs a) {
   }
 }
 
-// Upload: positions from the root solve of q0 (InitialConditionsSub); the k0
-// cells (KSA:49-326) from q0 clamped into the grid as KSA:55-129 does, and
-// from the input Vpar0/Vperp0.
-__global__ void k_upload(DevTube T, long long n, const double* q0, const double* vpar0,
-                         const double* vperp0, const double* qGH, const double* vparGH,
-                         const double* vperpGH, const double* rref, StepArgs a, int mixed) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  long long j = T.off + i;
-  double q = q0[i];
-  double rs = kaos_rs_of_q(q, T.p);
+// State of one initial ion in slot j: signed radius rs at dipole coordinate
+// q, velocity (Vpar, Vperp1, Vperp2), |Vperp| = vperp.  The k0 cells
+// (KSA:49-326) come from q clamped into the grid as KSA:55-129 does, and from
+// Vpar and vperp.
+__device__ __forceinline__ void kd_store_initial(const DevTube& T, const StepArgs& a, long long j, int id, double q,
+                                                 double rs, double vpar, double v1, double v2, double vperp,
+                                                 const double* qGH, const double* vparGH, const double* vperpGH,
+                                                 const double* rref, int mixed) {
   if (T.south) {
     if (q <= T.qGL0) q = T.qGL0;
     else if (q > T.qGHlast) q = T.qGHlast;
@@ -526,26 +524,56 @@ __global__ void k_upload(DevTube T, long long n, const double* q0, const double*
     else if (q < T.qGHlast) q = T.qGHlast;
   }
   int qc = kaos_find_qcell(q, qGH, T.qGL0, T.qGHlast, T.nq, T.desc);
-  a.id[j] = (int)i;
-  int ip = kaos_find_vcell(vpar0[i], vparGH, vparGH[T.nvpar], T.nvpar);
-  int iq = kaos_find_vcell(vperp0[i], vperpGH, vperpGH[T.nvperp], T.nvperp);
+  a.id[j] = id;
+  int ip = kaos_find_vcell(vpar, vparGH, vparGH[T.nvpar], T.nvpar);
+  int iq = kaos_find_vcell(vperp, vperpGH, vperpGH[T.nvperp], T.nvperp);
   if (mixed) {
     int Q = kx_refq(qc, T.nq);
     bool rflag, vflag;
     a.rdi[j] = kx_enc(rs - rref[Q - 1], a.rk[T.goff + Q - 1], rflag);
-    a.vpi[j] = kx_enc_v(vpar0[i], ip, vparGH, a.vk + T.voff, T.nvpar, vflag);
-    a.vrf[j] = (float)vperp0[i];
-    a.vrf2[j] = 0.0f;
+    a.vpi[j] = kx_enc_v(vpar, ip, vparGH, a.vk + T.voff, T.nvpar, vflag);
+    a.vrf[j] = (float)v1;
+    a.vrf2[j] = (float)v2;
     a.qkf[j] = (float)q;
     a.aef[j] = 0.0f; a.agf[j] = 0.0f;
     a.cell[j] = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip |
                 (rflag ? KAOS_RFLAG : 0u) | (vflag ? KAOS_VFLAG : 0u);
   } else {
     a.rs[j] = rs;
-    a.vp[j] = vpar0[i]; a.vr[j] = vperp0[i]; a.vr2[j] = 0.0; a.qk[j] = q;
+    a.vp[j] = vpar; a.vr[j] = v1; a.vr2[j] = v2; a.qk[j] = q;
     a.ae[j] = 0.0; a.ag[j] = 0.0;
     a.cell[j] = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip;
   }
+}
+
+// Upload: positions from the root solve of q0 (InitialConditionsSub),
+// velocities Vpar0, Vperp1 = Vperp0, Vperp2 = 0.
+__global__ void k_upload(DevTube T, long long n, const double* q0, const double* vpar0,
+                         const double* vperp0, const double* qGH, const double* vparGH,
+                         const double* vperpGH, const double* rref, StepArgs a, int mixed) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double q = q0[i];
+  kd_store_initial(T, a, T.off + i, (int)i, q, kaos_rs_of_q(q, T.p), vpar0[i], vperp0[i], 0.0, vperp0[i], qGH,
+                   vparGH, vperpGH, rref, mixed);
+}
+
+// Initial ions generated on the device: ion i of the tube (cell by cell,
+// coff = exclusive prefix of the per-cell counts, nq + 1 entries) sits at the
+// centre of its cell (DensityProfileBSub) with a Maxwell-Boltzmann velocity of
+// the cell's temperature (VelocityDistributionSub), from the seeded stream.
+__global__ void k_seed(DevTube T, long long n, const long long* coff, const double* temp, unsigned long long seed,
+                       int tube, const double* qGC, const double* qGH, const double* vparGH,
+                       const double* vperpGH, const double* rref, StepArgs a, int mixed) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int lo = 0, hi = T.nq - 1;                 // last cell Q with coff[Q] <= i
+  while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (coff[mid] <= i) lo = mid; else hi = mid - 1; }
+  const double q = qGC[lo];
+  double u[6], rs, vpar, v1, v2, vperp;
+  kaos_seed_uniforms(seed, tube, i, u);
+  kaos_seed_ion(q, T.p, kaos_thermal_speed(temp[lo], T.ms), T.cphi, T.sphi, u, &rs, &vpar, &v1, &v2, &vperp);
+  kd_store_initial(T, a, T.off + i, (int)i, q, rs, vpar, v1, v2, vperp, qGH, vparGH, vperpGH, rref, mixed);
 }
 
 // Both perpendicular components of the ions just uploaded; the Vperp cell of
@@ -1460,6 +1488,60 @@ extern "C" int kaos_upload_particles(kaos_ctx* c, int32_t tube, int64_t n, const
   HIPCHK(c, hipMemcpyAsync(c->d_next_id + tube, &cnt, sizeof(long long), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   (void)hipFree(dq0); (void)hipFree(dv0); (void)hipFree(dr0);
+  h.count = n;
+  h.uploaded = true;
+  return KAOS_OK;
+}
+
+extern "C" int kaos_cell_counts(int32_t nq, const double* density, const double* d3x, double nsnormfac,
+                                int64_t* counts) {
+  if (nq < 1 || !density || !d3x || !counts || !(nsnormfac > 0.0)) return KAOS_E_ARG;
+  for (int32_t Q = 0; Q < nq; ++Q) {
+    int64_t k = kaos_cell_count(density[Q], d3x[Q], nsnormfac);
+    if (k < 0) return KAOS_E_ARG;
+    counts[Q] = k;
+  }
+  return KAOS_OK;
+}
+
+extern "C" int kaos_seed_particles(kaos_ctx* c, int32_t tube, const int64_t* counts, const double* temperature,
+                                   uint64_t seed) {
+  if (!c) return KAOS_E_ARG;
+  if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
+  if (c->last_step != 0) return fail(c, KAOS_E_STATE, "seed after the first step");
+  if (!c->built) { int rc = build_device(c); if (rc) return rc; }
+  const DevTube& T = c->dt[tube];
+  if (!counts || !temperature) return fail(c, KAOS_E_ARG, "bad counts or temperature array");
+  std::vector<long long> coff((size_t)T.nq + 1, 0);
+  for (int Q = 0; Q < T.nq; ++Q) {
+    if (counts[Q] < 0) return fail(c, KAOS_E_ARG, "negative cell count");
+    if (counts[Q] > 0 && !(temperature[Q] > 0.0 && std::isfinite(temperature[Q])))
+      return fail(c, KAOS_E_ARG, "temperature of a populated cell must be finite and > 0");
+    coff[Q + 1] = coff[Q] + counts[Q];
+    if (coff[Q + 1] > (long long)INT32_MAX) return fail(c, KAOS_E_CAPACITY, "more ions than ion ids");
+  }
+  const long long n = coff[T.nq];
+  HostTube& h = c->ht[tube];
+  if (n > h.cap) return fail(c, KAOS_E_CAPACITY, "more ions than the tube capacity");
+  if (n) {
+    long long* dcoff = nullptr;
+    double* dtemp = nullptr;
+    HIPCHK(c, hipMalloc((void**)&dcoff, sizeof(long long) * coff.size()));
+    HIPCHK(c, hipMalloc((void**)&dtemp, sizeof(double) * T.nq));
+    HIPCHK(c, hipMemcpy(dcoff, coff.data(), sizeof(long long) * coff.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dtemp, temperature, sizeof(double) * T.nq, hipMemcpyHostToDevice));
+    StepArgs a = make_args(c, 0, 0, 0, 0);
+    hipLaunchKernelGGL(k_seed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, T, n, dcoff, dtemp,
+                       (unsigned long long)seed, (int)tube, c->d_qGC + T.goff, c->d_qGH + T.goff,
+                       c->d_vparGH + T.voff, c->d_vperpGH + T.woff, c->d_rref + T.goff, a,
+                       (int)(c->cfg.precision == KAOS_MIXED));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(dcoff); (void)hipFree(dtemp);
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_count + tube, &n, sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_next_id + tube, &n, sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   h.count = n;
   h.uploaded = true;
   return KAOS_OK;

@@ -43,7 +43,8 @@ module kaos_hip
     integer(c_int32_t) :: master, status
   end type kaos_step_out
 
-  public :: kaos_init, kaos_set_grid, kaos_upload_particles, kaos_upload_vperp, kaos_step, kaos_run, &
+  public :: kaos_init, kaos_set_grid, kaos_upload_particles, kaos_upload_vperp, kaos_cell_counts, &
+            kaos_seed_particles, kaos_step, kaos_run, &
             kaos_download_master, kaos_count, kaos_download_particles, kaos_set_launch, &
             kaos_get_launch, kaos_profile, &
             kaos_profile_read, kaos_last_error, kaos_finalize, kaos_error_message
@@ -79,6 +80,26 @@ module kaos_hip
       integer(c_int64_t), value :: n
       real(c_double), intent(in) :: vperp1(*), vperp2(*)
     end function kaos_upload_vperp
+
+    ! NsFARRp of every cell (DensityProfileA1Sub), host arithmetic only
+    integer(c_int) function kaos_cell_counts(nq, density, d3x, nsnormfac, counts) bind(c, name='kaos_cell_counts')
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      integer(c_int32_t), value :: nq
+      real(c_double), intent(in) :: density(*), d3x(*)
+      real(c_double), value :: nsnormfac
+      integer(c_int64_t), intent(out) :: counts(*)
+    end function kaos_cell_counts
+
+    ! DensityProfileBSub + VelocityDistributionSub + InitialConditionsSub on the device
+    integer(c_int) function kaos_seed_particles(ctx, tube, counts, temperature, seed) &
+        bind(c, name='kaos_seed_particles')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tube
+      integer(c_int64_t), intent(in) :: counts(*)
+      real(c_double), intent(in) :: temperature(*)
+      integer(c_int64_t), value :: seed           ! uint64_t on the C side
+    end function kaos_seed_particles
 
     integer(c_int) function kaos_step(ctx, n, uniforms, n_uniforms, out) bind(c, name='kaos_step')
       import :: c_int, c_ptr, c_int64_t

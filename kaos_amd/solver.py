@@ -79,6 +79,15 @@ class KineticSolver:
         v1, v2 = (np.ascontiguousarray(a, dtype=np.float64) for a in (vperp1, vperp2))
         self._chk(self.lib.kaos_upload_vperp(self.h, tube, len(v1), ptr(v1), ptr(v2)))
 
+    def seed(self, tube, counts, temperature, seed):
+        """Initial ions generated on the device (kaos_seed_particles): counts[Q]
+        ions at the centre of cell Q, Maxwellian of temperature[Q] kelvin."""
+        counts = np.ascontiguousarray(counts, dtype=np.int64)
+        temperature = np.ascontiguousarray(temperature, dtype=np.float64)
+        if len(counts) != self.tubes[tube].nq or len(temperature) != self.tubes[tube].nq:
+            raise ValueError("counts and temperature need one entry per config cell")
+        self._chk(self.lib.kaos_seed_particles(self.h, tube, ptr(counts, C.c_int64), ptr(temperature), seed))
+
     def step(self, n, uniforms=None, wait=True):
         out = KaosStepOut() if wait else None
         u = None if uniforms is None else np.ascontiguousarray(uniforms, dtype=np.float64)
@@ -158,3 +167,19 @@ class KineticSolver:
             self.close()
         except Exception:
             pass
+
+
+def cell_counts(density, d3x, nsnormfac):
+    """Macro-ions per config cell, nint((density / nsnormfac) d3x)
+    (kaos_cell_counts; _0_1_1_ConfigGridGeneratorB.f90:83-85,
+    _2_1_DensityProfileA1.f90:45).  Host arithmetic of the library: no GPU."""
+    lib = _abi.load()
+    density = np.ascontiguousarray(density, dtype=np.float64)
+    d3x = np.ascontiguousarray(d3x, dtype=np.float64)
+    if density.shape != d3x.shape or density.ndim != 1:
+        raise ValueError("density and d3x must be 1-D arrays of one length")
+    out = np.zeros(len(density), np.int64)
+    rc = lib.kaos_cell_counts(len(density), ptr(density), ptr(d3x), float(nsnormfac), ptr(out, C.c_int64))
+    if rc:
+        raise KaosError(rc, "kaos_cell_counts: bad input (nsnormfac <= 0, or a count not finite)")
+    return out

@@ -107,6 +107,12 @@ def load():
                                      C.c_double, _dp, _dp]
         lib.orc_set_vperp.argtypes = [P, _dp, _dp]
         lib.orc_get_vperp12.argtypes = [P, _dp, _dp, C.POINTER(C.c_int64)]
+        i64p = C.POINTER(C.c_int64)
+        lib.orc_cell_count.argtypes = [C.c_double, C.c_double, C.c_double]
+        lib.orc_cell_count.restype = C.c_int64
+        lib.orc_seed_ions.argtypes = [C.POINTER(OrcCfg), i64p, _dp, C.c_uint64, C.c_int] + [_dp] * 6
+        lib.orc_seed_ions.restype = C.c_int64
+        lib.orc_seed_uniforms.argtypes = [C.c_uint64, C.c_int, C.c_int64, _dp]
         _lib = lib
     return _lib
 
@@ -205,3 +211,18 @@ class Oracle:
             self.close()
         except Exception:
             pass
+
+
+def seed_ions(tube, counts, temperature, seed, tube_index=0):
+    """The oracle's initial ions of one tube (orc_seed_ions): dict of q0, rs,
+    vpar, v1, v2, vperp, ions numbered cell by cell."""
+    lib = load()
+    cfg = cfg_of(tube, tube_index=tube_index)
+    counts = np.ascontiguousarray(counts, dtype=np.int64)
+    temperature = np.ascontiguousarray(temperature, dtype=np.float64)
+    n = int(counts.sum())
+    out = {k: np.zeros(n) for k in ("q0", "rs", "vpar", "v1", "v2", "vperp")}
+    got = lib.orc_seed_ions(C.byref(cfg), ptr(counts, C.c_int64), ptr(temperature), seed, tube_index,
+                            *(ptr(out[k]) for k in ("q0", "rs", "vpar", "v1", "v2", "vperp")))
+    assert got == n
+    return out

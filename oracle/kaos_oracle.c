@@ -618,3 +618,35 @@ void orc_philox(const uint64_t *ctr, const uint64_t *key, uint64_t *out) { kaos_
 void orc_inject_uniforms(uint64_t seed, int64_t step, int tube, int64_t k, double *u) {
   kaos_inject_uniforms(seed, step, tube, k, u);
 }
+
+/* ---- initial ions ----------------------------------------------------------- */
+
+/* NsFARRp of one cell: nint(nsnormC0), nsnormC0 = (n/nsnormfac) d3x
+ * (_0_1_1_ConfigGridGeneratorB.f90:83-85, _2_1_DensityProfileA1.f90:45). */
+int64_t orc_cell_count(double density, double d3x, double nsnormfac) {
+  return kaos_cell_count(density, d3x, nsnormfac);
+}
+
+/* The initial ions of a tube as the reference builds them, loop for loop:
+ * for each config cell (DensityProfileBSub, _4_DensityProfileB.f90:31-46),
+ * counts[Q] ions at (pGC, qGC) of the cell; the root solve gives the signed
+ * radius (DipolePolynomialSolverSub); each ion's MB velocity at the cell's
+ * temperature (VelocityDistributionSub, _6_VelocityDistribution.f90:54-85),
+ * its uniforms from the seeded stream of its running index i
+ * (InitialConditionsSub's concatenation order, _7_InitialConditions.f90:72-87).
+ * Returns the number of ions written. */
+int64_t orc_seed_ions(const orc_cfg *c, const int64_t *counts, const double *temp, uint64_t seed, int tube,
+                      double *q0, double *rs, double *vpar, double *v1, double *v2, double *vperp) {
+  int64_t i = 0;
+  for (int Q = 0; Q < c->nq; ++Q) {
+    double sigma = kaos_thermal_speed(temp[Q], c->ms);
+    for (int64_t k = 0; k < counts[Q]; ++k, ++i) {
+      double u[6];
+      kaos_seed_uniforms(seed, tube, i, u);
+      q0[i] = c->qGC[Q];
+      kaos_seed_ion(c->qGC[Q], c->p, sigma, c->cphi, c->sphi, u, &rs[i], &vpar[i], &v1[i], &v2[i], &vperp[i]);
+    }
+  }
+  return i;
+}
+void orc_seed_uniforms(uint64_t seed, int tube, int64_t i, double *u) { kaos_seed_uniforms(seed, tube, i, u); }
