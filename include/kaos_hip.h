@@ -31,6 +31,7 @@ extern "C" {
 #define KAOS_E_DEVICE      3   /* HIP runtime error */
 #define KAOS_E_STATE       4   /* call out of order (e.g. step before upload) */
 #define KAOS_E_NAN         5   /* a NaN was produced in the particle state */
+#define KAOS_E_IO          6   /* a file could not be written */
 
 /* Precision forms. */
 #define KAOS_FP64   0   /* every stage in binary64 */
@@ -67,7 +68,8 @@ typedef struct kaos_grid {
   int32_t filter_pt;     /* M0MAfilterPt (odd) */
   int32_t lbcond;        /* LBCONDITIONflagT */
   int32_t ubcond;        /* UBCONDITIONflagT */
-  int32_t pad0;
+  int32_t filter_pt_m1par; /* M1ParMAfilterPt (odd; 0 = filter_pt): window of the filtered
+                            parallel drift M1ParFiltAvrgRT (_8_7_AmbipolarEfield.f90:64-80) */
   int64_t n_inj_lb;      /* ions injected at NqLB per master step: nint(nsnormCLBGT/ranksize) */
   int64_t n_inj_ub;      /* same at NqUB */
   double p;              /* L-shell pGCGT */
@@ -182,6 +184,35 @@ int kaos_run(kaos_ctx* ctx, int64_t n_first, int64_t nsteps, int64_t* particle_s
  * (_8_0_7_BoundaryConditions.f90:201-208, 278-285). */
 int kaos_download_master(kaos_ctx* ctx, int32_t tube, int64_t* nq_counts, int64_t* nph_counts,
                          double* NqRT, double* M, double* EA, double* EG, double* esc);
+
+/* What the export files hold that the hot path does not compute. */
+typedef struct kaos_export_info {
+  const char* dir;       /* dataexportdir; a '/' is put between it and the names if missing */
+  int32_t species;       /* s of the file names (1-based, SpecieT index) */
+  int32_t flux_tube;     /* f of the file names (1-based, FluxTubeT index) */
+  double ns0;            /* ns0(1), reference density [m^-3]: ns0GTfort.bin */
+  double phi;            /* phiGCGT of the tube's cells [rad]: phiGCGTfort.bin */
+  const double* theta;   /* thetaGCGT(nn, :), nq values [rad]: thetaGCGTfort.bin */
+  const double* Ts;      /* TsGT(nn, :), nq values [K]: TsGTfort.bin */
+} kaos_export_info;
+
+/* The files DataExport2Sub writes for one tube at master step nn
+ * (_8_0_2_DataExport2.f90:25-704), from the last step, which must be step 1
+ * (nn = 1) or a master step n (nn = n/ndatfac + 1; KSB:1090-1097, 1214-1221,
+ * KSA:419-428).  Names "<nn>_<s>_<f>_<Name>fort.bin" and, per config cell Q,
+ * "<nn>_<s>_<f>_<Q>_<Name>fort.bin"; each file is a Fortran stream-unformatted
+ * write: the values in array order, little-endian binary64 (ndatfacGT:
+ * int64), no record markers.  The flag set is that of this path
+ * (FLUIDIONEXPORT, PHASEIONDISTRIB, the moment and filter flags on;
+ * ION2VPERP, QEXCHANGE, SPINUP, EAINERTIAL, EPAR off): TimeT, ndatfacGT,
+ * ns0GT, qGCGT, pGCGT, rGCGT, phiGCGT, thetaGCGT, d3xCGT, TsGT; NphRT per cell
+ * ((Vperp, Vpar) column-major: NphReNormRT nsnormfac/d3x, 0 where the count
+ * is 0); M0phRT, M1PerpphRT, M1ParphRT, M2phRT, M2PerpphRT, M2ParphRT;
+ * M0FiltAvrgRT, M1ParFiltAvrgRT and M1Perp1/M1Perp2/M2ParFiltAvrgRT (zeros:
+ * only charge exchange sets them); EAInertialRT (zeros), EAPressureRT,
+ * EAmagRT, EGmagRT; EPmagRT and PhiParRT (zeros).  nn_out (may be NULL)
+ * receives nn.  KAOS_E_IO if a file cannot be written. */
+int kaos_export_master(kaos_ctx* ctx, int32_t tube, const kaos_export_info* info, int64_t* nn_out);
 
 /* Number of live ions of a tube. */
 int kaos_count(kaos_ctx* ctx, int32_t tube, int64_t* n);

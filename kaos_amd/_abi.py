@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libkaos_hip.so")
 
-KAOS_OK, KAOS_E_ARG, KAOS_E_CAPACITY, KAOS_E_DEVICE, KAOS_E_STATE, KAOS_E_NAN = range(6)
+KAOS_OK, KAOS_E_ARG, KAOS_E_CAPACITY, KAOS_E_DEVICE, KAOS_E_STATE, KAOS_E_NAN, KAOS_E_IO = range(7)
 KAOS_FP64, KAOS_MIXED = 0, 1
 KAOS_FLAG_NO_GRAPH, KAOS_FLAG_NO_AUTOTUNE = 1, 2
 
@@ -35,13 +35,18 @@ ICR_FIELDS = ("icr_lambda_perp", "icr_eta_lh", "icr_xi_perp1", "icr_xi_perp2", "
 
 class KaosGrid(C.Structure):
     _fields_ = ([(n, C.c_int32) for n in ("nq", "nvpar", "nvperp", "grav", "eamb", "eambsign",
-                                          "filter_pt", "lbcond", "ubcond", "pad0")]
+                                          "filter_pt", "lbcond", "ubcond", "filter_pt_m1par")]
                 + [("n_inj_lb", C.c_int64), ("n_inj_ub", C.c_int64)]
                 + [(n, C.c_double) for n in ("p", "cphi", "sphi", "h", "ms", "qs", "nsnormfac",
                                              "Ts_lb", "Ts_ub")]
                 + [(n, _dp) for n in GRID_ARRAYS]
                 + [("icr", C.c_int32), ("pad1", C.c_int32)]
                 + [(n, C.c_double) for n in ICR_FIELDS])
+
+
+class KaosExportInfo(C.Structure):
+    _fields_ = [("dir", C.c_char_p), ("species", C.c_int32), ("flux_tube", C.c_int32),
+                ("ns0", C.c_double), ("phi", C.c_double), ("theta", _dp), ("Ts", _dp)]
 
 
 class KaosStepOut(C.Structure):
@@ -51,7 +56,7 @@ class KaosStepOut(C.Structure):
 
 EXPORTS = ("kaos_init", "kaos_set_grid", "kaos_upload_particles", "kaos_upload_vperp", "kaos_cell_counts",
            "kaos_seed_particles", "kaos_step", "kaos_run",
-           "kaos_download_master", "kaos_count", "kaos_download_particles", "kaos_set_launch",
+           "kaos_download_master", "kaos_export_master", "kaos_count", "kaos_download_particles", "kaos_set_launch",
            "kaos_get_launch", "kaos_profile", "kaos_profile_read", "kaos_last_error", "kaos_finalize")
 
 _lib = None
@@ -77,6 +82,7 @@ def load(path=LIB_PATH):
     lib.kaos_step.argtypes = [P, C.c_int64, _dp, C.c_int64, C.POINTER(KaosStepOut)]
     lib.kaos_run.argtypes = [P, C.c_int64, C.c_int64, i64p]
     lib.kaos_download_master.argtypes = [P, C.c_int32, i64p, i64p, _dp, _dp, _dp, _dp, _dp]
+    lib.kaos_export_master.argtypes = [P, C.c_int32, C.POINTER(KaosExportInfo), i64p]
     lib.kaos_count.argtypes = [P, C.c_int32, i64p]
     lib.kaos_download_particles.argtypes = [P, C.c_int32, C.c_int64, i64p] + [_dp] * 10 + [i32p] * 3
     lib.kaos_set_launch.argtypes = [P, C.c_int32, C.c_int32, C.c_int32]

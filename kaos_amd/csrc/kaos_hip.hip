@@ -55,7 +55,7 @@
 struct DevTube {
   int nq, nvpar, nvperp, desc;
   int grav, eamb, eambsign, filter_pt;
-  int lbcond, ubcond, south, pad;
+  int lbcond, ubcond, south, filter_pt_m1;
   long long off, cap, n_inj_lb, n_inj_ub;
   long long goff, voff, woff, hoff, doff;    // grid / vpar / vperp / histogram / d3v offsets
   double p, cphi, sphi, h, ms, qs, qsms, nsnormfac, sig_lb, sig_ub;
@@ -616,15 +616,17 @@ __global__ void k_hbox_reset(int ncells, const DevTube* tubes, const int* blk_tu
 }
 
 __global__ void k_field(const DevTube* tubes, const double* hq, const double* dq, const double* Te,
-                        const double* Mprev, double* M0F, double* EA,
+                        const double* Mprev, double* M0F, double* M1F, double* EA,
                         unsigned long long* esc_acc, unsigned long long* esc_rep, int rotate_esc) {
   int t = blockIdx.x;
   DevTube T = tubes[t];
   const double* M0 = Mprev + 6 * T.goff;  // M0 block of tube t
   double* nF = M0F + T.goff;
   double* E = EA + T.goff;
-  for (int Q = threadIdx.x; Q < T.nq; Q += blockDim.x)
+  for (int Q = threadIdx.x; Q < T.nq; Q += blockDim.x) {
     nF[Q] = kaos_moment_filter_cell(M0, T.nq, T.filter_pt, Q + 1);
+    M1F[T.goff + Q] = kaos_moment_filter_cell(M0 + 2 * T.nq, T.nq, T.filter_pt_m1, Q + 1);
+  }
   __syncthreads();
   for (int Q = threadIdx.x; Q < T.nq; Q += blockDim.x)
     E[Q] = kaos_ea_cell(Q, T.nq, T.south, T.eamb, T.eambsign, T.qs, hq + T.goff, dq + T.goff,
@@ -1060,7 +1062,7 @@ struct kaos_ctx {
   int *d_cells_wave = nullptr, *d_cells_block = nullptr; int n_cells_wave = 0, n_cells_block = 0;
   double *d_qGH = nullptr, *d_qGC = nullptr, *d_hq = nullptr, *d_dq = nullptr, *d_Te = nullptr,
          *d_d3x = nullptr, *d_EA = nullptr, *d_EG = nullptr, *d_M = nullptr, *d_M0F = nullptr,
-         *d_NqRT = nullptr;
+         *d_M1F = nullptr, *d_NqRT = nullptr;
   double *d_vparGH = nullptr, *d_vperpGH = nullptr, *d_vparGC = nullptr, *d_vperpGC = nullptr,
          *d_dvpar = nullptr, *d_dvperp = nullptr, *d_d3v = nullptr;
   StateBufs st[2];                           // particle state; st[cur] is live, the other the reorder target
@@ -1166,6 +1168,9 @@ extern "C" int kaos_set_grid(kaos_ctx* c, int32_t tube, const kaos_grid* g, int6
     return fail(c, KAOS_E_ARG, "nvpar/nvperp must be in [1, 255]");
   if (g->filter_pt < 3 || (g->filter_pt % 2) == 0 || (g->filter_pt - 1) / 2 + 1 > g->nq / 2)
     return fail(c, KAOS_E_ARG, "filter_pt must be odd, >= 3 and fit the grid");
+  if (g->filter_pt_m1par != 0 && (g->filter_pt_m1par < 3 || (g->filter_pt_m1par % 2) == 0 ||
+                                  (g->filter_pt_m1par - 1) / 2 + 1 > g->nq / 2))
+    return fail(c, KAOS_E_ARG, "filter_pt_m1par must be 0, or odd, >= 3 and fit the grid");
   if (capacity < 0 || capacity >= (1LL << 28))
     return fail(c, KAOS_E_ARG, "capacity must be in [0, 2^28) slots per tube");
   const double* need[] = {g->qGL, g->qGH, g->qGC, g->hqC, g->dqC, g->d3xC, g->rGC, g->cthGC, g->ellGC,
@@ -1224,6 +1229,7 @@ static int build_device(kaos_ctx* c) {
     memset(&T, 0, sizeof(T));
     T.nq = g.nq; T.nvpar = g.nvpar; T.nvperp = g.nvperp; T.desc = g.qGL[0] > 0; T.south = !T.desc;
     T.grav = g.grav; T.eamb = g.eamb; T.eambsign = g.eambsign; T.filter_pt = g.filter_pt;
+    T.filter_pt_m1 = g.filter_pt_m1par ? g.filter_pt_m1par : g.filter_pt;
     T.lbcond = g.lbcond; T.ubcond = g.ubcond;
     T.off = off; T.cap = h.cap; T.n_inj_lb = g.n_inj_lb; T.n_inj_ub = g.n_inj_ub;
     T.goff = goff; T.voff = voff; T.woff = woff; T.hoff = hoff; T.doff = doff;
@@ -1313,6 +1319,7 @@ static int build_device(kaos_ctx* c) {
   std::vector<double> zeros(6 * goff, 0.0);
   HIPCHK(c, up((void**)&c->d_EA, zeros.data(), sizeof(double) * goff));
   HIPCHK(c, up((void**)&c->d_M0F, zeros.data(), sizeof(double) * goff));
+  HIPCHK(c, up((void**)&c->d_M1F, zeros.data(), sizeof(double) * goff));
   HIPCHK(c, up((void**)&c->d_NqRT, zeros.data(), sizeof(double) * goff));
   HIPCHK(c, up((void**)&c->d_M, zeros.data(), sizeof(double) * 6 * goff));
   HIPCHK(c, up((void**)&c->d_vparGH, vparGHx.data(), sizeof(double) * voff));
@@ -1713,7 +1720,7 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
     hipLaunchKernelGGL(k_bin_initial, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a);
     // field of master step 1 from "master step 0" (no moments: M = 0), then moments of step 1
     hipLaunchKernelGGL(k_field, dim3(nt), dim3(256), 0, c->stream, c->d_tubes, c->d_hq, c->d_dq, c->d_Te,
-                       c->d_M, c->d_M0F, c->d_EA, c->d_esc_acc, c->d_esc_rep, 0);
+                       c->d_M, c->d_M0F, c->d_M1F, c->d_EA, c->d_esc_acc, c->d_esc_rep, 0);
     launch_moments(c, nullptr);
     launch_box_update(c);
     if (c->any_icr && c->cfg.precision != KAOS_MIXED)
@@ -1722,7 +1729,7 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
   } else {
     if (master) {
       hipLaunchKernelGGL(k_field, dim3(nt), dim3(256), 0, c->stream, c->d_tubes, c->d_hq, c->d_dq, c->d_Te,
-                         c->d_M, c->d_M0F, c->d_EA, c->d_esc_acc, c->d_esc_rep, 1);
+                         c->d_M, c->d_M0F, c->d_M1F, c->d_EA, c->d_esc_acc, c->d_esc_rep, 1);
       zero_hist();
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1937,6 +1944,105 @@ extern "C" int kaos_download_master(kaos_ctx* c, int32_t tube, int64_t* nq_count
   return KAOS_OK;
 }
 
+// One stream-unformatted file of DataExport2Sub: open(status='replace',
+// form='unformatted', access='stream') + write + close writes the values'
+// bytes and nothing else.
+static bool export_raw(const std::string& path, const void* p, size_t bytes) {
+  FILE* f = fopen(path.c_str(), "wb");
+  if (!f) return false;
+  size_t w = bytes ? fwrite(p, 1, bytes, f) : 0;
+  bool ok = (w == bytes);
+  if (fclose(f) != 0) ok = false;
+  return ok;
+}
+
+extern "C" int kaos_export_master(kaos_ctx* c, int32_t tube, const kaos_export_info* info, int64_t* nn_out) {
+  if (!c) return KAOS_E_ARG;
+  if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
+  if (!info || !info->dir || !info->theta || !info->Ts || info->species < 1 || info->flux_tube < 1)
+    return fail(c, KAOS_E_ARG, "export info needs dir, theta, Ts and 1-based species/flux_tube");
+  const long long n = c->last_step;
+  if (!c->built || !(n == 1 || is_master(c, n)))
+    return fail(c, KAOS_E_STATE, "export after step 1 or a master step only");
+  const long long nn = (n == 1) ? 1 : n / c->cfg.ndatfac + 1;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const DevTube& T = c->dt[tube];
+  const HostTube& h = c->ht[tube];
+  const int nq = T.nq, nvpar = T.nvpar, nvperp = T.nvperp;
+  const long long ncell = (long long)nvpar * nvperp;
+  std::vector<double> M(6 * (size_t)nq), M0F(nq), M1F(nq), EA(nq), EG(nq);
+  std::vector<unsigned int> hist((size_t)(nq * ncell));
+  HIPCHK(c, hipMemcpy(M.data(), c->d_M + 6 * T.goff, sizeof(double) * 6 * nq, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(M0F.data(), c->d_M0F + T.goff, sizeof(double) * nq, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(M1F.data(), c->d_M1F + T.goff, sizeof(double) * nq, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(EA.data(), c->d_EA + T.goff, sizeof(double) * nq, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(EG.data(), c->d_EG + T.goff, sizeof(double) * nq, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(hist.data(), c->d_hist + T.hoff, sizeof(unsigned int) * hist.size(), hipMemcpyDeviceToHost));
+
+  std::string dir(info->dir);
+  if (!dir.empty() && dir.back() != '/') dir += '/';
+  const std::string pre = dir + std::to_string(nn) + "_" + std::to_string(info->species) + "_" +
+                          std::to_string(info->flux_tube) + "_";
+  bool ok = true;
+  auto put = [&](const std::string& name, const void* p, size_t bytes) {
+    if (ok && !export_raw(pre + name + "fort.bin", p, bytes)) ok = false;
+  };
+  auto put_cells = [&](const char* name, const double* v) { put(name, v, sizeof(double) * nq); };
+  // _8_0_2_DataExport2.f90:29-107: time, cadence, reference density, grid rows
+  const double time = (n == 1) ? 0.0 : (double)n * h.g.h;       // Time(1) of step n (_8_KineticSolver.f90:178-184)
+  const int64_t ndat = c->cfg.ndatfac;
+  put("TimeT", &time, sizeof(double));
+  put("ndatfacGT", &ndat, sizeof(int64_t));
+  put("ns0GT", &info->ns0, sizeof(double));
+  const std::vector<double> pGC(nq, h.g.p), phiGC(nq, info->phi), zeros(nq, 0.0);
+  put_cells("qGCGT", h.qGC.data());
+  put_cells("pGCGT", pGC.data());
+  put_cells("rGCGT", h.rGC.data());
+  put_cells("phiGCGT", phiGC.data());
+  put_cells("thetaGCGT", info->theta);
+  put_cells("d3xCGT", h.d3x.data());
+  put_cells("TsGT", info->Ts);
+  // :167-191: NphRTp(:, :, nn) of every cell, Vperp fastest; NphRT =
+  // NphReNormRT (nsnormfac / d3x), 0 for an empty bin (_8_3_1_IonParticleCounts.f90:713-729)
+  std::vector<double> nph(ncell);
+  for (int Q = 0; Q < nq && ok; ++Q) {
+    const double w = h.g.nsnormfac / h.d3x[Q];
+    for (int i = 0; i < nvperp; ++i)
+      for (int b = 0; b < nvpar; ++b) {
+        const unsigned int k = hist[(size_t)Q * ncell + (size_t)i * nvpar + b];
+        nph[(size_t)b * nvperp + i] = (k == 0u) ? 0.0 : (double)k * w;
+      }
+    if (!export_raw(pre + std::to_string(Q + 1) + "_NphRTfort.bin", nph.data(), sizeof(double) * ncell)) ok = false;
+  }
+  // :225-510: moments; :359-410: filtered moments (only M0 and M1Par are set
+  // without charge exchange)
+  put_cells("M0phRT", M.data());
+  put_cells("M1PerpphRT", M.data() + nq);
+  put_cells("M1ParphRT", M.data() + 2 * nq);
+  put_cells("M0FiltAvrgRT", M0F.data());
+  put_cells("M1Perp1FiltAvrgRT", zeros.data());
+  put_cells("M1Perp2FiltAvrgRT", zeros.data());
+  put_cells("M1ParFiltAvrgRT", M1F.data());
+  put_cells("M2ParFiltAvrgRT", zeros.data());
+  put_cells("M2phRT", M.data() + 3 * nq);
+  put_cells("M2PerpphRT", M.data() + 4 * nq);
+  put_cells("M2ParphRT", M.data() + 5 * nq);
+  // :645-704: fields.  EAINERTIALflag off: EAInertialRT = abs(0d0), and
+  // EAmagRT = EAInertialRT + EAPressureRT (_8_7_AmbipolarEfield.f90:153-155,
+  // 1296-1299); EPARflag off: PhiParRT = 0 and EPmagRT = 0 (_8_8_PotentialStructure.f90:46-96)
+  std::vector<double> EAmag(nq);
+  for (int Q = 0; Q < nq; ++Q) EAmag[Q] = 0.0 + EA[Q];
+  put_cells("EAInertialRT", zeros.data());
+  put_cells("EAPressureRT", EA.data());
+  put_cells("EAmagRT", EAmag.data());
+  put_cells("EGmagRT", EG.data());
+  put_cells("EPmagRT", zeros.data());
+  put_cells("PhiParRT", zeros.data());
+  if (!ok) return fail(c, KAOS_E_IO, "cannot write the export files under " + dir);
+  if (nn_out) *nn_out = nn;
+  return KAOS_OK;
+}
+
 extern "C" int kaos_download_particles(kaos_ctx* c, int32_t tube, int64_t n_cap, int64_t* n_out,
                                        double* x, double* y, double* z, double* Vx, double* Vy, double* Vz,
                                        double* vperp, double* qk4, double* aea, double* ag,
@@ -2068,7 +2174,7 @@ extern "C" int kaos_finalize(kaos_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   void* bufs[] = {c->d_tubes, c->d_tile_tube, c->d_tile_start, c->d_blk_tube, c->d_blk_q, c->d_cells_wave,
                   c->d_cells_block, c->d_qGH, c->d_qGC,
-                  c->d_hq, c->d_dq, c->d_Te, c->d_d3x, c->d_EA, c->d_EG, c->d_M, c->d_M0F, c->d_NqRT,
+                  c->d_hq, c->d_dq, c->d_Te, c->d_d3x, c->d_EA, c->d_EG, c->d_M, c->d_M0F, c->d_M1F, c->d_NqRT,
                   c->d_vparGH, c->d_vperpGH, c->d_vparGC, c->d_vperpGC, c->d_dvpar, c->d_dvperp, c->d_d3v,
                   c->d_rGH, c->d_rref, c->d_rk, c->d_vk, c->d_keycnt, c->d_cursor, c->d_cnt_new,
                   c->d_next_id, c->d_count, c->d_hist,

@@ -50,7 +50,7 @@ program kaos_fdriver
   cfg%ntubes = 1; cfg%precision = hdr(11); cfg%ndatfac = hdr(10); cfg%device = 0; cfg%seed = h64(5)
   cfg%reorder_every = 0; cfg%flags = 0
   g%nq = hdr(1); g%nvpar = hdr(2); g%nvperp = hdr(3); g%grav = hdr(4); g%eamb = hdr(5)
-  g%eambsign = hdr(6); g%filter_pt = hdr(7); g%lbcond = hdr(8); g%ubcond = hdr(9); g%pad0 = 0
+  g%eambsign = hdr(6); g%filter_pt = hdr(7); g%lbcond = hdr(8); g%ubcond = hdr(9); g%filter_pt_m1par = 0
   g%n_inj_lb = h64(1); g%n_inj_ub = h64(2)
   g%p = hd(1); g%cphi = hd(2); g%sphi = hd(3); g%h = hd(4); g%ms = hd(5); g%qs = hd(6)
   g%nsnormfac = hd(7); g%Ts_lb = hd(8); g%Ts_ub = hd(9)

@@ -15,7 +15,8 @@ module kaos_hip
   private
 
   integer(c_int), parameter, public :: KAOS_OK = 0, KAOS_E_ARG = 1, KAOS_E_CAPACITY = 2, &
-                                       KAOS_E_DEVICE = 3, KAOS_E_STATE = 4, KAOS_E_NAN = 5
+                                       KAOS_E_DEVICE = 3, KAOS_E_STATE = 4, KAOS_E_NAN = 5, &
+                                       KAOS_E_IO = 6
   integer(c_int32_t), parameter, public :: KAOS_FP64 = 0, KAOS_MIXED = 1
   integer(c_int32_t), parameter, public :: KAOS_FLAG_NO_GRAPH = 1, KAOS_FLAG_NO_AUTOTUNE = 2
 
@@ -27,7 +28,7 @@ module kaos_hip
   end type kaos_cfg
 
   type, bind(c), public :: kaos_grid
-    integer(c_int32_t) :: nq, nvpar, nvperp, grav, eamb, eambsign, filter_pt, lbcond, ubcond, pad0
+    integer(c_int32_t) :: nq, nvpar, nvperp, grav, eamb, eambsign, filter_pt, lbcond, ubcond, filter_pt_m1par
     integer(c_int64_t) :: n_inj_lb, n_inj_ub
     real(c_double) :: p, cphi, sphi, h, ms, qs, nsnormfac, Ts_lb, Ts_ub
     type(c_ptr) :: qGL, qGH, qGC, hqC, dqC, d3xC, rGC, cthGC, ellGC, Te
@@ -38,6 +39,14 @@ module kaos_hip
     real(c_double) :: icr_s0 = 0, icr_f0 = 0, icr_chi_perp1 = 0, icr_chi_perp2 = 0
   end type kaos_grid
 
+  ! dir: c_loc of a NUL-terminated character array (trim(dataexportdir)//c_null_char)
+  type, bind(c), public :: kaos_export_info
+    type(c_ptr) :: dir
+    integer(c_int32_t) :: species, flux_tube
+    real(c_double) :: ns0, phi
+    type(c_ptr) :: theta, Ts
+  end type kaos_export_info
+
   type, bind(c), public :: kaos_step_out
     integer(c_int64_t) :: n_alive, n_injected
     integer(c_int32_t) :: master, status
@@ -45,7 +54,7 @@ module kaos_hip
 
   public :: kaos_init, kaos_set_grid, kaos_upload_particles, kaos_upload_vperp, kaos_cell_counts, &
             kaos_seed_particles, kaos_step, kaos_run, &
-            kaos_download_master, kaos_count, kaos_download_particles, kaos_set_launch, &
+            kaos_download_master, kaos_export_master, kaos_count, kaos_download_particles, kaos_set_launch, &
             kaos_get_launch, kaos_profile, &
             kaos_profile_read, kaos_last_error, kaos_finalize, kaos_error_message
 
@@ -124,6 +133,15 @@ module kaos_hip
       integer(c_int32_t), value :: tube
       type(c_ptr), value :: nq_counts, nph_counts, NqRT, M, EA, EG, esc
     end function kaos_download_master
+
+    ! DataExport2Sub's files of the last master step
+    integer(c_int) function kaos_export_master(ctx, tube, info, nn) bind(c, name='kaos_export_master')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, kaos_export_info
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tube
+      type(kaos_export_info), intent(in) :: info
+      integer(c_int64_t), intent(out) :: nn
+    end function kaos_export_master
 
     integer(c_int) function kaos_count(ctx, tube, n) bind(c, name='kaos_count')
       import :: c_int, c_ptr, c_int32_t, c_int64_t

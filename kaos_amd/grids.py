@@ -65,6 +65,7 @@ class Tube:
     eamb: int = 1
     eambsign: int = 1
     filter_pt: int = 11
+    filter_pt_m1par: int = 0          # 0: the same window as filter_pt
     lbcond: int = 1
     ubcond: int = 1
     # ICR wave heating (stochastic); defaults are the reference's

@@ -14,7 +14,7 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
-from ._abi import (KaosCfg, KaosGrid, KaosStepOut, ptr, KAOS_FP64, KAOS_MIXED,  # noqa: F401
+from ._abi import (KaosCfg, KaosGrid, KaosStepOut, KaosExportInfo, ptr, KAOS_FP64, KAOS_MIXED,  # noqa: F401
                    KAOS_FLAG_NO_GRAPH, KAOS_FLAG_NO_AUTOTUNE)
 
 MOMENT_NAMES = ("M0", "M1perp", "M1par", "M2", "M2perp", "M2par")
@@ -34,6 +34,7 @@ def grid_struct(tube):
               "Ts_ub"):
         setattr(g, n, getattr(tube, n))
     g.cphi, g.sphi = tube.cphi, tube.sphi
+    g.filter_pt_m1par = getattr(tube, "filter_pt_m1par", 0)
     g.icr = getattr(tube, "icr", 0)
     for n in _abi.ICR_FIELDS:
         setattr(g, n, getattr(tube, n, 0.0))
@@ -142,6 +143,19 @@ class KineticSolver:
         for i, n in enumerate(MOMENT_NAMES):
             out[n] = M[i * nq:(i + 1) * nq].copy()
         return out
+
+    def export_master(self, tube, directory, theta, Ts, ns0, phi, species=1, flux_tube=1):
+        """DataExport2Sub's files of the last master step (kaos_export_master);
+        returns the master-step index nn of the file names."""
+        theta = np.ascontiguousarray(theta, dtype=np.float64)
+        Ts = np.ascontiguousarray(Ts, dtype=np.float64)
+        if len(theta) != self.tubes[tube].nq or len(Ts) != self.tubes[tube].nq:
+            raise ValueError("theta and Ts need one value per config cell")
+        info = KaosExportInfo(dir=str(directory).encode(), species=species, flux_tube=flux_tube, ns0=ns0,
+                              phi=phi, theta=ptr(theta), Ts=ptr(Ts))
+        nn = C.c_int64(0)
+        self._chk(self.lib.kaos_export_master(self.h, tube, C.byref(info), C.byref(nn)))
+        return nn.value
 
     def particles(self, tube, cap=None):
         cap = int(cap if cap is not None else self.count(tube))

@@ -30,7 +30,8 @@ class OrcCfg(C.Structure):
                 + [("tube", C.c_int), ("seed", C.c_uint64)]
                 + [(n, _dp) for n in ARRAYS]
                 + [("icr", C.c_int)]
-                + [(n, C.c_double) for n in ICR_FIELDS])
+                + [(n, C.c_double) for n in ICR_FIELDS]
+                + [("filter_pt_m1par", C.c_int)])
 
 
 _lib = None
@@ -113,6 +114,7 @@ def load():
         lib.orc_seed_ions.argtypes = [C.POINTER(OrcCfg), i64p, _dp, C.c_uint64, C.c_int] + [_dp] * 6
         lib.orc_seed_ions.restype = C.c_int64
         lib.orc_seed_uniforms.argtypes = [C.c_uint64, C.c_int, C.c_int64, _dp]
+        lib.orc_get_m1f.argtypes = [P, _dp]
         _lib = lib
     return _lib
 
@@ -131,6 +133,7 @@ def cfg_of(tube, ndatfac=1, seed=1, tube_index=0):
     c.tube = tube_index
     c.seed = seed
     c.icr = getattr(tube, "icr", 0)
+    c.filter_pt_m1par = getattr(tube, "filter_pt_m1par", 0)
     for n in ICR_FIELDS:
         setattr(c, n, getattr(tube, n, 0.0))
     keep = []
@@ -199,6 +202,9 @@ class Oracle:
                    NqRT=NqRT, EA=EA, EG=EG, M0F=M0F, esc=esc)
         for k, n in enumerate(("M0", "M1perp", "M1par", "M2", "M2perp", "M2par")):
             out[n] = M[k * nq:(k + 1) * nq].copy()
+        M1F = np.zeros(nq)
+        self.lib.orc_get_m1f(self.h, ptr(M1F))
+        out["M1F"] = M1F
         return out
 
     def close(self):

@@ -51,6 +51,7 @@ typedef struct {
   int icr;
   double icr_lambda_perp, icr_eta_lh, icr_xi_perp1, icr_xi_perp2, icr_s0, icr_f0, icr_chi_perp1,
       icr_chi_perp2;
+  int filter_pt_m1par;            /* M1ParMAfilterPt; 0 = filter_pt */
 } orc_cfg;
 
 typedef struct {
@@ -69,6 +70,7 @@ typedef struct {
   double *NqRT, *M;                /* M: 6*nq: M0, M1perp, M1par, M2, M2perp, M2par */
   double *Mprev;                   /* moments of the previous master step (6*nq) */
   double *EA, *EG, *M0F;           /* EAmagRT(nn), EGmagRT(nn), M0FiltAvrgRT(nn) */
+  double *M1F;                     /* M1ParFiltAvrgRT(nn) */
   double esc_lb, esc_ub, flux_lb, flux_ub;
   int64_t injected;
 } orc_t;
@@ -115,6 +117,11 @@ static int scan_vcell(const double *GL, const double *GH, int n, double v, int p
 static void ambipolar_field(const orc_cfg *c, const double *M0prev, double *M0F, double *EA) {
   kaos_ambipolar(c->nq, c->qGL[0] <= 0, c->eamb, c->eambsign, c->filter_pt, c->qs,
                  c->hqC, c->dqC, c->Te, M0prev, M0F, EA);
+}
+/* M1ParFiltAvrgRT(nn): the moving-average filter of the previous master
+ * step's M1Par (_8_7_AmbipolarEfield.f90:64-80), window M1ParMAfilterPt. */
+static void m1par_filter(const orc_cfg *c, const double *Mprev, double *M1F) {
+  kaos_moment_filter(Mprev + 2 * c->nq, M1F, c->nq, c->filter_pt_m1par ? c->filter_pt_m1par : c->filter_pt);
 }
 static void grav_field(const orc_cfg *c, double *EG) {
   for (int Q = 0; Q < c->nq; ++Q)
@@ -245,6 +252,7 @@ void *orc_create(const orc_cfg *cfg, int64_t ns, const double *q0, const double 
   o->EA = (double *)xc(sizeof(double) * nq);
   o->EG = (double *)xc(sizeof(double) * nq);
   o->M0F = (double *)xc(sizeof(double) * nq);
+  o->M1F = (double *)xc(sizeof(double) * nq);
   /* initial conditions: x0 = root(q0), V0 = Vpar0 b-hat(x0) */
   for (int64_t j = 0; j < ns; ++j) {
     o->rs[j] = kaos_rs_of_q(q0[j], cfg->p);
@@ -267,7 +275,7 @@ void orc_destroy(void *h) {
   ARR_D(FR) ARR_I(FR) ARR_L(FR)
 #undef FR
   free(o->nq_counts); free(o->nph_counts); free(o->nq_u); free(o->nph_u); free(o->NqRT); free(o->M); free(o->Mprev);
-  free(o->EA); free(o->EG); free(o->M0F);
+  free(o->EA); free(o->EG); free(o->M0F); free(o->M1F);
   free(o);
 }
 
@@ -403,6 +411,7 @@ int orc_step(void *h, int64_t n, const double *uniforms) {
     counts_and_moments(o, o->qind, o->vparind, o->vperpind, o->ns);
     memset(o->Mprev, 0, sizeof(double) * 6 * nq);   /* moments of master step 0: none */
     ambipolar_field(c, o->Mprev, o->M0F, o->EA);
+    m1par_filter(c, o->Mprev, o->M1F);
     grav_field(c, o->EG);
     for (int64_t j = 0; j < o->ns; ++j) {
       interp_scan(c, o->EA, o->q0[j], &o->aeaN[j], 0);
@@ -416,6 +425,7 @@ int orc_step(void *h, int64_t n, const double *uniforms) {
     if (master) {
       bc_remove(o);
       ambipolar_field(c, o->Mprev, o->M0F, o->EA);
+      m1par_filter(c, o->Mprev, o->M1F);
       grav_field(c, o->EG);
       for (int64_t j = 0; j < o->ns; ++j) {
         interp_scan(c, o->EA, o->qk4[j], &o->aeaN[j], 0);
@@ -503,6 +513,11 @@ void orc_get_master(void *h, double *nq_counts, double *nph_counts, double *NqRT
   if (EG) memcpy(EG, o->EG, sizeof(double) * nq);
   if (M0F) memcpy(M0F, o->M0F, sizeof(double) * nq);
   if (esc) { esc[0] = o->esc_lb; esc[1] = o->esc_ub; esc[2] = o->flux_lb; esc[3] = o->flux_ub; }
+}
+
+void orc_get_m1f(void *h, double *M1F) {
+  orc_t *o = (orc_t *)h;
+  memcpy(M1F, o->M1F, sizeof(double) * o->c.nq);
 }
 
 /* ---- standalone pieces exposed for the unit tests --------------------------- */

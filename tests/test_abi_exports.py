@@ -49,6 +49,9 @@ int main(void) {
   printf("%zu %zu %zu\n", offsetof(kaos_grid, n_inj_lb), offsetof(kaos_grid, p), offsetof(kaos_grid, qGL));
   printf("%zu %zu\n", sizeof(kaos_step_out), offsetof(kaos_step_out, status));
   printf("%zu %zu %zu\n", offsetof(kaos_grid, icr), offsetof(kaos_grid, icr_lambda_perp), offsetof(kaos_grid, icr_chi_perp2));
+  printf("%zu %zu\n", offsetof(kaos_grid, filter_pt_m1par), offsetof(kaos_grid, n_inj_ub));
+  printf("%zu %zu %zu %zu %zu\n", sizeof(kaos_export_info), offsetof(kaos_export_info, species),
+         offsetof(kaos_export_info, ns0), offsetof(kaos_export_info, theta), offsetof(kaos_export_info, Ts));
   return 0;
 }
 '''
@@ -66,5 +69,8 @@ int main(void) {
     want = [C.sizeof(_abi.KaosCfg), _abi.KaosCfg.seed.offset, C.sizeof(_abi.KaosGrid), _abi.KaosGrid.d3v.offset,
             _abi.KaosGrid.n_inj_lb.offset, _abi.KaosGrid.p.offset, _abi.KaosGrid.qGL.offset,
             C.sizeof(_abi.KaosStepOut), _abi.KaosStepOut.status.offset,
-            _abi.KaosGrid.icr.offset, _abi.KaosGrid.icr_lambda_perp.offset, _abi.KaosGrid.icr_chi_perp2.offset]
+            _abi.KaosGrid.icr.offset, _abi.KaosGrid.icr_lambda_perp.offset, _abi.KaosGrid.icr_chi_perp2.offset,
+            _abi.KaosGrid.filter_pt_m1par.offset, _abi.KaosGrid.n_inj_ub.offset,
+            C.sizeof(_abi.KaosExportInfo), _abi.KaosExportInfo.species.offset, _abi.KaosExportInfo.ns0.offset,
+            _abi.KaosExportInfo.theta.offset, _abi.KaosExportInfo.Ts.offset]
     assert got == want

@@ -1,0 +1,128 @@
+"""DataExport2Sub's files (kaos_export_master) against the oracle's export
+restatement (oracle/export.py), through the C-ABI.
+
+FP64: every file of every exported master step is byte-identical to the one
+the oracle writes from its own state (names, sizes, contents), for two tubes
+of one context (one southern, one with its own M1Par filter window), with
+ndatfac 2.  Mixed: the files hold exactly what the library's own downloads
+return.  Errors: export after a non-master step, an unwritable directory,
+missing arrays.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import export as oexp
+from kaos_amd import grids
+from kaos_amd.solver import KineticSolver, KaosError, KAOS_FP64, KAOS_MIXED
+
+pytestmark = pytest.mark.gpu
+
+
+def _tubes():
+    a = grids.dipole_tube(nq=40, nvpar=24, nvperp=20, n_total=12000, ratio=1.08)
+    a.filter_pt_m1par = 5
+    a.n_inj_ub = 6
+    b = grids.dipole_tube(nq=32, nvpar=16, nvperp=16, n_total=5000, south=True, p=6.0)
+    return [a, b]
+
+
+def _extra(t):
+    return np.arccos(t.arrays["cthGC"]), np.linspace(2000.0, 4000.0, t.nq)
+
+
+def _same_dirs(da, db):
+    fa, fb = sorted(os.listdir(da)), sorted(os.listdir(db))
+    assert fa == fb, (set(fa) ^ set(fb))
+    for n in fa:
+        with open(os.path.join(da, n), "rb") as x, open(os.path.join(db, n), "rb") as y:
+            if x.read() != y.read():
+                raise AssertionError(f"{n} differs")
+    return len(fa)
+
+
+def test_export_fp64_byte_identical_to_oracle(tmp_path):
+    tubes = _tubes()
+    ndatfac, nsteps, seed = 2, 8, 31
+    ions = [grids.initial_ions(t, seed=40 + i) for i, t in enumerate(tubes)]
+    caps = [len(q) + (nsteps + 2) * (t.n_inj_lb + t.n_inj_ub) + 256 for t, (q, _, _) in zip(tubes, ions)]
+    orcs = [oracle.Oracle(t, *ions[i], ndatfac=ndatfac, seed=seed, capacity=caps[i], tube_index=i)
+            for i, t in enumerate(tubes)]
+    ks = KineticSolver(tubes, precision=KAOS_FP64, ndatfac=ndatfac, seed=seed, capacity=caps)
+    for i, (q, v, w) in enumerate(ions):
+        ks.upload(i, q, v, w)
+    print("export fp64: 2 tubes, ndatfac 2, steps", nsteps, flush=True)
+    lib_dir, orc_dir = tmp_path / "lib", tmp_path / "orc"
+    lib_dir.mkdir()
+    orc_dir.mkdir()
+    nfiles = 0
+    for n in range(1, nsteps + 1):
+        for o in orcs:
+            o.step(n)
+        assert ks.step(n).status == 0
+        if n == 1 or n % ndatfac == 0:
+            for i, (t, o) in enumerate(zip(tubes, orcs)):
+                theta, Ts = _extra(t)
+                nn = ks.export_master(i, str(lib_dir), theta, Ts, ns0=1e8, phi=t.phi, species=1, flux_tube=i + 1)
+                assert nn == (1 if n == 1 else n // ndatfac + 1)
+                assert oexp.export_master(str(orc_dir), t, o.master(), n, ndatfac, theta, Ts, ns0=1e8,
+                                          phi=t.phi, s=1, f=i + 1) == nn
+            nfiles = _same_dirs(str(lib_dir), str(orc_dir))
+    assert nfiles == 5 * (2 * 27 + tubes[0].nq + tubes[1].nq)
+    # the filtered M1Par of the first tube used its own window
+    m1f = np.fromfile(str(lib_dir / "5_1_1_M1ParFiltAvrgRTfort.bin"))
+    assert m1f.any()
+    ks.close()
+
+
+def test_export_mixed_holds_the_downloads(tmp_path):
+    t = grids.config1(n_total=20000)
+    nsteps = 4
+    q, v, w = grids.initial_ions(t, seed=2)
+    ks = KineticSolver([t], precision=KAOS_MIXED, ndatfac=1, seed=5,
+                       capacity=[len(q) + (nsteps + 2) * (t.n_inj_lb + t.n_inj_ub) + 256])
+    ks.upload(0, q, v, w)
+    theta, Ts = _extra(t)
+    print("export mixed: config 1 tube, steps", nsteps, flush=True)
+    for n in range(1, nsteps + 1):
+        assert ks.step(n).status == 0
+    nn = ks.export_master(0, str(tmp_path) + "/", theta, Ts, ns0=2e8, phi=t.phi)
+    assert nn == nsteps + 1
+    m = ks.master(0)
+    rd = lambda name: np.fromfile(str(tmp_path / f"{nn}_1_1_{name}fort.bin"))
+    for name, key in (("M0phRT", "M0"), ("M1PerpphRT", "M1perp"), ("M1ParphRT", "M1par"), ("M2phRT", "M2"),
+                      ("M2PerpphRT", "M2perp"), ("M2ParphRT", "M2par"), ("EAPressureRT", "EA"),
+                      ("EGmagRT", "EG")):
+        assert np.array_equal(rd(name), m[key]), name
+    assert rd("TimeT").tolist() == [nsteps * t.h] and rd("ns0GT").tolist() == [2e8]
+    for Q in (0, 5, 40):
+        got = np.fromfile(str(tmp_path / f"{nn}_1_1_{Q + 1}_NphRTfort.bin")).reshape(t.nvpar, t.nvperp).T
+        c = m["nph_counts"][Q]
+        want = np.where(c == 0, 0.0, c * (t.nsnormfac / t.arrays["d3xC"][Q]))
+        assert np.array_equal(got, want), Q
+    ks.close()
+
+
+def test_export_errors(tmp_path):
+    t = grids.dipole_tube(nq=24, nvpar=16, nvperp=16, n_total=2000, ratio=1.1)
+    q, v, w = grids.initial_ions(t, seed=1)
+    ks = KineticSolver([t], precision=KAOS_FP64, ndatfac=3, seed=1, capacity=[len(q) + 512])
+    ks.upload(0, q, v, w)
+    theta, Ts = _extra(t)
+    with pytest.raises(KaosError):                     # nothing stepped yet
+        ks.export_master(0, str(tmp_path), theta, Ts, ns0=1e8, phi=t.phi)
+    assert ks.step(1).status == 0
+    assert ks.export_master(0, str(tmp_path), theta, Ts, ns0=1e8, phi=t.phi) == 1
+    assert ks.step(2).status == 0                      # not a master step (ndatfac 3)
+    with pytest.raises(KaosError):
+        ks.export_master(0, str(tmp_path), theta, Ts, ns0=1e8, phi=t.phi)
+    assert ks.step(3).status == 0
+    with pytest.raises(KaosError) as e:
+        ks.export_master(0, str(tmp_path / "no" / "such" / "dir"), theta, Ts, ns0=1e8, phi=t.phi)
+    assert e.value.code == 6
+    with pytest.raises(ValueError):
+        ks.export_master(0, str(tmp_path), theta[:-1], Ts, ns0=1e8, phi=t.phi)
+    assert ks.export_master(0, str(tmp_path), theta, Ts, ns0=1e8, phi=t.phi) == 2
+    ks.close()
