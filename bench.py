@@ -172,6 +172,28 @@ def seed_run(args, host_setup_s):
                     "host generation of the benchmark's ions plus kaos_upload_particles"}
 
 
+def export_run(ks, tube):
+    """kaos_export_master of the last master step of the full-size tube
+    (DataExport2Sub's files) into a scratch directory: wall time, files and
+    bytes; the directory is removed afterwards."""
+    import shutil
+    import tempfile
+    d = tempfile.mkdtemp(prefix="kaos_export_")
+    try:
+        theta = np.arccos(tube.arrays["cthGC"])
+        Ts = np.full(tube.nq, tube.Ts_lb)
+        t1 = time.perf_counter()
+        nn = ks.export_master(0, d, theta, Ts, ns0=1e8, phi=tube.phi)
+        dt = time.perf_counter() - t1
+        names = os.listdir(d)
+        nbytes = sum(os.path.getsize(os.path.join(d, n)) for n in names)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    log(f"[bench] export of master step {nn}: {len(names)} files, {nbytes} bytes in {dt*1e3:.1f} ms")
+    return {"nn": nn, "files": len(names), "bytes": nbytes, "seconds": dt,
+            "note": "kaos_export_master wall time: device-to-host copies and file writes"}
+
+
 def batched_run(args):
     """512 tubes, mixed form: timed with graph replay (no per-kernel events), then
     a few eager steps with hipEvents for the push kernel's mean duration."""
@@ -350,6 +372,7 @@ def main():
                         # edge after 200 steps of binary32 stages) moves it by ~1e-6
                         "max_l2_rel_moment_error": max(err[k]["l2_rel"] for k in MOMENTS if k != "EA"),
                         "l2_rel_field_error_EA": err["EA"]["l2_rel"]}
+    out["export"] = export_run(runs[forms[0]][0], tube)
     for ks, _ in runs.values():
         ks.close()
     out["initial_conditions"] = seed_run(args, host_gen_s + runs[forms[0]][1]["upload_s"])
