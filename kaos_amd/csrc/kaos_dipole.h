@@ -883,12 +883,17 @@ KAOS_HD int kaos_mom_groups(int nvpar) {
 /* Partial column aggregates of column b over rows g0, g0 + G, ... within
  * [a0, a1], and the partial count of those cells (empty cells contribute
  * nothing, so rows left out that hold no counts change nothing).  The device
- * compile unrolls the row loop so the count loads issue together; the
+ * compile unrolls the row loop (4 rows) so their loads issue together; the
  * arithmetic order is the same on both compilers. */
 #if defined(__HIP_DEVICE_COMPILE__)
-#define KAOS_UNROLL8 _Pragma("unroll 8")
+#ifndef KAOS_MOM_UNROLL
+#define KAOS_MOM_UNROLL 4
+#endif
+#define KAOS_PRAGMA_(x) _Pragma(#x)
+#define KAOS_UNROLL_N_(n) KAOS_PRAGMA_(unroll n)
+#define KAOS_MOM_UNROLL_PRAGMA KAOS_UNROLL_N_(KAOS_MOM_UNROLL)
 #else
-#define KAOS_UNROLL8
+#define KAOS_MOM_UNROLL_PRAGMA
 #endif
 KAOS_HD void kaos_mom_part(const kaos_vgrid_t* v, const uint32_t* cnt, int b, int g0, int G, int a0, int a1,
                            double k, double P[3], uint32_t* csum) {
@@ -896,15 +901,17 @@ KAOS_HD void kaos_mom_part(const kaos_vgrid_t* v, const uint32_t* cnt, int b, in
   uint32_t cs = 0;
   const double dvb = v->dvpar[b];
   int a = a0 + (((g0 - a0) % G) + G) % G;   /* first row >= a0 of this group */
-  KAOS_UNROLL8
+  KAOS_MOM_UNROLL_PRAGMA
   for (; a <= a1; a += G) {
     int i = a * v->nvpar + b;
-    uint32_t c = cnt[i];
+    /* every load of the row first (all in range), so that the unrolled rows'
+     * loads are in flight together; the arithmetic is unchanged */
+    const uint32_t c = cnt[i];
+    const double d3 = v->d3v[i], dvp = v->dvperp[a], vc = v->vperpGC[a];
     cs += c;
     if (c != 0u) {
-      double f = ((double)c * k) / v->d3v[i];
-      double w = v->dvperp[a] * dvb;
-      double vc = v->vperpGC[a];
+      double f = ((double)c * k) / d3;
+      double w = dvp * dvb;
       s0 = s0 + w * (vc * f);
       s1 = s1 + w * ((vc * vc) * f);
       s3 = s3 + w * ((vc * (vc * vc)) * f);

@@ -822,7 +822,12 @@ __device__ __forceinline__ double kaos_wave_tree(double x, int lane, int n) {
   return x;
 }
 
-__global__ void __launch_bounds__(KAOS_BLOCK) k_moments_wave(const DevTube* tubes, const int* blk_tube,
+// Waves per SIMD of the moment kernel: the row loop waits on its count loads,
+// so the kernel runs at full occupancy (62 VGPRs with the 4-row unroll).
+#ifndef KAOS_MOM_WAVES_EU
+#define KAOS_MOM_WAVES_EU 8
+#endif
+__global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MOM_WAVES_EU) k_moments_wave(const DevTube* tubes, const int* blk_tube,
     const int* blk_q, const int* cells, int ncells, const double* d3xC, const double* d3v,
     const double* dvpar, const double* dvperp, const double* vparGC, const double* vperpGC,
     const unsigned int* hist, const unsigned int* nqoff, unsigned int* nqc, double* NqRT, double* M,
