@@ -12,7 +12,10 @@ DRIVER = os.path.join(HERE, "kaos_fdriver")
 CONFIG_ARRAYS = ("qGL", "qGH", "qGC", "hqC", "dqC", "d3xC", "rGC", "cthGC", "ellGC", "Te")
 
 
-def write_input(path, tube, q0, vpar0, vperp0, nsteps, ndatfac=1, precision=0, capacity=None, seed=1):
+def write_input(path, tube, q0, vpar0, vperp0, nsteps, ndatfac=1, precision=0, capacity=None, seed=1,
+                export=None):
+    """export: None, or dict(theta, Ts, density (nq each), phi, ns0) for a run
+    with an export directory (and optionally device-seeded ions)."""
     a = tube.arrays
     n = len(q0)
     cap = capacity if capacity is not None else n + (nsteps + 2) * (tube.n_inj_lb + tube.n_inj_ub) + 1024
@@ -28,6 +31,10 @@ def write_input(path, tube, q0, vpar0, vperp0, nsteps, ndatfac=1, precision=0, c
             np.ascontiguousarray(a[k], dtype=np.float64).tofile(f)
         for v in (q0, vpar0, vperp0):
             np.ascontiguousarray(v, dtype=np.float64).tofile(f)
+        if export is not None:
+            for k in ("theta", "Ts", "density"):
+                np.ascontiguousarray(export[k], dtype=np.float64).tofile(f)
+            np.array([export["phi"], export["ns0"]], dtype=np.float64).tofile(f)
 
 
 def read_output(path, nq):

@@ -48,3 +48,43 @@ def test_fortran_driver_matches_oracle(tmp_path, ndatfac):
         assert np.array_equal(rec["EA"], m["EA"]), n
         assert np.array_equal(rec["esc"], m["esc"]), n
     orc.close()
+
+
+def test_fortran_driver_seed_and_export(tmp_path):
+    """The Fortran driver seeds the ions on the device (kaos_cell_counts +
+    kaos_seed_particles) and exports every master step (kaos_export_master):
+    the files equal, byte for byte, the oracle's export of the oracle started
+    from its own restatement of the seeding."""
+    from oracle import export as oexp
+    if not os.path.exists(kf.DRIVER):
+        pytest.fail("kaos_fdriver is not built (kaos_amd.build.build_fortran)")
+    tube = grids.dipole_tube(nq=48, nvpar=32, nvperp=32, n_total=15000, ratio=1.06)
+    nsteps, ndatfac, seed, iseed = 6, 2, 9, 777
+    lib = oracle.load()
+    density = 1e10 * np.exp(-tube.s_centre / 4.0e5)
+    counts = np.array([lib.orc_cell_count(a, b, tube.nsnormfac) for a, b in zip(density, tube.arrays["d3xC"])])
+    ex = dict(theta=np.arccos(tube.arrays["cthGC"]), Ts=np.linspace(2500.0, 5000.0, tube.nq), density=density,
+              phi=tube.phi, ns0=1e8)
+    cap = int(counts.sum()) + (nsteps + 2) * (tube.n_inj_lb + tube.n_inj_ub) + 512
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    fdir, odir = tmp_path / "fexp", tmp_path / "oexp"
+    fdir.mkdir()
+    odir.mkdir()
+    kf.write_input(fin, tube, np.zeros(0), np.zeros(0), np.zeros(0), nsteps, ndatfac=ndatfac, seed=seed,
+                   capacity=cap, export=ex)
+    print(f"[fortran] seeded run with export, ions {int(counts.sum())}", flush=True)
+    r = subprocess.run([kf.DRIVER, fin, fout, str(fdir), str(iseed)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    s = oracle.seed_ions(tube, counts, ex["Ts"], iseed)
+    orc = oracle.Oracle(tube, s["q0"], s["vpar"], s["vperp"], ndatfac=ndatfac, seed=seed, capacity=cap)
+    orc.set_vperp(s["v1"], s["v2"])
+    for n in range(1, nsteps + 1):
+        orc.step(n)
+        if n == 1 or n % ndatfac == 0:
+            oexp.export_master(str(odir), tube, orc.master(), n, ndatfac, ex["theta"], ex["Ts"], ns0=ex["ns0"],
+                               phi=ex["phi"])
+    orc.close()
+    names = sorted(os.listdir(fdir))
+    assert names == sorted(os.listdir(odir)) and len(names) == 4 * (27 + tube.nq)
+    for nm in names:
+        assert (fdir / nm).read_bytes() == (odir / nm).read_bytes(), nm
