@@ -105,7 +105,8 @@ typedef struct kaos_grid {
 
 /* Per-step summary. */
 typedef struct kaos_step_out {
-  int64_t n_alive;       /* ions pushed this step, all tubes (particle-steps of the step) */
+  int64_t n_alive;       /* live ions after the step, all tubes: the ions pushed this step
+                            plus the ions injected (kaos_count summed over the tubes) */
   int64_t n_injected;    /* ions injected this step */
   int32_t master;        /* 1 when this step was a master step */
   int32_t status;        /* KAOS_OK or an error code raised on the device */
