@@ -594,6 +594,19 @@ __global__ void k_set_vperp(DevTube T, long long n, const double* v1, const doub
 // that a captured step replays unchanged.
 __global__ void k_step_tick(long long* stepno) { *stepno += 1; }
 
+// Live slots of a tube segment (a dead slot: the dead cell word in the mixed
+// form, rs = 0 in the FP64 form), summed into *out (cleared by the caller).
+__global__ void k_count_live(const uint32_t* cell, const double* rs, long long n, int mixed,
+                             unsigned long long* out) {
+  unsigned int k = 0u;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    k += mixed ? (cell[i] != KAOS_DEAD) : (rs[i] != 0.0);
+  for (int st = 32; st >= 1; st >>= 1) k += __shfl_xor(k, st, 64);
+  if ((threadIdx.x & 63) == 0 && k) atomicAdd(out, (unsigned long long)k);
+}
+
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
 __global__ void k_fill_u32(unsigned int* p, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long st = (long long)gridDim.x * blockDim.x;
@@ -1096,6 +1109,7 @@ struct kaos_ctx {
   struct Launch { int tpb, hwords, persistent; };
   Launch launch = {KAOS_TPB, KAOS_HWORDS, 0};
   unsigned int* d_work = nullptr;
+  unsigned long long* d_live = nullptr;        // kaos_count's device sum
   int any_icr = 0;                              // some tube has ICR heating on
   int4* d_vbox = nullptr;                       // per tube velocity box of the push's LDS window
   int4* d_hbox = nullptr;                       // per (tube, cell) bounding box of the master step's counts
@@ -1378,6 +1392,7 @@ static int build_device(kaos_ctx* c) {
   HIPCHK(c, hipMemset(c->d_nqc, 0, sizeof(unsigned int) * (size_t)goff));
   HIPCHK(c, hipMemset(c->d_nqoff, 0, sizeof(unsigned int) * (size_t)goff));
   HIPCHK(c, hipMalloc((void**)&c->d_work, sizeof(unsigned int)));
+  HIPCHK(c, hipMalloc((void**)&c->d_live, sizeof(unsigned long long)));
   {
     std::vector<int4> bx(nt);
     for (int t = 0; t < nt; ++t) bx[t] = make_int4(1, c->dt[t].nvpar, 1, c->dt[t].nvperp);
@@ -1897,17 +1912,19 @@ extern "C" int kaos_count(kaos_ctx* c, int32_t tube, int64_t* n) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const HostTube& h = c->ht[tube];
   const DevTube& T = c->dt[tube];
-  int64_t k = 0;
-  if (c->cfg.precision == KAOS_MIXED) {
-    std::vector<uint32_t> cw((size_t)h.count);
-    if (h.count) HIPCHK(c, hipMemcpy(cw.data(), c->st[c->cur].cell + T.off, sizeof(uint32_t) * h.count, hipMemcpyDeviceToHost));
-    for (uint32_t w : cw) k += (w != KAOS_DEAD);
-  } else {
-    std::vector<double> rs((size_t)h.count);
-    if (h.count) HIPCHK(c, hipMemcpy(rs.data(), c->st[c->cur].rs + T.off, sizeof(double) * h.count, hipMemcpyDeviceToHost));
-    for (double r : rs) k += (r != 0.0);
+  unsigned long long k = 0ull;
+  if (h.count) {
+    const int mixed = c->cfg.precision == KAOS_MIXED;
+    const long long nb = std::min<long long>((h.count + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, c->stream, c->d_live);
+    hipLaunchKernelGGL(k_count_live, dim3((unsigned)nb), dim3(256), 0, c->stream,
+                       mixed ? c->st[c->cur].cell + T.off : nullptr, mixed ? nullptr : c->st[c->cur].rs + T.off,
+                       (long long)h.count, mixed, c->d_live);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(&k, c->d_live, sizeof(k), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
   }
-  *n = k;
+  *n = (int64_t)k;
   return KAOS_OK;
 }
 
@@ -2189,6 +2206,7 @@ extern "C" int kaos_finalize(kaos_ctx* c) {
   for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
   for (auto& ev : c->tune_ev) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
   (void)hipFree(c->d_work);
+  (void)hipFree(c->d_live);
   (void)hipFree(c->d_vbox); (void)hipFree(c->d_hbox);
   (void)hipFree(c->d_vmarg_par); (void)hipFree(c->d_vmarg_perp);
   (void)hipFree(c->d_stepno);
