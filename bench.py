@@ -122,13 +122,20 @@ def timed_run(form, tube, q0, v0, w0, cap, args):
     wall = time.perf_counter() - t1
     push_ms, launches, run_ms = ks.profile_read()
     ks.profile(False)
+    # the reference driver's pattern: one kaos_step per step, each returning its
+    # summary (a synchronisation, the live count and the status)
+    nd = 10
+    t2 = time.perf_counter()
+    for n in range(first + args.steps, first + args.steps + nd):
+        ks.step(n)
+    step_ms = (time.perf_counter() - t2) / nd * 1e3
     mean_push_s = push_ms / 1e3 / max(launches, 1)
     ions_per_launch = pushes / max(launches, 1)
     achieved = BYTES_PER_ION[form] * ions_per_launch / mean_push_s / 1e9
     log(f"[bench] {form}: {args.steps} steps, wall {wall:.3f}s, run events {run_ms/1e3:.3f}s, "
         f"push kernel {push_ms/max(launches,1):.3f} ms x {launches}")
     res = dict(value=pushes / wall, ms_per_step=wall / args.steps * 1e3, launch=ks.get_launch()[:3],
-               upload_s=upload_s,
+               upload_s=upload_s, drop_in_step_ms=step_ms,
                roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS,
                          "traffic": TRAFFIC_BYTES_PER_ION[form] * ions_per_launch,
@@ -358,6 +365,8 @@ def main():
                                f"{args.ions:.0e} O+ ions, dt=0.05 s, every step a master step",
                    "form": forms[0], "ions_end": ions_end},
         "roofline": head["roofline"],
+        "drop_in_step_ms": {f: r[1]["drop_in_step_ms"] for f, r in runs.items()},
+        "steps_run": 1 + args.warmup + args.steps + 10,
     }
     if "mixed" in runs and "fp64" in runs:
         m = runs["mixed"][1]
