@@ -48,6 +48,7 @@ BYTES_PER_ION = {"fp64": 28 + 44, "mixed": 16 + 24}
 # profiles/k_push_1e8_ions_256x64x64_hbm_traffic.txt.
 TRAFFIC_BYTES_PER_ION = {"fp64": 73.2, "mixed": 41.2}
 HBM_PEAK_GBS = 8000.0
+DROP_IN_STEPS = 10          # kaos_step calls with a summary each, after the timed steps
 MOMENTS = ("M0", "M1perp", "M1par", "M2", "M2perp", "M2par", "EA")
 
 
@@ -124,7 +125,7 @@ def timed_run(form, tube, q0, v0, w0, cap, args):
     ks.profile(False)
     # the reference driver's pattern: one kaos_step per step, each returning its
     # summary (a synchronisation, the live count and the status)
-    nd = 10
+    nd = DROP_IN_STEPS
     t2 = time.perf_counter()
     for n in range(first + args.steps, first + args.steps + nd):
         ks.step(n)
@@ -337,7 +338,7 @@ def main():
     t0 = time.perf_counter()
     tube = grids.config2(n_total=args.ions)
     q0, v0, w0 = grids.initial_ions(tube, seed=1)
-    n_steps_total = args.warmup + args.steps + 1
+    n_steps_total = args.warmup + args.steps + 1 + DROP_IN_STEPS
     cap = len(q0) + (n_steps_total + 2) * (tube.n_inj_lb + tube.n_inj_ub) + 1024
     log(f"[bench] ions {len(q0)}, inj/step {tube.n_inj_lb}+{tube.n_inj_ub}, setup {time.perf_counter()-t0:.1f}s")
     host_gen_s = time.perf_counter() - t0
@@ -366,7 +367,7 @@ def main():
                    "form": forms[0], "ions_end": ions_end},
         "roofline": head["roofline"],
         "drop_in_step_ms": {f: r[1]["drop_in_step_ms"] for f, r in runs.items()},
-        "steps_run": 1 + args.warmup + args.steps + 10,
+        "steps_run": 1 + args.warmup + args.steps + DROP_IN_STEPS,
     }
     if "mixed" in runs and "fp64" in runs:
         m = runs["mixed"][1]
