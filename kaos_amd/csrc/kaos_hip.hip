@@ -594,19 +594,6 @@ __global__ void k_set_vperp(DevTube T, long long n, const double* v1, const doub
 // that a captured step replays unchanged.
 __global__ void k_step_tick(long long* stepno) { *stepno += 1; }
 
-// Live slots of a tube segment (a dead slot: the dead cell word in the mixed
-// form, rs = 0 in the FP64 form), summed into *out (cleared by the caller).
-__global__ void k_count_live(const uint32_t* cell, const double* rs, long long n, int mixed,
-                             unsigned long long* out) {
-  unsigned int k = 0u;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    k += mixed ? (cell[i] != KAOS_DEAD) : (rs[i] != 0.0);
-  for (int st = 32; st >= 1; st >>= 1) k += __shfl_xor(k, st, 64);
-  if ((threadIdx.x & 63) == 0 && k) atomicAdd(out, (unsigned long long)k);
-}
-
-__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
-
 __global__ void k_fill_u32(unsigned int* p, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long st = (long long)gridDim.x * blockDim.x;
@@ -746,7 +733,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments(const DevTube* tubes, co
     const int* blk_q, const double* d3xC, const double* d3v, const double* dvpar, const double* dvperp,
     const double* vparGC, const double* vperpGC, const unsigned int* hist, const unsigned int* nqoff,
     unsigned int* nqc, double* NqRT, double* M, long long* count, long long* next_id,
-    const long long* inj_cnt, const int* cells, const int4* hbox) {
+    const long long* inj_cnt, const int* cells, const int4* hbox, long long* live, const unsigned long long* esc_acc) {
   __shared__ double s_P[KAOS_MOM_LANES][3];
   __shared__ unsigned int s_pc[KAOS_MOM_LANES];
   __shared__ double s_A[KAOS_MAXV + 1][3];
@@ -819,7 +806,11 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments(const DevTube* tubes, co
     for (int i = 0; i < 6; ++i) Mq[i] = s_M[i];
     kaos_mom_finish2(S2, T.ms, Mq);
     for (int i = 0; i < 6; ++i) M[6 * T.goff + (long long)i * T.nq + Q] = Mq[i];
-    if (Q == 0 && inj_cnt) { count[t] += inj_cnt[t]; next_id[t] += inj_cnt[t]; }
+    if (Q == 0 && inj_cnt) {
+      count[t] += inj_cnt[t]; next_id[t] += inj_cnt[t];
+      // live ions: + this step's injections - this push's removals
+      live[t] += inj_cnt[t] - (long long)(esc_acc[2 * t] + esc_acc[2 * t + 1]);
+    }
   }
 }
 
@@ -844,7 +835,8 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MOM_WAVES_EU) k_moments_wave(
     const int* blk_q, const int* cells, int ncells, const double* d3xC, const double* d3v,
     const double* dvpar, const double* dvperp, const double* vparGC, const double* vperpGC,
     const unsigned int* hist, const unsigned int* nqoff, unsigned int* nqc, double* NqRT, double* M,
-    long long* count, long long* next_id, const long long* inj_cnt, const int4* hbox) {
+    long long* count, long long* next_id, const long long* inj_cnt, const int4* hbox, long long* live,
+    const unsigned long long* esc_acc) {
   const int lane = threadIdx.x & 63;
   const int ci = blockIdx.x * (KAOS_BLOCK / 64) + (threadIdx.x >> 6);
   if (ci >= ncells) return;
@@ -880,7 +872,11 @@ __global__ void __launch_bounds__(KAOS_BLOCK, KAOS_MOM_WAVES_EU) k_moments_wave(
     nqc[T.goff + Q] = tot;
     NqRT[T.goff + Q] = (double)tot * k;
     for (int i = 0; i < 6; ++i) M[6 * T.goff + (long long)i * T.nq + Q] = Mq[i];
-    if (Q == 0 && inj_cnt) { count[t] += inj_cnt[t]; next_id[t] += inj_cnt[t]; }
+    if (Q == 0 && inj_cnt) {
+      count[t] += inj_cnt[t]; next_id[t] += inj_cnt[t];
+      // live ions: + this step's injections - this push's removals
+      live[t] += inj_cnt[t] - (long long)(esc_acc[2 * t] + esc_acc[2 * t + 1]);
+    }
   }
 }
 
@@ -1109,7 +1105,7 @@ struct kaos_ctx {
   struct Launch { int tpb, hwords, persistent; };
   Launch launch = {KAOS_TPB, KAOS_HWORDS, 0};
   unsigned int* d_work = nullptr;
-  unsigned long long* d_live = nullptr;        // kaos_count's device sum
+  long long* d_live = nullptr;                  // live ions per tube (kaos_count)
   int any_icr = 0;                              // some tube has ICR heating on
   int4* d_vbox = nullptr;                       // per tube velocity box of the push's LDS window
   int4* d_hbox = nullptr;                       // per (tube, cell) bounding box of the master step's counts
@@ -1392,7 +1388,7 @@ static int build_device(kaos_ctx* c) {
   HIPCHK(c, hipMemset(c->d_nqc, 0, sizeof(unsigned int) * (size_t)goff));
   HIPCHK(c, hipMemset(c->d_nqoff, 0, sizeof(unsigned int) * (size_t)goff));
   HIPCHK(c, hipMalloc((void**)&c->d_work, sizeof(unsigned int)));
-  HIPCHK(c, hipMalloc((void**)&c->d_live, sizeof(unsigned long long)));
+  HIPCHK(c, up((void**)&c->d_live, zc.data(), sizeof(long long) * nt));
   {
     std::vector<int4> bx(nt);
     for (int t = 0; t < nt; ++t) bx[t] = make_int4(1, c->dt[t].nvpar, 1, c->dt[t].nvperp);
@@ -1512,6 +1508,7 @@ extern "C" int kaos_upload_particles(kaos_ctx* c, int32_t tube, int64_t n, const
   }
   long long cnt = n;
   HIPCHK(c, hipMemcpyAsync(c->d_count + tube, &cnt, sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_live + tube, &cnt, sizeof(long long), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_next_id + tube, &cnt, sizeof(long long), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   (void)hipFree(dq0); (void)hipFree(dv0); (void)hipFree(dr0);
@@ -1567,6 +1564,7 @@ extern "C" int kaos_seed_particles(kaos_ctx* c, int32_t tube, const int64_t* cou
     (void)hipFree(dcoff); (void)hipFree(dtemp);
   }
   HIPCHK(c, hipMemcpyAsync(c->d_count + tube, &n, sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_live + tube, &n, sizeof(long long), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_next_id + tube, &n, sizeof(long long), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   h.count = n;
@@ -1594,12 +1592,12 @@ static void launch_moments(kaos_ctx* c, const long long* inj) {
     hipLaunchKernelGGL(k_moments_wave, dim3((c->n_cells_wave + wpb - 1) / wpb), dim3(KAOS_BLOCK), 0, c->stream,
                        c->d_tubes, c->d_blk_tube, c->d_blk_q, c->d_cells_wave, c->n_cells_wave, c->d_d3x, c->d_d3v,
                        c->d_dvpar, c->d_dvperp, c->d_vparGC, c->d_vperpGC, c->d_hist, c->d_nqoff, c->d_nqc,
-                       c->d_NqRT, c->d_M, c->d_count, c->d_next_id, inj, c->d_hbox);
+                       c->d_NqRT, c->d_M, c->d_count, c->d_next_id, inj, c->d_hbox, c->d_live, c->d_esc_acc);
   if (c->n_cells_block)
     hipLaunchKernelGGL(k_moments, dim3(c->n_cells_block), dim3(KAOS_BLOCK), 0, c->stream, c->d_tubes,
                        c->d_blk_tube, c->d_blk_q, c->d_d3x, c->d_d3v, c->d_dvpar, c->d_dvperp, c->d_vparGC,
                        c->d_vperpGC, c->d_hist, c->d_nqoff, c->d_nqc, c->d_NqRT, c->d_M, c->d_count, c->d_next_id,
-                       inj, c->d_cells_block, c->d_hbox);
+                       inj, c->d_cells_block, c->d_hbox, c->d_live, c->d_esc_acc);
 }
 
 // Velocity boxes of the push's LDS windows from the counts of the master step
@@ -1860,19 +1858,23 @@ extern "C" int kaos_step(kaos_ctx* c, int64_t n, const double* uniforms, int64_t
     HIPCHK(c, hipMemcpyAsync(c->d_uni, uniforms, sizeof(double) * 6 * c->inj_total, hipMemcpyHostToDevice, c->stream));
     d_uni = c->d_uni;
   }
-  long long alive_before = 0;
-  for (int t = 0; t < c->cfg.ntubes; ++t) alive_before += c->ht[t].count;
-  rc = enqueue_step(c, n, d_uni);
+  // the step's kernels as a replay of its captured graph (as kaos_run does),
+  // except with caller uniforms, profiling or tuning on, and at step 1
+  const bool graph = !d_uni && !(c->cfg.flags & KAOS_FLAG_NO_GRAPH) && !c->prof_on && !c->tuning && n > 1;
+  rc = graph ? enqueue_step_graph(c, n) : enqueue_step(c, n, d_uni);
   if (rc) return rc;
   if (out || d_uni) HIPCHK(c, hipStreamSynchronize(c->stream));
   if (out) {
     memset(out, 0, sizeof(*out));
     out->master = c->last_master;
     out->n_injected = is_master(c, n) ? c->inj_total : 0;
+    std::vector<long long> live(c->cfg.ntubes);
+    HIPCHK(c, hipMemcpyAsync(live.data(), c->d_live, sizeof(long long) * live.size(), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     int64_t alive = 0;
-    for (int t = 0; t < c->cfg.ntubes; ++t) { int64_t k = 0; kaos_count(c, t, &k); alive += k; }
+    for (long long k : live) alive += k;
     out->n_alive = alive;
-    (void)alive_before;
     rc = read_status(c, &out->status);
     if (rc) return rc;
     if (out->status) return fail(c, out->status, "NaN in the particle state");
@@ -1909,21 +1911,9 @@ extern "C" int kaos_count(kaos_ctx* c, int32_t tube, int64_t* n) {
   if (!c || !n) return KAOS_E_ARG;
   if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
   if (!c->built) { *n = 0; return KAOS_OK; }
+  long long k = 0;
+  HIPCHK(c, hipMemcpyAsync(&k, c->d_live + tube, sizeof(k), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  const HostTube& h = c->ht[tube];
-  const DevTube& T = c->dt[tube];
-  unsigned long long k = 0ull;
-  if (h.count) {
-    const int mixed = c->cfg.precision == KAOS_MIXED;
-    const long long nb = std::min<long long>((h.count + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, c->stream, c->d_live);
-    hipLaunchKernelGGL(k_count_live, dim3((unsigned)nb), dim3(256), 0, c->stream,
-                       mixed ? c->st[c->cur].cell + T.off : nullptr, mixed ? nullptr : c->st[c->cur].rs + T.off,
-                       (long long)h.count, mixed, c->d_live);
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(&k, c->d_live, sizeof(k), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-  }
   *n = (int64_t)k;
   return KAOS_OK;
 }
