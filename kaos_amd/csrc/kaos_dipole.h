@@ -959,4 +959,107 @@ KAOS_HD void kaos_mom_finish2(const double S[2], double ms, double M[6]) {
   M[5] = (M0 == 0.0) ? 0.0 : (ms / 2.0) * (((2.0 * KAOS_PI) * S[1]) / M0);
 }
 
+/* ------------------------------------------------------------------------- */
+/* The 3-D velocity grid (ION2VPERPflag = 1): Vperp1 x Vperp2 x Vpar cells.   */
+/* Counts N2PerpphReNormT(nn, V1, V2, Vpar) (_8_3_1_IonParticleCounts.f90:    */
+/* 336-361), F = N2PerpphRT / d3v with d3v = dV1 dV2 dVpar                    */
+/* (_8_4_1_IonDistributionFunctions.f90:50-60, _0_0_VelGridGenerator.f90:     */
+/* 395-398), and the moments of NonUniform3DIonIntegratorSub                  */
+/* (_8_5_1_NonUniform3DIonIntegrator.f90:25-48: sum of ((dV1 dV2) dVpar) g,   */
+/* no 2 pi):                                                                  */
+/*   M0 = sum F;  M1Perp1 = sum V1c F / M0;  M1Perp2 = sum V2c F / M0;        */
+/*   M1Par = sum Vparc F / M0 (_8_6_1..3);                                    */
+/*   M2Perp1 = (m/2) sum V1c^2 F / M0, M2Perp2 likewise (uncentred,           */
+/*   PHASEENERGYPERPIONMOMENTCENTERflag = 0, _8_6_5);                         */
+/*   M2 = (m/2) sum (V1c^2 + V2c^2 + (Vparc - M1Par)^2) F / M0 (_8_6_4:82-120) */
+/*   and M2Par = (m/2) sum (Vparc - M1Par)^2 F / M0 (_8_6_6:82-95), each      */
+/*   falling back to the uncentred integrand where the centred one is 0 or   */
+/*   NaN (applied per Vpar column, as the 2-D moments do).                    */
+/* Counts are [Q][V1][V2][Vpar], Vpar fastest; a row is one (V1, V2) pair,    */
+/* r = i1 nv2 + i2.  Summation order (fixed): column b over rows g, g + G,   */
+/* ... (G = kaos_mom3_groups), the G partials added in g order, then the      */
+/* columns by kaos_tree_sum.                                                  */
+
+typedef struct {
+  int nvpar, nv1, nv2;
+  const double *d3v, *dv1, *dv2, *dvpar, *v1GC, *v2GC, *vparGC;
+} kaos_vgrid3_t;
+
+KAOS_HD int kaos_mom3_groups(int nvpar) {
+  int g = KAOS_MOM_LANES / nvpar;
+  return g < 1 ? 1 : g;
+}
+
+/* Column b's partial sums over rows g0, g0 + G, ... within [r0, r1]:
+ * P = {sum w f, sum w V1c f, sum w V2c f, sum w V1c^2 f, sum w V2c^2 f} with
+ * w = (dV1 dV2) dVpar and f = (N k) / d3v, and the partial count.          */
+KAOS_HD void kaos_mom3_part(const kaos_vgrid3_t* v, const uint32_t* cnt, int b, int g0, int G, int r0, int r1,
+                            double k, double P[5], uint32_t* csum) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+  uint32_t cs = 0;
+  const double dvb = v->dvpar[b];
+  int r = r0 + (((g0 - r0) % G) + G) % G;   /* first row >= r0 of this group */
+  for (; r <= r1; r += G) {
+    const int i = r * v->nvpar + b;
+    const uint32_t c = cnt[i];
+    cs += c;
+    if (c != 0u) {
+      const int i1 = r / v->nv2, i2 = r - i1 * v->nv2;
+      double f = ((double)c * k) / v->d3v[i];
+      double w = (v->dv1[i1] * v->dv2[i2]) * dvb;
+      double u1 = v->v1GC[i1], u2 = v->v2GC[i2];
+      s0 = s0 + w * f;
+      s1 = s1 + w * (u1 * f);
+      s2 = s2 + w * (u2 * f);
+      s3 = s3 + w * ((u1 * u1) * f);
+      s4 = s4 + w * ((u2 * u2) * f);
+    }
+  }
+  P[0] = s0; P[1] = s1; P[2] = s2; P[3] = s3; P[4] = s4;
+  *csum = cs;
+}
+
+/* Column values of the first stage: [A0, A1, A2, uc A0, A3, A4]. */
+KAOS_HD void kaos_mom3_col_first(double uc, const double A[5], double o[6]) {
+  o[0] = A[0]; o[1] = A[1]; o[2] = A[2]; o[3] = uc * A[0]; o[4] = A[3]; o[5] = A[4];
+}
+
+/* M[8] = M0, M1Perp1, M1Perp2, M1Par, M2, M2Perp1, M2Perp2, M2Par: the
+ * first-stage ones from the summed column values S[6].                     */
+KAOS_HD void kaos_mom3_finish1(const double S[6], double ms, double M[8]) {
+  double M0 = S[0];
+  M[0] = M0;
+  M[1] = (M0 == 0.0) ? 0.0 : S[1] / M0;
+  M[2] = (M0 == 0.0) ? 0.0 : S[2] / M0;
+  M[3] = (M0 == 0.0) ? 0.0 : S[3] / M0;
+  M[5] = (M0 == 0.0) ? 0.0 : (ms / 2.0) * (S[4] / M0);
+  M[6] = (M0 == 0.0) ? 0.0 : (ms / 2.0) * (S[5] / M0);
+}
+
+/* Column values of the second stage (centred by M1Par): [M2, M2Par] integrands. */
+KAOS_HD void kaos_mom3_col_second(double uc, const double A[5], double M1par, double o[2]) {
+  double d = uc - M1par;
+  double c2 = (A[3] + A[4]) + (d * d) * A[0];
+  if (c2 != c2 || c2 == 0.0) c2 = (A[3] + A[4]) + (uc * uc) * A[0];
+  double cp = (d * d) * A[0];
+  if (cp != cp || cp == 0.0) cp = (uc * uc) * A[0];
+  o[0] = c2; o[1] = cp;
+}
+
+KAOS_HD void kaos_mom3_finish2(const double S[2], double ms, double M[8]) {
+  double M0 = M[0];
+  M[4] = (M0 == 0.0) ? 0.0 : (ms / 2.0) * (S[0] / M0);
+  M[7] = (M0 == 0.0) ? 0.0 : (ms / 2.0) * (S[1] / M0);
+}
+
+/* The three velocity cells of an ion on the 3-D grid (KUA:985-1080): cell 1
+ * is [GL, GH], the others (GL, GH]; 0 when off the grid (no stale index).  */
+KAOS_HD void kaos_vcells3(double vpar, double v1, double v2, const double* vparGH, double vparGL0, int nvpar,
+                          const double* v1GH, double v1GL0, int nv1, const double* v2GH, double v2GL0, int nv2,
+                          int* ip, int* i1, int* i2) {
+  *ip = kaos_find_vcell(vpar, vparGH, vparGL0, nvpar);
+  *i1 = kaos_find_vcell(v1, v1GH, v1GL0, nv1);
+  *i2 = kaos_find_vcell(v2, v2GH, v2GL0, nv2);
+}
+
 #endif /* KAOS_DIPOLE_H */

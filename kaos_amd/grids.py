@@ -79,6 +79,10 @@ class Tube:
     icr_f0: float = 6.5
     icr_chi_perp1: float = 2.1
     icr_chi_perp2: float = 2.1
+    # 3-D velocity grid (ION2VPERPflag = 1), set by with_vgrid3
+    v3d: int = 0
+    nv1: int = 0
+    nv2: int = 0
 
     @property
     def cphi(self):
@@ -103,6 +107,24 @@ def velocity_grid(nvpar, nvperp, vpar_max=1.0e4, vperp_max=1.0e4):
     d3v = ((np.abs(vperpGC)[:, None] * dvperp[:, None]) * dvpar[None, :]) * (2.0 * math.pi)
     return dict(vparGL=vparGL, vparGH=vparGH, vparGC=vparGC, dvpar=dvpar, vperpGL=vperpGL,
                 vperpGH=vperpGH, vperpGC=vperpGC, dvperp=dvperp, d3v=np.ascontiguousarray(d3v))
+
+
+def with_vgrid3(tube, nv1, nv2, vperp_max=None):
+    """Give a tube the 3-D velocity grid of ION2VPERPflag = 1: Vperp1 and
+    Vperp2 uniform in [-vperp_max, vperp_max] (default: the tube's Vpar range)
+    with contiguous edges, the tube's Vpar cells, and d3v = (dV1 dV2) dVpar
+    as [V1][V2][Vpar] (_0_0_VelGridGenerator.f90:384-398).  Returns the tube."""
+    a = tube.arrays
+    vmax = float(a["vparGH"][-1]) if vperp_max is None else vperp_max
+    for name, n in (("v1", nv1), ("v2", nv2)):
+        e = -vmax + (2.0 * vmax) * np.arange(n + 1) / n
+        a[name + "GL"], a[name + "GH"] = e[:-1].copy(), e[1:].copy()
+        a[name + "GC"] = (a[name + "GL"] + a[name + "GH"]) / 2.0
+        a["d" + name] = a[name + "GH"] - a[name + "GL"]
+    d3 = ((a["dv1"][:, None, None] * a["dv2"][None, :, None]) * a["dvpar"][None, None, :])
+    a["d3v3"] = np.ascontiguousarray(d3)
+    tube.v3d, tube.nv1, tube.nv2 = 1, nv1, nv2
+    return tube
 
 
 def dipole_tube(nq=64, p=4.0, r_lb=1.05, r_ub=3.0, ratio=1.05, south=False, nvpar=40, nvperp=40,

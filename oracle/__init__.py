@@ -19,6 +19,8 @@ ICR_FIELDS = ("icr_lambda_perp", "icr_eta_lh", "icr_xi_perp1", "icr_xi_perp2", "
               "icr_chi_perp1", "icr_chi_perp2")
 ARRAYS = ("qGL", "qGH", "qGC", "hqC", "dqC", "d3xC", "rGC", "cthGC", "ellGC", "Te",
           "vparGL", "vparGH", "vparGC", "dvpar", "vperpGL", "vperpGH", "vperpGC", "dvperp", "d3v")
+MOMENTS3 = ("M0", "M1perp1", "M1perp2", "M1par", "M2", "M2perp1", "M2perp2", "M2par")
+ARRAYS3 = ("v1GL", "v1GH", "v1GC", "dv1", "v2GL", "v2GH", "v2GC", "dv2", "d3v3")
 
 
 class OrcCfg(C.Structure):
@@ -31,7 +33,9 @@ class OrcCfg(C.Structure):
                 + [(n, _dp) for n in ARRAYS]
                 + [("icr", C.c_int)]
                 + [(n, C.c_double) for n in ICR_FIELDS]
-                + [("filter_pt_m1par", C.c_int)])
+                + [("filter_pt_m1par", C.c_int)]
+                + [("v3d", C.c_int), ("nv1", C.c_int), ("nv2", C.c_int)]
+                + [(n, _dp) for n in ARRAYS3])
 
 
 _lib = None
@@ -115,6 +119,7 @@ def load():
         lib.orc_seed_ions.restype = C.c_int64
         lib.orc_seed_uniforms.argtypes = [C.c_uint64, C.c_int, C.c_int64, _dp]
         lib.orc_get_m1f.argtypes = [P, _dp]
+        lib.orc_get_master3.argtypes = [P, _dp, _dp]
         _lib = lib
     return _lib
 
@@ -134,10 +139,13 @@ def cfg_of(tube, ndatfac=1, seed=1, tube_index=0):
     c.seed = seed
     c.icr = getattr(tube, "icr", 0)
     c.filter_pt_m1par = getattr(tube, "filter_pt_m1par", 0)
+    c.v3d = getattr(tube, "v3d", 0)
+    if c.v3d:
+        c.nv1, c.nv2 = tube.nv1, tube.nv2
     for n in ICR_FIELDS:
         setattr(c, n, getattr(tube, n, 0.0))
     keep = []
-    for n in ARRAYS:
+    for n in ARRAYS + (ARRAYS3 if c.v3d else ()):
         a = np.ascontiguousarray(tube.arrays[n], dtype=np.float64)
         keep.append(a)
         setattr(c, n, ptr(a))
@@ -205,6 +213,17 @@ class Oracle:
         M1F = np.zeros(nq)
         self.lib.orc_get_m1f(self.h, ptr(M1F))
         out["M1F"] = M1F
+        return out
+
+    def master3(self):
+        """3-D outputs of the last master step: counts [Q][V1][V2][Vpar] and the
+        8 moments (M0, M1Perp1, M1Perp2, M1Par, M2, M2Perp1, M2Perp2, M2Par)."""
+        t = self.tube
+        n3, M8 = np.zeros(t.nq * t.nv1 * t.nv2 * t.nvpar), np.zeros(8 * t.nq)
+        self.lib.orc_get_master3(self.h, ptr(n3), ptr(M8))
+        out = dict(n3_counts=n3.astype(np.int64).reshape(t.nq, t.nv1, t.nv2, t.nvpar))
+        for k, n in enumerate(MOMENTS3):
+            out[n] = M8[k * t.nq:(k + 1) * t.nq].copy()
         return out
 
     def close(self):

@@ -52,6 +52,10 @@ typedef struct {
   double icr_lambda_perp, icr_eta_lh, icr_xi_perp1, icr_xi_perp2, icr_s0, icr_f0, icr_chi_perp1,
       icr_chi_perp2;
   int filter_pt_m1par;            /* M1ParMAfilterPt; 0 = filter_pt */
+  /* 3-D velocity grid (ION2VPERPflag = 1): Vperp1, Vperp2 cells; d3v3 is
+   * [i1][i2][ipar] (Vpar fastest), dV1 dV2 dVpar */
+  int v3d, nv1, nv2;
+  const double *v1GL, *v1GH, *v1GC, *dv1, *v2GL, *v2GH, *v2GC, *dv2, *d3v3;
 } orc_cfg;
 
 typedef struct {
@@ -71,6 +75,9 @@ typedef struct {
   double *Mprev;                   /* moments of the previous master step (6*nq) */
   double *EA, *EG, *M0F;           /* EAmagRT(nn), EGmagRT(nn), M0FiltAvrgRT(nn) */
   double *M1F;                     /* M1ParFiltAvrgRT(nn) */
+  int *i1ind, *i2ind, *ip3ind;     /* 3-D velocity cells (Vperp1indk1, Vperp2indk1, Vparindk1) */
+  uint32_t *n3_u;                  /* N2PerpphReNormT, [Q][i1][i2][ipar] */
+  double *M3;                      /* 8*nq: M0, M1Perp1, M1Perp2, M1Par, M2, M2Perp1, M2Perp2, M2Par */
   double esc_lb, esc_ub, flux_lb, flux_ub;
   int64_t injected;
 } orc_t;
@@ -220,10 +227,86 @@ static void counts_and_moments(orc_t *o, const int *qi, const int *vpi, const in
   free(x);
 }
 
+/* 3-D velocity scan, _8_1_1_KineticUpdateA.f90:985-1080: 0 outside the grid
+ * (no stale index), else the first cell: cell 1 [GL,GH], others (GL,GH]. */
+static int scan_vcell3(const double *GL, const double *GH, int n, double v) {
+  if (GL[0] > v || v > GH[n - 1]) return 0;
+  for (int i = 0; i < n; ++i) {
+    if (i == 0 && GL[i] <= v && v <= GH[i]) return 1;
+    if (i != 0 && GL[i] < v && v <= GH[i]) return i + 1;
+  }
+  return 0;
+}
+
+/* IonParticleCountsSub (3-D branch, _8_3_1:300-361) and the moments of
+ * _8_6_1.._8_6_6 (3-D branches) in the fixed order of kaos_mom3_part. */
+static void counts3_and_moments(orc_t *o, int64_t ns) {
+  orc_cfg *c = &o->c;
+  int nq = c->nq, np = c->nvpar, n1 = c->nv1, n2 = c->nv2;
+  int64_t ncell = (int64_t)n1 * n2 * np;
+  memset(o->nq_u, 0, sizeof(uint32_t) * nq);
+  memset(o->n3_u, 0, sizeof(uint32_t) * nq * ncell);
+  for (int64_t j = 0; j < ns; ++j) {
+    int Q = o->qind[j];
+    if (Q < 1 || Q > nq) continue;
+    o->nq_u[Q - 1] += 1;
+    int a1 = o->i1ind[j], a2 = o->i2ind[j], b = o->ip3ind[j];
+    if (a1 >= 1 && a2 >= 1 && b >= 1)
+      o->n3_u[(int64_t)(Q - 1) * ncell + ((int64_t)(a1 - 1) * n2 + (a2 - 1)) * np + (b - 1)] += 1;
+  }
+  for (int Q = 0; Q < nq; ++Q) o->nq_counts[Q] = (double)o->nq_u[Q];
+  kaos_vgrid3_t v = { np, n1, n2, c->d3v3, c->dv1, c->dv2, c->dvpar, c->v1GC, c->v2GC, c->vparGC };
+  const int G = kaos_mom3_groups(np);
+  double *A = (double *)xc(sizeof(double) * 5 * np), *x = (double *)xc(sizeof(double) * 8 * np);
+  for (int Q = 0; Q < nq; ++Q) {
+    double k = c->nsnormfac / c->d3xC[Q];
+    o->NqRT[Q] = o->nq_counts[Q] * k;
+    const uint32_t *N = o->n3_u + (int64_t)Q * ncell;
+    for (int b = 0; b < np; ++b) {
+      double P[5];
+      uint32_t cs;
+      kaos_mom3_part(&v, N, b, 0, G, 0, n1 * n2 - 1, k, &A[5 * b], &cs);
+      for (int g = 1; g < G; ++g) {
+        kaos_mom3_part(&v, N, b, g, G, 0, n1 * n2 - 1, k, P, &cs);
+        for (int t = 0; t < 5; ++t) A[5 * b + t] = A[5 * b + t] + P[t];
+      }
+      double o6[6];
+      kaos_mom3_col_first(c->vparGC[b], &A[5 * b], o6);
+      for (int t = 0; t < 6; ++t) x[t * np + b] = o6[t];
+    }
+    double S[6], M8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int t = 0; t < 6; ++t) { kaos_tree_sum(&x[t * np], np); S[t] = x[t * np]; }
+    kaos_mom3_finish1(S, c->ms, M8);
+    for (int b = 0; b < np; ++b) {
+      double o2[2];
+      kaos_mom3_col_second(c->vparGC[b], &A[5 * b], M8[3], o2);
+      x[6 * np + b] = o2[0]; x[7 * np + b] = o2[1];
+    }
+    double S2[2];
+    for (int t = 0; t < 2; ++t) { kaos_tree_sum(&x[(6 + t) * np], np); S2[t] = x[(6 + t) * np]; }
+    kaos_mom3_finish2(S2, c->ms, M8);
+    for (int t = 0; t < 8; ++t) o->M3[t * nq + Q] = M8[t];
+    /* the 2-D moment block of this master step: the quantities both branches
+     * define (M0, M1Par, M2, M2Par; M1Perp and M2Perp are 2-D only) */
+    o->M[0 * nq + Q] = M8[0]; o->M[1 * nq + Q] = 0.0; o->M[2 * nq + Q] = M8[3];
+    o->M[3 * nq + Q] = M8[4]; o->M[4 * nq + Q] = 0.0; o->M[5 * nq + Q] = M8[7];
+  }
+  free(A);
+  free(x);
+}
+
+/* The 3-D velocity cells of ion j at velocity (vpar, v1, v2). */
+static void vcells3(orc_t *o, int64_t j, double vpar, double v1, double v2) {
+  orc_cfg *c = &o->c;
+  o->ip3ind[j] = scan_vcell3(c->vparGL, c->vparGH, c->nvpar, vpar);
+  o->i1ind[j] = scan_vcell3(c->v1GL, c->v1GH, c->nv1, v1);
+  o->i2ind[j] = scan_vcell3(c->v2GL, c->v2GH, c->nv2, v2);
+}
+
 /* ---- state management ------------------------------------------------------ */
 
 #define ARR_D(X) X(rs) X(vs) X(vperp) X(aea) X(ag) X(aeaN) X(agN) X(qk4) X(q0) X(vpar0) X(v1) X(v2)
-#define ARR_I(X) X(qind) X(vparind) X(vperpind)
+#define ARR_I(X) X(qind) X(vparind) X(vperpind) X(i1ind) X(i2ind) X(ip3ind)
 #define ARR_L(X) X(id)
 
 void *orc_create(const orc_cfg *cfg, int64_t ns, const double *q0, const double *vpar0,
@@ -253,6 +336,10 @@ void *orc_create(const orc_cfg *cfg, int64_t ns, const double *q0, const double 
   o->EG = (double *)xc(sizeof(double) * nq);
   o->M0F = (double *)xc(sizeof(double) * nq);
   o->M1F = (double *)xc(sizeof(double) * nq);
+  if (cfg->v3d) {
+    o->n3_u = (uint32_t *)xc(sizeof(uint32_t) * nq * (int64_t)cfg->nv1 * cfg->nv2 * cfg->nvpar);
+    o->M3 = (double *)xc(sizeof(double) * 8 * nq);
+  }
   /* initial conditions: x0 = root(q0), V0 = Vpar0 b-hat(x0) */
   for (int64_t j = 0; j < ns; ++j) {
     o->rs[j] = kaos_rs_of_q(q0[j], cfg->p);
@@ -275,7 +362,7 @@ void orc_destroy(void *h) {
   ARR_D(FR) ARR_I(FR) ARR_L(FR)
 #undef FR
   free(o->nq_counts); free(o->nph_counts); free(o->nq_u); free(o->nph_u); free(o->NqRT); free(o->M); free(o->Mprev);
-  free(o->EA); free(o->EG); free(o->M0F); free(o->M1F);
+  free(o->EA); free(o->EG); free(o->M0F); free(o->M1F); free(o->n3_u); free(o->M3);
   free(o);
 }
 
@@ -331,6 +418,7 @@ static int inject(orc_t *o, int64_t n, const double *uni) {
     o->id[j] = o->next_id + k;
     o->vparind[j] = scan_vcell(c->vparGL, c->vparGH, c->nvpar, vpar, 0);
     o->vperpind[j] = scan_vcell(c->vperpGL, c->vperpGH, c->nvperp, vperp, 0);
+    if (c->v3d) vcells3(o, j, vpar, v1, v2);   /* indices at injection, KSB:339-476 */
     o->aea[j] = o->aeaN[j] = fabs(qm * fabs(o->EA[Qb]));
     o->ag[j] = o->agN[j] = fabs(qm * fabs(o->EG[Qb]));
     o->qk4[j] = qc;
@@ -373,6 +461,7 @@ static void push_all(orc_t *o, int64_t n) {
     o->qind[j] = qs;
     o->vparind[j] = scan_vcell(c->vparGL, c->vparGH, c->nvpar, r.vpar, o->vparind[j]);
     o->vperpind[j] = scan_vcell(c->vperpGL, c->vperpGH, c->nvperp, o->vperp[j], o->vperpind[j]);
+    if (c->v3d) vcells3(o, j, r.vpar, o->v1[j], o->v2[j]);   /* pre-kick components */
     if (c->icr && qs != 0 && qs != -1) {
       double u[4], a1, a2, ap;
       kaos_icr_uniforms(c->seed, n, c->tube, o->id[j], u);
@@ -407,8 +496,10 @@ int orc_step(void *h, int64_t n, const double *uniforms) {
       o->qind[j] = scan_qcell(c, q, 0);
       o->vparind[j] = scan_vcell(c->vparGL, c->vparGH, c->nvpar, o->vpar0[j], 0);
       o->vperpind[j] = scan_vcell(c->vperpGL, c->vperpGH, c->nvperp, o->vperp[j], 0);
+      if (c->v3d) vcells3(o, j, o->vpar0[j], o->v1[j], o->v2[j]);   /* Vparindk0, Vperp1/2indk0 */
     }
-    counts_and_moments(o, o->qind, o->vparind, o->vperpind, o->ns);
+    if (c->v3d) counts3_and_moments(o, o->ns);
+    else counts_and_moments(o, o->qind, o->vparind, o->vperpind, o->ns);
     memset(o->Mprev, 0, sizeof(double) * 6 * nq);   /* moments of master step 0: none */
     ambipolar_field(c, o->Mprev, o->M0F, o->EA);
     m1par_filter(c, o->Mprev, o->M1F);
@@ -439,7 +530,8 @@ int orc_step(void *h, int64_t n, const double *uniforms) {
   if (n > 1 && o->master_last) {
     int rc = inject(o, n, uniforms);
     if (rc) return rc;
-    counts_and_moments(o, o->qind, o->vparind, o->vperpind, o->ns);
+    if (c->v3d) counts3_and_moments(o, o->ns);
+    else counts_and_moments(o, o->qind, o->vparind, o->vperpind, o->ns);
     memcpy(o->Mprev, o->M, sizeof(double) * 6 * nq);
     o->nn = n / c->ndatfac + 1;
   }
@@ -513,6 +605,16 @@ void orc_get_master(void *h, double *nq_counts, double *nph_counts, double *NqRT
   if (EG) memcpy(EG, o->EG, sizeof(double) * nq);
   if (M0F) memcpy(M0F, o->M0F, sizeof(double) * nq);
   if (esc) { esc[0] = o->esc_lb; esc[1] = o->esc_ub; esc[2] = o->flux_lb; esc[3] = o->flux_ub; }
+}
+
+/* The 3-D outputs of the last master step: counts [Q][i1][i2][ipar] and the
+ * 8 moments (M0, M1Perp1, M1Perp2, M1Par, M2, M2Perp1, M2Perp2, M2Par). */
+void orc_get_master3(void *h, double *n3, double *M8) {
+  orc_t *o = (orc_t *)h;
+  orc_cfg *c = &o->c;
+  int64_t n = (int64_t)c->nq * c->nv1 * c->nv2 * c->nvpar;
+  if (n3) for (int64_t i = 0; i < n; ++i) n3[i] = (double)o->n3_u[i];
+  if (M8) memcpy(M8, o->M3, sizeof(double) * 8 * c->nq);
 }
 
 void orc_get_m1f(void *h, double *M1F) {
