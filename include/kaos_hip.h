@@ -55,6 +55,9 @@ typedef struct kaos_cfg {
                                       captured hipGraph of the step's kernels */
 #define KAOS_FLAG_NO_AUTOTUNE  2   /* keep the default launch shape instead of timing candidates
                                       on the first master steps */
+#define KAOS_FLAG_VPERP3D      4   /* ION2VPERPflag = 1: count on the 3-D (Vperp1, Vperp2, Vpar)
+                                      grid of every tube and compute the 3-D moments
+                                      (kaos_download_master3d) instead of the 2-D (Vperp, Vpar) ones */
 
 /* One flux tube: grid, species and flags.  Replaces the FluxTubeT/QCellT/VCellT
  * grid fields (_KineticMainParams.f90:397-520) and the per-tube flags. */
@@ -101,6 +104,15 @@ typedef struct kaos_grid {
   double icr_f0;           /* f0p [Hz] (OmegaG0p = 2 pi f0p) */
   double icr_chi_perp1;    /* ChiPerp1p */
   double icr_chi_perp2;    /* ChiPerp2p */
+  /* 3-D velocity grid (ION2VPERPflag = 1; used when the context has
+   * KAOS_FLAG_VPERP3D): Vperp1 cells (Vperp1GLT/GHT/GCT/dVperp1GT, nvperp1),
+   * Vperp2 cells (nvperp2), contiguous edges, and d3v3 = dV1 dV2 dVpar
+   * (d3vCTp) as [Vperp1ind-1][Vperp2ind-1][Vparind-1], Vpar fastest
+   * (_0_0_VelGridGenerator.f90:384-398).  The Vpar cells are the ones above. */
+  int32_t nvperp1, nvperp2;
+  const double *vperp1GL, *vperp1GH, *vperp1GC, *dvperp1;
+  const double *vperp2GL, *vperp2GH, *vperp2GC, *dvperp2;
+  const double *d3v3;
 } kaos_grid;
 
 /* Per-step summary. */
@@ -186,6 +198,15 @@ int kaos_run(kaos_ctx* ctx, int64_t n_first, int64_t nsteps, int64_t* particle_s
 int kaos_download_master(kaos_ctx* ctx, int32_t tube, int64_t* nq_counts, int64_t* nph_counts,
                          double* NqRT, double* M, double* EA, double* EG, double* esc);
 
+/* The 3-D outputs of the last master step of a tube (KAOS_FLAG_VPERP3D):
+ * n3_counts (nq*nvperp1*nvperp2*nvpar, [Q][V1][V2][Vpar], Vpar fastest):
+ * N2PerpphReNormRT (_8_3_1_IonParticleCounts.f90:336-361); M (8*nq): M0phRT,
+ * M1Perp1phRT, M1Perp2phRT, M1ParphRT, M2phRT, M2Perp1phRT, M2Perp2phRT,
+ * M2ParphRT (3-D branches of _8_6_1.._8_6_6).  Either pointer may be NULL.
+ * In this mode kaos_download_master's M holds M0, 0, M1Par, M2, 0, M2Par
+ * (M1Perp and M2Perp are 2-D quantities) and its nph_counts are zero. */
+int kaos_download_master3d(kaos_ctx* ctx, int32_t tube, int64_t* n3_counts, double* M);
+
 /* What the export files hold that the hot path does not compute. */
 typedef struct kaos_export_info {
   const char* dir;       /* dataexportdir; a '/' is put between it and the names if missing */
@@ -211,8 +232,12 @@ typedef struct kaos_export_info {
  * is 0); M0phRT, M1PerpphRT, M1ParphRT, M2phRT, M2PerpphRT, M2ParphRT;
  * M0FiltAvrgRT, M1ParFiltAvrgRT and M1Perp1/M1Perp2/M2ParFiltAvrgRT (zeros:
  * only charge exchange sets them); EAInertialRT (zeros), EAPressureRT,
- * EAmagRT, EGmagRT; EPmagRT and PhiParRT (zeros).  nn_out (may be NULL)
- * receives nn.  KAOS_E_IO if a file cannot be written. */
+ * EAmagRT, EGmagRT; EPmagRT and PhiParRT (zeros).  With KAOS_FLAG_VPERP3D
+ * (ION2VPERP on, _8_0_2_DataExport2.f90:123-161, 276-307, 446-490) the cell
+ * files are N2PerpphRT ((V1, V2, Vpar) column-major), TeNT and d3vCTp instead
+ * of NphRT, and M1Perp1/M1Perp2phRT, M2Perp1/M2Perp2phRT replace M1PerpphRT,
+ * M2PerpphRT.  nn_out (may be NULL) receives nn.  KAOS_E_IO if a file cannot
+ * be written. */
 int kaos_export_master(kaos_ctx* ctx, int32_t tube, const kaos_export_info* info, int64_t* nn_out);
 
 /* Number of live ions of a tube. */

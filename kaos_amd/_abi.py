@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libkaos_hip.so")
 
 KAOS_OK, KAOS_E_ARG, KAOS_E_CAPACITY, KAOS_E_DEVICE, KAOS_E_STATE, KAOS_E_NAN, KAOS_E_IO = range(7)
 KAOS_FP64, KAOS_MIXED = 0, 1
-KAOS_FLAG_NO_GRAPH, KAOS_FLAG_NO_AUTOTUNE = 1, 2
+KAOS_FLAG_NO_GRAPH, KAOS_FLAG_NO_AUTOTUNE, KAOS_FLAG_VPERP3D = 1, 2, 4
 
 _dp = C.POINTER(C.c_double)
 
@@ -29,6 +29,11 @@ GRID_ARRAYS = ("qGL", "qGH", "qGC", "hqC", "dqC", "d3xC", "rGC", "cthGC", "ellGC
                "d3v")
 
 
+# 3-D velocity grid: kaos_grid field name -> grids.Tube array name
+GRID_ARRAYS3 = ("vperp1GL", "vperp1GH", "vperp1GC", "dvperp1", "vperp2GL", "vperp2GH", "vperp2GC", "dvperp2",
+                "d3v3")
+TUBE_ARRAYS3 = ("v1GL", "v1GH", "v1GC", "dv1", "v2GL", "v2GH", "v2GC", "dv2", "d3v3")
+
 ICR_FIELDS = ("icr_lambda_perp", "icr_eta_lh", "icr_xi_perp1", "icr_xi_perp2", "icr_s0", "icr_f0",
               "icr_chi_perp1", "icr_chi_perp2")
 
@@ -41,7 +46,9 @@ class KaosGrid(C.Structure):
                                              "Ts_lb", "Ts_ub")]
                 + [(n, _dp) for n in GRID_ARRAYS]
                 + [("icr", C.c_int32), ("pad1", C.c_int32)]
-                + [(n, C.c_double) for n in ICR_FIELDS])
+                + [(n, C.c_double) for n in ICR_FIELDS]
+                + [("nvperp1", C.c_int32), ("nvperp2", C.c_int32)]
+                + [(n, _dp) for n in GRID_ARRAYS3])
 
 
 class KaosExportInfo(C.Structure):
@@ -56,7 +63,7 @@ class KaosStepOut(C.Structure):
 
 EXPORTS = ("kaos_init", "kaos_set_grid", "kaos_upload_particles", "kaos_upload_vperp", "kaos_cell_counts",
            "kaos_seed_particles", "kaos_step", "kaos_run",
-           "kaos_download_master", "kaos_export_master", "kaos_count", "kaos_download_particles", "kaos_set_launch",
+           "kaos_download_master", "kaos_download_master3d", "kaos_export_master", "kaos_count", "kaos_download_particles", "kaos_set_launch",
            "kaos_get_launch", "kaos_profile", "kaos_profile_read", "kaos_last_error", "kaos_finalize")
 
 _lib = None
@@ -83,6 +90,7 @@ def load(path=LIB_PATH):
     lib.kaos_run.argtypes = [P, C.c_int64, C.c_int64, i64p]
     lib.kaos_download_master.argtypes = [P, C.c_int32, i64p, i64p, _dp, _dp, _dp, _dp, _dp]
     lib.kaos_export_master.argtypes = [P, C.c_int32, C.POINTER(KaosExportInfo), i64p]
+    lib.kaos_download_master3d.argtypes = [P, C.c_int32, i64p, _dp]
     lib.kaos_count.argtypes = [P, C.c_int32, i64p]
     lib.kaos_download_particles.argtypes = [P, C.c_int32, C.c_int64, i64p] + [_dp] * 10 + [i32p] * 3
     lib.kaos_set_launch.argtypes = [P, C.c_int32, C.c_int32, C.c_int32]

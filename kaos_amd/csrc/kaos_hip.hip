@@ -64,6 +64,8 @@ struct DevTube {
   long long koff;                           // offset of the tube's nq + 2 reorder keys
   int icr, pad2;                            // ICR wave heating on; its parameters
   kaos_icr_t icrp;
+  int nv1, nv2;                             // 3-D velocity grid: Vperp1 / Vperp2 cells (0: 2-D grid)
+  long long v1off, v2off, h3off, d3off;     // Vperp1 / Vperp2 edges, 3-D counts, d3v3 offsets
 };
 
 struct StepArgs {
@@ -99,6 +101,10 @@ struct StepArgs {
   // blocks taking groups from the counter `work` (else one group per block)
   int hwords, persistent, ngroups;
   unsigned int* work;
+  // 3-D velocity grid (null in the 2-D form): Vperp1 / Vperp2 high edges (GL0
+  // appended), counts [Q][V1][V2][Vpar] per tube
+  const double *v1GH, *v2GH;
+  unsigned int* hist3;
 };
 
 __device__ __forceinline__ void set_status(unsigned int* st, unsigned int bit) { atomicOr(st, bit); }
@@ -508,6 +514,66 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_bin_initial(StepArgs a) {
   }
 }
 
+// Counts on the 3-D velocity grid (ION2VPERPflag = 1; IonParticleCountsSub's
+// 3-D branch, _8_3_1_IonParticleCounts.f90:300-361) of a master step, as a
+// pass over the pre-push state before the push and before the FP64 heating
+// pass: the cells are taken from the pre-kick perpendicular components
+// (KUA:985-1080, ahead of WaveHeatingSub at KUA:1392).  Per live ion the
+// config cell and V|| are the push's stage-A ones (the same shared functions
+// on the same binary64 state; the mixed form decodes its words first); at the
+// first step (first = 1) the cells of the initial ions (KSA:49-326): the cell
+// word's config and Vpar cells, found from the clamped q0 and Vpar0.  Ions
+// the master push removes (previous cell 0 / -1) are not counted.  The three
+// velocity cells are 0 off the grid (no stale index); an ion in a config
+// cell with some velocity cell 0 counts in the config counts only.
+__global__ void __launch_bounds__(KAOS_BLOCK) k_bin3d(StepArgs a, int first, int mixed) {
+  const int tile = blockIdx.x;
+  if (tile >= a.ntiles) return;
+  const int tb = a.tile_tube[tile];
+  const DevTube& T = a.tubes[tb];
+  const long long end = T.off + a.count[tb];
+  const double* vGH = a.vparGH + T.voff;
+  const double* v1GH = a.v1GH + T.v1off;
+  const double* v2GH = a.v2GH + T.v2off;
+  for (int it = 0; it < KAOS_TILE_PPT; ++it) {
+    const long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
+    if (j >= end) break;
+    const uint32_t cw = a.cell[j];
+    if (mixed ? (cw == KAOS_DEAD) : (a.rs[j] == 0.0)) continue;
+    int qc = (int)((cw >> 16) & KAOS_QMASK) - 1;
+    int ip;
+    if (first) {
+      ip = (int)(cw & 0xFFu);
+    } else {
+      if (qc == 0 || qc == -1) continue;
+      double rs, vs;
+      if (mixed) {
+        const int Q = kx_refq(qc, T.nq);
+        rs = a.rref[T.goff + Q - 1] + kx_dec(a.rdi[j], a.rk[T.goff + Q - 1], (cw & KAOS_RFLAG) != 0);
+        vs = kx_dec_v(a.vpi[j], (int)(cw & 0xFFu), (cw & KAOS_VFLAG) != 0, vGH, a.vk + T.voff, T.nvpar);
+      } else {
+        rs = a.rs[j];
+        vs = a.vp[j];
+      }
+      const kaos_line_t L = kaos_line(rs, T.p);
+      const kaos_qb_t gA = kaos_stageA(L);
+      qc = kaos_find_qcell(gA.q, a.qGH + T.goff, T.qGL0, T.qGHlast, T.nq, T.desc);
+      const double vpar = gA.bR * (vs * gA.bR) + gA.bZ * (vs * gA.bZ);
+      ip = kaos_find_vcell(vpar, vGH, vGH[T.nvpar], T.nvpar);
+    }
+    if (qc < 1) continue;
+    const double v1 = mixed ? (double)a.vrf[j] : a.vr[j];
+    const double v2 = mixed ? (double)a.vrf2[j] : a.vr2[j];
+    const int i1 = kaos_find_vcell(v1, v1GH, v1GH[T.nv1], T.nv1);
+    const int i2 = kaos_find_vcell(v2, v2GH, v2GH[T.nv2], T.nv2);
+    if (ip >= 1 && i1 >= 1 && i2 >= 1)
+      atomicAdd(&a.hist3[T.h3off + (((long long)(qc - 1) * T.nv1 + (i1 - 1)) * T.nv2 + (i2 - 1)) * T.nvpar +
+                         (ip - 1)], 1u);
+    else
+      atomicAdd(&a.nqoff[T.goff + qc - 1], 1u);
+  }
+}
+
 // State of one initial ion in slot j: signed radius rs at dipole coordinate
 // q, velocity (Vpar, Vperp1, Vperp2), |Vperp| = vperp.  The k0 cells
 // (KSA:49-326) come from q clamped into the grid as KSA:55-129 does, and from
@@ -685,6 +751,19 @@ __global__ void k_inject(StepArgs a, const long long* inj_base, const long long*
   }
   a.cell[j] = ((uint32_t)(qcell + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip | flags;
   a.id[j] = (int)(next_id[t] + k);
+  if (a.hist3) {
+    // 3-D grid: the cells of the injected velocity (KSB:339-476), 0 off the grid
+    if (qcell >= 1) {
+      const int i1 = kaos_find_vcell(v1, a.v1GH + T.v1off, a.v1GH[T.v1off + T.nv1], T.nv1);
+      const int i2 = kaos_find_vcell(v2, a.v2GH + T.v2off, a.v2GH[T.v2off + T.nv2], T.nv2);
+      if (ip >= 1 && i1 >= 1 && i2 >= 1)
+        atomicAdd(&a.hist3[T.h3off + (((long long)(qcell - 1) * T.nv1 + (i1 - 1)) * T.nv2 + (i2 - 1)) * T.nvpar +
+                           (ip - 1)], 1u);
+      else
+        atomicAdd(&a.nqoff[T.goff + qcell - 1], 1u);
+    }
+    return;
+  }
   if (qcell >= 1) {
     if (ip >= 1 && iq >= 1) {
       atomicAdd(&a.hist[T.hoff + ((long long)(qcell - 1) * T.nvperp + (iq - 1)) * T.nvpar + (ip - 1)], 1u);
@@ -809,6 +888,102 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments(const DevTube* tubes, co
     if (Q == 0 && inj_cnt) {
       count[t] += inj_cnt[t]; next_id[t] += inj_cnt[t];
       // live ions: + this step's injections - this push's removals
+      live[t] += inj_cnt[t] - (long long)(esc_acc[2 * t] + esc_acc[2 * t + 1]);
+    }
+  }
+}
+
+// The 3-D moments (_8_6_1.._8_6_6, 3-D branches) of one (tube, config cell)
+// per block, one pass over the counts [V1][V2][Vpar]: lane (g, b) sums column
+// b over rows (V1, V2) g, g + G, ... (kaos_mom3_part); lane b adds the G
+// partials in g order and forms the column values; pairwise trees over the
+// columns give the sums in the oracle's order.  Writes the eight moments and
+// the 2-D block's M0, M1Par, M2, M2Par (M1Perp, M2Perp: 0), which the next
+// master step's fields read.  Block (t, 0) advances the tube's slot count,
+// next id and live count as the 2-D kernels do.
+__global__ void __launch_bounds__(KAOS_BLOCK) k_moments3d(const DevTube* tubes, const int* blk_tube,
+    const int* blk_q, const double* d3xC, const double* d3v3, const double* dvpar, const double* dv1,
+    const double* dv2, const double* vparGC, const double* v1GC, const double* v2GC,
+    const unsigned int* hist3, const unsigned int* nqoff, unsigned int* nqc, double* NqRT, double* M, double* M3,
+    long long* count, long long* next_id, const long long* inj_cnt, long long* live,
+    const unsigned long long* esc_acc) {
+  __shared__ double s_P[KAOS_MOM_LANES][5];
+  __shared__ unsigned int s_pc[KAOS_MOM_LANES];
+  __shared__ double s_A[KAOS_MAXV + 1][5];
+  __shared__ double s_x[8][KAOS_MAXV + 1];
+  __shared__ unsigned int s_cc[KAOS_MAXV + 1];
+  __shared__ double s_M[8];
+  const int t = blk_tube[blockIdx.x], Q = blk_q[blockIdx.x];
+  const DevTube& T = tubes[t];
+  const int np = T.nvpar, G = kaos_mom3_groups(np);
+  const int nrow = T.nv1 * T.nv2;
+  const int tid = threadIdx.x;
+  const double k = T.nsnormfac / d3xC[T.goff + Q];
+  const uint32_t* N = hist3 + T.h3off + (long long)Q * nrow * np;
+  kaos_vgrid3_t v = { np, T.nv1, T.nv2, d3v3 + T.d3off, dv1 + T.v1off, dv2 + T.v2off, dvpar + T.voff,
+                      v1GC + T.v1off, v2GC + T.v2off, vparGC + T.voff };
+  if (tid < G * np) {
+    double P[5];
+    uint32_t cs = 0u;
+    kaos_mom3_part(&v, N, tid % np, tid / np, G, 0, nrow - 1, k, P, &cs);
+    for (int i = 0; i < 5; ++i) s_P[tid][i] = P[i];
+    s_pc[tid] = cs;
+  }
+  __syncthreads();
+  if (tid < np) {
+    double A[5];
+    unsigned int cs = s_pc[tid];
+    for (int i = 0; i < 5; ++i) A[i] = s_P[tid][i];
+    for (int g = 1; g < G; ++g) {
+      for (int i = 0; i < 5; ++i) A[i] = A[i] + s_P[g * np + tid][i];
+      cs += s_pc[g * np + tid];
+    }
+    for (int i = 0; i < 5; ++i) s_A[tid][i] = A[i];
+    s_cc[tid] = cs;
+    double o6[6];
+    kaos_mom3_col_first(v.vparGC[tid], A, o6);
+    for (int i = 0; i < 6; ++i) s_x[i][tid] = o6[i];
+  }
+  __syncthreads();
+  for (int st = 1; st < np; st *= 2) {
+    if (tid < np && (tid % (2 * st)) == 0 && tid + st < np)
+      for (int i = 0; i < 6; ++i) s_x[i][tid] = s_x[i][tid] + s_x[i][tid + st];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    unsigned int tot = nqoff[T.goff + Q];
+    for (int c = 0; c < np; ++c) tot += s_cc[c];
+    nqc[T.goff + Q] = tot;
+    NqRT[T.goff + Q] = (double)tot * k;
+    double S[6], Mq[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 6; ++i) S[i] = s_x[i][0];
+    kaos_mom3_finish1(S, T.ms, Mq);
+    for (int i = 0; i < 8; ++i) s_M[i] = Mq[i];
+  }
+  __syncthreads();
+  if (tid < np) {
+    double A[5], o2[2];
+    for (int i = 0; i < 5; ++i) A[i] = s_A[tid][i];
+    kaos_mom3_col_second(v.vparGC[tid], A, s_M[3], o2);
+    s_x[6][tid] = o2[0]; s_x[7][tid] = o2[1];
+  }
+  __syncthreads();
+  for (int st = 1; st < np; st *= 2) {
+    if (tid < np && (tid % (2 * st)) == 0 && tid + st < np) {
+      s_x[6][tid] = s_x[6][tid] + s_x[6][tid + st];
+      s_x[7][tid] = s_x[7][tid] + s_x[7][tid + st];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double S2[2] = {s_x[6][0], s_x[7][0]}, Mq[8];
+    for (int i = 0; i < 8; ++i) Mq[i] = s_M[i];
+    kaos_mom3_finish2(S2, T.ms, Mq);
+    for (int i = 0; i < 8; ++i) M3[8 * T.goff + (long long)i * T.nq + Q] = Mq[i];
+    const double m6[6] = {Mq[0], 0.0, Mq[3], Mq[4], 0.0, Mq[7]};
+    for (int i = 0; i < 6; ++i) M[6 * T.goff + (long long)i * T.nq + Q] = m6[i];
+    if (Q == 0 && inj_cnt) {
+      count[t] += inj_cnt[t]; next_id[t] += inj_cnt[t];
       live[t] += inj_cnt[t] - (long long)(esc_acc[2 * t] + esc_acc[2 * t + 1]);
     }
   }
@@ -1048,6 +1223,7 @@ struct HostTube {
   long long cap = 0, count = 0;
   std::vector<double> qGL, qGH, qGC, hq, dq, d3x, rGC, cth, ell, Te;
   std::vector<double> vparGL, vparGH, vparGC, dvpar, vperpGL, vperpGH, vperpGC, dvperp, d3v;
+  std::vector<double> v1GL, v1GH, v1GC, dv1, v2GL, v2GH, v2GC, dv2, d3v3;   // 3-D velocity grid
 };
 
 // One set of particle-state arrays (the binary64 or the mixed form's words).
@@ -1119,6 +1295,13 @@ struct kaos_ctx {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> tune_ev;
   std::map<unsigned long long, hipGraphExec_t> graphs;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_push, ev_step;
+  // 3-D velocity grid (KAOS_FLAG_VPERP3D): Vperp1 / Vperp2 cells (high edges with
+  // GL0 appended, centres, widths), d3v3, counts [Q][V1][V2][Vpar], 8 moments per cell
+  bool v3d = false;
+  double *d_v1GH = nullptr, *d_v1GC = nullptr, *d_dv1 = nullptr, *d_v2GH = nullptr, *d_v2GC = nullptr,
+         *d_dv2 = nullptr, *d_d3v3 = nullptr, *d_M3 = nullptr;
+  unsigned int* d_hist3 = nullptr;
+  long long nhist3_tot = 0;
 };
 
 static std::string g_init_err;
@@ -1208,9 +1391,31 @@ extern "C" int kaos_set_grid(kaos_ctx* c, int32_t tube, const kaos_grid* g, int6
     if (!(g->icr_lambda_perp >= 0.0) || !std::isfinite(g->icr_chi_perp1) || !std::isfinite(g->icr_chi_perp2))
       return fail(c, KAOS_E_ARG, "bad ICR wavelength or spectral index");
   }
+  const bool v3d = (c->cfg.flags & KAOS_FLAG_VPERP3D) != 0;
+  if (v3d) {
+    if (g->nvperp1 < 1 || g->nvperp1 > KAOS_MAXV || g->nvperp2 < 1 || g->nvperp2 > KAOS_MAXV)
+      return fail(c, KAOS_E_ARG, "nvperp1/nvperp2 must be in [1, 255]");
+    const double* need3[] = {g->vperp1GL, g->vperp1GH, g->vperp1GC, g->dvperp1, g->vperp2GL, g->vperp2GH,
+                             g->vperp2GC, g->dvperp2, g->d3v3};
+    for (const double* p : need3) if (!p) return fail(c, KAOS_E_ARG, "null 3-D velocity grid array");
+    if (!contiguous(g->vperp1GL, g->vperp1GH, g->nvperp1) || !contiguous(g->vperp2GL, g->vperp2GH, g->nvperp2))
+      return fail(c, KAOS_E_ARG, "3-D velocity edges not contiguous");
+    for (int i = 0; i < g->nvperp1; ++i)
+      if (!(g->vperp1GL[i] < g->vperp1GH[i])) return fail(c, KAOS_E_ARG, "Vperp1 cells not increasing");
+    for (int i = 0; i < g->nvperp2; ++i)
+      if (!(g->vperp2GL[i] < g->vperp2GH[i])) return fail(c, KAOS_E_ARG, "Vperp2 cells not increasing");
+  }
   HostTube& h = c->ht[tube];
   h.g = *g;
   int nq = g->nq, np = g->nvpar, nr = g->nvperp;
+  if (v3d) {
+    const int n1 = g->nvperp1, n2 = g->nvperp2;
+    h.v1GL.assign(g->vperp1GL, g->vperp1GL + n1); h.v1GH.assign(g->vperp1GH, g->vperp1GH + n1);
+    h.v1GC.assign(g->vperp1GC, g->vperp1GC + n1); h.dv1.assign(g->dvperp1, g->dvperp1 + n1);
+    h.v2GL.assign(g->vperp2GL, g->vperp2GL + n2); h.v2GH.assign(g->vperp2GH, g->vperp2GH + n2);
+    h.v2GC.assign(g->vperp2GC, g->vperp2GC + n2); h.dv2.assign(g->dvperp2, g->dvperp2 + n2);
+    h.d3v3.assign(g->d3v3, g->d3v3 + (size_t)n1 * n2 * np);
+  }
   h.qGL.assign(g->qGL, g->qGL + nq); h.qGH.assign(g->qGH, g->qGH + nq); h.qGC.assign(g->qGC, g->qGC + nq);
   h.hq.assign(g->hqC, g->hqC + nq); h.dq.assign(g->dqC, g->dqC + nq); h.d3x.assign(g->d3xC, g->d3xC + nq);
   h.rGC.assign(g->rGC, g->rGC + nq); h.cth.assign(g->cthGC, g->cthGC + nq); h.ell.assign(g->ellGC, g->ellGC + nq);
@@ -1234,6 +1439,8 @@ static int build_device(kaos_ctx* c) {
   std::vector<double> qGH, qGC, hq, dq, Te, d3x, EG, vparGHx, vperpGHx, vparGC, vperpGC, dvpar, dvperp, d3v;
   std::vector<double> rGH, rref;
   std::vector<int> rk, vk;
+  std::vector<double> v1GHx, v1GC, dv1, v2GHx, v2GC, dv2, d3v3;
+  long long v1off = 0, v2off = 0, h3off = 0, d3off = 0;
   const bool mixed = c->cfg.precision == KAOS_MIXED;
   std::vector<int> tile_tube, blk_tube, blk_q;
   std::vector<long long> tile_start;
@@ -1288,6 +1495,18 @@ static int build_device(kaos_ctx* c) {
     vk.push_back(0);
     c->lds_nq = std::max(c->lds_nq, g.nq);
     c->lds_nv = std::max(c->lds_nv, g.nvpar);
+    if (c->cfg.flags & KAOS_FLAG_VPERP3D) {
+      T.nv1 = g.nvperp1; T.nv2 = g.nvperp2;
+      T.v1off = v1off; T.v2off = v2off; T.h3off = h3off; T.d3off = d3off;
+      for (int i = 0; i < T.nv1; ++i) { v1GHx.push_back(h.v1GH[i]); v1GC.push_back(h.v1GC[i]); dv1.push_back(h.dv1[i]); }
+      v1GHx.push_back(h.v1GL[0]); v1GC.push_back(0.0); dv1.push_back(0.0);
+      for (int i = 0; i < T.nv2; ++i) { v2GHx.push_back(h.v2GH[i]); v2GC.push_back(h.v2GC[i]); dv2.push_back(h.dv2[i]); }
+      v2GHx.push_back(h.v2GL[0]); v2GC.push_back(0.0); dv2.push_back(0.0);
+      d3v3.insert(d3v3.end(), h.d3v3.begin(), h.d3v3.end());
+      v1off += T.nv1 + 1; v2off += T.nv2 + 1;
+      d3off += (long long)T.nv1 * T.nv2 * g.nvpar;
+      h3off += (long long)g.nq * T.nv1 * T.nv2 * g.nvpar;
+    }
     c->dt[t] = T;
     for (int i = 0; i < g.nq; ++i) {
       qGH.push_back(h.qGH[i]); qGC.push_back(h.qGC[i]); hq.push_back(h.hq[i]); dq.push_back(h.dq[i]);
@@ -1404,6 +1623,21 @@ static int build_device(kaos_ctx* c) {
   }
   HIPCHK(c, hipMalloc((void**)&c->d_stepno, sizeof(long long)));
   HIPCHK(c, hipMemset(c->d_stepno, 0, sizeof(long long)));
+  if (c->cfg.flags & KAOS_FLAG_VPERP3D) {
+    c->v3d = true;
+    c->nhist3_tot = h3off;
+    HIPCHK(c, up((void**)&c->d_v1GH, v1GHx.data(), sizeof(double) * v1off));
+    HIPCHK(c, up((void**)&c->d_v1GC, v1GC.data(), sizeof(double) * v1off));
+    HIPCHK(c, up((void**)&c->d_dv1, dv1.data(), sizeof(double) * v1off));
+    HIPCHK(c, up((void**)&c->d_v2GH, v2GHx.data(), sizeof(double) * v2off));
+    HIPCHK(c, up((void**)&c->d_v2GC, v2GC.data(), sizeof(double) * v2off));
+    HIPCHK(c, up((void**)&c->d_dv2, dv2.data(), sizeof(double) * v2off));
+    HIPCHK(c, up((void**)&c->d_d3v3, d3v3.data(), sizeof(double) * d3off));
+    std::vector<double> z8(8 * goff, 0.0);
+    HIPCHK(c, up((void**)&c->d_M3, z8.data(), sizeof(double) * 8 * goff));
+    HIPCHK(c, hipMalloc((void**)&c->d_hist3, sizeof(unsigned int) * (size_t)h3off));
+    HIPCHK(c, hipMemset(c->d_hist3, 0, sizeof(unsigned int) * (size_t)h3off));
+  }
   HIPCHK(c, hipMalloc((void**)&c->d_pushed, sizeof(unsigned long long)));
   HIPCHK(c, hipMemset(c->d_pushed, 0, sizeof(unsigned long long)));
   HIPCHK(c, hipMalloc((void**)&c->d_status, sizeof(unsigned int)));
@@ -1453,7 +1687,8 @@ static StepArgs make_args(kaos_ctx* c, int interp, int count, int remove, int qk
   a.stepno = c->d_stepno;
   a.seed = c->cfg.seed;
   a.hbox = c->d_hbox;
-  a.do_interp = interp; a.do_count = count; a.do_remove = remove; a.do_qk = qk;
+  a.do_interp = interp; a.do_count = c->v3d ? 0 : count; a.do_remove = remove; a.do_qk = qk;
+  a.v1GH = c->d_v1GH; a.v2GH = c->d_v2GH; a.hist3 = c->d_hist3;
   return a;
 }
 
@@ -1588,6 +1823,13 @@ static size_t push_lds_bytes(const kaos_ctx* c) { return sizeof(unsigned int) * 
 // Vpar cells, block-per-cell kernel for the others.
 static void launch_moments(kaos_ctx* c, const long long* inj) {
   const int wpb = KAOS_BLOCK / 64;
+  if (c->v3d) {
+    hipLaunchKernelGGL(k_moments3d, dim3(c->nblk_mom), dim3(KAOS_BLOCK), 0, c->stream, c->d_tubes, c->d_blk_tube,
+                       c->d_blk_q, c->d_d3x, c->d_d3v3, c->d_dvpar, c->d_dv1, c->d_dv2, c->d_vparGC, c->d_v1GC,
+                       c->d_v2GC, c->d_hist3, c->d_nqoff, c->d_nqc, c->d_NqRT, c->d_M, c->d_M3, c->d_count,
+                       c->d_next_id, inj, c->d_live, c->d_esc_acc);
+    return;
+  }
   if (c->n_cells_wave)
     hipLaunchKernelGGL(k_moments_wave, dim3((c->n_cells_wave + wpb - 1) / wpb), dim3(KAOS_BLOCK), 0, c->stream,
                        c->d_tubes, c->d_blk_tube, c->d_blk_q, c->d_cells_wave, c->n_cells_wave, c->d_d3x, c->d_d3v,
@@ -1605,6 +1847,7 @@ static void launch_moments(kaos_ctx* c, const long long* inj) {
 // only steers where counts accumulate, so a stale box costs time, never a
 // result).
 static void launch_box_update(kaos_ctx* c) {
+  if (c->v3d) return;                         // the push bins nothing on the 3-D grid
   const int wpb = KAOS_BLOCK / 64;
   hipLaunchKernelGGL(k_marginals, dim3((c->nblk_mom + wpb - 1) / wpb), dim3(KAOS_BLOCK), 0, c->stream, c->d_tubes,
                      c->d_blk_tube, c->d_blk_q, c->nblk_mom, c->d_hbox, c->d_hist, c->d_vmarg_par, c->d_vmarg_perp);
@@ -1726,7 +1969,13 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
   const bool first = (n == 1);
   const bool premaster = is_master(c, n + 1);
   hipLaunchKernelGGL(k_step_tick, dim3(1), dim3(1), 0, c->stream, c->d_stepno);
+  const int mixed = c->cfg.precision == KAOS_MIXED;
   auto zero_hist = [&]() {
+    if (c->v3d) {
+      hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, c->stream, c->d_hist3, c->nhist3_tot);
+      hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, c->stream, c->d_nqoff, c->ncell_tot);
+      return;
+    }
     hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, c->stream, c->d_hist, c->nhist_tot);
     hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, c->stream, c->d_nqoff, c->ncell_tot);
     hipLaunchKernelGGL(k_hbox_reset, dim3((c->nblk_mom + 255) / 256), dim3(256), 0, c->stream, c->nblk_mom,
@@ -1735,7 +1984,10 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
   if (first) {
     StepArgs a = make_args(c, 1, 0, 0, premaster);
     zero_hist();
-    hipLaunchKernelGGL(k_bin_initial, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a);
+    if (c->v3d)
+      hipLaunchKernelGGL(k_bin3d, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a, 1, mixed);
+    else
+      hipLaunchKernelGGL(k_bin_initial, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a);
     // field of master step 1 from "master step 0" (no moments: M = 0), then moments of step 1
     hipLaunchKernelGGL(k_field, dim3(nt), dim3(256), 0, c->stream, c->d_tubes, c->d_hq, c->d_dq, c->d_Te,
                        c->d_M, c->d_M0F, c->d_M1F, c->d_EA, c->d_esc_acc, c->d_esc_rep, 0);
@@ -1756,6 +2008,8 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
       if (rc) return rc;
     }
     StepArgs a = make_args(c, master, master, master, premaster);
+    if (master && c->v3d)
+      hipLaunchKernelGGL(k_bin3d, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a, 0, mixed);
     if (c->any_icr && c->cfg.precision != KAOS_MIXED)
       hipLaunchKernelGGL(k_icr, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a);
     launch_push(c, a, e0, e1);
@@ -1764,7 +2018,7 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
         int nb = (int)((c->inj_total + 255) / 256);
         hipLaunchKernelGGL(k_inject, dim3(nb), dim3(256), 0, c->stream, a, c->d_inj_base, c->d_next_id, nt, c->d_EA,
                            c->d_EG, c->d_vperpGH, d_uni, c->d_uni_base, (unsigned long long)c->cfg.seed,
-                           c->d_stepno, c->inj_total, (int)(c->cfg.precision == KAOS_MIXED));
+                           c->d_stepno, c->inj_total, mixed);
       }
       launch_moments(c, c->d_inj_cnt);
     }
@@ -1953,6 +2207,23 @@ extern "C" int kaos_download_master(kaos_ctx* c, int32_t tube, int64_t* nq_count
     esc[2] = NLB / ((h.d3x[0] / (h.hq[0] * h.dq[0])) * dtm);
     esc[3] = NUB / ((h.d3x[ub] / (h.hq[ub] * h.dq[ub])) * dtm);
   }
+  return KAOS_OK;
+}
+
+extern "C" int kaos_download_master3d(kaos_ctx* c, int32_t tube, int64_t* n3_counts, double* M) {
+  if (!c) return KAOS_E_ARG;
+  if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
+  if (!c->built) return fail(c, KAOS_E_STATE, "nothing computed yet");
+  if (!c->v3d) return fail(c, KAOS_E_STATE, "the context has no 3-D velocity grid (KAOS_FLAG_VPERP3D)");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const DevTube& T = c->dt[tube];
+  if (n3_counts) {
+    const size_t n = (size_t)T.nq * T.nv1 * T.nv2 * T.nvpar;
+    std::vector<unsigned int> u(n);
+    HIPCHK(c, hipMemcpy(u.data(), c->d_hist3 + T.h3off, sizeof(unsigned int) * n, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) n3_counts[i] = u[i];
+  }
+  if (M) HIPCHK(c, hipMemcpy(M, c->d_M3 + 8 * T.goff, sizeof(double) * 8 * T.nq, hipMemcpyDeviceToHost));
   return KAOS_OK;
 }
 
@@ -2200,6 +2471,8 @@ extern "C" int kaos_finalize(kaos_ctx* c) {
   (void)hipFree(c->d_vbox); (void)hipFree(c->d_hbox);
   (void)hipFree(c->d_vmarg_par); (void)hipFree(c->d_vmarg_perp);
   (void)hipFree(c->d_stepno);
+  void* b3[] = {c->d_v1GH, c->d_v1GC, c->d_dv1, c->d_v2GH, c->d_v2GC, c->d_dv2, c->d_d3v3, c->d_M3, c->d_hist3};
+  for (void* p : b3) if (p) (void)hipFree(p);
   for (StateBufs& B : c->st) {
     void* sb[] = {B.rs, B.vp, B.vr, B.qk, B.ae, B.ag, B.rdi, B.vpi, B.vrf, B.qkf, B.aef, B.agf, B.cell, B.id,
                   B.vr2, B.vrf2};

@@ -18,7 +18,8 @@ module kaos_hip
                                        KAOS_E_DEVICE = 3, KAOS_E_STATE = 4, KAOS_E_NAN = 5, &
                                        KAOS_E_IO = 6
   integer(c_int32_t), parameter, public :: KAOS_FP64 = 0, KAOS_MIXED = 1
-  integer(c_int32_t), parameter, public :: KAOS_FLAG_NO_GRAPH = 1, KAOS_FLAG_NO_AUTOTUNE = 2
+  integer(c_int32_t), parameter, public :: KAOS_FLAG_NO_GRAPH = 1, KAOS_FLAG_NO_AUTOTUNE = 2, &
+                                            KAOS_FLAG_VPERP3D = 4
 
   type, bind(c), public :: kaos_cfg
     integer(c_int32_t) :: ntubes, precision, ndatfac, device
@@ -37,6 +38,11 @@ module kaos_hip
     integer(c_int32_t) :: icr = 0, pad1 = 0
     real(c_double) :: icr_lambda_perp = 0, icr_eta_lh = 0, icr_xi_perp1 = 0, icr_xi_perp2 = 0
     real(c_double) :: icr_s0 = 0, icr_f0 = 0, icr_chi_perp1 = 0, icr_chi_perp2 = 0
+    ! 3-D velocity grid (ION2VPERPflag = 1; read when the context has KAOS_FLAG_VPERP3D)
+    integer(c_int32_t) :: nvperp1 = 0, nvperp2 = 0
+    type(c_ptr) :: vperp1GL = c_null_ptr, vperp1GH = c_null_ptr, vperp1GC = c_null_ptr, dvperp1 = c_null_ptr
+    type(c_ptr) :: vperp2GL = c_null_ptr, vperp2GH = c_null_ptr, vperp2GC = c_null_ptr, dvperp2 = c_null_ptr
+    type(c_ptr) :: d3v3 = c_null_ptr
   end type kaos_grid
 
   ! dir: c_loc of a NUL-terminated character array (trim(dataexportdir)//c_null_char)
@@ -54,7 +60,7 @@ module kaos_hip
 
   public :: kaos_init, kaos_set_grid, kaos_upload_particles, kaos_upload_vperp, kaos_cell_counts, &
             kaos_seed_particles, kaos_step, kaos_run, &
-            kaos_download_master, kaos_export_master, kaos_count, kaos_download_particles, kaos_set_launch, &
+            kaos_download_master, kaos_download_master3d, kaos_export_master, kaos_count, kaos_download_particles, kaos_set_launch, &
             kaos_get_launch, kaos_profile, &
             kaos_profile_read, kaos_last_error, kaos_finalize, kaos_error_message
 
@@ -133,6 +139,15 @@ module kaos_hip
       integer(c_int32_t), value :: tube
       type(c_ptr), value :: nq_counts, nph_counts, NqRT, M, EA, EG, esc
     end function kaos_download_master
+
+    ! 3-D counts [Q][V1][V2][Vpar] and the eight 3-D moments of the last master step
+    integer(c_int) function kaos_download_master3d(ctx, tube, n3_counts, M) &
+        bind(c, name='kaos_download_master3d')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tube
+      type(c_ptr), value :: n3_counts, M
+    end function kaos_download_master3d
 
     ! DataExport2Sub's files of the last master step
     integer(c_int) function kaos_export_master(ctx, tube, info, nn) bind(c, name='kaos_export_master')

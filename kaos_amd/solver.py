@@ -15,9 +15,10 @@ import numpy as np
 
 from . import _abi
 from ._abi import (KaosCfg, KaosGrid, KaosStepOut, KaosExportInfo, ptr, KAOS_FP64, KAOS_MIXED,  # noqa: F401
-                   KAOS_FLAG_NO_GRAPH, KAOS_FLAG_NO_AUTOTUNE)
+                   KAOS_FLAG_NO_GRAPH, KAOS_FLAG_NO_AUTOTUNE, KAOS_FLAG_VPERP3D)
 
 MOMENT_NAMES = ("M0", "M1perp", "M1par", "M2", "M2perp", "M2par")
+MOMENTS3 = ("M0", "M1perp1", "M1perp2", "M1par", "M2", "M2perp1", "M2perp2", "M2par")
 
 
 class KaosError(RuntimeError):
@@ -35,6 +36,8 @@ def grid_struct(tube):
         setattr(g, n, getattr(tube, n))
     g.cphi, g.sphi = tube.cphi, tube.sphi
     g.filter_pt_m1par = getattr(tube, "filter_pt_m1par", 0)
+    if getattr(tube, "v3d", 0):
+        g.nvperp1, g.nvperp2 = tube.nv1, tube.nv2
     g.icr = getattr(tube, "icr", 0)
     for n in _abi.ICR_FIELDS:
         setattr(g, n, getattr(tube, n, 0.0))
@@ -43,6 +46,13 @@ def grid_struct(tube):
         a = np.ascontiguousarray(tube.arrays[n], dtype=np.float64)
         keep.append(a)
         setattr(g, n, ptr(a))
+    if getattr(tube, "v3d", 0):
+        for n, tn in zip(_abi.GRID_ARRAYS3, _abi.TUBE_ARRAYS3):
+            if tube.arrays.get(tn) is None:
+                continue                      # a null pointer: the library rejects the grid
+            a = np.ascontiguousarray(tube.arrays[tn], dtype=np.float64)
+            keep.append(a)
+            setattr(g, n, ptr(a))
     g._keep = keep
     return g
 
@@ -53,6 +63,11 @@ class KineticSolver:
         self.lib = _abi.load()
         self.tubes = list(tubes)
         self.ndatfac = ndatfac
+        v3 = [getattr(t, "v3d", 0) for t in self.tubes]
+        if any(v3):
+            if not all(v3):
+                raise ValueError("the 3-D velocity grid is a context-wide mode: give every tube one")
+            flags |= KAOS_FLAG_VPERP3D
         cfg = KaosCfg(ntubes=len(self.tubes), precision=precision, ndatfac=ndatfac, device=device,
                       seed=seed, reorder_every=reorder_every, flags=flags)
         h = C.c_void_p()
@@ -142,6 +157,18 @@ class KineticSolver:
                    EA=EA, EG=EG, esc=esc)
         for i, n in enumerate(MOMENT_NAMES):
             out[n] = M[i * nq:(i + 1) * nq].copy()
+        return out
+
+    def master3(self, tube):
+        """3-D outputs of the last master step (KAOS_FLAG_VPERP3D): counts
+        [Q][V1][V2][Vpar] and the eight 3-D moments."""
+        t = self.tubes[tube]
+        n3 = np.zeros(t.nq * t.nv1 * t.nv2 * t.nvpar, np.int64)
+        M = np.zeros(8 * t.nq)
+        self._chk(self.lib.kaos_download_master3d(self.h, tube, ptr(n3, C.c_int64), ptr(M)))
+        out = dict(n3_counts=n3.reshape(t.nq, t.nv1, t.nv2, t.nvpar))
+        for i, n in enumerate(MOMENTS3):
+            out[n] = M[i * t.nq:(i + 1) * t.nq].copy()
         return out
 
     def export_master(self, tube, directory, theta, Ts, ns0, phi, species=1, flux_tube=1):

@@ -52,6 +52,8 @@ int main(void) {
   printf("%zu %zu\n", offsetof(kaos_grid, filter_pt_m1par), offsetof(kaos_grid, n_inj_ub));
   printf("%zu %zu %zu %zu %zu\n", sizeof(kaos_export_info), offsetof(kaos_export_info, species),
          offsetof(kaos_export_info, ns0), offsetof(kaos_export_info, theta), offsetof(kaos_export_info, Ts));
+  printf("%zu %zu %zu %zu\n", offsetof(kaos_grid, nvperp1), offsetof(kaos_grid, vperp1GL),
+         offsetof(kaos_grid, dvperp2), offsetof(kaos_grid, d3v3));
   return 0;
 }
 '''
@@ -72,5 +74,7 @@ int main(void) {
             _abi.KaosGrid.icr.offset, _abi.KaosGrid.icr_lambda_perp.offset, _abi.KaosGrid.icr_chi_perp2.offset,
             _abi.KaosGrid.filter_pt_m1par.offset, _abi.KaosGrid.n_inj_ub.offset,
             C.sizeof(_abi.KaosExportInfo), _abi.KaosExportInfo.species.offset, _abi.KaosExportInfo.ns0.offset,
-            _abi.KaosExportInfo.theta.offset, _abi.KaosExportInfo.Ts.offset]
+            _abi.KaosExportInfo.theta.offset, _abi.KaosExportInfo.Ts.offset,
+            _abi.KaosGrid.nvperp1.offset, _abi.KaosGrid.vperp1GL.offset, _abi.KaosGrid.dvperp2.offset,
+            _abi.KaosGrid.d3v3.offset]
     assert got == want
