@@ -2254,13 +2254,22 @@ extern "C" int kaos_export_master(kaos_ctx* c, int32_t tube, const kaos_export_i
   const int nq = T.nq, nvpar = T.nvpar, nvperp = T.nvperp;
   const long long ncell = (long long)nvpar * nvperp;
   std::vector<double> M(6 * (size_t)nq), M0F(nq), M1F(nq), EA(nq), EG(nq);
-  std::vector<unsigned int> hist((size_t)(nq * ncell));
+  const bool v3d = c->v3d;
+  const long long ncell3 = v3d ? (long long)T.nv1 * T.nv2 * nvpar : 0;
+  std::vector<unsigned int> hist(v3d ? 0 : (size_t)(nq * ncell)), hist3((size_t)(nq * ncell3));
+  std::vector<double> M3(v3d ? 8 * (size_t)nq : 0);
   HIPCHK(c, hipMemcpy(M.data(), c->d_M + 6 * T.goff, sizeof(double) * 6 * nq, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(M0F.data(), c->d_M0F + T.goff, sizeof(double) * nq, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(M1F.data(), c->d_M1F + T.goff, sizeof(double) * nq, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(EA.data(), c->d_EA + T.goff, sizeof(double) * nq, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(EG.data(), c->d_EG + T.goff, sizeof(double) * nq, hipMemcpyDeviceToHost));
-  HIPCHK(c, hipMemcpy(hist.data(), c->d_hist + T.hoff, sizeof(unsigned int) * hist.size(), hipMemcpyDeviceToHost));
+  if (v3d) {
+    HIPCHK(c, hipMemcpy(hist3.data(), c->d_hist3 + T.h3off, sizeof(unsigned int) * hist3.size(),
+                        hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(M3.data(), c->d_M3 + 8 * T.goff, sizeof(double) * 8 * nq, hipMemcpyDeviceToHost));
+  } else {
+    HIPCHK(c, hipMemcpy(hist.data(), c->d_hist + T.hoff, sizeof(unsigned int) * hist.size(), hipMemcpyDeviceToHost));
+  }
 
   std::string dir(info->dir);
   if (!dir.empty() && dir.back() != '/') dir += '/';
@@ -2287,8 +2296,8 @@ extern "C" int kaos_export_master(kaos_ctx* c, int32_t tube, const kaos_export_i
   put_cells("TsGT", info->Ts);
   // :167-191: NphRTp(:, :, nn) of every cell, Vperp fastest; NphRT =
   // NphReNormRT (nsnormfac / d3x), 0 for an empty bin (_8_3_1_IonParticleCounts.f90:713-729)
-  std::vector<double> nph(ncell);
-  for (int Q = 0; Q < nq && ok; ++Q) {
+  std::vector<double> nph(v3d ? 0 : ncell);
+  for (int Q = 0; Q < nq && ok && !v3d; ++Q) {
     const double w = h.g.nsnormfac / h.d3x[Q];
     for (int i = 0; i < nvperp; ++i)
       for (int b = 0; b < nvpar; ++b) {
@@ -2297,10 +2306,42 @@ extern "C" int kaos_export_master(kaos_ctx* c, int32_t tube, const kaos_export_i
       }
     if (!export_raw(pre + std::to_string(Q + 1) + "_NphRTfort.bin", nph.data(), sizeof(double) * ncell)) ok = false;
   }
-  // :225-510: moments; :359-410: filtered moments (only M0 and M1Par are set
-  // without charge exchange)
+  // :124-165 (3-D grid): per cell TeNT(nn) (constant in time: dNTe = 0,
+  // _8_KineticSolver.f90:66-84), N2PerpphRTp(:, :, :, nn) and d3vCTp(:, :, :),
+  // both (Vperp1, Vperp2, Vpar) with Vperp1 fastest; N2PerpphRT = count
+  // (nsnormfac / d3x), 0 for an empty bin (_8_3_1_IonParticleCounts.f90:440-457)
+  if (v3d) {
+    const int n1 = T.nv1, n2 = T.nv2;
+    std::vector<double> n3((size_t)ncell3), d3((size_t)ncell3);
+    for (int i1 = 0; i1 < n1; ++i1)
+      for (int i2 = 0; i2 < n2; ++i2)
+        for (int b = 0; b < nvpar; ++b)
+          d3[((size_t)b * n2 + i2) * n1 + i1] = h.d3v3[((size_t)i1 * n2 + i2) * nvpar + b];
+    for (int Q = 0; Q < nq && ok; ++Q) {
+      const double w = h.g.nsnormfac / h.d3x[Q];
+      for (int i1 = 0; i1 < n1; ++i1)
+        for (int i2 = 0; i2 < n2; ++i2)
+          for (int b = 0; b < nvpar; ++b) {
+            const unsigned int k = hist3[(size_t)Q * ncell3 + ((size_t)i1 * n2 + i2) * nvpar + b];
+            n3[((size_t)b * n2 + i2) * n1 + i1] = (k == 0u) ? 0.0 : (double)k * w;
+          }
+      const std::string pq = pre + std::to_string(Q + 1) + "_";
+      if (!export_raw(pq + "TeNTfort.bin", &h.Te[Q], sizeof(double)) ||
+          !export_raw(pq + "N2PerpphRTfort.bin", n3.data(), sizeof(double) * ncell3) ||
+          !export_raw(pq + "d3vCTpfort.bin", d3.data(), sizeof(double) * ncell3))
+        ok = false;
+    }
+  }
+  // :225-510: moments (3-D grid: M1Perp1/M1Perp2 and M2Perp1/M2Perp2 in place
+  // of M1Perp and M2Perp, :279-305, 468-500); :359-410: filtered moments
+  // (only M0 and M1Par are set without charge exchange)
   put_cells("M0phRT", M.data());
-  put_cells("M1PerpphRT", M.data() + nq);
+  if (v3d) {
+    put_cells("M1Perp1phRT", M3.data() + nq);
+    put_cells("M1Perp2phRT", M3.data() + 2 * nq);
+  } else {
+    put_cells("M1PerpphRT", M.data() + nq);
+  }
   put_cells("M1ParphRT", M.data() + 2 * nq);
   put_cells("M0FiltAvrgRT", M0F.data());
   put_cells("M1Perp1FiltAvrgRT", zeros.data());
@@ -2308,7 +2349,12 @@ extern "C" int kaos_export_master(kaos_ctx* c, int32_t tube, const kaos_export_i
   put_cells("M1ParFiltAvrgRT", M1F.data());
   put_cells("M2ParFiltAvrgRT", zeros.data());
   put_cells("M2phRT", M.data() + 3 * nq);
-  put_cells("M2PerpphRT", M.data() + 4 * nq);
+  if (v3d) {
+    put_cells("M2Perp1phRT", M3.data() + 5 * nq);
+    put_cells("M2Perp2phRT", M3.data() + 6 * nq);
+  } else {
+    put_cells("M2PerpphRT", M.data() + 4 * nq);
+  }
   put_cells("M2ParphRT", M.data() + 5 * nq);
   // :645-704: fields.  EAINERTIALflag off: EAInertialRT = abs(0d0), and
   // EAmagRT = EAInertialRT + EAPressureRT (_8_7_AmbipolarEfield.f90:153-155,

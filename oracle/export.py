@@ -3,8 +3,9 @@
 tube at master step nn, written from the oracle's master outputs with numpy.
 
 Flag set (this path): FLUIDIONEXPORTflag, PHASEIONDISTRIBflag, the moment
-flags and MOMENTFILTERflag on; ION2VPERPflag, QEXCHANGEflag, SPINUPflag,
-EAINERTIALflag and EPARflag off (_ModelInput.f90).
+flags and MOMENTFILTERflag on; QEXCHANGEflag, SPINUPflag, EAINERTIALflag and
+EPARflag off (_ModelInput.f90); ION2VPERPflag either (the 3-D branch when
+the tube has a 3-D velocity grid and master3 is given).
 
 File names: DataExport2Sub builds "<nn>_<s>_<f>_" from I5 edits of nn, s
 and f with adjustl/adjustr (:32-38), and "<nn>_<s>_<f>_<Qind>_" for the
@@ -41,9 +42,24 @@ def nph_rt(counts, nsnormfac, d3x):
     return out
 
 
-def export_master(directory, tube, master, n, ndatfac, theta, Ts, ns0, phi, s=1, f=1):
+def n2perp_rt(counts3, nsnormfac, d3x):
+    """N2PerpphRTp(:, :, :, nn) of every cell (_8_3_1_IonParticleCounts.f90:
+    440-457): N2PerpphReNormRT (nsnormfac / d3xCGT), 0 where the count is 0;
+    counts are [Q][V1][V2][Vpar]; returns [Q] arrays in Fortran order
+    (Vperp1 fastest, then Vperp2, then Vpar)."""
+    out = []
+    for Q in range(counts3.shape[0]):
+        w = nsnormfac / d3x[Q]
+        v = np.where(counts3[Q] == 0, 0.0, counts3[Q].astype(np.float64) * w)
+        out.append(np.asfortranarray(v).ravel(order="F"))
+    return out
+
+
+def export_master(directory, tube, master, n, ndatfac, theta, Ts, ns0, phi, s=1, f=1, master3=None):
     """Write the files of master step n (n = 1, or a master step) of one tube
-    from the oracle's master() outputs; returns nn."""
+    from the oracle's master() outputs (and master3() on the 3-D grid);
+    returns nn."""
+    v3d = master3 is not None
     nn = 1 if n == 1 else n // ndatfac + 1
     nq = tube.nq
     a = tube.arrays
@@ -59,14 +75,27 @@ def export_master(directory, tube, master, n, ndatfac, theta, Ts, ns0, phi, s=1,
     put("thetaGCGT", theta)
     put("d3xCGT", a["d3xC"])
     put("TsGT", Ts)
-    # :167-191
-    for Q, v in enumerate(nph_rt(master["nph_counts"], tube.nsnormfac, a["d3xC"])):
-        _put(os.path.join(directory, file_name(nn, s, f, "NphRT", Q + 1)), v)
+    if v3d:
+        # :124-165: TeNT(nn) (constant: dNTe = 0, _8_KineticSolver.f90:66-84),
+        # N2PerpphRTp(:, :, :, nn), d3vCTp(:, :, :) of every cell
+        d3 = np.asfortranarray(a["d3v3"].reshape(tube.nv1, tube.nv2, tube.nvpar)).ravel(order="F")
+        for Q, v in enumerate(n2perp_rt(master3["n3_counts"], tube.nsnormfac, a["d3xC"])):
+            _put(os.path.join(directory, file_name(nn, s, f, "TeNT", Q + 1)), [a["Te"][Q]])
+            _put(os.path.join(directory, file_name(nn, s, f, "N2PerpphRT", Q + 1)), v)
+            _put(os.path.join(directory, file_name(nn, s, f, "d3vCTp", Q + 1)), d3)
+    else:
+        # :167-191
+        for Q, v in enumerate(nph_rt(master["nph_counts"], tube.nsnormfac, a["d3xC"])):
+            _put(os.path.join(directory, file_name(nn, s, f, "NphRT", Q + 1)), v)
     # :225-510 (2-D branch) and :359-410 (filter: M1Perp1/M1Perp2/M2Par are set
     # only by the charge-exchange collision frequency, _8_6_6_1:50-92)
     z = np.zeros(nq)
     put("M0phRT", master["M0"])
-    put("M1PerpphRT", master["M1perp"])
+    if v3d:                                                    # :279-305
+        put("M1Perp1phRT", master3["M1perp1"])
+        put("M1Perp2phRT", master3["M1perp2"])
+    else:
+        put("M1PerpphRT", master["M1perp"])
     put("M1ParphRT", master["M1par"])
     put("M0FiltAvrgRT", master["M0F"])
     put("M1Perp1FiltAvrgRT", z)
@@ -74,7 +103,11 @@ def export_master(directory, tube, master, n, ndatfac, theta, Ts, ns0, phi, s=1,
     put("M1ParFiltAvrgRT", master["M1F"])
     put("M2ParFiltAvrgRT", z)
     put("M2phRT", master["M2"])
-    put("M2PerpphRT", master["M2perp"])
+    if v3d:                                                    # :468-500
+        put("M2Perp1phRT", master3["M2perp1"])
+        put("M2Perp2phRT", master3["M2perp2"])
+    else:
+        put("M2PerpphRT", master["M2perp"])
     put("M2ParphRT", master["M2par"])
     # :645-704; EAINERTIALflag off (_8_7_AmbipolarEfield.f90:153-155, 1296-1299),
     # EPARflag off (_8_8_PotentialStructure.f90:46-96)

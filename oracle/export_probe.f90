@@ -6,16 +6,17 @@
 !    declarations _KineticMainParams.f90:35-47, 346-358), printed one a line
 !    for the (nn, s, f, Qind) given on the command line;
 !  - the bytes of open(status='replace', form='unformatted', access='stream')
-!    + write + close of a real(dp) scalar, an integer(dp) scalar and a real(dp)
-!    (3, 4) array, written to probe_scalar.bin, probe_int.bin, probe_array.bin
-!    in the working directory.
+!    + write + close of a real(dp) scalar, an integer(dp) scalar, a real(dp)
+!    (3, 4) array and a real(dp) (2, 3, 4) array written whole as a(:, :, :)
+!    (N2PerpphRTp, d3vCTp), to probe_scalar.bin, probe_int.bin,
+!    probe_array.bin, probe_array3.bin in the working directory.
 program export_probe
   implicit none
   integer, parameter :: dp = kind(1.d0)
   character(50) :: nnstring, sstring, fstring, Qindstring, expstring, fname
   character(32) :: arg
-  integer(kind=dp) :: nn, s, f, Qind, i, j
-  real(kind=dp) :: x, a(3, 4)
+  integer(kind=dp) :: nn, s, f, Qind, i, j, l
+  real(kind=dp) :: x, a(3, 4), b(2, 3, 4)
   integer(kind=dp) :: k
 
   call get_command_argument(1, arg); read (arg, *) nn
@@ -55,5 +56,15 @@ program export_probe
   close (10)
   open (unit=10, file='probe_array.bin', status='replace', form='unformatted', access='stream')
   write (10) a(:, :)
+  close (10)
+  do l = 1, 4
+    do j = 1, 3
+      do i = 1, 2
+        b(i, j, l) = real(100 * i + 10 * j + l, dp) + 0.5_dp
+      end do
+    end do
+  end do
+  open (unit=10, file='probe_array3.bin', status='replace', form='unformatted', access='stream')
+  write (10) b(:, :, :)
   close (10)
 end program export_probe

@@ -77,6 +77,47 @@ def test_export_fp64_byte_identical_to_oracle(tmp_path):
     ks.close()
 
 
+def test_export_3d_grid_fp64_byte_identical_to_oracle(tmp_path):
+    # the 3-D branch (ION2VPERPflag = 1): per cell TeNT, N2PerpphRT, d3vCTp in
+    # place of NphRT; M1Perp1/M1Perp2 and M2Perp1/M2Perp2 in place of M1Perp, M2Perp
+    t = grids.with_vgrid3(grids.dipole_tube(nq=24, nvpar=20, nvperp=16, n_total=8000, ratio=1.1), 9, 7,
+                          vperp_max=6.0e3)
+    t.arrays["Te"] = np.linspace(8.0e3, 1.2e4, t.nq)          # a per-cell TeNT
+    ndatfac, nsteps = 1, 3
+    q, v, w = grids.initial_ions(t, seed=12)
+    cap = len(q) + (nsteps + 2) * (t.n_inj_lb + t.n_inj_ub) + 256
+    o = oracle.Oracle(t, q, v, w, ndatfac=ndatfac, seed=4, capacity=cap)
+    ks = KineticSolver([t], precision=KAOS_FP64, ndatfac=ndatfac, seed=4, capacity=[cap])
+    ks.upload(0, q, v, w)
+    rng = np.random.default_rng(3)
+    ph = rng.uniform(0.0, 2.0 * np.pi, len(w))
+    ks.upload_vperp(0, w * np.cos(ph), w * np.sin(ph))
+    o.set_vperp(w * np.cos(ph), w * np.sin(ph))
+    print("export 3-D grid fp64: 9x7x20, steps", nsteps, flush=True)
+    lib_dir, orc_dir = tmp_path / "lib", tmp_path / "orc"
+    lib_dir.mkdir()
+    orc_dir.mkdir()
+    theta, Ts = _extra(t)
+    for n in range(1, nsteps + 1):
+        o.step(n)
+        assert ks.step(n).status == 0
+        nn = ks.export_master(0, str(lib_dir), theta, Ts, ns0=1e8, phi=t.phi)
+        assert oexp.export_master(str(orc_dir), t, o.master(), n, ndatfac, theta, Ts, ns0=1e8, phi=t.phi,
+                                  master3=o.master3()) == nn
+    nfiles = _same_dirs(str(lib_dir), str(orc_dir))
+    assert nfiles == (nsteps + 0) * (29 + 3 * t.nq)
+    rd = lambda name: np.fromfile(str(lib_dir / f"{nsteps + 1}_1_1_{name}fort.bin"))
+    assert rd("5_TeNT").tolist() == [t.arrays["Te"][4]]
+    m3 = ks.master3(0)
+    got = rd("7_N2PerpphRT").reshape(t.nvpar, t.nv2, t.nv1).T          # (V1, V2, Vpar)
+    c = m3["n3_counts"][6]
+    assert np.array_equal(got, np.where(c == 0, 0.0, c * (t.nsnormfac / t.arrays["d3xC"][6])))
+    assert np.array_equal(rd("M2Perp2phRT"), m3["M2perp2"]) and rd("M1Perp1phRT").any()
+    assert not os.path.exists(str(lib_dir / f"{nsteps + 1}_1_1_1_NphRTfort.bin"))
+    ks.close()
+    o.close()
+
+
 def test_export_mixed_holds_the_downloads(tmp_path):
     t = grids.config1(n_total=20000)
     nsteps = 4

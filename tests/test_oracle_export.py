@@ -44,7 +44,8 @@ def test_names_and_stream_bytes_match_the_fortran_runtime(probe, tmp_path, nn, s
     assert names == [oexp.file_name(nn, s, f, "TimeT"), oexp.file_name(nn, s, f, "M1Perp2FiltAvrgRT"),
                      oexp.file_name(nn, s, f, "NphRT", Q)]
     # stream writes: no record markers, Fortran (column-major) order, int64
-    got = {n: (tmp_path / n).read_bytes() for n in ("probe_scalar.bin", "probe_int.bin", "probe_array.bin")}
+    got = {n: (tmp_path / n).read_bytes()
+           for n in ("probe_scalar.bin", "probe_int.bin", "probe_array.bin", "probe_array3.bin")}
     mine = tmp_path / "mine"
     mine.mkdir()
     oexp._put(str(mine / "x"), [0.1 * nn])
@@ -54,6 +55,12 @@ def test_names_and_stream_bytes_match_the_fortran_runtime(probe, tmp_path, nn, s
     assert (mine / "x").read_bytes() == got["probe_scalar.bin"]
     assert (mine / "k").read_bytes() == got["probe_int.bin"]
     assert (mine / "a").read_bytes() == got["probe_array.bin"]
+    # 3-D: b(i, j, l) = 100 i + 10 j + l + 1/2 written whole is the layout of
+    # N2PerpphRTp(:, :, :, nn) and d3vCTp; n2perp_rt of counts [Q][V1][V2][Vpar]
+    # = 100 i + 10 j + l (weight 1) must give the same sequence less 1/2
+    b3 = np.frombuffer(got["probe_array3.bin"], "<f8")
+    cnt = np.array([[[100 * i + 10 * j + l for l in range(1, 5)] for j in range(1, 4)] for i in range(1, 3)])
+    assert np.array_equal(oexp.n2perp_rt(cnt[None], 1.0, np.ones(1))[0], b3 - 0.5)
 
 
 def _run(tube, nsteps, ndatfac=1, seed=3):
