@@ -202,6 +202,39 @@ def export_run(ks, tube):
             "note": "kaos_export_master wall time: device-to-host copies and file writes"}
 
 
+def vperp3d_run(args, nsteps=50, warmup=8):
+    """The single-tube workload on the 3-D velocity grid (ION2VPERPflag = 1,
+    the reference's own input setting): 32 x 32 (Vperp1, Vperp2) cells over
+    the tube's Vpar range x the tube's 64 Vpar cells, ions seeded on the
+    device, every step a master step; each form timed over nsteps replayed
+    steps after warmup steps, against the same run on the 2-D grid."""
+    from kaos_amd import grids
+    from kaos_amd.solver import KineticSolver, KAOS_FP64, KAOS_MIXED
+    res = {"grid": "32 x 32 x 64 (Vperp1, Vperp2, Vpar) per config cell", "steps": nsteps}
+    for form, prec in (("fp64", KAOS_FP64), ("mixed", KAOS_MIXED)):
+        for kind in ("3d", "2d"):
+            tube = grids.config2(n_total=args.ions)
+            if kind == "3d":
+                grids.with_vgrid3(tube, 32, 32)
+            counts = tube.cell_counts
+            cap = int(counts.sum()) + (nsteps + warmup + 4) * (tube.n_inj_lb + tube.n_inj_ub) + 1024
+            ks = KineticSolver([tube], precision=prec, ndatfac=1, seed=1, capacity=[cap])
+            ks.seed(0, counts, np.full(tube.nq, tube.Ts_lb), 20241)
+            ks.step(1)
+            ks.run(2, warmup)
+            t1 = time.perf_counter()
+            pushes = ks.run(2 + warmup, nsteps)
+            wall = time.perf_counter() - t1
+            r = {"value": pushes / wall, "ms_per_step": wall / nsteps * 1e3}
+            if kind == "3d":
+                m3 = ks.master3(0)
+                r["binned_fraction"] = float(m3["n3_counts"].sum() / max(ks.master(0)["nq_counts"].sum(), 1))
+            ks.close()
+            res[f"{form}_{kind}"] = r
+            log(f"[bench] vperp3d {form} {kind}: {r['ms_per_step']:.3f} ms/step")
+    return res
+
+
 def batched_run(args):
     """512 tubes, mixed form: timed with graph replay (no per-kernel events), then
     a few eager steps with hipEvents for the push kernel's mean duration."""
@@ -323,6 +356,8 @@ def main():
     ap.add_argument("--full-physics", action="store_true",
                     help="add the heated two-species run (H+ and O+, ICR heating) as 'full_physics'")
     ap.add_argument("--fp-ions", type=int, default=200_000_000, help="ions of the heated run (1/4 H+)")
+    ap.add_argument("--vperp3d", action="store_true",
+                    help="add the single-tube workload on the 3-D (Vperp1, Vperp2, Vpar) grid as 'vperp3d'")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the traffic-calibration copy of MIB MiB and exit")
     args = ap.parse_args()
@@ -390,6 +425,8 @@ def main():
         out["batched"] = batched_run(args)
     if args.full_physics:
         out["full_physics"] = full_physics_run(args)
+    if args.vperp3d:
+        out["vperp3d"] = vperp3d_run(args)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline()

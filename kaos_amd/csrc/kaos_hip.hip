@@ -526,51 +526,158 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_bin_initial(StepArgs a) {
 // the master push removes (previous cell 0 / -1) are not counted.  The three
 // velocity cells are 0 off the grid (no stale index); an ion in a config
 // cell with some velocity cell 0 counts in the config counts only.
-__global__ void __launch_bounds__(KAOS_BLOCK) k_bin3d(StepArgs a, int first, int mixed) {
-  const int tile = blockIdx.x;
-  if (tile >= a.ntiles) return;
-  const int tb = a.tile_tube[tile];
-  const DevTube& T = a.tubes[tb];
-  const long long end = T.off + a.count[tb];
-  const double* vGH = a.vparGH + T.voff;
-  const double* v1GH = a.v1GH + T.v1off;
-  const double* v2GH = a.v2GH + T.v2off;
-  for (int it = 0; it < KAOS_TILE_PPT; ++it) {
-    const long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
-    if (j >= end) break;
-    const uint32_t cw = a.cell[j];
-    if (mixed ? (cw == KAOS_DEAD) : (a.rs[j] == 0.0)) continue;
-    int qc = (int)((cw >> 16) & KAOS_QMASK) - 1;
-    int ip;
-    if (first) {
-      ip = (int)(cw & 0xFFu);
-    } else {
-      if (qc == 0 || qc == -1) continue;
-      double rs, vs;
-      if (mixed) {
-        const int Q = kx_refq(qc, T.nq);
-        rs = a.rref[T.goff + Q - 1] + kx_dec(a.rdi[j], a.rk[T.goff + Q - 1], (cw & KAOS_RFLAG) != 0);
-        vs = kx_dec_v(a.vpi[j], (int)(cw & 0xFFu), (cw & KAOS_VFLAG) != 0, vGH, a.vk + T.voff, T.nvpar);
-      } else {
-        rs = a.rs[j];
-        vs = a.vp[j];
+//
+// A block takes a group of KAOS_BIN3_TILES tiles (at most 64512 ions, so a
+// 16-bit counter cannot overflow) and counts into an LDS window of whole
+// config cells [wq0, wq0 + wnq) of the group's first tube, laid out as the
+// global counts (so the flush adds a contiguous range); the window is centred
+// on the cells of the group's first and last ions, which hold the group's
+// ions when they were last sorted.  A thermal distribution puts most ions of
+// a cell in a few hundred bins, so the window turns ~1 global atomic per ion
+// into ~1 per occupied bin per block.  Ions outside the window (or of another
+// tube) go to global atomics; no result depends on the window.  The cell
+// searches run on LDS copies of the tube's edges, from the ion's previous
+// config cell and from a uniform-spacing guess for the velocity cells: walks
+// with the bisections' results (kaos_find_qcell_from, kaos_find_vcell_guess)
+// and no dependent global loads.
+#define KAOS_BIN3_BLOCK 1024
+#define KAOS_BIN3_TILES 63
+#ifndef KAOS_BIN3_UNROLL
+#define KAOS_BIN3_UNROLL 4                     // tiles whose ions one thread loads together
+#endif
+#ifndef KAOS_BIN3_MAXWORDS
+#define KAOS_BIN3_MAXWORDS 32768              // 128 KB of LDS: one 32x32x64 cell of 16-bit counters
+#endif
+static_assert(KAOS_TILE == KAOS_BIN3_BLOCK, "k_bin3d takes one ion of a tile per thread");
+static_assert(KAOS_BIN3_TILES * KAOS_TILE < 65536, "16-bit window counters");
+// The state words one ion of the binning pass reads (either form).
+template <int MIXED> struct Bin3Ion;
+template <> struct Bin3Ion<0> { uint32_t cw; double rs, vs, v1, v2; };
+template <> struct Bin3Ion<1> { uint32_t cw; int rdi, vpi; float v1, v2; };
+
+__device__ __forceinline__ void bin3_load(const StepArgs& a, long long j, Bin3Ion<0>& x) {
+  x.cw = a.cell[j]; x.rs = a.rs[j]; x.vs = a.vp[j]; x.v1 = a.vr[j]; x.v2 = a.vr2[j];
+}
+__device__ __forceinline__ void bin3_load(const StepArgs& a, long long j, Bin3Ion<1>& x) {
+  x.cw = a.cell[j]; x.rdi = a.rdi[j]; x.vpi = a.vpi[j]; x.v1 = a.vrf[j]; x.v2 = a.vrf2[j];
+}
+__device__ __forceinline__ bool bin3_dead(const Bin3Ion<0>& x) { return x.rs == 0.0; }
+__device__ __forceinline__ bool bin3_dead(const Bin3Ion<1>& x) { return x.cw == KAOS_DEAD; }
+
+template <int MIXED>
+__global__ void __launch_bounds__(KAOS_BIN3_BLOCK) k_bin3d(StepArgs a, int first, int wwords) {
+  extern __shared__ unsigned int s_w3[];
+  __shared__ double s_qGH[KAOS_MAXQ], s_vGH[KAOS_MAXV + 1], s_v1GH[KAOS_MAXV + 1], s_v2GH[KAOS_MAXV + 1];
+  __shared__ double s_rref[MIXED ? KAOS_MAXQ : 1];
+  __shared__ int s_rk[MIXED ? KAOS_MAXQ : 1], s_vk[MIXED ? KAOS_MAXV + 1 : 1];
+  __shared__ int s_wq0, s_wnq;
+  const int t0 = blockIdx.x * KAOS_BIN3_TILES;
+  const int t1 = min(t0 + KAOS_BIN3_TILES, a.ntiles);
+  if (t0 >= t1) return;                                   // uniform over the block
+  const int tid = threadIdx.x;
+  const int tb0 = a.tile_tube[t0];
+  const DevTube& T0 = a.tubes[tb0];
+  const long long nc0 = (long long)T0.nv1 * T0.nv2 * T0.nvpar;
+  if (tid == 0) {
+    const int wnq = (int)min((long long)T0.nq, (2LL * wwords) / nc0);
+    s_wnq = wnq;
+    s_wq0 = (wnq > 0) ? kaos_window_q0(a, t0, t1, tb0, T0.nq, T0.off + a.count[tb0], wnq) : 1;
+  }
+  for (int i = tid; i < wwords; i += KAOS_BIN3_BLOCK) s_w3[i] = 0u;
+  __syncthreads();
+  const int wq0 = s_wq0, wnq = s_wnq;
+  int cur = -1;
+  long long end = 0;
+  double sp = 0.0, s1 = 0.0, s2 = 0.0;           // uniform-spacing guesses of the velocity grids
+  // tiles in runs of up to KAOS_BIN3_UNROLL of one tube: every thread loads its
+  // ion of each tile of the run before binning any (memory-level parallelism)
+  for (int tile = t0; tile < t1;) {
+    const int tb = a.tile_tube[tile];
+    const DevTube& T = a.tubes[tb];
+    if (tb != cur) {                               // uniform over the block: same tiles for every thread
+      __syncthreads();
+      for (int i = tid; i < T.nq; i += KAOS_BIN3_BLOCK) {
+        s_qGH[i] = a.qGH[T.goff + i];
+        if (MIXED) { s_rref[i] = a.rref[T.goff + i]; s_rk[i] = a.rk[T.goff + i]; }
       }
-      const kaos_line_t L = kaos_line(rs, T.p);
-      const kaos_qb_t gA = kaos_stageA(L);
-      qc = kaos_find_qcell(gA.q, a.qGH + T.goff, T.qGL0, T.qGHlast, T.nq, T.desc);
-      const double vpar = gA.bR * (vs * gA.bR) + gA.bZ * (vs * gA.bZ);
-      ip = kaos_find_vcell(vpar, vGH, vGH[T.nvpar], T.nvpar);
+      for (int i = tid; i <= T.nvpar; i += KAOS_BIN3_BLOCK) {
+        s_vGH[i] = a.vparGH[T.voff + i];
+        if (MIXED) s_vk[i] = a.vk[T.voff + i];
+      }
+      for (int i = tid; i <= T.nv1; i += KAOS_BIN3_BLOCK) s_v1GH[i] = a.v1GH[T.v1off + i];
+      for (int i = tid; i <= T.nv2; i += KAOS_BIN3_BLOCK) s_v2GH[i] = a.v2GH[T.v2off + i];
+      const double* vGHg = a.vparGH + T.voff;
+      sp = (double)T.nvpar / (vGHg[T.nvpar - 1] - vGHg[T.nvpar]);
+      s1 = (double)T.nv1 / (a.v1GH[T.v1off + T.nv1 - 1] - a.v1GH[T.v1off + T.nv1]);
+      s2 = (double)T.nv2 / (a.v2GH[T.v2off + T.nv2 - 1] - a.v2GH[T.v2off + T.nv2]);
+      end = T.off + a.count[tb];
+      cur = tb;
+      __syncthreads();
     }
-    if (qc < 1) continue;
-    const double v1 = mixed ? (double)a.vrf[j] : a.vr[j];
-    const double v2 = mixed ? (double)a.vrf2[j] : a.vr2[j];
-    const int i1 = kaos_find_vcell(v1, v1GH, v1GH[T.nv1], T.nv1);
-    const int i2 = kaos_find_vcell(v2, v2GH, v2GH[T.nv2], T.nv2);
-    if (ip >= 1 && i1 >= 1 && i2 >= 1)
-      atomicAdd(&a.hist3[T.h3off + (((long long)(qc - 1) * T.nv1 + (i1 - 1)) * T.nv2 + (i2 - 1)) * T.nvpar +
-                         (ip - 1)], 1u);
-    else
-      atomicAdd(&a.nqoff[T.goff + qc - 1], 1u);
+    int k = 1;
+    while (k < KAOS_BIN3_UNROLL && tile + k < t1 && a.tile_tube[tile + k] == tb) ++k;
+    Bin3Ion<MIXED> xs[KAOS_BIN3_UNROLL];
+    bool ok[KAOS_BIN3_UNROLL];
+#pragma unroll
+    for (int u = 0; u < KAOS_BIN3_UNROLL; ++u) {
+      const long long j = a.tile_start[tile + min(u, k - 1)] + tid;   // KAOS_TILE == KAOS_BIN3_BLOCK
+      ok[u] = (u < k) && (j < end);
+      if (ok[u]) bin3_load(a, j, xs[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < KAOS_BIN3_UNROLL; ++u) {
+      if (!ok[u]) continue;
+      const Bin3Ion<MIXED>& x = xs[u];
+      if (bin3_dead(x)) continue;
+      const uint32_t cw = x.cw;
+      int qc = (int)((cw >> 16) & KAOS_QMASK) - 1;
+      int ip;
+      if (first) {
+        ip = (int)(cw & 0xFFu);
+      } else {
+        if (qc == 0 || qc == -1) continue;
+        double rs, vs;
+        if constexpr (MIXED) {
+          const int Q = kx_refq(qc, T.nq);
+          rs = s_rref[Q - 1] + kx_dec(x.rdi, s_rk[Q - 1], (cw & KAOS_RFLAG) != 0);
+          vs = kx_dec_v(x.vpi, (int)(cw & 0xFFu), (cw & KAOS_VFLAG) != 0, s_vGH, s_vk, T.nvpar);
+        } else {
+          rs = x.rs;
+          vs = x.vs;
+        }
+        const kaos_line_t L = kaos_line(rs, T.p);
+        const kaos_qb_t gA = kaos_stageA(L);
+        qc = kaos_find_qcell_from(gA.q, s_qGH, T.qGL0, T.qGHlast, T.nq, T.desc, qc);
+        const double vpar = gA.bR * (vs * gA.bR) + gA.bZ * (vs * gA.bZ);
+        ip = kaos_find_vcell_guess(vpar, s_vGH, s_vGH[T.nvpar], T.nvpar, sp);
+      }
+      if (qc < 1) continue;
+      const int i1 = kaos_find_vcell_guess((double)x.v1, s_v1GH, s_v1GH[T.nv1], T.nv1, s1);
+      const int i2 = kaos_find_vcell_guess((double)x.v2, s_v2GH, s_v2GH[T.nv2], T.nv2, s2);
+      if (ip >= 1 && i1 >= 1 && i2 >= 1) {
+        const long long r = ((long long)(i1 - 1) * T.nv2 + (i2 - 1)) * T.nvpar + (ip - 1);
+        const int wql = qc - wq0;
+        if (tb == tb0 && (unsigned)wql < (unsigned)wnq) {
+          const long long li = (long long)wql * nc0 + r;
+          atomicAdd(&s_w3[li >> 1], 1u << (16 * (int)(li & 1)));
+        } else {
+          atomicAdd(&a.hist3[T.h3off + (long long)(qc - 1) * T.nv1 * T.nv2 * T.nvpar + r], 1u);
+        }
+      } else {
+        atomicAdd(&a.nqoff[T.goff + qc - 1], 1u);
+      }
+    }
+    tile += k;
+  }
+  __syncthreads();
+  if (wnq > 0) {
+    unsigned int* gb = a.hist3 + T0.h3off + (long long)(wq0 - 1) * nc0;
+    const long long nli = (long long)wnq * nc0;
+    for (long long w = tid; 2 * w < nli; w += KAOS_BIN3_BLOCK) {
+      const unsigned int v = s_w3[w];
+      if (v & 0xFFFFu) atomicAdd(&gb[2 * w], v & 0xFFFFu);
+      if (v >> 16) atomicAdd(&gb[2 * w + 1], v >> 16);      // zero beyond nli (never counted)
+    }
   }
 }
 
@@ -1302,6 +1409,7 @@ struct kaos_ctx {
          *d_dv2 = nullptr, *d_d3v3 = nullptr, *d_M3 = nullptr;
   unsigned int* d_hist3 = nullptr;
   long long nhist3_tot = 0;
+  int bin3_words = 0;                           // LDS window of k_bin3d (32-bit words)
 };
 
 static std::string g_init_err;
@@ -1626,6 +1734,23 @@ static int build_device(kaos_ctx* c) {
   if (c->cfg.flags & KAOS_FLAG_VPERP3D) {
     c->v3d = true;
     c->nhist3_tot = h3off;
+    // window of the binning pass: whole config cells, >= 16384 counters where the
+    // grid is small, one cell up to 65536 counters (larger grids: global atomics only)
+    long long words = 0;
+    for (int t = 0; t < nt; ++t) {
+      const DevTube& T = c->dt[t];
+      const long long nc3 = (long long)T.nv1 * T.nv2 * T.nvpar;
+      if (nc3 > 2LL * KAOS_BIN3_MAXWORDS) continue;
+      const long long wnq = std::min<long long>(T.nq, std::max<long long>(1, 16384 / nc3));
+      words = std::max(words, (wnq * nc3 + 1) / 2);
+    }
+    c->bin3_words = (int)std::min<long long>(words, KAOS_BIN3_MAXWORDS);
+    if (c->bin3_words > 0) {
+      HIPCHK(c, hipFuncSetAttribute((const void*)k_bin3d<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(sizeof(unsigned int) * c->bin3_words)));
+      HIPCHK(c, hipFuncSetAttribute((const void*)k_bin3d<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(sizeof(unsigned int) * c->bin3_words)));
+    }
     HIPCHK(c, up((void**)&c->d_v1GH, v1GHx.data(), sizeof(double) * v1off));
     HIPCHK(c, up((void**)&c->d_v1GC, v1GC.data(), sizeof(double) * v1off));
     HIPCHK(c, up((void**)&c->d_dv1, dv1.data(), sizeof(double) * v1off));
@@ -1970,6 +2095,15 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
   const bool premaster = is_master(c, n + 1);
   hipLaunchKernelGGL(k_step_tick, dim3(1), dim3(1), 0, c->stream, c->d_stepno);
   const int mixed = c->cfg.precision == KAOS_MIXED;
+  auto launch_bin3d = [&](const StepArgs& a, int first) {
+    const int nb = (c->ntiles + KAOS_BIN3_TILES - 1) / KAOS_BIN3_TILES;
+    if (mixed)
+      hipLaunchKernelGGL(k_bin3d<1>, dim3(nb), dim3(KAOS_BIN3_BLOCK), sizeof(unsigned int) * c->bin3_words,
+                         c->stream, a, first, c->bin3_words);
+    else
+      hipLaunchKernelGGL(k_bin3d<0>, dim3(nb), dim3(KAOS_BIN3_BLOCK), sizeof(unsigned int) * c->bin3_words,
+                         c->stream, a, first, c->bin3_words);
+  };
   auto zero_hist = [&]() {
     if (c->v3d) {
       hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, c->stream, c->d_hist3, c->nhist3_tot);
@@ -1985,7 +2119,7 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
     StepArgs a = make_args(c, 1, 0, 0, premaster);
     zero_hist();
     if (c->v3d)
-      hipLaunchKernelGGL(k_bin3d, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a, 1, mixed);
+      launch_bin3d(a, 1);
     else
       hipLaunchKernelGGL(k_bin_initial, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a);
     // field of master step 1 from "master step 0" (no moments: M = 0), then moments of step 1
@@ -2009,7 +2143,7 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
     }
     StepArgs a = make_args(c, master, master, master, premaster);
     if (master && c->v3d)
-      hipLaunchKernelGGL(k_bin3d, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a, 0, mixed);
+      launch_bin3d(a, 0);
     if (c->any_icr && c->cfg.precision != KAOS_MIXED)
       hipLaunchKernelGGL(k_icr, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a);
     launch_push(c, a, e0, e1);
