@@ -589,6 +589,10 @@ __global__ void __launch_bounds__(KAOS_BIN3_BLOCK) k_bin3d(StepArgs a, int first
   int cur = -1;
   long long end = 0;
   double sp = 0.0, s1 = 0.0, s2 = 0.0;           // uniform-spacing guesses of the velocity grids
+  // the tube's scalars in registers (the atomics below could alias the tube table)
+  int nq = 0, desc = 0, nvpar = 0, nv1 = 0, nv2 = 0;
+  long long goff = 0, h3off = 0, nc = 0;
+  double irep = 0.0, qGL0 = 0.0, qGHlast = 0.0;
   // tiles in runs of up to KAOS_BIN3_UNROLL of one tube: every thread loads its
   // ion of each tile of the run before binning any (memory-level parallelism)
   for (int tile = t0; tile < t1;) {
@@ -611,6 +615,9 @@ __global__ void __launch_bounds__(KAOS_BIN3_BLOCK) k_bin3d(StepArgs a, int first
       s1 = (double)T.nv1 / (a.v1GH[T.v1off + T.nv1 - 1] - a.v1GH[T.v1off + T.nv1]);
       s2 = (double)T.nv2 / (a.v2GH[T.v2off + T.nv2 - 1] - a.v2GH[T.v2off + T.nv2]);
       end = T.off + a.count[tb];
+      nq = T.nq; desc = T.desc; nvpar = T.nvpar; nv1 = T.nv1; nv2 = T.nv2;
+      goff = T.goff; h3off = T.h3off; nc = (long long)nv1 * nv2 * nvpar;
+      irep = 1.0 / (KAOS_RE * T.p); qGL0 = T.qGL0; qGHlast = T.qGHlast;
       cur = tb;
       __syncthreads();
     }
@@ -638,33 +645,33 @@ __global__ void __launch_bounds__(KAOS_BIN3_BLOCK) k_bin3d(StepArgs a, int first
         if (qc == 0 || qc == -1) continue;
         double rs, vs;
         if constexpr (MIXED) {
-          const int Q = kx_refq(qc, T.nq);
+          const int Q = kx_refq(qc, nq);
           rs = s_rref[Q - 1] + kx_dec(x.rdi, s_rk[Q - 1], (cw & KAOS_RFLAG) != 0);
-          vs = kx_dec_v(x.vpi, (int)(cw & 0xFFu), (cw & KAOS_VFLAG) != 0, s_vGH, s_vk, T.nvpar);
+          vs = kx_dec_v(x.vpi, (int)(cw & 0xFFu), (cw & KAOS_VFLAG) != 0, s_vGH, s_vk, nvpar);
         } else {
           rs = x.rs;
           vs = x.vs;
         }
-        const kaos_line_t L = kaos_line(rs, T.p);
+        const kaos_line_t L = kaos_line_i(rs, irep);       // kaos_line(rs, p), 1/(RE p) hoisted
         const kaos_qb_t gA = kaos_stageA(L);
-        qc = kaos_find_qcell_from(gA.q, s_qGH, T.qGL0, T.qGHlast, T.nq, T.desc, qc);
+        qc = kaos_find_qcell_from(gA.q, s_qGH, qGL0, qGHlast, nq, desc, qc);
         const double vpar = gA.bR * (vs * gA.bR) + gA.bZ * (vs * gA.bZ);
-        ip = kaos_find_vcell_guess(vpar, s_vGH, s_vGH[T.nvpar], T.nvpar, sp);
+        ip = kaos_find_vcell_guess(vpar, s_vGH, s_vGH[nvpar], nvpar, sp);
       }
       if (qc < 1) continue;
-      const int i1 = kaos_find_vcell_guess((double)x.v1, s_v1GH, s_v1GH[T.nv1], T.nv1, s1);
-      const int i2 = kaos_find_vcell_guess((double)x.v2, s_v2GH, s_v2GH[T.nv2], T.nv2, s2);
+      const int i1 = kaos_find_vcell_guess((double)x.v1, s_v1GH, s_v1GH[nv1], nv1, s1);
+      const int i2 = kaos_find_vcell_guess((double)x.v2, s_v2GH, s_v2GH[nv2], nv2, s2);
       if (ip >= 1 && i1 >= 1 && i2 >= 1) {
-        const long long r = ((long long)(i1 - 1) * T.nv2 + (i2 - 1)) * T.nvpar + (ip - 1);
+        const int r = ((i1 - 1) * nv2 + (i2 - 1)) * nvpar + (ip - 1);
         const int wql = qc - wq0;
         if (tb == tb0 && (unsigned)wql < (unsigned)wnq) {
-          const long long li = (long long)wql * nc0 + r;
-          atomicAdd(&s_w3[li >> 1], 1u << (16 * (int)(li & 1)));
+          const int li = wql * (int)nc + r;                  // < 2 * KAOS_BIN3_MAXWORDS
+          atomicAdd(&s_w3[li >> 1], 1u << (16 * (li & 1)));
         } else {
-          atomicAdd(&a.hist3[T.h3off + (long long)(qc - 1) * T.nv1 * T.nv2 * T.nvpar + r], 1u);
+          atomicAdd(&a.hist3[h3off + (long long)(qc - 1) * nc + r], 1u);
         }
       } else {
-        atomicAdd(&a.nqoff[T.goff + qc - 1], 1u);
+        atomicAdd(&a.nqoff[goff + qc - 1], 1u);
       }
     }
     tile += k;
