@@ -198,6 +198,34 @@ int kaos_run(kaos_ctx* ctx, int64_t n_first, int64_t nsteps, int64_t* particle_s
 int kaos_download_master(kaos_ctx* ctx, int32_t tube, int64_t* nq_counts, int64_t* nph_counts,
                          double* NqRT, double* M, double* EA, double* EG, double* esc);
 
+/* Spin-up (restart) output of DataExport2Sub (SPINUPflag = 1,
+ * _8_0_2_DataExport2.f90:704-802): the caller's grid bounds, injection
+ * normalisations and replenish flags, which this path does not hold. */
+typedef struct kaos_spinup_info {
+  const char* dir;         /* Densitydatadir, NUL-terminated (trailing '/' optional) */
+  int32_t species;         /* s (1-based) */
+  int32_t flux_tube;       /* f (1-based) */
+  int64_t nq_lb, nq_ub;    /* NqLBT(1), NqUBT(1) */
+  double nsnorm_clb;       /* nsnormCLBGT(1) */
+  double nsnorm_cub;       /* nsnormCUBGT(1) */
+  int64_t lb_replenish;    /* LBREPLENISHflagT(1) */
+  int64_t ub_replenish;    /* UBREPLENISHflagT(1) */
+} kaos_spinup_info;
+
+/* The spin-up files of the last master step (step 1 or a master step) of a
+ * tube, <s>_<f>_<Name>fort.bin in info->dir, replaced at every call as the
+ * reference replaces them at every export: NqLBT, NqUBT (int64),
+ * DensityOutputRT = M0phRT, TemperatureOutputRT = (2/3) M2ParphRT/kB +
+ * (2/3) M2Perp1phRT/kB + (2/3) M2Perp2phRT/kB (_8_6_6_SecondParIonMoment.f90:
+ * 659-673), EAInertialOutputRT (0: EAINERTIALflag off), EAPressureOutputRT,
+ * EAmagOutputRT (_8_7_AmbipolarEfield.f90:1419-1421), nsnormCLBOutput,
+ * nsnormCUBOutput (real64), LBREPLENISHflagTOutput, UBREPLENISHflagTOutput
+ * (int64).  Needs KAOS_FLAG_VPERP3D: the reference allocates M2Perp1phRT and
+ * M2Perp2phRT only with ION2VPERPflag = 1 (_8_0_5_DataTypeAllocB.f90:507-519),
+ * so its spin-up temperature is defined only there (KAOS_E_STATE otherwise).
+ * KAOS_E_IO if a file cannot be written. */
+int kaos_export_spinup(kaos_ctx* ctx, int32_t tube, const kaos_spinup_info* info);
+
 /* The 3-D outputs of the last master step of a tube (KAOS_FLAG_VPERP3D):
  * n3_counts (nq*nvperp1*nvperp2*nvpar, [Q][V1][V2][Vpar], Vpar fastest):
  * N2PerpphReNormRT (_8_3_1_IonParticleCounts.f90:336-361); M (8*nq): M0phRT,

@@ -56,6 +56,12 @@ class KaosExportInfo(C.Structure):
                 ("ns0", C.c_double), ("phi", C.c_double), ("theta", _dp), ("Ts", _dp)]
 
 
+class KaosSpinupInfo(C.Structure):
+    _fields_ = [("dir", C.c_char_p), ("species", C.c_int32), ("flux_tube", C.c_int32),
+                ("nq_lb", C.c_int64), ("nq_ub", C.c_int64), ("nsnorm_clb", C.c_double),
+                ("nsnorm_cub", C.c_double), ("lb_replenish", C.c_int64), ("ub_replenish", C.c_int64)]
+
+
 class KaosStepOut(C.Structure):
     _fields_ = [("n_alive", C.c_int64), ("n_injected", C.c_int64), ("master", C.c_int32),
                 ("status", C.c_int32)]
@@ -63,7 +69,7 @@ class KaosStepOut(C.Structure):
 
 EXPORTS = ("kaos_init", "kaos_set_grid", "kaos_upload_particles", "kaos_upload_vperp", "kaos_cell_counts",
            "kaos_seed_particles", "kaos_step", "kaos_run",
-           "kaos_download_master", "kaos_download_master3d", "kaos_export_master", "kaos_count", "kaos_download_particles", "kaos_set_launch",
+           "kaos_download_master", "kaos_download_master3d", "kaos_export_master", "kaos_export_spinup", "kaos_count", "kaos_download_particles", "kaos_set_launch",
            "kaos_get_launch", "kaos_profile", "kaos_profile_read", "kaos_last_error", "kaos_finalize")
 
 _lib = None
@@ -90,6 +96,7 @@ def load(path=LIB_PATH):
     lib.kaos_run.argtypes = [P, C.c_int64, C.c_int64, i64p]
     lib.kaos_download_master.argtypes = [P, C.c_int32, i64p, i64p, _dp, _dp, _dp, _dp, _dp]
     lib.kaos_export_master.argtypes = [P, C.c_int32, C.POINTER(KaosExportInfo), i64p]
+    lib.kaos_export_spinup.argtypes = [P, C.c_int32, C.POINTER(KaosSpinupInfo)]
     lib.kaos_download_master3d.argtypes = [P, C.c_int32, i64p, _dp]
     lib.kaos_count.argtypes = [P, C.c_int32, i64p]
     lib.kaos_download_particles.argtypes = [P, C.c_int32, C.c_int64, i64p] + [_dp] * 10 + [i32p] * 3

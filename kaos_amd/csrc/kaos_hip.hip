@@ -2513,6 +2513,53 @@ extern "C" int kaos_export_master(kaos_ctx* c, int32_t tube, const kaos_export_i
   return KAOS_OK;
 }
 
+extern "C" int kaos_export_spinup(kaos_ctx* c, int32_t tube, const kaos_spinup_info* info) {
+  if (!c) return KAOS_E_ARG;
+  if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
+  if (!info || !info->dir || info->species < 1 || info->flux_tube < 1)
+    return fail(c, KAOS_E_ARG, "spin-up info needs dir and 1-based species/flux_tube");
+  const long long n = c->last_step;
+  if (!c->built || !(n == 1 || is_master(c, n)))
+    return fail(c, KAOS_E_STATE, "spin-up export after step 1 or a master step only");
+  if (!c->v3d)
+    return fail(c, KAOS_E_STATE, "the spin-up temperature needs the 3-D moments (KAOS_FLAG_VPERP3D)");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const DevTube& T = c->dt[tube];
+  const int nq = T.nq;
+  std::vector<double> M3(8 * (size_t)nq), EA(nq);
+  HIPCHK(c, hipMemcpy(M3.data(), c->d_M3 + 8 * T.goff, sizeof(double) * 8 * nq, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(EA.data(), c->d_EA + T.goff, sizeof(double) * nq, hipMemcpyDeviceToHost));
+  // TemperatureOutputRT, evaluated left to right as the reference's statement:
+  // ((2/3) M2Par / kB + (2/3) M2Perp1 / kB) + (2/3) M2Perp2 / kB
+  std::vector<double> Tout(nq), EAmag(nq), zeros(nq, 0.0);
+  const double c23 = 2.0 / 3.0;
+  for (int Q = 0; Q < nq; ++Q) {
+    const double m2par = M3[7 * (size_t)nq + Q], m2p1 = M3[5 * (size_t)nq + Q], m2p2 = M3[6 * (size_t)nq + Q];
+    Tout[Q] = ((c23 * m2par) / KAOS_KB + (c23 * m2p1) / KAOS_KB) + (c23 * m2p2) / KAOS_KB;
+    EAmag[Q] = 0.0 + EA[Q];
+  }
+  std::string dir(info->dir);
+  if (!dir.empty() && dir.back() != '/') dir += '/';
+  const std::string pre = dir + std::to_string(info->species) + "_" + std::to_string(info->flux_tube) + "_";
+  bool ok = true;
+  auto put = [&](const char* name, const void* p, size_t bytes) {
+    if (ok && !export_raw(pre + name + "fort.bin", p, bytes)) ok = false;
+  };
+  put("NqLBT", &info->nq_lb, sizeof(int64_t));
+  put("NqUBT", &info->nq_ub, sizeof(int64_t));
+  put("DensityOutputRT", M3.data(), sizeof(double) * nq);
+  put("TemperatureOutputRT", Tout.data(), sizeof(double) * nq);
+  put("EAInertialOutputRT", zeros.data(), sizeof(double) * nq);
+  put("EAPressureOutputRT", EA.data(), sizeof(double) * nq);
+  put("EAmagOutputRT", EAmag.data(), sizeof(double) * nq);
+  put("nsnormCLBOutput", &info->nsnorm_clb, sizeof(double));
+  put("nsnormCUBOutput", &info->nsnorm_cub, sizeof(double));
+  put("LBREPLENISHflagTOutput", &info->lb_replenish, sizeof(int64_t));
+  put("UBREPLENISHflagTOutput", &info->ub_replenish, sizeof(int64_t));
+  if (!ok) return fail(c, KAOS_E_IO, "cannot write the spin-up files under " + dir);
+  return KAOS_OK;
+}
+
 extern "C" int kaos_download_particles(kaos_ctx* c, int32_t tube, int64_t n_cap, int64_t* n_out,
                                        double* x, double* y, double* z, double* Vx, double* Vy, double* Vz,
                                        double* vperp, double* qk4, double* aea, double* ag,

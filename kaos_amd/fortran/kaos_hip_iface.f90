@@ -53,6 +53,15 @@ module kaos_hip
     type(c_ptr) :: theta, Ts
   end type kaos_export_info
 
+  ! spin-up (restart) files: dir = c_loc of a NUL-terminated Densitydatadir
+  type, bind(c), public :: kaos_spinup_info
+    type(c_ptr) :: dir
+    integer(c_int32_t) :: species, flux_tube
+    integer(c_int64_t) :: nq_lb, nq_ub
+    real(c_double) :: nsnorm_clb, nsnorm_cub
+    integer(c_int64_t) :: lb_replenish, ub_replenish
+  end type kaos_spinup_info
+
   type, bind(c), public :: kaos_step_out
     integer(c_int64_t) :: n_alive, n_injected
     integer(c_int32_t) :: master, status
@@ -60,7 +69,8 @@ module kaos_hip
 
   public :: kaos_init, kaos_set_grid, kaos_upload_particles, kaos_upload_vperp, kaos_cell_counts, &
             kaos_seed_particles, kaos_step, kaos_run, &
-            kaos_download_master, kaos_download_master3d, kaos_export_master, kaos_count, kaos_download_particles, kaos_set_launch, &
+            kaos_download_master, kaos_download_master3d, kaos_export_master, kaos_export_spinup, &
+            kaos_count, kaos_download_particles, kaos_set_launch, &
             kaos_get_launch, kaos_profile, &
             kaos_profile_read, kaos_last_error, kaos_finalize, kaos_error_message
 
@@ -148,6 +158,14 @@ module kaos_hip
       integer(c_int32_t), value :: tube
       type(c_ptr), value :: n3_counts, M
     end function kaos_download_master3d
+
+    ! DataExport2Sub's spin-up files (SPINUPflag = 1) of the last master step
+    integer(c_int) function kaos_export_spinup(ctx, tube, info) bind(c, name='kaos_export_spinup')
+      import :: c_int, c_ptr, c_int32_t, kaos_spinup_info
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tube
+      type(kaos_spinup_info), intent(in) :: info
+    end function kaos_export_spinup
 
     ! DataExport2Sub's files of the last master step
     integer(c_int) function kaos_export_master(ctx, tube, info, nn) bind(c, name='kaos_export_master')

@@ -159,6 +159,19 @@ class KineticSolver:
             out[n] = M[i * nq:(i + 1) * nq].copy()
         return out
 
+    def export_spinup(self, tube, directory, nq_lb=1, nq_ub=None, nsnorm_clb=None, nsnorm_cub=None,
+                      lb_replenish=0, ub_replenish=0, species=1, flux_tube=1):
+        """kaos_export_spinup: the spin-up files of the last master step
+        (3-D grid only); defaults: cells 1..nq, the tube's injection counts."""
+        t = self.tubes[tube]
+        info = _abi.KaosSpinupInfo(
+            dir=str(directory).encode(), species=species, flux_tube=flux_tube, nq_lb=nq_lb,
+            nq_ub=t.nq if nq_ub is None else nq_ub,
+            nsnorm_clb=float(t.n_inj_lb if nsnorm_clb is None else nsnorm_clb),
+            nsnorm_cub=float(t.n_inj_ub if nsnorm_cub is None else nsnorm_cub),
+            lb_replenish=lb_replenish, ub_replenish=ub_replenish)
+        self._chk(self.lib.kaos_export_spinup(self.h, tube, C.byref(info)))
+
     def master3(self, tube):
         """3-D outputs of the last master step (KAOS_FLAG_VPERP3D): counts
         [Q][V1][V2][Vpar] and the eight 3-D moments."""

@@ -118,3 +118,38 @@ def export_master(directory, tube, master, n, ndatfac, theta, Ts, ns0, phi, s=1,
     put("EPmagRT", z)
     put("PhiParRT", z)
     return nn
+
+
+KB = 1.38e-23                       # kB = 138d-25 (_KineticMainParams.f90)
+
+
+def spinup_name(s, f, name):
+    """<s>_<f>_<name>fort.bin (_8_0_2_DataExport2.f90:706-712)."""
+    return f"{s}_{f}_{name}fort.bin"
+
+
+def export_spinup(directory, tube, master, master3, nq_lb, nq_ub, nsnorm_clb, nsnorm_cub,
+                  lb_replenish, ub_replenish, s=1, f=1):
+    """The spin-up files (SPINUPflag = 1, _8_0_2_DataExport2.f90:704-802) of
+    the last master step: DensityOutputRT = M0phRT and TemperatureOutputRT =
+    (2/3) M2ParphRT/kB + (2/3) M2Perp1phRT/kB + (2/3) M2Perp2phRT/kB
+    (_8_6_6_SecondParIonMoment.f90:659-673, left to right); the EA outputs of
+    _8_7_AmbipolarEfield.f90:1419-1421 (EAINERTIALflag off: 0); the caller's
+    NqLBT, NqUBT, nsnormC(L/U)BGT and replenish flags."""
+    put = lambda name, v, dt="<f8": _put(os.path.join(directory, spinup_name(s, f, name)), v, dt)
+    c23 = 2.0 / 3.0
+    T = np.array([((c23 * a) / KB + (c23 * b) / KB) + (c23 * c) / KB
+                  for a, b, c in zip(master3["M2par"], master3["M2perp1"], master3["M2perp2"])])
+    z = np.zeros(tube.nq)
+    put("NqLBT", [nq_lb], "<i8")
+    put("NqUBT", [nq_ub], "<i8")
+    put("DensityOutputRT", master3["M0"])
+    put("TemperatureOutputRT", T)
+    put("EAInertialOutputRT", z)
+    put("EAPressureOutputRT", master["EA"])
+    put("EAmagOutputRT", 0.0 + master["EA"])
+    put("nsnormCLBOutput", [nsnorm_clb])
+    put("nsnormCUBOutput", [nsnorm_cub])
+    put("LBREPLENISHflagTOutput", [lb_replenish], "<i8")
+    put("UBREPLENISHflagTOutput", [ub_replenish], "<i8")
+    return T

@@ -4,7 +4,8 @@
 !  - the file names built from I5 edits with adjustl/adjustr into
 !    character(50) variables (_8_0_2_DataExport2.f90:32-41, 129-141;
 !    declarations _KineticMainParams.f90:35-47, 346-358), printed one a line
-!    for the (nn, s, f, Qind) given on the command line;
+!    for the (nn, s, f, Qind) given on the command line, then a spin-up name
+!    (:706-712);
 !  - the bytes of open(status='replace', form='unformatted', access='stream')
 !    + write + close of a real(dp) scalar, an integer(dp) scalar, a real(dp)
 !    (3, 4) array and a real(dp) (2, 3, 4) array written whole as a(:, :, :)
@@ -39,6 +40,11 @@ program export_probe
   expstring = adjustl(adjustr(nnstring) // '_' // adjustl(adjustr(sstring) // '_' // &
               adjustl(adjustr(fstring) // '_' // adjustl(adjustr(Qindstring) // '_'))))
   fname = adjustl(adjustr(expstring) // adjustl(adjustr('NphRTfort.bin')))
+  print '(A)', trim(fname)
+
+  ! spin-up files (_8_0_2_DataExport2.f90:706-712): species and flux tube only
+  expstring = adjustl(adjustr(sstring) // '_' // adjustl(adjustr(fstring) // '_'))
+  fname = adjustl(adjustr(expstring) // adjustl(adjustr('TemperatureOutputRTfort.bin')))
   print '(A)', trim(fname)
 
   x = 0.1_dp * real(nn, dp)

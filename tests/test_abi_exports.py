@@ -54,6 +54,8 @@ int main(void) {
          offsetof(kaos_export_info, ns0), offsetof(kaos_export_info, theta), offsetof(kaos_export_info, Ts));
   printf("%zu %zu %zu %zu\n", offsetof(kaos_grid, nvperp1), offsetof(kaos_grid, vperp1GL),
          offsetof(kaos_grid, dvperp2), offsetof(kaos_grid, d3v3));
+  printf("%zu %zu %zu %zu\n", sizeof(kaos_spinup_info), offsetof(kaos_spinup_info, nq_lb),
+         offsetof(kaos_spinup_info, nsnorm_cub), offsetof(kaos_spinup_info, ub_replenish));
   return 0;
 }
 '''
@@ -76,5 +78,7 @@ int main(void) {
             C.sizeof(_abi.KaosExportInfo), _abi.KaosExportInfo.species.offset, _abi.KaosExportInfo.ns0.offset,
             _abi.KaosExportInfo.theta.offset, _abi.KaosExportInfo.Ts.offset,
             _abi.KaosGrid.nvperp1.offset, _abi.KaosGrid.vperp1GL.offset, _abi.KaosGrid.dvperp2.offset,
-            _abi.KaosGrid.d3v3.offset]
+            _abi.KaosGrid.d3v3.offset,
+            C.sizeof(_abi.KaosSpinupInfo), _abi.KaosSpinupInfo.nq_lb.offset, _abi.KaosSpinupInfo.nsnorm_cub.offset,
+            _abi.KaosSpinupInfo.ub_replenish.offset]
     assert got == want

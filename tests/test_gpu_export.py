@@ -114,6 +114,14 @@ def test_export_3d_grid_fp64_byte_identical_to_oracle(tmp_path):
     assert np.array_equal(got, np.where(c == 0, 0.0, c * (t.nsnormfac / t.arrays["d3xC"][6])))
     assert np.array_equal(rd("M2Perp2phRT"), m3["M2perp2"]) and rd("M1Perp1phRT").any()
     assert not os.path.exists(str(lib_dir / f"{nsteps + 1}_1_1_1_NphRTfort.bin"))
+    # the spin-up files of the last master step (SPINUPflag = 1)
+    su_lib, su_orc = tmp_path / "su_lib", tmp_path / "su_orc"
+    su_lib.mkdir()
+    su_orc.mkdir()
+    ks.export_spinup(0, str(su_lib), nq_lb=3, nq_ub=t.nq + 2, nsnorm_clb=123.5, nsnorm_cub=0.25,
+                     lb_replenish=1, ub_replenish=0, species=2, flux_tube=7)
+    oexp.export_spinup(str(su_orc), t, o.master(), o.master3(), 3, t.nq + 2, 123.5, 0.25, 1, 0, s=2, f=7)
+    assert _same_dirs(str(su_lib), str(su_orc)) == 11
     ks.close()
     o.close()
 
@@ -166,4 +174,7 @@ def test_export_errors(tmp_path):
     with pytest.raises(ValueError):
         ks.export_master(0, str(tmp_path), theta[:-1], Ts, ns0=1e8, phi=t.phi)
     assert ks.export_master(0, str(tmp_path), theta, Ts, ns0=1e8, phi=t.phi) == 2
+    with pytest.raises(KaosError) as e:                 # spin-up files need the 3-D moments
+        ks.export_spinup(0, str(tmp_path))
+    assert e.value.code == 4
     ks.close()

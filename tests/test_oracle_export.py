@@ -42,7 +42,7 @@ def test_names_and_stream_bytes_match_the_fortran_runtime(probe, tmp_path, nn, s
     assert r.returncode == 0, r.stderr
     names = r.stdout.split()
     assert names == [oexp.file_name(nn, s, f, "TimeT"), oexp.file_name(nn, s, f, "M1Perp2FiltAvrgRT"),
-                     oexp.file_name(nn, s, f, "NphRT", Q)]
+                     oexp.file_name(nn, s, f, "NphRT", Q), oexp.spinup_name(s, f, "TemperatureOutputRT")]
     # stream writes: no record markers, Fortran (column-major) order, int64
     got = {n: (tmp_path / n).read_bytes()
            for n in ("probe_scalar.bin", "probe_int.bin", "probe_array.bin", "probe_array3.bin")}
@@ -122,4 +122,27 @@ def test_m1par_filter_uses_the_previous_master_step(w1):
         lib.orc_filter(ptr(np.ascontiguousarray(prev["M1par"])), ptr(other), t.nq, 7)
         assert not np.array_equal(m["M1F"], other)
         prev = m
+    o.close()
+
+
+def test_spinup_temperature_of_a_maxwellian(tmp_path):
+    # TemperatureOutputRT = (2/3)(M2Par + M2Perp1 + M2Perp2)/kB of ions seeded
+    # from a Maxwellian at temperature T0 is T0 up to the binning of the
+    # velocity grid and sampling (a dropped term gives 2/3 T0, a wrong factor
+    # 3/2 or 1/3 misses by 50% or more); DensityOutputRT is M0
+    t = grids.with_vgrid3(grids.dipole_tube(nq=16, nvpar=40, nvperp=20, n_total=2000, ratio=1.1), 40, 40,
+                          vperp_max=1.2e4)
+    T0 = 5000.0
+    counts = np.full(t.nq, 4000, np.int64)
+    ions = oracle.seed_ions(t, counts, np.full(t.nq, T0), seed=7)
+    o = oracle.Oracle(t, ions["q0"], ions["vpar"], ions["vperp"], capacity=len(ions["q0"]) + 100)
+    o.set_vperp(ions["v1"], ions["v2"])
+    o.step(1)
+    m, m3 = o.master(), o.master3()
+    T = oexp.export_spinup(str(tmp_path), t, m, m3, 1, t.nq, 3.0, 4.0, 0, 1)
+    assert np.all(np.abs(T / T0 - 1.0) < 0.05), T / T0
+    rd = lambda name, dt="<f8": np.fromfile(str(tmp_path / oexp.spinup_name(1, 1, name)), dtype=dt)
+    assert np.array_equal(rd("TemperatureOutputRT"), T) and np.array_equal(rd("DensityOutputRT"), m3["M0"])
+    assert rd("NqUBT", "<i8").tolist() == [t.nq] and rd("UBREPLENISHflagTOutput", "<i8").tolist() == [1]
+    assert len(os.listdir(tmp_path)) == 11
     o.close()
