@@ -86,6 +86,7 @@ struct StepArgs {
   const int *rk, *vk;                        // fixed-point exponents per config / Vpar cell
   int lds_nq, lds_nv;                        // largest nq / nvpar over the tubes (LDS sizing)
   uint32_t* cell;
+  uint16_t* c12;                             // mixed form, 3-D grid: Vperp1 | Vperp2 << 8 cells (binary64)
   int* id;                                   // order key: upload index, then injection order
   const long long* count;
   unsigned int *hist, *nqoff;               // phase-space counts; config counts off the v-grid
@@ -553,13 +554,14 @@ static_assert(KAOS_BIN3_TILES * KAOS_TILE < 65536, "16-bit window counters");
 // The state words one ion of the binning pass reads (either form).
 template <int MIXED> struct Bin3Ion;
 template <> struct Bin3Ion<0> { uint32_t cw; double rs, vs, v1, v2; };
-template <> struct Bin3Ion<1> { uint32_t cw; int rdi, vpi; float v1, v2; };
+template <> struct Bin3Ion<1> { uint32_t cw; int rdi, vpi; float v1, v2; uint32_t c12; };
 
 __device__ __forceinline__ void bin3_load(const StepArgs& a, long long j, Bin3Ion<0>& x) {
   x.cw = a.cell[j]; x.rs = a.rs[j]; x.vs = a.vp[j]; x.v1 = a.vr[j]; x.v2 = a.vr2[j];
 }
 __device__ __forceinline__ void bin3_load(const StepArgs& a, long long j, Bin3Ion<1>& x) {
   x.cw = a.cell[j]; x.rdi = a.rdi[j]; x.vpi = a.vpi[j]; x.v1 = a.vrf[j]; x.v2 = a.vrf2[j];
+  x.c12 = a.c12[j];
 }
 __device__ __forceinline__ bool bin3_dead(const Bin3Ion<0>& x) { return x.rs == 0.0; }
 __device__ __forceinline__ bool bin3_dead(const Bin3Ion<1>& x) { return x.cw == KAOS_DEAD; }
@@ -590,7 +592,7 @@ __global__ void __launch_bounds__(KAOS_BIN3_BLOCK) k_bin3d(StepArgs a, int first
   long long end = 0;
   double sp = 0.0, s1 = 0.0, s2 = 0.0;           // uniform-spacing guesses of the velocity grids
   // the tube's scalars in registers (the atomics below could alias the tube table)
-  int nq = 0, desc = 0, nvpar = 0, nv1 = 0, nv2 = 0;
+  int nq = 0, desc = 0, nvpar = 0, nv1 = 0, nv2 = 0, heated = 0;
   long long goff = 0, h3off = 0, nc = 0;
   double irep = 0.0, qGL0 = 0.0, qGHlast = 0.0;
   // tiles in runs of up to KAOS_BIN3_UNROLL of one tube: every thread loads its
@@ -615,7 +617,7 @@ __global__ void __launch_bounds__(KAOS_BIN3_BLOCK) k_bin3d(StepArgs a, int first
       s1 = (double)T.nv1 / (a.v1GH[T.v1off + T.nv1 - 1] - a.v1GH[T.v1off + T.nv1]);
       s2 = (double)T.nv2 / (a.v2GH[T.v2off + T.nv2 - 1] - a.v2GH[T.v2off + T.nv2]);
       end = T.off + a.count[tb];
-      nq = T.nq; desc = T.desc; nvpar = T.nvpar; nv1 = T.nv1; nv2 = T.nv2;
+      nq = T.nq; desc = T.desc; nvpar = T.nvpar; nv1 = T.nv1; nv2 = T.nv2; heated = T.icr;
       goff = T.goff; h3off = T.h3off; nc = (long long)nv1 * nv2 * nvpar;
       irep = 1.0 / (KAOS_RE * T.p); qGL0 = T.qGL0; qGHlast = T.qGHlast;
       cur = tb;
@@ -659,8 +661,18 @@ __global__ void __launch_bounds__(KAOS_BIN3_BLOCK) k_bin3d(StepArgs a, int first
         ip = kaos_find_vcell_guess(vpar, s_vGH, s_vGH[nvpar], nvpar, sp);
       }
       if (qc < 1) continue;
-      const int i1 = kaos_find_vcell_guess((double)x.v1, s_v1GH, s_v1GH[nv1], nv1, s1);
-      const int i2 = kaos_find_vcell_guess((double)x.v2, s_v2GH, s_v2GH[nv2], nv2, s2);
+      int i1 = 0, i2 = 0;
+      bool have12 = false;
+      if constexpr (MIXED) {
+        if (!heated) {                               // cells of the binary64 components
+          i1 = (int)(x.c12 & 0xFFu); i2 = (int)((x.c12 >> 8) & 0xFFu);
+          have12 = true;
+        }
+      }
+      if (!have12) {
+        i1 = kaos_find_vcell_guess((double)x.v1, s_v1GH, s_v1GH[nv1], nv1, s1);
+        i2 = kaos_find_vcell_guess((double)x.v2, s_v2GH, s_v2GH[nv2], nv2, s2);
+      }
       if (ip >= 1 && i1 >= 1 && i2 >= 1) {
         const int r = ((i1 - 1) * nv2 + (i2 - 1)) * nvpar + (ip - 1);
         const int wql = qc - wq0;
@@ -686,6 +698,15 @@ __global__ void __launch_bounds__(KAOS_BIN3_BLOCK) k_bin3d(StepArgs a, int first
       if (v >> 16) atomicAdd(&gb[2 * w + 1], v >> 16);      // zero beyond nli (never counted)
     }
   }
+}
+
+// Vperp1 | Vperp2 << 8 cells (0 off the grid) of a binary64 perpendicular
+// velocity: the mixed form keeps them beside its binary32 components, so that
+// its 3-D counts bin the components the FP64 form bins.
+__device__ __forceinline__ uint16_t kd_cells12(const DevTube& T, const StepArgs& a, double v1, double v2) {
+  const int i1 = kaos_find_vcell(v1, a.v1GH + T.v1off, a.v1GH[T.v1off + T.nv1], T.nv1);
+  const int i2 = kaos_find_vcell(v2, a.v2GH + T.v2off, a.v2GH[T.v2off + T.nv2], T.nv2);
+  return (uint16_t)(i1 | (i2 << 8));
 }
 
 // State of one initial ion in slot j: signed radius rs at dipole coordinate
@@ -714,6 +735,7 @@ __device__ __forceinline__ void kd_store_initial(const DevTube& T, const StepArg
     a.vpi[j] = kx_enc_v(vpar, ip, vparGH, a.vk + T.voff, T.nvpar, vflag);
     a.vrf[j] = (float)v1;
     a.vrf2[j] = (float)v2;
+    if (a.c12) a.c12[j] = kd_cells12(T, a, v1, v2);
     a.qkf[j] = (float)q;
     a.aef[j] = 0.0f; a.agf[j] = 0.0f;
     a.cell[j] = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip |
@@ -766,8 +788,12 @@ __global__ void k_set_vperp(DevTube T, long long n, const double* v1, const doub
   const double vp = fabs(sqrt(v1[i] * v1[i] + v2[i] * v2[i]));
   const int iq = kaos_find_vcell(vp, vperpGH, vperpGH[T.nvperp], T.nvperp);
   a.cell[j] = (a.cell[j] & ~(0xFFu << 8)) | ((uint32_t)iq << 8);
-  if (mixed) { a.vrf[j] = (float)v1[i]; a.vrf2[j] = (float)v2[i]; }
-  else { a.vr[j] = v1[i]; a.vr2[j] = v2[i]; }
+  if (mixed) {
+    a.vrf[j] = (float)v1[i]; a.vrf2[j] = (float)v2[i];
+    if (a.c12) a.c12[j] = kd_cells12(T, a, v1[i], v2[i]);
+  } else {
+    a.vr[j] = v1[i]; a.vr2[j] = v2[i];
+  }
 }
 
 // Step counter on the device (the injection stream is keyed by the step), so
@@ -856,6 +882,7 @@ __global__ void k_inject(StepArgs a, const long long* inj_base, const long long*
     a.vpi[j] = kx_enc_v(vpar, ip, a.vparGH + T.voff, a.vk + T.voff, T.nvpar, vflag);
     a.vrf[j] = (float)v1;
     a.vrf2[j] = (float)v2;
+    if (a.c12) a.c12[j] = kd_cells12(T, a, v1, v2);
     a.qkf[j] = (float)qc;
     a.aef[j] = (float)aea; a.agf[j] = (float)agv;
     flags = (rflag ? KAOS_RFLAG : 0u) | (vflag ? KAOS_VFLAG : 0u);
@@ -1295,6 +1322,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_reorder_scatter(StepArgs a, Step
       b.qk[d] = a.qk[j]; b.ae[d] = a.ae[j]; b.ag[d] = a.ag[j];
     }
     b.cell[d] = a.cell[j];
+    if (a.c12) b.c12[d] = a.c12[j];
     b.id[d] = a.id[j];
   }
 }
@@ -1348,6 +1376,7 @@ struct StateBufs {
   float *vrf = nullptr, *qkf = nullptr, *aef = nullptr, *agf = nullptr;
   float* vrf2 = nullptr;
   uint32_t* cell = nullptr;
+  uint16_t* c12 = nullptr;
   int* id = nullptr;
 };
 
@@ -1706,6 +1735,10 @@ static int build_device(kaos_ctx* c) {
     HIPCHK(c, hipMalloc((void**)&B.cell, sizeof(uint32_t) * nslot));
     HIPCHK(c, hipMemset(B.cell, 0xFF, sizeof(uint32_t) * nslot));
     HIPCHK(c, hipMalloc((void**)&B.id, sizeof(int) * nslot));
+    if (mixed && (c->cfg.flags & KAOS_FLAG_VPERP3D)) {
+      HIPCHK(c, hipMalloc((void**)&B.c12, sizeof(uint16_t) * nslot));
+      HIPCHK(c, hipMemset(B.c12, 0, sizeof(uint16_t) * nslot));
+    }
   }
   c->nkeys_tot = goff + 2LL * nt;
   HIPCHK(c, hipMalloc((void**)&c->d_keycnt, sizeof(unsigned int) * c->nkeys_tot));
@@ -1796,7 +1829,7 @@ static int build_device(kaos_ctx* c) {
 static void set_state(StepArgs& a, const StateBufs& B) {
   a.rs = B.rs; a.vp = B.vp; a.vr = B.vr; a.qk = B.qk; a.ae = B.ae; a.ag = B.ag; a.vr2 = B.vr2;
   a.rdi = B.rdi; a.vpi = B.vpi; a.vrf = B.vrf; a.qkf = B.qkf; a.aef = B.aef; a.agf = B.agf; a.vrf2 = B.vrf2;
-  a.cell = B.cell; a.id = B.id;
+  a.cell = B.cell; a.id = B.id; a.c12 = B.c12;
 }
 
 static StepArgs make_args(kaos_ctx* c, int interp, int count, int remove, int qk) {
@@ -2709,7 +2742,7 @@ extern "C" int kaos_finalize(kaos_ctx* c) {
   for (void* p : b3) if (p) (void)hipFree(p);
   for (StateBufs& B : c->st) {
     void* sb[] = {B.rs, B.vp, B.vr, B.qk, B.ae, B.ag, B.rdi, B.vpi, B.vrf, B.qkf, B.aef, B.agf, B.cell, B.id,
-                  B.vr2, B.vrf2};
+                  B.vr2, B.vrf2, B.c12};
     for (void* p : sb) if (p) (void)hipFree(p);
   }
   if (c->stream) (void)hipStreamDestroy(c->stream);

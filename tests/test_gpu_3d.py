@@ -139,3 +139,73 @@ def test_3d_errors():
     with pytest.raises(KaosError) as e:             # a missing 3-D array
         KineticSolver([t3], precision=KAOS_FP64, ndatfac=1, seed=1, capacity=[4096])
     assert e.value.code == 1
+
+
+# ---------------------------------------------------------------------------
+# Full size on the 3-D grid: the benchmark tube (256 cells, 64 Vpar cells) with
+# 32 x 32 (Vperp1, Vperp2) cells and 1e8 ions seeded on the device, where the
+# oracle would take hours, checked through properties that hold at any size.
+
+N3_FULL = 100_000_000
+N3_STEPS = 6
+
+
+def _full3(precision):
+    tube = grids.with_vgrid3(grids.config2(n_total=N3_FULL), 32, 32)
+    counts = tube.cell_counts
+    cap = int(counts.sum()) + (N3_STEPS + 2) * (tube.n_inj_lb + tube.n_inj_ub) + 1024
+    ks = KineticSolver([tube], precision=precision, ndatfac=1, seed=5, capacity=[cap])
+    ks.seed(0, counts, np.full(tube.nq, tube.Ts_lb), 20241)
+    steps = []
+    for n in range(1, N3_STEPS + 1):
+        out = ks.step(n)
+        assert out.status == 0
+        m = ks.master(0)
+        m3 = ks.master3(0)
+        steps.append((out.n_alive, out.n_injected, m, m3))
+    ks.close()
+    return tube, steps
+
+
+@pytest.fixture(scope="module")
+def full3():
+    print(f"3-D full size: {N3_FULL} ions, 32x32x64 cells, {N3_STEPS} steps, fp64 then mixed", flush=True)
+    return {"fp64": _full3(KAOS_FP64), "mixed": _full3(KAOS_MIXED)}
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_3d_full_size_counts_and_bookkeeping(full3, form):
+    tube, st = full3[form]
+    k = tube.nsnormfac / tube.arrays["d3xC"]
+    # step 1: every seeded ion is in its cell; the 3-D counts hold those inside the grid
+    m, m3 = st[0][2], st[0][3]
+    assert np.array_equal(m["nq_counts"], tube.cell_counts)
+    inside = m3["n3_counts"].reshape(tube.nq, -1).sum(axis=1)
+    assert np.all(inside <= m["nq_counts"]) and inside.sum() >= 0.999 * N3_FULL
+    removed = [int(s[2]["esc"][0] + s[2]["esc"][1]) for s in st]
+    for i in range(1, N3_STEPS - 1):
+        alive, inj, m, m3 = st[i]
+        assert alive == st[i - 1][0] - removed[i] + inj, i + 1
+        assert m["nq_counts"].sum() == alive - removed[i + 1], i + 1
+    for _, _, m, m3 in st[1:]:
+        assert np.array_equal(m["NqRT"], m["nq_counts"] * k)
+        inside = m3["n3_counts"].reshape(tube.nq, -1).sum(axis=1)
+        assert np.allclose(m3["M0"], inside * k, rtol=1e-12, atol=0)
+        assert np.array_equal(m["M0"], m3["M0"]) and np.array_equal(m["M2par"], m3["M2par"])
+
+
+def test_3d_full_size_mixed_within_1e6_of_fp64(full3):
+    tube = full3["fp64"][0]
+    a, b = full3["fp64"][1][-1][3], full3["mixed"][1][-1][3]
+    for key in oracle.MOMENTS3:
+        # M1Perp1 and M1Perp2 are means of signed components, ~0 up to sampling
+        # noise for this isotropic population: their error is measured against
+        # the perpendicular thermal speed sqrt(2 M2Perp / m) (binary32 Vperp1/2
+        # put a few dozen of the 1e8 ions in a neighbouring cell)
+        if key in ("M1perp1", "M1perp2"):
+            scale = np.linalg.norm(np.sqrt(2.0 * a["M2perp" + key[-1]] / tube.ms))
+        else:
+            scale = np.linalg.norm(a[key])
+        err = np.linalg.norm(b[key] - a[key]) / scale
+        assert err <= 1e-6, (key, err)
+    assert np.abs(full3["fp64"][1][-1][2]["nq_counts"] - full3["mixed"][1][-1][2]["nq_counts"]).sum() <= 16
