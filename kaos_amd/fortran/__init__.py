@@ -37,10 +37,27 @@ def write_input(path, tube, q0, vpar0, vperp0, nsteps, ndatfac=1, precision=0, c
             np.array([export["phi"], export["ns0"]], dtype=np.float64).tofile(f)
 
 
-def read_output(path, nq):
-    """List of master-step records: dict(n, n_alive, nq_counts, M (6, nq), EA, esc)."""
-    rec = np.dtype([("n", "<i8"), ("alive", "<i8"), ("nqc", "<i8", (nq,)), ("M", "<f8", (6 * nq,)),
-                    ("EA", "<f8", (nq,)), ("esc", "<f8", (4,))])
-    raw = np.fromfile(path, dtype=rec)
-    return [dict(n=int(r["n"]), n_alive=int(r["alive"]), nq_counts=r["nqc"].copy(),
-                 M=r["M"].reshape(6, nq).copy(), EA=r["EA"].copy(), esc=r["esc"].copy()) for r in raw]
+def write_grid3(path, tube):
+    """The driver's GRID3D file of a tube with a 3-D velocity grid."""
+    a = tube.arrays
+    with open(path, "wb") as f:
+        np.array([tube.nv1, tube.nv2], dtype=np.int32).tofile(f)
+        for k in ("v1GL", "v1GH", "v1GC", "dv1", "v2GL", "v2GH", "v2GC", "dv2", "d3v3"):
+            np.ascontiguousarray(a[k], dtype=np.float64).tofile(f)
+
+
+def read_output(path, nq, v3d=False):
+    """List of master-step records: dict(n, n_alive, nq_counts, M (6, nq), EA, esc[, M8 (8, nq)])."""
+    fields = [("n", "<i8"), ("alive", "<i8"), ("nqc", "<i8", (nq,)), ("M", "<f8", (6 * nq,)),
+              ("EA", "<f8", (nq,)), ("esc", "<f8", (4,))]
+    if v3d:
+        fields.append(("M8", "<f8", (8 * nq,)))
+    raw = np.fromfile(path, dtype=np.dtype(fields))
+    out = []
+    for r in raw:
+        d = dict(n=int(r["n"]), n_alive=int(r["alive"]), nq_counts=r["nqc"].copy(),
+                 M=r["M"].reshape(6, nq).copy(), EA=r["EA"].copy(), esc=r["esc"].copy())
+        if v3d:
+            d["M8"] = r["M8"].reshape(8, nq).copy()
+        out.append(d)
+    return out

@@ -5,10 +5,15 @@
 ! time step, and the master-step outputs read back the way the reference
 ! stores them per master step.
 !
-! Usage: kaos_fdriver INPUT OUTPUT [EXPORTDIR [SEED]]
+! Usage: kaos_fdriver INPUT OUTPUT [EXPORTDIR [SEED [GRID3D]]]
 !   EXPORTDIR: DataExport2Sub's files of every master step (kaos_export_master)
 !   SEED: initial ions generated by the library (kaos_cell_counts from the
 !         density, kaos_seed_particles with that seed) instead of q0/vpar0/vperp0
+!   GRID3D: the 3-D velocity grid (ION2VPERPflag = 1; KAOS_FLAG_VPERP3D), stream:
+!         int32 nvperp1 nvperp2; real64 Vperp1 GL GH GC dV (nvperp1 each),
+!         Vperp2 GL GH GC dV (nvperp2 each), d3v3 (nvperp1*nvperp2*nvpar, Vpar
+!         fastest); the records then end with the eight 3-D moments M8(8*nq), and
+!         EXPORTDIR also receives the spin-up files (kaos_export_spinup)
 ! INPUT (stream, native endianness):
 !   int32  nq nvpar nvperp grav eamb eambsign filter_pt lbcond ubcond ndatfac precision nsteps
 !   int64  n_inj_lb n_inj_ub nions capacity seed
@@ -21,6 +26,7 @@
 !   with EXPORTDIR: real64 thetaGC TsGC densityGC (nq each), phi, ns0
 ! OUTPUT (stream), one record per master step (and step 1):
 !   int64 n, int64 n_alive, int64 NqReNorm(nq), real64 M(6*nq), real64 EAmag(nq), real64 esc(4)
+!   [, real64 M8(8*nq) with GRID3D]
 program kaos_fdriver
   use, intrinsic :: iso_c_binding
   use kaos_hip
@@ -36,6 +42,11 @@ program kaos_fdriver
   character(len=1, kind=c_char), allocatable, target :: cdir(:)
   integer(c_int64_t), allocatable, target :: nsfa(:)
   type(kaos_export_info) :: xinfo
+  type(kaos_spinup_info) :: sinfo
+  logical :: do_3d
+  integer(c_int32_t) :: n12(2)
+  real(c_double), allocatable, target :: v1g(:, :), v2g(:, :), d3v3(:), M8(:)
+  character(len=1024) :: g3file
   integer(c_int64_t) :: nnx, iseed
   logical :: do_export, do_seed
   integer :: i
@@ -53,6 +64,7 @@ program kaos_fdriver
   call get_command_argument(2, fout)
   do_export = command_argument_count() >= 3
   do_seed = command_argument_count() >= 4
+  do_3d = command_argument_count() >= 5
   if (do_export) call get_command_argument(3, xdir)
   if (do_seed) then
     call get_command_argument(4, sarg)
@@ -69,6 +81,14 @@ program kaos_fdriver
     read (u) th, ts, dens, phi_ns0
   end if
   close (u)
+  if (do_3d) then
+    call get_command_argument(5, g3file)
+    open (newunit=u, file=trim(g3file), access='stream', form='unformatted', status='old')
+    read (u) n12
+    allocate (v1g(n12(1), 4), v2g(n12(2), 4), d3v3(n12(1)*n12(2)*nvpar), M8(8*nq))
+    read (u) v1g, v2g, d3v3
+    close (u)
+  end if
 
   cfg%ntubes = 1; cfg%precision = hdr(11); cfg%ndatfac = hdr(10); cfg%device = 0; cfg%seed = h64(5)
   cfg%reorder_every = 0; cfg%flags = 0
@@ -83,6 +103,15 @@ program kaos_fdriver
   g%vparGL = c_loc(vpg(1, 1)); g%vparGH = c_loc(vpg(1, 2)); g%vparGC = c_loc(vpg(1, 3)); g%dvpar = c_loc(vpg(1, 4))
   g%vperpGL = c_loc(vrg(1, 1)); g%vperpGH = c_loc(vrg(1, 2)); g%vperpGC = c_loc(vrg(1, 3))
   g%dvperp = c_loc(vrg(1, 4)); g%d3v = c_loc(d3v(1))
+  if (do_3d) then
+    cfg%flags = ior(cfg%flags, KAOS_FLAG_VPERP3D)
+    g%nvperp1 = n12(1); g%nvperp2 = n12(2)
+    g%vperp1GL = c_loc(v1g(1, 1)); g%vperp1GH = c_loc(v1g(1, 2)); g%vperp1GC = c_loc(v1g(1, 3))
+    g%dvperp1 = c_loc(v1g(1, 4))
+    g%vperp2GL = c_loc(v2g(1, 1)); g%vperp2GH = c_loc(v2g(1, 2)); g%vperp2GC = c_loc(v2g(1, 3))
+    g%dvperp2 = c_loc(v2g(1, 4))
+    g%d3v3 = c_loc(d3v3(1))
+  end if
 
   rc = kaos_init(cfg, ctx)
   if (rc /= KAOS_OK) then
@@ -112,6 +141,12 @@ program kaos_fdriver
     xinfo%species = 1_c_int32_t; xinfo%flux_tube = 1_c_int32_t
     xinfo%ns0 = phi_ns0(2); xinfo%phi = phi_ns0(1)
     xinfo%theta = c_loc(th(1)); xinfo%Ts = c_loc(ts(1))
+    ! spin-up files into the same directory (the reference's Densitydatadir)
+    sinfo%dir = c_loc(cdir(1))
+    sinfo%species = 1_c_int32_t; sinfo%flux_tube = 1_c_int32_t
+    sinfo%nq_lb = 1_c_int64_t; sinfo%nq_ub = int(nq, c_int64_t)
+    sinfo%nsnorm_clb = real(h64(1), c_double); sinfo%nsnorm_cub = real(h64(2), c_double)
+    sinfo%lb_replenish = 0_c_int64_t; sinfo%ub_replenish = 0_c_int64_t
   end if
 
   open (newunit=uo, file=trim(fout), access='stream', form='unformatted', status='replace')
@@ -122,10 +157,20 @@ program kaos_fdriver
       rc = kaos_download_master(ctx, 0_c_int32_t, c_loc(nqc), c_null_ptr, c_null_ptr, c_loc(M), &
                                 c_loc(EA), c_null_ptr, c_loc(esc))
       call check('kaos_download_master')
-      write (uo) n, so%n_alive, nqc, M, EA, esc
+      if (do_3d) then
+        rc = kaos_download_master3d(ctx, 0_c_int32_t, c_null_ptr, c_loc(M8))
+        call check('kaos_download_master3d')
+        write (uo) n, so%n_alive, nqc, M, EA, esc, M8
+      else
+        write (uo) n, so%n_alive, nqc, M, EA, esc
+      end if
       if (do_export) then
         rc = kaos_export_master(ctx, 0_c_int32_t, xinfo, nnx)
         call check('kaos_export_master')
+        if (do_3d) then
+          rc = kaos_export_spinup(ctx, 0_c_int32_t, sinfo)
+          call check('kaos_export_spinup')
+        end if
       end if
     end if
   end do
