@@ -999,15 +999,19 @@ KAOS_HD void kaos_mom3_part(const kaos_vgrid3_t* v, const uint32_t* cnt, int b, 
   uint32_t cs = 0;
   const double dvb = v->dvpar[b];
   int r = r0 + (((g0 - r0) % G) + G) % G;   /* first row >= r0 of this group */
+  KAOS_MOM_UNROLL_PRAGMA
   for (; r <= r1; r += G) {
     const int i = r * v->nvpar + b;
+    /* every load of the row first (all in range), so that the unrolled rows'
+     * loads are in flight together; the arithmetic is unchanged */
     const uint32_t c = cnt[i];
+    const int i1 = r / v->nv2, i2 = r - i1 * v->nv2;
+    const double d3 = v->d3v[i], dv1 = v->dv1[i1], dv2 = v->dv2[i2];
+    const double u1 = v->v1GC[i1], u2 = v->v2GC[i2];
     cs += c;
     if (c != 0u) {
-      const int i1 = r / v->nv2, i2 = r - i1 * v->nv2;
-      double f = ((double)c * k) / v->d3v[i];
-      double w = (v->dv1[i1] * v->dv2[i2]) * dvb;
-      double u1 = v->v1GC[i1], u2 = v->v2GC[i2];
+      double f = ((double)c * k) / d3;
+      double w = (dv1 * dv2) * dvb;
       s0 = s0 + w * f;
       s1 = s1 + w * (u1 * f);
       s2 = s2 + w * (u2 * f);
