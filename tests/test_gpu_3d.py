@@ -209,3 +209,16 @@ def test_3d_full_size_mixed_within_1e6_of_fp64(full3):
         err = np.linalg.norm(b[key] - a[key]) / scale
         assert err <= 1e-6, (key, err)
     assert np.abs(full3["fp64"][1][-1][2]["nq_counts"] - full3["mixed"][1][-1][2]["nq_counts"]).sum() <= 16
+
+
+def test_3d_fp64_bitexact_grid_larger_than_the_window():
+    # 40 x 40 x 48 = 76800 bins per config cell: more than the binning pass's
+    # LDS window holds, so every count goes through global atomics
+    t = grids.with_vgrid3(grids.dipole_tube(nq=24, nvpar=48, nvperp=16, n_total=10000, ratio=1.1), 40, 40,
+                          vperp_max=7.0e3)
+    orcs, ks = _pair([t], KAOS_FP64, ndatfac=1, nsteps=5)
+    print("3-D fp64: 40x40x48 grid (no LDS window), 5 steps", flush=True)
+    for n, master in _steps(orcs, ks, 5, 1):
+        _bitexact(orcs, ks, n, master)
+    assert ks.master3(0)["n3_counts"].sum() > 0
+    ks.close()
