@@ -357,7 +357,9 @@ def main():
                     help="add the heated two-species run (H+ and O+, ICR heating) as 'full_physics'")
     ap.add_argument("--fp-ions", type=int, default=200_000_000, help="ions of the heated run (1/4 H+)")
     ap.add_argument("--vperp3d", action="store_true",
-                    help="add the single-tube workload on the 3-D (Vperp1, Vperp2, Vpar) grid as 'vperp3d'")
+                    help="the single-tube workload on the 3-D (Vperp1, Vperp2, Vpar) grid as 'vperp3d' "
+                         "(run by default; kept for explicit command lines)")
+    ap.add_argument("--no-vperp3d", action="store_true", help="skip the 3-D grid leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the traffic-calibration copy of MIB MiB and exit")
     args = ap.parse_args()
@@ -425,7 +427,7 @@ def main():
         out["batched"] = batched_run(args)
     if args.full_physics:
         out["full_physics"] = full_physics_run(args)
-    if args.vperp3d:
+    if args.vperp3d or not args.no_vperp3d:
         out["vperp3d"] = vperp3d_run(args)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle) ...")
