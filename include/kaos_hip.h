@@ -194,7 +194,10 @@ int kaos_run(kaos_ctx* ctx, int64_t n_first, int64_t nsteps, int64_t* particle_s
  * NqRT (nq); M (6*nq): M0phRT, M1PerpphRT, M1ParphRT, M2phRT, M2PerpphRT,
  * M2ParphRT; EA (nq): EAmagRT; EG (nq): EGmagRT; esc (4): LB and UB escaped
  * ion counts and the LB/UB escape fluxes LBoutfluxIonRT/UBoutfluxIonRT
- * (_8_0_7_BoundaryConditions.f90:201-208, 278-285). */
+ * (_8_0_7_BoundaryConditions.f90:201-208, 278-285).  With KAOS_FLAG_VPERP3D
+ * the 2-D counts are zero and M holds the moments both branches define (M0,
+ * M1Par, M2, M2Par; M1Perp and M2Perp zero): the 3-D ones come from
+ * kaos_download_master3d. */
 int kaos_download_master(kaos_ctx* ctx, int32_t tube, int64_t* nq_counts, int64_t* nph_counts,
                          double* NqRT, double* M, double* EA, double* EG, double* esc);
 
