@@ -106,6 +106,7 @@ struct StepArgs {
   // appended), counts [Q][V1][V2][Vpar] per tube
   const double *v1GH, *v2GH;
   unsigned int* hist3;
+  const int2* grp3;                          // per binning block: its tiles [x, y), all of one tube
 };
 
 __device__ __forceinline__ void set_status(unsigned int* st, unsigned int bit) { atomicOr(st, bit); }
@@ -568,13 +569,13 @@ __device__ __forceinline__ bool bin3_dead(const Bin3Ion<1>& x) { return x.cw == 
 
 template <int MIXED>
 __global__ void __launch_bounds__(KAOS_BIN3_BLOCK) k_bin3d(StepArgs a, int first, int wwords) {
+  const int2 grp = a.grp3[blockIdx.x];
   extern __shared__ unsigned int s_w3[];
   __shared__ double s_qGH[KAOS_MAXQ], s_vGH[KAOS_MAXV + 1], s_v1GH[KAOS_MAXV + 1], s_v2GH[KAOS_MAXV + 1];
   __shared__ double s_rref[MIXED ? KAOS_MAXQ : 1];
   __shared__ int s_rk[MIXED ? KAOS_MAXQ : 1], s_vk[MIXED ? KAOS_MAXV + 1 : 1];
   __shared__ int s_wq0, s_wnq;
-  const int t0 = blockIdx.x * KAOS_BIN3_TILES;
-  const int t1 = min(t0 + KAOS_BIN3_TILES, a.ntiles);
+  const int t0 = grp.x, t1 = grp.y;
   if (t0 >= t1) return;                                   // uniform over the block
   const int tid = threadIdx.x;
   const int tb0 = a.tile_tube[t0];
@@ -1446,6 +1447,8 @@ struct kaos_ctx {
   unsigned int* d_hist3 = nullptr;
   long long nhist3_tot = 0;
   int bin3_words = 0;                           // LDS window of k_bin3d (32-bit words)
+  int nb3 = 0;                                  // binning blocks: groups of <= KAOS_BIN3_TILES tiles of one tube
+  int2* d_grp3 = nullptr;
 };
 
 static std::string g_init_err;
@@ -1785,6 +1788,17 @@ static int build_device(kaos_ctx* c) {
       words = std::max(words, (wnq * nc3 + 1) / 2);
     }
     c->bin3_words = (int)std::min<long long>(words, KAOS_BIN3_MAXWORDS);
+    // binning groups: runs of <= KAOS_BIN3_TILES tiles that never cross a tube,
+    // so that every group's LDS window serves all its ions
+    std::vector<int2> grp;
+    for (int i = 0; i < c->ntiles;) {
+      int j = i + 1;
+      while (j < c->ntiles && j - i < KAOS_BIN3_TILES && tile_tube[j] == tile_tube[i]) ++j;
+      grp.push_back(make_int2(i, j));
+      i = j;
+    }
+    c->nb3 = (int)grp.size();
+    HIPCHK(c, up((void**)&c->d_grp3, grp.data(), sizeof(int2) * std::max<size_t>(grp.size(), 1)));
     if (c->bin3_words > 0) {
       HIPCHK(c, hipFuncSetAttribute((const void*)k_bin3d<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)(sizeof(unsigned int) * c->bin3_words)));
@@ -1853,7 +1867,7 @@ static StepArgs make_args(kaos_ctx* c, int interp, int count, int remove, int qk
   a.seed = c->cfg.seed;
   a.hbox = c->d_hbox;
   a.do_interp = interp; a.do_count = c->v3d ? 0 : count; a.do_remove = remove; a.do_qk = qk;
-  a.v1GH = c->d_v1GH; a.v2GH = c->d_v2GH; a.hist3 = c->d_hist3;
+  a.v1GH = c->d_v1GH; a.v2GH = c->d_v2GH; a.hist3 = c->d_hist3; a.grp3 = c->d_grp3;
   return a;
 }
 
@@ -2136,7 +2150,7 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
   hipLaunchKernelGGL(k_step_tick, dim3(1), dim3(1), 0, c->stream, c->d_stepno);
   const int mixed = c->cfg.precision == KAOS_MIXED;
   auto launch_bin3d = [&](const StepArgs& a, int first) {
-    const int nb = (c->ntiles + KAOS_BIN3_TILES - 1) / KAOS_BIN3_TILES;
+    const int nb = c->nb3;
     if (mixed)
       hipLaunchKernelGGL(k_bin3d<1>, dim3(nb), dim3(KAOS_BIN3_BLOCK), sizeof(unsigned int) * c->bin3_words,
                          c->stream, a, first, c->bin3_words);
@@ -2738,7 +2752,8 @@ extern "C" int kaos_finalize(kaos_ctx* c) {
   (void)hipFree(c->d_vbox); (void)hipFree(c->d_hbox);
   (void)hipFree(c->d_vmarg_par); (void)hipFree(c->d_vmarg_perp);
   (void)hipFree(c->d_stepno);
-  void* b3[] = {c->d_v1GH, c->d_v1GC, c->d_dv1, c->d_v2GH, c->d_v2GC, c->d_dv2, c->d_d3v3, c->d_M3, c->d_hist3};
+  void* b3[] = {c->d_v1GH, c->d_v1GC, c->d_dv1, c->d_v2GH, c->d_v2GC, c->d_dv2, c->d_d3v3, c->d_M3, c->d_hist3,
+                c->d_grp3};
   for (void* p : b3) if (p) (void)hipFree(p);
   for (StateBufs& B : c->st) {
     void* sb[] = {B.rs, B.vp, B.vr, B.qk, B.ae, B.ag, B.rdi, B.vpi, B.vrf, B.qkf, B.aef, B.agf, B.cell, B.id,
