@@ -985,8 +985,11 @@ typedef struct {
   const double *d3v, *dv1, *dv2, *dvpar, *v1GC, *v2GC, *vparGC;
 } kaos_vgrid3_t;
 
+/* Lanes of the 3-D moment sums: one 1024-thread block per config cell, so
+ * (V1, V2) rows split into 1024 / nvpar groups. */
+#define KAOS_MOM3_LANES 1024
 KAOS_HD int kaos_mom3_groups(int nvpar) {
-  int g = KAOS_MOM_LANES / nvpar;
+  int g = KAOS_MOM3_LANES / nvpar;
   return g < 1 ? 1 : g;
 }
 

@@ -1043,14 +1043,14 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_moments(const DevTube* tubes, co
 // the 2-D block's M0, M1Par, M2, M2Par (M1Perp, M2Perp: 0), which the next
 // master step's fields read.  Block (t, 0) advances the tube's slot count,
 // next id and live count as the 2-D kernels do.
-__global__ void __launch_bounds__(KAOS_BLOCK) k_moments3d(const DevTube* tubes, const int* blk_tube,
+__global__ void __launch_bounds__(KAOS_MOM3_LANES) k_moments3d(const DevTube* tubes, const int* blk_tube,
     const int* blk_q, const double* d3xC, const double* d3v3, const double* dvpar, const double* dv1,
     const double* dv2, const double* vparGC, const double* v1GC, const double* v2GC,
     const unsigned int* hist3, const unsigned int* nqoff, unsigned int* nqc, double* NqRT, double* M, double* M3,
     long long* count, long long* next_id, const long long* inj_cnt, long long* live,
     const unsigned long long* esc_acc) {
-  __shared__ double s_P[KAOS_MOM_LANES][5];
-  __shared__ unsigned int s_pc[KAOS_MOM_LANES];
+  __shared__ double s_P[KAOS_MOM3_LANES][5];
+  __shared__ unsigned int s_pc[KAOS_MOM3_LANES];
   __shared__ double s_A[KAOS_MAXV + 1][5];
   __shared__ double s_x[8][KAOS_MAXV + 1];
   __shared__ unsigned int s_cc[KAOS_MAXV + 1];
@@ -2003,7 +2003,7 @@ static size_t push_lds_bytes(const kaos_ctx* c) { return sizeof(unsigned int) * 
 static void launch_moments(kaos_ctx* c, const long long* inj) {
   const int wpb = KAOS_BLOCK / 64;
   if (c->v3d) {
-    hipLaunchKernelGGL(k_moments3d, dim3(c->nblk_mom), dim3(KAOS_BLOCK), 0, c->stream, c->d_tubes, c->d_blk_tube,
+    hipLaunchKernelGGL(k_moments3d, dim3(c->nblk_mom), dim3(KAOS_MOM3_LANES), 0, c->stream, c->d_tubes, c->d_blk_tube,
                        c->d_blk_q, c->d_d3x, c->d_d3v3, c->d_dvpar, c->d_dv1, c->d_dv2, c->d_vparGC, c->d_v1GC,
                        c->d_v2GC, c->d_hist3, c->d_nqoff, c->d_nqc, c->d_NqRT, c->d_M, c->d_M3, c->d_count,
                        c->d_next_id, inj, c->d_live, c->d_esc_acc);
