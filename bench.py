@@ -8,15 +8,21 @@ gravity fields, open-boundary removal and injection.  Synthetic inputs.
 
 Both precision forms run the same inputs and the same steps:
   * FP64 (binary64 throughout, bit-exact against the oracle): `value`;
-  * mixed (binary32 stages and state words): the `mixed` object, with the
-    relative errors of its moments and field against the FP64 run after the
-    last timed step.
+  * mixed (binary32 stages and state words): the `mixed` object.
+The `mixed_error` object is an untimed leg over the same inputs and steps
+(at least 200, the length of the full-size mixed workload): FP64 and mixed
+side by side, the relative errors of the mixed moments and ambipolar field
+against FP64 after every step, reported as the maximum over the steps and
+the last value.  The runs are deterministic, so these are the errors of the
+timed runs' steps too.
 The `batched` object is the many-tube workload: 512 independent tubes (L ~
 U[3,8], 64..256 cells, 64x64, log-uniform 1e4..1e6 ions each, T ~ U[0.1,1] eV),
 mixed form, every step replayed as one captured hipGraph, launch shape from
-the autotuner, 200 steps.
-The warmup steps include the autotuner's steps (2 warm-up and 16 candidate
-master steps, then the choice).
+the autotuner.
+Every timed region starts after the autotuner has chosen its launch shape:
+the warm-up steps are followed by untimed steps until kaos_get_launch
+reports that tuning is over, whatever --warmup is, and each form reports the
+shape it ran with.
 Prints one JSON line.  value = ion pushes (live ions summed over the timed
 steps, counted on the device) / wall time of the timed steps, inputs resident
 in HBM.  roofline: the push kernel's algorithmic HBM bytes per launch over its
@@ -105,6 +111,27 @@ def cpu_reference_push(n_ions=2_000_000):
                        f"{t:.2f} s; excludes binning and moments")
 
 
+TUNE_STEPS_MAX = 64        # the autotuner needs 2 + 2 x 8 master steps and one more to choose
+
+
+def finish_tuning(ks, n_next):
+    """Untimed steps from n_next until the autotuner has chosen the launch
+    shape; returns the next step number and the number of extra steps."""
+    extra = 0
+    while ks.get_launch()[3]:
+        if extra >= TUNE_STEPS_MAX:
+            raise RuntimeError("autotuner still tuning after %d extra steps" % extra)
+        ks.run(n_next, 1)
+        n_next += 1
+        extra += 1
+    return n_next, extra
+
+
+def launch_dict(ks):
+    t, w, p, tuning = ks.get_launch()
+    return {"tiles_per_group": t, "hist_words": w, "persistent": p, "tuning": tuning}
+
+
 def timed_run(form, tube, q0, v0, w0, cap, args):
     from kaos_amd.solver import KineticSolver, KAOS_FP64, KAOS_MIXED
     prec = KAOS_MIXED if form == "mixed" else KAOS_FP64
@@ -115,9 +142,9 @@ def timed_run(form, tube, q0, v0, w0, cap, args):
     ks.step(1)
     if args.warmup:
         ks.run(2, args.warmup)
+    first, tune_extra = finish_tuning(ks, 2 + args.warmup)
     ks.profile(True)
     ks.profile_read()
-    first = 2 + args.warmup
     t1 = time.perf_counter()
     pushes = ks.run(first, args.steps)
     wall = time.perf_counter() - t1
@@ -135,8 +162,8 @@ def timed_run(form, tube, q0, v0, w0, cap, args):
     achieved = BYTES_PER_ION[form] * ions_per_launch / mean_push_s / 1e9
     log(f"[bench] {form}: {args.steps} steps, wall {wall:.3f}s, run events {run_ms/1e3:.3f}s, "
         f"push kernel {push_ms/max(launches,1):.3f} ms x {launches}")
-    res = dict(value=pushes / wall, ms_per_step=wall / args.steps * 1e3, launch=ks.get_launch()[:3],
-               upload_s=upload_s, drop_in_step_ms=step_ms,
+    res = dict(value=pushes / wall, ms_per_step=wall / args.steps * 1e3, launch=launch_dict(ks),
+               untimed_tuning_steps=tune_extra, upload_s=upload_s, drop_in_step_ms=step_ms,
                roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS,
                          "traffic": TRAFFIC_BYTES_PER_ION[form] * ions_per_launch,
@@ -217,15 +244,18 @@ def vperp3d_run(args, nsteps=50, warmup=8):
             if kind == "3d":
                 grids.with_vgrid3(tube, 32, 32)
             counts = tube.cell_counts
-            cap = int(counts.sum()) + (nsteps + warmup + 4) * (tube.n_inj_lb + tube.n_inj_ub) + 1024
+            cap = int(counts.sum()) + (nsteps + warmup + TUNE_STEPS_MAX + 4) * (tube.n_inj_lb + tube.n_inj_ub) + 1024
             ks = KineticSolver([tube], precision=prec, ndatfac=1, seed=1, capacity=[cap])
             ks.seed(0, counts, np.full(tube.nq, tube.Ts_lb), 20241)
             ks.step(1)
             ks.run(2, warmup)
+            first, _ = finish_tuning(ks, 2 + warmup)
             t1 = time.perf_counter()
-            pushes = ks.run(2 + warmup, nsteps)
+            pushes = ks.run(first, nsteps)
             wall = time.perf_counter() - t1
             r = {"value": pushes / wall, "ms_per_step": wall / nsteps * 1e3}
+            if kind == "2d":
+                r["launch"] = launch_dict(ks)
             if kind == "3d":
                 m3 = ks.master3(0)
                 r["binned_fraction"] = float(m3["n3_counts"].sum() / max(ks.master(0)["nq_counts"].sum(), 1))
@@ -242,7 +272,7 @@ def batched_run(args):
     from kaos_amd.solver import KineticSolver, KAOS_MIXED
     t0 = time.perf_counter()
     tubes = grids.config4_tubes(ntubes=args.tubes)
-    n_steps_total = args.warmup + args.steps + 12
+    n_steps_total = args.warmup + args.steps + TUNE_STEPS_MAX + 12
     caps, ions = [], []
     for i, t in enumerate(tubes):
         q, v, w = grids.initial_ions(t, seed=100 + i)
@@ -256,7 +286,7 @@ def batched_run(args):
     log(f"[bench] batched: {len(tubes)} tubes, {n_ions} ions, setup {time.perf_counter() - t0:.1f}s")
     ks.step(1)
     ks.run(2, args.warmup)
-    first = 2 + args.warmup
+    first, tune_extra = finish_tuning(ks, 2 + args.warmup)
     t1 = time.perf_counter()
     pushes = ks.run(first, args.steps)
     wall = time.perf_counter() - t1
@@ -267,17 +297,63 @@ def batched_run(args):
     push_ms, launches, _ = ks.profile_read()
     mean_push_s = push_ms / 1e3 / max(launches, 1)
     achieved = BYTES_PER_ION["mixed"] * (p2 / max(launches, 1)) / mean_push_s / 1e9
-    launch = ks.get_launch()
+    launch = launch_dict(ks)
     ks.close()
     log(f"[bench] batched: {args.steps} steps (graphs), wall {wall:.3f}s; push {mean_push_s*1e3:.3f} ms "
-        f"(eager, {launches} launches); launch shape {launch[:3]}")
+        f"(eager, {launches} launches); launch shape {launch}")
     return {"value": pushes / wall, "ms_per_step": wall / args.steps * 1e3, "steps": args.steps,
             "tubes": len(tubes), "ions_start": n_ions, "dtype": "f32+f64", "graphs": True,
-            "launch": {"tiles_per_group": launch[0], "hist_words": launch[1], "persistent": launch[2]},
+            "launch": launch, "untimed_tuning_steps": tune_extra,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_push_mixed",
                          "bytes_per_ion": BYTES_PER_ION["mixed"], "mean_launch_ms": mean_push_s * 1e3,
                          "timing": f"hipEvents over {launches} eager launches after the timed steps"}}
+
+
+ERROR_STEPS_MIN = 200      # steps of the untimed mixed-vs-FP64 error leg (at least)
+
+
+def mixed_error_run(tube, q0, v0, w0, cap, nsteps):
+    """Untimed: an FP64 and a mixed context on the same uploaded ions, stepped
+    side by side for nsteps steps (every one a master step); after each step
+    the relative L2 errors of the mixed moments and ambipolar field against
+    FP64 and the config-cell count difference; the phase-space count
+    difference after the last step."""
+    from kaos_amd.solver import KineticSolver, KAOS_FP64, KAOS_MIXED
+    t0 = time.perf_counter()
+    ks = {f: KineticSolver([tube], precision=p, ndatfac=1, seed=1, capacity=[cap])
+          for f, p in (("fp64", KAOS_FP64), ("mixed", KAOS_MIXED))}
+    for k in ks.values():
+        k.upload(0, q0, v0, w0)
+    series = {k: [] for k in MOMENTS}
+    cdiff = []
+    for n in range(1, nsteps + 1):
+        m = {}
+        for f, k in ks.items():
+            k.step(n)
+            m[f] = k.master(0, phase=(n == nsteps))
+        for k in MOMENTS:
+            a, b = m["fp64"][k], m["mixed"][k]
+            na = np.linalg.norm(a)
+            series[k].append(float(np.linalg.norm(a - b) / na) if na > 0 else 0.0)
+        cdiff.append(int(np.abs(m["fp64"]["nq_counts"] - m["mixed"]["nq_counts"]).sum()))
+    last = moment_errors(m["fp64"], m["mixed"])
+    for k in ks.values():
+        k.close()
+    out = {"steps": nsteps, "ions_start": len(q0),
+           "note": "relative L2 error of each mixed moment (and of EA) against FP64 after every step; "
+                   "max over the steps and the value after the last step",
+           "max_l2_rel": {k: max(v) for k, v in series.items()},
+           "argmax_step": {k: int(np.argmax(v)) + 1 for k, v in series.items()},
+           "last_l2_rel": {k: v[-1] for k, v in series.items()},
+           "every_20th_step_l2_rel": {k: v[19::20] for k, v in series.items()},
+           "counts_abs_diff_max": max(cdiff), "counts_abs_diff_last": cdiff[-1],
+           "phase_counts_abs_diff_last": last["phase_counts_abs_diff"]}
+    out["max_l2_rel_moment_error"] = max(out["max_l2_rel"][k] for k in MOMENTS if k != "EA")
+    out["max_l2_rel_field_error_EA"] = out["max_l2_rel"]["EA"]
+    log(f"[bench] mixed vs fp64 error leg: {nsteps} steps, max moment error {out['max_l2_rel_moment_error']:.2e}, "
+        f"field {out['max_l2_rel_field_error_EA']:.2e}, {time.perf_counter() - t0:.1f}s")
+    return out
 
 
 def moment_errors(ma, mb):
@@ -301,7 +377,7 @@ def full_physics_run(args):
     from kaos_amd.solver import KineticSolver, KAOS_FP64, KAOS_MIXED
     t0 = time.perf_counter()
     tubes = grids.config5_tubes(n_h=args.fp_ions // 4, n_o=args.fp_ions - args.fp_ions // 4)
-    n_steps_total = args.warmup + args.steps + 2
+    n_steps_total = args.warmup + args.steps + TUNE_STEPS_MAX + 2
     ions = [grids.initial_ions(t, seed=500 + i) for i, t in enumerate(tubes)]
     caps = [len(q) + (n_steps_total + 2) * (t.n_inj_lb + t.n_inj_ub) + 1024 for t, (q, _, _) in zip(tubes, ions)]
     n_ions = sum(len(x[0]) for x in ions)
@@ -314,10 +390,13 @@ def full_physics_run(args):
             ks.upload(i, q, v, w)
         ks.step(1)
         ks.run(2, args.warmup)
-        first = 2 + args.warmup
+        first, _ = finish_tuning(ks, 2 + args.warmup)
         t1 = time.perf_counter()
         pushes = ks.run(first, args.steps)
         wall = time.perf_counter() - t1
+        # both forms end at the same step, whatever their tuning took
+        last = first + args.steps
+        ks.run(last, 2 + args.warmup + TUNE_STEPS_MAX + args.steps - last)
         masters[form] = [ks.master(i) for i in range(len(tubes))]
         ks.close()
         res[form] = {"value": pushes / wall, "ms_per_step": wall / args.steps * 1e3}
@@ -351,6 +430,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=24)
     ap.add_argument("--ions", type=int, default=100_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-error-leg", action="store_true", help="skip the untimed mixed-vs-FP64 error leg")
     ap.add_argument("--form", choices=("both", "fp64", "mixed"), default="both")
     ap.add_argument("--tubes", type=int, default=512, help="tubes of the batched run (0 = skip it)")
     ap.add_argument("--full-physics", action="store_true",
@@ -375,7 +455,7 @@ def main():
     t0 = time.perf_counter()
     tube = grids.config2(n_total=args.ions)
     q0, v0, w0 = grids.initial_ions(tube, seed=1)
-    n_steps_total = args.warmup + args.steps + 1 + DROP_IN_STEPS
+    n_steps_total = max(args.warmup + args.steps + TUNE_STEPS_MAX + 1 + DROP_IN_STEPS, ERROR_STEPS_MIN)
     cap = len(q0) + (n_steps_total + 2) * (tube.n_inj_lb + tube.n_inj_ub) + 1024
     log(f"[bench] ions {len(q0)}, inj/step {tube.n_inj_lb}+{tube.n_inj_ub}, setup {time.perf_counter()-t0:.1f}s")
     host_gen_s = time.perf_counter() - t0
@@ -383,7 +463,6 @@ def main():
     runs = {}
     for f in forms:
         runs[f] = timed_run(f, tube, q0, v0, w0, cap, args)
-    del q0, v0, w0
     head = runs[forms[0]][1]
     ions_end = int(runs[forms[0]][0].master(0)["nq_counts"].sum())
     out = {
@@ -404,24 +483,28 @@ def main():
                    "form": forms[0], "ions_end": ions_end},
         "roofline": head["roofline"],
         "drop_in_step_ms": {f: r[1]["drop_in_step_ms"] for f, r in runs.items()},
-        "steps_run": 1 + args.warmup + args.steps + DROP_IN_STEPS,
+        "steps_run": 1 + args.warmup + head["untimed_tuning_steps"] + args.steps + DROP_IN_STEPS,
     }
+    out["launch"] = head["launch"]
+    out["untimed_tuning_steps"] = head["untimed_tuning_steps"]
     if "mixed" in runs and "fp64" in runs:
         m = runs["mixed"][1]
-        err = moment_errors(runs["fp64"][0].master(0), runs["mixed"][0].master(0))
         out["mixed"] = {"value": m["value"], "ms_per_step": m["ms_per_step"],
                         "speedup_vs_fp64": m["value"] / head["value"], "dtype": "f32+f64",
-                        "roofline": m["roofline"],
-                        "error_vs_fp64_after_last_step": err,
-                        # the six moments, then the ambipolar field apart: the field is
-                        # a stencil derivative of the filtered density, so one ion
-                        # binned in a neighbouring cell (an ion within ~1e-4 m of a cell
-                        # edge after 200 steps of binary32 stages) moves it by ~1e-6
-                        "max_l2_rel_moment_error": max(err[k]["l2_rel"] for k in MOMENTS if k != "EA"),
-                        "l2_rel_field_error_EA": err["EA"]["l2_rel"]}
+                        "launch": m["launch"], "untimed_tuning_steps": m["untimed_tuning_steps"],
+                        "roofline": m["roofline"]}
     out["export"] = export_run(runs[forms[0]][0], tube)
     for ks, _ in runs.values():
         ks.close()
+    if "mixed" in runs and "fp64" in runs and not args.no_error_leg:
+        # the six moments, then the ambipolar field apart: the field is a stencil
+        # derivative of the filtered density, so one ion binned in a neighbouring
+        # cell (an ion within ~1e-4 m of a cell edge after 200 steps of binary32
+        # stages) moves it by ~1e-6
+        out["mixed_error"] = mixed_error_run(tube, q0, v0, w0, cap, max(ERROR_STEPS_MIN, out["steps_run"]))
+        out["mixed"]["max_l2_rel_moment_error"] = out["mixed_error"]["max_l2_rel_moment_error"]
+        out["mixed"]["max_l2_rel_field_error_EA"] = out["mixed_error"]["max_l2_rel_field_error_EA"]
+    del q0, v0, w0
     out["initial_conditions"] = seed_run(args, host_gen_s + runs[forms[0]][1]["upload_s"])
     if args.tubes > 0:
         out["batched"] = batched_run(args)

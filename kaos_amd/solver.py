@@ -141,11 +141,13 @@ class KineticSolver:
         self._chk(self.lib.kaos_count(self.h, tube, C.byref(n)))
         return n.value
 
-    def master(self, tube):
+    def master(self, tube, phase=True):
+        """Outputs of the last master step; phase=False leaves out the
+        phase-space counts (nph_counts is then None)."""
         t = self.tubes[tube]
         nq, nc = t.nq, t.nq * t.nvperp * t.nvpar
         nqc = np.zeros(nq, np.int64)
-        nph = np.zeros(nc, np.int64)
+        nph = np.zeros(nc, np.int64) if phase else None
         NqRT = np.zeros(nq)
         M = np.zeros(6 * nq)
         EA = np.zeros(nq)
@@ -153,7 +155,7 @@ class KineticSolver:
         esc = np.zeros(4)
         self._chk(self.lib.kaos_download_master(self.h, tube, ptr(nqc, C.c_int64), ptr(nph, C.c_int64),
                                                 ptr(NqRT), ptr(M), ptr(EA), ptr(EG), ptr(esc)))
-        out = dict(nq_counts=nqc, nph_counts=nph.reshape(nq, t.nvperp, t.nvpar), NqRT=NqRT,
+        out = dict(nq_counts=nqc, nph_counts=None if nph is None else nph.reshape(nq, t.nvperp, t.nvpar), NqRT=NqRT,
                    EA=EA, EG=EG, esc=esc)
         for i, n in enumerate(MOMENT_NAMES):
             out[n] = M[i * nq:(i + 1) * nq].copy()
