@@ -278,11 +278,63 @@ int kaos_count(kaos_ctx* ctx, int32_t tube, int64_t* n);
  * injected ions), any pointer may be NULL; n_cap is the capacity of the
  * caller's arrays.  qk4: stage-4 dipole q of the last push at a step before a
  * master step; aea/ag: AEAmagN/AGmagN; qind/vparind/vperpind: Qindk1,
- * Vparindk1, Vperpindk1 of the last push. */
+ * Vparindk1, Vperpindk1 of the last push, integer(kind=dp) as the reference
+ * declares them (_8_0_4_DataTypeAllocA.f90:35-83, dp = 8, _ModelInput.f90:15).
+ * In the mixed form vparind is the cell of the stored Vpar (the index the next
+ * push bins). */
 int kaos_download_particles(kaos_ctx* ctx, int32_t tube, int64_t n_cap, int64_t* n_out,
                             double* x, double* y, double* z, double* Vx, double* Vy, double* Vz,
                             double* vperp, double* qk4, double* aea, double* ag,
-                            int32_t* qind, int32_t* vparind, int32_t* vperpind);
+                            int64_t* qind, int64_t* vparind, int64_t* vperpind);
+
+/* Both perpendicular components (the reference's Vperp1, Vperp2 arrays) of
+ * the live ions of a tube, in the order of kaos_download_particles; either
+ * pointer may be NULL. */
+int kaos_download_vperp(kaos_ctx* ctx, int32_t tube, int64_t n_cap, int64_t* n_out, double* vperp1,
+                        double* vperp2);
+
+/* Hand a tube the per-particle state the Fortran driver holds
+ * (_8_0_4_DataTypeAllocA.f90:35-83): x, y, z, Vx, Vy, Vz, Vperp1, Vperp2,
+ * Vperp, AEAmag, AGmag, qk4 (_8_1_3_KineticUpdateC.f90:31-62, the field
+ * coordinate of the next master step) and the int64 cell indices Qindk1,
+ * Vparindk1, Vperpindk1 of the last push; vperp1/vperp2 may be NULL (then
+ * Vperp1 = Vperp, Vperp2 = 0).  The tube's ions become these n ions, numbered
+ * in the given order.  Allowed before the first step (in place of
+ * kaos_upload_particles) and between steps (the tube's other state stays).
+ * The Cartesian state is read as KineticUpdateASub reads it: r = |(x, y, z)|
+ * (_8_1_1_KineticUpdateA.f90:117), Vpar = b-hat . V (:978-980); so a
+ * download followed by this upload continues the run to rounding (the
+ * Cartesian triple of a line point does not fix its radius to the last bit;
+ * kaos_save_state / kaos_load_state continue bit for bit).  KAOS_E_ARG for a
+ * cell index out of range or a position at the origin or not finite. */
+int kaos_upload_state(kaos_ctx* ctx, int32_t tube, int64_t n, const double* x, const double* y, const double* z,
+                      const double* Vx, const double* Vy, const double* Vz, const double* vperp1,
+                      const double* vperp2, const double* vperp, const double* aea, const double* ag,
+                      const double* qk4, const int64_t* qind, const int64_t* vparind, const int64_t* vperpind);
+
+/* The moments of the last master step of a tube (6*nq, the M of
+ * kaos_download_master), which the next master step's moment filter and
+ * ambipolar field read (AmbipolarEfieldSub uses M*phRT(nn-1),
+ * _8_7_AmbipolarEfield.f90:40-82): for a context continuing another run. */
+int kaos_upload_moments(kaos_ctx* ctx, int32_t tube, const double* M);
+
+/* A context that has not stepped continues another run after its step n_last:
+ * the next kaos_step is n_last + 1 (the seeded streams are keyed by the step).
+ * Needs every tube's ions (kaos_upload_state) and, for the fields of the next
+ * master step, kaos_upload_moments.  Replaces the loop counter of
+ * ComputationalTimeStepLoop (_8_KineticSolver.f90:54). */
+int kaos_resume(kaos_ctx* ctx, int64_t n_last);
+
+/* Restart files of the hot path (no counterpart in the reference, whose only
+ * restart is re-seeding from the spin-up density, _1_SimParameterization.f90:
+ * 801-930): kaos_save_state writes the context's step state, per-cell outputs
+ * and every tube's ion state as stored on the device to `path`;
+ * kaos_load_state reads it into a context that has not stepped and was
+ * created with the same kaos_cfg and grids (else KAOS_E_ARG), which then
+ * continues bit for bit as the saved one would have.  KAOS_E_IO on a file
+ * error. */
+int kaos_save_state(kaos_ctx* ctx, const char* path);
+int kaos_load_state(kaos_ctx* ctx, const char* path);
 
 /* Launch shape of the push kernels: ions are processed in tiles of 1024, a
  * group of `tiles_per_group` tiles (1..63) shares one block-local LDS window of

@@ -69,7 +69,9 @@ class KaosStepOut(C.Structure):
 
 EXPORTS = ("kaos_init", "kaos_set_grid", "kaos_upload_particles", "kaos_upload_vperp", "kaos_cell_counts",
            "kaos_seed_particles", "kaos_step", "kaos_run",
-           "kaos_download_master", "kaos_download_master3d", "kaos_export_master", "kaos_export_spinup", "kaos_count", "kaos_download_particles", "kaos_set_launch",
+           "kaos_download_master", "kaos_download_master3d", "kaos_export_master", "kaos_export_spinup",
+           "kaos_count", "kaos_download_particles", "kaos_download_vperp", "kaos_upload_state",
+           "kaos_upload_moments", "kaos_resume", "kaos_save_state", "kaos_load_state", "kaos_set_launch",
            "kaos_get_launch", "kaos_profile", "kaos_profile_read", "kaos_last_error", "kaos_finalize")
 
 _lib = None
@@ -99,7 +101,13 @@ def load(path=LIB_PATH):
     lib.kaos_export_spinup.argtypes = [P, C.c_int32, C.POINTER(KaosSpinupInfo)]
     lib.kaos_download_master3d.argtypes = [P, C.c_int32, i64p, _dp]
     lib.kaos_count.argtypes = [P, C.c_int32, i64p]
-    lib.kaos_download_particles.argtypes = [P, C.c_int32, C.c_int64, i64p] + [_dp] * 10 + [i32p] * 3
+    lib.kaos_download_particles.argtypes = [P, C.c_int32, C.c_int64, i64p] + [_dp] * 10 + [i64p] * 3
+    lib.kaos_download_vperp.argtypes = [P, C.c_int32, C.c_int64, i64p, _dp, _dp]
+    lib.kaos_upload_state.argtypes = [P, C.c_int32, C.c_int64] + [_dp] * 12 + [i64p] * 3
+    lib.kaos_upload_moments.argtypes = [P, C.c_int32, _dp]
+    lib.kaos_resume.argtypes = [P, C.c_int64]
+    lib.kaos_save_state.argtypes = [P, C.c_char_p]
+    lib.kaos_load_state.argtypes = [P, C.c_char_p]
     lib.kaos_set_launch.argtypes = [P, C.c_int32, C.c_int32, C.c_int32]
     lib.kaos_get_launch.argtypes = [P, i32p, i32p, i32p, i32p]
     lib.kaos_profile.argtypes = [P, C.c_int32]

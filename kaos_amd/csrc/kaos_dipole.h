@@ -646,10 +646,11 @@ KAOS_HD void kaos_philox4x32(const uint32_t ctr_in[4], const uint32_t key_in[2],
 /* The uniforms in (0, 1] of one ion's two Gaussian kicks at step n
  * (GaussianRNGSub twice in the reference, _8_1_1_0_WaveHeating.f90:56-64;
  * here one Box-Muller pair from u[0], u[1]): Philox4x32-10 keyed by the seed,
- * counter (ion id, step, tube, "ICR").                                      */
+ * counter (low 32 bits of the ion id, step, tube, "ICR" xor the id's high 32
+ * bits: ids below 2^32 leave the last word "ICR0").                         */
 KAOS_HD void kaos_icr_uniforms(uint64_t seed, int64_t step, int tube, int64_t id, double u[4]) {
   uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) ^ 0x49435248u };
-  uint32_t ctr[4] = { (uint32_t)id, (uint32_t)step, (uint32_t)tube, 0x49435230u };
+  uint32_t ctr[4] = { (uint32_t)id, (uint32_t)step, (uint32_t)tube, 0x49435230u ^ (uint32_t)((uint64_t)id >> 32) };
   uint32_t o[4];
   kaos_philox4x32(ctr, key, o);
   for (int i = 0; i < 4; ++i) u[i] = ((double)o[i] + 1.0) * (1.0 / 4294967296.0);

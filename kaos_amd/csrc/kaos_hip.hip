@@ -87,7 +87,7 @@ struct StepArgs {
   int lds_nq, lds_nv;                        // largest nq / nvpar over the tubes (LDS sizing)
   uint32_t* cell;
   uint16_t* c12;                             // mixed form, 3-D grid: Vperp1 | Vperp2 << 8 cells (binary64)
-  int* id;                                   // order key: upload index, then injection order
+  long long* id;                             // order key: upload index, then injection order (64-bit: never wraps)
   const long long* count;
   unsigned int *hist, *nqoff;               // phase-space counts; config counts off the v-grid
   unsigned long long* esc_acc;
@@ -714,7 +714,7 @@ __device__ __forceinline__ uint16_t kd_cells12(const DevTube& T, const StepArgs&
 // q, velocity (Vpar, Vperp1, Vperp2), |Vperp| = vperp.  The k0 cells
 // (KSA:49-326) come from q clamped into the grid as KSA:55-129 does, and from
 // Vpar and vperp.
-__device__ __forceinline__ void kd_store_initial(const DevTube& T, const StepArgs& a, long long j, int id, double q,
+__device__ __forceinline__ void kd_store_initial(const DevTube& T, const StepArgs& a, long long j, long long id, double q,
                                                  double rs, double vpar, double v1, double v2, double vperp,
                                                  const double* qGH, const double* vparGH, const double* vperpGH,
                                                  const double* rref, int mixed) {
@@ -757,8 +757,63 @@ __global__ void k_upload(DevTube T, long long n, const double* q0, const double*
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double q = q0[i];
-  kd_store_initial(T, a, T.off + i, (int)i, q, kaos_rs_of_q(q, T.p), vpar0[i], vperp0[i], 0.0, vperp0[i], qGH,
+  kd_store_initial(T, a, T.off + i, i, q, kaos_rs_of_q(q, T.p), vpar0[i], vperp0[i], 0.0, vperp0[i], qGH,
                    vparGH, vperpGH, rref, mixed);
+}
+
+// Per-ion state in the reference's arrays (kaos_upload_state), read the way
+// KineticUpdateASub reads its Cartesian state: r = sqrt(x^2 + y^2 + z^2)
+// (_8_1_1_KineticUpdateA.f90:117), the hemisphere from the sign of z, and
+// Vpar = b-hat . V at that point (:978-980).  The cell word takes Qindk1 as
+// given (the removal flag of the next master step and the search hint), the
+// Vperp cell of |(Vperp1, Vperp2)| (the given one when that is off the grid:
+// the 2-D branch's stale index, :1087-1127) and, in the binary64 form, the
+// given Vparindk1 (the push finds the cell of the stored Vpar itself); the
+// mixed form stores the cell of the stored Vpar beside its offset, as its push
+// does.  Ions are numbered id0 + i.
+struct UpState {
+  const double *x, *y, *z, *vx, *vy, *vz, *v1, *v2, *vperp, *aea, *ag, *qk4;
+  const long long *qind, *vparind, *vperpind;
+};
+
+__global__ void k_upload_state(DevTube T, long long n, UpState u, long long id0, const double* vparGH,
+                               const double* vperpGH, const double* rref, StepArgs a, int mixed) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long j = T.off + i;
+  const double x = u.x[i], y = u.y[i], z = u.z[i];
+  const double r = sqrt((x * x + y * y) + z * z);
+  const double rs = (z < 0.0 || (z == 0.0 && T.south)) ? -r : r;
+  const kaos_line_t L = kaos_line(rs, T.p);
+  const kaos_qb_t g = kaos_stageA(L);
+  const double vpar = g.bR * (u.vx[i] * T.cphi + u.vy[i] * T.sphi) + g.bZ * u.vz[i];
+  const double v1 = u.v1 ? u.v1[i] : u.vperp[i];
+  const double v2 = u.v2 ? u.v2[i] : 0.0;
+  const double vperp = u.v1 ? fabs(sqrt(v1 * v1 + v2 * v2)) : u.vperp[i];
+  const int qc = (int)u.qind[i];
+  int iq = kaos_find_vcell(vperp, vperpGH, vperpGH[T.nvperp], T.nvperp);
+  if (iq < 1) iq = (int)u.vperpind[i];
+  a.id[j] = id0 + i;
+  if (mixed) {
+    const int Q = kx_refq(qc, T.nq);
+    bool rflag, vflag;
+    a.rdi[j] = kx_enc(rs - rref[Q - 1], a.rk[T.goff + Q - 1], rflag);
+    int ip = kaos_find_vcell(vpar, vparGH, vparGH[T.nvpar], T.nvpar);
+    a.vpi[j] = kx_enc_v(vpar, ip, vparGH, a.vk + T.voff, T.nvpar, vflag);
+    if (ip < 1) ip = (int)u.vparind[i];
+    a.vrf[j] = (float)v1;
+    a.vrf2[j] = (float)v2;
+    if (a.c12) a.c12[j] = kd_cells12(T, a, v1, v2);
+    a.qkf[j] = (float)u.qk4[i];
+    a.aef[j] = (float)u.aea[i]; a.agf[j] = (float)u.ag[i];
+    a.cell[j] = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip |
+                (rflag ? KAOS_RFLAG : 0u) | (vflag ? KAOS_VFLAG : 0u);
+  } else {
+    a.rs[j] = rs;
+    a.vp[j] = vpar; a.vr[j] = v1; a.vr2[j] = v2; a.qk[j] = u.qk4[i];
+    a.ae[j] = u.aea[i]; a.ag[j] = u.ag[i];
+    a.cell[j] = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)u.vparind[i];
+  }
 }
 
 // Initial ions generated on the device: ion i of the tube (cell by cell,
@@ -776,7 +831,7 @@ __global__ void k_seed(DevTube T, long long n, const long long* coff, const doub
   double u[6], rs, vpar, v1, v2, vperp;
   kaos_seed_uniforms(seed, tube, i, u);
   kaos_seed_ion(q, T.p, kaos_thermal_speed(temp[lo], T.ms), T.cphi, T.sphi, u, &rs, &vpar, &v1, &v2, &vperp);
-  kd_store_initial(T, a, T.off + i, (int)i, q, rs, vpar, v1, v2, vperp, qGH, vparGH, vperpGH, rref, mixed);
+  kd_store_initial(T, a, T.off + i, i, q, rs, vpar, v1, v2, vperp, qGH, vparGH, vperpGH, rref, mixed);
 }
 
 // Both perpendicular components of the ions just uploaded; the Vperp cell of
@@ -892,7 +947,7 @@ __global__ void k_inject(StepArgs a, const long long* inj_base, const long long*
     a.ae[j] = aea; a.ag[j] = agv;
   }
   a.cell[j] = ((uint32_t)(qcell + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip | flags;
-  a.id[j] = (int)(next_id[t] + k);
+  a.id[j] = next_id[t] + k;
   if (a.hist3) {
     // 3-D grid: the cells of the injected velocity (KSB:339-476), 0 off the grid
     if (qcell >= 1) {
@@ -1378,7 +1433,7 @@ struct StateBufs {
   float* vrf2 = nullptr;
   uint32_t* cell = nullptr;
   uint16_t* c12 = nullptr;
-  int* id = nullptr;
+  long long* id = nullptr;
 };
 
 struct kaos_ctx {
@@ -1737,7 +1792,7 @@ static int build_device(kaos_ctx* c) {
     }
     HIPCHK(c, hipMalloc((void**)&B.cell, sizeof(uint32_t) * nslot));
     HIPCHK(c, hipMemset(B.cell, 0xFF, sizeof(uint32_t) * nslot));
-    HIPCHK(c, hipMalloc((void**)&B.id, sizeof(int) * nslot));
+    HIPCHK(c, hipMalloc((void**)&B.id, sizeof(long long) * nslot));
     if (mixed && (c->cfg.flags & KAOS_FLAG_VPERP3D)) {
       HIPCHK(c, hipMalloc((void**)&B.c12, sizeof(uint16_t) * nslot));
       HIPCHK(c, hipMemset(B.c12, 0, sizeof(uint16_t) * nslot));
@@ -1956,7 +2011,7 @@ extern "C" int kaos_seed_particles(kaos_ctx* c, int32_t tube, const int64_t* cou
     if (counts[Q] > 0 && !(temperature[Q] > 0.0 && std::isfinite(temperature[Q])))
       return fail(c, KAOS_E_ARG, "temperature of a populated cell must be finite and > 0");
     coff[Q + 1] = coff[Q] + counts[Q];
-    if (coff[Q + 1] > (long long)INT32_MAX) return fail(c, KAOS_E_CAPACITY, "more ions than ion ids");
+    if (coff[Q + 1] >= (1LL << 28)) return fail(c, KAOS_E_CAPACITY, "more ions than the tube capacity");
   }
   const long long n = coff[T.nq];
   HostTube& h = c->ht[tube];
@@ -2607,78 +2662,358 @@ extern "C" int kaos_export_spinup(kaos_ctx* c, int32_t tube, const kaos_spinup_i
   return KAOS_OK;
 }
 
+// The live ions of a tube on the host, in the reference's order (survivors
+// in order, then injected ions: by id), with their Cartesian state.
+struct TubeDump {
+  long long n = 0;                              // slots
+  std::vector<double> xyz, vr, vr2, qk, ae, ag;
+  std::vector<uint32_t> cell;
+  std::vector<long long> ids, order;            // order: live slots sorted by id
+};
+
+static int fetch_tube(kaos_ctx* c, int32_t tube, TubeDump& d) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const DevTube& T = c->dt[tube];
+  const long long n = c->ht[tube].count;
+  const bool mixed = c->cfg.precision == KAOS_MIXED;
+  d.n = n;
+  d.xyz.assign(6 * (size_t)n, 0.0);
+  d.vr.assign(n, 0.0); d.vr2.assign(n, 0.0); d.qk.assign(n, 0.0); d.ae.assign(n, 0.0); d.ag.assign(n, 0.0);
+  d.cell.assign(n, 0u); d.ids.assign(n, 0); d.order.clear();
+  if (!n) return KAOS_OK;
+  std::vector<char> live(n, 0);
+  double* d6 = nullptr;
+  HIPCHK(c, hipMalloc((void**)&d6, sizeof(double) * 6 * n));
+  StepArgs a = make_args(c, 0, 0, 0, 0);
+  hipLaunchKernelGGL(k_reconstruct, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, T, n, a,
+                     (int)mixed, d6);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(d.xyz.data(), d6, sizeof(double) * 6 * n, hipMemcpyDeviceToHost));
+  (void)hipFree(d6);
+  const StateBufs& B = c->st[c->cur];
+  HIPCHK(c, hipMemcpy(d.cell.data(), B.cell + T.off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(d.ids.data(), B.id + T.off, sizeof(long long) * n, hipMemcpyDeviceToHost));
+  if (mixed) {
+    std::vector<float> f(n);
+    float* src[5] = {B.vrf, B.vrf2, B.qkf, B.aef, B.agf};
+    std::vector<double>* dst[5] = {&d.vr, &d.vr2, &d.qk, &d.ae, &d.ag};
+    for (int i = 0; i < 5; ++i) {
+      HIPCHK(c, hipMemcpy(f.data(), src[i] + T.off, sizeof(float) * n, hipMemcpyDeviceToHost));
+      for (long long j = 0; j < n; ++j) (*dst[i])[j] = (double)f[j];
+    }
+    for (long long j = 0; j < n; ++j) live[j] = d.cell[j] != KAOS_DEAD;
+  } else {
+    std::vector<double> rs(n);
+    HIPCHK(c, hipMemcpy(rs.data(), B.rs + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+    double* src[5] = {B.vr, B.vr2, B.qk, B.ae, B.ag};
+    std::vector<double>* dst[5] = {&d.vr, &d.vr2, &d.qk, &d.ae, &d.ag};
+    for (int i = 0; i < 5; ++i)
+      HIPCHK(c, hipMemcpy(dst[i]->data(), src[i] + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+    for (long long j = 0; j < n; ++j) live[j] = rs[j] != 0.0;
+  }
+  d.order.reserve(n);
+  for (long long j = 0; j < n; ++j) if (live[j]) d.order.push_back(j);
+  std::sort(d.order.begin(), d.order.end(), [&](long long p, long long q) { return d.ids[p] < d.ids[q]; });
+  return KAOS_OK;
+}
+
 extern "C" int kaos_download_particles(kaos_ctx* c, int32_t tube, int64_t n_cap, int64_t* n_out,
                                        double* x, double* y, double* z, double* Vx, double* Vy, double* Vz,
                                        double* vperp, double* qk4, double* aea, double* ag,
-                                       int32_t* qind, int32_t* vparind, int32_t* vperpind) {
+                                       int64_t* qind, int64_t* vparind, int64_t* vperpind) {
   if (!c || !n_out) return KAOS_E_ARG;
   if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
   if (!c->built) { *n_out = 0; return KAOS_OK; }
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  const DevTube& T = c->dt[tube];
-  long long n = c->ht[tube].count;
-  const bool mixed = c->cfg.precision == KAOS_MIXED;
-  std::vector<double> vr(n), vr2(n), qk(n), ae(n), agv(n), xyz(6 * (size_t)n);
-  std::vector<uint32_t> cell(n);
-  std::vector<int> ids(n);
-  std::vector<char> live(n, 0);
-  if (n) {
-    double* d6 = nullptr;
-    HIPCHK(c, hipMalloc((void**)&d6, sizeof(double) * 6 * n));
-    StepArgs a = make_args(c, 0, 0, 0, 0);
-    hipLaunchKernelGGL(k_reconstruct, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, T, n, a,
-                       (int)mixed, d6);
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(xyz.data(), d6, sizeof(double) * 6 * n, hipMemcpyDeviceToHost));
-    (void)hipFree(d6);
-    const StateBufs& B = c->st[c->cur];
-    HIPCHK(c, hipMemcpy(cell.data(), B.cell + T.off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(ids.data(), B.id + T.off, sizeof(int) * n, hipMemcpyDeviceToHost));
-    if (mixed) {
-      std::vector<float> f(n);
-      float* src[5] = {B.vrf, B.vrf2, B.qkf, B.aef, B.agf};
-      std::vector<double>* dst[5] = {&vr, &vr2, &qk, &ae, &agv};
-      for (int i = 0; i < 5; ++i) {
-        HIPCHK(c, hipMemcpy(f.data(), src[i] + T.off, sizeof(float) * n, hipMemcpyDeviceToHost));
-        for (long long j = 0; j < n; ++j) (*dst[i])[j] = (double)f[j];
-      }
-      for (long long j = 0; j < n; ++j) live[j] = cell[j] != KAOS_DEAD;
-    } else {
-      std::vector<double> rs(n);
-      HIPCHK(c, hipMemcpy(rs.data(), B.rs + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
-      HIPCHK(c, hipMemcpy(vr.data(), B.vr + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
-      HIPCHK(c, hipMemcpy(vr2.data(), B.vr2 + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
-      HIPCHK(c, hipMemcpy(qk.data(), B.qk + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
-      HIPCHK(c, hipMemcpy(ae.data(), B.ae + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
-      HIPCHK(c, hipMemcpy(agv.data(), B.ag + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
-      for (long long j = 0; j < n; ++j) live[j] = rs[j] != 0.0;
-    }
-  }
-  // live ions in the reference's order: by id (upload index, then injection order)
-  std::vector<long long> order;
-  order.reserve(n);
-  for (long long j = 0; j < n; ++j) if (live[j]) order.push_back(j);
-  std::sort(order.begin(), order.end(), [&](long long p, long long q) { return ids[p] < ids[q]; });
+  TubeDump d;
+  int rc = fetch_tube(c, tube, d);
+  if (rc) return rc;
+  if ((long long)d.order.size() > n_cap) return fail(c, KAOS_E_CAPACITY, "output arrays too small");
   long long k = 0;
-  for (long long j : order) {
-    if (k >= n_cap) return fail(c, KAOS_E_CAPACITY, "output arrays too small");
-    if (x) x[k] = xyz[6 * j + 0];
-    if (y) y[k] = xyz[6 * j + 1];
-    if (z) z[k] = xyz[6 * j + 2];
-    if (Vx) Vx[k] = xyz[6 * j + 3];
-    if (Vy) Vy[k] = xyz[6 * j + 4];
-    if (Vz) Vz[k] = xyz[6 * j + 5];
-    if (vperp) vperp[k] = fabs(sqrt(vr[j] * vr[j] + vr2[j] * vr2[j]));   // |(Vperp1, Vperp2)|
-    if (qk4) qk4[k] = qk[j];
-    if (aea) aea[k] = ae[j];
-    if (ag) ag[k] = agv[j];
-    if (qind) qind[k] = (int32_t)((cell[j] >> 16) & KAOS_QMASK) - 1;
-    if (vperpind) vperpind[k] = (int32_t)((cell[j] >> 8) & 0xFFu);
-    if (vparind) vparind[k] = (int32_t)(cell[j] & 0xFFu);
+  for (long long j : d.order) {
+    if (x) x[k] = d.xyz[6 * j + 0];
+    if (y) y[k] = d.xyz[6 * j + 1];
+    if (z) z[k] = d.xyz[6 * j + 2];
+    if (Vx) Vx[k] = d.xyz[6 * j + 3];
+    if (Vy) Vy[k] = d.xyz[6 * j + 4];
+    if (Vz) Vz[k] = d.xyz[6 * j + 5];
+    if (vperp) vperp[k] = fabs(sqrt(d.vr[j] * d.vr[j] + d.vr2[j] * d.vr2[j]));   // |(Vperp1, Vperp2)|
+    if (qk4) qk4[k] = d.qk[j];
+    if (aea) aea[k] = d.ae[j];
+    if (ag) ag[k] = d.ag[j];
+    if (qind) qind[k] = (int64_t)((d.cell[j] >> 16) & KAOS_QMASK) - 1;
+    if (vperpind) vperpind[k] = (int64_t)((d.cell[j] >> 8) & 0xFFu);
+    if (vparind) vparind[k] = (int64_t)(d.cell[j] & 0xFFu);
     ++k;
   }
   *n_out = k;
+  return KAOS_OK;
+}
+
+extern "C" int kaos_download_vperp(kaos_ctx* c, int32_t tube, int64_t n_cap, int64_t* n_out, double* vperp1,
+                                   double* vperp2) {
+  if (!c || !n_out) return KAOS_E_ARG;
+  if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
+  if (!c->built) { *n_out = 0; return KAOS_OK; }
+  TubeDump d;
+  int rc = fetch_tube(c, tube, d);
+  if (rc) return rc;
+  if ((long long)d.order.size() > n_cap) return fail(c, KAOS_E_CAPACITY, "output arrays too small");
+  long long k = 0;
+  for (long long j : d.order) {
+    if (vperp1) vperp1[k] = d.vr[j];
+    if (vperp2) vperp2[k] = d.vr2[j];
+    ++k;
+  }
+  *n_out = k;
+  return KAOS_OK;
+}
+
+extern "C" int kaos_upload_state(kaos_ctx* c, int32_t tube, int64_t n, const double* x, const double* y,
+                                 const double* z, const double* Vx, const double* Vy, const double* Vz,
+                                 const double* vperp1, const double* vperp2, const double* vperp,
+                                 const double* aea, const double* ag, const double* qk4, const int64_t* qind,
+                                 const int64_t* vparind, const int64_t* vperpind) {
+  if (!c) return KAOS_E_ARG;
+  if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
+  if (n < 0) return fail(c, KAOS_E_ARG, "negative ion count");
+  if (n > 0 && (!x || !y || !z || !Vx || !Vy || !Vz || !vperp || !aea || !ag || !qk4 || !qind || !vparind ||
+                !vperpind))
+    return fail(c, KAOS_E_ARG, "bad particle arrays (only vperp1 and vperp2 may be NULL)");
+  if ((vperp1 == nullptr) != (vperp2 == nullptr)) return fail(c, KAOS_E_ARG, "vperp1 and vperp2 go together");
+  if (!c->built) { int rc = build_device(c); if (rc) return rc; }
+  HostTube& h = c->ht[tube];
+  const DevTube& T = c->dt[tube];
+  if (n > h.cap) return fail(c, KAOS_E_CAPACITY, "more ions than the tube capacity");
+  for (int64_t i = 0; i < n; ++i) {
+    if (qind[i] < -1 || qind[i] > T.nq || vparind[i] < 0 || vparind[i] > T.nvpar || vperpind[i] < 0 ||
+        vperpind[i] > T.nvperp)
+      return fail(c, KAOS_E_ARG, "cell index out of range at ion " + std::to_string(i));
+    if (!(x[i] != 0.0 || y[i] != 0.0 || z[i] != 0.0) || !std::isfinite(x[i] + y[i] + z[i]))
+      return fail(c, KAOS_E_ARG, "position not finite or at the origin at ion " + std::to_string(i));
+  }
+  if (n) {
+    const double* src[12] = {x, y, z, Vx, Vy, Vz, vperp1, vperp2, vperp, aea, ag, qk4};
+    const int64_t* isrc[3] = {qind, vparind, vperpind};
+    double* dd[12] = {};
+    long long* di[3] = {};
+    int rc = KAOS_OK;
+    for (int k = 0; k < 12 && !rc; ++k) {
+      if (!src[k]) continue;
+      if (hipMalloc((void**)&dd[k], sizeof(double) * n) != hipSuccess ||
+          hipMemcpy(dd[k], src[k], sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(c, KAOS_E_DEVICE, "kaos_upload_state: device staging failed");
+    }
+    for (int k = 0; k < 3 && !rc; ++k)
+      if (hipMalloc((void**)&di[k], sizeof(long long) * n) != hipSuccess ||
+          hipMemcpy(di[k], isrc[k], sizeof(long long) * n, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(c, KAOS_E_DEVICE, "kaos_upload_state: device staging failed");
+    if (!rc) {
+      UpState u = {dd[0], dd[1], dd[2], dd[3], dd[4], dd[5], dd[6], dd[7], dd[8], dd[9], dd[10], dd[11],
+                   di[0], di[1], di[2]};
+      StepArgs a = make_args(c, 0, 0, 0, 0);
+      hipLaunchKernelGGL(k_upload_state, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, T,
+                         (long long)n, u, 0LL, c->d_vparGH + T.voff, c->d_vperpGH + T.woff, c->d_rref + T.goff, a,
+                         (int)(c->cfg.precision == KAOS_MIXED));
+      if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+        rc = fail(c, KAOS_E_DEVICE, "kaos_upload_state: kernel failed");
+    }
+    for (double* p : dd) if (p) (void)hipFree(p);
+    for (long long* p : di) if (p) (void)hipFree(p);
+    if (rc) return rc;
+  }
+  // slots past the new count are dead (a mid-run upload may replace more ions)
+  if (h.count > n) {
+    const long long nd = h.count - n;
+    const StateBufs& B = c->st[c->cur];
+    HIPCHK(c, hipMemsetAsync(B.cell + T.off + n, 0xFF, sizeof(uint32_t) * nd, c->stream));
+    if (B.rs) HIPCHK(c, hipMemsetAsync(B.rs + T.off + n, 0, sizeof(double) * nd, c->stream));
+  }
+  long long cnt = n, nid = 0;
+  HIPCHK(c, hipMemcpy(&nid, c->d_next_id + tube, sizeof(long long), hipMemcpyDeviceToHost));
+  if (nid < cnt) nid = cnt;
+  HIPCHK(c, hipMemcpyAsync(c->d_count + tube, &cnt, sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_live + tube, &cnt, sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_next_id + tube, &nid, sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  h.count = n;
+  h.uploaded = true;
+  return KAOS_OK;
+}
+
+extern "C" int kaos_upload_moments(kaos_ctx* c, int32_t tube, const double* M) {
+  if (!c) return KAOS_E_ARG;
+  if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
+  if (!M) return fail(c, KAOS_E_ARG, "no moments");
+  if (!c->built) { int rc = build_device(c); if (rc) return rc; }
+  const DevTube& T = c->dt[tube];
+  HIPCHK(c, hipMemcpy(c->d_M + 6 * T.goff, M, sizeof(double) * 6 * T.nq, hipMemcpyHostToDevice));
+  return KAOS_OK;
+}
+
+extern "C" int kaos_resume(kaos_ctx* c, int64_t n_last) {
+  if (!c) return KAOS_E_ARG;
+  if (n_last < 1) return fail(c, KAOS_E_ARG, "n_last must be >= 1");
+  if (c->last_step != 0) return fail(c, KAOS_E_STATE, "kaos_resume needs a context that has not stepped");
+  if (!c->built) return fail(c, KAOS_E_STATE, "kaos_resume needs the tubes' ions (kaos_upload_state)");
+  for (int t = 0; t < c->cfg.ntubes; ++t)
+    if (!c->ht[t].uploaded) return fail(c, KAOS_E_STATE, "tube " + std::to_string(t) + " has no ions");
+  long long s = n_last;
+  HIPCHK(c, hipMemcpy(c->d_stepno, &s, sizeof(long long), hipMemcpyHostToDevice));
+  c->last_step = n_last;
+  c->last_master = (n_last == 1) ? 1 : (int)is_master(c, n_last);
+  return KAOS_OK;
+}
+
+// Checkpoint file: a header, the context's step state and, per tube, its
+// bookkeeping and the slots [0, count) of every state array of the live
+// buffer, as stored (so the continuation is bit for bit the uninterrupted run
+// in either precision form).
+static const char KAOS_CKPT_MAGIC[8] = {'K', 'A', 'O', 'S', 'C', 'K', 'P', '1'};
+
+static uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+  const unsigned char* b = (const unsigned char*)p;
+  for (size_t i = 0; i < n; ++i) { h ^= b[i]; h *= 1099511628211ull; }
+  return h;
+}
+
+// Hash of everything that fixes a context's layout and arithmetic: the run
+// configuration and every tube's grid, species and flags.
+static uint64_t ctx_fingerprint(const kaos_ctx* c) {
+  uint64_t h = 1469598103934665603ull;
+  const int32_t cf[4] = {c->cfg.ntubes, c->cfg.precision, c->cfg.ndatfac, c->cfg.flags & KAOS_FLAG_VPERP3D};
+  h = fnv1a(h, cf, sizeof(cf));
+  h = fnv1a(h, &c->cfg.seed, sizeof(c->cfg.seed));
+  for (const HostTube& t : c->ht) {
+    const kaos_grid& g = t.g;
+    const int32_t gi[14] = {g.nq, g.nvpar, g.nvperp, g.grav, g.eamb, g.eambsign, g.filter_pt, g.lbcond,
+                            g.ubcond, g.filter_pt_m1par, g.icr, g.nvperp1, g.nvperp2, 0};
+    h = fnv1a(h, gi, sizeof(gi));
+    const double gd[10] = {g.p, g.cphi, g.sphi, g.h, g.ms, g.qs, g.nsnormfac, g.Ts_lb, g.Ts_ub,
+                           (double)(g.n_inj_lb + 3 * g.n_inj_ub)};
+    h = fnv1a(h, gd, sizeof(gd));
+    const std::vector<double>* arr[] = {&t.qGL, &t.qGH, &t.qGC, &t.hq, &t.dq, &t.d3x, &t.rGC, &t.cth, &t.ell,
+                                        &t.Te, &t.vparGL, &t.vparGH, &t.vperpGL, &t.vperpGH, &t.d3v,
+                                        &t.v1GL, &t.v1GH, &t.v2GL, &t.v2GH};
+    for (const std::vector<double>* v : arr) h = fnv1a(h, v->data(), sizeof(double) * v->size());
+  }
+  return h;
+}
+
+// The device arrays of a checkpoint: (pointer, element size, elements) per tube
+// slot array, then the per-cell and per-tube context arrays.
+static std::vector<std::pair<void*, size_t>> slot_arrays(kaos_ctx* c) {
+  StateBufs& B = c->st[c->cur];
+  std::vector<std::pair<void*, size_t>> v;
+  if (c->cfg.precision == KAOS_MIXED) {
+    v = {{B.rdi, 4}, {B.vpi, 4}, {B.vrf, 4}, {B.vrf2, 4}, {B.qkf, 4}, {B.aef, 4}, {B.agf, 4}};
+    if (B.c12) v.push_back({B.c12, 2});
+  } else {
+    v = {{B.rs, 8}, {B.vp, 8}, {B.vr, 8}, {B.vr2, 8}, {B.qk, 8}, {B.ae, 8}, {B.ag, 8}};
+  }
+  v.push_back({B.cell, 4});
+  v.push_back({B.id, 8});
+  return v;
+}
+
+static std::vector<std::pair<void*, size_t>> ctx_arrays(kaos_ctx* c) {
+  const size_t g = (size_t)c->ncell_tot, nt = (size_t)c->cfg.ntubes;
+  std::vector<std::pair<void*, size_t>> v = {
+      {c->d_M, 8 * 6 * g}, {c->d_M0F, 8 * g}, {c->d_M1F, 8 * g}, {c->d_EA, 8 * g}, {c->d_NqRT, 8 * g},
+      {c->d_nqc, 4 * g}, {c->d_nqoff, 4 * g}, {c->d_hist, 4 * (size_t)c->nhist_tot},
+      {c->d_esc_acc, 8 * 2 * nt}, {c->d_esc_rep, 8 * 2 * nt}, {c->d_stepno, 8}, {c->d_pushed, 8},
+      {c->d_count, 8 * nt}, {c->d_live, 8 * nt}, {c->d_next_id, 8 * nt}};
+  if (c->v3d) {
+    v.push_back({c->d_M3, 8 * 8 * g});
+    v.push_back({c->d_hist3, 4 * (size_t)c->nhist3_tot});
+  }
+  return v;
+}
+
+extern "C" int kaos_save_state(kaos_ctx* c, const char* path) {
+  if (!c || !path) return KAOS_E_ARG;
+  if (!c->built || c->last_step < 1) return fail(c, KAOS_E_STATE, "kaos_save_state needs a context that has stepped");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(c, KAOS_E_IO, std::string("cannot write ") + path);
+  bool ok = true;
+  auto put = [&](const void* p, size_t n) { if (ok && n) ok = fwrite(p, 1, n, f) == n; };
+  const uint64_t fp = ctx_fingerprint(c);
+  const long long hdr[4] = {c->last_step, c->last_master, c->masters_since_reorder, c->cfg.ntubes};
+  put(KAOS_CKPT_MAGIC, 8);
+  put(&fp, 8);
+  put(hdr, sizeof(hdr));
+  std::vector<char> buf;
+  int rc = KAOS_OK;
+  for (auto& a : ctx_arrays(c)) {
+    buf.resize(a.second);
+    if (hipMemcpy(buf.data(), a.first, a.second, hipMemcpyDeviceToHost) != hipSuccess) { rc = KAOS_E_DEVICE; break; }
+    put(buf.data(), a.second);
+  }
+  for (int t = 0; t < c->cfg.ntubes && !rc; ++t) {
+    const long long n = c->ht[t].count, off = c->dt[t].off;
+    put(&n, 8);
+    for (auto& a : slot_arrays(c)) {
+      buf.resize(a.second * (size_t)n);
+      if (n && hipMemcpy(buf.data(), (char*)a.first + a.second * off, a.second * n, hipMemcpyDeviceToHost) !=
+                   hipSuccess) { rc = KAOS_E_DEVICE; break; }
+      put(buf.data(), a.second * (size_t)n);
+    }
+  }
+  ok = (fclose(f) == 0) && ok;
+  if (rc) return fail(c, rc, "kaos_save_state: device copy failed");
+  if (!ok) return fail(c, KAOS_E_IO, std::string("short write to ") + path);
+  return KAOS_OK;
+}
+
+extern "C" int kaos_load_state(kaos_ctx* c, const char* path) {
+  if (!c || !path) return KAOS_E_ARG;
+  if (c->last_step != 0) return fail(c, KAOS_E_STATE, "kaos_load_state needs a context that has not stepped");
+  if (!c->built) { int rc = build_device(c); if (rc) return rc; }
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(c, KAOS_E_IO, std::string("cannot read ") + path);
+  bool ok = true;
+  auto get = [&](void* p, size_t n) { if (ok && n) ok = fread(p, 1, n, f) == n; };
+  char magic[8] = {};
+  uint64_t fp = 0;
+  long long hdr[4] = {};
+  get(magic, 8); get(&fp, 8); get(hdr, sizeof(hdr));
+  if (!ok || memcmp(magic, KAOS_CKPT_MAGIC, 8) != 0) { fclose(f); return fail(c, KAOS_E_ARG, "not a kaos checkpoint"); }
+  if (fp != ctx_fingerprint(c) || hdr[3] != c->cfg.ntubes) {
+    fclose(f);
+    return fail(c, KAOS_E_ARG, "checkpoint of another configuration (kaos_cfg or a tube's grid differs)");
+  }
+  std::vector<char> buf;
+  int rc = KAOS_OK;
+  for (auto& a : ctx_arrays(c)) {
+    buf.resize(a.second);
+    get(buf.data(), a.second);
+    if (!ok) break;
+    if (hipMemcpy(a.first, buf.data(), a.second, hipMemcpyHostToDevice) != hipSuccess) { rc = KAOS_E_DEVICE; break; }
+  }
+  std::vector<long long> cnt(c->cfg.ntubes, 0);
+  for (int t = 0; t < c->cfg.ntubes && ok && !rc; ++t) {
+    long long n = 0;
+    get(&n, 8);
+    if (!ok) break;
+    if (n < 0 || n > c->ht[t].cap) { fclose(f); return fail(c, KAOS_E_CAPACITY, "checkpoint holds more slots than the tube capacity"); }
+    cnt[t] = n;
+    const long long off = c->dt[t].off;
+    for (auto& a : slot_arrays(c)) {
+      buf.resize(a.second * (size_t)n);
+      get(buf.data(), a.second * (size_t)n);
+      if (!ok) break;
+      if (n && hipMemcpy((char*)a.first + a.second * off, buf.data(), a.second * n, hipMemcpyHostToDevice) !=
+                   hipSuccess) { rc = KAOS_E_DEVICE; break; }
+    }
+  }
+  fclose(f);
+  if (rc) return fail(c, rc, "kaos_load_state: device copy failed");
+  if (!ok) return fail(c, KAOS_E_IO, std::string("truncated checkpoint ") + path);
+  for (int t = 0; t < c->cfg.ntubes; ++t) { c->ht[t].count = cnt[t]; c->ht[t].uploaded = true; }
+  c->last_step = hdr[0];
+  c->last_master = (int)hdr[1];
+  c->masters_since_reorder = (int)hdr[2];
   return KAOS_OK;
 }
 

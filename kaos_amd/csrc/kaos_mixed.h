@@ -506,7 +506,9 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
         const DevTube& TI = a.tubes[cur];
         const float v1 = kx_at(a.vrf + toff, b4), v2 = kx_at(a.vrf2 + toff, b4);
         uint32_t key[2] = { (uint32_t)a.seed, (uint32_t)(a.seed >> 32) ^ 0x49435248u };
-        uint32_t ctr[4] = { (uint32_t)kx_at(a.id + toff, b4), (uint32_t)(*a.stepno), (uint32_t)cur, 0x49435230u };
+        const long long idj = kx_at(a.id + toff, 2u * b4);
+        uint32_t ctr[4] = { (uint32_t)idj, (uint32_t)(*a.stepno), (uint32_t)cur,
+                            0x49435230u ^ (uint32_t)((unsigned long long)idj >> 32) };
         uint32_t o4[4];
         kaos_philox4x32(ctr, key, o4);
         const float u0 = (float)(((double)o4[0] + 1.0) * (1.0 / 4294967296.0));

@@ -24,6 +24,11 @@
 !   real64 d3v                                                 (nvpar*nvperp, Vpar fastest)
 !   real64 q0 vpar0 vperp0                                     (nions each)
 !   with EXPORTDIR: real64 thetaGC TsGC densityGC (nq each), phi, ns0
+! Environment: KAOS_HANDOVER_STEP=k hands the run to a fresh context after
+!   step k: the ions through kaos_download_particles/kaos_download_vperp and
+!   kaos_upload_state, the last master step's moments through
+!   kaos_upload_moments, then kaos_resume(k); with KAOS_CHECKPOINT=path the
+!   fresh context instead reads a restart file (kaos_save_state/kaos_load_state).
 ! OUTPUT (stream), one record per master step (and step 1):
 !   int64 n, int64 n_alive, int64 NqReNorm(nq), real64 M(6*nq), real64 EAmag(nq), real64 esc(4)
 !   [, real64 M8(8*nq) with GRID3D]
@@ -58,7 +63,9 @@ program kaos_fdriver
   type(c_ptr) :: ctx
   integer :: u, uo, nq, nvpar, nvperp
   integer(c_int) :: rc
-  integer(c_int64_t) :: n, nsteps
+  integer(c_int64_t) :: n, nsteps, hstep
+  character(len=1024) :: ckpt, envv
+  integer :: elen, est
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -149,6 +156,13 @@ program kaos_fdriver
     sinfo%lb_replenish = 0_c_int64_t; sinfo%ub_replenish = 0_c_int64_t
   end if
 
+  hstep = -1
+  ckpt = ' '
+  call get_environment_variable('KAOS_HANDOVER_STEP', envv, elen, est)
+  if (est == 0 .and. elen > 0) read (envv, *) hstep
+  call get_environment_variable('KAOS_CHECKPOINT', ckpt, elen, est)
+  if (est /= 0) ckpt = ' '
+
   open (newunit=uo, file=trim(fout), access='stream', form='unformatted', status='replace')
   do n = 1, nsteps
     rc = kaos_step(ctx, n, c_null_ptr, 0_c_int64_t, c_loc(so))
@@ -173,12 +187,71 @@ program kaos_fdriver
         end if
       end if
     end if
+    if (n == hstep) call handover()
   end do
   close (uo)
   rc = kaos_finalize(ctx)
   print '(a,i0,a)', 'kaos_fdriver: ', nsteps, ' steps done'
 
 contains
+
+  ! Hand the run to a fresh context after step hstep (see the header).
+  subroutine handover()
+    type(c_ptr) :: ctx2
+    integer(c_int64_t) :: nlive, nget
+    real(c_double), allocatable, target :: px(:, :), pv1(:), pv2(:), Mh(:)
+    integer(c_int64_t), allocatable, target :: pidx(:, :)
+    character(len=1, kind=c_char), allocatable, target :: cpath(:)
+    integer :: k
+    rc = kaos_init(cfg, ctx2)
+    if (rc /= KAOS_OK) then
+      print *, 'ERROR: kaos_init (handover) ', rc
+      stop 1
+    end if
+    if (len_trim(ckpt) > 0) then
+      allocate (cpath(len_trim(ckpt) + 1))
+      do k = 1, len_trim(ckpt)
+        cpath(k) = ckpt(k:k)
+      end do
+      cpath(len_trim(ckpt) + 1) = c_null_char
+      rc = kaos_save_state(ctx, c_loc(cpath(1)))
+      call check('kaos_save_state')
+      rc = kaos_finalize(ctx)
+      ctx = ctx2
+      rc = kaos_set_grid(ctx, 0_c_int32_t, g, h64(4))
+      call check('kaos_set_grid (handover)')
+      rc = kaos_load_state(ctx, c_loc(cpath(1)))
+      call check('kaos_load_state')
+      return
+    end if
+    rc = kaos_count(ctx, 0_c_int32_t, nlive)
+    call check('kaos_count')
+    allocate (px(max(nlive, 1_c_int64_t), 10), pv1(max(nlive, 1_c_int64_t)), pv2(max(nlive, 1_c_int64_t)))
+    allocate (pidx(max(nlive, 1_c_int64_t), 3), Mh(6*nq))
+    rc = kaos_download_particles(ctx, 0_c_int32_t, nlive, nget, c_loc(px(1, 1)), c_loc(px(1, 2)), &
+         c_loc(px(1, 3)), c_loc(px(1, 4)), c_loc(px(1, 5)), c_loc(px(1, 6)), c_loc(px(1, 7)), &
+         c_loc(px(1, 8)), c_loc(px(1, 9)), c_loc(px(1, 10)), c_loc(pidx(1, 1)), c_loc(pidx(1, 2)), &
+         c_loc(pidx(1, 3)))
+    call check('kaos_download_particles')
+    rc = kaos_download_vperp(ctx, 0_c_int32_t, nlive, nget, c_loc(pv1(1)), c_loc(pv2(1)))
+    call check('kaos_download_vperp')
+    rc = kaos_download_master(ctx, 0_c_int32_t, c_null_ptr, c_null_ptr, c_null_ptr, c_loc(Mh(1)), &
+                              c_null_ptr, c_null_ptr, c_null_ptr)
+    call check('kaos_download_master (handover)')
+    rc = kaos_finalize(ctx)
+    ctx = ctx2
+    rc = kaos_set_grid(ctx, 0_c_int32_t, g, h64(4))
+    call check('kaos_set_grid (handover)')
+    rc = kaos_upload_state(ctx, 0_c_int32_t, nget, c_loc(px(1, 1)), c_loc(px(1, 2)), c_loc(px(1, 3)), &
+         c_loc(px(1, 4)), c_loc(px(1, 5)), c_loc(px(1, 6)), c_loc(pv1(1)), c_loc(pv2(1)), c_loc(px(1, 7)), &
+         c_loc(px(1, 9)), c_loc(px(1, 10)), c_loc(px(1, 8)), c_loc(pidx(1, 1)), c_loc(pidx(1, 2)), &
+         c_loc(pidx(1, 3)))
+    call check('kaos_upload_state')
+    rc = kaos_upload_moments(ctx, 0_c_int32_t, Mh)
+    call check('kaos_upload_moments')
+    rc = kaos_resume(ctx, hstep)
+    call check('kaos_resume')
+  end subroutine handover
 
   subroutine check(what)
     character(len=*), intent(in) :: what

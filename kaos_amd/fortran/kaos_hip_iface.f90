@@ -70,7 +70,8 @@ module kaos_hip
   public :: kaos_init, kaos_set_grid, kaos_upload_particles, kaos_upload_vperp, kaos_cell_counts, &
             kaos_seed_particles, kaos_step, kaos_run, &
             kaos_download_master, kaos_download_master3d, kaos_export_master, kaos_export_spinup, &
-            kaos_count, kaos_download_particles, kaos_set_launch, &
+            kaos_count, kaos_download_particles, kaos_download_vperp, kaos_upload_state, &
+            kaos_upload_moments, kaos_resume, kaos_save_state, kaos_load_state, kaos_set_launch, &
             kaos_get_launch, kaos_profile, &
             kaos_profile_read, kaos_last_error, kaos_finalize, kaos_error_message
 
@@ -192,6 +193,53 @@ module kaos_hip
       integer(c_int64_t), intent(out) :: n_out
       type(c_ptr), value :: x, y, z, Vx, Vy, Vz, vperp, qk4, aea, ag, qind, vparind, vperpind
     end function kaos_download_particles
+
+    ! Vperp1, Vperp2 of the live ions, in the order of kaos_download_particles
+    integer(c_int) function kaos_download_vperp(ctx, tube, n_cap, n_out, vperp1, vperp2) &
+        bind(c, name='kaos_download_vperp')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tube
+      integer(c_int64_t), value :: n_cap
+      integer(c_int64_t), intent(out) :: n_out
+      type(c_ptr), value :: vperp1, vperp2
+    end function kaos_download_vperp
+
+    ! The driver's per-particle arrays handed to a tube (integer(kind=dp) indices)
+    integer(c_int) function kaos_upload_state(ctx, tube, n, x, y, z, Vx, Vy, Vz, vperp1, vperp2, vperp, &
+        aea, ag, qk4, qind, vparind, vperpind) bind(c, name='kaos_upload_state')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tube
+      integer(c_int64_t), value :: n
+      type(c_ptr), value :: x, y, z, Vx, Vy, Vz, vperp1, vperp2, vperp, aea, ag, qk4, qind, vparind, vperpind
+    end function kaos_upload_state
+
+    ! M*phRT(nn, :) of the last master step (6*nq), read by the next master step's fields
+    integer(c_int) function kaos_upload_moments(ctx, tube, M) bind(c, name='kaos_upload_moments')
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tube
+      real(c_double), intent(in) :: M(*)
+    end function kaos_upload_moments
+
+    ! continue another run after its step n_last
+    integer(c_int) function kaos_resume(ctx, n_last) bind(c, name='kaos_resume')
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n_last
+    end function kaos_resume
+
+    ! restart files (path: NUL-terminated)
+    integer(c_int) function kaos_save_state(ctx, path) bind(c, name='kaos_save_state')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, path
+    end function kaos_save_state
+
+    integer(c_int) function kaos_load_state(ctx, path) bind(c, name='kaos_load_state')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, path
+    end function kaos_load_state
 
     integer(c_int) function kaos_set_launch(ctx, tiles_per_group, hist_words, persistent) &
         bind(c, name='kaos_set_launch')

@@ -199,19 +199,59 @@ class KineticSolver:
         self._chk(self.lib.kaos_export_master(self.h, tube, C.byref(info), C.byref(nn)))
         return nn.value
 
-    def particles(self, tube, cap=None):
+    def particles(self, tube, cap=None, vperp12=False):
+        """Live ions of a tube in the reference's order (kaos_download_particles;
+        with vperp12 also Vperp1, Vperp2 from kaos_download_vperp)."""
         cap = int(cap if cap is not None else self.count(tube))
         d = {n: np.zeros(cap) for n in ("x", "y", "z", "Vx", "Vy", "Vz", "vperp", "qk4", "aea", "ag")}
-        i = {n: np.zeros(cap, np.int32) for n in ("qind", "vparind", "vperpind")}
+        i = {n: np.zeros(cap, np.int64) for n in ("qind", "vparind", "vperpind")}
         n = C.c_int64(0)
         self._chk(self.lib.kaos_download_particles(
             self.h, tube, cap, C.byref(n), *(ptr(d[k]) for k in ("x", "y", "z", "Vx", "Vy", "Vz",
                                                                   "vperp", "qk4", "aea", "ag")),
-            *(ptr(i[k], C.c_int32) for k in ("qind", "vparind", "vperpind"))))
+            *(ptr(i[k], C.c_int64) for k in ("qind", "vparind", "vperpind"))))
         m = n.value
         out = {k: v[:m] for k, v in d.items()}
         out.update({k: v[:m] for k, v in i.items()})
+        if vperp12:
+            v1, v2 = np.zeros(cap), np.zeros(cap)
+            self._chk(self.lib.kaos_download_vperp(self.h, tube, cap, C.byref(n), ptr(v1), ptr(v2)))
+            out["vperp1"], out["vperp2"] = v1[:n.value], v2[:n.value]
         return out
+
+    def upload_state(self, tube, P):
+        """kaos_upload_state from a dict with the keys of particles(vperp12=True)
+        (vperp1/vperp2 optional)."""
+        d = {k: np.ascontiguousarray(P[k], dtype=np.float64)
+             for k in ("x", "y", "z", "Vx", "Vy", "Vz", "vperp", "aea", "ag", "qk4")}
+        v1 = P.get("vperp1")
+        v2 = P.get("vperp2")
+        v1 = None if v1 is None else np.ascontiguousarray(v1, dtype=np.float64)
+        v2 = None if v2 is None else np.ascontiguousarray(v2, dtype=np.float64)
+        ii = {k: np.ascontiguousarray(P[k], dtype=np.int64) for k in ("qind", "vparind", "vperpind")}
+        self._chk(self.lib.kaos_upload_state(
+            self.h, tube, len(d["x"]), *(ptr(d[k]) for k in ("x", "y", "z", "Vx", "Vy", "Vz")), ptr(v1), ptr(v2),
+            *(ptr(d[k]) for k in ("vperp", "aea", "ag", "qk4")),
+            *(ptr(ii[k], C.c_int64) for k in ("qind", "vparind", "vperpind"))))
+
+    def upload_moments(self, tube, M6):
+        """kaos_upload_moments: the 6*nq moments of the last master step (the
+        order of master()'s M0, M1perp, M1par, M2, M2perp, M2par)."""
+        if isinstance(M6, dict):
+            M6 = np.concatenate([M6[k] for k in MOMENT_NAMES])
+        M6 = np.ascontiguousarray(M6, dtype=np.float64)
+        if len(M6) != 6 * self.tubes[tube].nq:
+            raise ValueError("6 moments per config cell")
+        self._chk(self.lib.kaos_upload_moments(self.h, tube, ptr(M6)))
+
+    def resume(self, n_last):
+        self._chk(self.lib.kaos_resume(self.h, int(n_last)))
+
+    def save_state(self, path):
+        self._chk(self.lib.kaos_save_state(self.h, str(path).encode()))
+
+    def load_state(self, path):
+        self._chk(self.lib.kaos_load_state(self.h, str(path).encode()))
 
     def close(self):
         if getattr(self, "h", None):

@@ -683,6 +683,24 @@ def test_philox4x32_known_answers(lib):
         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
 
 
+def test_icr_stream_key_takes_the_whole_64_bit_id(lib):
+    """kaos_icr_uniforms: Philox4x32-10 with key (seed lo, seed hi ^ "ICRH") and
+    counter (id lo, step, tube, "ICR0" ^ id hi): ids that share their low 32
+    bits draw different kicks; ids below 2^32 keep the counter word "ICR0"."""
+    seed, step, tube = 0x1234567890ABCDEF, 77, 3
+    for idv in (5, 5 + 2 ** 33, 2 ** 40 + 123456):
+        u = np.zeros(4)
+        lib.orc_icr_uniforms(seed, step, tube, idv, ptr(u))
+        key = [seed & 0xFFFFFFFF, (seed >> 32) ^ 0x49435248]
+        ctr = [idv & 0xFFFFFFFF, step, tube, 0x49435230 ^ (idv >> 32)]
+        o = _philox4x32(lib, ctr, key)
+        assert list(u) == [(w + 1.0) / 4294967296.0 for w in o]
+    a, b = np.zeros(4), np.zeros(4)
+    lib.orc_icr_uniforms(seed, step, tube, 5, ptr(a))
+    lib.orc_icr_uniforms(seed, step, tube, 5 + 2 ** 32, ptr(b))
+    assert not np.array_equal(a, b)
+
+
 def test_exp_vs_libm(lib):
     for x in np.concatenate([np.linspace(-700.0, 700.0, 4001), np.linspace(-1.0, 1.0, 2001)]):
         e = math.exp(x)
