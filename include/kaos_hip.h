@@ -184,6 +184,19 @@ int kaos_seed_particles(kaos_ctx* ctx, int32_t tube, const int64_t* counts, cons
 int kaos_step(kaos_ctx* ctx, int64_t n, const double* uniforms, int64_t n_uniforms,
               kaos_step_out* out);
 
+/* kaos_step for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc); it has
+ * no counterpart in the reference.  Same arguments and results as kaos_step,
+ * and it calls no hipMalloc/hipFree or other allocating routine: the caller's
+ * uniforms are staged in a fixed device array of the library (at most
+ * 6 * 65536 doubles: up to 65536 injected ions per step, else KAOS_E_ARG),
+ * the step is enqueued without graph replay, and the autotuner and kaos_profile
+ * timing skip it (the current launch shape is used).  It returns after the step
+ * has finished.  The staging array is one per GPU, shared by every context on
+ * it: calls that overlap on one GPU race. */
+int kaos_step_noalloc(kaos_ctx* ctx, int64_t n, const double* uniforms, int64_t n_uniforms,
+                      kaos_step_out* out);
+
 /* Steps n_first .. n_first+nsteps-1 with the seeded stream and no host
  * synchronisation between steps; returns after the last one finishes.
  * particle_steps (may be NULL) receives the ions pushed summed over steps. */

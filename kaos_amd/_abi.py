@@ -68,7 +68,7 @@ class KaosStepOut(C.Structure):
 
 
 EXPORTS = ("kaos_init", "kaos_set_grid", "kaos_upload_particles", "kaos_upload_vperp", "kaos_cell_counts",
-           "kaos_seed_particles", "kaos_step", "kaos_run",
+           "kaos_seed_particles", "kaos_step", "kaos_step_noalloc", "kaos_run",
            "kaos_download_master", "kaos_download_master3d", "kaos_export_master", "kaos_export_spinup",
            "kaos_count", "kaos_download_particles", "kaos_download_vperp", "kaos_upload_state",
            "kaos_upload_moments", "kaos_resume", "kaos_save_state", "kaos_load_state", "kaos_set_launch",
@@ -95,6 +95,7 @@ def load(path=LIB_PATH):
     lib.kaos_cell_counts.argtypes = [C.c_int32, _dp, _dp, C.c_double, i64p]
     lib.kaos_seed_particles.argtypes = [P, C.c_int32, i64p, _dp, C.c_uint64]
     lib.kaos_step.argtypes = [P, C.c_int64, _dp, C.c_int64, C.POINTER(KaosStepOut)]
+    lib.kaos_step_noalloc.argtypes = [P, C.c_int64, _dp, C.c_int64, C.POINTER(KaosStepOut)]
     lib.kaos_run.argtypes = [P, C.c_int64, C.c_int64, i64p]
     lib.kaos_download_master.argtypes = [P, C.c_int32, i64p, i64p, _dp, _dp, _dp, _dp, _dp]
     lib.kaos_export_master.argtypes = [P, C.c_int32, C.POINTER(KaosExportInfo), i64p]

@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <mutex>
 
 #include "kaos_dipole.h"
 #include "../../include/kaos_hip.h"
@@ -1504,9 +1505,36 @@ struct kaos_ctx {
   int bin3_words = 0;                           // LDS window of k_bin3d (32-bit words)
   int nb3 = 0;                                  // binning blocks: groups of <= KAOS_BIN3_TILES tiles of one tube
   int2* d_grp3 = nullptr;
+  // host mirrors sized at build time (the no-allocation step reads through them)
+  std::vector<long long> cnt_h, live_h;
+  bool in_noalloc = false;                      // inside kaos_step_noalloc: no tuning, no events
 };
 
 static std::string g_init_err;
+
+// Scratch of kaos_step_noalloc: the caller's injection uniforms staged in a
+// file-scope device array instead of the context's lazily allocated buffer.
+// Every GPU holds its own copy for the life of the process; host code uses
+// only the per-device address from hipGetSymbolAddress (the array's name in
+// host code is not a device address).  All calls on one GPU share it, so
+// kaos_step_noalloc calls that overlap on one GPU race.
+#define KAOS_NOALLOC_UNIFORMS (6 * 65536)      // doubles: 6 per injected ion, 65536 ions per step (3 MiB)
+#define KAOS_MAX_DEVICES 64
+static __device__ __align__(256) double scratch[KAOS_NOALLOC_UNIFORMS];
+static double* g_scratch[KAOS_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the current device (once per device).
+static hipError_t scratch_setup(int dev) {
+  if (dev < 0 || dev >= KAOS_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[dev]) return hipSuccess;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch[dev] = (double*)p;
+  return hipSuccess;
+}
 
 #define HIPCHK(ctx, call)                                                          \
   do {                                                                             \
@@ -1535,6 +1563,10 @@ extern "C" int kaos_init(const kaos_cfg* cfg, kaos_ctx** out) {
     return KAOS_E_DEVICE;
   }
   if (hipSetDevice(cfg->device) != hipSuccess) { g_init_err = "hipSetDevice failed"; return KAOS_E_DEVICE; }
+  if (scratch_setup(cfg->device) != hipSuccess) {
+    g_init_err = "hipGetSymbolAddress of the no-allocation scratch failed";
+    return KAOS_E_DEVICE;
+  }
   kaos_ctx* c = new kaos_ctx();
   c->cfg = *cfg;
   int ncu = 0;
@@ -1888,6 +1920,8 @@ static int build_device(kaos_ctx* c) {
   }
   c->inj_total = ib[nt];
   c->inj_base_h = ib;
+  c->cnt_h.assign(nt, 0);
+  c->live_h.assign(nt, 0);
   HIPCHK(c, up((void**)&c->d_inj_base, ib.data(), sizeof(long long) * (nt + 1)));
   HIPCHK(c, up((void**)&c->d_inj_cnt, ic.data(), sizeof(long long) * nt));
   HIPCHK(c, up((void**)&c->d_uni_base, ib.data(), sizeof(long long) * (nt + 1)));
@@ -2099,7 +2133,8 @@ static void launch_push(kaos_ctx* c, const StepArgs& a, hipEvent_t tune0 = nullp
     nblocks = std::min(nblocks, per_cu * c->n_cu);
   }
   std::pair<hipEvent_t, hipEvent_t> ev;
-  if (c->prof_on) ev = ev_pair(c);
+  const bool prof = c->prof_on && !c->in_noalloc;
+  if (prof) ev = ev_pair(c);
   if (tune0) (void)hipEventRecord(tune0, c->stream);
   if (c->cfg.precision == KAOS_MIXED) {
     // the heating instantiation only when a tube heats (its extra registers
@@ -2122,7 +2157,7 @@ static void launch_push(kaos_ctx* c, const StepArgs& a, hipEvent_t tune0 = nullp
   else
     hipLaunchKernelGGL(k_push_fp64<KAOS_MAXQ>, dim3(nblocks), dim3(KAOS_FP64_BLOCK), push_lds_bytes(c), c->stream, a);
   if (tune1) (void)hipEventRecord(tune1, c->stream);
-  if (c->prof_on) { (void)hipEventRecord(ev.second, c->stream); c->ev_push.push_back(ev); }
+  if (prof) { (void)hipEventRecord(ev.second, c->stream); c->ev_push.push_back(ev); }
 }
 
 // Online autotuner: after two warm-up master steps, the next master steps
@@ -2189,10 +2224,9 @@ static int enqueue_reorder(kaos_ctx* c) {
   HIPCHK(c, hipGetLastError());
   c->cur ^= 1;
   c->masters_since_reorder = 0;
-  std::vector<long long> cnt(nt);
-  HIPCHK(c, hipMemcpyAsync(cnt.data(), c->d_count, sizeof(long long) * nt, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->cnt_h.data(), c->d_count, sizeof(long long) * nt, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  for (int t = 0; t < nt; ++t) c->ht[t].count = cnt[t];
+  for (int t = 0; t < nt; ++t) c->ht[t].count = c->cnt_h[t];
   return KAOS_OK;
 }
 
@@ -2246,7 +2280,7 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
       zero_hist();
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c->tuning && master) {
+    if (c->tuning && master && !c->in_noalloc) {
       int rc = tune_before_push(c, &e0, &e1);
       if (rc) return rc;
     }
@@ -2365,12 +2399,54 @@ extern "C" int kaos_step(kaos_ctx* c, int64_t n, const double* uniforms, int64_t
     memset(out, 0, sizeof(*out));
     out->master = c->last_master;
     out->n_injected = is_master(c, n) ? c->inj_total : 0;
-    std::vector<long long> live(c->cfg.ntubes);
-    HIPCHK(c, hipMemcpyAsync(live.data(), c->d_live, sizeof(long long) * live.size(), hipMemcpyDeviceToHost,
-                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->live_h.data(), c->d_live, sizeof(long long) * c->live_h.size(),
+                             hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     int64_t alive = 0;
-    for (long long k : live) alive += k;
+    for (long long k : c->live_h) alive += k;
+    out->n_alive = alive;
+    rc = read_status(c, &out->status);
+    if (rc) return rc;
+    if (out->status) return fail(c, out->status, "NaN in the particle state");
+  }
+  return KAOS_OK;
+}
+
+// kaos_step without allocations: the caller's uniforms go to the device's
+// `scratch` array, the step is enqueued eagerly (no graph to instantiate), the
+// autotuner and kernel timing do not run in it (the launch shape stays), and
+// host mirrors sized when the context was built stand in for temporaries.
+extern "C" int kaos_step_noalloc(kaos_ctx* c, int64_t n, const double* uniforms, int64_t n_uniforms,
+                                 kaos_step_out* out) {
+  if (!c) return KAOS_E_ARG;
+  int rc = check_next(c, n);
+  if (rc) return rc;
+  const double* d_uni = nullptr;
+  if (uniforms && is_master(c, n) && c->inj_total > 0) {
+    const long long need = 6 * c->inj_total;
+    if (need > KAOS_NOALLOC_UNIFORMS)
+      return fail(c, KAOS_E_ARG, "kaos_step_noalloc: more injected ions per step than its scratch holds "
+                                 "(65536); use kaos_step");
+    if (n_uniforms < need) return fail(c, KAOS_E_ARG, "too few uniforms for this step's injection");
+    double* sp = g_scratch[c->cfg.device];
+    if (!sp) return fail(c, KAOS_E_STATE, "no-allocation scratch not set up on this device");
+    HIPCHK(c, hipMemcpyAsync(sp, uniforms, sizeof(double) * need, hipMemcpyHostToDevice, c->stream));
+    d_uni = sp;
+  }
+  c->in_noalloc = true;
+  rc = enqueue_step(c, n, d_uni);
+  c->in_noalloc = false;
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (out) {
+    memset(out, 0, sizeof(*out));
+    out->master = c->last_master;
+    out->n_injected = is_master(c, n) ? c->inj_total : 0;
+    HIPCHK(c, hipMemcpyAsync(c->live_h.data(), c->d_live, sizeof(long long) * c->live_h.size(),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int64_t alive = 0;
+    for (long long k : c->live_h) alive += k;
     out->n_alive = alive;
     rc = read_status(c, &out->status);
     if (rc) return rc;

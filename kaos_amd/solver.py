@@ -104,11 +104,13 @@ class KineticSolver:
             raise ValueError("counts and temperature need one entry per config cell")
         self._chk(self.lib.kaos_seed_particles(self.h, tube, ptr(counts, C.c_int64), ptr(temperature), seed))
 
-    def step(self, n, uniforms=None, wait=True):
+    def step(self, n, uniforms=None, wait=True, noalloc=False):
+        """One time step (kaos_step; kaos_step_noalloc with noalloc=True)."""
         out = KaosStepOut() if wait else None
         u = None if uniforms is None else np.ascontiguousarray(uniforms, dtype=np.float64)
         nu = 0 if u is None else len(u)
-        self._chk(self.lib.kaos_step(self.h, n, ptr(u), nu, C.byref(out) if out is not None else None))
+        fn = self.lib.kaos_step_noalloc if noalloc else self.lib.kaos_step
+        self._chk(fn(self.h, n, ptr(u), nu, C.byref(out) if out is not None else None))
         return out
 
     def run(self, n_first, nsteps):
