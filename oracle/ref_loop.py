@@ -1,0 +1,152 @@
+"""TEST INFRASTRUCTURE ONLY -- run the reference's own step loop.
+
+oracle/_ref/kaos_ref_loop is built by oracle/ref/loop/Makefile from the
+reference's Fortran sources (every subroutine of the set-up and of
+KineticSolverSub, unmodified) with the harness's ModelInput, GaussianRNG and
+main program.  This module runs it under MPI, reads back the inputs the
+reference generated (grid, initial ions of every rank, the logged uniform
+draws) and the files its DataExport2Sub wrote at every master step, so that
+the CPU oracle (and the GPU library) can be run on the same inputs and their
+outputs compared with the reference's.
+"""
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EXE = os.path.join(HERE, "_ref", "kaos_ref_loop")
+REF_DIR = "/root/reference"
+MPIEXEC = "/opt/conda/bin/mpiexec"
+MPI_LIB = "/opt/conda/lib"
+
+
+def available():
+    return os.path.exists(EXE) and os.path.exists(MPIEXEC)
+
+
+def build():
+    """Build the harness when the reference sources are present (not on the GPU box)."""
+    if not os.path.isdir(REF_DIR):
+        return None
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "ref", "loop")], check=True)
+    return EXE
+
+
+def run(workdir, nt, nsnormfac, ranks=2, seed=1, timeout=3600):
+    """Run the reference with `ranks` MPI processes in workdir (created):
+    returns the timing line's fields (seconds of KineticSolverSub, the maximum
+    over ranks; initial and final ions; steps; ndatfac)."""
+    os.makedirs(os.path.join(workdir, "kaos_ref_out"), exist_ok=True)
+    env = dict(os.environ, KAOS_REF_NT=str(int(nt)), KAOS_REF_NSNORMFAC=repr(float(nsnormfac)).replace("e", "d"),
+               KAOS_REF_SEED=str(int(seed)))
+    with open(os.path.join(workdir, "stdout.txt"), "w") as fo:
+        r = subprocess.run([MPIEXEC, "-n", str(int(ranks)), EXE], cwd=workdir, env=env, stdout=fo,
+                           stderr=subprocess.STDOUT, timeout=timeout)
+    txt = open(os.path.join(workdir, "stdout.txt"), errors="replace").read()
+    m = re.search(r"kaos_ref_loop seconds\s+(\S+) ranks (\d+) ions0 (\d+) ions_end (\d+) steps (\d+) ndatfac (\d+)",
+                  txt)
+    if r.returncode != 0 or not m:
+        raise RuntimeError(f"reference loop failed (rc {r.returncode}): {txt[-2000:]}")
+    return dict(seconds=float(m.group(1).replace("D", "E")), ranks=int(m.group(2)), ions0=int(m.group(3)),
+                ions_end=int(m.group(4)), steps=int(m.group(5)), ndatfac=int(m.group(6)))
+
+
+def _nint(x):
+    """Fortran NINT: nearest integer, halves away from zero."""
+    return int(math.floor(abs(x) + 0.5)) * (1 if x >= 0 else -1)
+
+
+def load_inputs(workdir):
+    """The reference-generated inputs as a grids.Tube (3-D velocity grid) and
+    the initial ions of all ranks (rank 0 first), plus the logged uniforms."""
+    from kaos_amd import grids
+    with open(os.path.join(workdir, "kaos_ref_grid.bin"), "rb") as f:
+        hdr = np.frombuffer(f.read(64), dtype="<i8")
+        sc = np.frombuffer(f.read(64), dtype="<f8")
+        nq, nvpar, nv1, nv2, ndatfac, nnt, nt, ranks = (int(v) for v in hdr)
+        cfg = np.frombuffer(f.read(8 * 11 * nq), dtype="<f8").reshape(11, nq)
+        g1 = np.frombuffer(f.read(8 * 4 * nv1), dtype="<f8").reshape(4, nv1)
+        g2 = np.frombuffer(f.read(8 * 4 * nv2), dtype="<f8").reshape(4, nv2)
+        gp = np.frombuffer(f.read(8 * 4 * nvpar), dtype="<f8").reshape(4, nvpar)
+        d3v3 = np.frombuffer(f.read(8 * nv1 * nv2 * nvpar), dtype="<f8").reshape(nv1, nv2, nvpar)
+    h, L, phi, ms, qs, nsnormfac, Ts_lb, nsnorm_clb = (float(v) for v in sc)
+    qGL, qGH, qGC, hqC, dqC, d3xC, rGC, thGC, ellGC, Te, Ts = (np.ascontiguousarray(r) for r in cfg)
+    n_inj_rank = _nint(nsnorm_clb / ranks)           # nint(nsnormCLBGT/ranksize), BC:747-750
+    # a 2-D (Vperp, Vpar) grid is part of every tube; on the 3-D grid only the
+    # 3-D cells are counted, so it is a plain uniform grid over |Vperp1|
+    vmax = float(max(abs(g1[0][0]), abs(g1[1][-1])))
+    vg = grids.velocity_grid(nvpar, nv1, vpar_max=1.0, vperp_max=vmax)
+    arrays = dict(qGL=qGL, qGH=qGH, qGC=qGC, hqC=hqC, dqC=dqC, d3xC=d3xC, rGC=rGC, cthGC=np.cos(thGC),
+                  ellGC=ellGC, Te=Te)
+    arrays.update(vparGL=gp[0], vparGH=gp[1], vparGC=gp[2], dvpar=gp[3],
+                  vperpGL=vg["vperpGL"], vperpGH=vg["vperpGH"], vperpGC=vg["vperpGC"], dvperp=vg["dvperp"])
+    arrays["d3v"] = np.ascontiguousarray(
+        ((np.abs(vg["vperpGC"])[:, None] * vg["dvperp"][:, None]) * gp[3][None, :]) * (2.0 * math.pi))
+    arrays.update(v1GL=g1[0], v1GH=g1[1], v1GC=g1[2], dv1=g1[3], v2GL=g2[0], v2GH=g2[1], v2GC=g2[2], dv2=g2[3],
+                  d3v3=np.ascontiguousarray(d3v3))
+    t = grids.Tube(nq=nq, nvpar=nvpar, nvperp=nv1, p=L, phi=phi, h=h, ms=ms, qs=qs, nsnormfac=nsnormfac,
+                   Ts_lb=Ts_lb, Ts_ub=float(Ts[-1]), n_inj_lb=n_inj_rank * ranks, n_inj_ub=0,
+                   arrays={k: np.ascontiguousarray(v, dtype=np.float64) for k, v in arrays.items()})
+    t.ubcond = 0                                      # UBCONDITIONflag = 0 in the harness input
+    t.v3d, t.nv1, t.nv2 = 1, nv1, nv2
+    ions = {k: [] for k in ("q0", "vpar0", "v1", "v2", "vperp", "x", "y", "z", "Vx", "Vy", "Vz")}
+    counts, uni = [], []
+    for r in range(ranks):
+        with open(os.path.join(workdir, f"kaos_ref_ions_{r}.bin"), "rb") as f:
+            n = int(np.frombuffer(f.read(8), dtype="<i8")[0])
+            a = np.frombuffer(f.read(8 * 11 * n), dtype="<f8").reshape(11, n)
+        counts.append(n)
+        for k, row in zip(ions, a):
+            ions[k].append(row)
+        uni.append(np.fromfile(os.path.join(workdir, f"kaos_ref_uniforms_{r}.bin"), dtype="<f8"))
+    ions = {k: np.concatenate(v) for k, v in ions.items()}
+    return dict(tube=t, ions=ions, counts=counts, uniforms=uni, ranks=ranks, ndatfac=ndatfac, nt=nt, nnt=nnt,
+                n_inj_rank=n_inj_rank, Ts=Ts, theta=thGC)
+
+
+def injection_uniforms(inp, master_index):
+    """The 6 uniforms per injected ion of the k-th master step (k = 1, 2, ...),
+    in the oracle's order: rank 0's ions, then rank 1's, ..., each ion's
+    three Gaussians (Vx, Vy, Vz; _6_1_MBVelocityDistribution.f90:34-47) from two
+    uniforms each.  The first 6 N0 values of a rank's log are its initial ions'
+    draws (VelocityDistributionSub)."""
+    out = []
+    m = inp["n_inj_rank"]
+    for r, u in enumerate(inp["uniforms"]):
+        start = 6 * inp["counts"][r] + 6 * m * (master_index - 1)
+        if start + 6 * m > len(u):
+            raise ValueError("the uniform log of rank %d is shorter than master step %d needs" % (r, master_index))
+        out.append(u[start:start + 6 * m])
+    return np.concatenate(out)
+
+
+def read_master(workdir, nn, tube, s=1, f=1):
+    """The reference's DataExport2Sub files of master step nn (3-D branch):
+    n3 counts recovered from N2PerpphRT (= count nsnormfac/d3x), the eight
+    moments, the field and the escape outputs."""
+    from oracle import export
+    d = os.path.join(workdir, "kaos_ref_out")
+    rd = lambda name, dt="<f8": np.fromfile(os.path.join(d, export.file_name(nn, s, f, name)), dtype=dt)
+    out = {}
+    for key, name in (("M0", "M0phRT"), ("M1perp1", "M1Perp1phRT"), ("M1perp2", "M1Perp2phRT"),
+                      ("M1par", "M1ParphRT"), ("M2", "M2phRT"), ("M2perp1", "M2Perp1phRT"),
+                      ("M2perp2", "M2Perp2phRT"), ("M2par", "M2ParphRT"), ("EA", "EAmagRT"), ("EG", "EGmagRT"),
+                      ("M0F", "M0FiltAvrgRT"), ("M1F", "M1ParFiltAvrgRT")):
+        out[key] = rd(name)
+    for key, name in (("lb_out", "LBoutfluxIonRT"), ("ub_out", "UBoutfluxIonRT"), ("nlb_out", "NqLBoutfluxIonRT"),
+                      ("nub_out", "NqUBoutfluxIonRT")):
+        p = os.path.join(d, export.file_name(nn, s, f, name))
+        if os.path.exists(p):
+            out[key] = np.fromfile(p, dtype="<f8")
+    a = tube.arrays
+    n3 = np.zeros((tube.nq, tube.nv1, tube.nv2, tube.nvpar))
+    for Q in range(tube.nq):
+        v = np.fromfile(os.path.join(d, export.file_name(nn, s, f, "N2PerpphRT", Q + 1)), dtype="<f8")
+        v = v.reshape(tube.nvpar, tube.nv2, tube.nv1).transpose(2, 1, 0)    # Fortran (V1, V2, Vpar)
+        n3[Q] = v * (a["d3xC"][Q] / tube.nsnormfac)
+    out["n3_real"] = n3
+    out["n3_counts"] = np.rint(n3).astype(np.int64)
+    return out
