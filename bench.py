@@ -26,8 +26,11 @@ shape it ran with.
 Prints one JSON line.  value = ion pushes (live ions summed over the timed
 steps, counted on the device) / wall time of the timed steps, inputs resident
 in HBM.  roofline: the push kernel's algorithmic HBM bytes per launch over its
-mean duration (hipEvents on the library stream).  cpu_baseline: the oracle
-(single-threaded C restatement) on a bounded sample of the same workload.
+mean duration (hipEvents on the library stream).  cpu_baseline: the
+reference's own Fortran/MPI program on the host's cores (kind "reference",
+a config-1-sized tube at the reference's minimum master cadence), with the
+oracle (single-threaded C restatement on a bounded sample of the GPU
+workload) as cpu_baseline_port.
 """
 import argparse
 import json
@@ -81,6 +84,41 @@ def cpu_baseline(n_ions=2_000_000, steps=20):
     return dict(value=ps / dt, unit="particle-steps/s", cores=1, kind="port",
                 sample=f"oracle/kaos_oracle.c (FP64), same tube and grids, {len(q0)} ions x {steps} "
                        f"master steps, {dt:.1f} s")
+
+
+def cpu_reference_loop(nt=30, nsnormfac=7.7e16):
+    """The reference's own program (oracle/_ref/kaos_ref_loop: its set-up and
+    KineticSolverSub compiled from its Fortran sources, flang -O2, MPICH) on the
+    host's cores, one MPI rank per core (16 at most, the box's CPU share; >= 2
+    as _9_MPIReduceSum.f90:30-32 needs): one O+ tube on L = 4, 64 config
+    cells, 12 x 12 x 12 (Vperp1, Vperp2, Vpar) cells, about 1e5 ions, nt steps
+    with a master step (counts, distribution, moments, fields, boundary) every
+    third step, the reference's minimum cadence.  Particle-steps counted as the
+    mean of the initial and final ion counts times the steps; time is
+    MPI_Wtime around KineticSolverSub (maximum over ranks)."""
+    import shutil
+    import tempfile
+    import oracle.ref_loop as rl
+    if not rl.available():
+        return None
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 2
+    ranks = max(2, min(16, cores))
+    d = tempfile.mkdtemp(prefix="kaos_ref_loop_")
+    try:
+        r = rl.run(d, nt, nsnormfac, ranks=ranks, seed=1, timeout=600)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    ps = 0.5 * (r["ions0"] + r["ions_end"]) * r["steps"]
+    log(f"[bench] reference loop: {ranks} ranks, {r['ions0']} -> {r['ions_end']} ions, {r['steps']} steps in "
+        f"{r['seconds']:.1f} s")
+    return dict(value=ps / r["seconds"], unit="particle-steps/s", cores=ranks, kind="reference",
+                sample=f"oracle/_ref/kaos_ref_loop (the reference's set-up and KineticSolverSub, flang -O2, "
+                       f"MPICH, {ranks} ranks): one O+ tube, L=4, 64 cells, 12x12x12 (Vperp1,Vperp2,Vpar), "
+                       f"{r['ions0']} -> {r['ions_end']} ions, {r['steps']} steps, master every "
+                       f"{r['ndatfac']} steps, {r['seconds']:.1f} s in KineticSolverSub")
 
 
 def cpu_reference_push(n_ions=2_000_000):
@@ -513,8 +551,18 @@ def main():
     if args.vperp3d or not args.no_vperp3d:
         out["vperp3d"] = vperp3d_run(args)
     if not args.no_cpu_baseline:
-        log("[bench] cpu baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline()
+        log("[bench] cpu baseline (the reference's own loop, then the oracle) ...")
+        refl = None
+        try:
+            refl = cpu_reference_loop()
+        except Exception as e:                    # reported, never fatal for the GPU numbers
+            log(f"[bench] reference loop not run: {e}")
+        port = cpu_baseline()
+        if refl:
+            out["cpu_baseline"] = refl
+            out["cpu_baseline_port"] = port
+        else:
+            out["cpu_baseline"] = port
         ref = cpu_reference_push()
         if ref:
             out["cpu_reference_push"] = ref
