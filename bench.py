@@ -47,9 +47,9 @@ sys.path.insert(0, ROOT)
 # is a master step (interpolation, counting and the stage-4 q every step):
 #   FP64:  read cell 4 + rs, vp, qk 3x8            = 28
 #          write rs, vp, qk, ae, ag 5x8 + cell 4    = 44
-#   mixed: read cell, rdi, vpi, qkf 4x4             = 16
-#          write rdi, vpi, qkf, aef, agf, cell 6x4  = 24
-BYTES_PER_ION = {"fp64": 28 + 44, "mixed": 16 + 24}
+#   mixed: read cell, rdi, vpi, lo, qkf 5x4             = 20
+#          write rdi, vpi, lo, qkf, aef, agf, cell 7x4  = 28
+BYTES_PER_ION = {"fp64": 28 + 44, "mixed": 20 + 28}
 # Measured HBM bytes per ion and step of the same launches: rocprofv3
 # FETCH_SIZE and WRITE_SIZE of the push kernel at 1e8 ions, each scaled by the
 # ratio of a plain 1 GiB device copy's true bytes to its counter reading
@@ -536,9 +536,9 @@ def main():
         ks.close()
     if "mixed" in runs and "fp64" in runs and not args.no_error_leg:
         # the six moments, then the ambipolar field apart: the field is a stencil
-        # derivative of the filtered density, so one ion binned in a neighbouring
-        # cell (an ion within ~1e-4 m of a cell edge after 200 steps of binary32
-        # stages) moves it by ~1e-6
+        # derivative of the filtered density.  One ion binned in a neighbouring
+        # cell (its binary32 stage arithmetic left it on the other side of an
+        # edge) moves M1par of a dense cell by ~1e-6 relative L2 on this tube
         out["mixed_error"] = mixed_error_run(tube, q0, v0, w0, cap, max(ERROR_STEPS_MIN, out["steps_run"]))
         out["mixed"]["max_l2_rel_moment_error"] = out["mixed_error"]["max_l2_rel_moment_error"]
         out["mixed"]["max_l2_rel_field_error_EA"] = out["mixed_error"]["max_l2_rel_field_error_EA"]

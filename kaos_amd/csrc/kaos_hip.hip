@@ -80,8 +80,9 @@ struct StepArgs {
   // particles
   double *rs, *vp, *vr, *qk, *ae, *ag;       // binary64 form (vr: Vperp1)
   double *vr2;                               // binary64 Vperp2
-  int *rdi, *vpi;                            // mixed form: fixed-point radius / V|| offsets
-  float *vrf, *qkf, *aef, *agf;              // mixed form: Vperp1, stage-4 q offset, fields
+  int *rdi, *vpi;                            // mixed form: fixed-point radius / V|| offsets (bits 16..47)
+  uint32_t* lo;                              // mixed form: bits 0..15 of both offsets (radius | V|| << 16)
+  float *vrf, *qkf, *aef, *agf;              // mixed form: Vperp1, stage-4 q offset (from kx_qref), fields
   float *vrf2;                               // mixed form: Vperp2
   const double *rGH, *rref;                  // |r| of the qGH edges; signed r of the cell centres
   const int *rk, *vk;                        // fixed-point exponents per config / Vpar cell
@@ -556,13 +557,13 @@ static_assert(KAOS_BIN3_TILES * KAOS_TILE < 65536, "16-bit window counters");
 // The state words one ion of the binning pass reads (either form).
 template <int MIXED> struct Bin3Ion;
 template <> struct Bin3Ion<0> { uint32_t cw; double rs, vs, v1, v2; };
-template <> struct Bin3Ion<1> { uint32_t cw; int rdi, vpi; float v1, v2; uint32_t c12; };
+template <> struct Bin3Ion<1> { uint32_t cw; int rdi, vpi; float v1, v2; uint32_t c12, lo; };
 
 __device__ __forceinline__ void bin3_load(const StepArgs& a, long long j, Bin3Ion<0>& x) {
   x.cw = a.cell[j]; x.rs = a.rs[j]; x.vs = a.vp[j]; x.v1 = a.vr[j]; x.v2 = a.vr2[j];
 }
 __device__ __forceinline__ void bin3_load(const StepArgs& a, long long j, Bin3Ion<1>& x) {
-  x.cw = a.cell[j]; x.rdi = a.rdi[j]; x.vpi = a.vpi[j]; x.v1 = a.vrf[j]; x.v2 = a.vrf2[j];
+  x.cw = a.cell[j]; x.rdi = a.rdi[j]; x.vpi = a.vpi[j]; x.v1 = a.vrf[j]; x.v2 = a.vrf2[j]; x.lo = a.lo[j];
   x.c12 = a.c12[j];
 }
 __device__ __forceinline__ bool bin3_dead(const Bin3Ion<0>& x) { return x.rs == 0.0; }
@@ -650,8 +651,8 @@ __global__ void __launch_bounds__(KAOS_BIN3_BLOCK) k_bin3d(StepArgs a, int first
         double rs, vs;
         if constexpr (MIXED) {
           const int Q = kx_refq(qc, nq);
-          rs = s_rref[Q - 1] + kx_dec(x.rdi, s_rk[Q - 1], (cw & KAOS_RFLAG) != 0);
-          vs = kx_dec_v(x.vpi, (int)(cw & 0xFFu), (cw & KAOS_VFLAG) != 0, s_vGH, s_vk, nvpar);
+          rs = s_rref[Q - 1] + kx_dec(x.rdi, x.lo, s_rk[Q - 1], (cw & KAOS_RFLAG) != 0);
+          vs = kx_dec_v(x.vpi, x.lo >> 16, (int)(cw & 0xFFu), (cw & KAOS_VFLAG) != 0, s_vGH, s_vk, nvpar);
         } else {
           rs = x.rs;
           vs = x.vs;
@@ -733,12 +734,14 @@ __device__ __forceinline__ void kd_store_initial(const DevTube& T, const StepArg
   if (mixed) {
     int Q = kx_refq(qc, T.nq);
     bool rflag, vflag;
-    a.rdi[j] = kx_enc(rs - rref[Q - 1], a.rk[T.goff + Q - 1], rflag);
-    a.vpi[j] = kx_enc_v(vpar, ip, vparGH, a.vk + T.voff, T.nvpar, vflag);
+    uint32_t rlo, vlo;
+    a.rdi[j] = kx_enc(rs - rref[Q - 1], a.rk[T.goff + Q - 1], rflag, rlo);
+    a.vpi[j] = kx_enc_v(vpar, ip, vparGH, a.vk + T.voff, T.nvpar, vflag, vlo);
+    a.lo[j] = rlo | (vlo << 16);
     a.vrf[j] = (float)v1;
     a.vrf2[j] = (float)v2;
     if (a.c12) a.c12[j] = kd_cells12(T, a, v1, v2);
-    a.qkf[j] = (float)q;
+    a.qkf[j] = (float)(q - kx_qref(a.qGC[T.goff + Q - 1]));
     a.aef[j] = 0.0f; a.agf[j] = 0.0f;
     a.cell[j] = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip |
                 (rflag ? KAOS_RFLAG : 0u) | (vflag ? KAOS_VFLAG : 0u);
@@ -798,14 +801,16 @@ __global__ void k_upload_state(DevTube T, long long n, UpState u, long long id0,
   if (mixed) {
     const int Q = kx_refq(qc, T.nq);
     bool rflag, vflag;
-    a.rdi[j] = kx_enc(rs - rref[Q - 1], a.rk[T.goff + Q - 1], rflag);
+    uint32_t rlo, vlo;
+    a.rdi[j] = kx_enc(rs - rref[Q - 1], a.rk[T.goff + Q - 1], rflag, rlo);
     int ip = kaos_find_vcell(vpar, vparGH, vparGH[T.nvpar], T.nvpar);
-    a.vpi[j] = kx_enc_v(vpar, ip, vparGH, a.vk + T.voff, T.nvpar, vflag);
+    a.vpi[j] = kx_enc_v(vpar, ip, vparGH, a.vk + T.voff, T.nvpar, vflag, vlo);
+    a.lo[j] = rlo | (vlo << 16);
     if (ip < 1) ip = (int)u.vparind[i];
     a.vrf[j] = (float)v1;
     a.vrf2[j] = (float)v2;
     if (a.c12) a.c12[j] = kd_cells12(T, a, v1, v2);
-    a.qkf[j] = (float)u.qk4[i];
+    a.qkf[j] = (float)(u.qk4[i] - kx_qref(a.qGC[T.goff + Q - 1]));
     a.aef[j] = (float)u.aea[i]; a.agf[j] = (float)u.ag[i];
     a.cell[j] = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip |
                 (rflag ? KAOS_RFLAG : 0u) | (vflag ? KAOS_VFLAG : 0u);
@@ -935,12 +940,14 @@ __global__ void k_inject(StepArgs a, const long long* inj_base, const long long*
   if (mixed) {
     int Q = kx_refq(qcell, T.nq);
     bool rflag, vflag;
-    a.rdi[j] = kx_enc(r - a.rref[T.goff + Q - 1], a.rk[T.goff + Q - 1], rflag);
-    a.vpi[j] = kx_enc_v(vpar, ip, a.vparGH + T.voff, a.vk + T.voff, T.nvpar, vflag);
+    uint32_t rlo, vlo;
+    a.rdi[j] = kx_enc(r - a.rref[T.goff + Q - 1], a.rk[T.goff + Q - 1], rflag, rlo);
+    a.vpi[j] = kx_enc_v(vpar, ip, a.vparGH + T.voff, a.vk + T.voff, T.nvpar, vflag, vlo);
+    a.lo[j] = rlo | (vlo << 16);
     a.vrf[j] = (float)v1;
     a.vrf2[j] = (float)v2;
     if (a.c12) a.c12[j] = kd_cells12(T, a, v1, v2);
-    a.qkf[j] = (float)qc;
+    a.qkf[j] = (float)(qc - kx_qref(a.qGC[T.goff + Q - 1]));
     a.aef[j] = (float)aea; a.agf[j] = (float)agv;
     flags = (rflag ? KAOS_RFLAG : 0u) | (vflag ? KAOS_VFLAG : 0u);
   } else {
@@ -1372,7 +1379,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_reorder_scatter(StepArgs a, Step
     long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
     long long d = (long long)(s_base[key[it]] + atomicAdd(&s_cnt[key[it]], 1u));
     if (mixed) {
-      b.rdi[d] = a.rdi[j]; b.vpi[d] = a.vpi[j]; b.vrf[d] = a.vrf[j]; b.vrf2[d] = a.vrf2[j];
+      b.rdi[d] = a.rdi[j]; b.vpi[d] = a.vpi[j]; b.vrf[d] = a.vrf[j]; b.vrf2[d] = a.vrf2[j]; b.lo[d] = a.lo[j];
       b.qkf[d] = a.qkf[j]; b.aef[d] = a.aef[j]; b.agf[d] = a.agf[j];
     } else {
       b.rs[d] = a.rs[j]; b.vp[d] = a.vp[j]; b.vr[d] = a.vr[j]; b.vr2[d] = a.vr2[j];
@@ -1399,8 +1406,9 @@ __global__ void k_reconstruct(DevTube T, long long n, StepArgs a, int mixed, dou
     uint32_t cw = a.cell[j];
     if (cw != KAOS_DEAD) {
       int Q = kx_refq((int)((cw >> 16) & KAOS_QMASK) - 1, T.nq);
-      r = a.rref[T.goff + Q - 1] + kx_dec(a.rdi[j], a.rk[T.goff + Q - 1], (cw & KAOS_RFLAG) != 0);
-      v = kx_dec_v(a.vpi[j], (int)(cw & 0xFFu), (cw & KAOS_VFLAG) != 0, a.vparGH + T.voff, a.vk + T.voff,
+      r = a.rref[T.goff + Q - 1] + kx_dec(a.rdi[j], a.lo[j], a.rk[T.goff + Q - 1], (cw & KAOS_RFLAG) != 0);
+      v = kx_dec_v(a.vpi[j], a.lo[j] >> 16, (int)(cw & 0xFFu), (cw & KAOS_VFLAG) != 0, a.vparGH + T.voff,
+                   a.vk + T.voff,
                    T.nvpar);
     }
   } else {
@@ -1430,6 +1438,7 @@ struct StateBufs {
   double *rs = nullptr, *vp = nullptr, *vr = nullptr, *qk = nullptr, *ae = nullptr, *ag = nullptr;
   double* vr2 = nullptr;
   int *rdi = nullptr, *vpi = nullptr;
+  uint32_t* lo = nullptr;
   float *vrf = nullptr, *qkf = nullptr, *aef = nullptr, *agf = nullptr;
   float* vrf2 = nullptr;
   uint32_t* cell = nullptr;
@@ -1807,6 +1816,7 @@ static int build_device(kaos_ctx* c) {
     if (mixed) {
       HIPCHK(c, hipMalloc((void**)&B.rdi, sizeof(int) * nslot));
       HIPCHK(c, hipMalloc((void**)&B.vpi, sizeof(int) * nslot));
+      HIPCHK(c, hipMalloc((void**)&B.lo, sizeof(uint32_t) * nslot));
       HIPCHK(c, hipMalloc((void**)&B.vrf, sizeof(float) * nslot));
       HIPCHK(c, hipMalloc((void**)&B.vrf2, sizeof(float) * nslot));
       HIPCHK(c, hipMalloc((void**)&B.qkf, sizeof(float) * nslot));
@@ -1931,7 +1941,7 @@ static int build_device(kaos_ctx* c) {
 
 static void set_state(StepArgs& a, const StateBufs& B) {
   a.rs = B.rs; a.vp = B.vp; a.vr = B.vr; a.qk = B.qk; a.ae = B.ae; a.ag = B.ag; a.vr2 = B.vr2;
-  a.rdi = B.rdi; a.vpi = B.vpi; a.vrf = B.vrf; a.qkf = B.qkf; a.aef = B.aef; a.agf = B.agf; a.vrf2 = B.vrf2;
+  a.rdi = B.rdi; a.vpi = B.vpi; a.lo = B.lo; a.vrf = B.vrf; a.qkf = B.qkf; a.aef = B.aef; a.agf = B.agf; a.vrf2 = B.vrf2;
   a.cell = B.cell; a.id = B.id; a.c12 = B.c12;
 }
 
@@ -2778,7 +2788,11 @@ static int fetch_tube(kaos_ctx* c, int32_t tube, TubeDump& d) {
       HIPCHK(c, hipMemcpy(f.data(), src[i] + T.off, sizeof(float) * n, hipMemcpyDeviceToHost));
       for (long long j = 0; j < n; ++j) (*dst[i])[j] = (double)f[j];
     }
-    for (long long j = 0; j < n; ++j) live[j] = d.cell[j] != KAOS_DEAD;
+    const double* qGC = c->ht[tube].qGC.data();
+    for (long long j = 0; j < n; ++j) {
+      live[j] = d.cell[j] != KAOS_DEAD;
+      if (live[j]) d.qk[j] += kx_qref(qGC[kx_refq((int)((d.cell[j] >> 16) & KAOS_QMASK) - 1, T.nq) - 1]);
+    }
   } else {
     std::vector<double> rs(n);
     HIPCHK(c, hipMemcpy(rs.data(), B.rs + T.off, sizeof(double) * n, hipMemcpyDeviceToHost));
@@ -2945,7 +2959,7 @@ extern "C" int kaos_resume(kaos_ctx* c, int64_t n_last) {
 // bookkeeping and the slots [0, count) of every state array of the live
 // buffer, as stored (so the continuation is bit for bit the uninterrupted run
 // in either precision form).
-static const char KAOS_CKPT_MAGIC[8] = {'K', 'A', 'O', 'S', 'C', 'K', 'P', '1'};
+static const char KAOS_CKPT_MAGIC[8] = {'K', 'A', 'O', 'S', 'C', 'K', 'P', '2'};
 
 static uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
   const unsigned char* b = (const unsigned char*)p;
@@ -2982,7 +2996,7 @@ static std::vector<std::pair<void*, size_t>> slot_arrays(kaos_ctx* c) {
   StateBufs& B = c->st[c->cur];
   std::vector<std::pair<void*, size_t>> v;
   if (c->cfg.precision == KAOS_MIXED) {
-    v = {{B.rdi, 4}, {B.vpi, 4}, {B.vrf, 4}, {B.vrf2, 4}, {B.qkf, 4}, {B.aef, 4}, {B.agf, 4}};
+    v = {{B.rdi, 4}, {B.vpi, 4}, {B.lo, 4}, {B.vrf, 4}, {B.vrf2, 4}, {B.qkf, 4}, {B.aef, 4}, {B.agf, 4}};
     if (B.c12) v.push_back({B.c12, 2});
   } else {
     v = {{B.rs, 8}, {B.vp, 8}, {B.vr, 8}, {B.vr2, 8}, {B.qk, 8}, {B.ae, 8}, {B.ag, 8}};
@@ -3167,7 +3181,7 @@ extern "C" int kaos_finalize(kaos_ctx* c) {
                 c->d_grp3};
   for (void* p : b3) if (p) (void)hipFree(p);
   for (StateBufs& B : c->st) {
-    void* sb[] = {B.rs, B.vp, B.vr, B.qk, B.ae, B.ag, B.rdi, B.vpi, B.vrf, B.qkf, B.aef, B.agf, B.cell, B.id,
+    void* sb[] = {B.rs, B.vp, B.vr, B.qk, B.ae, B.ag, B.rdi, B.vpi, B.lo, B.vrf, B.qkf, B.aef, B.agf, B.cell, B.id,
                   B.vr2, B.vrf2, B.c12};
     for (void* p : sb) if (p) (void)hipFree(p);
   }

@@ -2,15 +2,21 @@
 // kaos_hip.hip after DevTube and StepArgs).
 //
 // State per ion, one 32-bit word each:
-//   rdi  radius: fixed-point offset from rref (binary64 radius of the centre of
-//        the cell-word cell), in units of 2^-k metres, k per cell so that two
-//        cell widths fit in 31 bits (resolution <= width / 2^30); cell-word
-//        flag KAOS_RFLAG: the word holds the binary32 offset instead (ions far
-//        outside the domain).
-//   vpi  V||: fixed-point offset from the centre of the velocity cell in the
-//        cell word (resolution <= dVpar / 2^30); flag KAOS_VFLAG (ions off the
-//        velocity grid): the word holds the binary32 offset of V|| from the
-//        nearer grid edge (positive above the top edge, negative below GL0).
+//   rdi  radius: bits 16..47 of a 48-bit fixed-point offset from rref (binary64
+//        radius of the centre of the cell-word cell), in units of 2^-(k+16)
+//        metres, k per cell so that two cell widths fit in 47 bits
+//        (resolution <= width / 2^46); cell-word flag KAOS_RFLAG: the word
+//        holds the binary32 offset instead (ions far outside the domain).
+//   vpi  V||: bits 16..47 of a 48-bit fixed-point offset from the centre of
+//        the velocity cell in the cell word (resolution <= dVpar / 2^46);
+//        flag KAOS_VFLAG (ions off the velocity grid): the word holds the
+//        binary32 offset of V|| from the nearer grid edge (positive above the
+//        top edge, negative below GL0).
+//   lo   bits 0..15 of both offsets: radius | V|| << 16.  With 32-bit offsets
+//        alone the radius is rounded by ~width/2^31 every step, which walks an
+//        ion ~1e-4 m from its binary64 trajectory in 200 steps (enough to bin
+//        a few ions in 1e8 in a neighbouring cell); the 48-bit offsets leave
+//        the binary32 stage arithmetic as the only rounding.
 //   qkf  stage-4 q of the last push before a master step (binary32; it only
 //        sets the field interpolation weight, so its 1e-7 relative rounding
 //        moves the interpolated field by ~1e-7 of the cell-to-cell change).
@@ -59,15 +65,20 @@ __device__ __forceinline__ double kx_rsq(double x) {
   return y * (1.5 - 0.5 * (x * (y * y)));
 }
 
-// 32-bit fixed-point offset with scale 2^k; binary32 fallback when it does not fit.
-__device__ __forceinline__ int kx_enc(double off, int k, bool& fallback) {
-  double x = ldexp(off, k);
-  if (!(fabs(x) < 2147483000.0)) { fallback = true; return __float_as_int((float)off); }
+// 48-bit fixed-point offset with scale 2^(k+16): the high 32 bits as the
+// returned word, the low 16 bits in lo16; binary32 fallback in the word (and
+// lo16 = 0) when it does not fit.
+__device__ __forceinline__ int kx_enc(double off, int k, bool& fallback, uint32_t& lo16) {
+  const double x = ldexp(off, k + 16);
+  if (!(fabs(x) < 140737488000000.0)) { fallback = true; lo16 = 0u; return __float_as_int((float)off); }
   fallback = false;
-  return (int)rint(x);
+  const long long xi = (long long)rint(x);
+  lo16 = (uint32_t)(xi & 0xFFFFll);
+  return (int)(xi >> 16);
 }
-__device__ __forceinline__ double kx_dec(int w, int k, bool fallback) {
-  return fallback ? (double)__int_as_float(w) : ldexp((double)w, -k);
+__device__ __forceinline__ double kx_dec(int w, uint32_t lo16, int k, bool fallback) {
+  return fallback ? (double)__int_as_float(w)
+                  : ldexp((double)((long long)w * 65536ll + (long long)(lo16 & 0xFFFFu)), -(k + 16));
 }
 
 // Element at byte offset `boff` of a per-tube base pointer: the base is uniform
@@ -77,7 +88,18 @@ template <typename T>
 __device__ __forceinline__ T& kx_at(T* base, uint32_t boff) { return *(T*)((char*)base + boff); }
 
 // Reference cell (1-based) of a cell-word config index qc: 0 -> 1, -1 -> nq.
-__device__ __forceinline__ int kx_refq(int qc, int nq) { return (qc >= 1) ? qc : (qc == 0 ? 1 : nq); }
+KAOS_HD int kx_refq(int qc, int nq) { return (qc >= 1) ? qc : (qc == 0 ? 1 : nq); }
+
+// Reference of the stored stage-4 q of an ion whose cell word names cell Q:
+// the centre qGC of Q split into two binary32 parts, hi + lo (the push's LDS
+// table holds the same two parts).  The state word holds the binary32 offset
+// q - (hi + lo), so q is kept to ~1e-7 of its distance from the centre
+// rather than of |q| (the field interpolation weight of a binary32 q would
+// be off by ~1e-7 |q| / (cell spacing), ~1e-5 on a 256-cell line).
+KAOS_HD double kx_qref(double qgc) {
+  const float hi = (float)qgc;
+  return (double)hi + (double)(float)(qgc - (double)hi);
+}
 
 // Parallel acceleration (signs as kaos_apar; qle0 = stage q <= 0).
 __device__ __forceinline__ float kf_apar(bool qle0, float AG, float AEA, int grav, int eamb,
@@ -117,30 +139,41 @@ __device__ __forceinline__ int kf_find_rcell_from(double r, const double* rGH, d
   return i + 1;
 }
 
-// Field interpolation at q in binary32 (as kaos_interp2_from), on the cell
-// centres packed as (qGC, EA, EG, 0).
-__device__ __forceinline__ int kf_interp2_from(float q, const float4* cE, int nq, float qGL0, float qGHlast,
+// q - qX for the centre row X = (hi, EA, EG, lo) of an ion stored as the
+// offset `off` from row R: the hi parts are exact binary32 numbers, so each
+// difference is rounded relative to itself, not to |q|.
+__device__ __forceinline__ float kf_dq(const float4& R, float off, const float4& X) {
+  KAOS_MIXED_CONTRACT
+  return (R.x - X.x) + ((R.w - X.w) + off);
+}
+
+// Field interpolation at q = (R.hi + R.lo) + off, R = cE[hint - 1] (as
+// kaos_interp2_from: the grid-edge test on q in binary64, the walk and the
+// weight on the differences above).  Cell centres packed as (hi, EA, EG, lo).
+__device__ __forceinline__ int kf_interp2_from(float off, const float4* cE, int nq, double qGL0, double qGHlast,
                                                int desc, int hint, float& out, float& out2) {
   KAOS_MIXED_CONTRACT
+  const float4 R = cE[hint - 1];
+  const double q = ((double)R.x + (double)R.w) + (double)off;
   int ub = nq - 1;
   int lo = hint - 1;
   if (lo < 1) lo = 1;
   if (lo > ub) lo = ub;
   if (desc) {
     if (!(q <= qGL0) || !(q >= qGHlast)) return 0;
-    if (q >= cE[0].x) { out = cE[0].y; out2 = cE[0].z; return 1; }
-    if (q <= cE[ub].x) { out = cE[ub].y; out2 = cE[ub].z; return 1; }
-    while (lo > 1 && q >= cE[lo - 1].x) --lo;
-    while (!(q >= cE[lo].x)) ++lo;
+    if (kf_dq(R, off, cE[0]) >= 0.0f) { out = cE[0].y; out2 = cE[0].z; return 1; }
+    if (kf_dq(R, off, cE[ub]) <= 0.0f) { out = cE[ub].y; out2 = cE[ub].z; return 1; }
+    while (lo > 1 && kf_dq(R, off, cE[lo - 1]) >= 0.0f) --lo;
+    while (!(kf_dq(R, off, cE[lo]) >= 0.0f)) ++lo;
   } else {
     if (!(q >= qGL0) || !(q <= qGHlast)) return 0;
-    if (q <= cE[0].x) { out = cE[0].y; out2 = cE[0].z; return 1; }
-    if (q >= cE[ub].x) { out = cE[ub].y; out2 = cE[ub].z; return 1; }
-    while (lo > 1 && q <= cE[lo - 1].x) --lo;
-    while (!(q <= cE[lo].x)) ++lo;
+    if (kf_dq(R, off, cE[0]) <= 0.0f) { out = cE[0].y; out2 = cE[0].z; return 1; }
+    if (kf_dq(R, off, cE[ub]) >= 0.0f) { out = cE[ub].y; out2 = cE[ub].z; return 1; }
+    while (lo > 1 && kf_dq(R, off, cE[lo - 1]) <= 0.0f) --lo;
+    while (!(kf_dq(R, off, cE[lo]) <= 0.0f)) ++lo;
   }
   const float4 A = cE[lo - 1], B = cE[lo];
-  float f = (q - A.x) * kf_rcp(B.x - A.x);
+  const float f = kf_dq(R, off, A) * kf_rcp((B.x - A.x) + (B.w - A.w));
   out = A.y + f * (B.y - A.y);
   out2 = A.z + f * (B.z - A.z);
   return 1;
@@ -151,22 +184,24 @@ __device__ __forceinline__ int kf_interp2_from(float q, const float4* cE, int nq
 // are read together, without a loop; a q elsewhere (a longer walk, beyond the
 // first or last centre, off the grid, NaN) takes the walk above.  Both give
 // the same interval, so the same value.  Needs nq >= 2.
-__device__ __forceinline__ int kf_interp2_fast(float q, const float4* cE, int nq, float qGL0, float qGHlast,
+__device__ __forceinline__ int kf_interp2_fast(float off, const float4* cE, int nq, double qGL0, double qGHlast,
                                                int desc, int hint, float& out, float& out2) {
   KAOS_MIXED_CONTRACT
   const int ub = nq - 1;
   const int lo = max(1, min(hint - 1, ub));
+  const float4 R = cE[hint - 1];
   const float4 L = cE[lo - 1], M = cE[lo], H = cE[min(lo + 1, ub)];
-  const bool up = desc ? !(q >= M.x) : !(q <= M.x);
-  const bool ok = up ? ((desc ? (q >= H.x) : (q <= H.x)) && lo < ub) : (desc ? (q < L.x) : (q > L.x));
+  const float dL = kf_dq(R, off, L), dM = kf_dq(R, off, M), dH = kf_dq(R, off, H);
+  const bool up = desc ? !(dM >= 0.0f) : !(dM <= 0.0f);
+  const bool ok = up ? ((desc ? (dH >= 0.0f) : (dH <= 0.0f)) && lo < ub) : (desc ? (dL < 0.0f) : (dL > 0.0f));
   if (ok) {
     const float4 A = up ? M : L, B = up ? H : M;
-    const float f = (q - A.x) * kf_rcp(B.x - A.x);
+    const float f = (up ? dM : dL) * kf_rcp((B.x - A.x) + (B.w - A.w));
     out = A.y + f * (B.y - A.y);
     out2 = A.z + f * (B.z - A.z);
     return 1;
   }
-  return kf_interp2_from(q, cE, nq, qGL0, qGHlast, desc, hint, out, out2);
+  return kf_interp2_from(off, cE, nq, qGL0, qGHlast, desc, hint, out, out2);
 }
 
 __device__ __forceinline__ float kf_aea_of(float e_interp, float qsms, int eambsign) {
@@ -182,23 +217,25 @@ __device__ __forceinline__ float kf_ag_of(float e_interp, float qsms) {
 
 // Encode the velocity state v (binary64) given the new Vparind ipn (0 = off grid).
 __device__ __forceinline__ int kx_enc_v(double v, int ipn, const double* vGH, const int* vk, int nv,
-                                        bool& vflag) {
+                                        bool& vflag, uint32_t& lo16) {
   if (ipn >= 1) {
     double lo = (ipn == 1) ? vGH[nv] : vGH[ipn - 2];
     double ref = 0.5 * (lo + vGH[ipn - 1]);
-    return kx_enc(v - ref, vk[ipn - 1], vflag);
+    return kx_enc(v - ref, vk[ipn - 1], vflag, lo16);
   }
   vflag = true;
+  lo16 = 0u;
   return __float_as_int((float)(v - ((v > vGH[nv - 1]) ? vGH[nv - 1] : vGH[nv])));
 }
-__device__ __forceinline__ double kx_dec_v(int w, int ip, bool vflag, const double* vGH, const int* vk, int nv) {
+__device__ __forceinline__ double kx_dec_v(int w, uint32_t lo16, int ip, bool vflag, const double* vGH, const int* vk,
+                                           int nv) {
   if (vflag) {
     double off = (double)__int_as_float(w);
     return ((off > 0.0) ? vGH[nv - 1] : vGH[nv]) + off;
   }
   double lo = (ip == 1) ? vGH[nv] : vGH[ip - 2];
   double ref = 0.5 * (lo + vGH[ip - 1]);
-  return ref + ldexp((double)w, -vk[ip - 1]);
+  return ref + kx_dec(w, lo16, vk[ip - 1], false);
 }
 
 // Stochastic ICR kick of one ion in binary32 (the mixed form's heating; the
@@ -241,13 +278,14 @@ __device__ __forceinline__ void kf_icr_kick(const kaos_icr_t& P, float r, float 
 // master step, pre-push binning, stale 2-D velocity index, stage-4 gravity
 // gate, field interpolation at the previous stage-4 q on master steps).
 // Static LDS (grids of up to MAXQ cells, so the arrays sit at fixed offsets):
-// binary64 rGH, rref, vparGH + GL0; binary32 qGC, EA, EG; int rk, vk.
+// binary64 rGH, rref, vparGH + GL0; qGC as two binary32 parts (kx_qref),
+// binary32 EA, EG; int rk, vk.
 // Dynamic LDS: the histogram window (a.hwords words).  Uniform values are
-// kept few (flag words, 32-bit offsets, h-multiples derived per ion) so the
-// loop does not spill scalar registers.
-// (launch bounds: minimum resident wavefronts per SIMD, which caps the VGPRs)
+// kept few (flag words, 32-bit offsets, h-multiples derived per ion).
+// (launch bounds: minimum resident wavefronts per SIMD, which caps the VGPRs;
+// 4 leaves the per-ion loop without VGPR spills)
 #ifndef KAOS_MIXED_WAVES_EU
-#define KAOS_MIXED_WAVES_EU 5
+#define KAOS_MIXED_WAVES_EU 4
 #endif
 template <int MAXQ, bool ICR>
 __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_mixed(StepArgs a) {
@@ -269,12 +307,14 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
   uint32_t tcount = 0;
   uint32_t* cellT = nullptr;
   int *rdiT = nullptr, *vpiT = nullptr;
+  uint32_t* loT = nullptr;
   float *qkfT = nullptr, *aefT = nullptr, *agfT = nullptr;
   int nq = 0, nvpar = 0, nvperp = 0;
   int tflags = 0;                      // desc | grav << 1 | eamb << 2 | eambsign << 3 | (rs < 0) << 4
   double rGL0 = 0.0, rGHlast = 0.0;
-  float qGL0 = 0.0f, qGHlast = 0.0f, qsms = 0.0f, vscale = 0.0f;
-  double ip = 0.0, hd = 0.0;
+  double qGL0 = 0.0, qGHlast = 0.0;
+  float qsms = 0.0f, vscale = 0.0f;
+  double ip = 0.0, ip2 = 0.0, hd = 0.0;
   uint32_t tbox = 0u;
   int icr = 0;                         // ICR heating in this tube (ICR instantiation only)
   for (int pass = 0;; ++pass) {
@@ -301,7 +341,7 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
   // software pipeline: the state words of this thread's next ion (same tube)
   // are loaded while the current ion is pushed
   bool pf = false;
-  uint32_t pf_j = 0u, pf_c = 0u;
+  uint32_t pf_j = 0u, pf_c = 0u, pf_l = 0u;
   int pf_r = 0, pf_v = 0;
   float pf_q = 0.0f;
   for (int tile = t0; tile < t1; ++tile) {
@@ -314,20 +354,25 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
       tflags = (T.desc ? 1 : 0) | (T.grav ? 2 : 0) | (T.eamb ? 4 : 0) | (T.eambsign ? 8 : 0) |
                (T.rsgn < 0.0 ? 16 : 0);
       goff = (int)T.goff; hoff = (int)T.hoff;
-      qGL0 = (float)T.qGL0; qGHlast = (float)T.qGHlast; rGL0 = T.rGL0; rGHlast = T.rGHlast;
+      qGL0 = T.qGL0; qGHlast = T.qGHlast; rGL0 = T.rGL0; rGHlast = T.rGHlast;
       qsms = (float)T.qsms;
       hd = T.h;
       ip = 1.0 / T.p;
+      ip2 = ip * ip;
       for (int i = tid; i < nq; i += BLK) {
         s_rGH[i] = a.rGH[goff + i]; s_rref[i] = a.rref[goff + i];
-        s_cE[i] = make_float4((float)a.qGC[goff + i], (float)a.EA[goff + i], (float)a.EG[goff + i], 0.0f);
+      {
+        const double qg = a.qGC[goff + i];
+        const float qh = (float)qg;
+        s_cE[i] = make_float4(qh, (float)a.EA[goff + i], (float)a.EG[goff + i], (float)(qg - (double)qh));
+      }
         s_rk[i] = a.rk[goff + i];
       }
       for (int i = tid; i <= nvpar; i += BLK) s_vGH[i] = a.vparGH[T.voff + i];
       for (int i = tid; i < nvpar; i += BLK) s_vk[i] = a.vk[T.voff + i];
       toff = T.off;
       tcount = (uint32_t)a.count[tb];
-      cellT = a.cell + toff; rdiT = a.rdi + toff; vpiT = a.vpi + toff;
+      cellT = a.cell + toff; rdiT = a.rdi + toff; vpiT = a.vpi + toff; loT = a.lo + toff;
       qkfT = a.qkf + toff; aefT = a.aef + toff; agfT = a.agf + toff;
       vscale = (float)((double)nvpar / (a.vparGH[T.voff + nvpar - 1] - a.vparGH[T.voff + nvpar]));
       tbox = kaos_pack_box(a.vbox[tb]);
@@ -341,11 +386,14 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
       const uint32_t jl = start + (uint32_t)(it * BLK + tid);
       if (jl >= tcount) break;
       const uint32_t b4 = jl * 4u;
-      uint32_t cold;
+      uint32_t cold, wl;
       int wr, wv;
       float wq;
-      if (pf && pf_j == jl) { cold = pf_c; wr = pf_r; wv = pf_v; wq = pf_q; }
-      else { cold = kx_at(cellT, b4); wr = kx_at(rdiT, b4); wv = kx_at(vpiT, b4); wq = kx_at(qkfT, b4); }
+      if (pf && pf_j == jl) { cold = pf_c; wr = pf_r; wv = pf_v; wq = pf_q; wl = pf_l; }
+      else {
+        cold = kx_at(cellT, b4); wr = kx_at(rdiT, b4); wv = kx_at(vpiT, b4); wq = kx_at(qkfT, b4);
+        wl = kx_at(loT, b4);
+      }
       {
         uint32_t jn = 0u;
         bool nxt = false;
@@ -358,6 +406,7 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
         if (nxt) {
           const uint32_t bn = jn * 4u;
           pf_j = jn; pf_c = kx_at(cellT, bn); pf_r = kx_at(rdiT, bn); pf_v = kx_at(vpiT, bn); pf_q = kx_at(qkfT, bn);
+          pf_l = kx_at(loT, bn);
         }
       }
       if (cold == KAOS_DEAD) continue;
@@ -375,11 +424,12 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
       const double rlo_h = (hint > 1) ? s_rGH[hint - 2] : rGL0, rhi_h = s_rGH[hint - 1];
       const int ipv = max(ipc, 1);
       const bool rfl = (cold & KAOS_RFLAG) != 0, vfl = (cold & KAOS_VFLAG) != 0;
-      const double rs = s_rref[hint - 1] + (rfl ? (double)__int_as_float(wr) : ldexp((double)wr, -s_rk[hint - 1]));
+      const double rs = s_rref[hint - 1] + kx_dec(wr, wl, s_rk[hint - 1], rfl);
       const double voff = (double)__int_as_float(wv);
       const double vs64 =
           vfl ? (((voff > 0.0) ? s_vGH[nvpar - 1] : s_vGH[nvpar]) + voff)
-              : 0.5 * (((ipv == 1) ? s_vGH[nvpar] : s_vGH[ipv - 2]) + s_vGH[ipv - 1]) + ldexp((double)wv, -s_vk[ipv - 1]);
+              : 0.5 * (((ipv == 1) ? s_vGH[nvpar] : s_vGH[ipv - 2]) + s_vGH[ipv - 1]) +
+                    kx_dec(wv, wl >> 16, s_vk[ipv - 1], false);
       float aea, ag;
       const int eambsign = (tflags >> 3) & 1, grav = (tflags >> 1) & 1, eamb = (tflags >> 2) & 1;
       if (sflags & 1) {
@@ -407,7 +457,8 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
         const bool in_hint = ((hint > 1) ? (r > rlo_h) : (r >= rlo_h)) && r <= rhi_h;
         if (other_hemi || !in_hint) qc = other_hemi ? -1 : kf_find_rcell_from(r, s_rGH, rGL0, rGHlast, nq, hint);
       }
-      const double s64 = w * kx_rsq(w);
+      const double rw = kx_rsq(w);
+      const double s64 = w * rw;
       double c64 = omw * kx_rsq(omw);
       if (rs < 0.0) c64 = -c64;
       const double isl64 = kx_rsq(4.0 - 3.0 * w);
@@ -474,10 +525,23 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
       }
       if (rsN != rsN || vN != vN) set_status(a.status, 1u << KAOS_E_NAN);
       // stage-4 q = RE^2 z / r^3 at the full-step predictor
+      // as the offset from the reference of the new cell word's cell:
+      // q_C - q = (RE/r)^2 [b S + c (S - 1)], S = (1 + x)^(-3/2),
+      // x = 2 (s a + c b) + a^2 + b^2, (a, b) = (dR, dZ)_C / r, with
+      // S - 1 = -x (t^2 + t + 1) / ((t + 1) t^3), t = sqrt(1 + x): binary32
+      // terms that are small against q, each to ~1e-7 of itself
       if (sflags & 8) {
-        const float rhoC = rho + dRC;
-        const float u2C = rhoC * rhoC + zC * zC;
-        kx_at(qkfT, b4) = (((float)KAOS_RE2) * zC) * (kf_rsq(u2C) * kf_rcp(u2C));
+        const float ir = kf_rcp((float)r);
+        const float ac = dRC * ir, bc = dZC * ir;
+        const float sf = (float)s64;
+        const float x = 2.0f * (sf * ac + cf * bc) + (ac * ac + bc * bc);
+        const float t = kf_sqrt(1.0f + x);
+        const float Sm1 = -x * ((t * t + t) + 1.0f) * kf_rcp((t + 1.0f) * (t * (t * t)));
+        const float dq = bc + (bc + cf) * Sm1;
+        const double rw2 = rw * rw;
+        const double K = ip2 * (rw2 * rw2);                      // (RE / r)^2
+        const float4 R = s_cE[kx_refq(qc, nq) - 1];
+        kx_at(qkfT, b4) = (float)(fma(K, (double)dq, c64 * K - ((double)R.x + (double)R.w)));
       }
       // next push's Vparind (stale when off the grid) and the new state words
       // (the cell of the stored V|| first, then the general search)
@@ -488,15 +552,17 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
           ipn = kaos_find_vcell_guess(vN, s_vGH, s_vGH[nvpar], nvpar, (double)vscale);
       }
       bool rflag, vflag;
+      uint32_t rlo, vlo = 0u;
       const int Qn = kx_refq(qc, nq);
-      kx_at(rdiT, b4) = kx_enc(rsN - s_rref[Qn - 1], s_rk[Qn - 1], rflag);
+      kx_at(rdiT, b4) = kx_enc(rsN - s_rref[Qn - 1], s_rk[Qn - 1], rflag, rlo);
       if (ipn >= 1) {
         const double vref = 0.5 * (((ipn == 1) ? s_vGH[nvpar] : s_vGH[ipn - 2]) + s_vGH[ipn - 1]);
-        kx_at(vpiT, b4) = kx_enc(vN - vref, s_vk[ipn - 1], vflag);
+        kx_at(vpiT, b4) = kx_enc(vN - vref, s_vk[ipn - 1], vflag, vlo);
       } else {
         vflag = true;
         kx_at(vpiT, b4) = __float_as_int((float)(vN - ((vN > s_vGH[nvpar - 1]) ? s_vGH[nvpar - 1] : s_vGH[nvpar])));
       }
+      kx_at(loT, b4) = rlo | (vlo << 16);
       // ICR heating (WaveHeatingSub at stage A, _8_1_1_KineticUpdateA.f90:1392) of
       // an ion inside the grid: binary32 kick (kf_icr_kick) from the Philox4x32
       // stream; the cell word then carries the Vperp cell of the kicked speed,

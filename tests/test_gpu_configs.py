@@ -146,11 +146,11 @@ def test_config4_bookkeeping_every_tube(config4, form):
 def test_config4_mixed_against_fp64(config4):
     """The mixed form against the FP64 form after the batched workload's 200
     steps, over all 512 tubes: the difference comes from ions binned in a
-    neighbouring cell (an ion within ~1e-4 m of a cell edge after binary32
-    stages and 32-bit state words); every tube whose phase-space counts agree
-    has bit-identical moments, and fewer than one ion in 1e6 is binned
-    elsewhere (measured: 14 of 1.13e8 in 7 of the 512 tubes, whose moments then
-    differ by up to 3e-4 relative L2: tubes of 1e4 ions over 64..256 cells)."""
+    neighbouring cell (an ion whose binary32 stage arithmetic left it on the
+    other side of a cell edge than the binary64 push); every tube whose
+    phase-space counts agree has bit-identical moments, and fewer than one ion
+    in 1e6 is binned elsewhere (one such ion moves the moments of a 1e4-ion
+    tube over 64..256 cells by up to ~3e-4 relative L2)."""
     tubes = config4["tubes"]
     a, b = config4["fp64"][1], config4["mixed"][1]
     ntot = int(config4["sizes"].sum())
