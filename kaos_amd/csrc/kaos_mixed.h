@@ -69,16 +69,17 @@ __device__ __forceinline__ double kx_rsq(double x) {
 // returned word, the low 16 bits in lo16; binary32 fallback in the word (and
 // lo16 = 0) when it does not fit.
 __device__ __forceinline__ int kx_enc(double off, int k, bool& fallback, uint32_t& lo16) {
-  const double x = ldexp(off, k + 16);
-  if (!(fabs(x) < 140737488000000.0)) { fallback = true; lo16 = 0u; return __float_as_int((float)off); }
-  fallback = false;
-  const long long xi = (long long)rint(x);
-  lo16 = (uint32_t)(xi & 0xFFFFll);
-  return (int)(xi >> 16);
+  // x = hi 2^16 + lo exactly (|x| < 2^47): hi = floor(x / 2^16), lo in [0, 2^16)
+  const double x = rint(ldexp(off, k + 16));
+  fallback = !(fabs(x) < 140737488000000.0);
+  const double hi = floor(x * (1.0 / 65536.0));
+  lo16 = fallback ? 0u : (uint32_t)fma(hi, -65536.0, x);
+  return fallback ? __float_as_int((float)off) : (int)hi;
 }
 __device__ __forceinline__ double kx_dec(int w, uint32_t lo16, int k, bool fallback) {
-  return fallback ? (double)__int_as_float(w)
-                  : ldexp((double)((long long)w * 65536ll + (long long)(lo16 & 0xFFFFu)), -(k + 16));
+  const double fx = ldexp(fma((double)w, 65536.0, (double)(lo16 & 0xFFFFu)), -(k + 16));   // exact
+  const double ff = (double)__int_as_float(w);
+  return fallback ? ff : fx;
 }
 
 // Element at byte offset `boff` of a per-tube base pointer: the base is uniform
@@ -179,40 +180,30 @@ __device__ __forceinline__ int kf_interp2_from(float off, const float4* cE, int 
   return 1;
 }
 
-// The same interpolation with the common case first: q between the centre of
-// the hint cell's walk start and one of its neighbours.  The three centres
-// are read together, without a loop; a q elsewhere (a longer walk, beyond the
-// first or last centre, off the grid, NaN) takes the walk above.  Both give
-// the same interval, so the same value.  Needs nq >= 2.
-__device__ __forceinline__ int kf_interp2_fast(float off, const float4* cE, int nq, double qGL0, double qGHlast,
-                                               int desc, int hint, float& out, float& out2) {
-  KAOS_MIXED_CONTRACT
-  const int ub = nq - 1;
-  const int lo = max(1, min(hint - 1, ub));
-  const float4 R = cE[hint - 1];
-  const float4 L = cE[lo - 1], M = cE[lo], H = cE[min(lo + 1, ub)];
-  const float dL = kf_dq(R, off, L), dM = kf_dq(R, off, M), dH = kf_dq(R, off, H);
-  const bool up = desc ? !(dM >= 0.0f) : !(dM <= 0.0f);
-  const bool ok = up ? ((desc ? (dH >= 0.0f) : (dH <= 0.0f)) && lo < ub) : (desc ? (dL < 0.0f) : (dL > 0.0f));
-  if (ok) {
-    const float4 A = up ? M : L, B = up ? H : M;
-    const float f = (up ? dM : dL) * kf_rcp((B.x - A.x) + (B.w - A.w));
-    out = A.y + f * (B.y - A.y);
-    out2 = A.z + f * (B.z - A.z);
-    return 1;
+// Linear pieces of the field along the tube for the common case: piece k
+// (0..nq) runs from cell centre k-1 to centre k (0-based), piece 0 from the
+// lower grid edge to centre 0 and piece nq from the last centre to the upper
+// edge (constant there, as kaos_interp2_from).  In the direction of
+// increasing cell index (sigma = -1 when q decreases with the index) an ion
+// of cell h whose stored offset from centre h-1 is off lies in piece h when
+// sigma off >= 0 (t = sigma off from centre h-1), else in piece h-1
+// (t = sigma off + W from centre h-2); it is in the piece when 0 <= t <= W,
+// W the piece's length, and the fields are (E, G) + t (dE, dG), already
+// multiplied by qs/ms.  Any other ion takes kf_interp2_from.
+__device__ __forceinline__ void kf_piece_set(float4* pe, float* pw, int k, int nq, const double* qGC, const double* EA,
+                                             const double* EG, double qGL0, double qGHlast, int desc, double qsms) {
+  const double sg = desc ? -1.0 : 1.0;
+  const int b = (k > 0) ? k - 1 : 0;
+  double w, dE = 0.0, dG = 0.0;
+  if (k == 0) w = sg * (qGC[0] - qGL0);
+  else if (k == nq) w = sg * (qGHlast - qGC[nq - 1]);
+  else {
+    w = sg * (qGC[k] - qGC[k - 1]);
+    dE = qsms * (EA[k] - EA[k - 1]) / w;
+    dG = qsms * (EG[k] - EG[k - 1]) / w;
   }
-  return kf_interp2_from(off, cE, nq, qGL0, qGHlast, desc, hint, out, out2);
-}
-
-__device__ __forceinline__ float kf_aea_of(float e_interp, float qsms, int eambsign) {
-  float e = eambsign ? e_interp : fabsf(e_interp);
-  if (e == 0.0f) return 0.0f;
-  return eambsign ? qsms * e : fabsf(qsms * e);
-}
-__device__ __forceinline__ float kf_ag_of(float e_interp, float qsms) {
-  float e = fabsf(e_interp);
-  if (e == 0.0f) return 0.0f;
-  return fabsf(qsms * e);
+  pe[k] = make_float4((float)(qsms * EA[b]), (float)(qsms * EG[b]), (float)dE, (float)dG);
+  pw[k] = (float)w;
 }
 
 // Encode the velocity state v (binary64) given the new Vparind ipn (0 = off grid).
@@ -293,6 +284,8 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
   constexpr int BLK = KAOS_MIXED_BLOCK, PPT = KAOS_TILE / KAOS_MIXED_BLOCK;
   __shared__ double s_rGH[MAXQ], s_rref[MAXQ], s_vGH[KAOS_MAXV + 1];
   __shared__ float4 s_cE[MAXQ];
+  __shared__ float4 s_pe[MAXQ + 1];
+  __shared__ float s_pw[MAXQ + 1];
   __shared__ int s_rk[MAXQ], s_vk[KAOS_MAXV + 1];
   extern __shared__ unsigned int s_hist[];
   __shared__ unsigned int s_pushed;
@@ -338,16 +331,9 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
   __syncthreads();
   const int wq0 = s_wq0, wnq = s_wnq, wtube = s_wtube;
   const int4 wb = s_wbox;
-  // software pipeline: the state words of this thread's next ion (same tube)
-  // are loaded while the current ion is pushed
-  bool pf = false;
-  uint32_t pf_j = 0u, pf_c = 0u, pf_l = 0u;
-  int pf_r = 0, pf_v = 0;
-  float pf_q = 0.0f;
   for (int tile = t0; tile < t1; ++tile) {
     int tb = a.tile_tube[tile];
     if (tb != cur) {
-      pf = false;
       __syncthreads();
       const DevTube& T = a.tubes[tb];
       nq = T.nq; nvpar = T.nvpar; nvperp = T.nvperp;
@@ -368,6 +354,8 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
       }
         s_rk[i] = a.rk[goff + i];
       }
+      for (int k = tid; k <= nq; k += BLK)
+        kf_piece_set(s_pe, s_pw, k, nq, a.qGC + goff, a.EA + goff, a.EG + goff, T.qGL0, T.qGHlast, T.desc, T.qsms);
       for (int i = tid; i <= nvpar; i += BLK) s_vGH[i] = a.vparGH[T.voff + i];
       for (int i = tid; i < nvpar; i += BLK) s_vk[i] = a.vk[T.voff + i];
       toff = T.off;
@@ -386,29 +374,9 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
       const uint32_t jl = start + (uint32_t)(it * BLK + tid);
       if (jl >= tcount) break;
       const uint32_t b4 = jl * 4u;
-      uint32_t cold, wl;
-      int wr, wv;
-      float wq;
-      if (pf && pf_j == jl) { cold = pf_c; wr = pf_r; wv = pf_v; wq = pf_q; wl = pf_l; }
-      else {
-        cold = kx_at(cellT, b4); wr = kx_at(rdiT, b4); wv = kx_at(vpiT, b4); wq = kx_at(qkfT, b4);
-        wl = kx_at(loT, b4);
-      }
-      {
-        uint32_t jn = 0u;
-        bool nxt = false;
-        if (it + 1 < PPT) { jn = jl + BLK; nxt = jn < tcount; }
-        else if (tile + 1 < t1 && a.tile_tube[tile + 1] == cur) {
-          jn = (uint32_t)(a.tile_start[tile + 1] - toff) + (uint32_t)tid;
-          nxt = jn < tcount;
-        }
-        pf = nxt;
-        if (nxt) {
-          const uint32_t bn = jn * 4u;
-          pf_j = jn; pf_c = kx_at(cellT, bn); pf_r = kx_at(rdiT, bn); pf_v = kx_at(vpiT, bn); pf_q = kx_at(qkfT, bn);
-          pf_l = kx_at(loT, bn);
-        }
-      }
+      const uint32_t cold = kx_at(cellT, b4), wl = kx_at(loT, b4);
+      const int wr = kx_at(rdiT, b4), wv = kx_at(vpiT, b4);
+      const float wq = kx_at(qkfT, b4);
       if (cold == KAOS_DEAD) continue;
       int qprev = (int)((cold >> 16) & KAOS_QMASK) - 1;
       if ((sflags & 4) && (qprev == 0 || qprev == -1)) {
@@ -433,10 +401,26 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
       float aea, ag;
       const int eambsign = (tflags >> 3) & 1, grav = (tflags >> 1) & 1, eamb = (tflags >> 2) & 1;
       if (sflags & 1) {
-        float ea, eg;
-        if (kf_interp2_fast(wq, s_cE, nq, qGL0, qGHlast, tflags & 1, hint, ea, eg)) {
-          aea = kf_aea_of(ea, qsms, eambsign);
-          ag = kf_ag_of(eg, qsms);
+        // fields at the stored stage-4 q: the hint cell's piece, else the walk
+        const float so = (tflags & 1) ? -wq : wq;
+        const bool up = so >= 0.0f;
+        const int k = up ? hint : hint - 1;
+        const float4 pe = s_pe[k];
+        const float pw = s_pw[k];
+        const float t = up ? so : so + pw;
+        float ea = 0.0f, eg = 0.0f;
+        bool ok = (t >= 0.0f) && (t <= pw);
+        if (ok) {
+          ea = fmaf(t, pe.z, pe.x);
+          eg = fmaf(t, pe.w, pe.y);
+        } else {
+          ok = kf_interp2_from(wq, s_cE, nq, qGL0, qGHlast, tflags & 1, hint, ea, eg);
+          ea *= qsms; eg *= qsms;
+        }
+        if (ok) {
+          // AEAmag = qs/ms E (|.| unless EAMBSIGN), AGmag = |qs/ms G|
+          aea = eambsign ? ea : fabsf(ea);
+          ag = fabsf(eg);
           kx_at(aefT, b4) = aea; kx_at(agfT, b4) = ag;
         } else {
           aea = kx_at(aefT, b4); ag = kx_at(agfT, b4);
@@ -528,15 +512,15 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
       // as the offset from the reference of the new cell word's cell:
       // q_C - q = (RE/r)^2 [b S + c (S - 1)], S = (1 + x)^(-3/2),
       // x = 2 (s a + c b) + a^2 + b^2, (a, b) = (dR, dZ)_C / r, with
-      // S - 1 = -x (t^2 + t + 1) / ((t + 1) t^3), t = sqrt(1 + x): binary32
+      // S - 1 = -3/2 x + 15/8 x^2 - 35/16 x^3 (the next term, 2.46 x^4, is
+      // below 1e-8 of S - 1 for |x| < 2e-3, a step of v dt < 6 km): binary32
       // terms that are small against q, each to ~1e-7 of itself
       if (sflags & 8) {
         const float ir = kf_rcp((float)r);
         const float ac = dRC * ir, bc = dZC * ir;
         const float sf = (float)s64;
         const float x = 2.0f * (sf * ac + cf * bc) + (ac * ac + bc * bc);
-        const float t = kf_sqrt(1.0f + x);
-        const float Sm1 = -x * ((t * t + t) + 1.0f) * kf_rcp((t + 1.0f) * (t * (t * t)));
+        const float Sm1 = x * fmaf(x, fmaf(x, -2.1875f, 1.875f), -1.5f);
         const float dq = bc + (bc + cf) * Sm1;
         const double rw2 = rw * rw;
         const double K = ip2 * (rw2 * rw2);                      // (RE / r)^2
