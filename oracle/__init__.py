@@ -111,6 +111,7 @@ def load():
         lib.orc_icr_kick.argtypes = [C.POINTER(OrcCfg), C.c_double, C.c_double, C.c_double, C.c_double,
                                      C.c_double, _dp, _dp]
         lib.orc_set_vperp.argtypes = [P, _dp, _dp]
+        lib.orc_set_q0.argtypes = [P, _dp]
         lib.orc_get_vperp12.argtypes = [P, _dp, _dp, C.POINTER(C.c_int64)]
         i64p = C.POINTER(C.c_int64)
         lib.orc_cell_count.argtypes = [C.c_double, C.c_double, C.c_double]
@@ -172,6 +173,11 @@ class Oracle:
 
     def count(self):
         return self.lib.orc_count(self.h)
+
+    def set_q0(self, q0):
+        """Replace the q0 that step 1 clamps, scans and interpolates the first
+        fields at (the positions stay those of the q0 given at creation)."""
+        self.lib.orc_set_q0(self.h, ptr(np.ascontiguousarray(q0, dtype=np.float64)))
 
     def set_vperp(self, v1, v2):
         v1, v2 = (np.ascontiguousarray(a, dtype=np.float64) for a in (v1, v2))

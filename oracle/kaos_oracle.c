@@ -576,6 +576,15 @@ void orc_set_vperp(void *h, const double *v1, const double *v2) {
   }
 }
 
+/* The initial dipole coordinate q0 that step 1 clamps and scans for the k0
+ * cells and interpolates the first fields at (KSA:55-320, Gravfield:132-300),
+ * replaced after the positions were set from the q0 given to orc_create; a
+ * test that feeds the reference's own step-1 inputs sets it. */
+void orc_set_q0(void *h, const double *q0) {
+  orc_t *o = (orc_t *)h;
+  for (int64_t j = 0; j < o->ns; ++j) o->q0[j] = q0[j];
+}
+
 void orc_get_vperp12(void *h, double *v1, double *v2, int64_t *id) {
   orc_t *o = (orc_t *)h;
   if (v1) memcpy(v1, o->v1, sizeof(double) * o->ns);

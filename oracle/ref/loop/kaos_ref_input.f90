@@ -11,6 +11,9 @@
 ! come out at 3, the minimum it allows (_0_1_1_ConfigGridGeneratorB.f90:29).
 ! Physical constants are the reference's values.  Output paths are relative:
 ! the harness runs in a scratch directory.
+#ifndef KAOS_REF_EAMB
+#define KAOS_REF_EAMB 1
+#endif
 module ModelInput
   implicit none
   integer, parameter :: dp = kind(1.d0)
@@ -56,7 +59,8 @@ module ModelInput
   integer(kind= dp), parameter :: ICRCOHERENCEflag = 0
   integer(kind= dp), parameter :: MIRRORflag = 0
   integer(kind= dp), parameter :: GRAVflag = 1
-  integer(kind= dp), parameter :: EAMBflag = 1
+  ! (KAOS_REF_EAMB: 1, or 0 for the build without the ambipolar field)
+  integer(kind= dp), parameter :: EAMBflag = KAOS_REF_EAMB
   integer(kind= dp), parameter :: EAMBSELFCONSISTflag = 1
   integer(kind= dp), parameter :: EAMBSIGNflag = 1
   integer(kind= dp), parameter :: EAINERTIALflag = 0

@@ -18,6 +18,11 @@
 !     fastest);
 !   kaos_ref_ions_<rank>.bin: int64 NsT; real64 q0T, Vpar0T, Vperp10T,
 !     Vperp20T, Vperp0T, x0T, y0T, z0T, Vx0T, Vy0T, Vz0T (NsT each).
+! After the loop every rank writes the state its last step left:
+!   kaos_ref_state_<rank>.bin: int64 NsT; real64 x, y, z, Vx, Vy, Vz, xN, yN,
+!     zN, VxN, VyN, VzN, AEAmag, AGmag, AEAmagN, AGmagN, Vperp1, Vperp2
+!     (NsT each, zeros for arrays the loop left unallocated); int64 Qindk1,
+!     Vparindk1 (NsT each).
 ! It times KineticSolverSub with MPI_Wtime between barriers and prints
 !   kaos_ref_loop seconds <max over ranks> ranks <n> ions0 <sum> ions_end <sum>
 !   steps <Nt> ndatfac <ndatfacGT(1)>
@@ -66,6 +71,7 @@ program kaos_ref_loop
   call MPI_BARRIER(MPI_COMM_WORLD, ierr)
   dtl = t1 - t0
   nend = SpecieT(1)%FluxTubeT(1)%NsT(1)
+  call write_state()
   call MPI_REDUCE(dtl, dtmax, 1, MPI_DOUBLE_PRECISION, MPI_MAX, 0, MPI_COMM_WORLD, ierr)
   call MPI_REDUCE(n0, n0sum, 1, MPI_INTEGER8, MPI_SUM, 0, MPI_COMM_WORLD, ierr)
   call MPI_REDUCE(nend, nendsum, 1, MPI_INTEGER8, MPI_SUM, 0, MPI_COMM_WORLD, ierr)
@@ -77,6 +83,56 @@ program kaos_ref_loop
   call MPI_FINALIZE(ierr)
 
 contains
+
+  subroutine put(u, a, ns)
+    integer, intent(in) :: u
+    real(kind= dp), allocatable, intent(in) :: a(:)
+    integer(kind= dp), intent(in) :: ns
+    real(kind= dp), allocatable :: z0(:)
+    if (allocated(a)) then
+      if (size(a) >= ns) then
+        write (u) a(1:ns)
+        return
+      end if
+    end if
+    allocate (z0(ns))
+    z0 = 0d0
+    write (u) z0
+  end subroutine put
+
+  subroutine puti(u, a, ns)
+    integer, intent(in) :: u
+    integer(kind= dp), allocatable, intent(in) :: a(:)
+    integer(kind= dp), intent(in) :: ns
+    integer(kind= dp), allocatable :: z0(:)
+    if (allocated(a)) then
+      if (size(a) >= ns) then
+        write (u) a(1:ns)
+        return
+      end if
+    end if
+    allocate (z0(ns))
+    z0 = 0
+    write (u) z0
+  end subroutine puti
+
+  subroutine write_state()
+    integer :: u
+    integer(kind= dp) :: ns
+    character(len= 64) :: name
+    ns = SpecieT(1)%FluxTubeT(1)%NsT(1)
+    write (name, '(a, i0, a)') 'kaos_ref_state_', rank, '.bin'
+    open (newunit= u, file= trim(name), access= 'stream', form= 'unformatted', status= 'replace')
+    write (u) ns
+    call put(u, x, ns); call put(u, y, ns); call put(u, z, ns)
+    call put(u, Vx, ns); call put(u, Vy, ns); call put(u, Vz, ns)
+    call put(u, xN, ns); call put(u, yN, ns); call put(u, zN, ns)
+    call put(u, VxN, ns); call put(u, VyN, ns); call put(u, VzN, ns)
+    call put(u, AEAmag, ns); call put(u, AGmag, ns); call put(u, AEAmagN, ns); call put(u, AGmagN, ns)
+    call put(u, Vperp1, ns); call put(u, Vperp2, ns)
+    call puti(u, Qindk1, ns); call puti(u, Vparindk1, ns)
+    close (u)
+  end subroutine write_state
 
   subroutine write_inputs()
     integer :: u, lb, ub, nvp, nv1, nv2, ia, ib, ic

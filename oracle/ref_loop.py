@@ -18,13 +18,14 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 EXE = os.path.join(HERE, "_ref", "kaos_ref_loop")
+EXE_EAMB0 = os.path.join(HERE, "_ref", "kaos_ref_loop_eamb0")      # EAMBflag = 0
 REF_DIR = "/root/reference"
 MPIEXEC = "/opt/conda/bin/mpiexec"
 MPI_LIB = "/opt/conda/lib"
 
 
-def available():
-    return os.path.exists(EXE) and os.path.exists(MPIEXEC)
+def available(exe=EXE):
+    return os.path.exists(exe) and os.path.exists(MPIEXEC)
 
 
 def build():
@@ -35,15 +36,16 @@ def build():
     return EXE
 
 
-def run(workdir, nt, nsnormfac, ranks=2, seed=1, timeout=3600):
-    """Run the reference with `ranks` MPI processes in workdir (created):
-    returns the timing line's fields (seconds of KineticSolverSub, the maximum
-    over ranks; initial and final ions; steps; ndatfac)."""
+def run(workdir, nt, nsnormfac, ranks=2, seed=1, timeout=3600, exe=EXE):
+    """Run the reference (`exe`: EXE, or EXE_EAMB0 without the ambipolar field)
+    with `ranks` MPI processes in workdir (created): returns the timing line's
+    fields (seconds of KineticSolverSub, the maximum over ranks; initial and
+    final ions; steps; ndatfac)."""
     os.makedirs(os.path.join(workdir, "kaos_ref_out"), exist_ok=True)
     env = dict(os.environ, KAOS_REF_NT=str(int(nt)), KAOS_REF_NSNORMFAC=repr(float(nsnormfac)).replace("e", "d"),
                KAOS_REF_SEED=str(int(seed)))
     with open(os.path.join(workdir, "stdout.txt"), "w") as fo:
-        r = subprocess.run([MPIEXEC, "-n", str(int(ranks)), EXE], cwd=workdir, env=env, stdout=fo,
+        r = subprocess.run([MPIEXEC, "-n", str(int(ranks)), exe], cwd=workdir, env=env, stdout=fo,
                            stderr=subprocess.STDOUT, timeout=timeout)
     txt = open(os.path.join(workdir, "stdout.txt"), errors="replace").read()
     m = re.search(r"kaos_ref_loop seconds\s+(\S+) ranks (\d+) ions0 (\d+) ions_end (\d+) steps (\d+) ndatfac (\d+)",

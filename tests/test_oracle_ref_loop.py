@@ -23,9 +23,28 @@ the reference generated and compared with what the reference computed:
   its eight 3-D moments of step 1 and the gravity field equal the reference's
   exported M0phRT .. M2ParphRT and EGmagRT within 1e-12.
 
-Later master steps are not compared value by value: the reference's field of
-master step 1 reads row nn - 1 = 0 of its moment arrays (out of range,
+Master steps 2 and 3 are compared on oracle/_ref/kaos_ref_loop_eamb0 (the same
+program with EAMBflag = 0): with the ambipolar field on, the reference's field
+of master step 1 reads row nn - 1 = 0 of its moment arrays (out of range,
 _8_7_AmbipolarEfield.f90:40-82), so its first pushes see an undefined field.
+Two properties of the compiled reference's sequence are measured there and
+fed to the oracle:
+
+* step 1 clamps q0 against the edge arrays read at that same out-of-range row
+  (KSA:55-66; in the column-major arrays, the next cell's edges): every ion
+  of cell 1 takes q0 = qGL of cell 2, and GravfieldSub interpolates the first
+  AGmagN there (_8_7_2_Gravfield.f90:132-300) -- Oracle.set_q0;
+* the ions advance at step 1 and at master steps only: the state the
+  reference's loop leaves after 6 steps is the reference's own push applied
+  at n = 1, 3 and 6, bit for bit, and not at every step (its non-master
+  branch, KSB:800-960, leaves the ions where they are) -- so the oracle is
+  stepped at n = 1, 3, 6.
+
+With both, the 3-D phase-space counts of master steps 2 and 3 are equal, the
+eight moments agree within 1e-13 and the gravity field and the lower-boundary
+escape flux (ions removed at the next master step, BC:158-256) equal the
+reference's: the cadence, removal lag, injection at the boundary cell with the
+logged uniforms and the field lag of the oracle are the reference's.
 """
 import math
 import os
@@ -129,3 +148,43 @@ def test_step1_moments_and_gravity_equal_the_references(ref):
         scale = max(np.abs(R[key]).max(), 1e-300)
         assert np.max(np.abs(m3[key] - R[key])) <= 1e-12 * scale, key
     assert np.max(np.abs(m["EG"] - R["EG"])) <= 1e-12 * np.abs(R["EG"]).max()
+
+
+NT_SEQ = 6                # steps of the sequencing run (master steps at 3 and 6)
+
+
+@pytest.fixture(scope="module")
+def ref_eamb0(tmp_path_factory):
+    if not rl.available(rl.EXE_EAMB0):
+        if rl.build() is None or not rl.available(rl.EXE_EAMB0):
+            pytest.skip("reference sources or MPI absent: the reference loop cannot be built or run")
+    wd = str(tmp_path_factory.mktemp("kaos_ref_loop_eamb0"))
+    rl.run(wd, NT_SEQ, NSNORMFAC, ranks=2, seed=3, exe=rl.EXE_EAMB0)
+    return dict(wd=wd, inp=rl.load_inputs(wd))
+
+
+def test_master_steps_2_and_3_equal_the_references(ref_eamb0):
+    inp = ref_eamb0["inp"]
+    t, I = inp["tube"], inp["ions"]
+    t.eamb = 0
+    qGL = t.arrays["qGL"]
+    # step 1's q0: clamped at cell 2's lower edge (the row past the end of cell 1's)
+    q0s = np.minimum(I["q0"], qGL[1]) if qGL[0] > 0 else np.maximum(I["q0"], qGL[1])
+    assert np.sum(q0s != I["q0"]) > 0
+    cap = len(I["q0"]) + 4 * inp["n_inj_rank"] * inp["ranks"] + 1024
+    o = oracle.Oracle(t, I["q0"], I["vpar0"], I["vperp"], ndatfac=3, seed=1, capacity=cap)
+    o.set_vperp(I["v1"], I["v2"])
+    o.set_q0(q0s)
+    o.step(1)
+    for n in (3, 6):
+        o.step(n, rl.injection_uniforms(inp, n // 3))
+        nn = n // 3 + 1
+        R = rl.read_master(ref_eamb0["wd"], nn, t)
+        m3, m = o.master3(), o.master()
+        assert np.array_equal(m3["n3_counts"], R["n3_counts"]), nn
+        for key in oracle.MOMENTS3:
+            scale = max(np.abs(R[key]).max(), 1e-300)
+            assert np.max(np.abs(m3[key] - R[key])) <= 1e-13 * scale, (nn, key)
+        assert np.max(np.abs(m["EG"] - R["EG"])) <= 1e-12 * np.abs(R["EG"]).max(), nn
+        assert abs(m["esc"][2] - R["lb_out"][0]) <= 1e-12 * max(abs(R["lb_out"][0]), 1.0), nn
+    assert m["esc"][0] > 0                       # ions left through the lower boundary by master step 3
