@@ -36,9 +36,10 @@ fed to the oracle:
   AGmagN there (_8_7_2_Gravfield.f90:132-300) -- Oracle.set_q0;
 * the ions advance at step 1 and at master steps only: the state the
   reference's loop leaves after 6 steps is the reference's own push applied
-  at n = 1, 3 and 6, bit for bit, and not at every step (its non-master
-  branch, KSB:800-960, leaves the ions where they are) -- so the oracle is
-  stepped at n = 1, 3, 6.
+  at n = 1, 3 and 6, bit for bit, and not at every step: the non-master
+  branch of KineticSolverBSub matches only for nn >= 2 (KSB:804-808), and the
+  `exit nnloopKSnonstat` that ends the loop body (KSB:1033) leaves the loop
+  during its first pass, nn = 1 -- so the oracle is stepped at n = 1, 3, 6.
 
 With both, the 3-D phase-space counts of master steps 2 and 3 are equal, the
 eight moments agree within 1e-13 and the gravity field and the lower-boundary
