@@ -149,7 +149,7 @@ def cpu_reference_push(n_ions=2_000_000):
                        f"{t:.2f} s; excludes binning and moments")
 
 
-TUNE_STEPS_MAX = 64        # the autotuner needs 2 + 2 x 8 master steps and one more to choose
+TUNE_STEPS_MAX = 64        # the autotuner needs 2 + 3 x 8 master steps and one more to choose
 
 
 def finish_tuning(ks, n_next):

@@ -52,6 +52,9 @@
 // window spans the tube's velocity box, so reorders are rare).
 #define KAOS_REORDER_FP64 128
 #define KAOS_REORDER_MIXED 128
+// Timed launches per autotuner candidate (its time is the fastest of them: a
+// single sample lets the step-to-step spread, a few percent, pick the shape)
+#define KAOS_TUNE_SAMPLES 3
 
 struct DevTube {
   int nq, nvpar, nvperp, desc;
@@ -2185,7 +2188,7 @@ static int tune_before_push(kaos_ctx* c, hipEvent_t* e0, hipEvent_t* e1) {
     return KAOS_OK;
   }
   const size_t nc = c->tune_cand.size();
-  if (c->tune_i < 2 * nc) {
+  if (c->tune_i < KAOS_TUNE_SAMPLES * nc) {
     std::pair<hipEvent_t, hipEvent_t> ev;
     HIPCHK(c, hipEventCreate(&ev.first));
     HIPCHK(c, hipEventCreate(&ev.second));

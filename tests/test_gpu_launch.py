@@ -78,7 +78,7 @@ def test_autotuned_run_bitexact_and_settles():
     ks.step(1)
     assert ks.get_launch()[3] == 1          # still tuning
     n = 2
-    while n < 24:                           # 2 warm-up + 16 candidate steps + the choice + graph steps
+    while n < 32:                           # 2 warm-up + 24 candidate steps + the choice + graph steps
         for o in orcs:
             o.step(n)
         ks.run(n, 1)
