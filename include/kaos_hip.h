@@ -242,6 +242,55 @@ typedef struct kaos_spinup_info {
  * KAOS_E_IO if a file cannot be written. */
 int kaos_export_spinup(kaos_ctx* ctx, int32_t tube, const kaos_spinup_info* info);
 
+/* The scalars of a previous run's spin-up files (SPINUPflag = 0 restart,
+ * _1_SimParameterization.f90:801-946). */
+typedef struct kaos_spinup_state {
+  int64_t nq_lb, nq_ub;    /* NqLBp, NqUBp (NqLBTfort.bin, NqUBTfort.bin) */
+  double nsnorm_clb;       /* nsnormCLBInput: the LB injection normalisation nsnormCLB
+                              (_2_2_DensityProfileA2.f90:327-337); one rank injects
+                              nint(nsnorm_clb) ions per master step (BC:747-750) */
+  double nsnorm_cub;       /* nsnormCUBInput */
+  int64_t lb_replenish;    /* LBREPLENISHflagInput */
+  int64_t ub_replenish;    /* UBREPLENISHflagInput */
+  int64_t nq;              /* cells of the arrays: nq_ub - nq_lb + 1 */
+} kaos_spinup_state;
+
+/* Read the spin-up files <s>_<f>_<Name>fort.bin of Densitydatadir `dir` as the
+ * reference's restart does (_1_SimParameterization.f90:801-946: Fortran
+ * stream-unformatted, little-endian int64/binary64, no record markers):
+ * NqLBT, NqUBT, nsnormCLBOutput, nsnormCUBOutput, DensityOutputRT,
+ * TemperatureOutputRT, EAInertialOutputRT, EAPressureOutputRT,
+ * EAmagOutputRT, LBREPLENISHflagTOutput, UBREPLENISHflagTOutput.  The arrays
+ * (nq values each, any pointer may be NULL) are the reference's
+ * DensityInputT (m^-3), TemperatureInputT (K) and the three EA inputs
+ * (V/m); a restart seeds a tube from them: kaos_cell_counts(density) for
+ * nsnormC0 (_0_1_1_ConfigGridGeneratorB.f90:75-86), kaos_seed_particles
+ * with the temperatures (_6_VelocityDistribution.f90:70-73) and
+ * kaos_set_field with ea_mag (_8_7_AmbipolarEfield.f90:1375-1407).  Host
+ * file reads only: no context, no device.  KAOS_E_IO if a file is missing
+ * or not of the size its name implies (nq from NqLBT/NqUBT), KAOS_E_ARG if
+ * nq exceeds nq_cap or nq < 1. */
+int kaos_read_spinup(const char* dir, int32_t species, int32_t flux_tube, int64_t nq_cap,
+                     kaos_spinup_state* st, double* density, double* temperature, double* ea_inertial,
+                     double* ea_pressure, double* ea_mag);
+
+/* Source of a tube's ambipolar field EAmagRT at master steps. */
+#define KAOS_FIELD_COMPUTED    0  /* AmbipolarEfieldSub's stencils (default) */
+#define KAOS_FIELD_INPUT_FIRST 1  /* the given field at step 1, computed later
+                                     (SPINUPflag = 0 with EAMBSELFCONSISTflag = 1
+                                     or dNTe /= 0, _8_7_AmbipolarEfield.f90:1375-1390) */
+#define KAOS_FIELD_INPUT_FIXED 2  /* the given field at step 1 and every master
+                                     step (SPINUPflag = 0, EAMBSELFCONSISTflag = 0,
+                                     dNTe = 0, _8_7_AmbipolarEfield.f90:1392-1407) */
+
+/* Set the input field of a tube (nq values of EAmagInputT, V/m; may be NULL
+ * with KAOS_FIELD_COMPUTED) and its mode.  The input replaces the computed
+ * EAmagRT after the stencils and their LB override, as the reference's
+ * assignment follows them; the per-ion interpolation and the injected ions'
+ * AEAmag then read it.  Allowed after kaos_set_grid and between steps; it
+ * takes effect at the next field evaluation (step 1 or a master step). */
+int kaos_set_field(kaos_ctx* ctx, int32_t tube, const double* EA, int32_t mode);
+
 /* The 3-D outputs of the last master step of a tube (KAOS_FLAG_VPERP3D):
  * n3_counts (nq*nvperp1*nvperp2*nvpar, [Q][V1][V2][Vpar], Vpar fastest):
  * N2PerpphReNormRT (_8_3_1_IonParticleCounts.f90:336-361); M (8*nq): M0phRT,

@@ -62,6 +62,15 @@ class KaosSpinupInfo(C.Structure):
                 ("nsnorm_cub", C.c_double), ("lb_replenish", C.c_int64), ("ub_replenish", C.c_int64)]
 
 
+class KaosSpinupState(C.Structure):
+    _fields_ = [("nq_lb", C.c_int64), ("nq_ub", C.c_int64), ("nsnorm_clb", C.c_double),
+                ("nsnorm_cub", C.c_double), ("lb_replenish", C.c_int64), ("ub_replenish", C.c_int64),
+                ("nq", C.c_int64)]
+
+
+KAOS_FIELD_COMPUTED, KAOS_FIELD_INPUT_FIRST, KAOS_FIELD_INPUT_FIXED = 0, 1, 2
+
+
 class KaosStepOut(C.Structure):
     _fields_ = [("n_alive", C.c_int64), ("n_injected", C.c_int64), ("master", C.c_int32),
                 ("status", C.c_int32)]
@@ -70,6 +79,7 @@ class KaosStepOut(C.Structure):
 EXPORTS = ("kaos_init", "kaos_set_grid", "kaos_upload_particles", "kaos_upload_vperp", "kaos_cell_counts",
            "kaos_seed_particles", "kaos_step", "kaos_step_noalloc", "kaos_run",
            "kaos_download_master", "kaos_download_master3d", "kaos_export_master", "kaos_export_spinup",
+           "kaos_read_spinup", "kaos_set_field",
            "kaos_count", "kaos_download_particles", "kaos_download_vperp", "kaos_upload_state",
            "kaos_upload_moments", "kaos_resume", "kaos_save_state", "kaos_load_state", "kaos_set_launch",
            "kaos_get_launch", "kaos_profile", "kaos_profile_read", "kaos_last_error", "kaos_finalize")
@@ -101,6 +111,8 @@ def load(path=LIB_PATH):
     lib.kaos_export_master.argtypes = [P, C.c_int32, C.POINTER(KaosExportInfo), i64p]
     lib.kaos_export_spinup.argtypes = [P, C.c_int32, C.POINTER(KaosSpinupInfo)]
     lib.kaos_download_master3d.argtypes = [P, C.c_int32, i64p, _dp]
+    lib.kaos_read_spinup.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int64, C.POINTER(KaosSpinupState)] + [_dp] * 5
+    lib.kaos_set_field.argtypes = [P, C.c_int32, _dp, C.c_int32]
     lib.kaos_count.argtypes = [P, C.c_int32, i64p]
     lib.kaos_download_particles.argtypes = [P, C.c_int32, C.c_int64, i64p] + [_dp] * 10 + [i64p] * 3
     lib.kaos_download_vperp.argtypes = [P, C.c_int32, C.c_int64, i64p, _dp, _dp]

@@ -66,7 +66,8 @@ struct DevTube {
   double qGL0, qGHlast, qGL_last0;
   double rGL0, rGHlast, rsgn;               // |r| of the outer q edges; sign of rs in the tube
   long long koff;                           // offset of the tube's nq + 2 reorder keys
-  int icr, pad2;                            // ICR wave heating on; its parameters
+  int icr;                                  // ICR wave heating on; its parameters (icrp)
+  int ea_in;                                // KAOS_FIELD_*: source of EAmagRT at master steps
   kaos_icr_t icrp;
   int nv1, nv2;                             // 3-D velocity grid: Vperp1 / Vperp2 cells (0: 2-D grid)
   long long v1off, v2off, h3off, d3off;     // Vperp1 / Vperp2 edges, 3-D counts, d3v3 offsets
@@ -886,8 +887,12 @@ __global__ void k_hbox_reset(int ncells, const DevTube* tubes, const int* blk_tu
   hbox[tubes[t].goff + blk_q[ci]] = make_int4(b.x, b.x + b.y - 1, b.z, b.z + b.w - 1);
 }
 
+// An input field (kaos_set_field) then replaces the computed one: at step 1
+// (rotate_esc = 0) in both input modes, at later master steps in the fixed
+// mode (_8_7_AmbipolarEfield.f90:1375-1407, after the stencils and the LB
+// override).
 __global__ void k_field(const DevTube* tubes, const double* hq, const double* dq, const double* Te,
-                        const double* Mprev, double* M0F, double* M1F, double* EA,
+                        const double* Mprev, double* M0F, double* M1F, double* EA, const double* EAin,
                         unsigned long long* esc_acc, unsigned long long* esc_rep, int rotate_esc) {
   int t = blockIdx.x;
   DevTube T = tubes[t];
@@ -906,6 +911,10 @@ __global__ void k_field(const DevTube* tubes, const double* hq, const double* dq
   if (threadIdx.x == 0) {
     kaos_ea_override(E, M0, T.eamb);
     if (rotate_esc) { esc_acc[2 * t] = 0ull; esc_acc[2 * t + 1] = 0ull; }
+  }
+  if (T.ea_in == KAOS_FIELD_INPUT_FIXED || (T.ea_in == KAOS_FIELD_INPUT_FIRST && !rotate_esc)) {
+    __syncthreads();
+    for (int Q = threadIdx.x; Q < T.nq; Q += blockDim.x) E[Q] = EAin[T.goff + Q];
   }
 }
 
@@ -1434,6 +1443,8 @@ struct HostTube {
   std::vector<double> qGL, qGH, qGC, hq, dq, d3x, rGC, cth, ell, Te;
   std::vector<double> vparGL, vparGH, vparGC, dvpar, vperpGL, vperpGH, vperpGC, dvperp, d3v;
   std::vector<double> v1GL, v1GH, v1GC, dv1, v2GL, v2GH, v2GC, dv2, d3v3;   // 3-D velocity grid
+  std::vector<double> ea_in;                // input field (kaos_set_field), nq values
+  int ea_mode = 0;                          // KAOS_FIELD_*
 };
 
 // One set of particle-state arrays (the binary64 or the mixed form's words).
@@ -1463,7 +1474,7 @@ struct kaos_ctx {
   int* d_blk_tube = nullptr; int* d_blk_q = nullptr; int nblk_mom = 0;
   int *d_cells_wave = nullptr, *d_cells_block = nullptr; int n_cells_wave = 0, n_cells_block = 0;
   double *d_qGH = nullptr, *d_qGC = nullptr, *d_hq = nullptr, *d_dq = nullptr, *d_Te = nullptr,
-         *d_d3x = nullptr, *d_EA = nullptr, *d_EG = nullptr, *d_M = nullptr, *d_M0F = nullptr,
+         *d_d3x = nullptr, *d_EA = nullptr, *d_EAin = nullptr, *d_EG = nullptr, *d_M = nullptr, *d_M0F = nullptr,
          *d_M1F = nullptr, *d_NqRT = nullptr;
   double *d_vparGH = nullptr, *d_vperpGH = nullptr, *d_vparGC = nullptr, *d_vperpGC = nullptr,
          *d_dvpar = nullptr, *d_dvperp = nullptr, *d_d3v = nullptr;
@@ -1682,7 +1693,7 @@ static int build_device(kaos_ctx* c) {
   for (int t = 0; t < nt; ++t) if (!c->ht[t].set) return fail(c, KAOS_E_STATE, "tube " + std::to_string(t) + " has no grid");
   c->dt.resize(nt);
   long long goff = 0, voff = 0, woff = 0, hoff = 0, doff = 0, off = 0;
-  std::vector<double> qGH, qGC, hq, dq, Te, d3x, EG, vparGHx, vperpGHx, vparGC, vperpGC, dvpar, dvperp, d3v;
+  std::vector<double> qGH, qGC, hq, dq, Te, d3x, EG, EAin, vparGHx, vperpGHx, vparGC, vperpGC, dvpar, dvperp, d3v;
   std::vector<double> rGH, rref;
   std::vector<int> rk, vk;
   std::vector<double> v1GHx, v1GC, dv1, v2GHx, v2GC, dv2, d3v3;
@@ -1713,6 +1724,7 @@ static int build_device(kaos_ctx* c) {
     T.rsgn = T.desc ? 1.0 : -1.0;
     T.icr = g.icr ? 1 : 0;
     if (T.icr) c->any_icr = 1;
+    T.ea_in = h.ea_mode;
     T.icrp.qs = g.qs; T.icrp.ms = g.ms; T.icrp.h = g.h;
     T.icrp.lambda_perp = g.icr_lambda_perp; T.icrp.eta_lh = g.icr_eta_lh;
     T.icrp.xi1 = g.icr_xi_perp1; T.icrp.xi2 = g.icr_xi_perp2; T.icrp.s0 = g.icr_s0;
@@ -1758,6 +1770,7 @@ static int build_device(kaos_ctx* c) {
       qGH.push_back(h.qGH[i]); qGC.push_back(h.qGC[i]); hq.push_back(h.hq[i]); dq.push_back(h.dq[i]);
       Te.push_back(h.Te[i]); d3x.push_back(h.d3x[i]);
       EG.push_back(kaos_grav_cell(g.grav, g.ms, g.qs, h.cth[i], h.rGC[i], h.ell[i]));
+      EAin.push_back(h.ea_in.empty() ? 0.0 : h.ea_in[i]);
       blk_tube.push_back(t); blk_q.push_back(i);
     }
     // velocity high edges followed by the low edge of cell 1 (read as GL0)
@@ -1798,6 +1811,7 @@ static int build_device(kaos_ctx* c) {
   HIPCHK(c, up((void**)&c->d_EG, EG.data(), sizeof(double) * goff));
   std::vector<double> zeros(6 * goff, 0.0);
   HIPCHK(c, up((void**)&c->d_EA, zeros.data(), sizeof(double) * goff));
+  HIPCHK(c, up((void**)&c->d_EAin, EAin.data(), sizeof(double) * goff));
   HIPCHK(c, up((void**)&c->d_M0F, zeros.data(), sizeof(double) * goff));
   HIPCHK(c, up((void**)&c->d_M1F, zeros.data(), sizeof(double) * goff));
   HIPCHK(c, up((void**)&c->d_NqRT, zeros.data(), sizeof(double) * goff));
@@ -2280,7 +2294,7 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
       hipLaunchKernelGGL(k_bin_initial, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a);
     // field of master step 1 from "master step 0" (no moments: M = 0), then moments of step 1
     hipLaunchKernelGGL(k_field, dim3(nt), dim3(256), 0, c->stream, c->d_tubes, c->d_hq, c->d_dq, c->d_Te,
-                       c->d_M, c->d_M0F, c->d_M1F, c->d_EA, c->d_esc_acc, c->d_esc_rep, 0);
+                       c->d_M, c->d_M0F, c->d_M1F, c->d_EA, c->d_EAin, c->d_esc_acc, c->d_esc_rep, 0);
     launch_moments(c, nullptr);
     launch_box_update(c);
     if (c->any_icr && c->cfg.precision != KAOS_MIXED)
@@ -2289,7 +2303,7 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
   } else {
     if (master) {
       hipLaunchKernelGGL(k_field, dim3(nt), dim3(256), 0, c->stream, c->d_tubes, c->d_hq, c->d_dq, c->d_Te,
-                         c->d_M, c->d_M0F, c->d_M1F, c->d_EA, c->d_esc_acc, c->d_esc_rep, 1);
+                         c->d_M, c->d_M0F, c->d_M1F, c->d_EA, c->d_EAin, c->d_esc_acc, c->d_esc_rep, 1);
       zero_hist();
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -2751,6 +2765,93 @@ extern "C" int kaos_export_spinup(kaos_ctx* c, int32_t tube, const kaos_spinup_i
   return KAOS_OK;
 }
 
+// Whole-file read of a spin-up file that must hold exactly `bytes` bytes.
+static bool read_raw(const std::string& path, void* p, size_t bytes) {
+  FILE* f = fopen(path.c_str(), "rb");
+  if (!f) return false;
+  size_t r = bytes ? fread(p, 1, bytes, f) : 0;
+  bool ok = (r == bytes);
+  char extra;
+  if (ok && fread(&extra, 1, 1, f) != 0) ok = false;   // longer than its name implies
+  fclose(f);
+  return ok;
+}
+
+extern "C" int kaos_read_spinup(const char* dir, int32_t species, int32_t flux_tube, int64_t nq_cap,
+                                kaos_spinup_state* st, double* density, double* temperature, double* ea_inertial,
+                                double* ea_pressure, double* ea_mag) {
+  if (!dir || !st || species < 1 || flux_tube < 1) {
+    g_init_err = "kaos_read_spinup needs dir, st and 1-based species/flux_tube";
+    return KAOS_E_ARG;
+  }
+  std::string d(dir);
+  if (!d.empty() && d.back() != '/') d += '/';
+  const std::string pre = d + std::to_string(species) + "_" + std::to_string(flux_tube) + "_";
+  std::string bad;
+  auto get = [&](const char* name, void* p, size_t bytes) {
+    if (bad.empty() && !read_raw(pre + name + "fort.bin", p, bytes)) bad = pre + name + "fort.bin";
+  };
+  kaos_spinup_state S;
+  memset(&S, 0, sizeof(S));
+  get("NqLBT", &S.nq_lb, sizeof(int64_t));
+  get("NqUBT", &S.nq_ub, sizeof(int64_t));
+  get("nsnormCLBOutput", &S.nsnorm_clb, sizeof(double));
+  get("nsnormCUBOutput", &S.nsnorm_cub, sizeof(double));
+  get("LBREPLENISHflagTOutput", &S.lb_replenish, sizeof(int64_t));
+  get("UBREPLENISHflagTOutput", &S.ub_replenish, sizeof(int64_t));
+  if (!bad.empty()) { g_init_err = "cannot read " + bad + " (missing or not 8 bytes)"; return KAOS_E_IO; }
+  S.nq = S.nq_ub - S.nq_lb + 1;
+  if (S.nq < 1 || S.nq > (1LL << 30)) {
+    g_init_err = "spin-up files give NqUBT - NqLBT + 1 = " + std::to_string(S.nq) + " cells";
+    return KAOS_E_IO;
+  }
+  if (S.nq > nq_cap) {
+    g_init_err = "spin-up files hold " + std::to_string(S.nq) + " cells, more than nq_cap";
+    return KAOS_E_ARG;
+  }
+  const size_t nb = sizeof(double) * (size_t)S.nq;
+  std::vector<double> tmp(S.nq);
+  const char* names[5] = {"DensityOutputRT", "TemperatureOutputRT", "EAInertialOutputRT", "EAPressureOutputRT",
+                          "EAmagOutputRT"};
+  double* outs[5] = {density, temperature, ea_inertial, ea_pressure, ea_mag};
+  for (int i = 0; i < 5; ++i) {
+    get(names[i], tmp.data(), nb);         // every file is read (and checked) even if not wanted
+    if (bad.empty() && outs[i]) memcpy(outs[i], tmp.data(), nb);
+  }
+  if (!bad.empty()) {
+    g_init_err = "cannot read " + bad + " (missing or not " + std::to_string(S.nq) + " binary64 values)";
+    return KAOS_E_IO;
+  }
+  *st = S;
+  return KAOS_OK;
+}
+
+extern "C" int kaos_set_field(kaos_ctx* c, int32_t tube, const double* EA, int32_t mode) {
+  if (!c) return KAOS_E_ARG;
+  if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
+  if (!c->ht[tube].set) return fail(c, KAOS_E_STATE, "kaos_set_field before kaos_set_grid");
+  if (mode != KAOS_FIELD_COMPUTED && mode != KAOS_FIELD_INPUT_FIRST && mode != KAOS_FIELD_INPUT_FIXED)
+    return fail(c, KAOS_E_ARG, "field mode must be KAOS_FIELD_COMPUTED, _INPUT_FIRST or _INPUT_FIXED");
+  if (mode != KAOS_FIELD_COMPUTED && !EA) return fail(c, KAOS_E_ARG, "an input field needs EA");
+  HostTube& h = c->ht[tube];
+  const int nq = h.g.nq;
+  if (EA) {
+    for (int Q = 0; Q < nq; ++Q)
+      if (!std::isfinite(EA[Q])) return fail(c, KAOS_E_ARG, "input field value not finite");
+    h.ea_in.assign(EA, EA + nq);
+  }
+  h.ea_mode = mode;
+  if (c->built) {
+    // device copies: the tube table (graphs read it at replay) and the values
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    DevTube& T = c->dt[tube];
+    T.ea_in = mode;
+    HIPCHK(c, hipMemcpy(c->d_tubes + tube, &T, sizeof(DevTube), hipMemcpyHostToDevice));
+    if (EA) HIPCHK(c, hipMemcpy(c->d_EAin + T.goff, h.ea_in.data(), sizeof(double) * nq, hipMemcpyHostToDevice));
+  }
+  return KAOS_OK;
+}
+
 // The live ions of a tube on the host, in the reference's order (survivors
 // in order, then injected ions: by id), with their Cartesian state.
 struct TubeDump {
@@ -3166,7 +3267,7 @@ extern "C" int kaos_finalize(kaos_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   void* bufs[] = {c->d_tubes, c->d_tile_tube, c->d_tile_start, c->d_blk_tube, c->d_blk_q, c->d_cells_wave,
                   c->d_cells_block, c->d_qGH, c->d_qGC,
-                  c->d_hq, c->d_dq, c->d_Te, c->d_d3x, c->d_EA, c->d_EG, c->d_M, c->d_M0F, c->d_M1F, c->d_NqRT,
+                  c->d_hq, c->d_dq, c->d_Te, c->d_d3x, c->d_EA, c->d_EAin, c->d_EG, c->d_M, c->d_M0F, c->d_M1F, c->d_NqRT,
                   c->d_vparGH, c->d_vperpGH, c->d_vparGC, c->d_vperpGC, c->d_dvpar, c->d_dvperp, c->d_d3v,
                   c->d_rGH, c->d_rref, c->d_rk, c->d_vk, c->d_keycnt, c->d_cursor, c->d_cnt_new,
                   c->d_next_id, c->d_count, c->d_hist,

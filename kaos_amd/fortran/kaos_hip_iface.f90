@@ -62,6 +62,16 @@ module kaos_hip
     integer(c_int64_t) :: lb_replenish, ub_replenish
   end type kaos_spinup_info
 
+  ! scalars of a previous run's spin-up files (kaos_read_spinup)
+  type, bind(c), public :: kaos_spinup_state
+    integer(c_int64_t) :: nq_lb, nq_ub
+    real(c_double) :: nsnorm_clb, nsnorm_cub
+    integer(c_int64_t) :: lb_replenish, ub_replenish, nq
+  end type kaos_spinup_state
+
+  integer(c_int32_t), parameter, public :: KAOS_FIELD_COMPUTED = 0, KAOS_FIELD_INPUT_FIRST = 1, &
+                                            KAOS_FIELD_INPUT_FIXED = 2
+
   type, bind(c), public :: kaos_step_out
     integer(c_int64_t) :: n_alive, n_injected
     integer(c_int32_t) :: master, status
@@ -70,6 +80,7 @@ module kaos_hip
   public :: kaos_init, kaos_set_grid, kaos_upload_particles, kaos_upload_vperp, kaos_cell_counts, &
             kaos_seed_particles, kaos_step, kaos_run, &
             kaos_download_master, kaos_download_master3d, kaos_export_master, kaos_export_spinup, &
+            kaos_read_spinup, kaos_set_field, &
             kaos_count, kaos_download_particles, kaos_download_vperp, kaos_upload_state, &
             kaos_upload_moments, kaos_resume, kaos_save_state, kaos_load_state, kaos_set_launch, &
             kaos_get_launch, kaos_profile, &
@@ -167,6 +178,26 @@ module kaos_hip
       integer(c_int32_t), value :: tube
       type(kaos_spinup_info), intent(in) :: info
     end function kaos_export_spinup
+
+    ! the spin-up files of a previous run (SPINUPflag = 0 restart); dir as in kaos_spinup_info
+    integer(c_int) function kaos_read_spinup(dir, species, flux_tube, nq_cap, st, density, temperature, &
+                                             ea_inertial, ea_pressure, ea_mag) bind(c, name='kaos_read_spinup')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, kaos_spinup_state
+      type(c_ptr), value :: dir
+      integer(c_int32_t), value :: species, flux_tube
+      integer(c_int64_t), value :: nq_cap
+      type(kaos_spinup_state), intent(out) :: st
+      type(c_ptr), value :: density, temperature, ea_inertial, ea_pressure, ea_mag
+    end function kaos_read_spinup
+
+    ! input ambipolar field (EAmagInputT) of a restart and its mode (KAOS_FIELD_*)
+    integer(c_int) function kaos_set_field(ctx, tube, EA, mode) bind(c, name='kaos_set_field')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tube
+      type(c_ptr), value :: EA
+      integer(c_int32_t), value :: mode
+    end function kaos_set_field
 
     ! DataExport2Sub's files of the last master step
     integer(c_int) function kaos_export_master(ctx, tube, info, nn) bind(c, name='kaos_export_master')

@@ -15,7 +15,8 @@ import numpy as np
 
 from . import _abi
 from ._abi import (KaosCfg, KaosGrid, KaosStepOut, KaosExportInfo, ptr, KAOS_FP64, KAOS_MIXED,  # noqa: F401
-                   KAOS_FLAG_NO_GRAPH, KAOS_FLAG_NO_AUTOTUNE, KAOS_FLAG_VPERP3D)
+                   KAOS_FLAG_NO_GRAPH, KAOS_FLAG_NO_AUTOTUNE, KAOS_FLAG_VPERP3D,
+                   KAOS_FIELD_COMPUTED, KAOS_FIELD_INPUT_FIRST, KAOS_FIELD_INPUT_FIXED)
 
 MOMENT_NAMES = ("M0", "M1perp", "M1par", "M2", "M2perp", "M2par")
 MOMENTS3 = ("M0", "M1perp1", "M1perp2", "M1par", "M2", "M2perp1", "M2perp2", "M2par")
@@ -176,6 +177,13 @@ class KineticSolver:
             lb_replenish=lb_replenish, ub_replenish=ub_replenish)
         self._chk(self.lib.kaos_export_spinup(self.h, tube, C.byref(info)))
 
+    def set_field(self, tube, EA, mode):
+        """kaos_set_field: input field EAmagInputT of a restart (nq values) and
+        its mode (KAOS_FIELD_INPUT_FIRST, KAOS_FIELD_INPUT_FIXED or
+        KAOS_FIELD_COMPUTED)."""
+        e = None if EA is None else np.ascontiguousarray(EA, dtype=np.float64)
+        self._chk(self.lib.kaos_set_field(self.h, tube, ptr(e), int(mode)))
+
     def master3(self, tube):
         """3-D outputs of the last master step (KAOS_FLAG_VPERP3D): counts
         [Q][V1][V2][Vpar] and the eight 3-D moments."""
@@ -280,4 +288,23 @@ def cell_counts(density, d3x, nsnormfac):
     rc = lib.kaos_cell_counts(len(density), ptr(density), ptr(d3x), float(nsnormfac), ptr(out, C.c_int64))
     if rc:
         raise KaosError(rc, "kaos_cell_counts: bad input (nsnormfac <= 0, or a count not finite)")
+    return out
+
+
+def read_spinup(directory, species=1, flux_tube=1, nq_cap=1 << 20):
+    """kaos_read_spinup: the spin-up files of a previous run as the reference's
+    restart reads them (_1_SimParameterization.f90:801-946).  Returns a dict of
+    the scalars (nq_lb, nq_ub, nsnorm_clb, nsnorm_cub, lb_replenish,
+    ub_replenish, nq) and the arrays density, temperature, ea_inertial,
+    ea_pressure, ea_mag."""
+    lib = _abi.load()
+    st = _abi.KaosSpinupState()
+    bufs = [np.zeros(nq_cap) for _ in range(5)]
+    rc = lib.kaos_read_spinup(str(directory).encode(), species, flux_tube, nq_cap, C.byref(st),
+                              *[ptr(b) for b in bufs])
+    if rc:
+        raise KaosError(rc, lib.kaos_last_error(None).decode())
+    out = {k: getattr(st, k) for k, _ in _abi.KaosSpinupState._fields_}
+    for k, b in zip(("density", "temperature", "ea_inertial", "ea_pressure", "ea_mag"), bufs):
+        out[k] = b[:st.nq].copy()
     return out

@@ -112,6 +112,7 @@ def load():
                                      C.c_double, _dp, _dp]
         lib.orc_set_vperp.argtypes = [P, _dp, _dp]
         lib.orc_set_q0.argtypes = [P, _dp]
+        lib.orc_set_field_input.argtypes = [P, _dp, C.c_int]
         lib.orc_get_vperp12.argtypes = [P, _dp, _dp, C.POINTER(C.c_int64)]
         i64p = C.POINTER(C.c_int64)
         lib.orc_cell_count.argtypes = [C.c_double, C.c_double, C.c_double]
@@ -178,6 +179,13 @@ class Oracle:
         """Replace the q0 that step 1 clamps, scans and interpolates the first
         fields at (the positions stay those of the q0 given at creation)."""
         self.lib.orc_set_q0(self.h, ptr(np.ascontiguousarray(q0, dtype=np.float64)))
+
+    def set_field_input(self, EA, mode):
+        """Restart field (EAmagInputT, nq values): mode 1 replaces the computed
+        EAmagRT at step 1, mode 2 at step 1 and every master step, 0 computes it
+        (_8_7_AmbipolarEfield.f90:1375-1407)."""
+        e = None if EA is None else np.ascontiguousarray(EA, dtype=np.float64)
+        self.lib.orc_set_field_input(self.h, ptr(e), int(mode))
 
     def set_vperp(self, v1, v2):
         v1, v2 = (np.ascontiguousarray(a, dtype=np.float64) for a in (v1, v2))

@@ -80,6 +80,8 @@ typedef struct {
   double *M3;                      /* 8*nq: M0, M1Perp1, M1Perp2, M1Par, M2, M2Perp1, M2Perp2, M2Par */
   double esc_lb, esc_ub, flux_lb, flux_ub;
   int64_t injected;
+  double *EAin;                    /* EAmagInputT of a restart (nq), or NULL */
+  int ea_mode;                     /* 0 computed, 1 input at step 1, 2 input at every master step */
 } orc_t;
 
 static void *xc(size_t n) { void *p = calloc(n ? n : 1, 1); return p; }
@@ -362,7 +364,7 @@ void orc_destroy(void *h) {
   ARR_D(FR) ARR_I(FR) ARR_L(FR)
 #undef FR
   free(o->nq_counts); free(o->nph_counts); free(o->nq_u); free(o->nph_u); free(o->NqRT); free(o->M); free(o->Mprev);
-  free(o->EA); free(o->EG); free(o->M0F); free(o->M1F); free(o->n3_u); free(o->M3);
+  free(o->EA); free(o->EG); free(o->M0F); free(o->M1F); free(o->n3_u); free(o->M3); free(o->EAin);
   free(o);
 }
 
@@ -474,6 +476,21 @@ static void push_all(orc_t *o, int64_t n) {
   }
 }
 
+/* The input field of a restart replaces the computed EAmagRT after the
+ * stencils and the LB override: at step 1 (modes 1, 2) and at every later
+ * master step (mode 2) (_8_7_AmbipolarEfield.f90:1375-1407). */
+static void field_input(orc_t *o, int first) {
+  if (o->EAin && (o->ea_mode == 2 || (o->ea_mode == 1 && first)))
+    memcpy(o->EA, o->EAin, sizeof(double) * o->c.nq);
+}
+
+void orc_set_field_input(void *h, const double *EA, int mode) {
+  orc_t *o = (orc_t *)h;
+  if (!o->EAin) o->EAin = (double *)xc(sizeof(double) * o->c.nq);
+  if (EA) memcpy(o->EAin, EA, sizeof(double) * o->c.nq);
+  o->ea_mode = mode;
+}
+
 /* One time step n (n = 1 is KineticSolverASub).  uniforms: 6 per injected
  * ion of this step (LB first), or NULL for the seeded Philox stream.        */
 int orc_step(void *h, int64_t n, const double *uniforms) {
@@ -502,6 +519,7 @@ int orc_step(void *h, int64_t n, const double *uniforms) {
     else counts_and_moments(o, o->qind, o->vparind, o->vperpind, o->ns);
     memset(o->Mprev, 0, sizeof(double) * 6 * nq);   /* moments of master step 0: none */
     ambipolar_field(c, o->Mprev, o->M0F, o->EA);
+    field_input(o, 1);
     m1par_filter(c, o->Mprev, o->M1F);
     grav_field(c, o->EG);
     for (int64_t j = 0; j < o->ns; ++j) {
@@ -516,6 +534,7 @@ int orc_step(void *h, int64_t n, const double *uniforms) {
     if (master) {
       bc_remove(o);
       ambipolar_field(c, o->Mprev, o->M0F, o->EA);
+      field_input(o, 0);
       m1par_filter(c, o->Mprev, o->M1F);
       grav_field(c, o->EG);
       for (int64_t j = 0; j < o->ns; ++j) {
