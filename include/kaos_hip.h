@@ -113,6 +113,24 @@ typedef struct kaos_grid {
   const double *vperp1GL, *vperp1GH, *vperp1GC, *dvperp1;
   const double *vperp2GL, *vperp2GH, *vperp2GC, *dvperp2;
   const double *d3v3;
+  /* Ion-neutral charge exchange (QEXCHANGEflagT = 1; needs KAOS_FLAG_VPERP3D,
+   * whose M1Perp1/M1Perp2 moments the ion mean velocity reads).  At step 1
+   * and every master step each config cell gets a Poisson collision count
+   * (IonNeutralCollisionFrequencyA/BSub, _8_6_6_1:93-300, _8_6_6_2:35-77);
+   * at every step of the next master interval the first nint(count) ions of
+   * the cell (id order) become ENAs (_8_KineticSolver.f90:95-166,
+   * IonNeutralCollisionsSub), which then move ballistically under gravity,
+   * are counted per config cell and escape at LB/UB. */
+  int32_t cx;              /* 1: charge exchange on in this tube */
+  int32_t cx_filter;       /* MOMENTFILTERflagT: 1 = the filtered M1Par (window
+                              filter_pt_m1par), M1Perp1, M1Perp2, M2Par (cx_filter_pt) */
+  int32_t cx_filter_pt;    /* M1Perp1/M1Perp2/M2ParMAfilterPt (odd) */
+  int32_t cx_pad;
+  double t_neut;           /* TNeut [K] */
+  double m_neut;           /* mNeut [kg] */
+  double cx_sigma_scale;   /* factor on sigmaIonNeutRT; 1 = the reference's expression as written */
+  const double* ns_neut;   /* nsnormCNeutGT(1, Q): neutral macro-particles per cell (nq) */
+  int64_t ena_cap;         /* ENA slots of the tube (conversions over the whole run) */
 } kaos_grid;
 
 /* Per-step summary. */
@@ -332,6 +350,24 @@ typedef struct kaos_export_info {
  * M2PerpphRT.  nn_out (may be NULL) receives nn.  KAOS_E_IO if a file cannot
  * be written. */
 int kaos_export_master(kaos_ctx* ctx, int32_t tube, const kaos_export_info* info, int64_t* nn_out);
+
+/* Charge-exchange state of a tube's last master step (kaos_grid.cx): per
+ * config cell sigmaIonNeutRT, nuIonNeutRT (the capped Poisson count, -2 for
+ * an empty cell or sigma = 0) and nuIonNeutPoiT (the one-rank share, ions
+ * converted per step is its nint); any pointer may be NULL. */
+int kaos_download_cx(kaos_ctx* ctx, int32_t tube, double* sigma, double* nu, double* poi);
+
+/* The live ENAs of a tube in id order (the reference's array order: an ENA
+ * keeps its ion's index): Cartesian position and velocity, id, and Qindk1 of
+ * the last push (_8_1_1_KineticUpdateA.f90:221-273 on the ENA's q). */
+int kaos_download_enas(kaos_ctx* ctx, int32_t tube, int64_t n_cap, int64_t* n_out, double* x, double* y,
+                       double* z, double* Vx, double* Vy, double* Vz, int64_t* id, int64_t* qind);
+
+/* ENA outputs of the last master step: nq_ena (nq): NqReNormENAT
+ * (ENAParticleCountsSub, _8_3_2:33-60); esc (4): LB and UB escaped ENA
+ * counts and the fluxes LBoutfluxENART/UBoutfluxENART
+ * (_8_0_7_BoundaryConditions.f90:309-470).  Either pointer may be NULL. */
+int kaos_download_ena_master(kaos_ctx* ctx, int32_t tube, int64_t* nq_ena, double* esc);
 
 /* Number of live ions of a tube. */
 int kaos_count(kaos_ctx* ctx, int32_t tube, int64_t* n);

@@ -48,7 +48,10 @@ class KaosGrid(C.Structure):
                 + [("icr", C.c_int32), ("pad1", C.c_int32)]
                 + [(n, C.c_double) for n in ICR_FIELDS]
                 + [("nvperp1", C.c_int32), ("nvperp2", C.c_int32)]
-                + [(n, _dp) for n in GRID_ARRAYS3])
+                + [(n, _dp) for n in GRID_ARRAYS3]
+                + [(n, C.c_int32) for n in ("cx", "cx_filter", "cx_filter_pt", "cx_pad")]
+                + [(n, C.c_double) for n in ("t_neut", "m_neut", "cx_sigma_scale")]
+                + [("ns_neut", _dp), ("ena_cap", C.c_int64)])
 
 
 class KaosExportInfo(C.Structure):
@@ -80,7 +83,8 @@ EXPORTS = ("kaos_init", "kaos_set_grid", "kaos_upload_particles", "kaos_upload_v
            "kaos_seed_particles", "kaos_step", "kaos_step_noalloc", "kaos_run",
            "kaos_download_master", "kaos_download_master3d", "kaos_export_master", "kaos_export_spinup",
            "kaos_read_spinup", "kaos_set_field",
-           "kaos_count", "kaos_download_particles", "kaos_download_vperp", "kaos_upload_state",
+           "kaos_count", "kaos_download_cx", "kaos_download_enas", "kaos_download_ena_master",
+           "kaos_download_particles", "kaos_download_vperp", "kaos_upload_state",
            "kaos_upload_moments", "kaos_resume", "kaos_save_state", "kaos_load_state", "kaos_set_launch",
            "kaos_get_launch", "kaos_profile", "kaos_profile_read", "kaos_last_error", "kaos_finalize")
 
@@ -114,6 +118,9 @@ def load(path=LIB_PATH):
     lib.kaos_read_spinup.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int64, C.POINTER(KaosSpinupState)] + [_dp] * 5
     lib.kaos_set_field.argtypes = [P, C.c_int32, _dp, C.c_int32]
     lib.kaos_count.argtypes = [P, C.c_int32, i64p]
+    lib.kaos_download_cx.argtypes = [P, C.c_int32, _dp, _dp, _dp]
+    lib.kaos_download_enas.argtypes = [P, C.c_int32, C.c_int64, i64p] + [_dp] * 6 + [i64p] * 2
+    lib.kaos_download_ena_master.argtypes = [P, C.c_int32, i64p, _dp]
     lib.kaos_download_particles.argtypes = [P, C.c_int32, C.c_int64, i64p] + [_dp] * 10 + [i64p] * 3
     lib.kaos_download_vperp.argtypes = [P, C.c_int32, C.c_int64, i64p, _dp, _dp]
     lib.kaos_upload_state.argtypes = [P, C.c_int32, C.c_int64] + [_dp] * 12 + [i64p] * 3

@@ -271,6 +271,7 @@ typedef struct {
   double vpar;           /* Vpar(j): parallel velocity of the input state  */
   double qk1;            /* q of the input state (cell search input)       */
   double qk4;            /* q at the full-step predictor (field coordinate) */
+  double zC, u2C;        /* predictor z and rho^2 + z^2: cos(thetak4) = zC / sqrt(u2C) */
 } kaos_push_out_t;
 
 /* Point of the field line with signed radius rs: r, w = r/(RE p) = sin^2,
@@ -414,6 +415,8 @@ KAOS_HD void kaos_push_ion(kaos_line_t L, kaos_qb_t gA, int qc, double vs, doubl
   o->vpar = vpar;
   o->qk1 = gA.q;
   o->qk4 = q4;
+  o->zC = zC;
+  o->u2C = u24;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1068,6 +1071,168 @@ KAOS_HD void kaos_vcells3(double vpar, double v1, double v2, const double* vparG
   *ip = kaos_find_vcell(vpar, vparGH, vparGL0, nvpar);
   *i1 = kaos_find_vcell(v1, v1GH, v1GL0, nv1);
   *i2 = kaos_find_vcell(v2, v2GH, v2GL0, nv2);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Ion-neutral charge exchange (QEXCHANGEflagT = 1): collision counts per     */
+/* config cell (IonNeutralCollisionFrequencyA/BSub), the ion -> ENA           */
+/* conversion (IonNeutralCollisionsSub) and the ballistic ENA push under the  */
+/* neutral's gravity (the ENA branches of KineticUpdateA/B/C and              */
+/* KineticRK4UpdateSub).                                                      */
+
+#define KAOS_CX_MAXITER 4096      /* bound of the Poisson product loop         */
+
+/* exp(x) for x <= 0, 0 below the binary64 range. */
+KAOS_HD double kaos_exp_neg(double x) { return (x < -745.2) ? 0.0 : kaos_exp(x); }
+
+/* Four uniforms of a cell's collision stream: Philox4x64-10 keyed by (seed,
+ * "CXNT") with counter (cell Q, master step n, tube, block).  Blocks 0-1
+ * give the neutral velocity's three Box-Muller pairs, blocks 2.. the Poisson
+ * loop's uniforms. */
+KAOS_HD void kaos_cx_block(uint64_t seed, int64_t step, int tube, int Q, uint64_t blk, double u[4]) {
+  uint64_t key[2] = { seed, 0x43584E54ULL };
+  uint64_t ctr[4] = { (uint64_t)Q, (uint64_t)step, (uint64_t)tube, blk };
+  uint64_t o[4];
+  kaos_philox4x64(ctr, key, o);
+  u[0] = kaos_u01(o[0]); u[1] = kaos_u01(o[1]); u[2] = kaos_u01(o[2]); u[3] = kaos_u01(o[3]);
+}
+
+/* Charge-exchange cross-section from the parallel energy moment M2par (J):
+ * E = 6.242e18 * 1e-3 * M2par (keV), sigma = (4.07 - 0.269 ln E)^2
+ * (1 - exp(-415/E))^0.8, and 0 for M2par = 0
+ * (_8_6_6_1_IonNeutralCollisionFrequencyA.f90:93-114). */
+KAOS_HD double kaos_cx_sigma(double m2par) {
+  if (m2par == 0.0) return 0.0;
+  const double E = (6.242e18 * 1e-3) * m2par;
+  const double a = 4.07 - 0.269 * kaos_log(E);
+  const double b = 1.0 - kaos_exp_neg(-415.0 / E);
+  return (a * a) * ((b > 0.0) ? kaos_exp(0.8 * kaos_log(b)) : 0.0);
+}
+
+/* Poisson variate of mean mu by the product of uniforms
+ * (_3_2_PoissonRNG.f90:30-40): uniforms are multiplied while the product
+ * stays above exp(-mu); the count is their number less one.  The uniforms
+ * come from blocks 2.. of the cell's stream; at most KAOS_CX_MAXITER. */
+KAOS_HD double kaos_poisson(double mu, uint64_t seed, int64_t step, int tube, int Q) {
+  const double L = kaos_exp_neg(-mu);
+  double prod = 1.0, k = -1.0, u[4];
+  int i = 4;
+  uint64_t blk = 2;
+  for (int it = 0; it < KAOS_CX_MAXITER && prod > L; ++it) {
+    if (i == 4) { kaos_cx_block(seed, step, tube, Q, blk++, u); i = 0; }
+    prod = prod * u[i++];
+    k = k + 1.0;
+  }
+  return k;
+}
+
+/* Inputs of one cell's collision count at a master step. */
+typedef struct {
+  double m0, m1par, m1p1, m1p2, m2par;  /* moments (filtered, MOMENTFILTERflagT = 1) */
+  double cth, ell;                      /* cos(thetaGCGT), ellGCGT of the cell centre */
+  double cphi, sphi;                    /* tube longitude */
+  double nsneut;                        /* nsnormCNeutGT: neutral macro-particles of the cell */
+  double h, sig_n;                      /* time step; neutral thermal speed sqrt(kB TNeut / mNeut) */
+  double NqRT, d3x, nsnormfac;          /* NqRTp and the cap's normalisation */
+  double nqc;                           /* NqReNormT: the cell's macro-ion count */
+  double sig_scale;                     /* factor on sigma (1: the expression as written) */
+} kaos_cx_in_t;
+
+typedef struct { double sigma, nu, poi; } kaos_cx_out_t;
+
+/* IonNeutralCollisionFrequencyASub for one cell (_8_6_6_1:93-300): sigma;
+ * for a populated cell a neutral velocity of three Gaussians, the ion mean
+ * velocity at the cell centre in Cartesian components, their relative
+ * speed, the Poisson count of mean nsneut sigma v h capped at the cell's
+ * macro-ions; -2 for an empty cell or sigma = 0.  Then the one-rank share of
+ * IonNeutralCollisionFrequencyBSub (_8_6_6_2:35-77): the count, at most the
+ * cell's NqReNormT, 0 unless positive (poi, converted nint(poi) times per
+ * step of the next master interval). */
+KAOS_HD kaos_cx_out_t kaos_cx_cell(const kaos_cx_in_t* I, uint64_t seed, int64_t step, int tube, int Q) {
+  kaos_cx_out_t o;
+  o.sigma = I->sig_scale * kaos_cx_sigma(I->m2par);
+  o.nu = -2.0;
+  if (I->m0 != 0.0) {
+    double u[8];
+    kaos_cx_block(seed, step, tube, Q, 0, u);
+    kaos_cx_block(seed, step, tube, Q, 1, u + 4);
+    const double Vnx = I->sig_n * kaos_gauss(u[0], u[1]);
+    const double Vny = I->sig_n * kaos_gauss(u[2], u[3]);
+    const double Vnz = I->sig_n * kaos_gauss(u[4], u[5]);
+    const double c = I->cth, s = sqrt(1.0 - c * c), sl = sqrt(I->ell);
+    const double prho = (3.0 * I->m1par * c * s + I->m1p1 * (1.0 - 3.0 * (c * c))) / sl;
+    const double Mx = I->cphi * prho - I->m1p2 * I->sphi;
+    const double My = I->sphi * prho + I->m1p2 * I->cphi;
+    const double Mz = I->m1par * (3.0 * (c * c) - 1.0) / sl + 3.0 * I->m1p1 * c * s / sl;
+    const double dx = Mx - Vnx, dy = My - Vny, dz = Mz - Vnz;
+    const double vrel = sqrt((dx * dx + dy * dy) + dz * dz);
+    if (o.sigma != 0.0) {
+      const double mu = ((I->nsneut * o.sigma) * vrel) * I->h;
+      double nu = kaos_poisson(mu, seed, step, tube, Q);
+      const double cap = I->NqRT * (I->d3x / I->nsnormfac);
+      if (nu > cap) nu = cap;
+      o.nu = nu;
+    }
+  }
+  o.poi = 0.0;
+  if (o.nu > 0.0 && I->nqc != 0.0) o.poi = (o.nu <= I->nqc) ? o.nu : I->nqc;
+  return o;
+}
+
+/* Ions converted per step: nint(poi). */
+KAOS_HD int64_t kaos_cx_take(double poi) { return (poi > 0.0) ? (int64_t)floor(poi + 0.5) : 0; }
+
+/* The ENA made from an ion (IonNeutralCollisionsSub, _8_1_1_1:31-58): its
+ * position, and the velocity (Vp, Vq, Vphi) = (Vperp1, b-hat(thetak4) . V,
+ * Vperp2) composed in the dipole frame of the stage-4 point of the ion's
+ * last push (cos(thetak4) = ck4, phik4 = the tube longitude); V = V|| b-hat
+ * is the ion's velocity after that push.  X = (x, y, z, Vx, Vy, Vz). */
+KAOS_HD void kaos_ena_from_ion(double rs, double vs, double v1, double v2, double ck4, double p, double cphi,
+                               double sphi, double X[6]) {
+  double x, y, z, Vx, Vy, Vz;
+  kaos_cartesian(rs, vs, p, cphi, sphi, &x, &y, &z, &Vx, &Vy, &Vz);
+  const double c = ck4, s = sqrt(1.0 - c * c), sl = sqrt(1.0 + 3.0 * (c * c));
+  const double vq = 3.0 * c * s * (Vx * cphi + Vy * sphi) / sl + Vz * (3.0 * (c * c) - 1.0) / sl;
+  const double vrho = (3.0 * vq * c * s + v1 * (1.0 - 3.0 * (c * c))) / sl;
+  X[0] = x; X[1] = y; X[2] = z;
+  X[3] = cphi * vrho - v2 * sphi;
+  X[4] = sphi * vrho + v2 * cphi;
+  X[5] = vq * (3.0 * (c * c) - 1.0) / sl + 3.0 * v1 * c * s / sl;
+}
+
+/* The neutral's gravitational acceleration -G ME / r^2 r-hat
+ * (AGENAx/y/zsub, _KineticMainParams.f90:695-714), 0 with GRAVflag off. */
+KAOS_HD void kaos_ena_accel(double x, double y, double z, int grav, double a[3]) {
+  if (!grav) { a[0] = a[1] = a[2] = 0.0; return; }
+  const double r2 = (x * x + y * y) + z * z, r = sqrt(r2);
+  const double g = (KAOS_GG * KAOS_ME) / r2;
+  a[0] = -g * (x / r); a[1] = -g * (y / r); a[2] = -g * (z / r);
+}
+
+/* q of an ENA at (x, y, z): RE^2 cos(theta) / r^2 (qsub), the coordinate its
+ * config cell is scanned on. */
+KAOS_HD double kaos_ena_q(double x, double y, double z) {
+  const double r = sqrt((x * x + y * y) + z * z);
+  return (KAOS_RE2 * (z / r)) / (r * r);
+}
+
+/* One RK4 step of an ENA with the reference's stages (k2 = k3,
+ * _8_1_KineticRK4Update.f90:29-156): accelerations at x, at x + h/2 V and at
+ * x + h V + h^2/2 a1; no re-projection, phi free. */
+KAOS_HD void kaos_ena_push(double X[6], double h, int grav) {
+  const double h2 = h / 2.0, hsq2 = (h * h) / 2.0, h6 = h / 6.0;
+  double a1[3], a2[3], a4[3], xn[3], vn[3];
+  kaos_ena_accel(X[0], X[1], X[2], grav, a1);
+  kaos_ena_accel(X[0] + h2 * X[3], X[1] + h2 * X[4], X[2] + h2 * X[5], grav, a2);
+  kaos_ena_accel((X[0] + h * X[3]) + hsq2 * a1[0], (X[1] + h * X[4]) + hsq2 * a1[1],
+                 (X[2] + h * X[5]) + hsq2 * a1[2], grav, a4);
+  for (int i = 0; i < 3; ++i) {
+    const double V = X[3 + i];
+    const double k2 = V + h2 * a1[i], k4 = V + h * a2[i];
+    vn[i] = V + h6 * (((a1[i] + 2.0 * a2[i]) + 2.0 * a2[i]) + a4[i]);
+    xn[i] = X[i] + h6 * (((V + 2.0 * k2) + 2.0 * k2) + k4);
+  }
+  for (int i = 0; i < 3; ++i) { X[i] = xn[i]; X[3 + i] = vn[i]; }
 }
 
 #endif /* KAOS_DIPOLE_H */

@@ -71,6 +71,9 @@ struct DevTube {
   kaos_icr_t icrp;
   int nv1, nv2;                             // 3-D velocity grid: Vperp1 / Vperp2 cells (0: 2-D grid)
   long long v1off, v2off, h3off, d3off;     // Vperp1 / Vperp2 edges, 3-D counts, d3v3 offsets
+  int cx, cx_filter, cx_filter_pt, cx_pad;  // charge exchange: on, MOMENTFILTERflagT, M1Perp/M2Par window
+  double sig_n, sig_scale;                  // neutral thermal speed sqrt(kB TNeut / mNeut); cross-section factor
+  long long eoff, ecap;                     // the tube's ENA slots [eoff, eoff + ecap)
 };
 
 struct StepArgs {
@@ -113,6 +116,9 @@ struct StepArgs {
   const double *v1GH, *v2GH;
   unsigned int* hist3;
   const int2* grp3;                          // per binning block: its tiles [x, y), all of one tube
+  // charge exchange: cos(thetak4) of each ion's last push (null without it)
+  double* ck4;                               // binary64 form
+  float* ck4f;                               // mixed form
 };
 
 __device__ __forceinline__ void set_status(unsigned int* st, unsigned int bit) { atomicOr(st, bit); }
@@ -272,7 +278,10 @@ __device__ __forceinline__ int kd_find_qcell_fast(double q, const double* qGH, d
 #ifndef KAOS_FP64_WAVES_EU
 #define KAOS_FP64_WAVES_EU 4
 #endif
-template <int MAXQ>
+// CX: the instantiation launched when a tube has charge exchange; it stores
+// cos(thetak4) of every ion of such a tube (the frame of a conversion at the
+// next step), so the plain push carries no extra store.
+template <int MAXQ, bool CX>
 __global__ void __launch_bounds__(KAOS_FP64_BLOCK, KAOS_FP64_WAVES_EU) k_push_fp64(StepArgs a) {
   constexpr int BLK = KAOS_FP64_BLOCK, PPT = KAOS_TILE / KAOS_FP64_BLOCK;
   __shared__ double s_qGH[MAXQ], s_vGH[KAOS_MAXV + 1];
@@ -289,6 +298,7 @@ __global__ void __launch_bounds__(KAOS_FP64_BLOCK, KAOS_FP64_WAVES_EU) k_push_fp
   uint32_t tcount = 0;
   long long toff = 0;
   double *rsT = nullptr, *vpT = nullptr, *qkT = nullptr, *aeT = nullptr, *agT = nullptr;
+  double* ck4T = nullptr;                  // CX instantiation: cos(thetak4) of a charge-exchange tube
   uint32_t* cellT = nullptr;
   double qGL0 = 0.0, qGHlast = 0.0, qsms = 0.0, vscale = 0.0;
   int eambsign = 0;
@@ -345,6 +355,7 @@ __global__ void __launch_bounds__(KAOS_FP64_BLOCK, KAOS_FP64_WAVES_EU) k_push_fp
       tcount = (uint32_t)a.count[tb];
       rsT = a.rs + toff; vpT = a.vp + toff; qkT = a.qk + toff; aeT = a.ae + toff; agT = a.ag + toff;
       cellT = a.cell + toff;
+      if (CX) ck4T = T.cx ? a.ck4 + toff : nullptr;
       vscale = (double)nvpar / (a.vparGH[T.voff + nvpar - 1] - a.vparGH[T.voff + nvpar]);
       cur = tb;
       __syncthreads();
@@ -436,6 +447,7 @@ __global__ void __launch_bounds__(KAOS_FP64_BLOCK, KAOS_FP64_WAVES_EU) k_push_fp
       kx_at(rsT, b8) = o.rsN;
       kx_at(vpT, b8) = o.vparN;
       if (sflags & 8) kx_at(qkT, b8) = o.qk4;
+      if (CX && ck4T) kx_at(ck4T, b8) = o.zC / sqrt(o.u2C);
       kx_at(cellT, b4) = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip;
     }
   }
@@ -968,6 +980,10 @@ __global__ void k_inject(StepArgs a, const long long* inj_base, const long long*
   }
   a.cell[j] = ((uint32_t)(qcell + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip | flags;
   a.id[j] = next_id[t] + k;
+  // charge exchange: a conversion before the ion's first push takes the frame
+  // of its injection point (the reference leaves thetak4 of a new slot unset)
+  if (a.ck4) a.ck4[j] = L.c;
+  if (a.ck4f) a.ck4f[j] = (float)L.c;
   if (a.hist3) {
     // 3-D grid: the cells of the injected velocity (KSB:339-476), 0 off the grid
     if (qcell >= 1) {
@@ -1399,6 +1415,8 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_reorder_scatter(StepArgs a, Step
     }
     b.cell[d] = a.cell[j];
     if (a.c12) b.c12[d] = a.c12[j];
+    if (a.ck4) b.ck4[d] = a.ck4[j];
+    if (a.ck4f) b.ck4f[d] = a.ck4f[j];
     b.id[d] = a.id[j];
   }
 }
@@ -1433,6 +1451,229 @@ __global__ void k_reconstruct(DevTube T, long long n, StepArgs a, int mixed, dou
   out6[6 * i + 3] = vx; out6[6 * i + 4] = vy; out6[6 * i + 5] = vz;
 }
 
+// ---------------------------------------------------------------------------
+// Ion-neutral charge exchange (QEXCHANGEflagT = 1).
+//
+// Collision counts: k_cx_freq, one block per tube, at step 1 and every master
+// step after the moments (IonNeutralCollisionFrequencyA/BSub through
+// kaos_cx_cell).  Conversion, at the start of every later step: the first
+// nint(poi) live ions of each cell in id order become ENAs (_8_KineticSolver.f90:
+// 95-166).  The id threshold of each cell is found by a radix select over
+// the 40-bit ids, four 10-bit digits: per digit a histogram pass over the
+// ions (k_cx_hist) and one wavefront per cell that finds the digit holding
+// the remaining count (k_cx_scan); a cell with no more ions than its count
+// takes all.  k_cx_convert then moves every ion at or below its cell's
+// threshold to the tube's ENA slots (IonNeutralCollisionsSub) and frees its
+// slot.  ENAs: k_ena_step removes those that left at LB/UB (at master
+// steps), finds each one's cell at its pre-push position, counts them per
+// cell at master steps and pushes them ballistically.
+
+#define KAOS_ENA_DEAD ((int)0x80000000)
+#define KAOS_CX_BINS 1024
+#define KAOS_CX_LEVELS 4
+
+// (signed radius, V||) of slot j of tube T in either form; false for a free slot.
+__device__ __forceinline__ bool kd_line_state(const StepArgs& a, const DevTube& T, long long j, int mixed,
+                                              double& r, double& v) {
+  if (mixed) {
+    const uint32_t cw = a.cell[j];
+    if (cw == KAOS_DEAD) return false;
+    const int Q = kx_refq((int)((cw >> 16) & KAOS_QMASK) - 1, T.nq);
+    r = a.rref[T.goff + Q - 1] + kx_dec(a.rdi[j], a.lo[j], a.rk[T.goff + Q - 1], (cw & KAOS_RFLAG) != 0);
+    v = kx_dec_v(a.vpi[j], a.lo[j] >> 16, (int)(cw & 0xFFu), (cw & KAOS_VFLAG) != 0, a.vparGH + T.voff,
+                 a.vk + T.voff, T.nvpar);
+    return true;
+  }
+  r = a.rs[j];
+  v = a.vp[j];
+  return r != 0.0;
+}
+
+__global__ void k_cx_freq(const DevTube* tubes, const double* M3, const double* NqRT, const unsigned int* nqc,
+                          const double* d3x, const double* cxgeo, const double* nsneut, double* cx,
+                          const long long* stepno, unsigned long long seed) {
+  __shared__ double s_f[4][KAOS_MAXQ];
+  const int t = blockIdx.x;
+  const DevTube& T = tubes[t];
+  if (!T.cx) return;
+  const int nq = T.nq;
+  const double* M = M3 + 8 * T.goff;
+  const double *m1par = M + 3 * nq, *m1p1 = M + nq, *m1p2 = M + 2 * nq, *m2par = M + 7 * nq;
+  if (T.cx_filter) {
+    for (int Q = threadIdx.x; Q < nq; Q += blockDim.x) {
+      s_f[0][Q] = kaos_moment_filter_cell(m1par, nq, T.filter_pt_m1, Q + 1);
+      s_f[1][Q] = kaos_moment_filter_cell(m1p1, nq, T.cx_filter_pt, Q + 1);
+      s_f[2][Q] = kaos_moment_filter_cell(m1p2, nq, T.cx_filter_pt, Q + 1);
+      s_f[3][Q] = kaos_moment_filter_cell(m2par, nq, T.cx_filter_pt, Q + 1);
+    }
+    __syncthreads();
+    m1par = s_f[0]; m1p1 = s_f[1]; m1p2 = s_f[2]; m2par = s_f[3];
+  }
+  const long long n = *stepno;
+  for (int Q = threadIdx.x; Q < nq; Q += blockDim.x) {
+    const long long g = T.goff + Q;
+    kaos_cx_in_t I;
+    I.m0 = M[Q]; I.m1par = m1par[Q]; I.m1p1 = m1p1[Q]; I.m1p2 = m1p2[Q]; I.m2par = m2par[Q];
+    if (T.cx_filter && I.m0 == 0.0) I.m2par = 0.0;     // sigma is set only for populated cells
+    I.cth = cxgeo[2 * g]; I.ell = cxgeo[2 * g + 1]; I.cphi = T.cphi; I.sphi = T.sphi;
+    I.nsneut = nsneut[g]; I.h = T.h; I.sig_n = T.sig_n;
+    I.NqRT = NqRT[g]; I.d3x = d3x[g]; I.nsnormfac = T.nsnormfac; I.nqc = (double)nqc[g];
+    I.sig_scale = T.sig_scale;
+    const kaos_cx_out_t r = kaos_cx_cell(&I, seed, n, t, Q + 1);
+    cx[3 * T.goff + Q] = r.sigma;
+    cx[3 * T.goff + nq + Q] = r.nu;
+    cx[3 * T.goff + 2 * nq + Q] = r.poi;
+  }
+}
+
+__global__ void k_cx_sel_init(int ncell, const DevTube* tubes, const int* blk_tube, const int* blk_q,
+                              const double* cx, long long* need, long long* pref, long long* thr) {
+  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= ncell) return;
+  const DevTube& T = tubes[blk_tube[ci]];
+  const int Q = blk_q[ci];
+  const long long g = T.goff + Q;
+  need[g] = T.cx ? kaos_cx_take(cx[3 * T.goff + 2 * T.nq + Q]) : 0;
+  pref[g] = 0;
+  thr[g] = -1;
+}
+
+// The cell (Qindk1) of a live ion of a charge-exchange tube whose cell still
+// selects, or 0.
+__device__ __forceinline__ int kd_cx_cell(const StepArgs& a, const DevTube& T, long long j, int mixed,
+                                          const long long* need) {
+  const uint32_t cw = a.cell[j];
+  if (mixed ? (cw == KAOS_DEAD) : (a.rs[j] == 0.0)) return 0;
+  const int qc = (int)((cw >> 16) & KAOS_QMASK) - 1;
+  if (qc < 1) return 0;
+  return need[T.goff + qc - 1] > 0 ? qc : 0;
+}
+
+__global__ void __launch_bounds__(KAOS_BLOCK) k_cx_hist(StepArgs a, int mixed, int level, const long long* need,
+                                                        const long long* pref, unsigned int* hist) {
+  const int tile = blockIdx.x, tb = a.tile_tube[tile];
+  const DevTube& T = a.tubes[tb];
+  if (!T.cx) return;
+  const long long end = T.off + a.count[tb];
+  const int sh = 30 - 10 * level;
+  for (int it = 0; it < KAOS_TILE_PPT; ++it) {
+    const long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
+    if (j >= end) break;
+    const int qc = kd_cx_cell(a, T, j, mixed, need);
+    if (!qc) continue;
+    const long long g = T.goff + qc - 1;
+    const long long id = a.id[j];
+    if (level == 0 && (id >> 40) != 0) set_status(a.status, 1u << KAOS_E_CAPACITY);   // beyond 40-bit ids
+    if (level > 0 && (id >> (sh + 10)) != pref[g]) continue;
+    atomicAdd(&hist[g * KAOS_CX_BINS + ((id >> sh) & (KAOS_CX_BINS - 1))], 1u);
+  }
+}
+
+// One wavefront per cell: lane l holds digits 16 l .. 16 l + 15.
+__global__ void k_cx_scan(int ncell, int level, long long* need, long long* pref, long long* thr,
+                          unsigned int* hist) {
+  const int lane = threadIdx.x & 63;
+  const long long g = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (g >= ncell) return;
+  const long long k = need[g];
+  if (k <= 0) return;
+  unsigned int* h = hist + g * KAOS_CX_BINS + 16 * lane;
+  unsigned int v[16];
+  unsigned long long sum = 0;
+  for (int i = 0; i < 16; ++i) { v[i] = h[i]; sum += v[i]; h[i] = 0u; }
+  unsigned long long incl = sum;
+  for (int st = 1; st < 64; st <<= 1) {
+    const unsigned long long y = __shfl_up(incl, st, 64);
+    if (lane >= st) incl += y;
+  }
+  const unsigned long long total = __shfl(incl, 63, 64);
+  if (level == 0 && total <= (unsigned long long)k) {         // every ion of the cell converts
+    if (lane == 0) { thr[g] = 0x7FFFFFFFFFFFFFFFLL; need[g] = 0; }
+    return;
+  }
+  const unsigned long long excl = incl - sum;
+  if (excl < (unsigned long long)k && (unsigned long long)k <= incl) {   // exactly one lane
+    unsigned long long c = excl;
+    int d = 0;
+    for (; d < 16; ++d) {
+      if (c + v[d] >= (unsigned long long)k) break;
+      c += v[d];
+    }
+    const long long np = (pref[g] << 10) | (long long)(16 * lane + d);
+    pref[g] = np;
+    if (level == KAOS_CX_LEVELS - 1) { thr[g] = np; need[g] = 0; }
+    else need[g] = k - (long long)c;
+  }
+}
+
+__global__ void __launch_bounds__(KAOS_BLOCK) k_cx_convert(StepArgs a, int mixed, const long long* thr, double* ena,
+                                                           long long* eid, int* eq, long long* ena_cnt,
+                                                           long long ena_tot, long long* live) {
+  const int tile = blockIdx.x, tb = a.tile_tube[tile];
+  const DevTube& T = a.tubes[tb];
+  if (!T.cx) return;
+  const long long end = T.off + a.count[tb];
+  for (int it = 0; it < KAOS_TILE_PPT; ++it) {
+    const long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
+    if (j >= end) break;
+    const uint32_t cw = a.cell[j];
+    double r, v;
+    if (!kd_line_state(a, T, j, mixed, r, v)) continue;
+    const int qc = (int)((cw >> 16) & KAOS_QMASK) - 1;
+    if (qc < 1) continue;
+    const long long th = thr[T.goff + qc - 1];
+    if (th < 0 || a.id[j] > th) continue;
+    const double v1 = mixed ? (double)a.vrf[j] : a.vr[j], v2 = mixed ? (double)a.vrf2[j] : a.vr2[j];
+    const double c4 = mixed ? (double)a.ck4f[j] : a.ck4[j];
+    double X[6];
+    kaos_ena_from_ion(r, v, v1, v2, c4, T.p, T.cphi, T.sphi, X);
+    const long long e = (long long)atomicAdd((unsigned long long*)&ena_cnt[tb], 1ull);
+    if (e >= T.ecap) { set_status(a.status, 1u << KAOS_E_CAPACITY); continue; }
+    const long long k = T.eoff + e;
+    for (int i = 0; i < 6; ++i) ena[(long long)i * ena_tot + k] = X[i];
+    eid[k] = a.id[j];
+    eq[k] = qc;
+    if (mixed) a.cell[j] = KAOS_DEAD;
+    else a.rs[j] = 0.0;
+    atomicAdd((unsigned long long*)&live[tb], (unsigned long long)(-1LL));
+  }
+}
+
+// Start of a master step's ENA outputs: counts per cell and escape counters.
+__global__ void k_ena_reset(int ncell, int ntubes, unsigned int* nq_ena, unsigned long long* ena_esc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ncell) nq_ena[i] = 0u;
+  if (i < 2 * ntubes) ena_esc[i] = 0ull;
+}
+
+__global__ void k_ena_step(const DevTube* tubes, int ntubes, const long long* eoff, const long long* ena_cnt,
+                           long long ena_tot, double* ena, int* eq, const double* qGH, unsigned int* nq_ena,
+                           unsigned long long* ena_esc, int master) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ena_tot) return;
+  int lo = 0, hi = ntubes;                   // the tube whose slots [eoff[t], eoff[t+1]) hold k
+  while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if (eoff[mid] <= k) lo = mid; else hi = mid; }
+  const DevTube& T = tubes[lo];
+  if (!T.cx) return;
+  const long long e = k - T.eoff;
+  if (e >= min(ena_cnt[lo], T.ecap)) return;
+  const int q0 = eq[k];
+  if (q0 == KAOS_ENA_DEAD) return;
+  if (master && (q0 == 0 || q0 == -1)) {     // BC:309-470
+    eq[k] = KAOS_ENA_DEAD;
+    atomicAdd(&ena_esc[2 * lo + (q0 == 0 ? 0 : 1)], 1ull);
+    return;
+  }
+  double X[6];
+  for (int i = 0; i < 6; ++i) X[i] = ena[(long long)i * ena_tot + k];
+  const double q = kaos_ena_q(X[0], X[1], X[2]);
+  const int Q = (q == q) ? kaos_find_qcell(q, qGH + T.goff, T.qGL0, T.qGHlast, T.nq, T.desc) : 0;
+  eq[k] = Q;
+  if (master && Q >= 1) atomicAdd(&nq_ena[T.goff + Q - 1], 1u);
+  kaos_ena_push(X, T.h, T.grav);
+  for (int i = 0; i < 6; ++i) ena[(long long)i * ena_tot + k] = X[i];
+}
+
 // ===========================================================================
 // Host side.
 
@@ -1445,6 +1686,7 @@ struct HostTube {
   std::vector<double> v1GL, v1GH, v1GC, dv1, v2GL, v2GH, v2GC, dv2, d3v3;   // 3-D velocity grid
   std::vector<double> ea_in;                // input field (kaos_set_field), nq values
   int ea_mode = 0;                          // KAOS_FIELD_*
+  std::vector<double> nsneut;               // charge exchange: nsnormCNeutGT per cell
 };
 
 // One set of particle-state arrays (the binary64 or the mixed form's words).
@@ -1458,6 +1700,8 @@ struct StateBufs {
   uint32_t* cell = nullptr;
   uint16_t* c12 = nullptr;
   long long* id = nullptr;
+  double* ck4 = nullptr;                     // charge exchange: cos(thetak4), binary64 form
+  float* ck4f = nullptr;                     // the same, mixed form
 };
 
 struct kaos_ctx {
@@ -1506,6 +1750,22 @@ struct kaos_ctx {
   unsigned int* d_work = nullptr;
   long long* d_live = nullptr;                  // live ions per tube (kaos_count)
   int any_icr = 0;                              // some tube has ICR heating on
+  // charge exchange (some tube has kaos_grid.cx): per cell nsnormCNeutGT,
+  // cos(thetaGCGT) and ellGCGT of the centre, sigma / nu / poi of the last
+  // master step, the id selection's remaining count, prefix and threshold
+  // and its 1024-bin digit histogram; the ENAs (SoA: x, y, z, Vx, Vy, Vz;
+  // id; Qindk1, KAOS_ENA_DEAD for a free slot), their slots used per tube,
+  // master-step counts per cell and LB/UB escapes per tube
+  int any_cx = 0;
+  long long ena_tot = 0;
+  double *d_nsneut = nullptr, *d_cxgeo = nullptr, *d_cx = nullptr;
+  long long *d_cx_need = nullptr, *d_cx_pref = nullptr, *d_cx_thr = nullptr;
+  unsigned int* d_cx_hist = nullptr;
+  double* d_ena = nullptr;
+  long long *d_eid = nullptr, *d_ena_cnt = nullptr, *d_eoff = nullptr;
+  int* d_eq = nullptr;
+  unsigned int* d_nq_ena = nullptr;
+  unsigned long long* d_ena_esc = nullptr;
   int4* d_vbox = nullptr;                       // per tube velocity box of the push's LDS window
   int4* d_hbox = nullptr;                       // per (tube, cell) bounding box of the master step's counts
   unsigned int *d_vmarg_par = nullptr, *d_vmarg_perp = nullptr;
@@ -1662,9 +1922,24 @@ extern "C" int kaos_set_grid(kaos_ctx* c, int32_t tube, const kaos_grid* g, int6
     for (int i = 0; i < g->nvperp2; ++i)
       if (!(g->vperp2GL[i] < g->vperp2GH[i])) return fail(c, KAOS_E_ARG, "Vperp2 cells not increasing");
   }
+  if (g->cx) {
+    if (!v3d)
+      return fail(c, KAOS_E_ARG, "charge exchange needs KAOS_FLAG_VPERP3D (its ion mean velocity reads M1Perp1/M1Perp2)");
+    if (!g->ns_neut) return fail(c, KAOS_E_ARG, "charge exchange needs ns_neut");
+    if (!(g->t_neut > 0.0) || !(g->m_neut > 0.0) || !(g->cx_sigma_scale >= 0.0) || !std::isfinite(g->cx_sigma_scale))
+      return fail(c, KAOS_E_ARG, "charge exchange needs t_neut > 0, m_neut > 0 and a finite cx_sigma_scale >= 0");
+    if (g->cx_filter && (g->cx_filter_pt < 3 || (g->cx_filter_pt % 2) == 0 || (g->cx_filter_pt - 1) / 2 + 1 > g->nq / 2))
+      return fail(c, KAOS_E_ARG, "cx_filter_pt must be odd, >= 3 and fit the grid");
+    if (g->ena_cap < 0 || g->ena_cap >= (1LL << 28)) return fail(c, KAOS_E_ARG, "ena_cap must be in [0, 2^28)");
+    for (int i = 0; i < g->nq; ++i)
+      if (!(g->ns_neut[i] >= 0.0) || !std::isfinite(g->ns_neut[i]))
+        return fail(c, KAOS_E_ARG, "ns_neut must be finite and >= 0");
+  }
   HostTube& h = c->ht[tube];
   h.g = *g;
   int nq = g->nq, np = g->nvpar, nr = g->nvperp;
+  if (g->cx) h.nsneut.assign(g->ns_neut, g->ns_neut + nq);
+  else h.nsneut.clear();
   if (v3d) {
     const int n1 = g->nvperp1, n2 = g->nvperp2;
     h.v1GL.assign(g->vperp1GL, g->vperp1GL + n1); h.v1GH.assign(g->vperp1GH, g->vperp1GH + n1);
@@ -1725,6 +2000,15 @@ static int build_device(kaos_ctx* c) {
     T.icr = g.icr ? 1 : 0;
     if (T.icr) c->any_icr = 1;
     T.ea_in = h.ea_mode;
+    if (g.cx) {
+      T.cx = 1; T.cx_filter = g.cx_filter ? 1 : 0;
+      T.cx_filter_pt = g.cx_filter_pt;
+      T.sig_n = sqrt((KAOS_KB * g.t_neut) / g.m_neut);
+      T.sig_scale = g.cx_sigma_scale;
+      T.eoff = c->ena_tot; T.ecap = g.ena_cap;
+      c->ena_tot += g.ena_cap;
+      c->any_cx = 1;
+    }
     T.icrp.qs = g.qs; T.icrp.ms = g.ms; T.icrp.h = g.h;
     T.icrp.lambda_perp = g.icr_lambda_perp; T.icrp.eta_lh = g.icr_eta_lh;
     T.icrp.xi1 = g.icr_xi_perp1; T.icrp.xi2 = g.icr_xi_perp2; T.icrp.s0 = g.icr_s0;
@@ -1856,6 +2140,15 @@ static int build_device(kaos_ctx* c) {
       HIPCHK(c, hipMalloc((void**)&B.c12, sizeof(uint16_t) * nslot));
       HIPCHK(c, hipMemset(B.c12, 0, sizeof(uint16_t) * nslot));
     }
+    if (c->any_cx) {
+      if (mixed) {
+        HIPCHK(c, hipMalloc((void**)&B.ck4f, sizeof(float) * nslot));
+        HIPCHK(c, hipMemset(B.ck4f, 0, sizeof(float) * nslot));
+      } else {
+        HIPCHK(c, hipMalloc((void**)&B.ck4, sizeof(double) * nslot));
+        HIPCHK(c, hipMemset(B.ck4, 0, sizeof(double) * nslot));
+      }
+    }
   }
   c->nkeys_tot = goff + 2LL * nt;
   HIPCHK(c, hipMalloc((void**)&c->d_keycnt, sizeof(unsigned int) * c->nkeys_tot));
@@ -1938,6 +2231,38 @@ static int build_device(kaos_ctx* c) {
   std::vector<unsigned long long> ze(2 * nt, 0ull);
   HIPCHK(c, up((void**)&c->d_esc_acc, ze.data(), sizeof(unsigned long long) * 2 * nt));
   HIPCHK(c, up((void**)&c->d_esc_rep, ze.data(), sizeof(unsigned long long) * 2 * nt));
+  if (c->any_cx) {
+    std::vector<double> nsn, geo;
+    std::vector<long long> eoff(nt + 1, 0);
+    for (int t = 0; t < nt; ++t) {
+      const HostTube& h = c->ht[t];
+      for (int i = 0; i < h.g.nq; ++i) {
+        nsn.push_back(h.nsneut.empty() ? 0.0 : h.nsneut[i]);
+        geo.push_back(h.cth[i]); geo.push_back(h.ell[i]);
+      }
+      eoff[t] = c->dt[t].eoff;
+    }
+    eoff[nt] = c->ena_tot;
+    const size_t ne = (size_t)std::max<long long>(c->ena_tot, 1);
+    HIPCHK(c, up((void**)&c->d_nsneut, nsn.data(), sizeof(double) * goff));
+    HIPCHK(c, up((void**)&c->d_cxgeo, geo.data(), sizeof(double) * 2 * goff));
+    HIPCHK(c, up((void**)&c->d_eoff, eoff.data(), sizeof(long long) * (nt + 1)));
+    HIPCHK(c, hipMalloc((void**)&c->d_cx, sizeof(double) * 3 * goff));
+    HIPCHK(c, hipMemset(c->d_cx, 0, sizeof(double) * 3 * goff));
+    HIPCHK(c, hipMalloc((void**)&c->d_cx_need, sizeof(long long) * goff));
+    HIPCHK(c, hipMalloc((void**)&c->d_cx_pref, sizeof(long long) * goff));
+    HIPCHK(c, hipMalloc((void**)&c->d_cx_thr, sizeof(long long) * goff));
+    HIPCHK(c, hipMalloc((void**)&c->d_cx_hist, sizeof(unsigned int) * 1024 * goff));
+    HIPCHK(c, hipMemset(c->d_cx_hist, 0, sizeof(unsigned int) * 1024 * goff));
+    HIPCHK(c, hipMalloc((void**)&c->d_ena, sizeof(double) * 6 * ne));
+    HIPCHK(c, hipMalloc((void**)&c->d_eid, sizeof(long long) * ne));
+    HIPCHK(c, hipMalloc((void**)&c->d_eq, sizeof(int) * ne));
+    HIPCHK(c, hipMemset(c->d_eq, 0xFF, sizeof(int) * ne));       // never used: see k_ena_step
+    HIPCHK(c, up((void**)&c->d_ena_cnt, zc.data(), sizeof(long long) * nt));
+    HIPCHK(c, hipMalloc((void**)&c->d_nq_ena, sizeof(unsigned int) * goff));
+    HIPCHK(c, hipMemset(c->d_nq_ena, 0, sizeof(unsigned int) * goff));
+    HIPCHK(c, up((void**)&c->d_ena_esc, ze.data(), sizeof(unsigned long long) * 2 * nt));
+  }
   // injection bookkeeping: ions per tube per master step and their prefix
   std::vector<long long> ib(nt + 1, 0), ic(nt, 0);
   for (int t = 0; t < nt; ++t) {
@@ -1960,6 +2285,7 @@ static void set_state(StepArgs& a, const StateBufs& B) {
   a.rs = B.rs; a.vp = B.vp; a.vr = B.vr; a.qk = B.qk; a.ae = B.ae; a.ag = B.ag; a.vr2 = B.vr2;
   a.rdi = B.rdi; a.vpi = B.vpi; a.lo = B.lo; a.vrf = B.vrf; a.qkf = B.qkf; a.aef = B.aef; a.agf = B.agf; a.vrf2 = B.vrf2;
   a.cell = B.cell; a.id = B.id; a.c12 = B.c12;
+  a.ck4 = B.ck4; a.ck4f = B.ck4f;
 }
 
 static StepArgs make_args(kaos_ctx* c, int interp, int count, int remove, int qk) {
@@ -2166,23 +2492,29 @@ static void launch_push(kaos_ctx* c, const StepArgs& a, hipEvent_t tune0 = nullp
   if (c->cfg.precision == KAOS_MIXED) {
     // the heating instantiation only when a tube heats (its extra registers
     // never cost the plain push)
-    if (c->lds_nq <= 256 && !c->any_icr)
+    if (c->lds_nq <= 256 && !c->any_icr && !c->any_cx)
       hipLaunchKernelGGL((k_push_mixed<256, false>), dim3(nblocks), dim3(KAOS_MIXED_BLOCK), push_lds_bytes(c),
                          c->stream, a);
     else if (c->lds_nq <= 256)
       hipLaunchKernelGGL((k_push_mixed<256, true>), dim3(nblocks), dim3(KAOS_MIXED_BLOCK), push_lds_bytes(c),
                          c->stream, a);
-    else if (!c->any_icr)
+    else if (!c->any_icr && !c->any_cx)
       hipLaunchKernelGGL((k_push_mixed<KAOS_MAXQ, false>), dim3(nblocks), dim3(KAOS_MIXED_BLOCK), push_lds_bytes(c),
                          c->stream, a);
     else
       hipLaunchKernelGGL((k_push_mixed<KAOS_MAXQ, true>), dim3(nblocks), dim3(KAOS_MIXED_BLOCK), push_lds_bytes(c),
                          c->stream, a);
   }
+  else if (c->lds_nq <= 256 && !c->any_cx)
+    hipLaunchKernelGGL((k_push_fp64<256, false>), dim3(nblocks), dim3(KAOS_FP64_BLOCK), push_lds_bytes(c), c->stream, a);
   else if (c->lds_nq <= 256)
-    hipLaunchKernelGGL(k_push_fp64<256>, dim3(nblocks), dim3(KAOS_FP64_BLOCK), push_lds_bytes(c), c->stream, a);
+    hipLaunchKernelGGL((k_push_fp64<256, true>), dim3(nblocks), dim3(KAOS_FP64_BLOCK), push_lds_bytes(c), c->stream, a);
+  else if (!c->any_cx)
+    hipLaunchKernelGGL((k_push_fp64<KAOS_MAXQ, false>), dim3(nblocks), dim3(KAOS_FP64_BLOCK), push_lds_bytes(c),
+                       c->stream, a);
   else
-    hipLaunchKernelGGL(k_push_fp64<KAOS_MAXQ>, dim3(nblocks), dim3(KAOS_FP64_BLOCK), push_lds_bytes(c), c->stream, a);
+    hipLaunchKernelGGL((k_push_fp64<KAOS_MAXQ, true>), dim3(nblocks), dim3(KAOS_FP64_BLOCK), push_lds_bytes(c),
+                       c->stream, a);
   if (tune1) (void)hipEventRecord(tune1, c->stream);
   if (prof) { (void)hipEventRecord(ev.second, c->stream); c->ev_push.push_back(ev); }
 }
@@ -2285,6 +2617,36 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
     hipLaunchKernelGGL(k_hbox_reset, dim3((c->nblk_mom + 255) / 256), dim3(256), 0, c->stream, c->nblk_mom,
                        c->d_tubes, c->d_blk_tube, c->d_blk_q, c->d_vbox, c->d_hbox);
   };
+  // charge exchange: counts of a master step, ENA outputs reset, conversions, ENA push
+  const int ncell = (int)c->ncell_tot;
+  auto cx_freq = [&]() {
+    hipLaunchKernelGGL(k_cx_freq, dim3(nt), dim3(256), 0, c->stream, c->d_tubes, c->d_M3, c->d_NqRT, c->d_nqc,
+                       c->d_d3x, c->d_cxgeo, c->d_nsneut, c->d_cx, c->d_stepno, (unsigned long long)c->cfg.seed);
+  };
+  auto ena_reset = [&]() {
+    const int nr = std::max(ncell, 2 * nt);
+    hipLaunchKernelGGL(k_ena_reset, dim3((nr + 255) / 256), dim3(256), 0, c->stream, ncell, nt, c->d_nq_ena,
+                       c->d_ena_esc);
+  };
+  auto cx_convert = [&]() {
+    StepArgs a = make_args(c, 0, 0, 0, 0);
+    hipLaunchKernelGGL(k_cx_sel_init, dim3((ncell + 255) / 256), dim3(256), 0, c->stream, ncell, c->d_tubes,
+                       c->d_blk_tube, c->d_blk_q, c->d_cx, c->d_cx_need, c->d_cx_pref, c->d_cx_thr);
+    for (int L = 0; L < KAOS_CX_LEVELS; ++L) {
+      hipLaunchKernelGGL(k_cx_hist, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a, mixed, L, c->d_cx_need,
+                         c->d_cx_pref, c->d_cx_hist);
+      hipLaunchKernelGGL(k_cx_scan, dim3((ncell + 3) / 4), dim3(256), 0, c->stream, ncell, L, c->d_cx_need,
+                         c->d_cx_pref, c->d_cx_thr, c->d_cx_hist);
+    }
+    hipLaunchKernelGGL(k_cx_convert, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a, mixed, c->d_cx_thr,
+                       c->d_ena, c->d_eid, c->d_eq, c->d_ena_cnt, c->ena_tot, c->d_live);
+  };
+  auto ena_step = [&](int m) {
+    if (c->ena_tot > 0)
+      hipLaunchKernelGGL(k_ena_step, dim3((unsigned)((c->ena_tot + 255) / 256)), dim3(256), 0, c->stream,
+                         c->d_tubes, nt, c->d_eoff, c->d_ena_cnt, c->ena_tot, c->d_ena, c->d_eq, c->d_qGH,
+                         c->d_nq_ena, c->d_ena_esc, m);
+  };
   if (first) {
     StepArgs a = make_args(c, 1, 0, 0, premaster);
     zero_hist();
@@ -2296,11 +2658,17 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
     hipLaunchKernelGGL(k_field, dim3(nt), dim3(256), 0, c->stream, c->d_tubes, c->d_hq, c->d_dq, c->d_Te,
                        c->d_M, c->d_M0F, c->d_M1F, c->d_EA, c->d_EAin, c->d_esc_acc, c->d_esc_rep, 0);
     launch_moments(c, nullptr);
+    if (c->any_cx) { ena_reset(); cx_freq(); }
     launch_box_update(c);
     if (c->any_icr && c->cfg.precision != KAOS_MIXED)
       hipLaunchKernelGGL(k_icr, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a);
     launch_push(c, a);
   } else {
+    if (c->any_cx) {
+      cx_convert();
+      if (master) ena_reset();
+      ena_step(master ? 1 : 0);
+    }
     if (master) {
       hipLaunchKernelGGL(k_field, dim3(nt), dim3(256), 0, c->stream, c->d_tubes, c->d_hq, c->d_dq, c->d_Te,
                          c->d_M, c->d_M0F, c->d_M1F, c->d_EA, c->d_EAin, c->d_esc_acc, c->d_esc_rep, 1);
@@ -2325,6 +2693,7 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
                            c->d_stepno, c->inj_total, mixed);
       }
       launch_moments(c, c->d_inj_cnt);
+      if (c->any_cx) cx_freq();
     }
   }
   HIPCHK(c, hipGetLastError());
@@ -2852,6 +3221,90 @@ extern "C" int kaos_set_field(kaos_ctx* c, int32_t tube, const double* EA, int32
   return KAOS_OK;
 }
 
+// Charge-exchange contexts keep ENAs and collision counts that the handover
+// and restart files do not hold.
+static bool any_cx_tube(const kaos_ctx* c) {
+  for (const HostTube& h : c->ht) if (h.set && h.g.cx) return true;
+  return false;
+}
+
+extern "C" int kaos_download_cx(kaos_ctx* c, int32_t tube, double* sigma, double* nu, double* poi) {
+  if (!c) return KAOS_E_ARG;
+  if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
+  if (!c->built || !c->dt[tube].cx) return fail(c, KAOS_E_STATE, "no charge exchange in this tube");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const DevTube& T = c->dt[tube];
+  const size_t nb = sizeof(double) * T.nq;
+  if (sigma) HIPCHK(c, hipMemcpy(sigma, c->d_cx + 3 * T.goff, nb, hipMemcpyDeviceToHost));
+  if (nu) HIPCHK(c, hipMemcpy(nu, c->d_cx + 3 * T.goff + T.nq, nb, hipMemcpyDeviceToHost));
+  if (poi) HIPCHK(c, hipMemcpy(poi, c->d_cx + 3 * T.goff + 2 * T.nq, nb, hipMemcpyDeviceToHost));
+  return KAOS_OK;
+}
+
+extern "C" int kaos_download_enas(kaos_ctx* c, int32_t tube, int64_t n_cap, int64_t* n_out, double* x, double* y,
+                                  double* z, double* Vx, double* Vy, double* Vz, int64_t* id, int64_t* qind) {
+  if (!c || !n_out) return KAOS_E_ARG;
+  if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
+  *n_out = 0;
+  if (!c->built) return KAOS_OK;
+  const DevTube& T = c->dt[tube];
+  if (!T.cx) return KAOS_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  long long used = 0;
+  HIPCHK(c, hipMemcpy(&used, c->d_ena_cnt + tube, sizeof(long long), hipMemcpyDeviceToHost));
+  used = std::min(used, T.ecap);
+  std::vector<double> X(6 * (size_t)std::max(used, 1LL));
+  std::vector<long long> ids(std::max(used, 1LL));
+  std::vector<int> q(std::max(used, 1LL));
+  for (int i = 0; i < 6 && used; ++i)
+    HIPCHK(c, hipMemcpy(X.data() + (size_t)i * used, c->d_ena + (size_t)i * c->ena_tot + T.eoff,
+                        sizeof(double) * used, hipMemcpyDeviceToHost));
+  if (used) {
+    HIPCHK(c, hipMemcpy(ids.data(), c->d_eid + T.eoff, sizeof(long long) * used, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(q.data(), c->d_eq + T.eoff, sizeof(int) * used, hipMemcpyDeviceToHost));
+  }
+  std::vector<long long> ord;
+  for (long long e = 0; e < used; ++e) if (q[e] != KAOS_ENA_DEAD) ord.push_back(e);
+  std::sort(ord.begin(), ord.end(), [&](long long u, long long v) { return ids[u] < ids[v]; });
+  const long long n = (long long)ord.size();
+  if (n > n_cap) return fail(c, KAOS_E_ARG, "n_cap smaller than the tube's " + std::to_string(n) + " ENAs");
+  double* out[6] = {x, y, z, Vx, Vy, Vz};
+  for (long long i = 0; i < n; ++i) {
+    const long long e = ord[i];
+    for (int k = 0; k < 6; ++k) if (out[k]) out[k][i] = X[(size_t)k * used + e];
+    if (id) id[i] = ids[e];
+    if (qind) qind[i] = q[e];
+  }
+  *n_out = n;
+  return KAOS_OK;
+}
+
+extern "C" int kaos_download_ena_master(kaos_ctx* c, int32_t tube, int64_t* nq_ena, double* esc) {
+  if (!c) return KAOS_E_ARG;
+  if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
+  if (!c->built || !c->dt[tube].cx) return fail(c, KAOS_E_STATE, "no charge exchange in this tube");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const DevTube& T = c->dt[tube];
+  const HostTube& h = c->ht[tube];
+  if (nq_ena) {
+    std::vector<unsigned int> u(T.nq);
+    HIPCHK(c, hipMemcpy(u.data(), c->d_nq_ena + T.goff, sizeof(unsigned int) * T.nq, hipMemcpyDeviceToHost));
+    for (int Q = 0; Q < T.nq; ++Q) nq_ena[Q] = u[Q];
+  }
+  if (esc) {
+    unsigned long long e2[2];
+    HIPCHK(c, hipMemcpy(e2, c->d_ena_esc + 2 * tube, sizeof(e2), hipMemcpyDeviceToHost));
+    // fluxes as BoundaryConditionsSub writes them (BC:355-366): count nsnormfac over
+    // (d3x / (hq dq)) of the boundary cell times ndatfac h
+    const int ub = T.nq - 1;
+    const double dtm = (double)c->cfg.ndatfac * T.h;
+    esc[0] = (double)e2[0]; esc[1] = (double)e2[1];
+    esc[2] = ((double)e2[0] * T.nsnormfac) / ((h.d3x[0] / (h.hq[0] * h.dq[0])) * dtm);
+    esc[3] = ((double)e2[1] * T.nsnormfac) / ((h.d3x[ub] / (h.hq[ub] * h.dq[ub])) * dtm);
+  }
+  return KAOS_OK;
+}
+
 // The live ions of a tube on the host, in the reference's order (survivors
 // in order, then injected ions: by id), with their Cartesian state.
 struct TubeDump {
@@ -2970,6 +3423,7 @@ extern "C" int kaos_upload_state(kaos_ctx* c, int32_t tube, int64_t n, const dou
                                  const int64_t* vparind, const int64_t* vperpind) {
   if (!c) return KAOS_E_ARG;
   if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
+  if (any_cx_tube(c)) return fail(c, KAOS_E_STATE, "kaos_upload_state: not with charge exchange (ENAs)");
   if (n < 0) return fail(c, KAOS_E_ARG, "negative ion count");
   if (n > 0 && (!x || !y || !z || !Vx || !Vy || !Vz || !vperp || !aea || !ag || !qk4 || !qind || !vparind ||
                 !vperpind))
@@ -3127,6 +3581,7 @@ static std::vector<std::pair<void*, size_t>> ctx_arrays(kaos_ctx* c) {
 extern "C" int kaos_save_state(kaos_ctx* c, const char* path) {
   if (!c || !path) return KAOS_E_ARG;
   if (!c->built || c->last_step < 1) return fail(c, KAOS_E_STATE, "kaos_save_state needs a context that has stepped");
+  if (any_cx_tube(c)) return fail(c, KAOS_E_STATE, "kaos_save_state: not with charge exchange (ENAs)");
   HIPCHK(c, hipStreamSynchronize(c->stream));
   FILE* f = fopen(path, "wb");
   if (!f) return fail(c, KAOS_E_IO, std::string("cannot write ") + path);
@@ -3163,6 +3618,7 @@ extern "C" int kaos_save_state(kaos_ctx* c, const char* path) {
 extern "C" int kaos_load_state(kaos_ctx* c, const char* path) {
   if (!c || !path) return KAOS_E_ARG;
   if (c->last_step != 0) return fail(c, KAOS_E_STATE, "kaos_load_state needs a context that has not stepped");
+  if (any_cx_tube(c)) return fail(c, KAOS_E_STATE, "kaos_load_state: not with charge exchange (ENAs)");
   if (!c->built) { int rc = build_device(c); if (rc) return rc; }
   FILE* f = fopen(path, "rb");
   if (!f) return fail(c, KAOS_E_IO, std::string("cannot read ") + path);
@@ -3284,9 +3740,12 @@ extern "C" int kaos_finalize(kaos_ctx* c) {
   void* b3[] = {c->d_v1GH, c->d_v1GC, c->d_dv1, c->d_v2GH, c->d_v2GC, c->d_dv2, c->d_d3v3, c->d_M3, c->d_hist3,
                 c->d_grp3};
   for (void* p : b3) if (p) (void)hipFree(p);
+  void* bcx[] = {c->d_nsneut, c->d_cxgeo, c->d_cx, c->d_cx_need, c->d_cx_pref, c->d_cx_thr, c->d_cx_hist,
+                 c->d_ena, c->d_eid, c->d_ena_cnt, c->d_eoff, c->d_eq, c->d_nq_ena, c->d_ena_esc};
+  for (void* p : bcx) if (p) (void)hipFree(p);
   for (StateBufs& B : c->st) {
     void* sb[] = {B.rs, B.vp, B.vr, B.qk, B.ae, B.ag, B.rdi, B.vpi, B.lo, B.vrf, B.qkf, B.aef, B.agf, B.cell, B.id,
-                  B.vr2, B.vrf2, B.c12};
+                  B.vr2, B.vrf2, B.c12, B.ck4, B.ck4f};
     for (void* p : sb) if (p) (void)hipFree(p);
   }
   if (c->stream) (void)hipStreamDestroy(c->stream);

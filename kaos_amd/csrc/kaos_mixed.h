@@ -278,6 +278,8 @@ __device__ __forceinline__ void kf_icr_kick(const kaos_icr_t& P, float r, float 
 #ifndef KAOS_MIXED_WAVES_EU
 #define KAOS_MIXED_WAVES_EU 4
 #endif
+// ICR: the instantiation launched when a tube heats or has charge exchange
+// (the plain one carries neither's registers).
 template <int MAXQ, bool ICR>
 __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_mixed(StepArgs a) {
   KAOS_MIXED_CONTRACT
@@ -466,6 +468,12 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
       const float dRC = h * VRR + hsq2 * a1R, dZC = h * VRZ + hsq2 * a1Z;
       const float zC = z + dZC;
       kf_bhat_pt(rho + dRC, zC, b4R, b4Z);
+      // charge exchange (heating/charge-exchange instantiation): cos(thetak4),
+      // the frame of a conversion at the next step
+      if (ICR && a.ck4f && a.tubes[cur].cx) {
+        const float rC = rho + dRC;
+        kx_at(a.ck4f + toff, b4) = zC * kf_rsq(rC * rC + zC * zC);
+      }
       const float ap4 = kf_apar(zC <= 0.0f, ag, aea, grav, eamb, eambsign, qc != 0 && qc != -1);
       const float a4R = ap4 * b4R, a4Z = ap4 * b4Z;
       // RK4 (k2 = k3): dV = h (a1 + 4 a2 + a4)/6, dx = h V + h^2 (2 a1 + a2)/6

@@ -43,6 +43,11 @@ module kaos_hip
     type(c_ptr) :: vperp1GL = c_null_ptr, vperp1GH = c_null_ptr, vperp1GC = c_null_ptr, dvperp1 = c_null_ptr
     type(c_ptr) :: vperp2GL = c_null_ptr, vperp2GH = c_null_ptr, vperp2GC = c_null_ptr, dvperp2 = c_null_ptr
     type(c_ptr) :: d3v3 = c_null_ptr
+    ! ion-neutral charge exchange (QEXCHANGEflagT = 1; needs the 3-D grid)
+    integer(c_int32_t) :: cx = 0, cx_filter = 0, cx_filter_pt = 11, cx_pad = 0
+    real(c_double) :: t_neut = 0, m_neut = 0, cx_sigma_scale = 1
+    type(c_ptr) :: ns_neut = c_null_ptr
+    integer(c_int64_t) :: ena_cap = 0
   end type kaos_grid
 
   ! dir: c_loc of a NUL-terminated character array (trim(dataexportdir)//c_null_char)
@@ -80,7 +85,8 @@ module kaos_hip
   public :: kaos_init, kaos_set_grid, kaos_upload_particles, kaos_upload_vperp, kaos_cell_counts, &
             kaos_seed_particles, kaos_step, kaos_run, &
             kaos_download_master, kaos_download_master3d, kaos_export_master, kaos_export_spinup, &
-            kaos_read_spinup, kaos_set_field, &
+            kaos_read_spinup, kaos_set_field, kaos_download_cx, kaos_download_enas, &
+            kaos_download_ena_master, &
             kaos_count, kaos_download_particles, kaos_download_vperp, kaos_upload_state, &
             kaos_upload_moments, kaos_resume, kaos_save_state, kaos_load_state, kaos_set_launch, &
             kaos_get_launch, kaos_profile, &
@@ -178,6 +184,33 @@ module kaos_hip
       integer(c_int32_t), value :: tube
       type(kaos_spinup_info), intent(in) :: info
     end function kaos_export_spinup
+
+    ! charge exchange: sigmaIonNeutRT, nuIonNeutRT, nuIonNeutPoiT of the last master step
+    integer(c_int) function kaos_download_cx(ctx, tube, sigma, nu, poi) bind(c, name='kaos_download_cx')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tube
+      type(c_ptr), value :: sigma, nu, poi
+    end function kaos_download_cx
+
+    ! the tube's ENAs in id order
+    integer(c_int) function kaos_download_enas(ctx, tube, n_cap, n_out, x, y, z, Vx, Vy, Vz, id, qind) &
+        bind(c, name='kaos_download_enas')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tube
+      integer(c_int64_t), value :: n_cap
+      integer(c_int64_t), intent(out) :: n_out
+      type(c_ptr), value :: x, y, z, Vx, Vy, Vz, id, qind
+    end function kaos_download_enas
+
+    ! NqReNormENAT and the LB/UB ENA escapes (counts, fluxes) of the last master step
+    integer(c_int) function kaos_download_ena_master(ctx, tube, nq_ena, esc) bind(c, name='kaos_download_ena_master')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tube
+      type(c_ptr), value :: nq_ena, esc
+    end function kaos_download_ena_master
 
     ! the spin-up files of a previous run (SPINUPflag = 0 restart); dir as in kaos_spinup_info
     integer(c_int) function kaos_read_spinup(dir, species, flux_tube, nq_cap, st, density, temperature, &

@@ -83,6 +83,14 @@ class Tube:
     v3d: int = 0
     nv1: int = 0
     nv2: int = 0
+    # charge exchange (QEXCHANGEflag = 1), set by with_charge_exchange
+    cx: int = 0
+    cx_filter: int = 0
+    cx_filter_pt: int = 11
+    t_neut: float = 848.0
+    m_neut: float = 0.0
+    cx_sigma_scale: float = 1.0
+    ena_cap: int = 0
 
     @property
     def cphi(self):
@@ -124,6 +132,30 @@ def with_vgrid3(tube, nv1, nv2, vperp_max=None):
     d3 = ((a["dv1"][:, None, None] * a["dv2"][None, :, None]) * a["dvpar"][None, None, :])
     a["d3v3"] = np.ascontiguousarray(d3)
     tube.v3d, tube.nv1, tube.nv2 = 1, nv1, nv2
+    return tube
+
+
+def with_charge_exchange(tube, n0=1.423e11, r0=None, H=2.0e5, t_neut=848.0, filt=0, filt_pt=11, ena_cap=None,
+                         sigma_scale=1.0):
+    """Switch on ion-neutral charge exchange (QEXCHANGEflag = 1; the tube needs
+    the 3-D grid).  The neutrals (mass of the tube's species, TNeut kelvin)
+    have an exponential density n0 exp(-(r - r0)/H) at the cell centres (H =
+    200 km; n0 and r0 = RE + 719 km the reference's ns0NeutIC and zns0NeutIC,
+    _ModelInput.f90:211-212); nsnormCNeutGT = n / nsnormfac d3x per cell
+    (_0_1_1_ConfigGridGeneratorB.f90:113-114).  filt: MOMENTFILTERflag with
+    the M1Perp1/M1Perp2/M2Par window filt_pt.  ena_cap: ENA slots (default:
+    the initial ions).  sigma_scale multiplies the cross-section (1: the
+    reference's expression as written).  Returns the tube."""
+    if not tube.v3d:
+        raise ValueError("charge exchange needs the 3-D velocity grid (with_vgrid3)")
+    r0 = RE + 7.19e5 if r0 is None else r0
+    a = tube.arrays
+    n = n0 * np.exp(-(a["rGC"] - r0) / H)
+    a["nsneut"] = (n / tube.nsnormfac) * a["d3xC"]
+    tube.cx, tube.cx_filter, tube.cx_filter_pt = 1, int(filt), int(filt_pt)
+    tube.t_neut, tube.m_neut = float(t_neut), float(tube.ms)
+    tube.cx_sigma_scale = float(sigma_scale)
+    tube.ena_cap = int(tube.cell_counts.sum()) if ena_cap is None else int(ena_cap)
     return tube
 
 

@@ -47,6 +47,12 @@ def grid_struct(tube):
         a = np.ascontiguousarray(tube.arrays[n], dtype=np.float64)
         keep.append(a)
         setattr(g, n, ptr(a))
+    if getattr(tube, "cx", 0):
+        for n in ("cx", "cx_filter", "cx_filter_pt", "t_neut", "m_neut", "cx_sigma_scale", "ena_cap"):
+            setattr(g, n, getattr(tube, n))
+        a = np.ascontiguousarray(tube.arrays["nsneut"], dtype=np.float64)
+        keep.append(a)
+        g.ns_neut = ptr(a)
     if getattr(tube, "v3d", 0):
         for n, tn in zip(_abi.GRID_ARRAYS3, _abi.TUBE_ARRAYS3):
             if tube.arrays.get(tn) is None:
@@ -176,6 +182,35 @@ class KineticSolver:
             nsnorm_cub=float(t.n_inj_ub if nsnorm_cub is None else nsnorm_cub),
             lb_replenish=lb_replenish, ub_replenish=ub_replenish)
         self._chk(self.lib.kaos_export_spinup(self.h, tube, C.byref(info)))
+
+    def cx(self, tube):
+        """kaos_download_cx: sigmaIonNeutRT, nuIonNeutRT, nuIonNeutPoiT of the
+        last master step."""
+        nq = self.tubes[tube].nq
+        sg, nu, poi = np.zeros(nq), np.zeros(nq), np.zeros(nq)
+        self._chk(self.lib.kaos_download_cx(self.h, tube, ptr(sg), ptr(nu), ptr(poi)))
+        return {"sigma": sg, "nu": nu, "poi": poi}
+
+    def enas(self, tube, n_cap=None):
+        """kaos_download_enas: the tube's ENAs in id order."""
+        n_cap = int(self.tubes[tube].ena_cap) if n_cap is None else n_cap
+        bufs = [np.zeros(max(n_cap, 1)) for _ in range(6)]
+        ids, q = np.zeros(max(n_cap, 1), np.int64), np.zeros(max(n_cap, 1), np.int64)
+        n = C.c_int64(0)
+        self._chk(self.lib.kaos_download_enas(self.h, tube, n_cap, C.byref(n), *[ptr(b) for b in bufs],
+                                              ptr(ids, C.c_int64), ptr(q, C.c_int64)))
+        k = n.value
+        out = {name: b[:k].copy() for name, b in zip(("x", "y", "z", "Vx", "Vy", "Vz"), bufs)}
+        out["id"], out["qind"] = ids[:k].copy(), q[:k].copy()
+        return out
+
+    def ena_master(self, tube):
+        """kaos_download_ena_master: NqReNormENAT and the LB/UB ENA escapes
+        (counts, fluxes) of the last master step."""
+        nq = np.zeros(self.tubes[tube].nq, np.int64)
+        esc = np.zeros(4)
+        self._chk(self.lib.kaos_download_ena_master(self.h, tube, ptr(nq, C.c_int64), ptr(esc)))
+        return {"nq_ena": nq, "esc": esc}
 
     def set_field(self, tube, EA, mode):
         """kaos_set_field: input field EAmagInputT of a restart (nq values) and

@@ -35,7 +35,10 @@ class OrcCfg(C.Structure):
                 + [(n, C.c_double) for n in ICR_FIELDS]
                 + [("filter_pt_m1par", C.c_int)]
                 + [("v3d", C.c_int), ("nv1", C.c_int), ("nv2", C.c_int)]
-                + [(n, _dp) for n in ARRAYS3])
+                + [(n, _dp) for n in ARRAYS3]
+                + [(n, C.c_int) for n in ("cx", "cx_filter", "cx_filter_pt", "pad_cx")]
+                + [("t_neut", C.c_double), ("m_neut", C.c_double), ("cx_sigma_scale", C.c_double),
+                   ("nsneut", _dp), ("ena_cap", C.c_int64)])
 
 
 _lib = None
@@ -121,6 +124,19 @@ def load():
         lib.orc_seed_ions.restype = C.c_int64
         lib.orc_seed_uniforms.argtypes = [C.c_uint64, C.c_int, C.c_int64, _dp]
         lib.orc_get_m1f.argtypes = [P, _dp]
+        lib.orc_get_cx.argtypes = [P, _dp, _dp, _dp]
+        lib.orc_ena_count.argtypes = [P]
+        lib.orc_ena_count.restype = C.c_int64
+        lib.orc_get_enas.argtypes = [P, _dp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        lib.orc_get_ena_master.argtypes = [P, _dp, _dp]
+        lib.orc_cx_cell.argtypes = [_dp, C.c_uint64, C.c_int64, C.c_int, C.c_int, _dp]
+        lib.orc_poisson.argtypes = [C.c_double, C.c_uint64, C.c_int64, C.c_int, C.c_int]
+        lib.orc_poisson.restype = C.c_double
+        lib.orc_cx_sigma.argtypes = [C.c_double]
+        lib.orc_cx_sigma.restype = C.c_double
+        lib.orc_ena_from_ion.argtypes = [C.c_double] * 8 + [C.c_double, _dp]
+        lib.orc_ena_push.argtypes = [_dp, C.c_double, C.c_int]
+        lib.orc_cx_block.argtypes = [C.c_uint64, C.c_int64, C.c_int, C.c_int, C.c_uint64, _dp]
         lib.orc_get_master3.argtypes = [P, _dp, _dp]
         _lib = lib
     return _lib
@@ -147,6 +163,16 @@ def cfg_of(tube, ndatfac=1, seed=1, tube_index=0):
     for n in ICR_FIELDS:
         setattr(c, n, getattr(tube, n, 0.0))
     keep = []
+    c.cx = getattr(tube, "cx", 0)
+    if c.cx:
+        if not c.v3d:
+            raise ValueError("charge exchange needs the 3-D velocity grid (M1Perp1, M1Perp2)")
+        c.cx_filter, c.cx_filter_pt = tube.cx_filter, tube.cx_filter_pt
+        c.t_neut, c.m_neut, c.ena_cap = tube.t_neut, tube.m_neut, tube.ena_cap
+        c.cx_sigma_scale = tube.cx_sigma_scale
+        a = np.ascontiguousarray(tube.arrays["nsneut"], dtype=np.float64)
+        keep.append(a)
+        c.nsneut = ptr(a)
     for n in ARRAYS + (ARRAYS3 if c.v3d else ()):
         a = np.ascontiguousarray(tube.arrays[n], dtype=np.float64)
         keep.append(a)
@@ -186,6 +212,33 @@ class Oracle:
         (_8_7_AmbipolarEfield.f90:1375-1407)."""
         e = None if EA is None else np.ascontiguousarray(EA, dtype=np.float64)
         self.lib.orc_set_field_input(self.h, ptr(e), int(mode))
+
+    def cx(self):
+        """Collision counts of the last master step: sigmaIonNeutRT, nuIonNeutRT,
+        nuIonNeutPoiT per cell."""
+        nq = self.tube.nq
+        sg, nu, poi = np.zeros(nq), np.zeros(nq), np.zeros(nq)
+        self.lib.orc_get_cx(self.h, ptr(sg), ptr(nu), ptr(poi))
+        return {"sigma": sg, "nu": nu, "poi": poi}
+
+    def enas(self):
+        """The ENAs in id order: x, y, z, Vx, Vy, Vz, id, qind."""
+        n = self.lib.orc_ena_count(self.h)
+        X = np.zeros(6 * max(n, 1))
+        ids, q = np.zeros(max(n, 1), np.int64), np.zeros(max(n, 1), np.int64)
+        self.lib.orc_get_enas(self.h, ptr(X), ptr(ids, C.c_int64), ptr(q, C.c_int64))
+        X = X[:6 * n].reshape(n, 6)
+        out = {k: X[:, i].copy() for i, k in enumerate(("x", "y", "z", "Vx", "Vy", "Vz"))}
+        out["id"], out["qind"] = ids[:n], q[:n]
+        return out
+
+    def ena_master(self):
+        """NqReNormENAT of the last master step and the LB/UB ENA escapes
+        (counts, fluxes)."""
+        nq = np.zeros(self.tube.nq)
+        esc = np.zeros(4)
+        self.lib.orc_get_ena_master(self.h, ptr(nq), ptr(esc))
+        return {"nq_ena": nq, "esc": esc}
 
     def set_vperp(self, v1, v2):
         v1, v2 = (np.ascontiguousarray(a, dtype=np.float64) for a in (v1, v2))

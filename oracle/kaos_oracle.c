@@ -56,6 +56,13 @@ typedef struct {
    * [i1][i2][ipar] (Vpar fastest), dV1 dV2 dVpar */
   int v3d, nv1, nv2;
   const double *v1GL, *v1GH, *v1GC, *dv1, *v2GL, *v2GH, *v2GC, *dv2, *d3v3;
+  /* charge exchange (QEXCHANGEflagT = 1, 3-D grid): MOMENTFILTERflagT, the
+   * M1Perp1/M1Perp2/M2Par filter window (M1Par: filter_pt_m1par), TNeut,
+   * mNeut, nsnormCNeutGT per cell, ENA slots */
+  int cx, cx_filter, cx_filter_pt, pad_cx;
+  double t_neut, m_neut, cx_sigma_scale;
+  const double *nsneut;
+  int64_t ena_cap;
 } orc_cfg;
 
 typedef struct {
@@ -63,6 +70,7 @@ typedef struct {
   int64_t ns, cap;
   double *rs, *vs, *vperp, *aea, *ag, *aeaN, *agN, *qk4, *q0, *vpar0;
   double *v1, *v2;                 /* Vperp1, Vperp2 */
+  double *ck4;                     /* cos(thetak4) of the last push (KUC), the frame of a conversion */
   int *qind, *vparind, *vperpind;
   int64_t *id;                     /* upload index, then injection order (keys the heating stream) */
   int64_t next_id;
@@ -82,6 +90,16 @@ typedef struct {
   int64_t injected;
   double *EAin;                    /* EAmagInputT of a restart (nq), or NULL */
   int ea_mode;                     /* 0 computed, 1 input at step 1, 2 input at every master step */
+  /* charge exchange: per cell sigmaIonNeutRT, nuIonNeutRT, nuIonNeutPoiT of the
+   * last master step; the ENAs (Cartesian state, id, Qindk1) and their
+   * master-step counts NqReNormENAT and LB/UB escapes */
+  double *cx_sigma, *cx_nu, *cx_poi;
+  double *ex, *ey, *ez, *evx, *evy, *evz;
+  int64_t *eid;
+  int *eq;
+  int64_t ens, ena_made;
+  uint32_t *nq_ena;
+  double ena_esc_lb, ena_esc_ub, ena_flux_lb, ena_flux_ub;
 } orc_t;
 
 static void *xc(size_t n) { void *p = calloc(n ? n : 1, 1); return p; }
@@ -307,7 +325,7 @@ static void vcells3(orc_t *o, int64_t j, double vpar, double v1, double v2) {
 
 /* ---- state management ------------------------------------------------------ */
 
-#define ARR_D(X) X(rs) X(vs) X(vperp) X(aea) X(ag) X(aeaN) X(agN) X(qk4) X(q0) X(vpar0) X(v1) X(v2)
+#define ARR_D(X) X(rs) X(vs) X(vperp) X(aea) X(ag) X(aeaN) X(agN) X(qk4) X(q0) X(vpar0) X(v1) X(v2) X(ck4)
 #define ARR_I(X) X(qind) X(vparind) X(vperpind) X(i1ind) X(i2ind) X(ip3ind)
 #define ARR_L(X) X(id)
 
@@ -342,6 +360,18 @@ void *orc_create(const orc_cfg *cfg, int64_t ns, const double *q0, const double 
     o->n3_u = (uint32_t *)xc(sizeof(uint32_t) * nq * (int64_t)cfg->nv1 * cfg->nv2 * cfg->nvpar);
     o->M3 = (double *)xc(sizeof(double) * 8 * nq);
   }
+  if (cfg->cx) {
+    int64_t ec = cfg->ena_cap > 0 ? cfg->ena_cap : 1;
+    o->cx_sigma = (double *)xc(sizeof(double) * nq);
+    o->cx_nu = (double *)xc(sizeof(double) * nq);
+    o->cx_poi = (double *)xc(sizeof(double) * nq);
+    o->nq_ena = (uint32_t *)xc(sizeof(uint32_t) * nq);
+    o->ex = (double *)xc(sizeof(double) * ec); o->ey = (double *)xc(sizeof(double) * ec);
+    o->ez = (double *)xc(sizeof(double) * ec); o->evx = (double *)xc(sizeof(double) * ec);
+    o->evy = (double *)xc(sizeof(double) * ec); o->evz = (double *)xc(sizeof(double) * ec);
+    o->eid = (int64_t *)xc(sizeof(int64_t) * ec);
+    o->eq = (int *)xc(sizeof(int) * ec);
+  }
   /* initial conditions: x0 = root(q0), V0 = Vpar0 b-hat(x0) */
   for (int64_t j = 0; j < ns; ++j) {
     o->rs[j] = kaos_rs_of_q(q0[j], cfg->p);
@@ -365,6 +395,8 @@ void orc_destroy(void *h) {
 #undef FR
   free(o->nq_counts); free(o->nph_counts); free(o->nq_u); free(o->nph_u); free(o->NqRT); free(o->M); free(o->Mprev);
   free(o->EA); free(o->EG); free(o->M0F); free(o->M1F); free(o->n3_u); free(o->M3); free(o->EAin);
+  free(o->cx_sigma); free(o->cx_nu); free(o->cx_poi); free(o->nq_ena);
+  free(o->ex); free(o->ey); free(o->ez); free(o->evx); free(o->evy); free(o->evz); free(o->eid); free(o->eq);
   free(o);
 }
 
@@ -424,6 +456,7 @@ static int inject(orc_t *o, int64_t n, const double *uni) {
     o->aea[j] = o->aeaN[j] = fabs(qm * fabs(o->EA[Qb]));
     o->ag[j] = o->agN[j] = fabs(qm * fabs(o->EG[Qb]));
     o->qk4[j] = qc;
+    o->ck4[j] = L.c;     /* a conversion before the first push: the frame of the injection point */
   }
   o->ns += nl + nu;
   o->next_id += nl + nu;
@@ -460,6 +493,7 @@ static void push_all(orc_t *o, int64_t n) {
     int qs = scan_qcell(c, gA.q, o->qind[j]);    /* the reference's linear scan */
     kaos_push_out_t r;
     kaos_push_ion(L, gA, qs, o->vs[j], o->ag[j], o->aea[j], &P, &r);
+    o->ck4[j] = r.zC / sqrt(r.u2C);               /* cos(thetak4), KUC */
     o->qind[j] = qs;
     o->vparind[j] = scan_vcell(c->vparGL, c->vparGH, c->nvpar, r.vpar, o->vparind[j]);
     o->vperpind[j] = scan_vcell(c->vperpGL, c->vperpGH, c->nvperp, o->vperp[j], o->vperpind[j]);
@@ -474,6 +508,165 @@ static void push_all(orc_t *o, int64_t n) {
     o->vs[j] = r.vparN;
     o->qk4[j] = r.qk4;
   }
+}
+
+/* ---- charge exchange -------------------------------------------------- */
+
+/* Collision counts of every cell at a master step n (IonNeutralCollision-
+ * FrequencyASub + the one-rank share of ...BSub, kaos_cx_cell), from the 3-D
+ * moments of this master step, filtered with MOMENTFILTERflagT = 1
+ * (_8_6_6_1:22-86: M1Par with its own window, M1Perp1, M1Perp2, M2Par with
+ * theirs). */
+static void cx_freq(orc_t *o, int64_t n) {
+  orc_cfg *c = &o->c;
+  int nq = c->nq;
+  const double *M3 = o->M3;
+  double *f = (double *)xc(sizeof(double) * 4 * nq);
+  const double *m1par = M3 + 3 * nq, *m1p1 = M3 + 1 * nq, *m1p2 = M3 + 2 * nq, *m2par = M3 + 7 * nq;
+  if (c->cx_filter) {
+    int w1 = c->filter_pt_m1par ? c->filter_pt_m1par : c->filter_pt;
+    kaos_moment_filter(m1par, f, nq, w1);
+    kaos_moment_filter(m1p1, f + nq, nq, c->cx_filter_pt);
+    kaos_moment_filter(m1p2, f + 2 * nq, nq, c->cx_filter_pt);
+    kaos_moment_filter(m2par, f + 3 * nq, nq, c->cx_filter_pt);
+    m1par = f; m1p1 = f + nq; m1p2 = f + 2 * nq; m2par = f + 3 * nq;
+  }
+  const double sig_n = sqrt((KAOS_KB * c->t_neut) / c->m_neut);
+  for (int Q = 0; Q < nq; ++Q) {
+    kaos_cx_in_t I;
+    I.m0 = M3[Q]; I.m1par = m1par[Q]; I.m1p1 = m1p1[Q]; I.m1p2 = m1p2[Q]; I.m2par = m2par[Q];
+    if (c->cx_filter && I.m0 == 0.0) I.m2par = 0.0;   /* sigma is set only for populated cells */
+    I.cth = c->cthGC[Q]; I.ell = c->ellGC[Q]; I.cphi = c->cphi; I.sphi = c->sphi;
+    I.nsneut = c->nsneut[Q]; I.h = c->h; I.sig_n = sig_n;
+    I.NqRT = o->NqRT[Q]; I.d3x = c->d3xC[Q]; I.nsnormfac = c->nsnormfac; I.nqc = o->nq_counts[Q];
+    I.sig_scale = c->cx_sigma_scale;
+    kaos_cx_out_t r = kaos_cx_cell(&I, c->seed, n, c->tube, Q + 1);
+    o->cx_sigma[Q] = r.sigma; o->cx_nu[Q] = r.nu; o->cx_poi[Q] = r.poi;
+  }
+  free(f);
+}
+
+/* ENA production at a step n >= 2 (_8_KineticSolver.f90:95-166): in cell
+ * order, the first nint(nuIonNeutPoiT) ions (array order = id order) whose
+ * Qindk1 is the cell become ENAs (IonNeutralCollisionsSub); they leave the
+ * ion arrays (stable pack) for the ENA arrays. */
+static int cx_convert(orc_t *o) {
+  orc_cfg *c = &o->c;
+  int nq = c->nq;
+  int64_t *need = (int64_t *)xc(sizeof(int64_t) * nq);
+  int64_t tot = 0;
+  for (int Q = 0; Q < nq; ++Q) { need[Q] = kaos_cx_take(o->cx_poi[Q]); tot += need[Q]; }
+  if (!tot) { free(need); return 0; }
+  char *conv = (char *)xc((size_t)o->ns);
+  for (int64_t j = 0; j < o->ns; ++j) {
+    int Q = o->qind[j];
+    if (Q >= 1 && need[Q - 1] > 0) { conv[j] = 1; --need[Q - 1]; }
+  }
+  int64_t k = 0;
+  for (int64_t j = 0; j < o->ns; ++j) {
+    if (conv[j]) {
+      if (o->ens >= c->ena_cap) { free(conv); free(need); return 3; }
+      double X[6];
+      kaos_ena_from_ion(o->rs[j], o->vs[j], o->v1[j], o->v2[j], o->ck4[j], c->p, c->cphi, c->sphi, X);
+      int64_t e = o->ens++;
+      o->ex[e] = X[0]; o->ey[e] = X[1]; o->ez[e] = X[2];
+      o->evx[e] = X[3]; o->evy[e] = X[4]; o->evz[e] = X[5];
+      o->eid[e] = o->id[j];
+      o->eq[e] = o->qind[j];
+      ++o->ena_made;
+      continue;
+    }
+    if (k != j) {
+#define MV(n) o->n[k] = o->n[j];
+      ARR_D(MV) ARR_I(MV) ARR_L(MV)
+#undef MV
+    }
+    ++k;
+  }
+  o->ns = k;
+  free(conv); free(need);
+  return 0;
+}
+
+/* ENA escapes at a master step (_8_0_7_BoundaryConditions.f90:309-470):
+ * ENAs whose Qindk1 is 0 (LB) or -1 (UB) are counted and removed. */
+static void ena_remove(orc_t *o) {
+  orc_cfg *c = &o->c;
+  int64_t k = 0, nlb = 0, nub = 0;
+  for (int64_t e = 0; e < o->ens; ++e) {
+    if (o->eq[e] == 0) { ++nlb; continue; }
+    if (o->eq[e] == -1) { ++nub; continue; }
+    o->ex[k] = o->ex[e]; o->ey[k] = o->ey[e]; o->ez[k] = o->ez[e];
+    o->evx[k] = o->evx[e]; o->evy[k] = o->evy[e]; o->evz[k] = o->evz[e];
+    o->eid[k] = o->eid[e]; o->eq[k] = o->eq[e];
+    ++k;
+  }
+  o->ens = k;
+  int ub = c->nq - 1;
+  double dtm = (double)c->ndatfac * c->h;
+  o->ena_esc_lb = (double)nlb; o->ena_esc_ub = (double)nub;
+  o->ena_flux_lb = ((double)nlb * c->nsnormfac) / ((c->d3xC[0] / (c->hqC[0] * c->dqC[0])) * dtm);
+  o->ena_flux_ub = ((double)nub * c->nsnormfac) / ((c->d3xC[ub] / (c->hqC[ub] * c->dqC[ub])) * dtm);
+}
+
+/* The ENA push of a step: Qindk1 at the pre-push position (the ion scan of
+ * KUA:221-273 on the ENA's q), counted at master steps (ENAParticleCountsSub,
+ * _8_3_2:33-60), then the ballistic RK4. */
+static void ena_push(orc_t *o, int master) {
+  orc_cfg *c = &o->c;
+  if (master) memset(o->nq_ena, 0, sizeof(uint32_t) * c->nq);
+  for (int64_t e = 0; e < o->ens; ++e) {
+    double X[6] = {o->ex[e], o->ey[e], o->ez[e], o->evx[e], o->evy[e], o->evz[e]};
+    double q = kaos_ena_q(X[0], X[1], X[2]);
+    int Q = (q == q) ? scan_qcell(c, q, o->eq[e]) : 0;
+    o->eq[e] = Q;
+    if (master && Q >= 1) o->nq_ena[Q - 1]++;
+    kaos_ena_push(X, c->h, c->grav);
+    o->ex[e] = X[0]; o->ey[e] = X[1]; o->ez[e] = X[2];
+    o->evx[e] = X[3]; o->evy[e] = X[4]; o->evz[e] = X[5];
+  }
+}
+
+void orc_get_cx(void *h, double *sigma, double *nu, double *poi) {
+  orc_t *o = (orc_t *)h;
+  int nq = o->c.nq;
+  if (!o->c.cx) return;
+  if (sigma) memcpy(sigma, o->cx_sigma, sizeof(double) * nq);
+  if (nu) memcpy(nu, o->cx_nu, sizeof(double) * nq);
+  if (poi) memcpy(poi, o->cx_poi, sizeof(double) * nq);
+}
+
+int64_t orc_ena_count(void *h) { return ((orc_t *)h)->ens; }
+
+/* The ENAs in id order: x, y, z, Vx, Vy, Vz, id, Qindk1. */
+void orc_get_enas(void *h, double *X6, int64_t *id, int64_t *qind) {
+  orc_t *o = (orc_t *)h;
+  int64_t n = o->ens;
+  int64_t *ord = (int64_t *)xc(sizeof(int64_t) * (n ? n : 1));
+  for (int64_t i = 0; i < n; ++i) ord[i] = i;
+  /* insertion sort by id (few ENAs in tests) */
+  for (int64_t i = 1; i < n; ++i) {
+    int64_t v = ord[i], k = i - 1;
+    while (k >= 0 && o->eid[ord[k]] > o->eid[v]) { ord[k + 1] = ord[k]; --k; }
+    ord[k + 1] = v;
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t e = ord[i];
+    double v[6] = {o->ex[e], o->ey[e], o->ez[e], o->evx[e], o->evy[e], o->evz[e]};
+    for (int k = 0; k < 6; ++k) X6[6 * i + k] = v[k];
+    id[i] = o->eid[e];
+    qind[i] = o->eq[e];
+  }
+  free(ord);
+}
+
+/* ENA counts NqReNormENAT of the last master step and the LB/UB ENA escapes
+ * (counts, then fluxes). */
+void orc_get_ena_master(void *h, double *nq_ena, double *esc4) {
+  orc_t *o = (orc_t *)h;
+  if (!o->c.cx) return;
+  for (int Q = 0; Q < o->c.nq; ++Q) nq_ena[Q] = (double)o->nq_ena[Q];
+  esc4[0] = o->ena_esc_lb; esc4[1] = o->ena_esc_ub; esc4[2] = o->ena_flux_lb; esc4[3] = o->ena_flux_ub;
 }
 
 /* The input field of a restart replaces the computed EAmagRT after the
@@ -498,6 +691,10 @@ int orc_step(void *h, int64_t n, const double *uniforms) {
   orc_cfg *c = &o->c;
   int nq = c->nq;
   o->injected = 0;
+  if (c->cx && n >= 2) {
+    int rc = cx_convert(o);
+    if (rc) return rc;
+  }
   if (n == 1) {
     /* k0 indices of the initial state, with q0 clamped into the grid (KSA:55-320) */
     for (int64_t j = 0; j < o->ns; ++j) {
@@ -517,6 +714,7 @@ int orc_step(void *h, int64_t n, const double *uniforms) {
     }
     if (c->v3d) counts3_and_moments(o, o->ns);
     else counts_and_moments(o, o->qind, o->vparind, o->vperpind, o->ns);
+    if (c->cx) cx_freq(o, 1);
     memset(o->Mprev, 0, sizeof(double) * 6 * nq);   /* moments of master step 0: none */
     ambipolar_field(c, o->Mprev, o->M0F, o->EA);
     field_input(o, 1);
@@ -533,6 +731,7 @@ int orc_step(void *h, int64_t n, const double *uniforms) {
     int master = (n % c->ndatfac) == 0;
     if (master) {
       bc_remove(o);
+      if (c->cx) ena_remove(o);
       ambipolar_field(c, o->Mprev, o->M0F, o->EA);
       field_input(o, 0);
       m1par_filter(c, o->Mprev, o->M1F);
@@ -546,11 +745,13 @@ int orc_step(void *h, int64_t n, const double *uniforms) {
   }
   for (int64_t j = 0; j < o->ns; ++j) { o->aea[j] = o->aeaN[j]; o->ag[j] = o->agN[j]; }
   push_all(o, n);
+  if (c->cx) ena_push(o, n == 1 || o->master_last);
   if (n > 1 && o->master_last) {
     int rc = inject(o, n, uniforms);
     if (rc) return rc;
     if (c->v3d) counts3_and_moments(o, o->ns);
     else counts_and_moments(o, o->qind, o->vparind, o->vperpind, o->ns);
+    if (c->cx) cx_freq(o, n);
     memcpy(o->Mprev, o->M, sizeof(double) * 6 * nq);
     o->nn = n / c->ndatfac + 1;
   }
@@ -795,3 +996,30 @@ int64_t orc_seed_ions(const orc_cfg *c, const int64_t *counts, const double *tem
   return i;
 }
 void orc_seed_uniforms(uint64_t seed, int tube, int64_t i, double *u) { kaos_seed_uniforms(seed, tube, i, u); }
+
+/* Charge-exchange pieces for the pins (tests/test_oracle_cx.py). */
+void orc_cx_block(uint64_t seed, int64_t step, int tube, int Q, uint64_t blk, double *u) {
+  kaos_cx_block(seed, step, tube, Q, blk, u);
+}
+double orc_cx_sigma(double m2par) { return kaos_cx_sigma(m2par); }
+double orc_poisson(double mu, uint64_t seed, int64_t step, int tube, int Q) {
+  return kaos_poisson(mu, seed, step, tube, Q);
+}
+/* in: m0, m1par, m1p1, m1p2, m2par, cth, ell, cphi, sphi, nsneut, h, sig_n, NqRT, d3x, nsnormfac, nqc, sig_scale;
+ * out: sigma, nu, poi */
+void orc_cx_cell(const double *in, uint64_t seed, int64_t step, int tube, int Q, double *out) {
+  kaos_cx_in_t I;
+  I.m0 = in[0]; I.m1par = in[1]; I.m1p1 = in[2]; I.m1p2 = in[3]; I.m2par = in[4];
+  I.cth = in[5]; I.ell = in[6]; I.cphi = in[7]; I.sphi = in[8]; I.nsneut = in[9]; I.h = in[10];
+  I.sig_n = in[11]; I.NqRT = in[12]; I.d3x = in[13]; I.nsnormfac = in[14]; I.nqc = in[15];
+  I.sig_scale = in[16];
+  kaos_cx_out_t r = kaos_cx_cell(&I, seed, step, tube, Q);
+  out[0] = r.sigma; out[1] = r.nu; out[2] = r.poi;
+}
+void orc_ena_from_ion(double rs, double vs, double v1, double v2, double ck4, double p, double cphi, double sphi,
+                      double unused, double *X) {
+  (void)unused;
+  kaos_ena_from_ion(rs, vs, v1, v2, ck4, p, cphi, sphi, X);
+}
+void orc_ena_push(double *X, double h, int grav) { kaos_ena_push(X, h, grav); }
+
