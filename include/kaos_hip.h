@@ -131,6 +131,17 @@ typedef struct kaos_grid {
   double cx_sigma_scale;   /* factor on sigmaIonNeutRT; 1 = the reference's expression as written */
   const double* ns_neut;   /* nsnormCNeutGT(1, Q): neutral macro-particles per cell (nq) */
   int64_t ena_cap;         /* ENA slots of the tube (conversions over the whole run) */
+  /* The ENA velocity grid (_0_0_VelGridGenerator.f90:482-857): |Vp| cells
+   * (nvp), Vq cells (nvq), Vphi cells (nvphi), each with contiguous edges,
+   * centres and widths (<= 255 cells); per bin [ip][iq][iphi] (Vphi fastest)
+   * the metric product hVp hVq hVphi and d33vCT.  An ENA is binned by the
+   * cells of its (|Vp|, Vq, Vphi) at its pre-push position (KUA:1156-1290;
+   * off the grid a component keeps its previous cell). */
+  int32_t nvp, nvq, nvphi, ena_pad;
+  const double *vpGL, *vpGH, *vpGC, *dvp;
+  const double *vqGL, *vqGH, *vqGC, *dvq;
+  const double *vphiGL, *vphiGH, *vphiGC, *dvphi;
+  const double *ena_hh, *ena_d33v;
 } kaos_grid;
 
 /* Per-step summary. */
@@ -368,6 +379,14 @@ int kaos_download_enas(kaos_ctx* ctx, int32_t tube, int64_t n_cap, int64_t* n_ou
  * counts and the fluxes LBoutfluxENART/UBoutfluxENART
  * (_8_0_7_BoundaryConditions.f90:309-470).  Either pointer may be NULL. */
 int kaos_download_ena_master(kaos_ctx* ctx, int32_t tube, int64_t* nq_ena, double* esc);
+
+/* ENA phase-space counts NphReNormENAT of the last master step, [Q][ip][iq][iphi]
+ * (nq * nvp * nvq * nvphi; ENAParticleCountsSub, _8_3_2:300-361), and M (8*nq):
+ * M0phENART, M1PphENART, M1QphENART, M1PHIphENART, M2phENART, M2PphENART,
+ * M2QphENART, M2PHIphENART of ENADistributionFunctionsSub and the ENA moment
+ * subroutines (_8_4_2, _8_9, _8_10_1.._8_10_8; second moments in eV).  Either
+ * pointer may be NULL. */
+int kaos_download_ena_moments(kaos_ctx* ctx, int32_t tube, int64_t* counts, double* M);
 
 /* Number of live ions of a tube. */
 int kaos_count(kaos_ctx* ctx, int32_t tube, int64_t* n);

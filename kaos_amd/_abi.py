@@ -34,6 +34,10 @@ GRID_ARRAYS3 = ("vperp1GL", "vperp1GH", "vperp1GC", "dvperp1", "vperp2GL", "vper
                 "d3v3")
 TUBE_ARRAYS3 = ("v1GL", "v1GH", "v1GC", "dv1", "v2GL", "v2GH", "v2GC", "dv2", "d3v3")
 
+# ENA velocity grid of a charge-exchange tube (kaos_grid field name = grids.Tube array name)
+ENA_ARRAYS = ("vpGL", "vpGH", "vpGC", "dvp", "vqGL", "vqGH", "vqGC", "dvq", "vphiGL", "vphiGH", "vphiGC", "dvphi",
+              "ena_hh", "ena_d33v")
+
 ICR_FIELDS = ("icr_lambda_perp", "icr_eta_lh", "icr_xi_perp1", "icr_xi_perp2", "icr_s0", "icr_f0",
               "icr_chi_perp1", "icr_chi_perp2")
 
@@ -51,7 +55,9 @@ class KaosGrid(C.Structure):
                 + [(n, _dp) for n in GRID_ARRAYS3]
                 + [(n, C.c_int32) for n in ("cx", "cx_filter", "cx_filter_pt", "cx_pad")]
                 + [(n, C.c_double) for n in ("t_neut", "m_neut", "cx_sigma_scale")]
-                + [("ns_neut", _dp), ("ena_cap", C.c_int64)])
+                + [("ns_neut", _dp), ("ena_cap", C.c_int64)]
+                + [(n, C.c_int32) for n in ("nvp", "nvq", "nvphi", "ena_pad")]
+                + [(n, _dp) for n in ENA_ARRAYS])
 
 
 class KaosExportInfo(C.Structure):
@@ -84,6 +90,7 @@ EXPORTS = ("kaos_init", "kaos_set_grid", "kaos_upload_particles", "kaos_upload_v
            "kaos_download_master", "kaos_download_master3d", "kaos_export_master", "kaos_export_spinup",
            "kaos_read_spinup", "kaos_set_field",
            "kaos_count", "kaos_download_cx", "kaos_download_enas", "kaos_download_ena_master",
+           "kaos_download_ena_moments",
            "kaos_download_particles", "kaos_download_vperp", "kaos_upload_state",
            "kaos_upload_moments", "kaos_resume", "kaos_save_state", "kaos_load_state", "kaos_set_launch",
            "kaos_get_launch", "kaos_profile", "kaos_profile_read", "kaos_last_error", "kaos_finalize")
@@ -121,6 +128,7 @@ def load(path=LIB_PATH):
     lib.kaos_download_cx.argtypes = [P, C.c_int32, _dp, _dp, _dp]
     lib.kaos_download_enas.argtypes = [P, C.c_int32, C.c_int64, i64p] + [_dp] * 6 + [i64p] * 2
     lib.kaos_download_ena_master.argtypes = [P, C.c_int32, i64p, _dp]
+    lib.kaos_download_ena_moments.argtypes = [P, C.c_int32, i64p, _dp]
     lib.kaos_download_particles.argtypes = [P, C.c_int32, C.c_int64, i64p] + [_dp] * 10 + [i64p] * 3
     lib.kaos_download_vperp.argtypes = [P, C.c_int32, C.c_int64, i64p, _dp, _dp]
     lib.kaos_upload_state.argtypes = [P, C.c_int32, C.c_int64] + [_dp] * 12 + [i64p] * 3

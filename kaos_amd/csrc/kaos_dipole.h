@@ -1235,4 +1235,77 @@ KAOS_HD void kaos_ena_push(double X[6], double h, int grav) {
   for (int i = 0; i < 3; ++i) { X[i] = xn[i]; X[3 + i] = vn[i]; }
 }
 
+/* ENA velocity space (QEXCHANGEflagT = 1): the ENA's (|Vp|, Vq, Vphi) at its
+ * position, in the local dipole frame there (KUA:1156-1160 and the Vpar of
+ * KUA:978-980: theta from z/r, phi from the ENA's own (x, y)).             */
+KAOS_HD void kaos_ena_vcomp(const double X[6], double* vp, double* vq, double* vphi) {
+  const double rho = sqrt(X[0] * X[0] + X[1] * X[1]);
+  const double r = sqrt(rho * rho + X[2] * X[2]);
+  const double c = X[2] / r, s = rho / r, sl = sqrt(1.0 + 3.0 * (c * c));
+  const double cp = (rho > 0.0) ? X[0] / rho : 1.0, sp = (rho > 0.0) ? X[1] / rho : 0.0;
+  const double vrho = X[3] * cp + X[4] * sp;
+  *vq = 3.0 * c * s * vrho / sl + X[5] * (3.0 * (c * c) - 1.0) / sl;
+  *vp = fabs((1.0 - 3.0 * (c * c)) * vrho / sl + 3.0 * X[5] * c * s / sl);
+  *vphi = -X[3] * sp + X[4] * cp;
+}
+
+/* The ENA grid of a tube: Vp, Vq, Vphi cells with their centres and widths,
+ * and per bin [ip][iq][iphi] (Vphi fastest) the metric product hVp hVq hVphi
+ * and d33v = hVp hVq hVphi dVp dVq dVphi (_0_0_VelGridGenerator.f90:820-857). */
+typedef struct {
+  int nvp, nvq, nvphi;
+  const double *dvp, *dvq, *dvphi, *vpGC, *vqGC, *vphiGC, *hh, *d33v;
+} kaos_enagrid_t;
+
+#define KAOS_ENA_LANES 256
+
+/* Per-lane partial sums of a config cell's ENA moments over the bins b =
+ * lane, lane + 256, ... in order: F = (count nsnormfac/d3x) / d33v
+ * (ENADistributionFunctionsSub, _8_4_2:24-36), weight dVp dVq dVphi hh F
+ * (NonUniform3DIntegratorSub, _8_9:25-35); stage 1 sums w, w Vp, w Vq,
+ * w Vphi (ZerothENAMomentSub, FirstP/Q/PHIENAMomentSub), stage 2 sums w
+ * |dV|^2, w dVp^2, w dVq^2, w dVphi^2 of the offsets from the first moments
+ * m1 (SecondENAMomentSub, SecondP/Q/PHIENAMomentSub, _8_10_5:30-40). */
+KAOS_HD void kaos_ena_mom_part(const kaos_enagrid_t* g, const uint32_t* cnt, int lane, double k, int stage,
+                               const double m1[3], double o[4]) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  const int nb = g->nvp * g->nvq * g->nvphi;
+  for (int b = lane; b < nb; b += KAOS_ENA_LANES) {
+    const uint32_t c = cnt[b];
+    if (c == 0u) continue;
+    const int iphi = b % g->nvphi, iq = (b / g->nvphi) % g->nvq, ip = b / (g->nvphi * g->nvq);
+    const double F = ((double)c * k) / g->d33v[b];
+    const double w = ((g->dvp[ip] * g->dvq[iq]) * g->dvphi[iphi]) * (g->hh[b] * F);
+    if (stage == 1) {
+      s0 = s0 + w;
+      s1 = s1 + w * g->vpGC[ip];
+      s2 = s2 + w * g->vqGC[iq];
+      s3 = s3 + w * g->vphiGC[iphi];
+    } else {
+      const double dp = m1[0] - g->vpGC[ip], dq = m1[1] - g->vqGC[iq], df = m1[2] - g->vphiGC[iphi];
+      s0 = s0 + w * ((dp * dp + dq * dq) + df * df);
+      s1 = s1 + w * (dp * dp);
+      s2 = s2 + w * (dq * dq);
+      s3 = s3 + w * (df * df);
+    }
+  }
+  o[0] = s0; o[1] = s1; o[2] = s2; o[3] = s3;
+}
+
+/* M0phENART, M1PphENART, M1QphENART, M1PHIphENART from the stage-1 sums
+ * (0 without ENAs in the cell). */
+KAOS_HD void kaos_ena_mom_finish1(const double S[4], double M[8]) {
+  const double M0 = S[0];
+  M[0] = M0;
+  M[1] = (M0 == 0.0) ? 0.0 : S[1] / M0;
+  M[2] = (M0 == 0.0) ? 0.0 : S[2] / M0;
+  M[3] = (M0 == 0.0) ? 0.0 : S[3] / M0;
+}
+
+/* M2phENART, M2PphENART, M2QphENART, M2PHIphENART [eV]: 6.242e18 (ms/2) S / M0. */
+KAOS_HD void kaos_ena_mom_finish2(const double S[4], double ms, double M[8]) {
+  const double M0 = M[0];
+  for (int i = 0; i < 4; ++i) M[4 + i] = (M0 == 0.0) ? 0.0 : (6.242e18 * (ms / 2.0)) * (S[i] / M0);
+}
+
 #endif /* KAOS_DIPOLE_H */

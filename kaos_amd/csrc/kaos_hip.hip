@@ -74,6 +74,9 @@ struct DevTube {
   int cx, cx_filter, cx_filter_pt, cx_pad;  // charge exchange: on, MOMENTFILTERflagT, M1Perp/M2Par window
   double sig_n, sig_scale;                  // neutral thermal speed sqrt(kB TNeut / mNeut); cross-section factor
   long long eoff, ecap;                     // the tube's ENA slots [eoff, eoff + ecap)
+  int nvp, nvq, nvphi, ena_pad;             // ENA velocity grid cells
+  long long epoff, eqoff, efoff;            // offsets of its Vp / Vq / Vphi cell arrays
+  long long eboff, ehoff;                   // per-bin arrays (hh, d33v); counts [Q][bin]
 };
 
 struct StepArgs {
@@ -1607,8 +1610,8 @@ __global__ void k_cx_scan(int ncell, int level, long long* need, long long* pref
 }
 
 __global__ void __launch_bounds__(KAOS_BLOCK) k_cx_convert(StepArgs a, int mixed, const long long* thr, double* ena,
-                                                           long long* eid, int* eq, long long* ena_cnt,
-                                                           long long ena_tot, long long* live) {
+                                                           long long* eid, int* eq, unsigned int* evc,
+                                                           long long* ena_cnt, long long ena_tot, long long* live) {
   const int tile = blockIdx.x, tb = a.tile_tube[tile];
   const DevTube& T = a.tubes[tb];
   if (!T.cx) return;
@@ -1633,9 +1636,60 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_cx_convert(StepArgs a, int mixed
     for (int i = 0; i < 6; ++i) ena[(long long)i * ena_tot + k] = X[i];
     eid[k] = a.id[j];
     eq[k] = qc;
+    evc[k] = 0u;
     if (mixed) a.cell[j] = KAOS_DEAD;
     else a.rs[j] = 0.0;
     atomicAdd((unsigned long long*)&live[tb], (unsigned long long)(-1LL));
+  }
+}
+
+// The eight ENA moments of one (tube, config cell) per block at a master
+// step: lane l of 256 sums the bins l, l + 256, ... (kaos_ena_mom_part), a
+// pairwise tree over the lanes gives the sums (the oracle's order); the
+// second moments take the first ones as their centre.
+__global__ void __launch_bounds__(KAOS_ENA_LANES) k_ena_moments(const DevTube* tubes, const int* blk_tube,
+    const int* blk_q, const unsigned int* nq_ena, const unsigned int* ena_hist, const double* evp,
+    const double* evq, const double* evf, const double* hh, const double* d33v, const double* d3x, double* M) {
+  __shared__ double s_x[4][KAOS_ENA_LANES];
+  __shared__ double s_m1[3];
+  const int ci = blockIdx.x;
+  const DevTube& T = tubes[blk_tube[ci]];
+  if (!T.cx) return;
+  const int Q = blk_q[ci], nq = T.nq, tid = threadIdx.x;
+  double* Mt = M + 8 * T.goff;
+  if (nq_ena[T.goff + Q] == 0u) {              // NqENART = 0: F = 0 in every bin
+    if (tid < 8) Mt[(long long)tid * nq + Q] = 0.0;
+    return;
+  }
+  const int nb = T.nvp * T.nvq * T.nvphi;
+  const double *GHp = evp + T.epoff, *GHq = evq + T.eqoff, *GHf = evf + T.efoff;
+  kaos_enagrid_t g = {T.nvp, T.nvq, T.nvphi, GHp + 2 * (T.nvp + 1), GHq + 2 * (T.nvq + 1), GHf + 2 * (T.nvphi + 1),
+                      GHp + (T.nvp + 1), GHq + (T.nvq + 1), GHf + (T.nvphi + 1), hh + T.eboff, d33v + T.eboff};
+  const uint32_t* cnt = ena_hist + T.ehoff + (long long)Q * nb;
+  const double k = T.nsnormfac / d3x[T.goff + Q];
+  double Mq[8];
+  for (int stage = 1; stage <= 2; ++stage) {
+    double m1[3] = {0.0, 0.0, 0.0}, o4[4];
+    if (stage == 2) { m1[0] = s_m1[0]; m1[1] = s_m1[1]; m1[2] = s_m1[2]; }
+    kaos_ena_mom_part(&g, cnt, tid, k, stage, m1, o4);
+    for (int i = 0; i < 4; ++i) s_x[i][tid] = o4[i];
+    __syncthreads();
+    for (int st = 1; st < KAOS_ENA_LANES; st *= 2) {
+      if ((tid % (2 * st)) == 0 && tid + st < KAOS_ENA_LANES)
+        for (int i = 0; i < 4; ++i) s_x[i][tid] = s_x[i][tid] + s_x[i][tid + st];
+      __syncthreads();
+    }
+    if (tid == 0) {
+      const double S[4] = {s_x[0][0], s_x[1][0], s_x[2][0], s_x[3][0]};
+      if (stage == 1) {
+        kaos_ena_mom_finish1(S, Mq);
+        s_m1[0] = Mq[1]; s_m1[1] = Mq[2]; s_m1[2] = Mq[3];
+      } else {
+        kaos_ena_mom_finish2(S, T.ms, Mq);
+        for (int i = 0; i < 8; ++i) Mt[(long long)i * nq + Q] = Mq[i];
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -1648,7 +1702,8 @@ __global__ void k_ena_reset(int ncell, int ntubes, unsigned int* nq_ena, unsigne
 
 __global__ void k_ena_step(const DevTube* tubes, int ntubes, const long long* eoff, const long long* ena_cnt,
                            long long ena_tot, double* ena, int* eq, const double* qGH, unsigned int* nq_ena,
-                           unsigned long long* ena_esc, int master) {
+                           unsigned long long* ena_esc, int master, unsigned int* evc, const double* evp,
+                           const double* evq, const double* evf, unsigned int* ena_hist) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= ena_tot) return;
   int lo = 0, hi = ntubes;                   // the tube whose slots [eoff[t], eoff[t+1]) hold k
@@ -1669,7 +1724,26 @@ __global__ void k_ena_step(const DevTube* tubes, int ntubes, const long long* eo
   const double q = kaos_ena_q(X[0], X[1], X[2]);
   const int Q = (q == q) ? kaos_find_qcell(q, qGH + T.goff, T.qGL0, T.qGHlast, T.nq, T.desc) : 0;
   eq[k] = Q;
-  if (master && Q >= 1) atomicAdd(&nq_ena[T.goff + Q - 1], 1u);
+  if (Q >= 1) {
+    // velocity cells of the pre-push state; a component off its grid keeps its cell
+    double vp, vq, vf;
+    kaos_ena_vcomp(X, &vp, &vq, &vf);
+    const uint32_t old = evc[k];
+    const double *GHp = evp + T.epoff, *GHq = evq + T.eqoff, *GHf = evf + T.efoff;
+    int ip = kaos_find_vcell(vp, GHp, GHp[T.nvp], T.nvp);
+    int iq = kaos_find_vcell(vq, GHq, GHq[T.nvq], T.nvq);
+    int jf = kaos_find_vcell(vf, GHf, GHf[T.nvphi], T.nvphi);
+    if (!ip) ip = (int)(old & 0xFFu);
+    if (!iq) iq = (int)((old >> 8) & 0xFFu);
+    if (!jf) jf = (int)((old >> 16) & 0xFFu);
+    evc[k] = (uint32_t)ip | ((uint32_t)iq << 8) | ((uint32_t)jf << 16);
+    if (master) {
+      atomicAdd(&nq_ena[T.goff + Q - 1], 1u);
+      if (ip && iq && jf)
+        atomicAdd(&ena_hist[T.ehoff + (long long)(Q - 1) * T.nvp * T.nvq * T.nvphi +
+                            ((long long)(ip - 1) * T.nvq + (iq - 1)) * T.nvphi + (jf - 1)], 1u);
+    }
+  }
   kaos_ena_push(X, T.h, T.grav);
   for (int i = 0; i < 6; ++i) ena[(long long)i * ena_tot + k] = X[i];
 }
@@ -1687,6 +1761,7 @@ struct HostTube {
   std::vector<double> ea_in;                // input field (kaos_set_field), nq values
   int ea_mode = 0;                          // KAOS_FIELD_*
   std::vector<double> nsneut;               // charge exchange: nsnormCNeutGT per cell
+  std::vector<double> vpGL, vpGH, vpGC, dvp, vqGL, vqGH, vqGC, dvq, vphiGL, vphiGH, vphiGC, dvphi, ena_hh, ena_d33v;
 };
 
 // One set of particle-state arrays (the binary64 or the mixed form's words).
@@ -1766,6 +1841,14 @@ struct kaos_ctx {
   int* d_eq = nullptr;
   unsigned int* d_nq_ena = nullptr;
   unsigned long long* d_ena_esc = nullptr;
+  // ENA velocity grid (per tube and axis [high edges, GL0 | centres | widths]
+  // at DevTube.ep/eq/efoff; per-bin hh and d33v at eboff), per-ENA packed
+  // cells ip | iq << 8 | iphi << 16, phase-space counts [Q][bin] at ehoff and
+  // the eight moments per cell ([8][nq] per tube at 8 goff)
+  double *d_ena_vp = nullptr, *d_ena_vq = nullptr, *d_ena_vf = nullptr;
+  double *d_ena_hh = nullptr, *d_ena_d33v = nullptr, *d_ena_M = nullptr;
+  unsigned int *d_evc = nullptr, *d_ena_hist = nullptr;
+  long long ena_hist_tot = 0;
   int4* d_vbox = nullptr;                       // per tube velocity box of the push's LDS window
   int4* d_hbox = nullptr;                       // per (tube, cell) bounding box of the master step's counts
   unsigned int *d_vmarg_par = nullptr, *d_vmarg_perp = nullptr;
@@ -1934,12 +2017,32 @@ extern "C" int kaos_set_grid(kaos_ctx* c, int32_t tube, const kaos_grid* g, int6
     for (int i = 0; i < g->nq; ++i)
       if (!(g->ns_neut[i] >= 0.0) || !std::isfinite(g->ns_neut[i]))
         return fail(c, KAOS_E_ARG, "ns_neut must be finite and >= 0");
+    if (g->nvp < 1 || g->nvp > KAOS_MAXV || g->nvq < 1 || g->nvq > KAOS_MAXV || g->nvphi < 1 || g->nvphi > KAOS_MAXV)
+      return fail(c, KAOS_E_ARG, "nvp/nvq/nvphi must be in [1, 255]");
+    const double* ne[] = {g->vpGL, g->vpGH, g->vpGC, g->dvp, g->vqGL, g->vqGH, g->vqGC, g->dvq,
+                          g->vphiGL, g->vphiGH, g->vphiGC, g->dvphi, g->ena_hh, g->ena_d33v};
+    for (const double* p : ne) if (!p) return fail(c, KAOS_E_ARG, "null ENA velocity grid array");
+    if (!contiguous(g->vpGL, g->vpGH, g->nvp) || !contiguous(g->vqGL, g->vqGH, g->nvq) ||
+        !contiguous(g->vphiGL, g->vphiGH, g->nvphi))
+      return fail(c, KAOS_E_ARG, "ENA velocity edges not contiguous");
   }
   HostTube& h = c->ht[tube];
   h.g = *g;
   int nq = g->nq, np = g->nvpar, nr = g->nvperp;
-  if (g->cx) h.nsneut.assign(g->ns_neut, g->ns_neut + nq);
-  else h.nsneut.clear();
+  if (g->cx) {
+    h.nsneut.assign(g->ns_neut, g->ns_neut + nq);
+    const int a = g->nvp, b = g->nvq, f = g->nvphi;
+    const size_t nb = (size_t)a * b * f;
+    h.vpGL.assign(g->vpGL, g->vpGL + a); h.vpGH.assign(g->vpGH, g->vpGH + a);
+    h.vpGC.assign(g->vpGC, g->vpGC + a); h.dvp.assign(g->dvp, g->dvp + a);
+    h.vqGL.assign(g->vqGL, g->vqGL + b); h.vqGH.assign(g->vqGH, g->vqGH + b);
+    h.vqGC.assign(g->vqGC, g->vqGC + b); h.dvq.assign(g->dvq, g->dvq + b);
+    h.vphiGL.assign(g->vphiGL, g->vphiGL + f); h.vphiGH.assign(g->vphiGH, g->vphiGH + f);
+    h.vphiGC.assign(g->vphiGC, g->vphiGC + f); h.dvphi.assign(g->dvphi, g->dvphi + f);
+    h.ena_hh.assign(g->ena_hh, g->ena_hh + nb); h.ena_d33v.assign(g->ena_d33v, g->ena_d33v + nb);
+  } else {
+    h.nsneut.clear();
+  }
   if (v3d) {
     const int n1 = g->nvperp1, n2 = g->nvperp2;
     h.v1GL.assign(g->vperp1GL, g->vperp1GL + n1); h.v1GH.assign(g->vperp1GH, g->vperp1GH + n1);
@@ -1971,6 +2074,7 @@ static int build_device(kaos_ctx* c) {
   std::vector<double> qGH, qGC, hq, dq, Te, d3x, EG, EAin, vparGHx, vperpGHx, vparGC, vperpGC, dvpar, dvperp, d3v;
   std::vector<double> rGH, rref;
   std::vector<int> rk, vk;
+  std::vector<double> ena_vp, ena_vq, ena_vf, ena_hh, ena_d33;     // charge exchange: ENA grids
   std::vector<double> v1GHx, v1GC, dv1, v2GHx, v2GC, dv2, d3v3;
   long long v1off = 0, v2off = 0, h3off = 0, d3off = 0;
   const bool mixed = c->cfg.precision == KAOS_MIXED;
@@ -2008,6 +2112,24 @@ static int build_device(kaos_ctx* c) {
       T.eoff = c->ena_tot; T.ecap = g.ena_cap;
       c->ena_tot += g.ena_cap;
       c->any_cx = 1;
+      T.nvp = g.nvp; T.nvq = g.nvq; T.nvphi = g.nvphi;
+      // per tube and axis a block [GH..., GL0 | GC..., 0 | widths..., 0] starting at e*off
+      T.epoff = (long long)ena_vp.size(); T.eqoff = (long long)ena_vq.size(); T.efoff = (long long)ena_vf.size();
+      auto axis = [](std::vector<double>& v, const std::vector<double>& GL, const std::vector<double>& GH,
+                     const std::vector<double>& GC, const std::vector<double>& dv) {
+        const size_t n = GH.size(), base = v.size();
+        v.resize(base + 3 * (n + 1), 0.0);
+        for (size_t i = 0; i < n; ++i) { v[base + i] = GH[i]; v[base + (n + 1) + i] = GC[i]; v[base + 2 * (n + 1) + i] = dv[i]; }
+        v[base + n] = GL[0];
+      };
+      axis(ena_vp, h.vpGL, h.vpGH, h.vpGC, h.dvp);
+      axis(ena_vq, h.vqGL, h.vqGH, h.vqGC, h.dvq);
+      axis(ena_vf, h.vphiGL, h.vphiGH, h.vphiGC, h.dvphi);
+      T.eboff = (long long)ena_hh.size();
+      ena_hh.insert(ena_hh.end(), h.ena_hh.begin(), h.ena_hh.end());
+      ena_d33.insert(ena_d33.end(), h.ena_d33v.begin(), h.ena_d33v.end());
+      T.ehoff = c->ena_hist_tot;
+      c->ena_hist_tot += (long long)g.nq * g.nvp * g.nvq * g.nvphi;
     }
     T.icrp.qs = g.qs; T.icrp.ms = g.ms; T.icrp.h = g.h;
     T.icrp.lambda_perp = g.icr_lambda_perp; T.icrp.eta_lh = g.icr_eta_lh;
@@ -2262,6 +2384,16 @@ static int build_device(kaos_ctx* c) {
     HIPCHK(c, hipMalloc((void**)&c->d_nq_ena, sizeof(unsigned int) * goff));
     HIPCHK(c, hipMemset(c->d_nq_ena, 0, sizeof(unsigned int) * goff));
     HIPCHK(c, up((void**)&c->d_ena_esc, ze.data(), sizeof(unsigned long long) * 2 * nt));
+    HIPCHK(c, up((void**)&c->d_ena_vp, ena_vp.data(), sizeof(double) * ena_vp.size()));
+    HIPCHK(c, up((void**)&c->d_ena_vq, ena_vq.data(), sizeof(double) * ena_vq.size()));
+    HIPCHK(c, up((void**)&c->d_ena_vf, ena_vf.data(), sizeof(double) * ena_vf.size()));
+    HIPCHK(c, up((void**)&c->d_ena_hh, ena_hh.data(), sizeof(double) * ena_hh.size()));
+    HIPCHK(c, up((void**)&c->d_ena_d33v, ena_d33.data(), sizeof(double) * ena_d33.size()));
+    std::vector<double> z8e(8 * goff, 0.0);
+    HIPCHK(c, up((void**)&c->d_ena_M, z8e.data(), sizeof(double) * 8 * goff));
+    HIPCHK(c, hipMalloc((void**)&c->d_evc, sizeof(unsigned int) * ne));
+    HIPCHK(c, hipMalloc((void**)&c->d_ena_hist, sizeof(unsigned int) * (size_t)std::max(c->ena_hist_tot, 1LL)));
+    HIPCHK(c, hipMemset(c->d_ena_hist, 0, sizeof(unsigned int) * (size_t)std::max(c->ena_hist_tot, 1LL)));
   }
   // injection bookkeeping: ions per tube per master step and their prefix
   std::vector<long long> ib(nt + 1, 0), ic(nt, 0);
@@ -2627,6 +2759,12 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
     const int nr = std::max(ncell, 2 * nt);
     hipLaunchKernelGGL(k_ena_reset, dim3((nr + 255) / 256), dim3(256), 0, c->stream, ncell, nt, c->d_nq_ena,
                        c->d_ena_esc);
+    hipLaunchKernelGGL(k_fill_u32, dim3(256), dim3(256), 0, c->stream, c->d_ena_hist, c->ena_hist_tot);
+  };
+  auto ena_moments = [&]() {
+    hipLaunchKernelGGL(k_ena_moments, dim3(ncell), dim3(KAOS_ENA_LANES), 0, c->stream, c->d_tubes, c->d_blk_tube,
+                       c->d_blk_q, c->d_nq_ena, c->d_ena_hist, c->d_ena_vp, c->d_ena_vq, c->d_ena_vf, c->d_ena_hh,
+                       c->d_ena_d33v, c->d_d3x, c->d_ena_M);
   };
   auto cx_convert = [&]() {
     StepArgs a = make_args(c, 0, 0, 0, 0);
@@ -2639,13 +2777,14 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
                          c->d_cx_pref, c->d_cx_thr, c->d_cx_hist);
     }
     hipLaunchKernelGGL(k_cx_convert, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a, mixed, c->d_cx_thr,
-                       c->d_ena, c->d_eid, c->d_eq, c->d_ena_cnt, c->ena_tot, c->d_live);
+                       c->d_ena, c->d_eid, c->d_eq, c->d_evc, c->d_ena_cnt, c->ena_tot, c->d_live);
   };
   auto ena_step = [&](int m) {
     if (c->ena_tot > 0)
       hipLaunchKernelGGL(k_ena_step, dim3((unsigned)((c->ena_tot + 255) / 256)), dim3(256), 0, c->stream,
                          c->d_tubes, nt, c->d_eoff, c->d_ena_cnt, c->ena_tot, c->d_ena, c->d_eq, c->d_qGH,
-                         c->d_nq_ena, c->d_ena_esc, m);
+                         c->d_nq_ena, c->d_ena_esc, m, c->d_evc, c->d_ena_vp, c->d_ena_vq, c->d_ena_vf,
+                         c->d_ena_hist);
   };
   if (first) {
     StepArgs a = make_args(c, 1, 0, 0, premaster);
@@ -2658,7 +2797,7 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
     hipLaunchKernelGGL(k_field, dim3(nt), dim3(256), 0, c->stream, c->d_tubes, c->d_hq, c->d_dq, c->d_Te,
                        c->d_M, c->d_M0F, c->d_M1F, c->d_EA, c->d_EAin, c->d_esc_acc, c->d_esc_rep, 0);
     launch_moments(c, nullptr);
-    if (c->any_cx) { ena_reset(); cx_freq(); }
+    if (c->any_cx) { ena_reset(); ena_moments(); cx_freq(); }
     launch_box_update(c);
     if (c->any_icr && c->cfg.precision != KAOS_MIXED)
       hipLaunchKernelGGL(k_icr, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a);
@@ -2668,6 +2807,7 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
       cx_convert();
       if (master) ena_reset();
       ena_step(master ? 1 : 0);
+      if (master) ena_moments();
     }
     if (master) {
       hipLaunchKernelGGL(k_field, dim3(nt), dim3(256), 0, c->stream, c->d_tubes, c->d_hq, c->d_dq, c->d_Te,
@@ -3279,6 +3419,22 @@ extern "C" int kaos_download_enas(kaos_ctx* c, int32_t tube, int64_t n_cap, int6
   return KAOS_OK;
 }
 
+extern "C" int kaos_download_ena_moments(kaos_ctx* c, int32_t tube, int64_t* counts, double* M) {
+  if (!c) return KAOS_E_ARG;
+  if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
+  if (!c->built || !c->dt[tube].cx) return fail(c, KAOS_E_STATE, "no charge exchange in this tube");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const DevTube& T = c->dt[tube];
+  if (counts) {
+    const size_t n = (size_t)T.nq * T.nvp * T.nvq * T.nvphi;
+    std::vector<unsigned int> u(n);
+    HIPCHK(c, hipMemcpy(u.data(), c->d_ena_hist + T.ehoff, sizeof(unsigned int) * n, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) counts[i] = u[i];
+  }
+  if (M) HIPCHK(c, hipMemcpy(M, c->d_ena_M + 8 * T.goff, sizeof(double) * 8 * T.nq, hipMemcpyDeviceToHost));
+  return KAOS_OK;
+}
+
 extern "C" int kaos_download_ena_master(kaos_ctx* c, int32_t tube, int64_t* nq_ena, double* esc) {
   if (!c) return KAOS_E_ARG;
   if (tube < 0 || tube >= c->cfg.ntubes) return fail(c, KAOS_E_ARG, "tube out of range");
@@ -3741,7 +3897,9 @@ extern "C" int kaos_finalize(kaos_ctx* c) {
                 c->d_grp3};
   for (void* p : b3) if (p) (void)hipFree(p);
   void* bcx[] = {c->d_nsneut, c->d_cxgeo, c->d_cx, c->d_cx_need, c->d_cx_pref, c->d_cx_thr, c->d_cx_hist,
-                 c->d_ena, c->d_eid, c->d_ena_cnt, c->d_eoff, c->d_eq, c->d_nq_ena, c->d_ena_esc};
+                 c->d_ena, c->d_eid, c->d_ena_cnt, c->d_eoff, c->d_eq, c->d_nq_ena, c->d_ena_esc,
+                 c->d_ena_vp, c->d_ena_vq, c->d_ena_vf, c->d_ena_hh, c->d_ena_d33v, c->d_ena_M, c->d_evc,
+                 c->d_ena_hist};
   for (void* p : bcx) if (p) (void)hipFree(p);
   for (StateBufs& B : c->st) {
     void* sb[] = {B.rs, B.vp, B.vr, B.qk, B.ae, B.ag, B.rdi, B.vpi, B.lo, B.vrf, B.qkf, B.aef, B.agf, B.cell, B.id,

@@ -48,6 +48,12 @@ module kaos_hip
     real(c_double) :: t_neut = 0, m_neut = 0, cx_sigma_scale = 1
     type(c_ptr) :: ns_neut = c_null_ptr
     integer(c_int64_t) :: ena_cap = 0
+    ! ENA velocity grid (|Vp|, Vq, Vphi cells; per bin hVp hVq hVphi and d33vCT)
+    integer(c_int32_t) :: nvp = 0, nvq = 0, nvphi = 0, ena_pad = 0
+    type(c_ptr) :: vpGL = c_null_ptr, vpGH = c_null_ptr, vpGC = c_null_ptr, dvp = c_null_ptr
+    type(c_ptr) :: vqGL = c_null_ptr, vqGH = c_null_ptr, vqGC = c_null_ptr, dvq = c_null_ptr
+    type(c_ptr) :: vphiGL = c_null_ptr, vphiGH = c_null_ptr, vphiGC = c_null_ptr, dvphi = c_null_ptr
+    type(c_ptr) :: ena_hh = c_null_ptr, ena_d33v = c_null_ptr
   end type kaos_grid
 
   ! dir: c_loc of a NUL-terminated character array (trim(dataexportdir)//c_null_char)
@@ -86,7 +92,7 @@ module kaos_hip
             kaos_seed_particles, kaos_step, kaos_run, &
             kaos_download_master, kaos_download_master3d, kaos_export_master, kaos_export_spinup, &
             kaos_read_spinup, kaos_set_field, kaos_download_cx, kaos_download_enas, &
-            kaos_download_ena_master, &
+            kaos_download_ena_master, kaos_download_ena_moments, &
             kaos_count, kaos_download_particles, kaos_download_vperp, kaos_upload_state, &
             kaos_upload_moments, kaos_resume, kaos_save_state, kaos_load_state, kaos_set_launch, &
             kaos_get_launch, kaos_profile, &
@@ -211,6 +217,14 @@ module kaos_hip
       integer(c_int32_t), value :: tube
       type(c_ptr), value :: nq_ena, esc
     end function kaos_download_ena_master
+
+    ! ENA phase-space counts [Q][Vp][Vq][Vphi] and the eight ENA moments of the last master step
+    integer(c_int) function kaos_download_ena_moments(ctx, tube, counts, M) bind(c, name='kaos_download_ena_moments')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tube
+      type(c_ptr), value :: counts, M
+    end function kaos_download_ena_moments
 
     ! the spin-up files of a previous run (SPINUPflag = 0 restart); dir as in kaos_spinup_info
     integer(c_int) function kaos_read_spinup(dir, species, flux_tube, nq_cap, st, density, temperature, &

@@ -91,6 +91,9 @@ class Tube:
     m_neut: float = 0.0
     cx_sigma_scale: float = 1.0
     ena_cap: int = 0
+    nvp: int = 0
+    nvq: int = 0
+    nvphi: int = 0
 
     @property
     def cphi(self):
@@ -136,7 +139,7 @@ def with_vgrid3(tube, nv1, nv2, vperp_max=None):
 
 
 def with_charge_exchange(tube, n0=1.423e11, r0=None, H=2.0e5, t_neut=848.0, filt=0, filt_pt=11, ena_cap=None,
-                         sigma_scale=1.0):
+                         sigma_scale=1.0, ena_grid=(8, 12, 10)):
     """Switch on ion-neutral charge exchange (QEXCHANGEflag = 1; the tube needs
     the 3-D grid).  The neutrals (mass of the tube's species, TNeut kelvin)
     have an exponential density n0 exp(-(r - r0)/H) at the cell centres (H =
@@ -145,7 +148,8 @@ def with_charge_exchange(tube, n0=1.423e11, r0=None, H=2.0e5, t_neut=848.0, filt
     (_0_1_1_ConfigGridGeneratorB.f90:113-114).  filt: MOMENTFILTERflag with
     the M1Perp1/M1Perp2/M2Par window filt_pt.  ena_cap: ENA slots (default:
     the initial ions).  sigma_scale multiplies the cross-section (1: the
-    reference's expression as written).  Returns the tube."""
+    reference's expression as written).  ena_grid: (Vp, Vq, Vphi) cells of
+    the ENA velocity grid.  Returns the tube."""
     if not tube.v3d:
         raise ValueError("charge exchange needs the 3-D velocity grid (with_vgrid3)")
     r0 = RE + 7.19e5 if r0 is None else r0
@@ -155,6 +159,17 @@ def with_charge_exchange(tube, n0=1.423e11, r0=None, H=2.0e5, t_neut=848.0, filt
     tube.cx, tube.cx_filter, tube.cx_filter_pt = 1, int(filt), int(filt_pt)
     tube.t_neut, tube.m_neut = float(t_neut), float(tube.ms)
     tube.cx_sigma_scale = float(sigma_scale)
+    # ENA velocity grid: |Vp| in [0, vmax], Vq and Vphi in [-vmax, vmax], the
+    # Vpar range of the tube; unit metric factors (d33v = dVp dVq dVphi)
+    vmax = float(a["vparGH"][-1])
+    for name, n, lo in (("vp", ena_grid[0], 0.0), ("vq", ena_grid[1], -vmax), ("vphi", ena_grid[2], -vmax)):
+        e = lo + (vmax - lo) * np.arange(n + 1) / n
+        a[name + "GL"], a[name + "GH"] = e[:-1].copy(), e[1:].copy()
+        a[name + "GC"] = (a[name + "GL"] + a[name + "GH"]) / 2.0
+        a["d" + name] = a[name + "GH"] - a[name + "GL"]
+    tube.nvp, tube.nvq, tube.nvphi = ena_grid
+    a["ena_hh"] = np.ones(ena_grid[0] * ena_grid[1] * ena_grid[2])
+    a["ena_d33v"] = ((a["dvp"][:, None, None] * a["dvq"][None, :, None]) * a["dvphi"][None, None, :]).ravel()
     tube.ena_cap = int(tube.cell_counts.sum()) if ena_cap is None else int(ena_cap)
     return tube
 

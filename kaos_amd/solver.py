@@ -53,6 +53,11 @@ def grid_struct(tube):
         a = np.ascontiguousarray(tube.arrays["nsneut"], dtype=np.float64)
         keep.append(a)
         g.ns_neut = ptr(a)
+        g.nvp, g.nvq, g.nvphi = tube.nvp, tube.nvq, tube.nvphi
+        for n in _abi.ENA_ARRAYS:
+            a = np.ascontiguousarray(tube.arrays[n], dtype=np.float64)
+            keep.append(a)
+            setattr(g, n, ptr(a))
     if getattr(tube, "v3d", 0):
         for n, tn in zip(_abi.GRID_ARRAYS3, _abi.TUBE_ARRAYS3):
             if tube.arrays.get(tn) is None:
@@ -202,6 +207,18 @@ class KineticSolver:
         k = n.value
         out = {name: b[:k].copy() for name, b in zip(("x", "y", "z", "Vx", "Vy", "Vz"), bufs)}
         out["id"], out["qind"] = ids[:k].copy(), q[:k].copy()
+        return out
+
+    def ena_moments(self, tube):
+        """kaos_download_ena_moments: ENA phase-space counts [Q][Vp][Vq][Vphi] and
+        M0, M1P, M1Q, M1PHI, M2, M2P, M2Q, M2PHI per cell."""
+        t = self.tubes[tube]
+        n = np.zeros(t.nq * t.nvp * t.nvq * t.nvphi, np.int64)
+        M = np.zeros(8 * t.nq)
+        self._chk(self.lib.kaos_download_ena_moments(self.h, tube, ptr(n, C.c_int64), ptr(M)))
+        out = {"counts": n.reshape(t.nq, t.nvp, t.nvq, t.nvphi)}
+        for k, name in enumerate(("M0", "M1P", "M1Q", "M1PHI", "M2", "M2P", "M2Q", "M2PHI")):
+            out[name] = M[k * t.nq:(k + 1) * t.nq].copy()
         return out
 
     def ena_master(self, tube):

@@ -21,6 +21,9 @@ ARRAYS = ("qGL", "qGH", "qGC", "hqC", "dqC", "d3xC", "rGC", "cthGC", "ellGC", "T
           "vparGL", "vparGH", "vparGC", "dvpar", "vperpGL", "vperpGH", "vperpGC", "dvperp", "d3v")
 MOMENTS3 = ("M0", "M1perp1", "M1perp2", "M1par", "M2", "M2perp1", "M2perp2", "M2par")
 ARRAYS3 = ("v1GL", "v1GH", "v1GC", "dv1", "v2GL", "v2GH", "v2GC", "dv2", "d3v3")
+ENA_ARRAYS = ("vpGL", "vpGH", "vpGC", "dvp", "vqGL", "vqGH", "vqGC", "dvq", "vphiGL", "vphiGH", "vphiGC", "dvphi",
+              "ena_hh", "ena_d33v")
+ENA_MOMENTS = ("M0", "M1P", "M1Q", "M1PHI", "M2", "M2P", "M2Q", "M2PHI")
 
 
 class OrcCfg(C.Structure):
@@ -38,7 +41,9 @@ class OrcCfg(C.Structure):
                 + [(n, _dp) for n in ARRAYS3]
                 + [(n, C.c_int) for n in ("cx", "cx_filter", "cx_filter_pt", "pad_cx")]
                 + [("t_neut", C.c_double), ("m_neut", C.c_double), ("cx_sigma_scale", C.c_double),
-                   ("nsneut", _dp), ("ena_cap", C.c_int64)])
+                   ("nsneut", _dp), ("ena_cap", C.c_int64)]
+                + [(n, C.c_int) for n in ("nvp", "nvq", "nvphi", "pad_ena")]
+                + [(n, _dp) for n in ENA_ARRAYS])
 
 
 _lib = None
@@ -129,6 +134,9 @@ def load():
         lib.orc_ena_count.restype = C.c_int64
         lib.orc_get_enas.argtypes = [P, _dp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         lib.orc_get_ena_master.argtypes = [P, _dp, _dp]
+        lib.orc_get_ena_moments.argtypes = [P, _dp, _dp]
+        lib.orc_ena_moments_from_counts.argtypes = [C.POINTER(OrcCfg), _dp, _dp, _dp]
+        lib.orc_ena_vcomp.argtypes = [_dp, _dp]
         lib.orc_cx_cell.argtypes = [_dp, C.c_uint64, C.c_int64, C.c_int, C.c_int, _dp]
         lib.orc_poisson.argtypes = [C.c_double, C.c_uint64, C.c_int64, C.c_int, C.c_int]
         lib.orc_poisson.restype = C.c_double
@@ -173,6 +181,11 @@ def cfg_of(tube, ndatfac=1, seed=1, tube_index=0):
         a = np.ascontiguousarray(tube.arrays["nsneut"], dtype=np.float64)
         keep.append(a)
         c.nsneut = ptr(a)
+        c.nvp, c.nvq, c.nvphi = tube.nvp, tube.nvq, tube.nvphi
+        for n in ENA_ARRAYS:
+            a = np.ascontiguousarray(tube.arrays[n], dtype=np.float64)
+            keep.append(a)
+            setattr(c, n, ptr(a))
     for n in ARRAYS + (ARRAYS3 if c.v3d else ()):
         a = np.ascontiguousarray(tube.arrays[n], dtype=np.float64)
         keep.append(a)
@@ -230,6 +243,18 @@ class Oracle:
         X = X[:6 * n].reshape(n, 6)
         out = {k: X[:, i].copy() for i, k in enumerate(("x", "y", "z", "Vx", "Vy", "Vz"))}
         out["id"], out["qind"] = ids[:n], q[:n]
+        return out
+
+    def ena_moments(self):
+        """ENA phase-space counts [Q][Vp][Vq][Vphi] and the eight ENA moments of the
+        last master step."""
+        t = self.tube
+        n = np.zeros(t.nq * t.nvp * t.nvq * t.nvphi)
+        M = np.zeros(8 * t.nq)
+        self.lib.orc_get_ena_moments(self.h, ptr(n), ptr(M))
+        out = {"counts": n.astype(np.int64).reshape(t.nq, t.nvp, t.nvq, t.nvphi)}
+        for k, name in enumerate(ENA_MOMENTS):
+            out[name] = M[k * t.nq:(k + 1) * t.nq].copy()
         return out
 
     def ena_master(self):

@@ -63,6 +63,11 @@ typedef struct {
   double t_neut, m_neut, cx_sigma_scale;
   const double *nsneut;
   int64_t ena_cap;
+  /* ENA velocity grid: Vp, Vq, Vphi cells; per bin [ip][iq][iphi] the metric
+   * product hVp hVq hVphi and d33vCT */
+  int nvp, nvq, nvphi, pad_ena;
+  const double *vpGL, *vpGH, *vpGC, *dvp, *vqGL, *vqGH, *vqGC, *dvq, *vphiGL, *vphiGH, *vphiGC, *dvphi;
+  const double *ena_hh, *ena_d33v;
 } orc_cfg;
 
 typedef struct {
@@ -96,7 +101,9 @@ typedef struct {
   double *cx_sigma, *cx_nu, *cx_poi;
   double *ex, *ey, *ez, *evx, *evy, *evz;
   int64_t *eid;
-  int *eq;
+  int *eq, *evp, *evq, *evphi;     /* ENA Qindk1 and Vp/Vq/Vphi cells (stale off the grid) */
+  uint32_t *ena_hist;              /* NphReNormENAT per cell, [Q][ip][iq][iphi] */
+  double *ena_M;                   /* 8*nq: M0, M1P, M1Q, M1PHI, M2, M2P, M2Q, M2PHI of the ENAs */
   int64_t ens, ena_made;
   uint32_t *nq_ena;
   double ena_esc_lb, ena_esc_ub, ena_flux_lb, ena_flux_ub;
@@ -371,6 +378,9 @@ void *orc_create(const orc_cfg *cfg, int64_t ns, const double *q0, const double 
     o->evy = (double *)xc(sizeof(double) * ec); o->evz = (double *)xc(sizeof(double) * ec);
     o->eid = (int64_t *)xc(sizeof(int64_t) * ec);
     o->eq = (int *)xc(sizeof(int) * ec);
+    o->evp = (int *)xc(sizeof(int) * ec); o->evq = (int *)xc(sizeof(int) * ec); o->evphi = (int *)xc(sizeof(int) * ec);
+    o->ena_hist = (uint32_t *)xc(sizeof(uint32_t) * nq * (int64_t)cfg->nvp * cfg->nvq * cfg->nvphi);
+    o->ena_M = (double *)xc(sizeof(double) * 8 * nq);
   }
   /* initial conditions: x0 = root(q0), V0 = Vpar0 b-hat(x0) */
   for (int64_t j = 0; j < ns; ++j) {
@@ -397,6 +407,7 @@ void orc_destroy(void *h) {
   free(o->EA); free(o->EG); free(o->M0F); free(o->M1F); free(o->n3_u); free(o->M3); free(o->EAin);
   free(o->cx_sigma); free(o->cx_nu); free(o->cx_poi); free(o->nq_ena);
   free(o->ex); free(o->ey); free(o->ez); free(o->evx); free(o->evy); free(o->evz); free(o->eid); free(o->eq);
+  free(o->evp); free(o->evq); free(o->evphi); free(o->ena_hist); free(o->ena_M);
   free(o);
 }
 
@@ -573,6 +584,7 @@ static int cx_convert(orc_t *o) {
       o->evx[e] = X[3]; o->evy[e] = X[4]; o->evz[e] = X[5];
       o->eid[e] = o->id[j];
       o->eq[e] = o->qind[j];
+      o->evp[e] = o->evq[e] = o->evphi[e] = 0;
       ++o->ena_made;
       continue;
     }
@@ -599,6 +611,7 @@ static void ena_remove(orc_t *o) {
     o->ex[k] = o->ex[e]; o->ey[k] = o->ey[e]; o->ez[k] = o->ez[e];
     o->evx[k] = o->evx[e]; o->evy[k] = o->evy[e]; o->evz[k] = o->evz[e];
     o->eid[k] = o->eid[e]; o->eq[k] = o->eq[e];
+    o->evp[k] = o->evp[e]; o->evq[k] = o->evq[e]; o->evphi[k] = o->evphi[e];
     ++k;
   }
   o->ens = k;
@@ -609,21 +622,97 @@ static void ena_remove(orc_t *o) {
   o->ena_flux_ub = ((double)nub * c->nsnormfac) / ((c->d3xC[ub] / (c->hqC[ub] * c->dqC[ub])) * dtm);
 }
 
+/* ENADistributionFunctionsSub and the eight ENA moments of every cell from
+ * its counts: partial sums of lanes 0..255 (kaos_ena_mom_part), then the
+ * pairwise tree; zeros for a cell without ENAs (NqENART = 0). */
+static void ena_moments(const orc_cfg *c, const uint32_t *nq_ena, const uint32_t *hist, double *M8) {
+  const int nb = c->nvp * c->nvq * c->nvphi;
+  kaos_enagrid_t g = {c->nvp, c->nvq, c->nvphi, c->dvp, c->dvq, c->dvphi, c->vpGC, c->vqGC, c->vphiGC,
+                      c->ena_hh, c->ena_d33v};
+  double P[4][KAOS_ENA_LANES];
+  for (int Q = 0; Q < c->nq; ++Q) {
+    double *M = M8 + 8 * Q;
+    for (int i = 0; i < 8; ++i) M[i] = 0.0;
+    if (nq_ena[Q] == 0) continue;
+    const uint32_t *cnt = hist + (int64_t)Q * nb;
+    const double k = c->nsnormfac / c->d3xC[Q];
+    double S[4], m1[3] = {0.0, 0.0, 0.0};
+    for (int stage = 1; stage <= 2; ++stage) {
+      for (int l = 0; l < KAOS_ENA_LANES; ++l) {
+        double o4[4];
+        kaos_ena_mom_part(&g, cnt, l, k, stage, m1, o4);
+        for (int i = 0; i < 4; ++i) P[i][l] = o4[i];
+      }
+      for (int i = 0; i < 4; ++i) { kaos_tree_sum(P[i], KAOS_ENA_LANES); S[i] = P[i][0]; }
+      if (stage == 1) { kaos_ena_mom_finish1(S, M); m1[0] = M[1]; m1[1] = M[2]; m1[2] = M[3]; }
+      else kaos_ena_mom_finish2(S, c->ms, M);
+    }
+  }
+}
+
+/* The eight ENA moments of given counts (nq * bins) and per-cell totals, as
+ * [8][nq] (for the pins). */
+void orc_ena_moments_from_counts(const orc_cfg *c, const double *counts, const double *nq_ena, double *M8) {
+  const int64_t n = (int64_t)c->nq * c->nvp * c->nvq * c->nvphi;
+  uint32_t *h = (uint32_t *)xc(sizeof(uint32_t) * n), *q = (uint32_t *)xc(sizeof(uint32_t) * c->nq);
+  double *M = (double *)xc(sizeof(double) * 8 * c->nq);
+  for (int64_t i = 0; i < n; ++i) h[i] = (uint32_t)counts[i];
+  for (int Q = 0; Q < c->nq; ++Q) q[Q] = (uint32_t)nq_ena[Q];
+  ena_moments(c, q, h, M);
+  for (int Q = 0; Q < c->nq; ++Q)
+    for (int i = 0; i < 8; ++i) M8[(int64_t)i * c->nq + Q] = M[8 * Q + i];
+  free(h); free(q); free(M);
+}
+
+void orc_ena_vcomp(const double *X, double *o3) { kaos_ena_vcomp(X, &o3[0], &o3[1], &o3[2]); }
+
 /* The ENA push of a step: Qindk1 at the pre-push position (the ion scan of
  * KUA:221-273 on the ENA's q), counted at master steps (ENAParticleCountsSub,
  * _8_3_2:33-60), then the ballistic RK4. */
 static void ena_push(orc_t *o, int master) {
   orc_cfg *c = &o->c;
-  if (master) memset(o->nq_ena, 0, sizeof(uint32_t) * c->nq);
+  const int nb = c->nvp * c->nvq * c->nvphi;
+  if (master) {
+    memset(o->nq_ena, 0, sizeof(uint32_t) * c->nq);
+    memset(o->ena_hist, 0, sizeof(uint32_t) * c->nq * (int64_t)nb);
+  }
   for (int64_t e = 0; e < o->ens; ++e) {
     double X[6] = {o->ex[e], o->ey[e], o->ez[e], o->evx[e], o->evy[e], o->evz[e]};
     double q = kaos_ena_q(X[0], X[1], X[2]);
     int Q = (q == q) ? scan_qcell(c, q, o->eq[e]) : 0;
     o->eq[e] = Q;
-    if (master && Q >= 1) o->nq_ena[Q - 1]++;
+    if (Q >= 1) {
+      /* the ENA velocity cells at the pre-push state, stale off the grid (KUA:1156-1290) */
+      double vp, vq, vphi;
+      kaos_ena_vcomp(X, &vp, &vq, &vphi);
+      o->evp[e] = scan_vcell(c->vpGL, c->vpGH, c->nvp, vp, o->evp[e]);
+      o->evq[e] = scan_vcell(c->vqGL, c->vqGH, c->nvq, vq, o->evq[e]);
+      o->evphi[e] = scan_vcell(c->vphiGL, c->vphiGH, c->nvphi, vphi, o->evphi[e]);
+    }
+    if (master && Q >= 1) {
+      o->nq_ena[Q - 1]++;
+      if (o->evp[e] >= 1 && o->evq[e] >= 1 && o->evphi[e] >= 1)
+        o->ena_hist[(int64_t)(Q - 1) * nb + ((int64_t)(o->evp[e] - 1) * c->nvq + (o->evq[e] - 1)) * c->nvphi +
+                    (o->evphi[e] - 1)]++;
+    }
     kaos_ena_push(X, c->h, c->grav);
     o->ex[e] = X[0]; o->ey[e] = X[1]; o->ez[e] = X[2];
     o->evx[e] = X[3]; o->evy[e] = X[4]; o->evz[e] = X[5];
+  }
+  if (master) ena_moments(c, o->nq_ena, o->ena_hist, o->ena_M);
+}
+
+/* ENA phase-space counts [Q][ip][iq][iphi] and the eight ENA moments per cell
+ * (M0, M1P, M1Q, M1PHI, M2, M2P, M2Q, M2PHI) of the last master step. */
+void orc_get_ena_moments(void *h, double *counts, double *M8) {
+  orc_t *o = (orc_t *)h;
+  orc_cfg *c = &o->c;
+  if (!c->cx) return;
+  int64_t n = (int64_t)c->nq * c->nvp * c->nvq * c->nvphi;
+  if (counts) for (int64_t i = 0; i < n; ++i) counts[i] = (double)o->ena_hist[i];
+  if (M8) {
+    for (int Q = 0; Q < c->nq; ++Q)
+      for (int i = 0; i < 8; ++i) M8[(int64_t)i * c->nq + Q] = o->ena_M[8 * Q + i];
   }
 }
 

@@ -4,7 +4,9 @@ C-ABI, against the oracle.
 FP64 bit-exact after every step: every ion's state and indices, the ENAs
 (Cartesian state, id, Qindk1, in id order), and at master steps the 3-D
 counts and moments, the collision counts (sigmaIonNeutRT, nuIonNeutRT,
-nuIonNeutPoiT), the ENA counts per cell and the LB/UB ENA escapes and fluxes:
+nuIonNeutPoiT), the ENA counts per cell, the ENA phase-space counts on the
+(|Vp|, Vq, Vphi) grid with the eight ENA moments, and the LB/UB ENA escapes
+and fluxes:
 - a 3-D tube, ndatfac 2, raw moments (MOMENTFILTERflag = 0) and filtered ones
   (= 1, M1Par window 9, M1Perp1/M1Perp2/M2Par window 7), with heating-free
   pushes and the seeded injection stream;
@@ -69,6 +71,8 @@ def _check_step(o, ks, n, ndatfac):
         ea, eb = o.ena_master(), ks.ena_master(0)
         assert np.array_equal(ea["nq_ena"], eb["nq_ena"]), n
         assert np.array_equal(ea["esc"], eb["esc"]), n
+        _same(o.ena_moments(), ks.ena_moments(0), ("counts", "M0", "M1P", "M1Q", "M1PHI", "M2", "M2P", "M2Q",
+                                                   "M2PHI"), f"ENA moments {n}")
 
 
 @pytest.mark.parametrize("filt", [0, 1])
@@ -126,8 +130,12 @@ def test_cx_mixed_close_to_oracle():
 
 def test_cx_refusals(tmp_path):
     t = grids.dipole_tube(nq=24, nvpar=20, nvperp=16, n_total=3000, ratio=1.1)
-    t.cx, t.t_neut, t.m_neut, t.cx_sigma_scale, t.ena_cap = 1, 848.0, t.ms, 1.0, 100
-    t.arrays["nsneut"] = np.ones(t.nq)
+    src = _tube()
+    for k in ("cx", "t_neut", "m_neut", "cx_sigma_scale", "ena_cap", "nvp", "nvq", "nvphi"):
+        setattr(t, k, getattr(src, k))
+    for k in ("nsneut", "vpGL", "vpGH", "vpGC", "dvp", "vqGL", "vqGH", "vqGC", "dvq", "vphiGL", "vphiGH",
+              "vphiGC", "dvphi", "ena_hh", "ena_d33v"):
+        t.arrays[k] = src.arrays[k]
     with pytest.raises(KaosError) as e:                     # needs the 3-D grid
         KineticSolver([t], precision=KAOS_FP64)
     assert e.value.code == 1

@@ -19,8 +19,14 @@
 * The ballistic ENA push against a numpy RK4 of the same stages, the exact
   straight line without gravity, and energy conservation under gravity.
 * An oracle run: each step's conversions are the first nint(poi) ions of
-  every cell in id order; ions + ENAs + escapes + removed = the ions ever
-  present.
+  every cell in id order (raw and filtered moments).
+* The ENA velocity components (|Vp|, Vq, Vphi) against the reference's
+  trigonometric forms (theta = acos(z/r), phi = atan2(y, x)).
+* The ENA distribution function and eight moments against a per-bin numpy
+  restatement of _8_4_2 / _8_9 / _8_10_1..8 with non-uniform metric factors
+  (1e-12), and closed forms: one occupied bin (M0 = count nsnormfac/d3x, M1 =
+  the bin centre, second moments 0), two bins (weighted mean and spread), an
+  empty cell (zeros).
 """
 import ctypes as C
 import math
@@ -249,3 +255,95 @@ def test_oracle_converts_the_first_ions_of_each_cell(filt):
     cx = o.cx()
     assert np.all((cx["nu"] == -2.0) | (cx["nu"] >= 0.0)) and np.all(cx["poi"] >= 0)
     o.close()
+
+
+# --- ENA velocity space -------------------------------------------------------
+
+def _np_vcomp(X):
+    """The reference's ENA components at its point: Vpar = b-hat . V
+    (KUA:978-980), Vp = |p-hat . V| and Vphi = phi-hat . V (KUA:1156-1160),
+    with theta = acos(z/r) and phi = atan2(y, x)."""
+    x, y, z, vx, vy, vz = X
+    r = math.sqrt(x * x + y * y + z * z)
+    th, ph = math.acos(z / r), math.atan2(y, x)
+    c, s, ell = math.cos(th), math.sin(th), 1 + 3 * math.cos(th) ** 2
+    vpar = 3 * c * s * (vx * math.cos(ph) + vy * math.sin(ph)) / math.sqrt(ell) + vz * (3 * c * c - 1) / math.sqrt(ell)
+    vp = abs((1 - 3 * c * c) * (vx * math.cos(ph) + vy * math.sin(ph)) / math.sqrt(ell) + 3 * vz * c * s / math.sqrt(ell))
+    vphi = -vx * math.sin(ph) + vy * math.cos(ph)
+    return vp, vpar, vphi
+
+
+def test_ena_velocity_components(lib):
+    rng = np.random.default_rng(21)
+    for _ in range(300):
+        r = rng.uniform(1.1, 3.0) * RE
+        th, ph = rng.uniform(0.2, 2.9), rng.uniform(-3.1, 3.1)
+        X = np.array([r * math.sin(th) * math.cos(ph), r * math.sin(th) * math.sin(ph), r * math.cos(th),
+                      *rng.normal(0, 2e3, 3)])
+        o3 = np.zeros(3)
+        lib.orc_ena_vcomp(ptr(X), ptr(o3))
+        assert np.allclose(o3, _np_vcomp(X), rtol=0, atol=1e-9)
+        assert o3[0] >= 0.0
+
+
+def _ena_tube():
+    t = _cx_tube()
+    return t, oracle.cfg_of(t, 1, 1)
+
+
+def _np_ena_moments(t, counts, ms):
+    """Per-bin restatement of ENADistributionFunctionsSub and the ENA moment
+    integrands (_8_4_2:24-36, _8_9:25-35, _8_10_1..8) for every cell."""
+    a = t.arrays
+    nvp, nvq, nvphi = t.nvp, t.nvq, t.nvphi
+    dV = (a["dvp"][:, None, None] * a["dvq"][None, :, None]) * a["dvphi"][None, None, :]
+    hh = a["ena_hh"].reshape(nvp, nvq, nvphi)
+    d33 = a["ena_d33v"].reshape(nvp, nvq, nvphi)
+    VP, VQ, VF = np.meshgrid(a["vpGC"], a["vqGC"], a["vphiGC"], indexing="ij")
+    out = np.zeros((8, t.nq))
+    for Q in range(t.nq):
+        c = counts[Q]
+        if c.sum() == 0:
+            continue
+        F = (c * (t.nsnormfac / a["d3xC"][Q])) / d33
+        w = dV * hh * F
+        M0 = w.sum()
+        m1 = [(w * V).sum() / M0 for V in (VP, VQ, VF)]
+        d = [m1[0] - VP, m1[1] - VQ, m1[2] - VF]
+        s2 = [(w * (d[0] ** 2 + d[1] ** 2 + d[2] ** 2)).sum()] + [(w * x ** 2).sum() for x in d]
+        out[:, Q] = [M0, *m1, *[6.242e18 * (ms / 2) * x / M0 for x in s2]]
+    return out
+
+
+def test_ena_moments_against_restatement(lib):
+    t, cfg = _ena_tube()
+    # non-uniform metric factors, as the reference's grid generator makes them
+    rng = np.random.default_rng(4)
+    nb = t.nvp * t.nvq * t.nvphi
+    hh = rng.uniform(0.5, 2.0, nb)
+    t.arrays["ena_hh"] = hh
+    dV = ((t.arrays["dvp"][:, None, None] * t.arrays["dvq"][None, :, None]) * t.arrays["dvphi"][None, None, :]).ravel()
+    t.arrays["ena_d33v"] = hh * dV
+    cfg = oracle.cfg_of(t, 1, 1)
+    counts = np.zeros((t.nq, t.nvp, t.nvq, t.nvphi))
+    counts[0] = rng.poisson(3.0, (t.nvp, t.nvq, t.nvphi))           # a populated cell
+    counts[3, 2, 5, 4] = 7                                           # one occupied bin
+    counts[5, 1, 2, 3], counts[5, 6, 9, 8] = 1, 3                    # two bins
+    M = np.zeros(8 * t.nq)
+    lib.orc_ena_moments_from_counts(C.byref(cfg), ptr(counts.ravel()), ptr(counts.reshape(t.nq, -1).sum(1)),
+                                    ptr(M))
+    M = M.reshape(8, t.nq)
+    ref = _np_ena_moments(t, counts, t.ms)
+    assert np.allclose(M, ref, rtol=1e-12, atol=0)
+    a = t.arrays
+    # one bin: M0 = count nsnormfac/d3x (hh and d33v cancel), M1 = the bin centre, second moments 0
+    assert abs(M[0, 3] - 7 * t.nsnormfac / a["d3xC"][3]) < 1e-12 * M[0, 3]
+    assert (M[1, 3], M[2, 3], M[3, 3]) == (a["vpGC"][2], a["vqGC"][5], a["vphiGC"][4])
+    assert np.all(M[4:, 3] == 0.0)
+    # two bins: the count-weighted mean and spread (equal cell volumes)
+    w = np.array([1.0, 3.0])
+    vp = np.array([a["vpGC"][1], a["vpGC"][6]])
+    m1p = (w * vp).sum() / w.sum()
+    assert abs(M[1, 5] - m1p) < 1e-9
+    assert abs(M[5, 5] - 6.242e18 * t.ms / 2 * (w * (vp - m1p) ** 2).sum() / w.sum()) < 1e-9 * M[5, 5]
+    assert np.all(M[:, 1] == 0.0)                                    # empty cell
