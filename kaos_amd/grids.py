@@ -273,16 +273,28 @@ def config4_tubes(ntubes=512, seed=4, scale=1.0):
     return tubes
 
 
-def config5_tubes(n_h=50_000_000, n_o=150_000_000):
-    """Heated two-species case: one L=6 tube (1.05-3 RE, 256 cells, 64x64), H+
-    (n_h ions, 0.5 eV, V grid +-40 km/s) and O+ (n_o ions, 0.3 eV, +-10 km/s) as two
-    tubes of one context, dt = 0.02 s, stochastic ICR heating with the reference's
-    parameters except spectral indices 2 (diffusion coefficient ~ B^-2)."""
+def config5_tubes(n_h=50_000_000, n_o=150_000_000, nv1=32, nv2=32, cx=True, sigma_scale=1e-9,
+                  ena_grid=(16, 64, 16)):
+    """Full physics: one L=6 tube (1.05-3 RE, 256 cells) holding H+ (n_h ions,
+    0.5 eV, V grid +-40 km/s) and O+ (n_o ions, 0.3 eV, +-10 km/s) as two
+    tubes of one context, dt = 0.02 s; stochastic ICR heating with the
+    reference's parameters except spectral indices 2 (diffusion coefficient
+    ~ B^-2); the 3-D velocity grid nv1 x nv2 x 64 (Vperp1, Vperp2, Vpar: 65536
+    cells per config cell, the count of a 64 x 64 x 16 (Vpar, Vperp, phi)
+    grid; nv1 = 0: the 2-D 64 x 64 grid); charge exchange with neutrals of
+    the species' mass, 848 K and an exponential density of 200 km scale
+    height, the cross-section scaled by sigma_scale, ENAs on a (|Vp|, Vq,
+    Vphi) grid of ena_grid cells with slots for 3% of the ions."""
     tubes = []
     for ms, n, T, vmax in ((MH, n_h, 0.5, 4.0e4), (MO, n_o, 0.3, 1.0e4)):
         t = dipole_tube(nq=256, p=6.0, r_ub=3.0, nvpar=64, nvperp=64, n_total=n, T_eV=T, h=0.02,
                         ratio=1.05 ** (64.0 / 256.0), ms=ms, vmax=vmax)
         t.icr = 1
         t.icr_chi_perp1 = t.icr_chi_perp2 = 2.0
+        if nv1:
+            with_vgrid3(t, nv1, nv2, vperp_max=vmax)
+            if cx:
+                with_charge_exchange(t, H=2.0e5, sigma_scale=sigma_scale, ena_grid=ena_grid,
+                                     ena_cap=max(1024, int(0.03 * n)))
         tubes.append(t)
     return tubes
