@@ -427,14 +427,23 @@ def full_physics_run(args):
         f"ICR heating, charge exchange; setup {time.perf_counter() - t0:.1f}s")
     res, m3, enas = {}, {}, {}
     for form, prec in (("fp64", KAOS_FP64), ("mixed", KAOS_MIXED)):
+        tf = time.perf_counter()
         ks = KineticSolver(tubes, precision=prec, ndatfac=1, seed=1, capacity=caps)
         for i, t in enumerate(tubes):
             ks.seed(i, t.cell_counts, np.full(t.nq, t.Ts_lb), 700 + i)
         ks.step(1)
+        log(f"[bench] full physics {form}: seeded, step 1 done ({time.perf_counter() - tf:.1f}s)")
         ks.run(2, args.warmup)
         first, _ = finish_tuning(ks, 2 + args.warmup)
+        log(f"[bench] full physics {form}: warm-up and tuning to step {first - 1} ({time.perf_counter() - tf:.1f}s)")
+        # timed in chunks of at most 100 steps, a line after each
         t1 = time.perf_counter()
-        pushes = ks.run(first, steps)
+        pushes, done = 0, 0
+        while done < steps:
+            k = min(100, steps - done)
+            pushes += ks.run(first + done, k)
+            done += k
+            log(f"[bench] full physics {form}: {done}/{steps} timed steps ({time.perf_counter() - t1:.1f}s)")
         wall = time.perf_counter() - t1
         # both forms end at the same step, whatever their tuning took
         last = first + steps

@@ -27,7 +27,7 @@ extern "C" {
 /* Status codes. */
 #define KAOS_OK            0
 #define KAOS_E_ARG         1   /* invalid argument or configuration */
-#define KAOS_E_CAPACITY    2   /* particle capacity of a tube exceeded */
+#define KAOS_E_CAPACITY    2   /* particle or ENA capacity of a tube exceeded */
 #define KAOS_E_DEVICE      3   /* HIP runtime error */
 #define KAOS_E_STATE       4   /* call out of order (e.g. step before upload) */
 #define KAOS_E_NAN         5   /* a NaN was produced in the particle state */
@@ -133,8 +133,10 @@ typedef struct kaos_grid {
   int64_t ena_cap;         /* ENA slots of the tube (conversions over the whole run) */
   /* The ENA velocity grid (_0_0_VelGridGenerator.f90:482-857): |Vp| cells
    * (nvp), Vq cells (nvq), Vphi cells (nvphi), each with contiguous edges,
-   * centres and widths (<= 255 cells); per bin [ip][iq][iphi] (Vphi fastest)
-   * the metric product hVp hVq hVphi and d33vCT.  An ENA is binned by the
+   * centres and widths (<= 255 cells); per config cell and bin
+   * [Q-1][ip][iq][iphi] (Vphi fastest; nq * nvp * nvq * nvphi values) the
+   * metric product hVp hVq hVphi and d33vCT, which depend on the cell's
+   * thetaGC (_0_0_VelGridGenerator.f90:820-857).  An ENA is binned by the
    * cells of its (|Vp|, Vq, Vphi) at its pre-push position (KUA:1156-1290;
    * off the grid a component keeps its previous cell). */
   int32_t nvp, nvq, nvphi, ena_pad;

@@ -1250,8 +1250,9 @@ KAOS_HD void kaos_ena_vcomp(const double X[6], double* vp, double* vq, double* v
 }
 
 /* The ENA grid of a tube: Vp, Vq, Vphi cells with their centres and widths,
- * and per bin [ip][iq][iphi] (Vphi fastest) the metric product hVp hVq hVphi
- * and d33v = hVp hVq hVphi dVp dVq dVphi (_0_0_VelGridGenerator.f90:820-857). */
+ * and, for the config cell at hand, per bin [ip][iq][iphi] (Vphi fastest) the
+ * metric product hVp hVq hVphi and d33v = hVp hVq hVphi dVp dVq dVphi
+ * (_0_0_VelGridGenerator.f90:820-857; both depend on the cell's thetaGC). */
 typedef struct {
   int nvp, nvq, nvphi;
   const double *dvp, *dvq, *dvphi, *vpGC, *vqGC, *vphiGC, *hh, *d33v;

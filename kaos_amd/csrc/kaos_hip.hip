@@ -76,7 +76,7 @@ struct DevTube {
   long long eoff, ecap;                     // the tube's ENA slots [eoff, eoff + ecap)
   int nvp, nvq, nvphi, ena_pad;             // ENA velocity grid cells
   long long epoff, eqoff, efoff;            // offsets of its Vp / Vq / Vphi cell arrays
-  long long eboff, ehoff;                   // per-bin arrays (hh, d33v); counts [Q][bin]
+  long long eboff, ehoff;                   // metric arrays [Q][bin] (hh, d33v); counts [Q][bin]
 };
 
 struct StepArgs {
@@ -1552,6 +1552,24 @@ __device__ __forceinline__ int kd_cx_cell(const StepArgs& a, const DevTube& T, l
   return need[T.goff + qc - 1] > 0 ? qc : 0;
 }
 
+// Adds 1 to hist[key] for every active lane of the wavefront: lanes with the
+// same key add together (one atomic per distinct key and wavefront).  The ions
+// of a tile share their cell and mostly their id digit, so per-lane atomics on
+// one address would serialise (millions of ions per cell on a large tube).
+__device__ __forceinline__ void kd_wave_count(unsigned int* hist, bool active, uint32_t key) {
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    const unsigned long long m = __ballot(active);
+    if (m == 0ull) break;
+    const int leader = __ffsll((long long)m) - 1;
+    const uint32_t lk = (uint32_t)__shfl((int)key, leader, 64);
+    const bool same = active && key == lk;
+    const unsigned long long ms = __ballot(same);
+    if (lane == leader) atomicAdd(&hist[lk], (unsigned int)__popcll(ms));
+    if (same) active = false;
+  }
+}
+
 __global__ void __launch_bounds__(KAOS_BLOCK) k_cx_hist(StepArgs a, int mixed, int level, const long long* need,
                                                         const long long* pref, unsigned int* hist) {
   const int tile = blockIdx.x, tb = a.tile_tube[tile];
@@ -1559,16 +1577,21 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_cx_hist(StepArgs a, int mixed, i
   if (!T.cx) return;
   const long long end = T.off + a.count[tb];
   const int sh = 30 - 10 * level;
-  for (int it = 0; it < KAOS_TILE_PPT; ++it) {
+  for (int it = 0; it < KAOS_TILE_PPT; ++it) {           // every lane runs every iteration (wave counts)
     const long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
-    if (j >= end) break;
-    const int qc = kd_cx_cell(a, T, j, mixed, need);
-    if (!qc) continue;
-    const long long g = T.goff + qc - 1;
-    const long long id = a.id[j];
-    if (level == 0 && (id >> 40) != 0) set_status(a.status, 1u << KAOS_E_CAPACITY);   // beyond 40-bit ids
-    if (level > 0 && (id >> (sh + 10)) != pref[g]) continue;
-    atomicAdd(&hist[g * KAOS_CX_BINS + ((id >> sh) & (KAOS_CX_BINS - 1))], 1u);
+    bool on = false;
+    uint32_t key = 0u;
+    if (j < end) {
+      const int qc = kd_cx_cell(a, T, j, mixed, need);
+      if (qc) {
+        const long long g = T.goff + qc - 1;
+        const long long id = a.id[j];
+        if (level == 0 && (id >> 40) != 0) set_status(a.status, 1u << KAOS_E_CAPACITY);   // beyond 40-bit ids
+        on = level == 0 || (id >> (sh + 10)) == pref[g];
+        key = (uint32_t)(g * KAOS_CX_BINS + ((id >> sh) & (KAOS_CX_BINS - 1)));
+      }
+    }
+    kd_wave_count(hist, on, key);
   }
 }
 
@@ -1616,22 +1639,41 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_cx_convert(StepArgs a, int mixed
   const DevTube& T = a.tubes[tb];
   if (!T.cx) return;
   const long long end = T.off + a.count[tb];
-  for (int it = 0; it < KAOS_TILE_PPT; ++it) {
+  const int lane = threadIdx.x & 63;
+  for (int it = 0; it < KAOS_TILE_PPT; ++it) {           // every lane runs every iteration (wave slot counts)
     const long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
-    if (j >= end) break;
-    const uint32_t cw = a.cell[j];
-    double r, v;
-    if (!kd_line_state(a, T, j, mixed, r, v)) continue;
-    const int qc = (int)((cw >> 16) & KAOS_QMASK) - 1;
-    if (qc < 1) continue;
-    const long long th = thr[T.goff + qc - 1];
-    if (th < 0 || a.id[j] > th) continue;
+    bool conv = false;
+    double r = 0.0, v = 0.0;
+    int qc = 0;
+    if (j < end) {
+      const uint32_t cw = a.cell[j];
+      if (kd_line_state(a, T, j, mixed, r, v)) {
+        qc = (int)((cw >> 16) & KAOS_QMASK) - 1;
+        if (qc >= 1) {
+          const long long th = thr[T.goff + qc - 1];
+          conv = th >= 0 && a.id[j] <= th;
+        }
+      }
+    }
+    // ENA slots of the wavefront's conversions: one atomic per wavefront
+    const unsigned long long m = __ballot(conv);
+    if (m == 0ull) continue;
+    const int leader = __ffsll((long long)m) - 1;
+    const int nconv = __popcll(m);
+    long long base = 0;
+    if (lane == leader) {
+      base = (long long)atomicAdd((unsigned long long*)&ena_cnt[tb], (unsigned long long)nconv);
+      const long long fit = min(max(T.ecap - base, 0LL), (long long)nconv);   // slots left for them
+      if (fit > 0) atomicAdd((unsigned long long*)&live[tb], (unsigned long long)(-fit));
+    }
+    base = __shfl(base, leader, 64);
+    if (!conv) continue;
+    const long long e = base + __popcll(m & ((1ull << lane) - 1ull));
+    if (e >= T.ecap) { set_status(a.status, 1u << KAOS_E_CAPACITY); continue; }
     const double v1 = mixed ? (double)a.vrf[j] : a.vr[j], v2 = mixed ? (double)a.vrf2[j] : a.vr2[j];
     const double c4 = mixed ? (double)a.ck4f[j] : a.ck4[j];
     double X[6];
     kaos_ena_from_ion(r, v, v1, v2, c4, T.p, T.cphi, T.sphi, X);
-    const long long e = (long long)atomicAdd((unsigned long long*)&ena_cnt[tb], 1ull);
-    if (e >= T.ecap) { set_status(a.status, 1u << KAOS_E_CAPACITY); continue; }
     const long long k = T.eoff + e;
     for (int i = 0; i < 6; ++i) ena[(long long)i * ena_tot + k] = X[i];
     eid[k] = a.id[j];
@@ -1639,7 +1681,6 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_cx_convert(StepArgs a, int mixed
     evc[k] = 0u;
     if (mixed) a.cell[j] = KAOS_DEAD;
     else a.rs[j] = 0.0;
-    atomicAdd((unsigned long long*)&live[tb], (unsigned long long)(-1LL));
   }
 }
 
@@ -1664,7 +1705,8 @@ __global__ void __launch_bounds__(KAOS_ENA_LANES) k_ena_moments(const DevTube* t
   const int nb = T.nvp * T.nvq * T.nvphi;
   const double *GHp = evp + T.epoff, *GHq = evq + T.eqoff, *GHf = evf + T.efoff;
   kaos_enagrid_t g = {T.nvp, T.nvq, T.nvphi, GHp + 2 * (T.nvp + 1), GHq + 2 * (T.nvq + 1), GHf + 2 * (T.nvphi + 1),
-                      GHp + (T.nvp + 1), GHq + (T.nvq + 1), GHf + (T.nvphi + 1), hh + T.eboff, d33v + T.eboff};
+                      GHp + (T.nvp + 1), GHq + (T.nvq + 1), GHf + (T.nvphi + 1), hh + T.eboff + (long long)Q * nb,
+                      d33v + T.eboff + (long long)Q * nb};
   const uint32_t* cnt = ena_hist + T.ehoff + (long long)Q * nb;
   const double k = T.nsnormfac / d3x[T.goff + Q];
   double Mq[8];
@@ -1842,7 +1884,7 @@ struct kaos_ctx {
   unsigned int* d_nq_ena = nullptr;
   unsigned long long* d_ena_esc = nullptr;
   // ENA velocity grid (per tube and axis [high edges, GL0 | centres | widths]
-  // at DevTube.ep/eq/efoff; per-bin hh and d33v at eboff), per-ENA packed
+  // at DevTube.ep/eq/efoff; hh and d33v [Q][bin] at eboff), per-ENA packed
   // cells ip | iq << 8 | iphi << 16, phase-space counts [Q][bin] at ehoff and
   // the eight moments per cell ([8][nq] per tube at 8 goff)
   double *d_ena_vp = nullptr, *d_ena_vq = nullptr, *d_ena_vf = nullptr;
@@ -2039,7 +2081,7 @@ extern "C" int kaos_set_grid(kaos_ctx* c, int32_t tube, const kaos_grid* g, int6
     h.vqGC.assign(g->vqGC, g->vqGC + b); h.dvq.assign(g->dvq, g->dvq + b);
     h.vphiGL.assign(g->vphiGL, g->vphiGL + f); h.vphiGH.assign(g->vphiGH, g->vphiGH + f);
     h.vphiGC.assign(g->vphiGC, g->vphiGC + f); h.dvphi.assign(g->dvphi, g->dvphi + f);
-    h.ena_hh.assign(g->ena_hh, g->ena_hh + nb); h.ena_d33v.assign(g->ena_d33v, g->ena_d33v + nb);
+    h.ena_hh.assign(g->ena_hh, g->ena_hh + nb * g->nq); h.ena_d33v.assign(g->ena_d33v, g->ena_d33v + nb * g->nq);
   } else {
     h.nsneut.clear();
   }
@@ -2902,11 +2944,18 @@ static int check_next(kaos_ctx* c, long long n) {
   return KAOS_OK;
 }
 
+// Error bits the kernels set: a NaN in the state, or a capacity the device
+// ran out of (ENA slots, 40-bit ids of the charge-exchange selection).
 static int read_status(kaos_ctx* c, int32_t* st) {
   unsigned int s = 0;
   HIPCHK(c, hipMemcpy(&s, c->d_status, sizeof(s), hipMemcpyDeviceToHost));
-  *st = (s & (1u << KAOS_E_NAN)) ? KAOS_E_NAN : KAOS_OK;
+  *st = (s & (1u << KAOS_E_NAN)) ? KAOS_E_NAN : (s & (1u << KAOS_E_CAPACITY)) ? KAOS_E_CAPACITY : KAOS_OK;
   return KAOS_OK;
+}
+
+static const char* status_msg(int32_t st) {
+  return st == KAOS_E_NAN ? "NaN in the particle state"
+                          : "device capacity exceeded (ENA slots of a tube, or ion ids beyond 2^40)";
 }
 
 extern "C" int kaos_step(kaos_ctx* c, int64_t n, const double* uniforms, int64_t n_uniforms,
@@ -2943,7 +2992,7 @@ extern "C" int kaos_step(kaos_ctx* c, int64_t n, const double* uniforms, int64_t
     out->n_alive = alive;
     rc = read_status(c, &out->status);
     if (rc) return rc;
-    if (out->status) return fail(c, out->status, "NaN in the particle state");
+    if (out->status) return fail(c, out->status, status_msg(out->status));
   }
   return KAOS_OK;
 }
@@ -2986,7 +3035,7 @@ extern "C" int kaos_step_noalloc(kaos_ctx* c, int64_t n, const double* uniforms,
     out->n_alive = alive;
     rc = read_status(c, &out->status);
     if (rc) return rc;
-    if (out->status) return fail(c, out->status, "NaN in the particle state");
+    if (out->status) return fail(c, out->status, status_msg(out->status));
   }
   return KAOS_OK;
 }
@@ -3012,7 +3061,7 @@ extern "C" int kaos_run(kaos_ctx* c, int64_t n_first, int64_t nsteps, int64_t* p
   int32_t st = 0;
   int rc = read_status(c, &st);
   if (rc) return rc;
-  if (st) return fail(c, st, "NaN in the particle state");
+  if (st) return fail(c, st, status_msg(st));
   return KAOS_OK;
 }
 

@@ -48,7 +48,7 @@ module kaos_hip
     real(c_double) :: t_neut = 0, m_neut = 0, cx_sigma_scale = 1
     type(c_ptr) :: ns_neut = c_null_ptr
     integer(c_int64_t) :: ena_cap = 0
-    ! ENA velocity grid (|Vp|, Vq, Vphi cells; per bin hVp hVq hVphi and d33vCT)
+    ! ENA velocity grid (|Vp|, Vq, Vphi cells; per config cell and bin hVp hVq hVphi and d33vCT)
     integer(c_int32_t) :: nvp = 0, nvq = 0, nvphi = 0, ena_pad = 0
     type(c_ptr) :: vpGL = c_null_ptr, vpGH = c_null_ptr, vpGC = c_null_ptr, dvp = c_null_ptr
     type(c_ptr) :: vqGL = c_null_ptr, vqGH = c_null_ptr, vqGC = c_null_ptr, dvq = c_null_ptr

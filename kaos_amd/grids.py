@@ -168,8 +168,10 @@ def with_charge_exchange(tube, n0=1.423e11, r0=None, H=2.0e5, t_neut=848.0, filt
         a[name + "GC"] = (a[name + "GL"] + a[name + "GH"]) / 2.0
         a["d" + name] = a[name + "GH"] - a[name + "GL"]
     tube.nvp, tube.nvq, tube.nvphi = ena_grid
-    a["ena_hh"] = np.ones(ena_grid[0] * ena_grid[1] * ena_grid[2])
-    a["ena_d33v"] = ((a["dvp"][:, None, None] * a["dvq"][None, :, None]) * a["dvphi"][None, None, :]).ravel()
+    # per config cell and bin [Q][ip][iq][iphi]
+    dV = ((a["dvp"][:, None, None] * a["dvq"][None, :, None]) * a["dvphi"][None, None, :]).ravel()
+    a["ena_hh"] = np.ones(tube.nq * dV.size)
+    a["ena_d33v"] = np.tile(dV, tube.nq)
     tube.ena_cap = int(tube.cell_counts.sum()) if ena_cap is None else int(ena_cap)
     return tube
 
@@ -273,7 +275,7 @@ def config4_tubes(ntubes=512, seed=4, scale=1.0):
     return tubes
 
 
-def config5_tubes(n_h=50_000_000, n_o=150_000_000, nv1=32, nv2=32, cx=True, sigma_scale=1e-9,
+def config5_tubes(n_h=50_000_000, n_o=150_000_000, nv1=32, nv2=32, cx=True, sigma_scale=2e-10,
                   ena_grid=(16, 64, 16)):
     """Full physics: one L=6 tube (1.05-3 RE, 256 cells) holding H+ (n_h ions,
     0.5 eV, V grid +-40 km/s) and O+ (n_o ions, 0.3 eV, +-10 km/s) as two
@@ -284,7 +286,7 @@ def config5_tubes(n_h=50_000_000, n_o=150_000_000, nv1=32, nv2=32, cx=True, sigm
     grid; nv1 = 0: the 2-D 64 x 64 grid); charge exchange with neutrals of
     the species' mass, 848 K and an exponential density of 200 km scale
     height, the cross-section scaled by sigma_scale, ENAs on a (|Vp|, Vq,
-    Vphi) grid of ena_grid cells with slots for 3% of the ions."""
+    Vphi) grid of ena_grid cells with slots for 8% of the ions."""
     tubes = []
     for ms, n, T, vmax in ((MH, n_h, 0.5, 4.0e4), (MO, n_o, 0.3, 1.0e4)):
         t = dipole_tube(nq=256, p=6.0, r_ub=3.0, nvpar=64, nvperp=64, n_total=n, T_eV=T, h=0.02,
@@ -295,6 +297,6 @@ def config5_tubes(n_h=50_000_000, n_o=150_000_000, nv1=32, nv2=32, cx=True, sigm
             with_vgrid3(t, nv1, nv2, vperp_max=vmax)
             if cx:
                 with_charge_exchange(t, H=2.0e5, sigma_scale=sigma_scale, ena_grid=ena_grid,
-                                     ena_cap=max(1024, int(0.03 * n)))
+                                     ena_cap=max(1024, int(0.08 * n)))
         tubes.append(t)
     return tubes

@@ -63,8 +63,8 @@ typedef struct {
   double t_neut, m_neut, cx_sigma_scale;
   const double *nsneut;
   int64_t ena_cap;
-  /* ENA velocity grid: Vp, Vq, Vphi cells; per bin [ip][iq][iphi] the metric
-   * product hVp hVq hVphi and d33vCT */
+  /* ENA velocity grid: Vp, Vq, Vphi cells; per config cell and bin
+   * [Q][ip][iq][iphi] the metric product hVp hVq hVphi and d33vCT */
   int nvp, nvq, nvphi, pad_ena;
   const double *vpGL, *vpGH, *vpGC, *dvp, *vqGL, *vqGH, *vqGC, *dvq, *vphiGL, *vphiGH, *vphiGC, *dvphi;
   const double *ena_hh, *ena_d33v;
@@ -631,6 +631,8 @@ static void ena_moments(const orc_cfg *c, const uint32_t *nq_ena, const uint32_t
                       c->ena_hh, c->ena_d33v};
   double P[4][KAOS_ENA_LANES];
   for (int Q = 0; Q < c->nq; ++Q) {
+    g.hh = c->ena_hh + (int64_t)Q * nb;             /* the cell's metric row */
+    g.d33v = c->ena_d33v + (int64_t)Q * nb;
     double *M = M8 + 8 * Q;
     for (int i = 0; i < 8; ++i) M[i] = 0.0;
     if (nq_ena[Q] == 0) continue;

@@ -297,16 +297,16 @@ def _np_ena_moments(t, counts, ms):
     a = t.arrays
     nvp, nvq, nvphi = t.nvp, t.nvq, t.nvphi
     dV = (a["dvp"][:, None, None] * a["dvq"][None, :, None]) * a["dvphi"][None, None, :]
-    hh = a["ena_hh"].reshape(nvp, nvq, nvphi)
-    d33 = a["ena_d33v"].reshape(nvp, nvq, nvphi)
+    hh = a["ena_hh"].reshape(t.nq, nvp, nvq, nvphi)
+    d33 = a["ena_d33v"].reshape(t.nq, nvp, nvq, nvphi)
     VP, VQ, VF = np.meshgrid(a["vpGC"], a["vqGC"], a["vphiGC"], indexing="ij")
     out = np.zeros((8, t.nq))
     for Q in range(t.nq):
         c = counts[Q]
         if c.sum() == 0:
             continue
-        F = (c * (t.nsnormfac / a["d3xC"][Q])) / d33
-        w = dV * hh * F
+        F = (c * (t.nsnormfac / a["d3xC"][Q])) / d33[Q]
+        w = dV * hh[Q] * F
         M0 = w.sum()
         m1 = [(w * V).sum() / M0 for V in (VP, VQ, VF)]
         d = [m1[0] - VP, m1[1] - VQ, m1[2] - VF]
@@ -317,13 +317,14 @@ def _np_ena_moments(t, counts, ms):
 
 def test_ena_moments_against_restatement(lib):
     t, cfg = _ena_tube()
-    # non-uniform metric factors, as the reference's grid generator makes them
+    # non-uniform metric factors that differ from cell to cell, as the
+    # reference's grid generator makes them (they depend on thetaGC)
     rng = np.random.default_rng(4)
     nb = t.nvp * t.nvq * t.nvphi
-    hh = rng.uniform(0.5, 2.0, nb)
-    t.arrays["ena_hh"] = hh
+    hh = rng.uniform(0.5, 2.0, (t.nq, nb))
+    t.arrays["ena_hh"] = hh.ravel()
     dV = ((t.arrays["dvp"][:, None, None] * t.arrays["dvq"][None, :, None]) * t.arrays["dvphi"][None, None, :]).ravel()
-    t.arrays["ena_d33v"] = hh * dV
+    t.arrays["ena_d33v"] = (hh * dV[None, :]).ravel()
     cfg = oracle.cfg_of(t, 1, 1)
     counts = np.zeros((t.nq, t.nvp, t.nvq, t.nvphi))
     counts[0] = rng.poisson(3.0, (t.nvp, t.nvq, t.nvphi))           # a populated cell
