@@ -181,13 +181,93 @@ int kaos_upload_particles(kaos_ctx* ctx, int32_t tube, int64_t n, const double* 
  * vperp0 and Vperp2 = 0 (only the magnitude enters unless ICR heating is on). */
 int kaos_upload_vperp(kaos_ctx* ctx, int32_t tube, int64_t n, const double* vperp1, const double* vperp2);
 
-/* Macro-ions per config cell of a tube: counts[Q] = nint((density[Q] /
- * nsnormfac) d3x[Q]), halves away from zero; density in m^-3, d3x the cell
- * volumes.  Replaces nsnormC0 (_0_1_1_ConfigGridGeneratorB.f90:83-85) and
- * NsFARRp (_2_1_DensityProfileA1.f90:45) for one rank.  Host arithmetic
+/* Macro-ions per config cell of a tube: counts[Q] = (density[Q] /
+ * nsnormfac) d3x[Q] truncated toward zero (the reference stores it in the
+ * integer array nsnormC0, _KineticMainParams.f90:64); density in m^-3, d3x
+ * the cell volumes.  Replaces nsnormC0 (_0_1_1_ConfigGridGeneratorB.f90:72-85)
+ * and NsFARRp (_2_1_DensityProfileA1.f90:45) for one rank.  Host arithmetic
  * only: no context, no device.  KAOS_E_ARG for nq < 1, nsnormfac <= 0 or a
  * count that is not finite or does not fit in int64. */
 int kaos_cell_counts(int32_t nq, const double* density, const double* d3x, double nsnormfac, int64_t* counts);
+
+/* Grid set-up of one tube from the model inputs, computed on the device
+ * (one thread per cell or bin).  Replaces ConfigGridGeneratorSub
+ * (_0_ConfigGridGenerator.f90:88-668: the q grid, the grid root solve, the
+ * metric factors and cell volumes), ConfigGridGeneratorSubB
+ * (_0_1_1_ConfigGridGeneratorB.f90:25-218: the hydrostatic ion density and
+ * its macro-ion counts nsnormC0, the neutral density), VelGridGeneratorSub
+ * (_0_0_VelGridGenerator.f90:25-857: the ion velocity grid, 2-D or 3-D, and
+ * the ENA grid with its metric) and the re-index onto the loop's cells
+ * (_1_SimParameterization.f90:1156-1179: loop cell Q is set-up cell Q + 1).
+ * NqLB is 1 and the grid-point strides (dq, ddVpar, ...) are 1: other values
+ * index past the reference's arrays. */
+typedef struct kaos_gridgen_in {
+  int64_t nq_ub;           /* NqUB (NqLB = 1): nq = NqUB config cells */
+  double qGA, qGB;         /* q of the first and the last grid point */
+  double L, phi;           /* Lshell, phiLshell (phi: carried to kaos_grid only) */
+  double ms;               /* species mass msT [kg] */
+  double ti, te;           /* Ti(1), Te(1) [K]: TsPerp = TsPar = Ti, Te0 = Te */
+  double ns0;              /* ion density at set-up cell Qindns0 = 1 [m^-3] */
+  double nsnormfac;
+  int32_t vperp3d;         /* ION2VPERPflag: 1 = the (Vperp1, Vperp2, V||) grid, 0 = (Vperp, V||) */
+  int32_t symvpar;         /* SYMVPARflag: 1 = V|| grid symmetric about 0 */
+  int32_t cx;              /* QEXCHANGEflag: the neutral density and the ENA grid */
+  int32_t device;          /* HIP device ordinal */
+  int64_t nvpar_gp;        /* NVparGpF (even, >= 4) */
+  int64_t nvperp_gp;       /* NVperpGpF (even, >= 4) */
+  double vpar_sigma_fac;   /* VparsigmaFac(1) */
+  double vperp_sigma_fac;  /* VperpsigmaFac(1) (also the ENA Vp/Vphi factor) */
+  double t_neut;           /* TNeut(1) [K] (cx) */
+  double zns0_neut;        /* zns0Neut(1) [m] (cx) */
+  double ns0_neut;         /* ns0Neut(1) [m^-3] (cx) */
+} kaos_gridgen_in;
+
+/* Cell counts of the grids kaos_generate_grid writes. */
+typedef struct kaos_gridgen_dims {
+  int32_t nq;              /* NqUB */
+  int32_t nvpar;           /* NVparGpF - 2 */
+  int32_t nvperp;          /* NVperpGpF - 2 (2-D grid), else 0 */
+  int32_t nvperp1, nvperp2;/* NVperpGpF - 2 each (3-D grid), else 0 */
+  int32_t nvp, nvq, nvphi; /* ENA grid (cx): NVperpGpF - 2, NVparGpF - 2, NVperpGpF - 2, else 0 */
+} kaos_gridgen_dims;
+
+/* Caller-allocated outputs (any pointer may be NULL).  Per config cell
+ * (nq): the kaos_grid arrays of the tube, the ion temperature Ts0 = Ti/3 +
+ * 2 Ti/3 and Te0, the hydrostatic density nsC [m^-3], its macro-ion counts
+ * nsnormC0 (truncated: an integer array in the reference) and the neutral
+ * macro-particle counts nsnormCNeutGT (truncated likewise).  The velocity
+ * grids as kaos_grid takes them (V|| cells with the centre 0 replaced by
+ * 1e-15, _0_0_VelGridGenerator.f90:1000, 1168); 2-D: d3v = 2 pi |Vperp_c|
+ * dVperp dV|| as [Vperp][V||]; 3-D: d3v3 = dV1 dV2 dV|| as [V1][V2][V||].
+ * ENA (cx): the Vp, Vq, Vphi cells and per config cell and bin
+ * [Q][ip][iq][iphi] hh = hVp hVq hVphi and d33vCT, from the theta of set-up
+ * cell Q (the ENA grid is built before the re-index, so loop cell Q takes
+ * the centre one cell below its own). */
+typedef struct kaos_gridgen_out {
+  double *qGL, *qGH, *qGC, *hqC, *dqC, *d3xC, *rGC, *cthGC, *ellGC, *Ts, *Te;
+  double *density;         /* nsC */
+  int64_t *counts;         /* nsnormC0 of the loop's cells */
+  double *ns_neut;         /* nsnormCNeutGT (cx) */
+  double nsnorm_clb;       /* nsnormCLBG = nsnormC0 of loop cell 1 (the LB injection per
+                              master step is nint(nsnorm_clb / ranks)) */
+  double nsnorm_cub;       /* nsnormCUBG = nsnormC0 of loop cell nq */
+  double *vparGL, *vparGH, *vparGC, *dvpar;
+  double *vperpGL, *vperpGH, *vperpGC, *dvperp, *d3v;            /* 2-D */
+  double *vperp1GL, *vperp1GH, *vperp1GC, *dvperp1;               /* 3-D */
+  double *vperp2GL, *vperp2GH, *vperp2GC, *dvperp2, *d3v3;
+  double *vpGL, *vpGH, *vpGC, *dvp;                               /* ENA */
+  double *vqGL, *vqGH, *vqGC, *dvq;
+  double *vphiGL, *vphiGH, *vphiGC, *dvphi;
+  double *ena_hh, *ena_d33v;
+} kaos_gridgen_out;
+
+/* The grid sizes of `in` (host arithmetic, no device).  KAOS_E_ARG for
+ * NqUB < 11, odd or < 4 grid-point counts, a cell count above 255, or
+ * non-positive masses, temperatures or normalisation. */
+int kaos_gridgen_dims_of(const kaos_gridgen_in* in, kaos_gridgen_dims* dims);
+
+/* Generate the grids of `in` on device in->device into `out`. */
+int kaos_generate_grid(const kaos_gridgen_in* in, kaos_gridgen_out* out);
 
 /* Initial ions of a tube generated on the device, in place of
  * kaos_upload_particles + kaos_upload_vperp: counts[Q] ions at the centre
@@ -277,9 +357,12 @@ int kaos_export_spinup(kaos_ctx* ctx, int32_t tube, const kaos_spinup_info* info
  * _1_SimParameterization.f90:801-946). */
 typedef struct kaos_spinup_state {
   int64_t nq_lb, nq_ub;    /* NqLBp, NqUBp (NqLBTfort.bin, NqUBTfort.bin) */
-  double nsnorm_clb;       /* nsnormCLBInput: the LB injection normalisation nsnormCLB
-                              (_2_2_DensityProfileA2.f90:327-337); one rank injects
-                              nint(nsnorm_clb) ions per master step (BC:747-750) */
+  double nsnorm_clb;       /* nsnormCLBInput: the count of the lower ghost cell
+                              (nsnormC0(NqLB), _0_1_1_ConfigGridGeneratorB.f90:76-78).
+                              The LB injection of the restarted run is not this value
+                              but nsnormCLBGT = nsnormC0(NqLB + 1) (:189), the first
+                              computational cell's count kaos_cell_counts gives:
+                              nint(counts[0] / ranks) ions per master step (BC:747-750) */
   double nsnorm_cub;       /* nsnormCUBInput */
   int64_t lb_replenish;    /* LBREPLENISHflagInput */
   int64_t ub_replenish;    /* UBREPLENISHflagInput */

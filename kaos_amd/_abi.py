@@ -86,7 +86,7 @@ class KaosStepOut(C.Structure):
 
 
 EXPORTS = ("kaos_init", "kaos_set_grid", "kaos_upload_particles", "kaos_upload_vperp", "kaos_cell_counts",
-           "kaos_seed_particles", "kaos_step", "kaos_step_noalloc", "kaos_run",
+           "kaos_gridgen_dims_of", "kaos_generate_grid", "kaos_seed_particles", "kaos_step", "kaos_step_noalloc", "kaos_run",
            "kaos_download_master", "kaos_download_master3d", "kaos_export_master", "kaos_export_spinup",
            "kaos_read_spinup", "kaos_set_field",
            "kaos_count", "kaos_download_cx", "kaos_download_enas", "kaos_download_ena_master",
@@ -96,6 +96,40 @@ EXPORTS = ("kaos_init", "kaos_set_grid", "kaos_upload_particles", "kaos_upload_v
            "kaos_get_launch", "kaos_profile", "kaos_profile_read", "kaos_last_error", "kaos_finalize")
 
 _lib = None
+
+
+# grid set-up (kaos_generate_grid): inputs, sizes and the per-cell output arrays
+GRIDGEN_CFG = ("qGL", "qGH", "qGC", "hqC", "dqC", "d3xC", "rGC", "cthGC", "ellGC", "Ts", "Te", "density")
+GRIDGEN_AXES = (("vpar", ("vparGL", "vparGH", "vparGC", "dvpar")),
+                ("vperp", ("vperpGL", "vperpGH", "vperpGC", "dvperp")),
+                ("vperp1", ("vperp1GL", "vperp1GH", "vperp1GC", "dvperp1")),
+                ("vperp2", ("vperp2GL", "vperp2GH", "vperp2GC", "dvperp2")),
+                ("vp", ("vpGL", "vpGH", "vpGC", "dvp")),
+                ("vq", ("vqGL", "vqGH", "vqGC", "dvq")),
+                ("vphi", ("vphiGL", "vphiGH", "vphiGC", "dvphi")))
+
+
+class KaosGridgenIn(C.Structure):
+    _fields_ = ([("nq_ub", C.c_int64)]
+                + [(n, C.c_double) for n in ("qGA", "qGB", "L", "phi", "ms", "ti", "te", "ns0", "nsnormfac")]
+                + [(n, C.c_int32) for n in ("vperp3d", "symvpar", "cx", "device")]
+                + [("nvpar_gp", C.c_int64), ("nvperp_gp", C.c_int64)]
+                + [(n, C.c_double) for n in ("vpar_sigma_fac", "vperp_sigma_fac", "t_neut", "zns0_neut",
+                                             "ns0_neut")])
+
+
+class KaosGridgenDims(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("nq", "nvpar", "nvperp", "nvperp1", "nvperp2", "nvp", "nvq", "nvphi")]
+
+
+class KaosGridgenOut(C.Structure):
+    _fields_ = ([(n, _dp) for n in GRIDGEN_CFG]
+                + [("counts", C.POINTER(C.c_int64)), ("ns_neut", _dp),
+                   ("nsnorm_clb", C.c_double), ("nsnorm_cub", C.c_double)]
+                + [(n, _dp) for _, names in GRIDGEN_AXES[:2] for n in names] + [("d3v", _dp)]
+                + [(n, _dp) for _, names in GRIDGEN_AXES[2:4] for n in names] + [("d3v3", _dp)]
+                + [(n, _dp) for _, names in GRIDGEN_AXES[4:] for n in names]
+                + [("ena_hh", _dp), ("ena_d33v", _dp)])
 
 
 def load(path=LIB_PATH):
@@ -115,6 +149,8 @@ def load(path=LIB_PATH):
     lib.kaos_upload_vperp.argtypes = [P, C.c_int32, C.c_int64, _dp, _dp]
     lib.kaos_cell_counts.argtypes = [C.c_int32, _dp, _dp, C.c_double, i64p]
     lib.kaos_seed_particles.argtypes = [P, C.c_int32, i64p, _dp, C.c_uint64]
+    lib.kaos_gridgen_dims_of.argtypes = [C.POINTER(KaosGridgenIn), C.POINTER(KaosGridgenDims)]
+    lib.kaos_generate_grid.argtypes = [C.POINTER(KaosGridgenIn), C.POINTER(KaosGridgenOut)]
     lib.kaos_step.argtypes = [P, C.c_int64, _dp, C.c_int64, C.POINTER(KaosStepOut)]
     lib.kaos_step_noalloc.argtypes = [P, C.c_int64, _dp, C.c_int64, C.POINTER(KaosStepOut)]
     lib.kaos_run.argtypes = [P, C.c_int64, C.c_int64, i64p]

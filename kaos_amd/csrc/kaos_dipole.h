@@ -563,18 +563,16 @@ KAOS_HD void kaos_mb_project(const double u[6], double sigma, double cphi, doubl
 /* Initial ions (DensityProfileA1Sub, DensityProfileBSub, DipolePolynomial-  */
 /* SolverSub, VelocityDistributionSub, InitialConditionsSub).                */
 
-/* Macro-ions of one config cell: nint((n / nsnormfac) d3x), the reference's
- * nsnormC0 (_0_1_1_ConfigGridGeneratorB.f90:83-85) rounded as NsFARRp
- * (_2_1_DensityProfileA1.f90:45): to the nearest integer, halves away from
- * zero.  -1 when the value is not finite or does not fit in int64.         */
+/* Macro-ions of one config cell: (n / nsnormfac) d3x truncated toward zero.
+ * The reference assigns the value to nsnormC0, an integer(kind=dp) array
+ * (_KineticMainParams.f90:64; _0_1_1_ConfigGridGeneratorB.f90:83-85, and
+ * :72-75 for a restart's DensityInputT), which truncates; NsFARRp =
+ * nint(nsnormCGT) (_2_1_DensityProfileA1.f90:45) then keeps that integer.
+ * -1 when the value is not finite or does not fit in int64.                */
 KAOS_HD int64_t kaos_cell_count(double density, double d3x, double nsnormfac) {
   double x = (density / nsnormfac) * d3x;
   if (!(x > -9.2e18 && x < 9.2e18)) return -1;
-  double t = trunc(x);
-  double f = x - t;                       /* exact: |f| < 1, same sign as x */
-  if (f >= 0.5) t = t + 1.0;
-  else if (f <= -0.5) t = t - 1.0;
-  return (int64_t)t;
+  return (int64_t)trunc(x);
 }
 
 /* Thermal speed sqrt(kB T / m) of MBVelocityDistributionSub
@@ -1307,6 +1305,155 @@ KAOS_HD void kaos_ena_mom_finish1(const double S[4], double M[8]) {
 KAOS_HD void kaos_ena_mom_finish2(const double S[4], double ms, double M[8]) {
   const double M0 = M[0];
   for (int i = 0; i < 4; ++i) M[4 + i] = (M0 == 0.0) ? 0.0 : (6.242e18 * (ms / 2.0)) * (S[i] / M0);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Grid set-up of one tube from the model inputs: ConfigGridGeneratorSub     */
+/* (_0_ConfigGridGenerator.f90), ConfigGridGeneratorSubB                     */
+/* (_0_1_1_ConfigGridGeneratorB.f90), VelGridGeneratorSub                    */
+/* (_0_0_VelGridGenerator.f90), with NqLB = 1 and every grid-point stride    */
+/* (dq, ddVpar, ...) 1, the only values for which the reference's indexing   */
+/* stays inside its arrays.                                                  */
+
+/* sin(x) for |x| < 2^20 pi/2: Cody-Waite reduction by pi/2 with a three-part
+ * constant (the split of fdlibm's __ieee754_rem_pio2: 33 + 33 + 53 bits, so
+ * k P1 and k P2 are exact), then the Taylor polynomials on [-pi/4, pi/4].   */
+KAOS_HD double kaos_sin(double x) {
+  const double k = rint(x * 6.36619772367581382433e-01);           /* 2/pi */
+  const double r = ((x - k * 1.57079632673412561417e+00) - k * 6.07710050630396597660e-11) -
+                   k * 2.02226624879595063154e-21;
+  const int q = (int)((long long)k & 3);
+  if (q == 0) return kaos_sin_poly(r);
+  if (q == 1) return kaos_cos_poly(r);
+  if (q == 2) return -kaos_sin_poly(r);
+  return -kaos_cos_poly(r);
+}
+
+/* exp over the whole binary64 range (0 below -745.2, +inf above 709.78). */
+KAOS_HD double kaos_exp_full(double x) {
+  if (x < -745.2) return 0.0;
+  if (x > 709.78) return HUGE_VAL;
+  return kaos_exp(x);
+}
+
+/* q of grid point i (1..ngp, ngp = NqUB - NqLB + 4): uniform steps of
+ * |qGB - qGA| / ngp from qGA, toward qGB, and qGB itself at the last point
+ * (_0_ConfigGridGenerator.f90:299-310).                                     */
+KAOS_HD double kaos_gg_qpoint(double qGA, double qGB, long long ngp, long long i) {
+  if (i == ngp) return qGB;
+  const double step = fabs(qGB - qGA) / (double)ngp;
+  return (qGA <= 0.0) ? qGA + ((double)i - 1.0) * step : qGA - ((double)i - 1.0) * step;
+}
+
+/* One cell of the set-up arrays ("0" arrays, Qind = 1..NqUB + 2, cell 1 the
+ * lower ghost; the loop's cell Q is set-up cell Q + 1,
+ * _1_SimParameterization.f90:1156-1175).                                    */
+typedef struct {
+  double qGL, qGH, qGC, r, cth, sth, ell, hq, hp, hphi, dq, d3x;
+} kaos_gg_cell_t;
+
+/* Edges at points Qind and Qind + 1, centre, the grid root solve at the
+ * centre (GridDipolePolynomialSolverSub: r of the quartic, sin(theta) =
+ * sqrt(r/(RE p)), theta -> pi - theta for q <= 0), ellGC0, the metric
+ * factors hqC0, hpC0, hphiC0, dqC0 and d3xC0 = hq hp hphi dq dp dphi with
+ * dp = 1e-3, dphi = pi/180 (1e-9 if 0) (_0_ConfigGridGenerator.f90:433-547). */
+KAOS_HD void kaos_gg_cell(double qGA, double qGB, long long ngp, double L, long long Qind, kaos_gg_cell_t* c) {
+  c->qGL = kaos_gg_qpoint(qGA, qGB, ngp, Qind);
+  c->qGH = kaos_gg_qpoint(qGA, qGB, ngp, Qind + 1);
+  c->qGC = (c->qGL + c->qGH) / 2.0;
+  c->r = kaos_root_r(c->qGC, L);
+  const double w = c->r / (KAOS_RE * L);
+  c->sth = sqrt(w);
+  c->cth = (c->qGC <= 0.0) ? -sqrt(1.0 - w) : sqrt(1.0 - w);
+  c->ell = 1.0 + 3.0 * (c->cth * c->cth);
+  const double se = sqrt(c->ell);
+  c->hq = fabs(((c->r * c->r) * c->r) / (KAOS_RE2 * se));
+  c->hp = fabs((KAOS_RE * ((c->sth * c->sth) * c->sth)) / se);
+  c->hphi = fabs(c->r * c->sth);
+  c->dq = fabs(c->qGH - c->qGL);
+  c->d3x = ((((c->hq * c->hp) * c->hphi) * c->dq) * 1e-3) * (KAOS_PI / 180.0);
+  if (c->d3x == 0.0) c->d3x = 1e-9;
+}
+
+/* The cell's term ICbbp of the field-aligned gravitational potential
+ * (_0_1_1_ConfigGridGeneratorB.f90:46-48). */
+KAOS_HD double kaos_gg_icbbp(const kaos_gg_cell_t* c) {
+  return ((2.0 * KAOS_GG) * KAOS_ME) * (((c->hq * c->dq) * c->cth) / ((c->r * c->r) * sqrt(c->ell)));
+}
+
+/* Hydrostatic ion density nsC = ns0 exp(-+ms gC / (kB (TsPar + Te))) of a
+ * cell whose potential sum is gC = ICbb - IC0bb; sign + when qGA <= 0
+ * (_0_1_1_ConfigGridGeneratorB.f90:50-73). */
+KAOS_HD double kaos_gg_ns(double gC, double ms, double tpar, double te, double ns0, double qGA) {
+  const double a = (ms * gC) / (KAOS_KB * (tpar + te));
+  return ns0 * kaos_exp_full((qGA <= 0.0) ? a : -a);
+}
+
+/* Neutral density nsCNeut = ns0Neut exp(-+mNeut gCNeut (1/zns0Neut - 1/r)),
+ * gCNeut = G ME/(kB TNeut), sign + when qGA <= 0
+ * (_0_1_1_ConfigGridGeneratorB.f90:97-115). */
+KAOS_HD double kaos_gg_ns_neut(double r, double mneut, double tneut, double z0, double n0, double qGA) {
+  const double g = (KAOS_GG * KAOS_ME) / (KAOS_KB * tneut);
+  const double a = (mneut * g) * ((1.0 / z0) - (1.0 / r));
+  return n0 * kaos_exp_full((qGA <= 0.0) ? a : -a);
+}
+
+/* Point i (1 .. 2N - 1) of a symmetric velocity grid of N = NVGpF/2 points
+ * per side (SYMVPARflag = 1 V||, Vperp, Vperp1/2; _0_0_VelGridGenerator.f90:
+ * 70-95, 126-151, 173-198): -GB, then steps of GB/N up to the N-th point 0,
+ * mirrored: P(N + k) = -P(N - k).                                           */
+KAOS_HD double kaos_gg_vsym(double N, double GB, long long i) {
+  const long long n = (long long)N;
+  const long long j = (i > n) ? 2 * n - i : i;
+  double v;
+  if (j == n) v = 0.0;
+  else if (j == 1) v = -GB;
+  else v = -GB + ((double)j - 1.0) * (fabs(GB - 0.0) / N);
+  return (i > n) ? -v : v;
+}
+
+/* Point i (1 .. N - 1) of the one-sided V|| grid (SYMVPARflag = 0, N =
+ * NVparGpF points from GA = -4 sigma to GB; _0_0_VelGridGenerator.f90:
+ * 203-245), reversed and negated in the southern hemisphere (qGA <= 0).     */
+KAOS_HD double kaos_gg_vlin_p1(double N, double GA, double GB, long long i) {
+  return (i == (long long)N) ? GB : GA + ((double)i - 1.0) * (fabs(GB - GA) / N);
+}
+KAOS_HD double kaos_gg_vlin(double N, double GA, double GB, int south, long long i) {
+  return south ? -kaos_gg_vlin_p1(N, GA, GB, (long long)N - i + 1) : kaos_gg_vlin_p1(N, GA, GB, i);
+}
+
+/* Point i (1 .. 2N - 1) of the ENA Vp / Vphi grid (and Vq with SYMVPARflag
+ * = 1; _0_0_VelGridGenerator.f90:500-615, 636-668): P1 = 0, GB/N, 2 GB/N,
+ * ... with P1(N) = GB; the first N points are -P1 reversed (-GB .. -0), the
+ * rest P1(i - N) (the reference reads P1(i) there, past the end of its N
+ * elements).                                                                */
+KAOS_HD double kaos_gg_vena(double N, double GB, long long i) {
+  const long long n = (long long)N;
+  if (i <= n) return -kaos_gg_vlin_p1(N, 0.0, GB, n - i + 1);
+  return kaos_gg_vlin_p1(N, 0.0, GB, i - n);
+}
+
+/* ENA metric of one bin of a config cell whose centre has (cos, sin) of
+ * theta and ell (_0_0_VelGridGenerator.f90:790-857): Vr, Vtheta of the bin
+ * centre, hVp = |RE sin^3(Vtheta)| / sqrt(Vell), hVq = |Vr^3| / (RE^2
+ * sqrt(Vell)), hVphi = |Vr sin(Vtheta)|; hh = hVp hVq hVphi, d33v = hh dVp
+ * dVq dVphi (1e-9 if 0).  Vtheta is a speed that the reference passes to
+ * sin as an angle; kept as written.                                         */
+KAOS_HD void kaos_gg_ena_metric(double cth, double sth, double ell, double vpC, double vqC, double dvp, double dvq,
+                                double dvphi, double* hh, double* d33v) {
+  const double se = sqrt(ell);
+  const double vr = ((2.0 * vqC) * cth) / se + (vpC * sth) / se;
+  const double vt = (vqC * sth) / se - ((2.0 * vpC) * cth) / se;
+  const double vell = 1.0 + 3.0 * (cth * cth);
+  const double sv = kaos_sin(vt);
+  const double sve = sqrt(vell);
+  const double hvp = fabs((KAOS_RE * ((sv * sv) * sv)) / sve);
+  const double hvq = fabs(((vr * vr) * vr) / (KAOS_RE2 * sve));
+  const double hvphi = fabs(vr * sv);
+  *hh = (hvp * hvq) * hvphi;
+  double d = ((((hvp * hvq) * hvphi) * dvp) * dvq) * dvphi;
+  if (d == 0.0) d = 1e-9;
+  *d33v = d;
 }
 
 #endif /* KAOS_DIPOLE_H */

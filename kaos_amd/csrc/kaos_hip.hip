@@ -2558,6 +2558,234 @@ extern "C" int kaos_cell_counts(int32_t nq, const double* density, const double*
   return KAOS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Grid set-up of one tube on the device (kaos_generate_grid).  The shared
+// header holds the arithmetic (kaos_gg_*); the kernels below give each cell,
+// velocity cell and ENA bin a thread.
+
+struct GGPar {
+  kaos_gridgen_in in;
+  kaos_gridgen_dims d;
+  long long ngp;                 // q grid points NqUB + 3
+  double npar, nperp;            // points per side of the symmetric grids (NVGpF / 2)
+  double gb_par, gb_perp;        // VparGB, VperpGB (ions)
+  double sig_par;                // Vparsigma (the SYMVPARflag = 0 V|| grid starts at -4 sigma)
+  double gb_pphi, gb_q, sig_n;   // ENA: VpGB = VphiGB, VqGB, Vqsigma
+};
+
+__global__ void k_gg_cells(GGPar P, kaos_gg_cell_t* cells, double* icb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.d.nq + 2) return;
+  kaos_gg_cell_t c;
+  kaos_gg_cell(P.in.qGA, P.in.qGB, P.ngp, P.in.L, i + 1, &c);
+  cells[i] = c;
+  icb[i] = kaos_gg_icbbp(&c);
+}
+
+// Loop cell Q = set-up cell Q + 1: geometry, temperatures, the hydrostatic
+// density (potential sum over set-up cells 1 .. Q + 1 in index order, minus
+// the first) and its truncated counts, the neutral counts.
+__global__ void k_gg_config(GGPar P, const kaos_gg_cell_t* cells, const double* icb, double* o, long long* counts) {
+  const int Q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nq = P.d.nq;
+  if (Q >= nq) return;
+  const kaos_gg_cell_t c = cells[Q + 1];
+  double sum = 0.0;
+  for (int k = 0; k <= Q + 1; ++k) sum = sum + icb[k];
+  const double gC = sum - icb[0];
+  const double ns = kaos_gg_ns(gC, P.in.ms, P.in.ti, P.in.te, P.in.ns0, P.in.qGA);
+  const double x = (ns / P.in.nsnormfac) * c.d3x;
+  double xn = 0.0;
+  if (P.in.cx) xn = (kaos_gg_ns_neut(c.r, P.in.ms, P.in.t_neut, P.in.zns0_neut, P.in.ns0_neut, P.in.qGA) /
+                     P.in.nsnormfac) * c.d3x;
+  const double v[13] = {c.qGL, c.qGH, c.qGC, c.hq, c.dq, c.d3x, c.r, c.cth, c.ell,
+                        (1.0 / 3.0) * P.in.ti + (2.0 / 3.0) * P.in.ti, P.in.te, ns, trunc(xn)};
+  for (int k = 0; k < 13; ++k) o[(long long)k * nq + Q] = v[k];
+  counts[Q] = (x > -9.2e18 && x < 9.2e18) ? (long long)trunc(x) : -1LL;
+}
+
+// Point i (1-based) of velocity axis a: 0 V||, 1 Vperp (2-D), 2 Vperp1,
+// 3 Vperp2, 4 ENA Vp, 5 ENA Vq, 6 ENA Vphi.
+__device__ double kd_gg_point(const GGPar& P, int a, long long i) {
+  const int south = P.in.qGA <= 0.0;
+  switch (a) {
+    case 0:
+      return P.in.symvpar ? kaos_gg_vsym(P.npar, P.gb_par, i)
+                          : kaos_gg_vlin((double)P.in.nvpar_gp, -4.0 * P.sig_par, P.gb_par, south, i);
+    case 1: case 2: case 3:
+      return kaos_gg_vsym(P.nperp, P.gb_perp, i);
+    case 5:
+      return P.in.symvpar ? kaos_gg_vena(P.npar, P.gb_q, i)
+                          : kaos_gg_vlin((double)P.in.nvpar_gp, -4.0 * P.sig_n, P.gb_q, south, i);
+    default:
+      return kaos_gg_vena(P.nperp, P.gb_pphi, i);
+  }
+}
+
+// Cells of the seven axes (ncells[a], 0 = axis not generated): edges,
+// centre, width into o + 4 * off[a] ([GL | GH | GC | d], n each); V||
+// centres of 0 become 1e-15.
+__global__ void k_gg_vel(GGPar P, int4 n03, int4 n46, int4 off03, int4 off46, double* o) {
+  const int ncell[7] = {n03.x, n03.y, n03.z, n03.w, n46.x, n46.y, n46.z};
+  const int off[7] = {off03.x, off03.y, off03.z, off03.w, off46.x, off46.y, off46.z};
+  const int a = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = ncell[a];
+  if (i >= n) return;
+  const double gl = kd_gg_point(P, a, i + 1), gh = kd_gg_point(P, a, i + 2);
+  double gc = (gl + gh) / 2.0;
+  if (a == 0 && gc == 0.0) gc = 1e-15;
+  double* b = o + off[a];
+  b[i] = gl; b[n + i] = gh; b[2 * n + i] = gc; b[3 * n + i] = fabs(gh - gl);
+}
+
+// Ion d3v: 2-D [Vperp][V||] = 2 pi |Vperp_c| dVperp dV|| (1e-9 if 0); 3-D
+// [V1][V2][V||] = dV1 dV2 dV|| (d3vCTp).
+__global__ void k_gg_d3v(int three, int n1, int n2, int np, const double* c1, const double* c2, const double* cp,
+                         double* d3v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = three ? (long long)n1 * n2 * np : (long long)n1 * np;
+  if (i >= n) return;
+  const int ip = (int)(i % np);
+  const double dpar = cp[3 * np + ip];
+  if (three) {
+    const int i2 = (int)((i / np) % n2), i1 = (int)(i / ((long long)np * n2));
+    d3v[i] = (((1.0 * 1.0) * 1.0 * c1[3 * n1 + i1]) * c2[3 * n2 + i2]) * dpar;
+  } else {
+    const int i1 = (int)(i / np);
+    const double gc = c1[2 * n1 + i1];
+    double d = ((((1.0 * 1.0) * fabs(gc)) * c1[3 * n1 + i1]) * dpar) * (2.0 * KAOS_PI);
+    if (d == 0.0) d = 1e-9;
+    d3v[i] = d;
+  }
+}
+
+// ENA metric of loop cell Q and bin (ip, iq, iphi), from set-up cell Q's
+// theta (cells[Q], 0-based).
+__global__ void k_gg_ena(int nq, int nvp, int nvq, int nvphi, const kaos_gg_cell_t* cells, const double* vp,
+                         const double* vq, const double* vphi, double* hh, double* d33) {
+  const long long nb = (long long)nvp * nvq * nvphi;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb * nq) return;
+  const int Q = (int)(i / nb);
+  const long long b = i % nb;
+  const int iphi = (int)(b % nvphi), iq = (int)((b / nvphi) % nvq), ip = (int)(b / ((long long)nvphi * nvq));
+  const kaos_gg_cell_t c = cells[Q];
+  kaos_gg_ena_metric(c.cth, c.sth, c.ell, vp[2 * nvp + ip], vq[2 * nvq + iq], vp[3 * nvp + ip], vq[3 * nvq + iq],
+                     vphi[3 * nvphi + iphi], &hh[i], &d33[i]);
+}
+
+extern "C" int kaos_gridgen_dims_of(const kaos_gridgen_in* in, kaos_gridgen_dims* d) {
+  if (!in || !d) return KAOS_E_ARG;
+  memset(d, 0, sizeof(*d));
+  if (in->nq_ub < 11 || in->nq_ub > KAOS_MAXQ) return KAOS_E_ARG;
+  if (in->nvpar_gp < 4 || in->nvperp_gp < 4 || (in->nvpar_gp & 1) || (in->nvperp_gp & 1)) return KAOS_E_ARG;
+  if (in->nvpar_gp - 2 > 255 || in->nvperp_gp - 2 > 255) return KAOS_E_ARG;
+  if (!(in->ms > 0.0) || !(in->ti > 0.0) || !(in->te >= 0.0) || !(in->nsnormfac > 0.0) || !(in->L > 1.0) ||
+      !(in->ns0 >= 0.0) || !(in->vpar_sigma_fac > 0.0) || !(in->vperp_sigma_fac > 0.0) ||
+      !std::isfinite(in->qGA) || !std::isfinite(in->qGB) || in->qGA == in->qGB)
+    return KAOS_E_ARG;
+  if (in->cx && (!(in->t_neut > 0.0) || !(in->zns0_neut > 0.0) || !(in->ns0_neut >= 0.0))) return KAOS_E_ARG;
+  d->nq = (int32_t)in->nq_ub;
+  d->nvpar = (int32_t)(in->nvpar_gp - 2);
+  if (in->vperp3d) d->nvperp1 = d->nvperp2 = (int32_t)(in->nvperp_gp - 2);
+  else d->nvperp = (int32_t)(in->nvperp_gp - 2);
+  if (in->cx) {
+    d->nvp = d->nvphi = (int32_t)(in->nvperp_gp - 2);
+    d->nvq = (int32_t)(in->nvpar_gp - 2);
+  }
+  return KAOS_OK;
+}
+
+extern "C" int kaos_generate_grid(const kaos_gridgen_in* in, kaos_gridgen_out* out) {
+  if (!in || !out) return KAOS_E_ARG;
+  GGPar P;
+  int rc = kaos_gridgen_dims_of(in, &P.d);
+  if (rc) return rc;
+  P.in = *in;
+  P.ngp = in->nq_ub + 3;
+  P.npar = (double)in->nvpar_gp / 2.0;
+  P.nperp = (double)in->nvperp_gp / 2.0;
+  P.sig_par = kaos_thermal_speed(in->ti, in->ms);
+  P.gb_par = in->vpar_sigma_fac * P.sig_par;
+  P.gb_perp = in->vperp_sigma_fac * kaos_thermal_speed(in->ti, in->ms);
+  P.sig_n = in->cx ? kaos_thermal_speed(in->t_neut, in->ms) : 0.0;        // mNeut = msT
+  P.gb_pphi = in->vperp_sigma_fac * P.sig_n;
+  P.gb_q = in->vpar_sigma_fac * P.sig_n;
+  const kaos_gridgen_dims& d = P.d;
+  const int nq = d.nq;
+  const int nax[7] = {d.nvpar, d.nvperp, d.nvperp1, d.nvperp2, d.nvp, d.nvq, d.nvphi};
+  int off[7], tot = 0;
+  for (int a = 0; a < 7; ++a) { off[a] = tot; tot += 4 * nax[a]; }
+  const long long n3 = in->vperp3d ? (long long)d.nvperp1 * d.nvperp2 * d.nvpar : (long long)d.nvperp * d.nvpar;
+  const long long nena = (long long)nq * d.nvp * d.nvq * d.nvphi;
+  // one device block: cells, potential terms, config outputs, velocity cells, d3v, ENA metric, counts
+  const size_t ncell0 = (size_t)nq + 2;
+  const size_t bytes = ncell0 * sizeof(kaos_gg_cell_t) + ncell0 * 8 + (size_t)13 * nq * 8 + (size_t)tot * 8 +
+                       (size_t)n3 * 8 + (size_t)2 * nena * 8 + (size_t)nq * 8;
+  if (hipSetDevice(in->device) != hipSuccess) return KAOS_E_DEVICE;
+  char* buf = nullptr;
+  if (hipMalloc((void**)&buf, bytes) != hipSuccess) return KAOS_E_DEVICE;
+  kaos_gg_cell_t* cells = (kaos_gg_cell_t*)buf;
+  double* icb = (double*)(buf + ncell0 * sizeof(kaos_gg_cell_t));
+  double* cfg = icb + ncell0;
+  double* vel = cfg + (size_t)13 * nq;
+  double* d3v = vel + tot;
+  double* hh = d3v + n3;
+  double* d33 = hh + nena;
+  long long* cnt = (long long*)(d33 + nena);
+  hipLaunchKernelGGL(k_gg_cells, dim3((unsigned)((ncell0 + 255) / 256)), dim3(256), 0, 0, P, cells, icb);
+  hipLaunchKernelGGL(k_gg_config, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, 0, P, cells, icb, cfg, cnt);
+  hipLaunchKernelGGL(k_gg_vel, dim3(1, 7), dim3(256), 0, 0, P, make_int4(nax[0], nax[1], nax[2], nax[3]),
+                     make_int4(nax[4], nax[5], nax[6], 0), make_int4(off[0], off[1], off[2], off[3]),
+                     make_int4(off[4], off[5], off[6], 0), vel);
+  if (n3 > 0) {
+    const int a1 = in->vperp3d ? 2 : 1;
+    hipLaunchKernelGGL(k_gg_d3v, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, 0, in->vperp3d ? 1 : 0,
+                       nax[a1], in->vperp3d ? nax[3] : 1, nax[0], vel + off[a1], vel + off[3], vel + off[0], d3v);
+  }
+  if (nena > 0)
+    hipLaunchKernelGGL(k_gg_ena, dim3((unsigned)((nena + 255) / 256)), dim3(256), 0, 0, nq, d.nvp, d.nvq, d.nvphi,
+                       cells, vel + off[4], vel + off[5], vel + off[6], hh, d33);
+  rc = KAOS_OK;
+  std::vector<double> hcfg((size_t)13 * nq), hvel(tot > 0 ? tot : 1);
+  std::vector<long long> hcnt(nq);
+  if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+      hipMemcpy(hcfg.data(), cfg, sizeof(double) * 13 * nq, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(hvel.data(), vel, sizeof(double) * tot, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(hcnt.data(), cnt, sizeof(long long) * nq, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = KAOS_E_DEVICE;
+  auto down = [&](double* dst, const double* src, long long n) {
+    if (rc == KAOS_OK && dst && n > 0 && hipMemcpy(dst, src, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = KAOS_E_DEVICE;
+  };
+  if (in->vperp3d) down(out->d3v3, d3v, n3);
+  else down(out->d3v, d3v, n3);
+  down(out->ena_hh, hh, nena);
+  down(out->ena_d33v, d33, nena);
+  (void)hipFree(buf);
+  if (rc) return rc;
+  double* cdst[13] = {out->qGL, out->qGH, out->qGC, out->hqC, out->dqC, out->d3xC, out->rGC, out->cthGC, out->ellGC,
+                      out->Ts, out->Te, out->density, in->cx ? out->ns_neut : nullptr};
+  for (int k = 0; k < 13; ++k)
+    if (cdst[k]) memcpy(cdst[k], hcfg.data() + (size_t)k * nq, sizeof(double) * nq);
+  for (int Q = 0; Q < nq; ++Q)
+    if (hcnt[Q] < 0) return KAOS_E_ARG;                   // a count beyond int64 (or not finite)
+  if (out->counts) memcpy(out->counts, hcnt.data(), sizeof(long long) * nq);
+  out->nsnorm_clb = (double)hcnt[0];
+  out->nsnorm_cub = (double)hcnt[nq - 1];
+  double* ax[7][4] = {{out->vparGL, out->vparGH, out->vparGC, out->dvpar},
+                      {out->vperpGL, out->vperpGH, out->vperpGC, out->dvperp},
+                      {out->vperp1GL, out->vperp1GH, out->vperp1GC, out->dvperp1},
+                      {out->vperp2GL, out->vperp2GH, out->vperp2GC, out->dvperp2},
+                      {out->vpGL, out->vpGH, out->vpGC, out->dvp},
+                      {out->vqGL, out->vqGH, out->vqGC, out->dvq},
+                      {out->vphiGL, out->vphiGH, out->vphiGC, out->dvphi}};
+  for (int a = 0; a < 7; ++a)
+    for (int k = 0; k < 4; ++k)
+      if (nax[a] > 0 && ax[a][k]) memcpy(ax[a][k], hvel.data() + off[a] + (size_t)k * nax[a], sizeof(double) * nax[a]);
+  return KAOS_OK;
+}
+
 extern "C" int kaos_seed_particles(kaos_ctx* c, int32_t tube, const int64_t* counts, const double* temperature,
                                    uint64_t seed) {
   if (!c) return KAOS_E_ARG;

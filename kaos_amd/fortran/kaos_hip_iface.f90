@@ -80,6 +80,38 @@ module kaos_hip
     integer(c_int64_t) :: lb_replenish, ub_replenish, nq
   end type kaos_spinup_state
 
+  ! grid set-up inputs (ConfigGridGeneratorSub, ConfigGridGeneratorSubB, VelGridGeneratorSub)
+  type, bind(c), public :: kaos_gridgen_in
+    integer(c_int64_t) :: nq_ub
+    real(c_double) :: qGA, qGB, L, phi, ms, ti, te, ns0, nsnormfac
+    integer(c_int32_t) :: vperp3d = 1, symvpar = 1, cx = 0, device = 0
+    integer(c_int64_t) :: nvpar_gp, nvperp_gp
+    real(c_double) :: vpar_sigma_fac, vperp_sigma_fac
+    real(c_double) :: t_neut = 0d0, zns0_neut = 0d0, ns0_neut = 0d0
+  end type kaos_gridgen_in
+
+  type, bind(c), public :: kaos_gridgen_dims
+    integer(c_int32_t) :: nq, nvpar, nvperp, nvperp1, nvperp2, nvp, nvq, nvphi
+  end type kaos_gridgen_dims
+
+  ! grid set-up outputs: c_loc of caller arrays (c_null_ptr: not wanted)
+  type, bind(c), public :: kaos_gridgen_out
+    type(c_ptr) :: qGL = c_null_ptr, qGH = c_null_ptr, qGC = c_null_ptr, hqC = c_null_ptr, dqC = c_null_ptr, &
+                   d3xC = c_null_ptr, rGC = c_null_ptr, cthGC = c_null_ptr, ellGC = c_null_ptr, Ts = c_null_ptr, &
+                   Te = c_null_ptr, density = c_null_ptr, counts = c_null_ptr, ns_neut = c_null_ptr
+    real(c_double) :: nsnorm_clb = 0d0, nsnorm_cub = 0d0
+    type(c_ptr) :: vparGL = c_null_ptr, vparGH = c_null_ptr, vparGC = c_null_ptr, dvpar = c_null_ptr
+    type(c_ptr) :: vperpGL = c_null_ptr, vperpGH = c_null_ptr, vperpGC = c_null_ptr, dvperp = c_null_ptr, &
+                   d3v = c_null_ptr
+    type(c_ptr) :: vperp1GL = c_null_ptr, vperp1GH = c_null_ptr, vperp1GC = c_null_ptr, dvperp1 = c_null_ptr
+    type(c_ptr) :: vperp2GL = c_null_ptr, vperp2GH = c_null_ptr, vperp2GC = c_null_ptr, dvperp2 = c_null_ptr, &
+                   d3v3 = c_null_ptr
+    type(c_ptr) :: vpGL = c_null_ptr, vpGH = c_null_ptr, vpGC = c_null_ptr, dvp = c_null_ptr
+    type(c_ptr) :: vqGL = c_null_ptr, vqGH = c_null_ptr, vqGC = c_null_ptr, dvq = c_null_ptr
+    type(c_ptr) :: vphiGL = c_null_ptr, vphiGH = c_null_ptr, vphiGC = c_null_ptr, dvphi = c_null_ptr
+    type(c_ptr) :: ena_hh = c_null_ptr, ena_d33v = c_null_ptr
+  end type kaos_gridgen_out
+
   integer(c_int32_t), parameter, public :: KAOS_FIELD_COMPUTED = 0, KAOS_FIELD_INPUT_FIRST = 1, &
                                             KAOS_FIELD_INPUT_FIXED = 2
 
@@ -89,7 +121,7 @@ module kaos_hip
   end type kaos_step_out
 
   public :: kaos_init, kaos_set_grid, kaos_upload_particles, kaos_upload_vperp, kaos_cell_counts, &
-            kaos_seed_particles, kaos_step, kaos_run, &
+            kaos_gridgen_dims_of, kaos_generate_grid, kaos_seed_particles, kaos_step, kaos_run, &
             kaos_download_master, kaos_download_master3d, kaos_export_master, kaos_export_spinup, &
             kaos_read_spinup, kaos_set_field, kaos_download_cx, kaos_download_enas, &
             kaos_download_ena_master, kaos_download_ena_moments, &
@@ -138,6 +170,20 @@ module kaos_hip
       real(c_double), value :: nsnormfac
       integer(c_int64_t), intent(out) :: counts(*)
     end function kaos_cell_counts
+
+    ! grid sizes of a set-up (host arithmetic only)
+    integer(c_int) function kaos_gridgen_dims_of(gin, dims) bind(c, name='kaos_gridgen_dims_of')
+      import :: c_int, kaos_gridgen_in, kaos_gridgen_dims
+      type(kaos_gridgen_in), intent(in) :: gin
+      type(kaos_gridgen_dims), intent(out) :: dims
+    end function kaos_gridgen_dims_of
+
+    ! ConfigGridGeneratorSub + ConfigGridGeneratorSubB + VelGridGeneratorSub on the device
+    integer(c_int) function kaos_generate_grid(gin, gout) bind(c, name='kaos_generate_grid')
+      import :: c_int, kaos_gridgen_in, kaos_gridgen_out
+      type(kaos_gridgen_in), intent(in) :: gin
+      type(kaos_gridgen_out), intent(inout) :: gout
+    end function kaos_generate_grid
 
     ! DensityProfileBSub + VelocityDistributionSub + InitialConditionsSub on the device
     integer(c_int) function kaos_seed_particles(ctx, tube, counts, temperature, seed) &

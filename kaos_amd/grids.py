@@ -120,6 +120,64 @@ def velocity_grid(nvpar, nvperp, vpar_max=1.0e4, vperp_max=1.0e4):
                 vperpGH=vperpGH, vperpGC=vperpGC, dvperp=dvperp, d3v=np.ascontiguousarray(d3v))
 
 
+def tube_from_setup(g, p, h, qs, ranks=1, ubcond=0, ena_cap=0, sigma_scale=1.0):
+    """A Tube on the grids of the reference's set-up: `g` the dict of
+    solver.generate_grid (kaos_generate_grid, on the device) for the model
+    inputs `p` (the keys of kaos_gridgen_in), time step h [s], charge qs [C].
+    Initial ions per cell: g["counts"] (cell_counts); LB injection per master
+    step: nint(nsnormCLBG / ranks) (_8_0_7_BoundaryConditions.f90:747-750), UB
+    likewise when ubcond.  On the 3-D grid the tube also carries the 2-D arrays
+    kaos_grid takes, a uniform Vperp grid over |Vperp1| (counted only in the
+    2-D form).  cx: the neutral counts and the ENA grid with its metric;
+    ena_cap ENA slots, sigma_scale the cross-section factor."""
+    nq = len(g["qGL"])
+    nint = lambda x: int(math.floor(abs(x) + 0.5)) * (1 if x >= 0 else -1)
+    a = {k: np.ascontiguousarray(g[k]) for k in ("qGL", "qGH", "qGC", "hqC", "dqC", "d3xC", "rGC", "cthGC", "ellGC",
+                                                    "Te")}
+    for n, k in zip(("vparGL", "vparGH", "vparGC", "dvpar"), g["vpar"]):
+        a[n] = k
+    v3 = "vperp1" in g
+    if v3:
+        for pre, ax in (("v1", "vperp1"), ("v2", "vperp2")):
+            for suf, k in zip(("GL", "GH", "GC"), g[ax][:3]):
+                a[pre + suf] = k
+            a["d" + pre] = g[ax][3]
+        a["d3v3"] = np.ascontiguousarray(g["d3v3"])
+        nv1 = len(g["vperp1"][0])
+        vmax = float(max(abs(g["vperp1"][0][0]), abs(g["vperp1"][1][-1])))
+        vg = velocity_grid(len(a["vparGL"]), nv1, vpar_max=1.0, vperp_max=vmax)
+        for n in ("vperpGL", "vperpGH", "vperpGC", "dvperp"):
+            a[n] = vg[n]
+        a["d3v"] = ((np.abs(vg["vperpGC"])[:, None] * vg["dvperp"][:, None]) * a["dvpar"][None, :]) * (2.0 * math.pi)
+        nvperp = nv1
+    else:
+        for n, k in zip(("vperpGL", "vperpGH", "vperpGC", "dvperp"), g["vperp"]):
+            a[n] = k
+        a["d3v"] = np.ascontiguousarray(g["d3v"])
+        nvperp = len(g["vperp"][0])
+    t = Tube(nq=nq, nvpar=len(a["vparGL"]), nvperp=nvperp, p=float(p["L"]), phi=float(p.get("phi", math.pi / 2)),
+             h=float(h), ms=float(p["ms"]), qs=float(qs), nsnormfac=float(p["nsnormfac"]), Ts_lb=float(g["Ts"][0]),
+             Ts_ub=float(g["Ts"][-1]), n_inj_lb=nint(g["nsnorm_clb"] / ranks),
+             n_inj_ub=nint(g["nsnorm_cub"] / ranks) if ubcond else 0,
+             arrays={k: np.ascontiguousarray(v, dtype=np.float64) for k, v in a.items()})
+    t.ubcond = int(ubcond)
+    t.cell_counts = np.asarray(g["counts"], dtype=np.int64)
+    if v3:
+        t.v3d, t.nv1, t.nv2 = 1, len(g["vperp1"][0]), len(g["vperp2"][0])
+    if "vp" in g:
+        t.cx, t.t_neut, t.m_neut, t.cx_sigma_scale = 1, float(p["t_neut"]), float(p["ms"]), float(sigma_scale)
+        t.arrays["nsneut"] = np.ascontiguousarray(g["ns_neut"])
+        for ax in ("vp", "vq", "vphi"):
+            for suf, k in zip(("GL", "GH", "GC"), g[ax][:3]):
+                t.arrays[ax + suf] = np.ascontiguousarray(k)
+            t.arrays["d" + ax] = np.ascontiguousarray(g[ax][3])
+        t.nvp, t.nvq, t.nvphi = len(g["vp"][0]), len(g["vq"][0]), len(g["vphi"][0])
+        t.arrays["ena_hh"] = np.ascontiguousarray(g["ena_hh"])
+        t.arrays["ena_d33v"] = np.ascontiguousarray(g["ena_d33v"])
+        t.ena_cap = int(ena_cap)
+    return t
+
+
 def with_vgrid3(tube, nv1, nv2, vperp_max=None):
     """Give a tube the 3-D velocity grid of ION2VPERPflag = 1: Vperp1 and
     Vperp2 uniform in [-vperp_max, vperp_max] (default: the tube's Vpar range)

@@ -328,7 +328,7 @@ class KineticSolver:
 
 
 def cell_counts(density, d3x, nsnormfac):
-    """Macro-ions per config cell, nint((density / nsnormfac) d3x)
+    """Macro-ions per config cell, (density / nsnormfac) d3x truncated toward zero
     (kaos_cell_counts; _0_1_1_ConfigGridGeneratorB.f90:83-85,
     _2_1_DensityProfileA1.f90:45).  Host arithmetic of the library: no GPU."""
     lib = _abi.load()
@@ -341,6 +341,84 @@ def cell_counts(density, d3x, nsnormfac):
     if rc:
         raise KaosError(rc, "kaos_cell_counts: bad input (nsnormfac <= 0, or a count not finite)")
     return out
+
+
+GRIDGEN_KEYS = ("nq_ub", "qGA", "qGB", "L", "phi", "ms", "ti", "te", "ns0", "nsnormfac", "vperp3d", "symvpar", "cx",
+                "device", "nvpar_gp", "nvperp_gp", "vpar_sigma_fac", "vperp_sigma_fac", "t_neut", "zns0_neut",
+                "ns0_neut")
+
+
+def _gridgen_in(p):
+    g = _abi.KaosGridgenIn()
+    for k in GRIDGEN_KEYS:
+        if k in p:
+            setattr(g, k, p[k])
+    if "vperp3d" not in p:
+        g.vperp3d = 1
+    if "symvpar" not in p:
+        g.symvpar = 1
+    return g
+
+
+def gridgen_dims(p):
+    """Cell counts of the grids kaos_generate_grid makes from the model inputs
+    `p` (the keys of kaos_gridgen_in; host arithmetic, no device)."""
+    d = _abi.KaosGridgenDims()
+    rc = _abi.load().kaos_gridgen_dims_of(C.byref(_gridgen_in(p)), C.byref(d))
+    if rc:
+        raise KaosError(rc, "kaos_gridgen_dims_of: invalid grid set-up inputs")
+    return {n: getattr(d, n) for n, _ in d._fields_}
+
+
+def generate_grid(p):
+    """Grid set-up of one tube on the device (kaos_generate_grid) from the model
+    inputs `p` (ConfigGridGeneratorSub, ConfigGridGeneratorSubB and
+    VelGridGeneratorSub, _0_ConfigGridGenerator.f90, _0_1_1_ConfigGridGeneratorB.f90,
+    _0_0_VelGridGenerator.f90): dict of the per-cell arrays (GRIDGEN_CFG names
+    plus ns_neut), counts (nsnormC0), nsnorm_clb / nsnorm_cub, every axis as
+    (GL, GH, GC, d), d3v (2-D) or d3v3 (3-D), ena_hh and ena_d33v [Q][bin],
+    and the cell counts of each axis under "ncell"."""
+    dims = gridgen_dims(p)
+    nq = dims["nq"]
+    out = _abi.KaosGridgenOut()
+    arrs = {}
+    for k in _abi.GRIDGEN_CFG + ("ns_neut",):
+        arrs[k] = np.zeros(nq)
+        setattr(out, k, ptr(arrs[k]))
+    counts = np.zeros(nq, np.int64)
+    out.counts = ptr(counts, C.c_int64)
+    ncell = dict(zip(("vpar", "vperp", "vperp1", "vperp2", "vp", "vq", "vphi"),
+                     (dims["nvpar"], dims["nvperp"], dims["nvperp1"], dims["nvperp2"], dims["nvp"], dims["nvq"],
+                      dims["nvphi"])))
+    for ax, names in _abi.GRIDGEN_AXES:
+        if ncell[ax]:
+            for n in names:
+                arrs[n] = np.zeros(ncell[ax])
+                setattr(out, n, ptr(arrs[n]))
+    if dims["nvperp1"]:
+        arrs["d3v3"] = np.zeros(dims["nvperp1"] * dims["nvperp2"] * dims["nvpar"])
+        out.d3v3 = ptr(arrs["d3v3"])
+    else:
+        arrs["d3v"] = np.zeros(dims["nvperp"] * dims["nvpar"])
+        out.d3v = ptr(arrs["d3v"])
+    nb = dims["nvp"] * dims["nvq"] * dims["nvphi"]
+    if nb:
+        arrs["ena_hh"], arrs["ena_d33v"] = np.zeros(nq * nb), np.zeros(nq * nb)
+        out.ena_hh, out.ena_d33v = ptr(arrs["ena_hh"]), ptr(arrs["ena_d33v"])
+    rc = _abi.load().kaos_generate_grid(C.byref(_gridgen_in(p)), C.byref(out))
+    if rc:
+        raise KaosError(rc, "kaos_generate_grid failed")
+    res = {k: arrs[k] for k in _abi.GRIDGEN_CFG + ("ns_neut",)}
+    res["counts"] = counts
+    res["nsnorm_clb"], res["nsnorm_cub"] = out.nsnorm_clb, out.nsnorm_cub
+    for ax, names in _abi.GRIDGEN_AXES:
+        if ncell[ax]:
+            res[ax] = tuple(arrs[n] for n in names)
+    for k in ("d3v", "d3v3", "ena_hh", "ena_d33v"):
+        if k in arrs:
+            res[k] = arrs[k]
+    res["ncell"] = ncell
+    return res
 
 
 def read_spinup(directory, species=1, flux_tube=1, nq_cap=1 << 20):

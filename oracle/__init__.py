@@ -146,6 +146,13 @@ def load():
         lib.orc_ena_push.argtypes = [_dp, C.c_double, C.c_int]
         lib.orc_cx_block.argtypes = [C.c_uint64, C.c_int64, C.c_int, C.c_int, C.c_uint64, _dp]
         lib.orc_get_master3.argtypes = [P, _dp, _dp]
+        lib.orc_generate_grid.argtypes = ([C.c_int64] + [C.c_double] * 8 + [C.c_int] * 3 + [C.c_int64] * 2
+                                          + [C.c_double] * 5 + [_dp, i64p, _dp, _dp, _dp, _dp])
+        lib.orc_gg_qpoint.argtypes = [C.c_double, C.c_double, C.c_int64, C.c_int64]
+        lib.orc_gg_qpoint.restype = C.c_double
+        lib.orc_sin.argtypes = [C.c_double]
+        lib.orc_sin.restype = C.c_double
+        lib.orc_gg_ena_metric.argtypes = [C.c_double] * 8 + [_dp]
         _lib = lib
     return _lib
 
@@ -342,4 +349,44 @@ def seed_ions(tube, counts, temperature, seed, tube_index=0):
     got = lib.orc_seed_ions(C.byref(cfg), ptr(counts, C.c_int64), ptr(temperature), seed, tube_index,
                             *(ptr(out[k]) for k in ("q0", "rs", "vpar", "v1", "v2", "vperp")))
     assert got == n
+    return out
+
+
+GRIDGEN_CFG = ("qGL", "qGH", "qGC", "hqC", "dqC", "d3xC", "rGC", "cthGC", "ellGC", "Ts", "Te", "density", "ns_neut")
+GRIDGEN_AXES = ("vpar", "vperp", "vperp1", "vperp2", "vp", "vq", "vphi")
+
+
+def generate_grid(p):
+    """The oracle's grid set-up of one tube (orc_generate_grid) from the
+    model inputs `p` (the keys of kaos_gridgen_in): dict of the per-cell
+    arrays, counts, each axis as (GL, GH, GC, d), d3v (2-D or 3-D) and the
+    ENA metric arrays hh, d33v [Q][bin]."""
+    lib = load()
+    nq = int(p["nq_ub"])
+    npar, nperp = int(p["nvpar_gp"]) - 2, int(p["nvperp_gp"]) - 2
+    v3, cx = int(p.get("vperp3d", 1)), int(p.get("cx", 0))
+    nc = [npar, 0 if v3 else nperp, nperp if v3 else 0, nperp if v3 else 0,
+          nperp if cx else 0, npar if cx else 0, nperp if cx else 0]
+    cfg = np.zeros(13 * nq)
+    counts = np.zeros(nq, np.int64)
+    vel = np.zeros(max(1, 4 * sum(nc)))
+    d3v = np.zeros(max(1, nperp * nperp * npar if v3 else nperp * npar))
+    nb = nc[4] * nc[5] * nc[6]
+    hh = np.zeros(max(1, nq * nb))
+    d33 = np.zeros(max(1, nq * nb))
+    rc = lib.orc_generate_grid(nq, p["qGA"], p["qGB"], p["L"], p["ms"], p["ti"], p["te"], p["ns0"], p["nsnormfac"],
+                               v3, int(p.get("symvpar", 1)), cx, int(p["nvpar_gp"]), int(p["nvperp_gp"]),
+                               p["vpar_sigma_fac"], p["vperp_sigma_fac"], p.get("t_neut", 0.0),
+                               p.get("zns0_neut", 0.0), p.get("ns0_neut", 0.0), ptr(cfg), ptr(counts, C.c_int64),
+                               ptr(vel), ptr(d3v), ptr(hh), ptr(d33))
+    assert rc == 0
+    out = {k: cfg[i * nq:(i + 1) * nq].copy() for i, k in enumerate(GRIDGEN_CFG)}
+    out["counts"] = counts
+    off = 0
+    for a, n in zip(GRIDGEN_AXES, nc):
+        out[a] = tuple(vel[off + k * n:off + (k + 1) * n].copy() for k in range(4))
+        off += 4 * n
+    out["d3v3" if v3 else "d3v"] = d3v
+    out["ena_hh"], out["ena_d33v"] = hh[:nq * nb], d33[:nq * nb]
+    out["ncell"] = dict(zip(GRIDGEN_AXES, nc))
     return out

@@ -1058,8 +1058,9 @@ void orc_inject_uniforms(uint64_t seed, int64_t step, int tube, int64_t k, doubl
 
 /* ---- initial ions ----------------------------------------------------------- */
 
-/* NsFARRp of one cell: nint(nsnormC0), nsnormC0 = (n/nsnormfac) d3x
- * (_0_1_1_ConfigGridGeneratorB.f90:83-85, _2_1_DensityProfileA1.f90:45). */
+/* NsFARRp of one cell: nsnormC0 = (n/nsnormfac) d3x truncated on its
+ * assignment to the integer array (_KineticMainParams.f90:64,
+ * _0_1_1_ConfigGridGeneratorB.f90:83-85, _2_1_DensityProfileA1.f90:45). */
 int64_t orc_cell_count(double density, double d3x, double nsnormfac) {
   return kaos_cell_count(density, d3x, nsnormfac);
 }
@@ -1114,3 +1115,114 @@ void orc_ena_from_ion(double rs, double vs, double v1, double v2, double ck4, do
 }
 void orc_ena_push(double *X, double h, int grav) { kaos_ena_push(X, h, grav); }
 
+
+/* ---- grid set-up ------------------------------------------------------------ */
+
+/* ConfigGridGeneratorSub, ConfigGridGeneratorSubB and VelGridGeneratorSub for
+ * one tube, loop for loop over the set-up cells Qind = 1..NqUB+2
+ * (_0_ConfigGridGenerator.f90:299-547, _0_1_1_ConfigGridGeneratorB.f90:44-189),
+ * the re-index onto loop cells Q = Qind - 1 (_1_SimParameterization.f90:1156-1179)
+ * and the velocity axes (_0_0_VelGridGenerator.f90:59-857).  Outputs:
+ * cfg[13][nq] = qGL, qGH, qGC, hqC, dqC, d3xC, rGC, cos(thetaGC), ellGC, Ts0,
+ * Te0, nsC, nsnormCNeut (truncated); counts[nq] = nsnormC0 (truncated);
+ * vel = for the axes V||, Vperp (2-D), Vperp1, Vperp2, Vp, Vq, Vphi (ncell[a]
+ * cells, 0 = not generated) [GL | GH | GC | d]; d3v (2-D [Vperp][V||] or 3-D
+ * [V1][V2][V||]); hh, d33 [Q][ip][iq][iphi] from set-up cell Q's theta.
+ * Returns 0, or -1 for a count that is not finite. */
+int orc_generate_grid(int64_t nq_ub, double qGA, double qGB, double L, double ms, double ti, double te, double ns0,
+                      double nsnormfac, int vperp3d, int symvpar, int cx, int64_t nvpar_gp, int64_t nvperp_gp,
+                      double fpar, double fperp, double tneut, double z0, double n0, double *cfg, int64_t *counts,
+                      double *vel, double *d3v, double *hh, double *d33) {
+  const int nq = (int)nq_ub, n0c = nq + 2;
+  const long long ngp = nq_ub + 3;
+  kaos_gg_cell_t *cells = (kaos_gg_cell_t *)malloc(sizeof(kaos_gg_cell_t) * n0c);
+  double *icb = (double *)malloc(sizeof(double) * n0c);
+  for (int i = 0; i < n0c; ++i) {
+    kaos_gg_cell(qGA, qGB, ngp, L, i + 1, &cells[i]);
+    icb[i] = kaos_gg_icbbp(&cells[i]);
+  }
+  /* ICbb = sum(ICbbp(1:Qind)) as a running sum, gC = ICbb - ICbbp(1) */
+  double run = icb[0];
+  int bad = 0;
+  for (int Q = 0; Q < nq; ++Q) {
+    const kaos_gg_cell_t *c = &cells[Q + 1];
+    run = run + icb[Q + 1];
+    const double ns = kaos_gg_ns(run - icb[0], ms, ti, te, ns0, qGA);
+    const double x = (ns / nsnormfac) * c->d3x;
+    const double xn = cx ? (kaos_gg_ns_neut(c->r, ms, tneut, z0, n0, qGA) / nsnormfac) * c->d3x : 0.0;
+    const double v[13] = {c->qGL, c->qGH, c->qGC, c->hq, c->dq, c->d3x, c->r, c->cth, c->ell,
+                          (1.0 / 3.0) * ti + (2.0 / 3.0) * ti, te, ns, trunc(xn)};
+    for (int k = 0; k < 13; ++k) cfg[(int64_t)k * nq + Q] = v[k];
+    if (!(x > -9.2e18 && x < 9.2e18)) bad = 1;
+    counts[Q] = bad ? -1 : (int64_t)trunc(x);
+  }
+  /* velocity axes */
+  const double Npar = (double)nvpar_gp / 2.0, Nperp = (double)nvperp_gp / 2.0;
+  const double sp = kaos_thermal_speed(ti, ms), sn = cx ? kaos_thermal_speed(tneut, ms) : 0.0;
+  const int south = qGA <= 0.0;
+  const int nc[7] = {(int)nvpar_gp - 2, vperp3d ? 0 : (int)nvperp_gp - 2, vperp3d ? (int)nvperp_gp - 2 : 0,
+                     vperp3d ? (int)nvperp_gp - 2 : 0, cx ? (int)nvperp_gp - 2 : 0, cx ? (int)nvpar_gp - 2 : 0,
+                     cx ? (int)nvperp_gp - 2 : 0};
+  int off[7], tot = 0;
+  for (int a = 0; a < 7; ++a) { off[a] = tot; tot += 4 * nc[a]; }
+  double P[520];
+  for (int a = 0; a < 7; ++a) {
+    const int n = nc[a];
+    for (int i = 1; i <= n + 1; ++i) {
+      double x;
+      if (a == 0) x = symvpar ? kaos_gg_vsym(Npar, fpar * sp, i) : kaos_gg_vlin((double)nvpar_gp, -4.0 * sp, fpar * sp, south, i);
+      else if (a <= 3) x = kaos_gg_vsym(Nperp, fperp * sp, i);
+      else if (a == 5) x = symvpar ? kaos_gg_vena(Npar, fpar * sn, i) : kaos_gg_vlin((double)nvpar_gp, -4.0 * sn, fpar * sn, south, i);
+      else x = kaos_gg_vena(Nperp, fperp * sn, i);
+      P[i] = x;
+    }
+    double *b = vel + off[a];
+    for (int i = 0; i < n; ++i) {
+      b[i] = P[i + 1];
+      b[n + i] = P[i + 2];
+      b[2 * n + i] = (P[i + 1] + P[i + 2]) / 2.0;
+      if (a == 0 && b[2 * n + i] == 0.0) b[2 * n + i] = 1e-15;
+      b[3 * n + i] = fabs(P[i + 2] - P[i + 1]);
+    }
+  }
+  const int np = nc[0];
+  if (vperp3d) {
+    const double *c1 = vel + off[2], *c2 = vel + off[3], *cp = vel + off[0];
+    const int n1 = nc[2], n2 = nc[3];
+    for (int i1 = 0; i1 < n1; ++i1)
+      for (int i2 = 0; i2 < n2; ++i2)
+        for (int ip = 0; ip < np; ++ip)
+          d3v[((int64_t)i1 * n2 + i2) * np + ip] = (((1.0 * 1.0) * 1.0 * c1[3 * n1 + i1]) * c2[3 * n2 + i2]) * cp[3 * np + ip];
+  } else {
+    const double *c1 = vel + off[1], *cp = vel + off[0];
+    const int n1 = nc[1];
+    for (int i1 = 0; i1 < n1; ++i1)
+      for (int ip = 0; ip < np; ++ip) {
+        double d = ((((1.0 * 1.0) * fabs(c1[2 * n1 + i1])) * c1[3 * n1 + i1]) * cp[3 * np + ip]) * (2.0 * KAOS_PI);
+        if (d == 0.0) d = 1e-9;
+        d3v[(int64_t)i1 * np + ip] = d;
+      }
+  }
+  if (cx) {
+    const double *vp = vel + off[4], *vq = vel + off[5], *vf = vel + off[6];
+    const int a = nc[4], b = nc[5], f = nc[6];
+    int64_t k = 0;
+    for (int Q = 0; Q < nq; ++Q)
+      for (int ip = 0; ip < a; ++ip)
+        for (int iq = 0; iq < b; ++iq)
+          for (int iphi = 0; iphi < f; ++iphi, ++k)
+            kaos_gg_ena_metric(cells[Q].cth, cells[Q].sth, cells[Q].ell, vp[2 * a + ip], vq[2 * b + iq], vp[3 * a + ip],
+                               vq[3 * b + iq], vf[3 * f + iphi], &hh[k], &d33[k]);
+  }
+  free(cells);
+  free(icb);
+  return bad ? -1 : 0;
+}
+
+/* Shared-header pieces of the set-up, for the pins. */
+double orc_gg_qpoint(double qGA, double qGB, int64_t ngp, int64_t i) { return kaos_gg_qpoint(qGA, qGB, ngp, i); }
+double orc_sin(double x) { return kaos_sin(x); }
+void orc_gg_ena_metric(double cth, double sth, double ell, double vp, double vq, double dvp, double dvq, double dvphi,
+                       double *o2) {
+  kaos_gg_ena_metric(cth, sth, ell, vp, vq, dvp, dvq, dvphi, &o2[0], &o2[1]);
+}

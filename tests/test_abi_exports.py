@@ -56,6 +56,11 @@ int main(void) {
          offsetof(kaos_grid, dvperp2), offsetof(kaos_grid, d3v3));
   printf("%zu %zu %zu %zu\n", sizeof(kaos_spinup_info), offsetof(kaos_spinup_info, nq_lb),
          offsetof(kaos_spinup_info, nsnorm_cub), offsetof(kaos_spinup_info, ub_replenish));
+  printf("%zu %zu %zu %zu\n", sizeof(kaos_gridgen_in), offsetof(kaos_gridgen_in, vperp3d),
+         offsetof(kaos_gridgen_in, nvpar_gp), offsetof(kaos_gridgen_in, ns0_neut));
+  printf("%zu %zu %zu %zu %zu\n", sizeof(kaos_gridgen_out), offsetof(kaos_gridgen_out, counts),
+         offsetof(kaos_gridgen_out, nsnorm_cub), offsetof(kaos_gridgen_out, d3v3), offsetof(kaos_gridgen_out, ena_d33v));
+  printf("%zu\n", sizeof(kaos_gridgen_dims));
   return 0;
 }
 '''
@@ -80,5 +85,10 @@ int main(void) {
             _abi.KaosGrid.nvperp1.offset, _abi.KaosGrid.vperp1GL.offset, _abi.KaosGrid.dvperp2.offset,
             _abi.KaosGrid.d3v3.offset,
             C.sizeof(_abi.KaosSpinupInfo), _abi.KaosSpinupInfo.nq_lb.offset, _abi.KaosSpinupInfo.nsnorm_cub.offset,
-            _abi.KaosSpinupInfo.ub_replenish.offset]
+            _abi.KaosSpinupInfo.ub_replenish.offset,
+            C.sizeof(_abi.KaosGridgenIn), _abi.KaosGridgenIn.vperp3d.offset, _abi.KaosGridgenIn.nvpar_gp.offset,
+            _abi.KaosGridgenIn.ns0_neut.offset,
+            C.sizeof(_abi.KaosGridgenOut), _abi.KaosGridgenOut.counts.offset, _abi.KaosGridgenOut.nsnorm_cub.offset,
+            _abi.KaosGridgenOut.d3v3.offset, _abi.KaosGridgenOut.ena_d33v.offset,
+            C.sizeof(_abi.KaosGridgenDims)]
     assert got == want

@@ -8,7 +8,10 @@ then takes, as the reference's restart does:
   nsnormC0 of _0_1_1_ConfigGridGeneratorB.f90:75-86);
 * the per-cell temperatures of the initial Maxwellians from
   TemperatureOutputRT (kaos_seed_particles, _6_VelocityDistribution.f90:70-73);
-* the LB injection count nint(nsnormCLBInput) (one rank, BC:747-750);
+* the LB injection count nint(nsnormCLBGT) (one rank, BC:747-750), where
+  nsnormCLBGT = nsnormC0(NqLB + 1) is the first computational cell's count
+  (_0_1_1_ConfigGridGeneratorB.f90:189; nsnormCLBInput fills only the ghost
+  cell nsnormC0(NqLB), :76-78);
 * the ambipolar field EAmagOutputRT (kaos_set_field), at step 1 only or at
   every master step (_8_7_AmbipolarEfield.f90:1375-1407).
 The oracle restarts from the same arrays (orc_seed_ions, set_field_input);
@@ -65,7 +68,7 @@ def test_restart_from_spinup_files_bitexact(tmp_path, mode):
     assert _abi.load().kaos_cell_counts(t.nq, _abi.ptr(sp["density"]), _abi.ptr(d3x), t.nsnormfac,
                                         _abi.ptr(counts, C.c_int64)) == 0
     assert counts.sum() > 1000
-    t.n_inj_lb = int(np.floor(sp["nsnorm_clb"] + 0.5))       # nint, one rank: 37 per master step
+    t.n_inj_lb = int(counts[0])                              # nsnormCLBGT: the first cell's count, one rank
     t.Ts_lb = float(sp["temperature"][0])
     temp = sp["temperature"]
     nsteps, seed = 6, 4242

@@ -27,22 +27,23 @@ def lib():
     return oracle.load()
 
 
-def _nint_exact(x):
-    """Fortran NINT of a binary64 value, in exact rational arithmetic."""
+def _int_exact(x):
+    """Fortran INT (assignment to an integer) of a binary64 value: truncation
+    toward zero, in exact rational arithmetic."""
     f = Fraction(x)
-    a = abs(f)
-    n = math.floor(a)
-    if a - n >= Fraction(1, 2):
-        n += 1
+    n = math.floor(abs(f))
     return n if f >= 0 else -n
 
 
-def test_cell_count_rounds_half_away_from_zero(lib):
-    # exact halves and their neighbours
-    for dens, fac, d3x, want in ((5.0, 2.0, 1.0, 3), (1.0, 2.0, 1.0, 1), (3.0, 2.0, 1.0, 2),
-                                 (7.0, 2.0, 1.0, 4), (0.0, 1.0, 5.0, 0), (1.0, 4.0, 1.0, 0),
-                                 (0.49999999999999994, 1.0, 1.0, 0), (2.5000000000000004, 1.0, 1.0, 3),
-                                 (-2.5, 1.0, 1.0, -3), (-2.4999999999999996, 1.0, 1.0, -2)):
+def test_cell_count_truncates_toward_zero(lib):
+    """nsnormC0 is an integer array (_KineticMainParams.f90:64): the density
+    times d3x / nsnormfac is truncated on assignment; the reference's initial
+    ions per cell are these counts (pinned against the reference's own run in
+    test_oracle_ref_loop.py)."""
+    for dens, fac, d3x, want in ((5.0, 2.0, 1.0, 2), (1.0, 2.0, 1.0, 0), (3.0, 2.0, 1.0, 1),
+                                 (7.0, 2.0, 1.0, 3), (0.0, 1.0, 5.0, 0), (1.0, 4.0, 1.0, 0),
+                                 (0.9999999999999999, 1.0, 1.0, 0), (3.0, 1.0, 1.0, 3), (2.9999999999999996, 1.0, 1.0, 2),
+                                 (-2.5, 1.0, 1.0, -2), (-2.9999999999999996, 1.0, 1.0, -2)):
         assert lib.orc_cell_count(dens, d3x, fac) == want, (dens, fac, d3x)
     rng = np.random.default_rng(3)
     dens = 10.0 ** rng.uniform(6, 12, 4000)
@@ -51,7 +52,7 @@ def test_cell_count_rounds_half_away_from_zero(lib):
     for a, b, c in zip(dens, d3x, fac):
         x = (a / c) * b
         if x < 9e18:
-            assert lib.orc_cell_count(a, b, c) == _nint_exact(x)
+            assert lib.orc_cell_count(a, b, c) == _int_exact(x)
     assert lib.orc_cell_count(float("nan"), 1.0, 1.0) == -1
     assert lib.orc_cell_count(1e30, 1e10, 1.0) == -1
 
