@@ -159,6 +159,18 @@ KAOS_HD int kaos_find_qcell_from(double q, const double* qGH, double qGL0, doubl
   return i + 1;
 }
 
+/* Velocity cell by a walk from a hint cell (1-based; 0 or out of range:
+ * bisection), the same result as kaos_find_vcell on ascending contiguous
+ * edges: the first cell whose high edge is not passed.                    */
+KAOS_HD int kaos_find_vcell_from(double v, const double* GH, double GL0, int n, int hint) {
+  if (!(v >= GL0) || !(v <= GH[n - 1])) return 0;
+  if (hint < 1 || hint > n) return kaos_find_vcell(v, GH, GL0, n);
+  int i = hint - 1;
+  while (i > 0 && v <= GH[i - 1]) --i;
+  while (!(v <= GH[i])) ++i;
+  return i + 1;
+}
+
 /* Velocity cell from a uniform-spacing guess (scale = n/(GH[n-1]-GL0)). */
 KAOS_HD int kaos_find_vcell_guess(double v, const double* GH, double GL0, int n, double scale) {
   if (!(v >= GL0) || !(v <= GH[n - 1])) return 0;

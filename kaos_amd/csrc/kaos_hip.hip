@@ -497,7 +497,7 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_icr(StepArgs a) {
     const double vs = a.vp[j];
     const double v1 = a.vr[j], v2 = a.vr2[j];
     int iq = (int)((cw >> 8) & 0xFFu);
-    const int iqn = kaos_find_vcell(fabs(sqrt(v1 * v1 + v2 * v2)), GH, GH[T.nvperp], T.nvperp);
+    const int iqn = kaos_find_vcell_from(fabs(sqrt(v1 * v1 + v2 * v2)), GH, GH[T.nvperp], T.nvperp, iq);
     if (iqn >= 1) iq = iqn;
     a.cell[j] = (cw & ~(0xFFu << 8)) | ((uint32_t)iq << 8);
     const kaos_line_t L = kaos_line_i(rs, irep);
@@ -1462,18 +1462,31 @@ __global__ void k_reconstruct(DevTube T, long long n, StepArgs a, int mixed, dou
 // kaos_cx_cell).  Conversion, at the start of every later step: the first
 // nint(poi) live ions of each cell in id order become ENAs (_8_KineticSolver.f90:
 // 95-166).  The id threshold of each cell is found by a radix select over
-// the 40-bit ids, four 10-bit digits: per digit a histogram pass over the
-// ions (k_cx_hist) and one wavefront per cell that finds the digit holding
-// the remaining count (k_cx_scan); a cell with no more ions than its count
-// takes all.  k_cx_convert then moves every ion at or below its cell's
+// the ids in 14-bit digits, as many as the tube's ids need (ids below
+// next_id: two digits up to 2^28 ids, three up to the 40-bit limit): per
+// digit a histogram pass over the ions (k_cx_hist, counts gathered per
+// wavefront and per block in LDS before the global atomics, since the ions
+// of a tile share their cell and mostly their digit) and one wavefront per
+// cell that finds the digit holding the remaining count (k_cx_scan); a cell
+// with no more ions than its count takes all.  k_cx_convert then moves every ion at or below its cell's
 // threshold to the tube's ENA slots (IonNeutralCollisionsSub) and frees its
 // slot.  ENAs: k_ena_step removes those that left at LB/UB (at master
 // steps), finds each one's cell at its pre-push position, counts them per
 // cell at master steps and pushes them ballistically.
 
 #define KAOS_ENA_DEAD ((int)0x80000000)
-#define KAOS_CX_BINS 1024
-#define KAOS_CX_LEVELS 4
+#define KAOS_CX_DBITS 14
+#define KAOS_CX_BINS (1 << KAOS_CX_DBITS)
+#define KAOS_CX_LEVELS 3                     // 42 bits >= the 40-bit id limit
+#define KAOS_CX_SLOTS 64                     // per-block LDS table of (cell, digit) counts
+
+// Digits of the radix select for ids below next_id (1..3).
+__device__ __forceinline__ int kd_cx_levels(long long next_id) {
+  const unsigned long long m = (unsigned long long)(next_id > 1 ? next_id - 1 : 1);
+  const int bits = 64 - __clzll((long long)m);
+  const int l = (bits + KAOS_CX_DBITS - 1) / KAOS_CX_DBITS;
+  return l < 1 ? 1 : (l > KAOS_CX_LEVELS ? KAOS_CX_LEVELS : l);
+}
 
 // (signed radius, V||) of slot j of tube T in either form; false for a free slot.
 __device__ __forceinline__ bool kd_line_state(const StepArgs& a, const DevTube& T, long long j, int mixed,
@@ -1530,7 +1543,8 @@ __global__ void k_cx_freq(const DevTube* tubes, const double* M3, const double* 
 }
 
 __global__ void k_cx_sel_init(int ncell, const DevTube* tubes, const int* blk_tube, const int* blk_q,
-                              const double* cx, long long* need, long long* pref, long long* thr) {
+                              const double* cx, const long long* next_id, long long* need, long long* pref,
+                              long long* thr, int* lev) {
   const int ci = blockIdx.x * blockDim.x + threadIdx.x;
   if (ci >= ncell) return;
   const DevTube& T = tubes[blk_tube[ci]];
@@ -1539,6 +1553,7 @@ __global__ void k_cx_sel_init(int ncell, const DevTube* tubes, const int* blk_tu
   need[g] = T.cx ? kaos_cx_take(cx[3 * T.goff + 2 * T.nq + Q]) : 0;
   pref[g] = 0;
   thr[g] = -1;
+  lev[g] = kd_cx_levels(next_id[blk_tube[ci]]);
 }
 
 // The cell (Qindk1) of a live ion of a charge-exchange tube whose cell still
@@ -1552,84 +1567,139 @@ __device__ __forceinline__ int kd_cx_cell(const StepArgs& a, const DevTube& T, l
   return need[T.goff + qc - 1] > 0 ? qc : 0;
 }
 
-// Adds 1 to hist[key] for every active lane of the wavefront: lanes with the
-// same key add together (one atomic per distinct key and wavefront).  The ions
-// of a tile share their cell and mostly their id digit, so per-lane atomics on
-// one address would serialise (millions of ions per cell on a large tube).
-__device__ __forceinline__ void kd_wave_count(unsigned int* hist, bool active, uint32_t key) {
+// Counts of (cell, digit) keys: each lane brings one key with a count (a run
+// of its ions).  Up to KAOS_CX_ROUNDS times, the lanes of a wavefront that
+// share the first remaining lane's key add their counts together into the
+// block's LDS table (flushed to the global histogram at the end of the
+// block); lanes left after that add theirs to the global histogram
+// directly.  High digits are shared by whole tiles (the ions of a cell have
+// neighbouring ids; ions that moved to a neighbouring cell bring a second or
+// third key) and are gathered in the rounds; low digits are mostly distinct
+// and go straight to distinct addresses.  The table starts filled with an
+// invalid key, and a slot's key is written after it is reserved, so a reader
+// sees either that invalid key or the right one: a reader that misses a key
+// being added adds a second entry for it; a full table sends the count to
+// the global histogram.
+#define KAOS_CX_ROUNDS 4
+__device__ __forceinline__ void kd_cx_count(unsigned int* hist, bool active, uint32_t key, unsigned int cnt,
+                                            uint32_t* s_key, unsigned int* s_cnt, int* s_n) {
   const int lane = threadIdx.x & 63;
-  for (;;) {
+  for (int round = 0; round < KAOS_CX_ROUNDS; ++round) {
     const unsigned long long m = __ballot(active);
-    if (m == 0ull) break;
+    if (m == 0ull) return;
     const int leader = __ffsll((long long)m) - 1;
     const uint32_t lk = (uint32_t)__shfl((int)key, leader, 64);
     const bool same = active && key == lk;
-    const unsigned long long ms = __ballot(same);
-    if (lane == leader) atomicAdd(&hist[lk], (unsigned int)__popcll(ms));
+    unsigned int c = same ? cnt : 0u;
+    for (int st = 32; st >= 1; st >>= 1) c += (unsigned int)__shfl_xor((int)c, st, 64);
+    if (lane == leader) {
+      const int n = min(*(volatile int*)s_n, KAOS_CX_SLOTS);
+      int i = 0;
+      while (i < n && ((volatile uint32_t*)s_key)[i] != lk) ++i;
+      if (i < n) {
+        atomicAdd(&s_cnt[i], c);
+      } else {
+        const int k = atomicAdd(s_n, 1);
+        if (k < KAOS_CX_SLOTS) {
+          atomicAdd(&s_cnt[k], c);
+          ((volatile uint32_t*)s_key)[k] = lk;
+        } else {
+          atomicAdd(&hist[lk], c);
+        }
+      }
+    }
     if (same) active = false;
   }
+  if (active) atomicAdd(&hist[key], cnt);
 }
 
+// One digit of the id select: the tile's ions of selecting cells (loads for
+// all of a lane's ions first), runs of equal keys within a lane merged, then
+// counted per wavefront and block.  A tube whose ids need fewer digits than
+// `level` + 1 returns at once.
 __global__ void __launch_bounds__(KAOS_BLOCK) k_cx_hist(StepArgs a, int mixed, int level, const long long* need,
-                                                        const long long* pref, unsigned int* hist) {
+                                                        const long long* pref, const int* lev, unsigned int* hist) {
+  __shared__ uint32_t s_key[KAOS_CX_SLOTS];
+  __shared__ unsigned int s_cnt[KAOS_CX_SLOTS];
+  __shared__ int s_n;
   const int tile = blockIdx.x, tb = a.tile_tube[tile];
   const DevTube& T = a.tubes[tb];
   if (!T.cx) return;
+  const int L = lev[T.goff];                              // the same for every cell of the tube
+  if (level >= L) return;
+  if (threadIdx.x < KAOS_CX_SLOTS) { s_key[threadIdx.x] = 0xFFFFFFFFu; s_cnt[threadIdx.x] = 0u; }
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
   const long long end = T.off + a.count[tb];
-  const int sh = 30 - 10 * level;
-  for (int it = 0; it < KAOS_TILE_PPT; ++it) {           // every lane runs every iteration (wave counts)
+  const int sh = KAOS_CX_DBITS * (L - 1 - level);
+  bool on[KAOS_TILE_PPT];
+  uint32_t key[KAOS_TILE_PPT];
+#pragma unroll
+  for (int it = 0; it < KAOS_TILE_PPT; ++it) {
     const long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
-    bool on = false;
-    uint32_t key = 0u;
+    on[it] = false;
+    key[it] = 0u;
     if (j < end) {
       const int qc = kd_cx_cell(a, T, j, mixed, need);
       if (qc) {
         const long long g = T.goff + qc - 1;
         const long long id = a.id[j];
         if (level == 0 && (id >> 40) != 0) set_status(a.status, 1u << KAOS_E_CAPACITY);   // beyond 40-bit ids
-        on = level == 0 || (id >> (sh + 10)) == pref[g];
-        key = (uint32_t)(g * KAOS_CX_BINS + ((id >> sh) & (KAOS_CX_BINS - 1)));
+        on[it] = level == 0 || (id >> (sh + KAOS_CX_DBITS)) == pref[g];
+        key[it] = (uint32_t)(g * KAOS_CX_BINS + ((id >> sh) & (KAOS_CX_BINS - 1)));
       }
     }
-    kd_wave_count(hist, on, key);
   }
+  unsigned int run[KAOS_TILE_PPT];
+#pragma unroll
+  for (int it = KAOS_TILE_PPT - 1; it >= 0; --it) {
+    run[it] = on[it] ? 1u : 0u;
+    if (it + 1 < KAOS_TILE_PPT && on[it] && on[it + 1] && key[it + 1] == key[it]) run[it] += run[it + 1];
+  }
+#pragma unroll
+  for (int it = 0; it < KAOS_TILE_PPT; ++it) {           // every lane runs every round (wave counts)
+    const bool starts = on[it] && !(it > 0 && on[it - 1] && key[it - 1] == key[it]);
+    kd_cx_count(hist, starts, key[it], run[it], s_key, s_cnt, &s_n);
+  }
+  __syncthreads();
+  const int n = min(s_n, KAOS_CX_SLOTS);
+  if ((int)threadIdx.x < n && s_cnt[threadIdx.x]) atomicAdd(&hist[s_key[threadIdx.x]], s_cnt[threadIdx.x]);
 }
 
-// One wavefront per cell: lane l holds digits 16 l .. 16 l + 15.
-__global__ void k_cx_scan(int ncell, int level, long long* need, long long* pref, long long* thr,
+// One wavefront per cell: lane l holds digits 256 l .. 256 l + 255.
+__global__ void k_cx_scan(int ncell, int level, const int* lev, long long* need, long long* pref, long long* thr,
                           unsigned int* hist) {
+  constexpr int PER = KAOS_CX_BINS / 64;
   const int lane = threadIdx.x & 63;
   const long long g = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (g >= ncell) return;
   const long long k = need[g];
-  if (k <= 0) return;
-  unsigned int* h = hist + g * KAOS_CX_BINS + 16 * lane;
-  unsigned int v[16];
+  if (k <= 0 || level >= lev[g]) return;
+  unsigned int* h = hist + g * KAOS_CX_BINS + PER * lane;
   unsigned long long sum = 0;
-  for (int i = 0; i < 16; ++i) { v[i] = h[i]; sum += v[i]; h[i] = 0u; }
+  for (int i = 0; i < PER; ++i) sum += h[i];
   unsigned long long incl = sum;
   for (int st = 1; st < 64; st <<= 1) {
     const unsigned long long y = __shfl_up(incl, st, 64);
     if (lane >= st) incl += y;
   }
   const unsigned long long total = __shfl(incl, 63, 64);
-  if (level == 0 && total <= (unsigned long long)k) {         // every ion of the cell converts
-    if (lane == 0) { thr[g] = 0x7FFFFFFFFFFFFFFFLL; need[g] = 0; }
-    return;
-  }
   const unsigned long long excl = incl - sum;
-  if (excl < (unsigned long long)k && (unsigned long long)k <= incl) {   // exactly one lane
+  const bool all = level == 0 && total <= (unsigned long long)k;     // every ion of the cell converts
+  if (!all && excl < (unsigned long long)k && (unsigned long long)k <= incl) {   // exactly one lane
     unsigned long long c = excl;
     int d = 0;
-    for (; d < 16; ++d) {
-      if (c + v[d] >= (unsigned long long)k) break;
-      c += v[d];
+    for (; d < PER; ++d) {
+      if (c + h[d] >= (unsigned long long)k) break;
+      c += h[d];
     }
-    const long long np = (pref[g] << 10) | (long long)(16 * lane + d);
+    const long long np = (pref[g] << KAOS_CX_DBITS) | (long long)(PER * lane + d);
     pref[g] = np;
-    if (level == KAOS_CX_LEVELS - 1) { thr[g] = np; need[g] = 0; }
+    if (level == lev[g] - 1) { thr[g] = np; need[g] = 0; }
     else need[g] = k - (long long)c;
   }
+  for (int i = 0; i < PER; ++i) h[i] = 0u;                    // the histogram starts at zero next time
+  if (all && lane == 0) { thr[g] = 0x7FFFFFFFFFFFFFFFLL; need[g] = 0; }
 }
 
 __global__ void __launch_bounds__(KAOS_BLOCK) k_cx_convert(StepArgs a, int mixed, const long long* thr, double* ena,
@@ -1643,11 +1713,10 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_cx_convert(StepArgs a, int mixed
   for (int it = 0; it < KAOS_TILE_PPT; ++it) {           // every lane runs every iteration (wave slot counts)
     const long long j = a.tile_start[tile] + (long long)it * KAOS_BLOCK + threadIdx.x;
     bool conv = false;
-    double r = 0.0, v = 0.0;
     int qc = 0;
     if (j < end) {
       const uint32_t cw = a.cell[j];
-      if (kd_line_state(a, T, j, mixed, r, v)) {
+      if (mixed ? (cw != KAOS_DEAD) : (a.rs[j] != 0.0)) {
         qc = (int)((cw >> 16) & KAOS_QMASK) - 1;
         if (qc >= 1) {
           const long long th = thr[T.goff + qc - 1];
@@ -1670,6 +1739,8 @@ __global__ void __launch_bounds__(KAOS_BLOCK) k_cx_convert(StepArgs a, int mixed
     if (!conv) continue;
     const long long e = base + __popcll(m & ((1ull << lane) - 1ull));
     if (e >= T.ecap) { set_status(a.status, 1u << KAOS_E_CAPACITY); continue; }
+    double r, v;
+    kd_line_state(a, T, j, mixed, r, v);                  // a converting ion is live
     const double v1 = mixed ? (double)a.vrf[j] : a.vr[j], v2 = mixed ? (double)a.vrf2[j] : a.vr2[j];
     const double c4 = mixed ? (double)a.ck4f[j] : a.ck4[j];
     double X[6];
@@ -1878,6 +1949,7 @@ struct kaos_ctx {
   double *d_nsneut = nullptr, *d_cxgeo = nullptr, *d_cx = nullptr;
   long long *d_cx_need = nullptr, *d_cx_pref = nullptr, *d_cx_thr = nullptr;
   unsigned int* d_cx_hist = nullptr;
+  int* d_cx_lev = nullptr;                      // radix-select digits of each cell's tube
   double* d_ena = nullptr;
   long long *d_eid = nullptr, *d_ena_cnt = nullptr, *d_eoff = nullptr;
   int* d_eq = nullptr;
@@ -2416,8 +2488,11 @@ static int build_device(kaos_ctx* c) {
     HIPCHK(c, hipMalloc((void**)&c->d_cx_need, sizeof(long long) * goff));
     HIPCHK(c, hipMalloc((void**)&c->d_cx_pref, sizeof(long long) * goff));
     HIPCHK(c, hipMalloc((void**)&c->d_cx_thr, sizeof(long long) * goff));
-    HIPCHK(c, hipMalloc((void**)&c->d_cx_hist, sizeof(unsigned int) * 1024 * goff));
-    HIPCHK(c, hipMemset(c->d_cx_hist, 0, sizeof(unsigned int) * 1024 * goff));
+    if ((double)goff * KAOS_CX_BINS >= 4294967296.0)
+      return fail(c, KAOS_E_ARG, "charge exchange: too many config cells for the id select's 32-bit keys");
+    HIPCHK(c, hipMalloc((void**)&c->d_cx_lev, sizeof(int) * goff));
+    HIPCHK(c, hipMalloc((void**)&c->d_cx_hist, sizeof(unsigned int) * KAOS_CX_BINS * (size_t)goff));
+    HIPCHK(c, hipMemset(c->d_cx_hist, 0, sizeof(unsigned int) * KAOS_CX_BINS * (size_t)goff));
     HIPCHK(c, hipMalloc((void**)&c->d_ena, sizeof(double) * 6 * ne));
     HIPCHK(c, hipMalloc((void**)&c->d_eid, sizeof(long long) * ne));
     HIPCHK(c, hipMalloc((void**)&c->d_eq, sizeof(int) * ne));
@@ -3039,12 +3114,13 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
   auto cx_convert = [&]() {
     StepArgs a = make_args(c, 0, 0, 0, 0);
     hipLaunchKernelGGL(k_cx_sel_init, dim3((ncell + 255) / 256), dim3(256), 0, c->stream, ncell, c->d_tubes,
-                       c->d_blk_tube, c->d_blk_q, c->d_cx, c->d_cx_need, c->d_cx_pref, c->d_cx_thr);
+                       c->d_blk_tube, c->d_blk_q, c->d_cx, c->d_next_id, c->d_cx_need, c->d_cx_pref, c->d_cx_thr,
+                       c->d_cx_lev);
     for (int L = 0; L < KAOS_CX_LEVELS; ++L) {
       hipLaunchKernelGGL(k_cx_hist, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a, mixed, L, c->d_cx_need,
-                         c->d_cx_pref, c->d_cx_hist);
-      hipLaunchKernelGGL(k_cx_scan, dim3((ncell + 3) / 4), dim3(256), 0, c->stream, ncell, L, c->d_cx_need,
-                         c->d_cx_pref, c->d_cx_thr, c->d_cx_hist);
+                         c->d_cx_pref, c->d_cx_lev, c->d_cx_hist);
+      hipLaunchKernelGGL(k_cx_scan, dim3((ncell + 3) / 4), dim3(256), 0, c->stream, ncell, L, c->d_cx_lev,
+                         c->d_cx_need, c->d_cx_pref, c->d_cx_thr, c->d_cx_hist);
     }
     hipLaunchKernelGGL(k_cx_convert, dim3(c->ntiles), dim3(KAOS_BLOCK), 0, c->stream, a, mixed, c->d_cx_thr,
                        c->d_ena, c->d_eid, c->d_eq, c->d_evc, c->d_ena_cnt, c->ena_tot, c->d_live);
@@ -4173,7 +4249,7 @@ extern "C" int kaos_finalize(kaos_ctx* c) {
   void* b3[] = {c->d_v1GH, c->d_v1GC, c->d_dv1, c->d_v2GH, c->d_v2GC, c->d_dv2, c->d_d3v3, c->d_M3, c->d_hist3,
                 c->d_grp3};
   for (void* p : b3) if (p) (void)hipFree(p);
-  void* bcx[] = {c->d_nsneut, c->d_cxgeo, c->d_cx, c->d_cx_need, c->d_cx_pref, c->d_cx_thr, c->d_cx_hist,
+  void* bcx[] = {c->d_nsneut, c->d_cxgeo, c->d_cx, c->d_cx_need, c->d_cx_pref, c->d_cx_thr, c->d_cx_hist, c->d_cx_lev,
                  c->d_ena, c->d_eid, c->d_ena_cnt, c->d_eoff, c->d_eq, c->d_nq_ena, c->d_ena_esc,
                  c->d_ena_vp, c->d_ena_vq, c->d_ena_vf, c->d_ena_hh, c->d_ena_d33v, c->d_ena_M, c->d_evc,
                  c->d_ena_hist};

@@ -229,32 +229,52 @@ __device__ __forceinline__ double kx_dec_v(int w, uint32_t lo16, int ip, bool vf
   return ref + kx_dec(w, lo16, vk[ip - 1], false);
 }
 
+// Binary32 constants of a tube's heating, set once per tube in LDS (the
+// kick reads them instead of forming them from the binary64 parameters per
+// ion): qs/ms, 1/OmegaG0, 0.5 (qs/ms)^2 EtaLH S0, XiPerp1/2, -ChiPerp1/2,
+// lambdaPerp, the heated arc length and h.
+struct kf_icr_c {
+  float qm, iog0, c, xi1, xi2, nchi1, nchi2, lam, sumds, h;
+};
+__host__ __device__ inline kf_icr_c kf_icr_consts(const kaos_icr_t& P) {
+  kf_icr_c k;
+  const float qm = (float)(P.qs / P.ms);
+  k.qm = qm;
+  k.iog0 = (float)(1.0 / P.omega_g0);
+  k.c = 0.5f * (qm * qm) * (float)(P.eta_lh * P.s0);
+  k.xi1 = (float)P.xi1; k.xi2 = (float)P.xi2;
+  k.nchi1 = -(float)P.chi1; k.nchi2 = -(float)P.chi2;
+  k.lam = (float)P.lambda_perp;
+  k.sumds = (float)P.sum_ds;
+  k.h = (float)P.h;
+  return k;
+}
+
 // Stochastic ICR kick of one ion in binary32 (the mixed form's heating; the
 // arithmetic of kaos_icr_kick with hardware log2/exp2/sqrt/rcp and the
 // sine/cosine of revolutions).  u0, u1 in (0, 1]: one Box-Muller pair.
 // Products are ordered so that no intermediate leaves the binary32 range
 // (qs/ms first: qs^2 and ms^2 alone underflow).
-__device__ __forceinline__ void kf_icr_kick(const kaos_icr_t& P, float r, float w, float vpar, float v1, float v2,
+__device__ __forceinline__ void kf_icr_kick(const kf_icr_c& P, float r, float w, float vpar, float v1, float v2,
                                             float u0, float u1, float& v1N, float& v2N) {
   KAOS_MIXED_CONTRACT
   const float R = kf_sqrt(-2.0f * 0.69314718f * __builtin_amdgcn_logf(u0));
   const float g1 = R * __builtin_amdgcn_cosf(u1), g2 = R * __builtin_amdgcn_sinf(u1);
-  const float qm = (float)(P.qs / P.ms);
+  const float qm = P.qm;
   const float B = (float)KAOS_MDIP * (kf_sqrt(4.0f - 3.0f * w) * kf_rcp((r * r) * r));
   const float og = qm * B;
   const float vperp = kf_sqrt(v1 * v1 + v2 * v2);
   float isig3 = 1.0f;
-  if (P.lambda_perp != 0.0 && ((float)KAOS_TWOPI * vperp) * kf_rcp(og) >= (float)P.lambda_perp) {
-    const float sig = ((float)KAOS_TWOPI * vperp) * kf_rcp((float)P.lambda_perp * og);
+  if (P.lam != 0.0f && ((float)KAOS_TWOPI * vperp) * kf_rcp(og) >= P.lam) {
+    const float sig = ((float)KAOS_TWOPI * vperp) * kf_rcp(P.lam * og);
     isig3 = kf_rcp((sig * sig) * sig);
   }
-  const float lx = __builtin_amdgcn_logf(og * (float)(1.0 / P.omega_g0));
-  const float c = 0.5f * (qm * qm) * (float)(P.eta_lh * P.s0);
-  const float D1 = (c * (float)P.xi1) * __builtin_amdgcn_exp2f(-(float)P.chi1 * lx) * isig3;
-  const float D2 = (c * (float)P.xi2) * __builtin_amdgcn_exp2f(-(float)P.chi2 * lx) * isig3;
-  const float eps = fabsf(vpar) * kf_rcp(og * (float)P.sum_ds);
+  const float lx = __builtin_amdgcn_logf(og * P.iog0);
+  const float D1 = (P.c * P.xi1) * __builtin_amdgcn_exp2f(P.nchi1 * lx) * isig3;
+  const float D2 = (P.c * P.xi2) * __builtin_amdgcn_exp2f(P.nchi2 * lx) * isig3;
+  const float eps = fabsf(vpar) * kf_rcp(og * P.sumds);
   const float tau = (float)KAOS_TWOPI * kf_rcp(og * kf_sqrt(eps));
-  const float h = (float)P.h;
+  const float h = P.h;
   float dv1 = 0.0f, dv2 = 0.0f;
   if (h <= tau) {
     const float f = h * kf_rcp(tau);
@@ -292,6 +312,7 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
   extern __shared__ unsigned int s_hist[];
   __shared__ unsigned int s_pushed;
   __shared__ int s_wq0, s_wnq, s_wtube, s_group;
+  __shared__ kf_icr_c s_icc;                     // heating constants of the current tube (ICR instantiation)
   __shared__ int4 s_wbox;
   const int tid = threadIdx.x;
   int cur = -1;
@@ -312,6 +333,8 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
   double ip = 0.0, ip2 = 0.0, hd = 0.0;
   uint32_t tbox = 0u;
   int icr = 0;                         // ICR heating in this tube (ICR instantiation only)
+  int tcx = 0;                         // charge exchange in this tube (cos(thetak4) kept)
+  int woff = 0;                        // the tube's Vperp edges in a.vperpGH
   for (int pass = 0;; ++pass) {
   const int g = kaos_next_group(a, pass, &s_group);
   if (g < 0) break;
@@ -366,7 +389,12 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
       qkfT = a.qkf + toff; aefT = a.aef + toff; agfT = a.agf + toff;
       vscale = (float)((double)nvpar / (a.vparGH[T.voff + nvpar - 1] - a.vparGH[T.voff + nvpar]));
       tbox = kaos_pack_box(a.vbox[tb]);
-      if (ICR) icr = T.icr;
+      if (ICR) {
+        icr = T.icr;
+        tcx = T.cx;
+        woff = (int)T.woff;
+        if (tid == 0 && icr) s_icc = kf_icr_consts(T.icrp);
+      }
       cur = tb;
       __syncthreads();
     }
@@ -470,7 +498,7 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
       kf_bhat_pt(rho + dRC, zC, b4R, b4Z);
       // charge exchange (heating/charge-exchange instantiation): cos(thetak4),
       // the frame of a conversion at the next step
-      if (ICR && a.ck4f && a.tubes[cur].cx) {
+      if (ICR && tcx) {
         const float rC = rho + dRC;
         kx_at(a.ck4f + toff, b4) = zC * kf_rsq(rC * rC + zC * zC);
       }
@@ -561,7 +589,6 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
       // the next push's binning cell (off the grid the previous one stays)
       int iqw = iq;
       if (ICR && icr && qc != 0 && qc != -1) {
-        const DevTube& TI = a.tubes[cur];
         const float v1 = kx_at(a.vrf + toff, b4), v2 = kx_at(a.vrf2 + toff, b4);
         uint32_t key[2] = { (uint32_t)a.seed, (uint32_t)(a.seed >> 32) ^ 0x49435248u };
         const long long idj = kx_at(a.id + toff, 2u * b4);
@@ -572,11 +599,11 @@ __global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_
         const float u0 = (float)(((double)o4[0] + 1.0) * (1.0 / 4294967296.0));
         const float u1 = (float)(((double)o4[1] + 1.0) * (1.0 / 4294967296.0));
         float v1N, v2N;
-        kf_icr_kick(TI.icrp, (float)r, (float)w, vs, v1, v2, u0, u1, v1N, v2N);
+        kf_icr_kick(s_icc, (float)r, (float)w, vs, v1, v2, u0, u1, v1N, v2N);
         kx_at(a.vrf + toff, b4) = v1N;
         kx_at(a.vrf2 + toff, b4) = v2N;
-        const double* GH = a.vperpGH + TI.woff;
-        const int iqn = kaos_find_vcell((double)kf_sqrt(v1N * v1N + v2N * v2N), GH, GH[nvperp], nvperp);
+        const double* GH = a.vperpGH + woff;
+        const int iqn = kaos_find_vcell_from((double)kf_sqrt(v1N * v1N + v2N * v2N), GH, GH[nvperp], nvperp, iq);
         if (iqn >= 1) iqw = iqn;
       }
       kx_at(cellT, b4) = ((uint32_t)(qc + 1) << 16) | ((uint32_t)iqw << 8) | (uint32_t)(ipn >= 1 ? ipn : ipc) |

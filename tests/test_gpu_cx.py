@@ -12,7 +12,8 @@ and fluxes:
   pushes and the seeded injection stream;
 - a long time step (2 s, 30 steps) in which ENAs fall out of the lower
   boundary, with a reorder every master step (cos(thetak4) moves with the
-  ions).
+  ions);
+- 40000 ions, whose ids need two digits of the device's id select.
 Mixed form: the same tube; its moments differ from FP64's by ~1e-7, so a
 Poisson count may differ by one where the running product sits at its
 threshold: totals of conversions within 2% of the oracle's, ENA counts per
@@ -33,12 +34,12 @@ ENA = ("x", "y", "z", "Vx", "Vy", "Vz", "id", "qind")
 MASTER = ("nq_counts", "NqRT", "M0", "M1par", "M2", "M2par", "EA", "EG", "esc")
 
 
-def _tube(h=0.05, filt=0, nq=24, ratio=1.1):
-    t = grids.with_vgrid3(grids.dipole_tube(nq=nq, nvpar=20, nvperp=16, n_total=12000, ratio=ratio, h=h), 10, 8,
+def _tube(h=0.05, filt=0, nq=24, ratio=1.1, n_total=12000):
+    t = grids.with_vgrid3(grids.dipole_tube(nq=nq, nvpar=20, nvperp=16, n_total=n_total, ratio=ratio, h=h), 10, 8,
                           vperp_max=6.0e3)
     if filt:
         t.filter_pt_m1par = 9
-    return grids.with_charge_exchange(t, ena_cap=20000, sigma_scale=1e-7, filt=filt, filt_pt=7)
+    return grids.with_charge_exchange(t, ena_cap=max(20000, n_total), sigma_scale=1e-7, filt=filt, filt_pt=7)
 
 
 def _pair(t, precision, ndatfac, nsteps, reorder=0):
@@ -88,6 +89,25 @@ def test_cx_fp64_bitexact(filt):
         _check_step(o, ks, n, ndatfac)
     assert len(ks.enas(0)["id"]) > 300
     assert ks.cx(0)["poi"].max() > 0
+    ks.close()
+    o.close()
+
+
+def test_cx_fp64_two_digit_id_select():
+    """40000 ions: ids past 2^14, so the device's id select takes two 14-bit
+    digits per cell (one below that); the converted ids are still the first
+    nint(poi) of every cell."""
+    t = _tube(n_total=40000)
+    ndatfac, nsteps = 2, 8
+    print(f"charge exchange fp64: 40000 ions, {nsteps} steps", flush=True)
+    o, ks = _pair(t, KAOS_FP64, ndatfac, nsteps)
+    for n in range(1, nsteps + 1):
+        o.step(n)
+        out = ks.step(n)
+        assert out.status == 0 and out.n_alive == o.count(), n
+        _check_step(o, ks, n, ndatfac)
+    e = ks.enas(0)["id"]
+    assert len(e) > 300 and e.max() >= 2 ** 14
     ks.close()
     o.close()
 
