@@ -55,7 +55,7 @@ BYTES_PER_ION = {"fp64": 28 + 44, "mixed": 20 + 28}
 # ratio of a plain 1 GiB device copy's true bytes to its counter reading
 # (fetch x2.000, write x1.000), from
 # profiles/k_push_1e8_ions_256x64x64_hbm_traffic.txt.
-TRAFFIC_BYTES_PER_ION = {"fp64": 73.2, "mixed": 41.2}
+TRAFFIC_BYTES_PER_ION = {"fp64": 73.3, "mixed": 48.8}
 HBM_PEAK_GBS = 8000.0
 DROP_IN_STEPS = 10          # kaos_step calls with a summary each, after the timed steps
 MOMENTS = ("M0", "M1perp", "M1par", "M2", "M2perp", "M2par", "EA")
