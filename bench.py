@@ -149,7 +149,7 @@ def cpu_reference_push(n_ions=2_000_000):
                        f"{t:.2f} s; excludes binning and moments")
 
 
-TUNE_STEPS_MAX = 64        # the autotuner needs 2 + 3 x 8 master steps and one more to choose
+TUNE_STEPS_MAX = 64        # the autotuner needs 2 + 3 x 16 master steps and one more to choose
 
 
 def finish_tuning(ks, n_next):
@@ -167,7 +167,7 @@ def finish_tuning(ks, n_next):
 
 def launch_dict(ks):
     t, w, p, tuning = ks.get_launch()
-    return {"tiles_per_group": t, "hist_words": w, "persistent": p, "tuning": tuning}
+    return {"tiles_per_group": t, "hist_words": w, "persistent": p, "block": ks.get_launch_block(), "tuning": tuning}
 
 
 def timed_run(form, tube, q0, v0, w0, cap, args):

@@ -549,6 +549,14 @@ int kaos_set_launch(kaos_ctx* ctx, int32_t tiles_per_group, int32_t hist_words, 
 /* The launch shape in use and whether the autotuner is still timing candidates. */
 int kaos_get_launch(kaos_ctx* ctx, int32_t* tiles_per_group, int32_t* hist_words, int32_t* persistent,
                     int32_t* tuning);
+/* Threads per block of the push (256: four ions of a 1024-ion tile per
+ * thread, 512: two), a dimension the autotuner times along with the shape
+ * above; only the plain push (config cells <= 256, no charge exchange, and in
+ * the mixed form no heating) has the 512-thread variant, the others run 256.
+ * Setting it ends the autotuner.  Results do not depend on it.  No
+ * counterpart in the reference. */
+int kaos_set_launch_block(kaos_ctx* ctx, int32_t block);
+int kaos_get_launch_block(kaos_ctx* ctx, int32_t* block);
 
 /* Kernel timing: when enabled, hipEvents on the library's stream bracket every
  * push launch and every kaos_run call.  kaos_profile_read returns the summed

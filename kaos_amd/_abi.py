@@ -93,7 +93,8 @@ EXPORTS = ("kaos_init", "kaos_set_grid", "kaos_upload_particles", "kaos_upload_v
            "kaos_download_ena_moments",
            "kaos_download_particles", "kaos_download_vperp", "kaos_upload_state",
            "kaos_upload_moments", "kaos_resume", "kaos_save_state", "kaos_load_state", "kaos_set_launch",
-           "kaos_get_launch", "kaos_profile", "kaos_profile_read", "kaos_last_error", "kaos_finalize")
+           "kaos_get_launch", "kaos_set_launch_block", "kaos_get_launch_block", "kaos_profile",
+           "kaos_profile_read", "kaos_last_error", "kaos_finalize")
 
 _lib = None
 
@@ -174,6 +175,8 @@ def load(path=LIB_PATH):
     lib.kaos_load_state.argtypes = [P, C.c_char_p]
     lib.kaos_set_launch.argtypes = [P, C.c_int32, C.c_int32, C.c_int32]
     lib.kaos_get_launch.argtypes = [P, i32p, i32p, i32p, i32p]
+    lib.kaos_set_launch_block.argtypes = [P, C.c_int32]
+    lib.kaos_get_launch_block.argtypes = [P, i32p]
     lib.kaos_profile.argtypes = [P, C.c_int32]
     lib.kaos_profile_read.argtypes = [P, _dp, i64p, _dp]
     lib.kaos_last_error.argtypes = [P]

@@ -41,6 +41,7 @@
 // the same wavefronts per SIMD)
 #define KAOS_MIXED_BLOCK 256
 #define KAOS_FP64_BLOCK 256
+#define KAOS_PUSH_BLOCK_DEFAULT 256          // push threads per block: 256 (4 ions each) or 512 (2 each)
 #define KAOS_MAXQ 512
 #define KAOS_MAXV 255
 #ifndef KAOS_HWORDS
@@ -284,9 +285,11 @@ __device__ __forceinline__ int kd_find_qcell_fast(double q, const double* qGH, d
 // CX: the instantiation launched when a tube has charge exchange; it stores
 // cos(thetak4) of every ion of such a tube (the frame of a conversion at the
 // next step), so the plain push carries no extra store.
-template <int MAXQ, bool CX>
-__global__ void __launch_bounds__(KAOS_FP64_BLOCK, KAOS_FP64_WAVES_EU) k_push_fp64(StepArgs a) {
-  constexpr int BLK = KAOS_FP64_BLOCK, PPT = KAOS_TILE / KAOS_FP64_BLOCK;
+// BLOCK: threads per block; each thread pushes KAOS_TILE / BLOCK ions of a
+// tile (the autotuner times 256 and 512 for the plain instantiation).
+template <int MAXQ, bool CX, int BLOCK = KAOS_FP64_BLOCK>
+__global__ void __launch_bounds__(BLOCK, KAOS_FP64_WAVES_EU) k_push_fp64(StepArgs a) {
+  constexpr int BLK = BLOCK, PPT = KAOS_TILE / BLOCK;
   __shared__ double s_qGH[MAXQ], s_vGH[KAOS_MAXV + 1];
   __shared__ KaosCE64 s_cE[MAXQ];
   extern __shared__ unsigned int s_hist[];
@@ -1933,8 +1936,8 @@ struct kaos_ctx {
   // kernel timing (hipEvents on the library stream around each push launch)
   bool prof_on = false;
   // launch shape of the push kernels, the online autotuner and captured steps
-  struct Launch { int tpb, hwords, persistent; };
-  Launch launch = {KAOS_TPB, KAOS_HWORDS, 0};
+  struct Launch { int tpb, hwords, persistent, block; };
+  Launch launch = {KAOS_TPB, KAOS_HWORDS, 0, KAOS_PUSH_BLOCK_DEFAULT};
   unsigned int* d_work = nullptr;
   long long* d_live = nullptr;                  // live ions per tube (kaos_count)
   int any_icr = 0;                              // some tube has ICR heating on
@@ -2056,7 +2059,8 @@ extern "C" int kaos_init(const kaos_cfg* cfg, kaos_ctx** out) {
     c->tuning = true;
     for (int pers = 0; pers < 2; ++pers)
       for (int hw : {4096, 6144})
-        for (int tpb : {8, 16}) c->tune_cand.push_back({tpb, hw, pers});
+        for (int tpb : {8, 16})
+          for (int blk : {256, 512}) c->tune_cand.push_back({tpb, hw, pers, blk});
   }
   c->ht.resize(cfg->ntubes);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -2180,6 +2184,8 @@ extern "C" int kaos_set_grid(kaos_ctx* c, int32_t tube, const kaos_grid* g, int6
 }
 
 // Allocate the device arrays once every tube's grid is known.
+static bool plain_push(const kaos_ctx* c);
+
 static int build_device(kaos_ctx* c) {
   int nt = c->cfg.ntubes;
   for (int t = 0; t < nt; ++t) if (!c->ht[t].set) return fail(c, KAOS_E_STATE, "tube " + std::to_string(t) + " has no grid");
@@ -2527,6 +2533,12 @@ static int build_device(kaos_ctx* c) {
   HIPCHK(c, up((void**)&c->d_inj_cnt, ic.data(), sizeof(long long) * nt));
   HIPCHK(c, up((void**)&c->d_uni_base, ib.data(), sizeof(long long) * (nt + 1)));
   c->built = true;
+  if (!plain_push(c)) {                         // only the plain push has a 512-thread variant
+    std::vector<kaos_ctx::Launch> keep;
+    for (const auto& l : c->tune_cand) if (l.block == 256) keep.push_back(l);
+    c->tune_cand = keep;
+    c->launch.block = 256;
+  }
   return KAOS_OK;
 }
 
@@ -2953,55 +2965,63 @@ static void launch_box_update(kaos_ctx* c) {
                      c->d_vmarg_perp, c->d_vbox);
 }
 
+// Whether the context's push runs the plain instantiation (256-cell LDS
+// tables, no charge exchange, and in the mixed form no heating), the one with
+// a 512-thread variant.
+static bool plain_push(const kaos_ctx* c) {
+  return c->lds_nq <= 256 && !c->any_cx && (c->cfg.precision != KAOS_MIXED || !c->any_icr);
+}
+
 static void launch_push(kaos_ctx* c, const StepArgs& a, hipEvent_t tune0 = nullptr, hipEvent_t tune1 = nullptr) {
   int nblocks = a.ngroups;
+  const bool wide = c->launch.block == 512 && plain_push(c);
+  const int bthreads = wide ? 512 : 256;
   if (a.persistent) {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, c->d_work, 1LL);
     // as many blocks as are resident at the kernel's wavefronts per SIMD (4 SIMDs per CU)
-    const int per_cu = c->cfg.precision == KAOS_MIXED ? KAOS_MIXED_WAVES_EU * 256 / KAOS_MIXED_BLOCK
-                                                      : KAOS_FP64_WAVES_EU * 256 / KAOS_FP64_BLOCK;
+    const int per_cu = (c->cfg.precision == KAOS_MIXED ? KAOS_MIXED_WAVES_EU : KAOS_FP64_WAVES_EU) * 256 / bthreads;
     nblocks = std::min(nblocks, per_cu * c->n_cu);
   }
   std::pair<hipEvent_t, hipEvent_t> ev;
   const bool prof = c->prof_on && !c->in_noalloc;
   if (prof) ev = ev_pair(c);
   if (tune0) (void)hipEventRecord(tune0, c->stream);
+  const size_t lds = push_lds_bytes(c);
   if (c->cfg.precision == KAOS_MIXED) {
     // the heating instantiation only when a tube heats (its extra registers
     // never cost the plain push)
-    if (c->lds_nq <= 256 && !c->any_icr && !c->any_cx)
-      hipLaunchKernelGGL((k_push_mixed<256, false>), dim3(nblocks), dim3(KAOS_MIXED_BLOCK), push_lds_bytes(c),
-                         c->stream, a);
+    if (wide)
+      hipLaunchKernelGGL((k_push_mixed<256, false, 512>), dim3(nblocks), dim3(512), lds, c->stream, a);
+    else if (c->lds_nq <= 256 && !c->any_icr && !c->any_cx)
+      hipLaunchKernelGGL((k_push_mixed<256, false>), dim3(nblocks), dim3(KAOS_MIXED_BLOCK), lds, c->stream, a);
     else if (c->lds_nq <= 256)
-      hipLaunchKernelGGL((k_push_mixed<256, true>), dim3(nblocks), dim3(KAOS_MIXED_BLOCK), push_lds_bytes(c),
-                         c->stream, a);
+      hipLaunchKernelGGL((k_push_mixed<256, true>), dim3(nblocks), dim3(KAOS_MIXED_BLOCK), lds, c->stream, a);
     else if (!c->any_icr && !c->any_cx)
-      hipLaunchKernelGGL((k_push_mixed<KAOS_MAXQ, false>), dim3(nblocks), dim3(KAOS_MIXED_BLOCK), push_lds_bytes(c),
-                         c->stream, a);
+      hipLaunchKernelGGL((k_push_mixed<KAOS_MAXQ, false>), dim3(nblocks), dim3(KAOS_MIXED_BLOCK), lds, c->stream, a);
     else
-      hipLaunchKernelGGL((k_push_mixed<KAOS_MAXQ, true>), dim3(nblocks), dim3(KAOS_MIXED_BLOCK), push_lds_bytes(c),
-                         c->stream, a);
+      hipLaunchKernelGGL((k_push_mixed<KAOS_MAXQ, true>), dim3(nblocks), dim3(KAOS_MIXED_BLOCK), lds, c->stream, a);
   }
+  else if (wide)
+    hipLaunchKernelGGL((k_push_fp64<256, false, 512>), dim3(nblocks), dim3(512), lds, c->stream, a);
   else if (c->lds_nq <= 256 && !c->any_cx)
-    hipLaunchKernelGGL((k_push_fp64<256, false>), dim3(nblocks), dim3(KAOS_FP64_BLOCK), push_lds_bytes(c), c->stream, a);
+    hipLaunchKernelGGL((k_push_fp64<256, false>), dim3(nblocks), dim3(KAOS_FP64_BLOCK), lds, c->stream, a);
   else if (c->lds_nq <= 256)
-    hipLaunchKernelGGL((k_push_fp64<256, true>), dim3(nblocks), dim3(KAOS_FP64_BLOCK), push_lds_bytes(c), c->stream, a);
+    hipLaunchKernelGGL((k_push_fp64<256, true>), dim3(nblocks), dim3(KAOS_FP64_BLOCK), lds, c->stream, a);
   else if (!c->any_cx)
-    hipLaunchKernelGGL((k_push_fp64<KAOS_MAXQ, false>), dim3(nblocks), dim3(KAOS_FP64_BLOCK), push_lds_bytes(c),
-                       c->stream, a);
+    hipLaunchKernelGGL((k_push_fp64<KAOS_MAXQ, false>), dim3(nblocks), dim3(KAOS_FP64_BLOCK), lds, c->stream, a);
   else
-    hipLaunchKernelGGL((k_push_fp64<KAOS_MAXQ, true>), dim3(nblocks), dim3(KAOS_FP64_BLOCK), push_lds_bytes(c),
-                       c->stream, a);
+    hipLaunchKernelGGL((k_push_fp64<KAOS_MAXQ, true>), dim3(nblocks), dim3(KAOS_FP64_BLOCK), lds, c->stream, a);
   if (tune1) (void)hipEventRecord(tune1, c->stream);
   if (prof) { (void)hipEventRecord(ev.second, c->stream); c->ev_push.push_back(ev); }
 }
 
 // Online autotuner: after two warm-up master steps, the next master steps
 // each run the push with one candidate launch shape (tiles per group,
-// histogram window, grid or persistent), every candidate twice in two passes;
-// the next master step reads the timings and keeps the candidate with the
-// smallest of its two times (two passes, so that a trend over the first steps
-// does not favour the candidates timed last).  Results do not depend on the
+// histogram window, grid or persistent, 256 or 512 threads per block), every
+// candidate once in each of KAOS_TUNE_SAMPLES passes; the next master step
+// reads the timings and keeps the candidate with the smallest of its times
+// (several passes, so that a trend over the first steps does not favour the
+// candidates timed last).  Results do not depend on the
 // launch shape (integer histograms, independent ions), so tuning steps are
 // ordinary steps.
 static int tune_before_push(kaos_ctx* c, hipEvent_t* e0, hipEvent_t* e1) {
@@ -3216,7 +3236,8 @@ static int enqueue_step(kaos_ctx* c, long long n, const double* d_uni) {
 static int enqueue_step_graph(kaos_ctx* c, long long n) {
   unsigned long long key = (unsigned long long)is_master(c, n) | ((unsigned long long)is_master(c, n + 1) << 1) |
                            ((unsigned long long)c->cur << 2) | ((unsigned long long)c->launch.persistent << 3) |
-                           ((unsigned long long)c->launch.tpb << 4) | ((unsigned long long)c->launch.hwords << 20);
+                           ((unsigned long long)c->launch.tpb << 4) | ((unsigned long long)c->launch.hwords << 20) |
+                           ((unsigned long long)(c->launch.block == 512) << 40);
   auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
     hipGraph_t g = nullptr;
@@ -4181,7 +4202,7 @@ extern "C" int kaos_set_launch(kaos_ctx* c, int32_t tiles_per_group, int32_t his
   if (tiles_per_group < 1 || tiles_per_group * KAOS_TILE > 65535)
     return fail(c, KAOS_E_ARG, "tiles_per_group must be in [1, 63] (16-bit LDS counters)");
   if (hist_words < 0 || hist_words > 12288) return fail(c, KAOS_E_ARG, "hist_words must be in [0, 12288]");
-  c->launch = {tiles_per_group, hist_words, persistent ? 1 : 0};
+  c->launch = {tiles_per_group, hist_words, persistent ? 1 : 0, c->launch.block};
   c->tuning = false;
   for (auto& ev : c->tune_ev) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
   c->tune_ev.clear();
@@ -4195,6 +4216,22 @@ extern "C" int kaos_get_launch(kaos_ctx* c, int32_t* tiles_per_group, int32_t* h
   if (hist_words) *hist_words = c->launch.hwords;
   if (persistent) *persistent = c->launch.persistent;
   if (tuning) *tuning = c->tuning ? 1 : 0;
+  return KAOS_OK;
+}
+
+extern "C" int kaos_set_launch_block(kaos_ctx* c, int32_t block) {
+  if (!c) return KAOS_E_ARG;
+  if (block != 256 && block != 512) return fail(c, KAOS_E_ARG, "push block must be 256 or 512 threads");
+  c->launch.block = block;
+  c->tuning = false;
+  for (auto& ev : c->tune_ev) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
+  c->tune_ev.clear();
+  return KAOS_OK;
+}
+
+extern "C" int kaos_get_launch_block(kaos_ctx* c, int32_t* block) {
+  if (!c || !block) return KAOS_E_ARG;
+  *block = (c->launch.block == 512 && (!c->built || plain_push(c))) ? 512 : 256;
   return KAOS_OK;
 }
 

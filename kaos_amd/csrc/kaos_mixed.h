@@ -300,10 +300,12 @@ __device__ __forceinline__ void kf_icr_kick(const kf_icr_c& P, float r, float w,
 #endif
 // ICR: the instantiation launched when a tube heats or has charge exchange
 // (the plain one carries neither's registers).
-template <int MAXQ, bool ICR>
-__global__ void __launch_bounds__(KAOS_MIXED_BLOCK, KAOS_MIXED_WAVES_EU) k_push_mixed(StepArgs a) {
+// BLOCK: threads per block; each thread pushes KAOS_TILE / BLOCK ions of a
+// tile (the autotuner times 256 and 512 for the plain instantiation).
+template <int MAXQ, bool ICR, int BLOCK = KAOS_MIXED_BLOCK>
+__global__ void __launch_bounds__(BLOCK, KAOS_MIXED_WAVES_EU) k_push_mixed(StepArgs a) {
   KAOS_MIXED_CONTRACT
-  constexpr int BLK = KAOS_MIXED_BLOCK, PPT = KAOS_TILE / KAOS_MIXED_BLOCK;
+  constexpr int BLK = BLOCK, PPT = KAOS_TILE / BLOCK;
   __shared__ double s_rGH[MAXQ], s_rref[MAXQ], s_vGH[KAOS_MAXV + 1];
   __shared__ float4 s_cE[MAXQ];
   __shared__ float4 s_pe[MAXQ + 1];

@@ -127,7 +127,7 @@ module kaos_hip
             kaos_download_ena_master, kaos_download_ena_moments, &
             kaos_count, kaos_download_particles, kaos_download_vperp, kaos_upload_state, &
             kaos_upload_moments, kaos_resume, kaos_save_state, kaos_load_state, kaos_set_launch, &
-            kaos_get_launch, kaos_profile, &
+            kaos_get_launch, kaos_set_launch_block, kaos_get_launch_block, kaos_profile, &
             kaos_profile_read, kaos_last_error, kaos_finalize, kaos_error_message
 
   interface
@@ -378,6 +378,18 @@ module kaos_hip
       type(c_ptr), value :: ctx
       integer(c_int32_t), intent(out) :: tiles_per_group, hist_words, persistent, tuning
     end function kaos_get_launch
+
+    integer(c_int) function kaos_set_launch_block(ctx, block) bind(c, name='kaos_set_launch_block')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: block
+    end function kaos_set_launch_block
+
+    integer(c_int) function kaos_get_launch_block(ctx, block) bind(c, name='kaos_get_launch_block')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: block
+    end function kaos_get_launch_block
 
     integer(c_int) function kaos_profile(ctx, enable) bind(c, name='kaos_profile')
       import :: c_int, c_ptr, c_int32_t

@@ -141,6 +141,16 @@ class KineticSolver:
         self._chk(self.lib.kaos_get_launch(self.h, *(C.byref(x) for x in v)))
         return tuple(x.value for x in v)
 
+    def set_launch_block(self, block):
+        """Threads per block of the push (256 or 512; ends the autotuner)."""
+        self._chk(self.lib.kaos_set_launch_block(self.h, int(block)))
+
+    def get_launch_block(self):
+        """Threads per block the push runs with (256 or 512)."""
+        v = C.c_int32(0)
+        self._chk(self.lib.kaos_get_launch_block(self.h, C.byref(v)))
+        return v.value
+
     def profile(self, enable=True):
         self._chk(self.lib.kaos_profile(self.h, int(enable)))
 

@@ -2,7 +2,9 @@
 bit-exact against the oracle (a) for forced launch shapes, grid and
 persistent, with and without the LDS histogram window, (b) when kaos_run
 replays a captured hipGraph per step, checked after every replay with eager
-downloads in between, and (c) across the online autotuner's candidate steps."""
+downloads in between, and (c) across the online autotuner's candidate steps
+(tiles per group, LDS window, grid or persistent launch, 256 or 512 threads
+per block); the mixed push gives the same results with 256 and 512 threads."""
 import numpy as np
 import pytest
 
@@ -18,7 +20,7 @@ PART_KEYS = ("x", "y", "z", "Vx", "Vy", "Vz", "qind", "vparind", "vperpind")
 
 def _pair(tubes, flags, ndatfac=1, seed=5, reorder_every=0):
     ions = [grids.initial_ions(t, seed=60 + i) for i, t in enumerate(tubes)]
-    caps = [len(q) + 40 * (t.n_inj_lb + t.n_inj_ub) + 512 for t, (q, _, _) in zip(tubes, ions)]
+    caps = [len(q) + 64 * (t.n_inj_lb + t.n_inj_ub) + 512 for t, (q, _, _) in zip(tubes, ions)]
     orcs = [oracle.Oracle(t, *ions[i], ndatfac=ndatfac, seed=seed, capacity=caps[i], tube_index=i)
             for i, t in enumerate(tubes)]
     ks = KineticSolver(tubes, precision=KAOS_FP64, ndatfac=ndatfac, seed=seed, capacity=caps, flags=flags,
@@ -39,12 +41,14 @@ def _check(orcs, ks, n, master=True):
                 assert np.array_equal(ma[k], mb[k]), f"step {n} tube {i}: {k}"
 
 
-@pytest.mark.parametrize("shape", [(1, 0, 0), (4, 2048, 1), (32, 6144, 1), (63, 12288, 0)])
-def test_forced_launch_shapes_bitexact(shape):
+@pytest.mark.parametrize("shape,block", [((1, 0, 0), 256), ((4, 2048, 1), 512), ((32, 6144, 1), 256),
+                                         ((63, 12288, 0), 512), ((8, 4096, 1), 512)])
+def test_forced_launch_shapes_bitexact(shape, block):
     tubes = [grids.config1(n_total=30000), grids.dipole_tube(nq=40, nvpar=24, nvperp=20, n_total=9000, ratio=1.08)]
     orcs, ks = _pair(tubes, KAOS_FLAG_NO_AUTOTUNE)
     ks.set_launch(*shape)
-    assert ks.get_launch()[:3] == shape
+    ks.set_launch_block(block)
+    assert ks.get_launch()[:3] == shape and ks.get_launch_block() == block
     for n in range(1, 7):
         for o in orcs:
             o.step(n)
@@ -78,7 +82,7 @@ def test_autotuned_run_bitexact_and_settles():
     ks.step(1)
     assert ks.get_launch()[3] == 1          # still tuning
     n = 2
-    while n < 32:                           # 2 warm-up + 24 candidate steps + the choice + graph steps
+    while n < 56:                           # 2 warm-up + 3 x 16 candidate steps + the choice + graph steps
         for o in orcs:
             o.step(n)
         ks.run(n, 1)
@@ -86,6 +90,7 @@ def test_autotuned_run_bitexact_and_settles():
         n += 1
     tpb, hw, pers, tuning = ks.get_launch()
     assert tuning == 0 and tpb in (8, 16) and hw in (4096, 6144) and pers in (0, 1)
+    assert ks.get_launch_block() in (256, 512)
     ks.close()
 
 
@@ -111,3 +116,25 @@ def test_mixed_graph_and_tuning_match_eager_untuned():
         assert np.array_equal(pa[k], pb[k]), k
     a.close()
     b.close()
+
+
+def test_mixed_push_block_sizes_agree():
+    """The mixed push with 512 threads per block (two ions per thread) gives
+    the same state and outputs as with 256 (four per thread)."""
+    t = grids.config1(n_total=40000)
+    q, v, w = grids.initial_ions(t, seed=9)
+    cap = len(q) + 20 * (t.n_inj_lb + t.n_inj_ub) + 512
+    runs = []
+    for block in (256, 512):
+        ks = KineticSolver([t], precision=KAOS_MIXED, ndatfac=2, seed=3, capacity=[cap], flags=KAOS_FLAG_NO_AUTOTUNE)
+        ks.upload(0, q, v, w)
+        ks.set_launch_block(block)
+        for n in range(1, 11):
+            assert ks.step(n).status == 0
+        runs.append((ks.particles(0), ks.master(0)))
+        ks.close()
+    (pa, ma), (pb, mb) = runs
+    for k in PART_KEYS:
+        assert np.array_equal(pa[k], pb[k]), k
+    for k in MASTER_KEYS:
+        assert np.array_equal(ma[k], mb[k]), k
