@@ -940,7 +940,7 @@ __global__ void k_field(const DevTube* tubes, const double* hq, const double* dq
 // to slot off + count + k.  Uniforms from the caller's stream (6 per ion) or
 // from the seeded Philox stream.
 __global__ void k_inject(StepArgs a, const long long* inj_base, const long long* next_id, int ntubes,
-                         const double* EA, const double* EG, const double* vperpGH, const double* uni,
+                         const double* vperpGH, const double* uni,
                          const long long* uni_base, unsigned long long seed, const long long* stepno,
                          long long total, int mixed) {
   long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -964,8 +964,13 @@ __global__ void k_inject(StepArgs a, const long long* inj_base, const long long*
   kaos_mb_project(u, lb ? T.sig_lb : T.sig_ub, T.cphi, T.sphi, L, &vpar, &v1, &v2, &vperp);
   int ip = kaos_find_vcell(vpar, a.vparGH + T.voff, a.vparGH[T.voff + T.nvpar], T.nvpar);
   int iq = kaos_find_vcell(vperp, vperpGH + T.woff, vperpGH[T.woff + T.nvperp], T.nvperp);
-  double aea = fabs(T.qsms * fabs(EA[T.goff + Qb]));
-  double agv = fabs(T.qsms * fabs(EG[T.goff + Qb]));
+  // a new slot's fields and stage-4 q are unset until its first push
+  // (DataTypeReAllocA copies only the old ions, _8_0_6_DataTypeReAllocA.f90:
+  // 259-313; the reference reads uninitialised memory there, 0 for most
+  // batches): both fields start at 0 and q = 0 lies in no cell, so the next
+  // master step's interpolation keeps them (_8_7_2_Gravfield.f90:305-420)
+  // and the first push after injection runs without either field
+  const double qk0 = 0.0;
   uint32_t flags = 0u;
   if (mixed) {
     int Q = kx_refq(qcell, T.nq);
@@ -977,12 +982,12 @@ __global__ void k_inject(StepArgs a, const long long* inj_base, const long long*
     a.vrf[j] = (float)v1;
     a.vrf2[j] = (float)v2;
     if (a.c12) a.c12[j] = kd_cells12(T, a, v1, v2);
-    a.qkf[j] = (float)(qc - kx_qref(a.qGC[T.goff + Q - 1]));
-    a.aef[j] = (float)aea; a.agf[j] = (float)agv;
+    a.qkf[j] = (float)(qk0 - kx_qref(a.qGC[T.goff + Q - 1]));
+    a.aef[j] = 0.0f; a.agf[j] = 0.0f;
     flags = (rflag ? KAOS_RFLAG : 0u) | (vflag ? KAOS_VFLAG : 0u);
   } else {
-    a.rs[j] = r; a.vp[j] = vpar; a.vr[j] = v1; a.vr2[j] = v2; a.qk[j] = qc;
-    a.ae[j] = aea; a.ag[j] = agv;
+    a.rs[j] = r; a.vp[j] = vpar; a.vr[j] = v1; a.vr2[j] = v2; a.qk[j] = qk0;
+    a.ae[j] = 0.0; a.ag[j] = 0.0;
   }
   a.cell[j] = ((uint32_t)(qcell + 1) << 16) | ((uint32_t)iq << 8) | (uint32_t)ip | flags;
   a.id[j] = next_id[t] + k;
@@ -3194,8 +3199,8 @@ static int enqueue_step_kernels(kaos_ctx* c, long long n, const double* d_uni) {
     if (master) {
       if (c->inj_total > 0) {
         int nb = (int)((c->inj_total + 255) / 256);
-        hipLaunchKernelGGL(k_inject, dim3(nb), dim3(256), 0, c->stream, a, c->d_inj_base, c->d_next_id, nt, c->d_EA,
-                           c->d_EG, c->d_vperpGH, d_uni, c->d_uni_base, (unsigned long long)c->cfg.seed,
+        hipLaunchKernelGGL(k_inject, dim3(nb), dim3(256), 0, c->stream, a, c->d_inj_base, c->d_next_id, nt,
+                           c->d_vperpGH, d_uni, c->d_uni_base, (unsigned long long)c->cfg.seed,
                            c->d_stepno, c->inj_total, mixed);
       }
       launch_moments(c, c->d_inj_cnt);

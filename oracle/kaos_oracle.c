@@ -440,7 +440,6 @@ static int inject(orc_t *o, int64_t n, const double *uni) {
   orc_cfg *c = &o->c;
   int64_t nl = c->lbcond ? c->n_inj_lb : 0, nu = c->ubcond ? c->n_inj_ub : 0;
   if (o->ns + nl + nu > o->cap) return 2;
-  double qm = c->qs / c->ms;
   for (int64_t k = 0; k < nl + nu; ++k) {
     int lb = (k < nl);
     int Qb = lb ? 0 : c->nq - 1;
@@ -464,9 +463,18 @@ static int inject(orc_t *o, int64_t n, const double *uni) {
     o->vparind[j] = scan_vcell(c->vparGL, c->vparGH, c->nvpar, vpar, 0);
     o->vperpind[j] = scan_vcell(c->vperpGL, c->vperpGH, c->nvperp, vperp, 0);
     if (c->v3d) vcells3(o, j, vpar, v1, v2);   /* indices at injection, KSB:339-476 */
-    o->aea[j] = o->aeaN[j] = fabs(qm * fabs(o->EA[Qb]));
-    o->ag[j] = o->agN[j] = fabs(qm * fabs(o->EG[Qb]));
-    o->qk4[j] = qc;
+    /* A new slot's fields and stage-4 q are never written before its first
+     * push: DataTypeReAllocA copies only the old ions (_8_0_6_DataTypeReAllocA.
+     * f90:259-313) and the injection sets neither.  The field scans at the next
+     * master step match no cell at an unset qk4 = 0 (_8_7_2_Gravfield.f90:305-
+     * 420, AmbipolarEfield likewise) and leave AGmagN/AEAmagN unset, so the
+     * first push after injection reads uninitialised memory for both fields.
+     * The compiled reference reads 0 there for most batches (stale values of
+     * other slots for some); 0 is taken here, with which the reference's own
+     * loop is reproduced over four master steps. */
+    o->aea[j] = o->aeaN[j] = 0.0;
+    o->ag[j] = o->agN[j] = 0.0;
+    o->qk4[j] = 0.0;
     o->ck4[j] = L.c;     /* a conversion before the first push: the frame of the injection point */
   }
   o->ns += nl + nu;

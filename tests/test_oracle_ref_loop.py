@@ -23,12 +23,12 @@ the reference generated and compared with what the reference computed:
   its eight 3-D moments of step 1 and the gravity field equal the reference's
   exported M0phRT .. M2ParphRT and EGmagRT within 1e-12.
 
-Master steps 2 and 3 are compared on oracle/_ref/kaos_ref_loop_eamb0 (the same
+Master steps 2 to 5 are compared on oracle/_ref/kaos_ref_loop_eamb0 (the same
 program with EAMBflag = 0): with the ambipolar field on, the reference's field
 of master step 1 reads row nn - 1 = 0 of its moment arrays (out of range,
 _8_7_AmbipolarEfield.f90:40-82), so its first pushes see an undefined field.
-Two properties of the compiled reference's sequence are measured there and
-fed to the oracle:
+Three properties of the compiled reference's sequence are measured there;
+the first two are fed to the oracle and the third is the oracle's own rule:
 
 * step 1 clamps q0 against the edge arrays read at that same out-of-range row
   (KSA:55-66; in the column-major arrays, the next cell's edges): every ion
@@ -39,9 +39,17 @@ fed to the oracle:
   at n = 1, 3 and 6, bit for bit, and not at every step: the non-master
   branch of KineticSolverBSub matches only for nn >= 2 (KSB:804-808), and the
   `exit nnloopKSnonstat` that ends the loop body (KSB:1033) leaves the loop
-  during its first pass, nn = 1 -- so the oracle is stepped at n = 1, 3, 6.
+  during its first pass, nn = 1 -- so the oracle is stepped at n = 1, 3, 6,
+  9, 12;
+* an injected ion's first push runs without gravity: its slot's AGmagN and
+  qk4 are never written before that push (DataTypeReAllocA copies only the
+  old ions, _8_0_6_DataTypeReAllocA.f90:259-313, and no cell of GravfieldSub's
+  scan holds an unset qk4, _8_7_2_Gravfield.f90:305-420), and the compiled
+  reference reads 0 there for the batches these master steps export (the
+  injection cell's field instead bins a few ions per 1e4 differently from
+  master step 4 on).
 
-With both, the 3-D phase-space counts of master steps 2 and 3 are equal, the
+With these, the 3-D phase-space counts of master steps 2 to 5 are equal, the
 eight moments agree within 1e-13 and the gravity field and the lower-boundary
 escape flux (ions removed at the next master step, BC:158-256) equal the
 reference's: the cadence, removal lag, injection at the boundary cell with the
@@ -151,7 +159,7 @@ def test_step1_moments_and_gravity_equal_the_references(ref):
     assert np.max(np.abs(m["EG"] - R["EG"])) <= 1e-12 * np.abs(R["EG"]).max()
 
 
-NT_SEQ = 6                # steps of the sequencing run (master steps at 3 and 6)
+NT_SEQ = 12               # steps of the sequencing run (master steps at 3, 6, 9, 12)
 
 
 @pytest.fixture(scope="module")
@@ -164,7 +172,7 @@ def ref_eamb0(tmp_path_factory):
     return dict(wd=wd, inp=rl.load_inputs(wd))
 
 
-def test_master_steps_2_and_3_equal_the_references(ref_eamb0):
+def test_master_steps_2_to_5_equal_the_references(ref_eamb0):
     inp = ref_eamb0["inp"]
     t, I = inp["tube"], inp["ions"]
     t.eamb = 0
@@ -172,12 +180,12 @@ def test_master_steps_2_and_3_equal_the_references(ref_eamb0):
     # step 1's q0: clamped at cell 2's lower edge (the row past the end of cell 1's)
     q0s = np.minimum(I["q0"], qGL[1]) if qGL[0] > 0 else np.maximum(I["q0"], qGL[1])
     assert np.sum(q0s != I["q0"]) > 0
-    cap = len(I["q0"]) + 4 * inp["n_inj_rank"] * inp["ranks"] + 1024
+    cap = len(I["q0"]) + 8 * inp["n_inj_rank"] * inp["ranks"] + 1024
     o = oracle.Oracle(t, I["q0"], I["vpar0"], I["vperp"], ndatfac=3, seed=1, capacity=cap)
     o.set_vperp(I["v1"], I["v2"])
     o.set_q0(q0s)
     o.step(1)
-    for n in (3, 6):
+    for n in (3, 6, 9, 12):
         o.step(n, rl.injection_uniforms(inp, n // 3))
         nn = n // 3 + 1
         R = rl.read_master(ref_eamb0["wd"], nn, t)
@@ -188,4 +196,4 @@ def test_master_steps_2_and_3_equal_the_references(ref_eamb0):
             assert np.max(np.abs(m3[key] - R[key])) <= 1e-13 * scale, (nn, key)
         assert np.max(np.abs(m["EG"] - R["EG"])) <= 1e-12 * np.abs(R["EG"]).max(), nn
         assert abs(m["esc"][2] - R["lb_out"][0]) <= 1e-12 * max(abs(R["lb_out"][0]), 1.0), nn
-    assert m["esc"][0] > 0                       # ions left through the lower boundary by master step 3
+    assert m["esc"][0] > 0                       # ions left through the lower boundary by master step 5
