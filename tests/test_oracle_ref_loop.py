@@ -65,7 +65,7 @@ import oracle
 import oracle.ref_loop as rl
 from oracle import ptr
 
-NT = 6                    # two master steps (ndatfac 3)
+NT = 9                    # three master steps (ndatfac 3)
 NSNORMFAC = 3.85e17       # about 2e4 ions
 
 
@@ -197,3 +197,45 @@ def test_master_steps_2_to_5_equal_the_references(ref_eamb0):
         assert np.max(np.abs(m["EG"] - R["EG"])) <= 1e-12 * np.abs(R["EG"]).max(), nn
         assert abs(m["esc"][2] - R["lb_out"][0]) <= 1e-12 * max(abs(R["lb_out"][0]), 1.0), nn
     assert m["esc"][0] > 0                       # ions left through the lower boundary by master step 5
+
+
+def test_ambipolar_field_of_master_steps_3_and_4_equals_the_references(ref):
+    """The reference's loop with its ambipolar field on.  The fields of master
+    steps 1 and 2 come from moments the oracle does not reproduce (row nn - 1 =
+    0 of the moment arrays, out of range, at step 1; step 1's counts one cell
+    low at step 2), so both are fed from the reference's exports
+    (Oracle.set_field_input).  From there the oracle's own field -- the
+    stencils of M0, the LB override and the M1par filter (_8_7_AmbipolarEfield.
+    f90) -- equals the reference's at master steps 3 and 4, and the counts,
+    moments and LB escape flux of master steps 2 and 3 are the reference's."""
+    inp = ref["inp"]
+    t, I = inp["tube"], inp["ions"]
+    assert t.eamb == 1
+    qGL = t.arrays["qGL"]
+    q0s = np.minimum(I["q0"], qGL[1]) if qGL[0] > 0 else np.maximum(I["q0"], qGL[1])
+    cap = len(I["q0"]) + 4 * inp["n_inj_rank"] * inp["ranks"] + 1024
+    R = {nn: rl.read_master(ref["wd"], nn, t) for nn in (1, 2, 3, 4)}
+    o = oracle.Oracle(t, I["q0"], I["vpar0"], I["vperp"], ndatfac=3, seed=1, capacity=cap)
+    o.set_vperp(I["v1"], I["v2"])
+    o.set_q0(q0s)
+    o.set_field_input(np.ascontiguousarray(R[1]["EA"]), 1)
+    o.step(1)
+    o.set_field_input(np.ascontiguousarray(R[2]["EA"]), 2)
+    for n in (3, 6, 9):
+        nn = n // 3 + 1
+        if n == 6:
+            o.set_field_input(None, 0)                  # the oracle's own field from here on
+        o.step(n, rl.injection_uniforms(inp, n // 3))
+        m3, m = o.master3(), o.master()
+        if nn >= 3:
+            scale = np.abs(R[nn]["EA"]).max()
+            assert scale > 0 and np.max(np.abs(m["EA"] - R[nn]["EA"])) <= 1e-13 * scale, nn
+        if nn <= 3:
+            assert np.array_equal(m3["n3_counts"], R[nn]["n3_counts"]), nn
+            for key in oracle.MOMENTS3:
+                sc = max(np.abs(R[nn][key]).max(), 1e-300)
+                assert np.max(np.abs(m3[key] - R[nn][key])) <= 1e-13 * sc, (nn, key)
+            assert abs(m["esc"][2] - R[nn]["lb_out"][0]) <= 1e-12 * max(abs(R[nn]["lb_out"][0]), 1.0), nn
+    # the field of master step 3 is not the fed field of step 2 carried over:
+    # the oracle computed it, so the pin is not vacuous
+    assert np.max(np.abs(R[3]["EA"] - R[2]["EA"])) > 1e-3 * np.abs(R[3]["EA"]).max()
