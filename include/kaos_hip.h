@@ -479,7 +479,9 @@ int kaos_count(kaos_ctx* ctx, int32_t tube, int64_t* n);
 /* Live ions of a tube in the reference's order (survivors in order, then
  * injected ions), any pointer may be NULL; n_cap is the capacity of the
  * caller's arrays.  qk4: stage-4 dipole q of the last push at a step before a
- * master step; aea/ag: AEAmagN/AGmagN; qind/vparind/vperpind: Qindk1,
+ * master step; aea/ag: AEAmagN/AGmagN.  An ion not yet pushed since its
+ * injection has qk4 = aea = ag = 0 (the reference leaves a new slot's values
+ * unset until that push).  qind/vparind/vperpind: Qindk1,
  * Vparindk1, Vperpindk1 of the last push, integer(kind=dp) as the reference
  * declares them (_8_0_4_DataTypeAllocA.f90:35-83, dp = 8, _ModelInput.f90:15).
  * In the mixed form vparind is the cell of the stored Vpar (the index the next
